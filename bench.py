@@ -1,0 +1,362 @@
+#!/usr/bin/env python3
+"""Benchmark: train ms/tree at 1M x 100 (binary logistic, 10% NaN, depth 8,
+256 bins) with the batched engine vs plain per-node launches, plus tree-ensemble
+prediction rows/s and p50/p99 latency for the serving loop vs one plain launch
+per request (500 trees, depth 8, 100 features).
+
+One step = one boosting iteration (one tree) over the fixed 1M x 100 matrix,
+inputs resident in HBM before the timed region.  Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from boosters_amd import _abi as A  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gen_binary_nan(n, F, seed):
+    """1M x 100 N(0,1), 10% NaN, label ~ Bernoulli(sigmoid(w . x[0:10])) on pre-mask x."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((F, n), dtype=np.float32)
+    mask = rng.random((F, n), dtype=np.float32) < 0.1
+    w = rng.standard_normal(10)
+    score = w @ X[:10].astype(np.float64)
+    y = (rng.random(n) < 1.0 / (1.0 + np.exp(-score))).astype(np.float32)
+    X[mask] = np.nan
+    return X, y
+
+
+def cfg_c2(n_trees, depth):
+    c = A.GBDTConfigC()
+    A.lib().bst_gbdt_config_default(C.byref(c))
+    c.objective = A.OBJ_LOGISTIC
+    c.n_trees = n_trees
+    c.max_depth = depth
+    c.growth = A.GROW_DEPTHWISE
+    return c
+
+
+def ext(schedule):
+    e = A.ExtConfig()
+    A.lib().bst_ext_config_default(C.byref(e))
+    e.schedule = schedule
+    return e
+
+
+class Dev:
+    """A device buffer allocated through the engine's ABI."""
+
+    def __init__(self, ctx, arr):
+        arr = np.ascontiguousarray(arr)
+        self.ctx = ctx
+        self.p = C.c_void_p()
+        A.check(A.lib().bst_device_alloc(ctx, arr.nbytes, C.byref(self.p)))
+        A.check(A.lib().bst_memcpy_h2d(ctx, self.p, arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+
+    @staticmethod
+    def empty(ctx, nbytes):
+        d = Dev.__new__(Dev)
+        d.ctx = ctx
+        d.p = C.c_void_p()
+        A.check(A.lib().bst_device_alloc(ctx, nbytes, C.byref(d.p)))
+        return d
+
+    def free(self):
+        A.lib().bst_device_free(self.ctx, self.p)
+
+
+def train_timed(ctx, ds, y_dev, n, cfg, e):
+    A.check(A.lib().bst_ctx_sync(ctx))
+    h = C.c_void_p()
+    t0 = time.perf_counter()
+    A.check(A.lib().bst_train_device(ctx, ds, C.cast(y_dev.p, C.POINTER(C.c_float)), n, None,
+                                     C.byref(cfg), C.byref(e), C.byref(h)))
+    A.check(A.lib().bst_ctx_sync(ctx))
+    return h, (time.perf_counter() - t0) * 1e3
+
+
+def kernel_stats(ctx, kind):
+    la, ms, by = C.c_int64(), C.c_double(), C.c_double()
+    A.check(A.lib().bst_ctx_kernel_stats(ctx, kind, C.byref(la), C.byref(ms), C.byref(by)))
+    return la.value, ms.value, by.value
+
+
+def random_forest(n_trees, depth, F, seed):
+    """Complete depth-`depth` trees: thresholds U[-5,5], leaves U[-1,1] (the
+    shape of the reference's bench_medium model, benches/common/models.rs:14-66)."""
+    rng = np.random.default_rng(seed)
+    nn = (1 << (depth + 1)) - 1
+    n_inner = (1 << depth) - 1
+    info = A.ModelInfo()
+    info.n_trees, info.n_groups, info.n_features = n_trees, 1, F
+    info.total_nodes, info.total_cat_words = n_trees * nn, 0
+    info.output_transform, info.objective = A.TRANSFORM_IDENTITY, A.OBJ_SQUARED
+    a = A.alloc_forest(info)
+    ids = np.arange(nn)
+    for t in range(n_trees):
+        s = slice(t * nn, (t + 1) * nn)
+        leaf = ids >= n_inner
+        a["is_leaf"][s] = leaf
+        a["split_index"][s] = np.where(leaf, 0, rng.integers(0, F, nn))
+        a["split_threshold"][s] = np.where(leaf, 0, rng.uniform(-5, 5, nn)).astype(np.float32)
+        a["default_left"][s] = rng.integers(0, 2, nn)
+        a["left"][s] = np.where(leaf, 0, 2 * ids + 1)
+        a["right"][s] = np.where(leaf, 0, 2 * ids + 2)
+        a["leaf_value"][s] = np.where(leaf, rng.uniform(-1, 1, nn), 0).astype(np.float32)
+    a["tree_offset"][:] = np.arange(n_trees + 1) * nn
+    return info, a
+
+
+def serving_bench(ctx, model, F, n_req, qps, seed, budget, inflight):
+    """Poisson arrivals at `qps`, rows log-uniform in [1, 100000]."""
+    rng = np.random.default_rng(seed)
+    rows = np.floor(np.exp(rng.uniform(0, math.log(100001), n_req))).astype(np.int64)
+    gaps = rng.exponential(1.0 / qps, n_req)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = [rng.standard_normal((F, int(r)), dtype=np.float32) for r in rows]
+    outs = [np.zeros(int(r), np.float32) for r in rows]
+    total_rows = int(rows.sum())
+
+    # Plain: one synchronous predict launch per request, in arrival order.
+    lat_plain = np.zeros(n_req)
+    t0 = time.perf_counter()
+    for i, X in enumerate(reqs):
+        now = time.perf_counter() - t0
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        A.check(A.lib().bst_predict(ctx, model, A.ptr(X), X.shape[1], F, 1, A.ptr(outs[i]), None))
+        lat_plain[i] = time.perf_counter() - t0 - arrivals[i]
+    span_plain = time.perf_counter() - t0
+    plain_outs = [o.copy() for o in outs]
+
+    # Server: continuous batching.
+    srv = C.c_void_p()
+    A.check(A.lib().bst_server_create(ctx, model, int(budget), int(inflight), C.byref(srv)))
+    tickets = [None] * n_req
+    lat_srv = np.zeros(n_req)
+    done = threading.Event()
+    t_end = [0.0]
+
+    def waiter():
+        for i in range(n_req):
+            while tickets[i] is None:
+                time.sleep(1e-5)
+            ln = C.c_int64()
+            A.check(A.lib().bst_server_wait(srv, C.c_uint64(tickets[i]), C.byref(ln)))
+            lat_srv[i] = ln.value * 1e-9
+        t_end[0] = time.perf_counter()
+        done.set()
+
+    th = threading.Thread(target=waiter)
+    for o in outs:
+        o[:] = 0
+    th.start()
+    t0 = time.perf_counter()
+    for i, X in enumerate(reqs):
+        now = time.perf_counter() - t0
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        t = C.c_uint64()
+        A.check(A.lib().bst_server_submit(srv, A.ptr(X), X.shape[1], F, A.ptr(outs[i]), C.byref(t)))
+        tickets[i] = t.value
+    done.wait()
+    th.join()
+    span_srv = t_end[0] - t0
+    A.check(A.lib().bst_server_destroy(srv))
+    identical = all(np.array_equal(a, b) for a, b in zip(plain_outs, outs))
+    return {
+        "requests": n_req,
+        "qps_offered": qps,
+        "rows_total": total_rows,
+        "server": {
+            "rows_per_s": total_rows / span_srv,
+            "p50_ms": float(np.percentile(lat_srv, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat_srv, 99) * 1e3),
+        },
+        "plain": {
+            "rows_per_s": total_rows / span_plain,
+            "p50_ms": float(np.percentile(lat_plain, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat_plain, 99) * 1e3),
+        },
+        "results_identical": identical,
+    }
+
+
+def cpu_baseline(X, y, n_trees, depth, threads):
+    """The oracle (C++ restatement of the reference CPU path) on the same data."""
+    from oracle import oracle as O
+
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    b = O.Binned(X)
+    t_bin = time.perf_counter() - t0
+    cfg = cfg_c2(n_trees, depth)
+    m = O.train(b, y, cfg)
+    _, ms = m.stats()
+    return ms / n_trees, t_bin
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=1_000_000)
+    ap.add_argument("--features", type=int, default=100)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-trees", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--qps", type=float, default=1000.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU row "
+              "sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    L = A.lib()
+    ctx = C.c_void_p()
+    A.check(L.bst_ctx_create(0, C.byref(ctx)))
+    n, F, depth = args.rows, args.features, args.depth
+    log(f"generating {n} x {F} binary-logistic data with 10% NaN")
+    X, y = gen_binary_nan(n, F, seed=0)
+    t0 = time.perf_counter()
+    ds = C.c_void_p()
+    A.check(L.bst_dataset_create(ctx, A.ptr(X), n, F, None, None, C.byref(ds)))
+    bin_ms = (time.perf_counter() - t0) * 1e3
+    y_dev = Dev(ctx, y)
+    log(f"binned in {bin_ms:.0f} ms")
+
+    # Warmup (untimed), then exactly `steps` timed boosting iterations.
+    h, _ = train_timed(ctx, ds, y_dev, n, cfg_c2(args.warmup, depth), ext(A.SCHED_BATCHED))
+    L.bst_model_destroy(h)
+    A.check(L.bst_ctx_set_profiling(ctx, 1))
+    h_model, wall_ms = train_timed(ctx, ds, y_dev, n, cfg_c2(args.steps, depth), ext(A.SCHED_BATCHED))
+    hist = kernel_stats(ctx, 0)
+    split = kernel_stats(ctx, 1)
+    part = kernel_stats(ctx, 2)
+    A.check(L.bst_ctx_set_profiling(ctx, 0))
+    ms_tree = wall_ms / args.steps
+    log(f"batched: {ms_tree:.3f} ms/tree ({args.steps} trees in {wall_ms:.1f} ms)")
+
+    h_plain, plain_ms = train_timed(ctx, ds, y_dev, n, cfg_c2(args.plain_steps, depth), ext(A.SCHED_PLAIN))
+    L.bst_model_destroy(h_plain)
+    plain_ms_tree = plain_ms / args.plain_steps
+    log(f"plain: {plain_ms_tree:.3f} ms/tree")
+
+    # Held-out validation loss of the timed model (20% split drawn from the same generator).
+    Xv, yv = gen_binary_nan(n // 4, F, seed=1)
+    pv = np.zeros(len(yv), np.float32)
+    A.check(L.bst_predict(ctx, h_model, A.ptr(Xv), len(yv), F, 0, A.ptr(pv), None))
+    p = np.clip(pv.astype(np.float64), 1e-15, 1 - 1e-15)
+    val_logloss = float(-np.mean(yv * np.log(p) + (1 - yv) * np.log(1 - p)))
+
+    launches, hms, hbytes = hist
+    roofline = None
+    if launches:
+        achieved = hbytes / (hms * 1e-3) / 1e9
+        roofline = {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "kernel": "k_hist",
+            "launches": launches,
+            "avg_launch_us": round(hms / launches * 1e3, 2),
+            "alg_bytes_per_launch": round(hbytes / launches),
+        }
+
+    result = {
+        "metric": "train ms/tree at 1M x 100 (binary logistic, 10% NaN, depth 8, 256 bins)",
+        "value": round(ms_tree, 4),
+        "unit": "ms/tree",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_tree, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 grad/hess, int64 fixed-point histograms, u8 bins",
+        "data": "synthetic",
+        "config": {
+            "workload": f"binary logistic {n} x {F}, 10% NaN, depth-wise {depth}, 256 bins, lr 0.3",
+            "schedule": "batched (level-batched hist/split/partition launches)",
+            "plain_ms_per_tree": round(plain_ms_tree, 4),
+            "batched_speedup_vs_plain": round(plain_ms_tree / ms_tree, 2),
+            "binning_ms": round(bin_ms, 1),
+            "val_logloss": round(val_logloss, 6),
+            "kernel_ms_per_tree": {
+                "hist": round(hms / args.steps, 4),
+                "split": round(split[1] / args.steps, 4),
+                "partition": round(part[1] / args.steps, 4),
+            },
+        },
+        "roofline": roofline,
+    }
+
+    if not args.no_serving:
+        log("serving: 500 trees x depth 8 x 100 features")
+        info, arrays = random_forest(500, 8, 100, seed=0xB002)
+        fa = A.forest_struct(arrays)
+        mh = C.c_void_p()
+        A.check(L.bst_model_import(ctx, C.byref(info), C.byref(fa), C.byref(mh)))
+        # Device-resident batch throughput.
+        nb = 1 << 20
+        Xb = np.random.default_rng(5).standard_normal((100, nb), dtype=np.float32)
+        xd = Dev(ctx, Xb)
+        od = Dev.empty(ctx, nb * 4)
+        A.check(L.bst_predict_device(ctx, mh, xd.p, nb, 100, 1, od.p, None))
+        A.check(L.bst_ctx_sync(ctx))
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            A.check(L.bst_predict_device(ctx, mh, xd.p, nb, 100, 1, od.p, None))
+        A.check(L.bst_ctx_sync(ctx))
+        pred_rows_s = reps * nb / (time.perf_counter() - t0)
+        xd.free()
+        od.free()
+        sv = serving_bench(ctx, mh, 100, args.requests, args.qps, seed=7, budget=4 << 30, inflight=4)
+        sv["device_batch_rows_per_s"] = pred_rows_s
+        result["predict"] = sv
+        L.bst_model_destroy(mh)
+
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"cpu baseline: oracle, {args.cpu_trees} trees on the same data, {threads} threads")
+        cpu_ms_tree, cpu_bin_s = cpu_baseline(X, y, args.cpu_trees, depth, threads)
+        result["cpu_baseline"] = {
+            "value": round(cpu_ms_tree, 2),
+            "unit": "ms/tree",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"same {n} x {F} matrix, {args.cpu_trees} trees (binning {cpu_bin_s:.1f} s excluded)",
+        }
+    L.bst_model_destroy(h_model)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

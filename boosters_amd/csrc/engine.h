@@ -1,0 +1,161 @@
+// engine.h — internal interfaces of the gfx950 GBDT engine (host <-> kernels).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bst {
+
+constexpr int kMaxBinsDev = 256;  // u8 bins per feature; bin 256 lives in the hi plane
+constexpr int kMaxCatWords = 8;   // categorical bitsets cover bins 0..255
+
+// Per effective feature, for value_to_bin (BinMapper, bin_mapper.rs:210-258).
+struct BinMeta {
+  uint32_t orig;        // column in the raw [F][N] matrix
+  uint32_t n_bins;
+  uint32_t n_search;    // numeric: bins searched (NaN bin excluded)
+  uint32_t table_off;   // offset into the f32 threshold / i32 category tables
+  uint32_t default_bin;
+  uint8_t is_cat;
+  uint8_t missing_nan;
+  uint8_t pad[2];
+};
+
+// Per effective feature, for split finding and partitioning.
+struct FeatMeta {
+  uint32_t n_bins;
+  uint32_t default_bin;
+  uint8_t is_cat;
+  uint8_t has_missing;
+  uint8_t pad[2];
+};
+
+// One block of histogram work: a tile of a node's rows x one feature group.
+struct HistItem {
+  uint32_t node;     // global node id (tree slot * max_nodes + partition id)
+  uint32_t slot;     // histogram slot to accumulate into
+  uint32_t begin;    // first position in idx/ogh of this tile
+  uint32_t count;    // rows in this tile
+  uint32_t group;    // feature group
+  uint32_t tree;     // tree slot (selects the quantisation scale)
+};
+
+// A split decision on device (SplitInfo, split/types.rs:34-55).
+struct DevSplit {
+  float gain;  // -inf = invalid
+  uint32_t feature;
+  uint32_t bin;
+  uint32_t flags;  // bit0 default_left, bit1 categorical
+  uint32_t cats[kMaxCatWords];
+};
+
+// One node to split: rows [begin, begin+count) of idx/ogh.
+struct PartItem {
+  uint32_t node;       // global id of the parent (keeps the left rows)
+  uint32_t begin;
+  uint32_t count;
+  uint32_t tile0;      // first partition tile of this node
+  uint32_t n_tiles;
+  uint32_t right_node; // global id of the new right child
+  uint32_t feature;    // effective column
+  uint32_t pad;
+};
+
+struct PartTile {
+  uint32_t item;   // index into PartItem list
+  uint32_t begin;  // absolute first position
+  uint32_t count;
+  uint32_t pad;
+};
+
+// Per-node segment for sums / leaf updates.
+struct Segment {
+  uint32_t node;
+  uint32_t begin;
+  uint32_t count;
+  uint32_t tree;
+};
+
+// Packed tree node for traversal (Tree SoA, repr/gbdt/tree.rs:32-47).
+struct DevNode {
+  uint32_t meta;   // bits 0..23 feature, 24 leaf, 25 default_left, 26 categorical
+  float value;     // threshold (numeric) or leaf value
+  uint32_t left;   // tree-local ids
+  uint32_t right;
+};
+constexpr uint32_t kNodeLeaf = 1u << 24, kNodeDefLeft = 1u << 25, kNodeCat = 1u << 26;
+
+struct DatasetDev {
+  int64_t n = 0;
+  int32_t F = 0;        // original features
+  int32_t E = 0;        // effective (non-trivial) features
+  int32_t S = 0;        // row stride of bins_rm in bytes
+  int32_t hi_off = -1;  // byte offset of the hi-bit mask in a row, -1 = none
+  int32_t FG = 32;      // features per histogram group (lanes of one row)
+  int32_t NG = 0;       // number of groups
+  int64_t hi_words = 0; // u32 words per feature in hi_cm
+  uint8_t* bins_rm = nullptr;   // [n][S]
+  uint8_t* bins_cm = nullptr;   // [E][n]
+  uint32_t* hi_cm = nullptr;    // [E][hi_words]
+  FeatMeta* meta = nullptr;     // [E]
+  float* raw = nullptr;         // [F][n] raw features (sampled prediction updates)
+};
+
+// ---- launchers (kernels.hip) ----------------------------------------------
+void launch_value_to_bin(hipStream_t s, const float* X, int64_t n, int E, const BinMeta* meta,
+                         const float* thr, const int32_t* cats, uint8_t* bins_cm, uint32_t* hi_cm,
+                         int64_t hi_words);
+void launch_to_rowmajor(hipStream_t s, const DatasetDev& d);
+
+void launch_gradients(hipStream_t s, int objective, const float* preds, const float* y,
+                      const float* w, int64_t n, int K, float* g, float* h, uint32_t* absmax);
+void launch_qscale(hipStream_t s, const uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp);
+
+void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g, const float* h,
+                      const uint32_t* sampled, int64_t m);
+
+void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
+                 const uint32_t* idx, const float2* ogh, int64_t tree_stride_rows,
+                 const int32_t* qexp, long long* hist, int64_t slot_elems, long long* node_sum,
+                 bool gather, int rows_per_block_hint);
+void launch_node_sums(hipStream_t s, const Segment* segs, int n_segs, const float2* ogh,
+                      int64_t tree_stride_rows, const int32_t* qexp, long long* node_sum);
+void launch_subtract(hipStream_t s, const uint32_t* triples, int n, long long* hist,
+                     int64_t slot_elems, long long* node_sum);
+void launch_zero_slots(hipStream_t s, const uint32_t* slots, int n, long long* hist,
+                       int64_t slot_elems);
+
+struct SplitParams {
+  double lambda;
+  double min_gain;
+  double min_child_weight;
+  uint32_t min_samples_leaf;
+  uint32_t max_onehot_cats;
+  int32_t rightmost;  // rayon tie rule (rightmost max) vs sequential (first max)
+};
+void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, const uint32_t* slots,
+                  int n_nodes, const uint32_t* feats, int n_feats, const long long* hist,
+                  int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
+                  const int32_t* qexp, uint32_t max_nodes, SplitParams p,
+                  unsigned long long* node_key, DevSplit* cand, DevSplit* node_split);
+
+// Stable partition of every listed node (partition.rs:73-145): count, scan,
+// scatter, copy back, then children counts/sums.
+void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items, int n_items,
+                      const PartTile* tiles, int n_tiles, const DevSplit* node_split,
+                      uint32_t* idx, float2* ogh, uint32_t* idx_tmp, float2* ogh_tmp,
+                      uint8_t* flag, uint32_t* tile_left, uint32_t* tile_off,
+                      uint32_t* item_left, uint32_t* node_count, long long* node_sum);
+
+void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const float* leaf_val,
+                        const uint32_t* idx, int64_t tree_stride_rows, float* preds,
+                        const uint32_t* tree_output, int64_t n_rows);
+
+void launch_predict(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
+                    const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
+                    const uint32_t* cat_size, const uint32_t* cat_words, const float* base,
+                    int K, const float* X, int64_t n, int64_t x_stride, int F, int transform,
+                    float* out, int64_t out_stride, uint32_t* leaf_idx);
+int predict_rows_per_block(int F);
+constexpr int kPredMaxGroups = 16;
+
+}  // namespace bst

@@ -1,0 +1,183 @@
+// host_common.h — host-side helpers shared by the engine's C++ files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/boosters_amd.h"
+#include "engine.h"
+
+namespace bst {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] inline void raise(int code, const std::string& msg) { throw Error{code, msg}; }
+
+// Thread-local message returned by bst_last_error() (defined in api.cpp).
+std::string& last_error_ref();
+
+#define BST_HIP(x)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      ::bst::raise(BST_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+// Device buffer owned by the engine.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void alloc(size_t count) {
+    reset();
+    if (count == 0) return;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      raise(BST_E_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                           " bytes failed: " + hipGetErrorString(e));
+    }
+    n = count;
+  }
+  // Grow-only reallocation (contents are not preserved).
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+template <class T>
+void h2d(hipStream_t s, T* dst, const T* src, size_t count) {
+  if (count) BST_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+void d2h(hipStream_t s, T* dst, const T* src, size_t count) {
+  if (count) BST_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+// A numeric or categorical bin mapper (BinMapper, data/binned/bin_mapper.rs:55-90).
+struct Mapper {
+  bool categorical = false;
+  bool trivial = false;
+  std::vector<double> bounds;  // numeric upper bounds incl. f64::MAX sentinels
+  std::vector<int32_t> cats;   // categorical: sorted category values
+  uint32_t n_bins = 0;
+  int missing = 0;  // 0 None, 2 NaN
+  uint32_t default_bin = 0;
+};
+
+// Build bin mappers for every feature of a [F][n] host matrix
+// (BinnedDataset::from_dataset, data/binned/dataset.rs:207-309).
+std::vector<Mapper> build_mappers(const float* X, int64_t n, int32_t F, const uint8_t* is_cat,
+                                  const bst_binning_config& cfg);
+
+float next_up_f32(float x);
+float round_down_f32(double b);
+
+}  // namespace bst
+
+struct bst_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double last_ms_total = 0, last_ms_per_tree = 0;
+  int64_t last_hist_rows = 0;
+  std::vector<double> last_metric;
+  // Kernel profiling: events bracketing launches on `stream`.
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  bool profile = false;
+  std::vector<Pending> pending;
+  int64_t k_launches[3] = {0, 0, 0};
+  double k_ms[3] = {0, 0, 0}, k_bytes[3] = {0, 0, 0};
+  // Record the start of a launch of `kind`; returns an index for prof_end.
+  int prof_begin(int kind, double bytes) {
+    if (!profile) return -1;
+    Pending p{kind, nullptr, nullptr, bytes};
+    BST_HIP(hipEventCreate(&p.a));
+    BST_HIP(hipEventCreate(&p.b));
+    BST_HIP(hipEventRecord(p.a, stream));
+    pending.push_back(p);
+    return (int)pending.size() - 1;
+  }
+  void prof_end(int i) {
+    if (i >= 0) BST_HIP(hipEventRecord(pending[i].b, stream));
+  }
+  // After a stream synchronisation: fold the finished launches in.
+  void prof_drain() {
+    for (auto& p : pending) {
+      float ms = 0.0f;
+      BST_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      k_launches[p.kind]++;
+      k_ms[p.kind] += ms;
+      k_bytes[p.kind] += p.bytes;
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending.clear();
+  }
+};
+
+struct bst_dataset {
+  bst_ctx* ctx = nullptr;
+  std::vector<bst::Mapper> mappers;      // per original feature
+  std::vector<uint32_t> eff_to_orig;     // effective column -> original feature
+  std::vector<bst::FeatMeta> fmeta;      // per effective column (host copy)
+  std::vector<float> thr_up;             // [E][256]: next_up((f32)bound[bin])
+  bst::DatasetDev d;
+  bst::DevBuf<uint8_t> bins_rm, bins_cm;
+  bst::DevBuf<uint32_t> hi_cm;
+  bst::DevBuf<bst::FeatMeta> meta;
+  bst::DevBuf<float> raw;
+  double ms_binning = 0;
+};
+
+struct bst_model {
+  bst_model_info info{};
+  std::vector<float> base_score;
+  std::vector<int64_t> tree_offset;
+  std::vector<uint32_t> tree_group, split_index, left, right, cat_start, cat_size, cat_words;
+  std::vector<float> split_threshold, leaf_value, gain, cover;
+  std::vector<uint8_t> default_left, is_leaf, split_type;
+  // Device copy for traversal (built on first predict).
+  int device_for = -1;
+  bst::DevBuf<bst::DevNode> d_nodes;
+  bst::DevBuf<int64_t> d_tree_off;
+  bst::DevBuf<uint32_t> d_tree_group, d_cat_start, d_cat_size, d_cat_words;
+  bst::DevBuf<float> d_base;
+  void upload(hipStream_t s);
+};

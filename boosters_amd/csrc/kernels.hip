@@ -1,0 +1,1244 @@
+// kernels.hip — gfx950 device code of the GBDT engine.
+//
+// Reference behaviour restated here (paths relative to crates/boosters/src/):
+//   value_to_bin          data/binned/bin_mapper.rs:210-258
+//   objectives            training/objectives/{regression.rs:20-42, classification.rs:20-187}
+//   histograms            training/gbdt/histograms/ops.rs:79-283 (+ subtraction :345-351)
+//   split scan            training/gbdt/split/find.rs:286-573, gain.rs:16-52,123-137
+//   partition             training/gbdt/partition.rs:73-145,296-382
+//   prediction            inference/gbdt/predictor.rs:197-302, repr/gbdt/tree_view.rs:119-160
+//   transforms            model/transform.rs:83-115
+//
+// Numerics: compiled with -ffp-contract=off so every f32/f64 operation rounds
+// as written (like rustc).  Gradients and hessians are quantised per round to
+// 64-bit fixed point with a power-of-two scale 2^e chosen so that any sum over
+// the rows fits in 62 bits; histogram sums are then exact integer sums,
+// independent of atomic order, and are converted to f64 only for the gain.
+#include "engine.h"
+
+#include <float.h>
+#include <math.h>
+
+namespace bst {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ float exp32(float x) { return (float)exp((double)x); }
+
+__device__ __forceinline__ long long shfl_up_ll(long long v, int d) {
+  int lo = (int)(v & 0xFFFFFFFFLL), hi = (int)(v >> 32);
+  lo = __shfl_up(lo, d, WAVE);
+  hi = __shfl_up(hi, d, WAVE);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ unsigned long long shfl_xor_ull(unsigned long long v, int m) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  lo = __shfl_xor(lo, m, WAVE);
+  hi = __shfl_xor(hi, m, WAVE);
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ long long shfl_xor_ll(long long v, int m) {
+  return (long long)shfl_xor_ull((unsigned long long)v, m);
+}
+// Inclusive scan over the 64 lanes of a wave.
+__device__ __forceinline__ long long wave_incl_scan(long long v) {
+  const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    long long o = shfl_up_ll(v, d);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_ull(unsigned long long v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    unsigned long long o = shfl_xor_ull(v, m);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_ll(v, m);
+  return v;
+}
+// Order-preserving map of f32 to u32 (larger float -> larger integer).
+__device__ __forceinline__ uint32_t ord_f32(float x) {
+  uint32_t u = __float_as_uint(x);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// ===========================================================================
+// Quantisation: value_to_bin
+// ===========================================================================
+// Numeric: first index i < n_search with v <= bound[i] (f64), else n_search.
+// thr[] holds, per bound, the largest f32 not above it, so the test is done
+// exactly in f32.  NaN -> n_bins-1 if the feature has a NaN bin, else the
+// default bin.  Categorical: (i32)trunc(v) looked up in the sorted category
+// list, unknown -> 0, NaN -> default bin.
+__global__ __launch_bounds__(256) void k_value_to_bin(const float* __restrict__ X, int64_t n,
+                                                      const BinMeta* __restrict__ meta,
+                                                      const float* __restrict__ thr,
+                                                      const int32_t* __restrict__ cats,
+                                                      uint8_t* __restrict__ bins_cm,
+                                                      uint32_t* __restrict__ hi_cm,
+                                                      int64_t hi_words) {
+  __shared__ float s_thr[kMaxBinsDev + 1];
+  __shared__ int32_t s_cat[kMaxBinsDev + 1];
+  const int e = blockIdx.y;
+  const BinMeta m = meta[e];
+  const int tabn = m.is_cat ? (int)m.n_bins : (int)m.n_search;
+  for (int i = threadIdx.x; i < tabn; i += blockDim.x) {
+    if (m.is_cat)
+      s_cat[i] = cats[m.table_off + i];
+    else
+      s_thr[i] = thr[m.table_off + i];
+  }
+  __syncthreads();
+  const float* col = X + (int64_t)m.orig * n;
+  const int lane = threadIdx.x & (WAVE - 1);
+  // Each wave owns aligned groups of 64 rows so the hi-bit ballot maps to two words.
+  for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)));
+       base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = base + lane;
+    uint32_t b = 0;
+    if (r < n) {
+      const float v = col[r];
+      if (isnan(v)) {
+        b = m.missing_nan ? m.n_bins - 1 : m.default_bin;
+      } else if (m.is_cat) {
+        const double dv = (double)v;
+        int32_t c = dv >= 2147483647.0 ? 2147483647
+                                       : (dv <= -2147483648.0 ? (int32_t)0x80000000 : (int32_t)dv);
+        int lo = 0, hi = (int)m.n_bins;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (s_cat[mid] < c)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        b = (lo < (int)m.n_bins && s_cat[lo] == c) ? (uint32_t)lo : 0u;
+      } else {
+        int lo = 0, hi = (int)m.n_search;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (v <= s_thr[mid])
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        b = (uint32_t)lo;
+      }
+      bins_cm[(int64_t)e * n + r] = (uint8_t)(b & 0xFF);
+    }
+    const unsigned long long hb = __ballot(b >= 256u);
+    if (hi_cm && lane == 0) {
+      const int64_t w = base >> 5;
+      if (w < hi_words) hi_cm[(int64_t)e * hi_words + w] = (uint32_t)hb;
+      if (w + 1 < hi_words) hi_cm[(int64_t)e * hi_words + w + 1] = (uint32_t)(hb >> 32);
+    }
+  }
+}
+
+void launch_value_to_bin(hipStream_t s, const float* X, int64_t n, int E, const BinMeta* meta,
+                         const float* thr, const int32_t* cats, uint8_t* bins_cm, uint32_t* hi_cm,
+                         int64_t hi_words) {
+  int64_t blocks_x = (n + 255) / 256;
+  if (blocks_x > 2048) blocks_x = 2048;
+  dim3 grid((unsigned)blocks_x, (unsigned)E);
+  hipLaunchKernelGGL(k_value_to_bin, grid, dim3(256), 0, s, X, n, meta, thr, cats, bins_cm, hi_cm,
+                     hi_words);
+}
+
+// Column-major -> row-major: 64 rows per block staged through LDS.
+__global__ __launch_bounds__(256) void k_to_rowmajor(const uint8_t* __restrict__ bins_cm,
+                                                     const uint32_t* __restrict__ hi_cm,
+                                                     int64_t n, int E, int S, int hi_off,
+                                                     int64_t hi_words, uint8_t* __restrict__ bins_rm) {
+  extern __shared__ uint32_t s_tile32[];  // [64][S/4]
+  uint8_t* s_tile = (uint8_t*)s_tile32;
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  for (int i = threadIdx.x; i < 64 * S / 4; i += blockDim.x) s_tile32[i] = 0;
+  __syncthreads();
+  // Bins: thread -> (feature, row) with rows fastest (coalesced column reads).
+  for (int i = threadIdx.x; i < 64 * E; i += blockDim.x) {
+    const int e = i >> 6, r = i & 63;
+    if (r0 + r < n) s_tile[r * S + e] = bins_cm[(int64_t)e * n + r0 + r];
+  }
+  if (hi_off >= 0) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * E; i += blockDim.x) {
+      const int e = i >> 6, r = i & 63;
+      const int64_t row = r0 + r;
+      if (row < n) {
+        uint32_t w = hi_cm[(int64_t)e * hi_words + (row >> 5)];
+        if ((w >> (row & 31)) & 1u) atomicOr(&s_tile32[(r * S + hi_off + (e >> 3)) >> 2],
+                                             1u << ((((r * S + hi_off + (e >> 3)) & 3) << 3) + (e & 7)));
+      }
+    }
+  }
+  __syncthreads();
+  const int words = S / 4;
+  uint32_t* out = (uint32_t*)(bins_rm + r0 * S);
+  for (int i = threadIdx.x; i < 64 * words; i += blockDim.x) {
+    const int r = i / words;
+    if (r0 + r < n) out[i] = s_tile32[i];
+  }
+}
+
+void launch_to_rowmajor(hipStream_t s, const DatasetDev& d) {
+  const int64_t blocks = (d.n + 63) / 64;
+  hipLaunchKernelGGL(k_to_rowmajor, dim3((unsigned)blocks), dim3(256), 64 * d.S, s, d.bins_cm,
+                     d.hi_cm, d.n, d.E, d.S, d.hi_off, d.hi_words, d.bins_rm);
+}
+
+// ===========================================================================
+// Objectives
+// ===========================================================================
+__device__ __forceinline__ void wave_atomic_max_abs(uint32_t* dst, float v) {
+  uint32_t u = __float_as_uint(fabsf(v));
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    uint32_t o = __shfl_xor(u, m, WAVE);
+    u = o > u ? o : u;
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(dst, u);
+}
+
+// grad/hess [K][n]; absmax[2k] = max|g|, absmax[2k+1] = max|h| (as f32 bits).
+__global__ __launch_bounds__(256) void k_gradients(int objective, const float* __restrict__ preds,
+                                                   const float* __restrict__ y,
+                                                   const float* __restrict__ w, int64_t n, int K,
+                                                   float* __restrict__ g, float* __restrict__ h,
+                                                   uint32_t* __restrict__ absmax) {
+  const float HESS_MIN = 1e-6f;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < n;
+  const float wi = (in && w) ? w[i] : 1.0f;
+  const float yi = in ? y[i] : 0.0f;
+  if (objective == 2) {  // softmax, classification.rs:141-187
+    float mx = -INFINITY;
+    for (int k = 0; k < K; k++) mx = fmaxf(mx, in ? preds[(int64_t)k * n + i] : 0.0f);
+    float s = 0.0f;
+    for (int k = 0; k < K; k++) s += exp32((in ? preds[(int64_t)k * n + i] : 0.0f) - mx);
+    const int label = (int)yi;
+    for (int k = 0; k < K; k++) {
+      float gv = 0.0f, hv = 0.0f;
+      if (in) {
+        const float p = exp32(preds[(int64_t)k * n + i] - mx) / s;
+        const float ind = k == label ? 1.0f : 0.0f;
+        gv = wi * (p - ind);
+        hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
+        g[(int64_t)k * n + i] = gv;
+        h[(int64_t)k * n + i] = hv;
+      }
+      wave_atomic_max_abs(&absmax[2 * k], gv);
+      wave_atomic_max_abs(&absmax[2 * k + 1], hv);
+    }
+    return;
+  }
+  for (int k = 0; k < K; k++) {
+    float gv = 0.0f, hv = 0.0f;
+    if (in) {
+      const float p0 = preds[(int64_t)k * n + i];
+      if (objective == 1) {  // logistic, classification.rs:24-52
+        const float p = 1.0f / (1.0f + exp32(-p0));
+        gv = wi * (p - yi);
+        hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
+      } else {  // squared, regression.rs:20-42
+        gv = wi * (p0 - yi);
+        hv = wi;
+      }
+      g[(int64_t)k * n + i] = gv;
+      h[(int64_t)k * n + i] = hv;
+    }
+    wave_atomic_max_abs(&absmax[2 * k], gv);
+    wave_atomic_max_abs(&absmax[2 * k + 1], hv);
+  }
+}
+
+void launch_gradients(hipStream_t s, int objective, const float* preds, const float* y,
+                      const float* w, int64_t n, int K, float* g, float* h, uint32_t* absmax) {
+  hipLaunchKernelGGL(k_gradients, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, objective,
+                     preds, y, w, n, K, g, h, absmax);
+}
+
+// Scale exponent per (tree, g/h): largest e with n * max|x| * 2^e <= 2^62.
+__global__ void k_qscale(const uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
+  const int i = threadIdx.x;
+  if (i >= 2 * K) return;
+  const float m = __uint_as_float(absmax[i]);
+  int lg = 0;
+  while ((1LL << lg) < n_rows && lg < 62) lg++;
+  int e = 0;
+  if (m > 0.0f && isfinite(m)) {
+    int x;
+    frexpf(m, &x);  // m < 2^x
+    e = 62 - lg - x;
+    if (e > 1000) e = 1000;
+  }
+  qexp[i] = e;
+}
+
+void launch_qscale(hipStream_t s, const uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
+  hipLaunchKernelGGL(k_qscale, dim3(1), dim3(64 * ((2 * K + 63) / 64)), 0, s, absmax, K, n_rows,
+                     qexp);
+}
+
+// idx = row ids (sampled list or 0..m-1); ogh = gradient pairs in that order.
+__global__ __launch_bounds__(256) void k_init_rows(uint32_t* __restrict__ idx,
+                                                   float2* __restrict__ ogh,
+                                                   const float* __restrict__ g,
+                                                   const float* __restrict__ h,
+                                                   const uint32_t* __restrict__ sampled, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t r = sampled ? sampled[i] : (uint32_t)i;
+  idx[i] = r;
+  ogh[i] = make_float2(g[r], h[r]);
+}
+
+void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g, const float* h,
+                      const uint32_t* sampled, int64_t m) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_init_rows, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, idx, ogh, g,
+                     h, sampled, m);
+}
+
+// ===========================================================================
+// Histograms
+// ===========================================================================
+// One block = one HistItem: a tile of a node's rows x one group of FG
+// features.  Lane (threadIdx % FG) owns one feature; threadIdx / FG picks the
+// row, so the FG lanes of a row read FG consecutive bytes of that row.  The
+// LDS sub-histogram is laid out [bin][FG]: lanes of one row always hit FG
+// distinct 8-byte words, so ds_add_u64 is bank-conflict free whatever the
+// bins are, and hot bins (many rows in one bin) never collide inside a row.
+// Global layout per slot: [group][bin][FG] of (g, h) int64 pairs, which the
+// flush writes with unit stride.
+template <int FG, bool GATHER, bool HAS_HI>
+__global__ __launch_bounds__(1024) void k_hist(const HistItem* __restrict__ items,
+                                               const uint32_t* __restrict__ idx,
+                                               const float2* __restrict__ ogh,
+                                               int64_t tree_stride,
+                                               const uint8_t* __restrict__ bins_rm, int S,
+                                               int hi_off, int E, const int32_t* __restrict__ qexp,
+                                               long long* __restrict__ hist, int64_t slot_elems,
+                                               long long* __restrict__ node_sum) {
+  __shared__ long long s_g[256 * FG];
+  __shared__ long long s_h[256 * FG];
+  __shared__ long long s_red[2][1024 / WAVE];
+  const HistItem it = items[blockIdx.x];
+  const bool direct = (it.tree & 0x80000000u) != 0;  // this tile covers the whole node
+  const uint32_t tree = it.tree & 0x7FFFFFFFu;
+  for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
+    s_g[i] = 0;
+    s_h[i] = 0;
+  }
+  __syncthreads();
+  const int lf = threadIdx.x % FG;
+  const int rs = threadIdx.x / FG;
+  constexpr int RPP = 1024 / FG;
+  constexpr int U = 4;
+  const int f = (int)it.group * FG + lf;
+  const bool fvalid = f < E;
+  const double sg = ldexp(1.0, qexp[2 * tree]);
+  const double sh = ldexp(1.0, qexp[2 * tree + 1]);
+  const uint32_t* tidx = idx + (int64_t)tree * tree_stride;
+  const float2* togh = ogh + (int64_t)tree * tree_stride;
+  long long accg = 0, acch = 0;
+  const bool do_sum = it.group == 0 && lf == 0;
+  for (uint32_t p0 = rs; p0 < it.count; p0 += RPP * U) {
+    uint32_t rows[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t p = p0 + u * RPP;
+      ok[u] = p < it.count;
+      const uint32_t pos = it.begin + (ok[u] ? p : 0);
+      rows[u] = GATHER ? tidx[pos] : pos;
+    }
+    float2 gh[U];
+    uint32_t b[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t pos = it.begin + (ok[u] ? p0 + u * RPP : 0);
+      gh[u] = togh[pos];
+      const uint8_t* rp = bins_rm + (int64_t)rows[u] * S;
+      b[u] = fvalid ? (uint32_t)rp[f] : 0u;
+      if (HAS_HI && fvalid) b[u] |= ((uint32_t)(rp[hi_off + (f >> 3)] >> (f & 7)) & 1u) << 8;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!ok[u]) continue;
+      const long long gq = __double2ll_rn((double)gh[u].x * sg);
+      const long long hq = __double2ll_rn((double)gh[u].y * sh);
+      if (do_sum) {
+        accg += gq;
+        acch += hq;
+      }
+      // bin 256 (NaN bin of a 257-bin feature) is not accumulated: it equals
+      // the node total minus the 256 real bins and is never read by the scan.
+      if (fvalid && b[u] < 256u) {
+        atomicAdd((unsigned long long*)&s_g[b[u] * FG + lf], (unsigned long long)gq);
+        atomicAdd((unsigned long long*)&s_h[b[u] * FG + lf], (unsigned long long)hq);
+      }
+    }
+  }
+  if (it.group == 0) {
+    accg = wave_sum_ll(accg);
+    acch = wave_sum_ll(acch);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+      s_red[0][threadIdx.x / WAVE] = accg;
+      s_red[1][threadIdx.x / WAVE] = acch;
+    }
+  }
+  __syncthreads();
+  if (it.group == 0 && threadIdx.x == 0) {
+    long long tg = 0, th = 0;
+    for (int wv = 0; wv < 1024 / WAVE; wv++) {
+      tg += s_red[0][wv];
+      th += s_red[1][wv];
+    }
+    if (direct) {
+      node_sum[2 * it.node] = tg;
+      node_sum[2 * it.node + 1] = th;
+    } else {
+      atomicAdd((unsigned long long*)&node_sum[2 * it.node], (unsigned long long)tg);
+      atomicAdd((unsigned long long*)&node_sum[2 * it.node + 1], (unsigned long long)th);
+    }
+  }
+  long long* dst = hist + (int64_t)it.slot * slot_elems + (int64_t)it.group * 256 * FG * 2;
+  if (direct) {
+    for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
+      longlong2 v;
+      v.x = s_g[i];
+      v.y = s_h[i];
+      ((longlong2*)dst)[i] = v;
+    }
+  } else {
+    for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
+      const long long gv = s_g[i], hv = s_h[i];
+      if (gv | hv) {
+        atomicAdd((unsigned long long*)&dst[2 * i], (unsigned long long)gv);
+        atomicAdd((unsigned long long*)&dst[2 * i + 1], (unsigned long long)hv);
+      }
+    }
+  }
+}
+
+template <int FG>
+static void launch_hist_fg(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
+                           const uint32_t* idx, const float2* ogh, int64_t tree_stride,
+                           const int32_t* qexp, long long* hist, int64_t slot_elems,
+                           long long* node_sum, bool gather) {
+  const bool hi = d.hi_off >= 0;
+#define BST_HIST_LAUNCH(G, H)                                                                      \
+  hipLaunchKernelGGL((k_hist<FG, G, H>), dim3(n_items), dim3(1024), 0, s, items, idx, ogh,         \
+                     tree_stride, d.bins_rm, d.S, d.hi_off, d.E, qexp, hist, slot_elems, node_sum)
+  if (gather) {
+    if (hi)
+      BST_HIST_LAUNCH(true, true);
+    else
+      BST_HIST_LAUNCH(true, false);
+  } else {
+    if (hi)
+      BST_HIST_LAUNCH(false, true);
+    else
+      BST_HIST_LAUNCH(false, false);
+  }
+#undef BST_HIST_LAUNCH
+}
+
+void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
+                 const uint32_t* idx, const float2* ogh, int64_t tree_stride, const int32_t* qexp,
+                 long long* hist, int64_t slot_elems, long long* node_sum, bool gather,
+                 int /*rows_per_block_hint*/) {
+  if (n_items <= 0) return;
+  if (d.FG == 16)
+    launch_hist_fg<16>(s, d, items, n_items, idx, ogh, tree_stride, qexp, hist, slot_elems,
+                       node_sum, gather);
+  else
+    launch_hist_fg<32>(s, d, items, n_items, idx, ogh, tree_stride, qexp, hist, slot_elems,
+                       node_sum, gather);
+}
+
+// Exact node totals over row segments (used where no histogram is needed).
+__global__ __launch_bounds__(256) void k_node_sums(const Segment* __restrict__ segs,
+                                                   const float2* __restrict__ ogh,
+                                                   int64_t tree_stride,
+                                                   const int32_t* __restrict__ qexp,
+                                                   long long* __restrict__ node_sum) {
+  __shared__ long long s_red[2][256 / WAVE];
+  const Segment sg = segs[blockIdx.x];
+  const double a = ldexp(1.0, qexp[2 * sg.tree]), b = ldexp(1.0, qexp[2 * sg.tree + 1]);
+  const float2* t = ogh + (int64_t)sg.tree * tree_stride;
+  long long g = 0, h = 0;
+  for (uint32_t p = threadIdx.x; p < sg.count; p += blockDim.x) {
+    const float2 v = t[sg.begin + p];
+    g += __double2ll_rn((double)v.x * a);
+    h += __double2ll_rn((double)v.y * b);
+  }
+  g = wave_sum_ll(g);
+  h = wave_sum_ll(h);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    s_red[0][threadIdx.x / WAVE] = g;
+    s_red[1][threadIdx.x / WAVE] = h;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long tg = 0, th = 0;
+    for (int i = 0; i < 256 / WAVE; i++) {
+      tg += s_red[0][i];
+      th += s_red[1][i];
+    }
+    atomicAdd((unsigned long long*)&node_sum[2 * sg.node], (unsigned long long)tg);
+    atomicAdd((unsigned long long*)&node_sum[2 * sg.node + 1], (unsigned long long)th);
+  }
+}
+
+void launch_node_sums(hipStream_t s, const Segment* segs, int n_segs, const float2* ogh,
+                      int64_t tree_stride, const int32_t* qexp, long long* node_sum) {
+  if (n_segs <= 0) return;
+  hipLaunchKernelGGL(k_node_sums, dim3(n_segs), dim3(256), 0, s, segs, ogh, tree_stride, qexp,
+                     node_sum);
+}
+
+// Subtraction trick: for each (large_slot, small_slot, large_node, small_node)
+// hist[large] -= hist[small], node_sum[large] -= node_sum[small].
+__global__ __launch_bounds__(256) void k_subtract(const uint32_t* __restrict__ quads,
+                                                  long long* __restrict__ hist,
+                                                  int64_t slot_elems,
+                                                  long long* __restrict__ node_sum) {
+  const uint32_t* q = quads + 4 * blockIdx.y;
+  longlong2* dst = (longlong2*)(hist + (int64_t)q[0] * slot_elems);
+  const longlong2* src = (const longlong2*)(hist + (int64_t)q[1] * slot_elems);
+  const int64_t m = slot_elems / 2;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    longlong2 a = dst[i];
+    const longlong2 b = src[i];
+    a.x -= b.x;
+    a.y -= b.y;
+    dst[i] = a;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2) node_sum[2 * q[2] + threadIdx.x] -= node_sum[2 * q[3] + threadIdx.x];
+}
+
+void launch_subtract(hipStream_t s, const uint32_t* quads, int n, long long* hist,
+                     int64_t slot_elems, long long* node_sum) {
+  if (n <= 0) return;
+  int64_t bx = (slot_elems / 2 + 255) / 256;
+  if (bx > 64) bx = 64;
+  if (bx < 1) bx = 1;
+  hipLaunchKernelGGL(k_subtract, dim3((unsigned)bx, n), dim3(256), 0, s, quads, hist, slot_elems,
+                     node_sum);
+}
+
+__global__ __launch_bounds__(256) void k_zero_slots(const uint32_t* __restrict__ slots,
+                                                    long long* __restrict__ hist,
+                                                    int64_t slot_elems) {
+  longlong2* dst = (longlong2*)(hist + (int64_t)slots[blockIdx.y] * slot_elems);
+  const int64_t m = slot_elems / 2;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = make_longlong2(0, 0);
+}
+
+void launch_zero_slots(hipStream_t s, const uint32_t* slots, int n, long long* hist,
+                       int64_t slot_elems) {
+  if (n <= 0) return;
+  int64_t bx = (slot_elems / 2 + 255) / 256;
+  if (bx > 64) bx = 64;
+  hipLaunchKernelGGL(k_zero_slots, dim3((unsigned)bx, n), dim3(256), 0, s, slots, hist,
+                     slot_elems);
+}
+
+// ===========================================================================
+// Split finding: one wave per (node, feature)
+// ===========================================================================
+struct GainCtx {  // NodeGainContext, gain.rs:16-52
+  double lambda, offset;
+  __device__ GainCtx(double pg, double ph, const SplitParams& p) {
+    lambda = p.lambda;
+    const double ps = pg * pg / (ph + lambda);
+    offset = 0.5 * ps + p.min_gain;
+  }
+  __device__ __forceinline__ float gain(double gl, double hl, double gr, double hr) const {
+    const double sl = gl * gl / (hl + lambda);
+    const double sr = gr * gr / (hr + lambda);
+    return (float)(0.5 * (sl + sr) - offset);
+  }
+};
+
+__device__ __forceinline__ bool valid_split(const SplitParams& p, double hl, double hr,
+                                            uint32_t cl, uint32_t cr) {
+  return hl >= p.min_child_weight && hr >= p.min_child_weight && cl >= p.min_samples_leaf &&
+         cr >= p.min_samples_leaf;
+}
+__device__ __forceinline__ uint32_t sat_sub(uint32_t a, uint32_t b) { return a > b ? a - b : 0u; }
+
+// Candidate key inside a feature: max gain, then earliest in scan order
+// (forward bins ascending, then backward bins descending; strict '>' in the
+// reference keeps the first maximum).
+__device__ __forceinline__ unsigned long long cand_key(float gain, uint32_t rank) {
+  return ((unsigned long long)ord_f32(gain) << 32) | (0xFFFFFFFFu - rank);
+}
+
+__global__ __launch_bounds__(256) void k_split(
+    const uint32_t* __restrict__ nodes, const uint32_t* __restrict__ slots,
+    const uint32_t* __restrict__ feats, int n_feats, const FeatMeta* __restrict__ fmeta, int FG,
+    int NG, int E, const long long* __restrict__ hist, int64_t slot_elems,
+    const long long* __restrict__ node_sum, const uint32_t* __restrict__ node_count,
+    const int32_t* __restrict__ qexp, uint32_t max_nodes, SplitParams p,
+    unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand) {
+  __shared__ double s_ratio[4][256];
+  __shared__ uint16_t s_order[4][256];
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int fi = blockIdx.x * 4 + wv;
+  const uint32_t node = nodes[blockIdx.y];
+  if (fi >= n_feats) return;
+  const uint32_t f = feats[fi];
+  const uint32_t slot = slots[blockIdx.y];
+  const uint32_t tree = node / max_nodes;
+  const double ig = ldexp(1.0, -qexp[2 * tree]), ih = ldexp(1.0, -qexp[2 * tree + 1]);
+  const long long G = node_sum[2 * node], H = node_sum[2 * node + 1];
+  const uint32_t cnt = node_count[node];
+  const double pg = (double)G * ig, ph = (double)H * ih;
+  const GainCtx ctx(pg, ph, p);
+  const FeatMeta fm = fmeta[f];
+  const uint32_t nb = fm.n_bins;
+  const long long* base = hist + (int64_t)slot * slot_elems;
+  const int grp = (int)f / FG, lf = (int)f % FG;
+  // Load bins 4*lane .. 4*lane+3.
+  long long vg[4], vh[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int b = 4 * lane + j;
+    const int64_t e = ((int64_t)(grp * 256 + b) * FG + lf) * 2;
+    const bool in = (uint32_t)b < nb;
+    vg[j] = in ? base[e] : 0;
+    vh[j] = in ? base[e + 1] : 0;
+  }
+  unsigned long long best = 0;  // 0 = no valid candidate
+  DevSplit out;
+  out.gain = -INFINITY;
+  out.feature = f;
+  out.bin = 0;
+  out.flags = 0;
+#pragma unroll
+  for (int w = 0; w < kMaxCatWords; w++) out.cats[w] = 0;
+
+  if (!fm.is_cat) {
+    if (nb >= 2) {
+      long long lg = vg[0] + vg[1] + vg[2] + vg[3], lh = vh[0] + vh[1] + vh[2] + vh[3];
+      const long long ig_incl = wave_incl_scan(lg), ih_incl = wave_incl_scan(lh);
+      long long pgv = ig_incl - lg, phv = ih_incl - lh;  // exclusive prefix before bin 4*lane
+      long long P[4], Q[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        pgv += vg[j];
+        phv += vh[j];
+        P[j] = pgv;
+        Q[j] = phv;
+      }
+      // Backward scan bound: bins [1, max_excl) accumulate right; the missing
+      // (last) bin stays left.
+      const uint32_t max_excl = fm.has_missing ? nb - 1 : nb;
+      const int me = (int)max_excl - 1;
+      // Totals up to bin me (inclusive) from the lane holding it.
+      long long Pme = 0, Qme = 0;
+      if (me >= 0) {
+        const int src = me >> 2, j = me & 3;
+        long long tg = P[0], th = Q[0];
+        if (j == 1) { tg = P[1]; th = Q[1]; }
+        if (j == 2) { tg = P[2]; th = Q[2]; }
+        if (j == 3) { tg = P[3]; th = Q[3]; }
+        Pme = (long long)__shfl((int)(tg & 0xFFFFFFFF), src, WAVE) & 0xFFFFFFFFLL;
+        Pme |= (long long)((unsigned long long)(unsigned)__shfl((int)(tg >> 32), src, WAVE) << 32);
+        Qme = (long long)__shfl((int)(th & 0xFFFFFFFF), src, WAVE) & 0xFFFFFFFFLL;
+        Qme |= (long long)((unsigned long long)(unsigned)__shfl((int)(th >> 32), src, WAVE) << 32);
+      }
+      // Exclusive prefix (bins < b) for the backward candidates.
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t b = 4 * lane + j;
+        // Forward: bins 0..b left (missing right), b in [0, nb-2].
+        if (b + 1 < nb) {
+          const long long GL = P[j], HL = Q[j];
+          const uint32_t cl = b + 1, cr = sat_sub(cnt, cl);
+          const double gl = (double)GL * ig, hl = (double)HL * ih;
+          const double gr = (double)(G - GL) * ig, hr = (double)(H - HL) * ih;
+          if (valid_split(p, hl, hr, cl, cr)) {
+            const float gn = ctx.gain(gl, hl, gr, hr);
+            const unsigned long long k = cand_key(gn, b);
+            if (k > best) best = k;
+          }
+        }
+        // Backward: bins b..me right, split at b-1, missing left; b in [1, me].
+        if (fm.has_missing && b >= 1 && (int)b <= me) {
+          const long long excl_g = P[j] - vg[j], excl_h = Q[j] - vh[j];
+          const long long GR = Pme - excl_g, HR = Qme - excl_h;
+          const uint32_t cr = max_excl - b, cl = sat_sub(cnt, cr);
+          const double gr = (double)GR * ig, hr = (double)HR * ih;
+          const double gl = (double)(G - GR) * ig, hl = (double)(H - HR) * ih;
+          if (valid_split(p, hl, hr, cl, cr)) {
+            const float gn = ctx.gain(gl, hl, gr, hr);
+            const unsigned long long k = cand_key(gn, 256u + (uint32_t)(me - (int)b));
+            if (k > best) best = k;
+          }
+        }
+      }
+      best = wave_max_ull(best);
+      if (best) {
+        const uint32_t rank = 0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFu);
+        out.gain = unord_f32((uint32_t)(best >> 32));
+        if (rank < 256u) {
+          out.bin = rank;
+          out.flags = 0;
+        } else {
+          const uint32_t b = (uint32_t)me - (rank - 256u);
+          out.bin = b - 1;
+          out.flags = 1;  // default_left
+        }
+      }
+    }
+  } else if (nb >= 2) {
+    const uint32_t nc = nb;
+    if (nc <= p.max_onehot_cats) {
+      // One-hot: each category alone goes left (find.rs:472-508).
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t k = 4 * lane + j;
+        if (k < nc) {
+          const double gl = (double)vg[j] * ig, hl = (double)vh[j] * ih;
+          const double gr = (double)(G - vg[j]) * ig, hr = (double)(H - vh[j]) * ih;
+          if (valid_split(p, hl, hr, 1u, sat_sub(cnt, 1u))) {
+            const float gn = ctx.gain(gl, hl, gr, hr);
+            const unsigned long long key = cand_key(gn, k);
+            if (key > best) best = key;
+          }
+        }
+      }
+      best = wave_max_ull(best);
+      if (best) {
+        const uint32_t k = 0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFu);
+        out.gain = unord_f32((uint32_t)(best >> 32));
+        out.flags = 2;
+        out.cats[k >> 5] = 1u << (k & 31);
+      }
+    } else {
+      // Sorted ratio partition (find.rs:511-573): categories with hess > 1e-10
+      // sorted stably by -G/(H+1e-6), then a prefix scan in that order.
+      double* ratio = s_ratio[wv];
+      uint16_t* order = s_order[wv];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t k = 4 * lane + j;
+        if (k < 256) {
+          const double hv = (double)vh[j] * ih;
+          const bool valid = k < nc && hv > 1e-10;
+          ratio[k] = valid ? -((double)vg[j] * ig) / (hv + 1e-6) : NAN;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      uint32_t m = 0;
+      for (uint32_t k = 0; k < nc; k++) m += isnan(ratio[k]) ? 0u : 1u;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t k = 4 * lane + j;
+        if (k < nc && !isnan(ratio[k])) {
+          const double rk = ratio[k];
+          uint32_t pos = 0;
+          for (uint32_t q = 0; q < nc; q++) {
+            const double rq = ratio[q];
+            if (!isnan(rq) && (rq < rk || (rq == rk && q < k))) pos++;
+          }
+          order[pos] = (uint16_t)k;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (m >= 2) {
+        // Positions 4*lane+j < m-1 are candidates; gather bin sums in sorted order.
+        long long og[4], oh[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint32_t q = 4 * lane + j;
+          og[j] = 0;
+          oh[j] = 0;
+          if (q < m) {
+            const uint32_t k = order[q];
+            const int src = (int)(k >> 2), jj = (int)(k & 3);
+            (void)src;
+            (void)jj;
+            const int64_t e = ((int64_t)(grp * 256 + (int)k) * FG + lf) * 2;
+            og[j] = base[e];
+            oh[j] = base[e + 1];
+          }
+        }
+        const long long sgl = og[0] + og[1] + og[2] + og[3], shl = oh[0] + oh[1] + oh[2] + oh[3];
+        long long pg_ = wave_incl_scan(sgl) - sgl, ph_ = wave_incl_scan(shl) - shl;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint32_t q = 4 * lane + j;
+          pg_ += og[j];
+          ph_ += oh[j];
+          if (q + 1 < m) {
+            const uint32_t cl = q + 1, cr = sat_sub(cnt, cl);
+            const double gl = (double)pg_ * ig, hl = (double)ph_ * ih;
+            const double gr = (double)(G - pg_) * ig, hr = (double)(H - ph_) * ih;
+            if (valid_split(p, hl, hr, cl, cr)) {
+              const float gn = ctx.gain(gl, hl, gr, hr);
+              const unsigned long long key = cand_key(gn, q);
+              if (key > best) best = key;
+            }
+          }
+        }
+        best = wave_max_ull(best);
+        if (best) {
+          const uint32_t qb = 0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFu);
+          out.gain = unord_f32((uint32_t)(best >> 32));
+          out.flags = 2;
+          for (uint32_t q = 0; q <= qb; q++) {
+            const uint32_t k = order[q];
+            out.cats[k >> 5] |= 1u << (k & 31);
+          }
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    cand[(int64_t)node * E + f] = out;
+    if (best) {
+      const uint32_t tb = p.rightmost ? f : (0xFFFFFFFFu - f);
+      atomicMax(&node_key[node], ((unsigned long long)ord_f32(out.gain) << 32) | tb);
+    }
+  }
+}
+
+__global__ void k_split_finalize(const uint32_t* __restrict__ nodes, int n_nodes, int E,
+                                 int rightmost, unsigned long long* __restrict__ node_key,
+                                 const DevSplit* __restrict__ cand,
+                                 DevSplit* __restrict__ node_split) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const uint32_t node = nodes[i];
+  const unsigned long long k = node_key[node];
+  node_key[node] = 0;
+  DevSplit s;
+  if (k == 0) {
+    s.gain = -INFINITY;
+    s.feature = 0;
+    s.bin = 0;
+    s.flags = 0;
+    for (int w = 0; w < kMaxCatWords; w++) s.cats[w] = 0;
+  } else {
+    const uint32_t lo = (uint32_t)(k & 0xFFFFFFFFu);
+    const uint32_t f = rightmost ? lo : (0xFFFFFFFFu - lo);
+    s = cand[(int64_t)node * E + f];
+  }
+  node_split[node] = s;
+}
+
+void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, const uint32_t* slots,
+                  int n_nodes, const uint32_t* feats, int n_feats, const long long* hist,
+                  int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
+                  const int32_t* qexp, uint32_t max_nodes, SplitParams p,
+                  unsigned long long* node_key, DevSplit* cand, DevSplit* node_split) {
+  if (n_nodes <= 0) return;
+  dim3 grid((unsigned)((n_feats + 3) / 4), (unsigned)n_nodes);
+  hipLaunchKernelGGL(k_split, grid, dim3(256), 0, s, nodes, slots, feats, n_feats, d.meta, d.FG,
+                     d.NG, d.E, hist, slot_elems, node_sum, node_count, qexp, max_nodes, p,
+                     node_key, cand);
+  hipLaunchKernelGGL(k_split_finalize, dim3((n_nodes + 63) / 64), dim3(64), 0, s, nodes, n_nodes,
+                     d.E, p.rightmost, node_key, cand, node_split);
+}
+
+// ===========================================================================
+// Stable partition (ballot + prefix sums), three passes
+// ===========================================================================
+__device__ __forceinline__ bool goes_left(const DevSplit& sp, const FeatMeta& fm, uint32_t bin) {
+  if (fm.has_missing && bin == fm.default_bin) return (sp.flags & 1u) != 0;
+  if (sp.flags & 2u) return bin < 256u && ((sp.cats[bin >> 5] >> (bin & 31)) & 1u);
+  return bin <= sp.bin;
+}
+
+constexpr int kPartThreads = 256;
+
+// Pass 1: per tile, count rows that go left; leave a flag per position.
+__global__ __launch_bounds__(kPartThreads) void k_part_count(
+    const PartItem* __restrict__ items, const PartTile* __restrict__ tiles,
+    const DevSplit* __restrict__ node_split, const FeatMeta* __restrict__ fmeta,
+    const uint8_t* __restrict__ bins_cm, const uint32_t* __restrict__ hi_cm, int64_t n,
+    int64_t hi_words, const uint32_t* __restrict__ idx, uint8_t* __restrict__ flag,
+    uint32_t* __restrict__ tile_left) {
+  __shared__ uint32_t s_cnt[kPartThreads / WAVE];
+  const PartTile t = tiles[blockIdx.x];
+  const PartItem it = items[t.item];
+  const DevSplit sp = node_split[it.node];
+  const FeatMeta fm = fmeta[it.feature];
+  const uint8_t* col = bins_cm + (int64_t)it.feature * n;
+  const uint32_t* hcol = hi_cm ? hi_cm + (int64_t)it.feature * hi_words : nullptr;
+  uint32_t c = 0;
+  for (uint32_t p = threadIdx.x; p < t.count; p += kPartThreads) {
+    const uint32_t pos = t.begin + p;
+    const uint32_t row = idx[pos];
+    uint32_t bin = col[row];
+    if (hcol) bin |= ((hcol[row >> 5] >> (row & 31)) & 1u) << 8;
+    const bool l = goes_left(sp, fm, bin);
+    flag[pos] = l ? 1 : 0;
+    c += l ? 1u : 0u;
+  }
+  for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, WAVE);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_cnt[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int i = 0; i < kPartThreads / WAVE; i++) s += s_cnt[i];
+    tile_left[blockIdx.x] = s;
+  }
+}
+
+// Pass 2: exclusive scan of left counts over the tiles of each node.
+__global__ void k_part_scan(const PartItem* __restrict__ items, int n_items,
+                            const uint32_t* __restrict__ tile_left, uint32_t* __restrict__ tile_off,
+                            uint32_t* __restrict__ item_left) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += gridDim.x * blockDim.x) {
+    const PartItem it = items[i];
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < it.n_tiles; k++) {
+      tile_off[it.tile0 + k] = s;
+      s += tile_left[it.tile0 + k];
+    }
+    item_left[i] = s;
+  }
+}
+
+// Pass 3: scatter each tile's rows to their stable destinations in tmp, then
+// (pass 4) copy the node ranges back.
+__global__ __launch_bounds__(kPartThreads) void k_part_scatter(
+    const PartItem* __restrict__ items, const PartTile* __restrict__ tiles,
+    const uint32_t* __restrict__ tile_off, const uint32_t* __restrict__ item_left,
+    const uint8_t* __restrict__ flag, const uint32_t* __restrict__ idx,
+    const float2* __restrict__ ogh, uint32_t* __restrict__ idx_tmp,
+    float2* __restrict__ ogh_tmp) {
+  __shared__ uint32_t s_wave[kPartThreads / WAVE];
+  __shared__ uint32_t s_carry;
+  const PartTile t = tiles[blockIdx.x];
+  const PartItem it = items[t.item];
+  const uint32_t L = item_left[t.item];
+  const uint32_t lbefore = tile_off[blockIdx.x];
+  const uint32_t rbefore = (t.begin - it.begin) - lbefore;
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t p0 = 0; p0 < t.count; p0 += kPartThreads) {
+    const uint32_t p = p0 + threadIdx.x;
+    const bool in = p < t.count;
+    const bool l = in && flag[t.begin + p];
+    const unsigned long long bal = __ballot(l);
+    const uint32_t below = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave[wv] = __popcll(bal);
+    __syncthreads();
+    uint32_t woff = s_carry;
+    for (int k = 0; k < wv; k++) woff += s_wave[k];
+    if (in) {
+      const uint32_t lrank = woff + below;          // lefts before p in this tile
+      const uint32_t rrank = p - lrank;             // rights before p in this tile
+      const uint32_t dst = l ? it.begin + lbefore + lrank : it.begin + L + rbefore + rrank;
+      idx_tmp[dst] = idx[t.begin + p];
+      ogh_tmp[dst] = ogh[t.begin + p];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t s = s_carry;
+      for (int k = 0; k < kPartThreads / WAVE; k++) s += s_wave[k];
+      s_carry = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kPartThreads) void k_part_copyback(
+    const PartTile* __restrict__ tiles, const uint32_t* __restrict__ idx_tmp,
+    const float2* __restrict__ ogh_tmp, uint32_t* __restrict__ idx, float2* __restrict__ ogh) {
+  const PartTile t = tiles[blockIdx.x];
+  for (uint32_t p = threadIdx.x; p < t.count; p += kPartThreads) {
+    idx[t.begin + p] = idx_tmp[t.begin + p];
+    ogh[t.begin + p] = ogh_tmp[t.begin + p];
+  }
+}
+
+// Children bookkeeping after the scan: counts of both children, the parent
+// total copied to the right child, and the smaller child's total cleared
+// (it is rebuilt from its rows; the larger one is parent - smaller).
+__global__ void k_part_children(const PartItem* __restrict__ items, int n_items,
+                                const uint32_t* __restrict__ item_left,
+                                uint32_t* __restrict__ node_count,
+                                long long* __restrict__ node_sum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_items) return;
+  const PartItem it = items[i];
+  const uint32_t L = item_left[i], R = it.count - L;
+  node_count[it.node] = L;
+  node_count[it.right_node] = R;
+  const long long g = node_sum[2 * it.node], h = node_sum[2 * it.node + 1];
+  node_sum[2 * it.right_node] = g;
+  node_sum[2 * it.right_node + 1] = h;
+  const uint32_t small = L <= R ? it.node : it.right_node;
+  node_sum[2 * small] = 0;
+  node_sum[2 * small + 1] = 0;
+}
+
+void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items, int n_items,
+                      const PartTile* tiles, int n_tiles, const DevSplit* node_split,
+                      uint32_t* idx, float2* ogh, uint32_t* idx_tmp, float2* ogh_tmp,
+                      uint8_t* flag, uint32_t* tile_left, uint32_t* tile_off,
+                      uint32_t* item_left, uint32_t* node_count, long long* node_sum) {
+  if (n_items <= 0) return;
+  hipLaunchKernelGGL(k_part_count, dim3(n_tiles), dim3(kPartThreads), 0, s, items, tiles,
+                     node_split, d.meta, d.bins_cm, d.hi_cm, d.n, d.hi_words, idx, flag,
+                     tile_left);
+  hipLaunchKernelGGL(k_part_scan, dim3((n_items + 255) / 256), dim3(256), 0, s, items, n_items,
+                     tile_left, tile_off, item_left);
+  hipLaunchKernelGGL(k_part_scatter, dim3(n_tiles), dim3(kPartThreads), 0, s, items, tiles,
+                     tile_off, item_left, flag, idx, ogh, idx_tmp, ogh_tmp);
+  hipLaunchKernelGGL(k_part_copyback, dim3(n_tiles), dim3(kPartThreads), 0, s, tiles, idx_tmp,
+                     ogh_tmp, idx, ogh);
+  hipLaunchKernelGGL(k_part_children, dim3((n_items + 63) / 64), dim3(64), 0, s, items, n_items,
+                     item_left, node_count, node_sum);
+}
+
+// ===========================================================================
+// Prediction updates and traversal
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_leaf_update(const Segment* __restrict__ segs,
+                                                     const float* __restrict__ leaf_val,
+                                                     const uint32_t* __restrict__ idx,
+                                                     int64_t tree_stride, float* __restrict__ preds,
+                                                     const uint32_t* __restrict__ tree_output,
+                                                     int64_t n_rows) {
+  const Segment sg = segs[blockIdx.x];
+  const float v = leaf_val[sg.node];
+  const uint32_t* t = idx + (int64_t)sg.tree * tree_stride;
+  float* pr = preds + (int64_t)tree_output[sg.tree] * n_rows;
+  for (uint32_t p = threadIdx.x; p < sg.count; p += blockDim.x) pr[t[sg.begin + p]] += v;
+}
+
+void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const float* leaf_val,
+                        const uint32_t* idx, int64_t tree_stride, float* preds,
+                        const uint32_t* tree_output, int64_t n_rows) {
+  if (n_segs <= 0) return;
+  hipLaunchKernelGGL(k_leaf_update, dim3(n_segs), dim3(256), 0, s, segs, leaf_val, idx,
+                     tree_stride, preds, tree_output, n_rows);
+}
+
+__device__ __forceinline__ float sigmoid32(float x) {  // transform.rs:106-115
+  // f32::clamp keeps NaN (fminf/fmaxf would not).
+  float c = x;
+  if (c < -500.0f) c = -500.0f;
+  if (c > 500.0f) c = 500.0f;
+  if (c >= 0.0f) return 1.0f / (1.0f + exp32(-c));
+  const float e = exp32(c);
+  return e / (1.0f + e);
+}
+
+constexpr int kPredMaxK = 16;
+
+// One lane per row; the block's rows are staged feature-major in LDS so node
+// tests read s_x[f][lane] (consecutive lanes, no bank conflicts).  Trees are
+// visited in forest order and each group accumulates in f32 from its base
+// score, the reference's exact summation order.
+template <bool STAGE, bool ACCUM>
+__global__ __launch_bounds__(256) void k_predict(
+    const DevNode* __restrict__ nodes, const int64_t* __restrict__ tree_off,
+    const uint32_t* __restrict__ tree_group, int64_t n_trees, const uint32_t* __restrict__ cat_start,
+    const uint32_t* __restrict__ cat_size, const uint32_t* __restrict__ cat_words,
+    const float* __restrict__ base, int K, const float* __restrict__ X, int64_t n,
+    int64_t x_stride, int F, int transform, float* __restrict__ out, int64_t out_stride,
+    uint32_t* __restrict__ leaf_idx) {
+  extern __shared__ float s_x[];  // [F][blockDim.x]
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = row < n;
+  if (STAGE) {
+    for (int f = 0; f < F; f++) s_x[f * blockDim.x + threadIdx.x] = in ? X[(int64_t)f * x_stride + row] : 0.0f;
+    __syncthreads();
+  }
+  if (!in) return;
+  float acc[kPredMaxK];
+#pragma unroll
+  for (int k = 0; k < kPredMaxK; k++)
+    acc[k] = k < K ? (ACCUM ? out[(int64_t)k * out_stride + row] : base[k]) : 0.0f;
+  for (int64_t t = 0; t < n_trees; t++) {
+    const int64_t b = tree_off[t];
+    uint32_t node = 0;
+    DevNode nd = nodes[b];
+    while (!(nd.meta & kNodeLeaf)) {
+      const uint32_t f = nd.meta & 0xFFFFFFu;
+      const float x = STAGE ? s_x[f * blockDim.x + threadIdx.x] : X[(int64_t)f * x_stride + row];
+      if (isnan(x)) {
+        node = (nd.meta & kNodeDefLeft) ? nd.left : nd.right;
+      } else if (!(nd.meta & kNodeCat)) {
+        node = x < nd.value ? nd.left : nd.right;
+      } else {
+        const uint32_t cat = x <= 0.0f ? 0u : (x >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)x);
+        const uint32_t sz = cat_size[b + node], w = cat >> 5;
+        const bool right = sz != 0 && w < sz && ((cat_words[cat_start[b + node] + w] >> (cat & 31)) & 1u);
+        node = right ? nd.right : nd.left;
+      }
+      nd = nodes[b + node];
+    }
+    const uint32_t g = tree_group[t];
+#pragma unroll
+    for (int k = 0; k < kPredMaxK; k++)
+      if ((uint32_t)k == g) acc[k] += nd.value;
+    if (leaf_idx) leaf_idx[t * n + row] = node;
+  }
+  if (transform == 1) {
+#pragma unroll
+    for (int k = 0; k < kPredMaxK; k++)
+      if (k < K) acc[k] = sigmoid32(acc[k]);
+  } else if (transform == 2) {  // softmax, transform.rs:83-103
+    float mx = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < kPredMaxK; k++)
+      if (k < K) mx = fmaxf(mx, acc[k]);
+    float ssum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kPredMaxK; k++)
+      if (k < K) {
+        acc[k] = exp32(acc[k] - mx);
+        ssum += acc[k];
+      }
+    if (ssum > 0.0f) {
+#pragma unroll
+      for (int k = 0; k < kPredMaxK; k++)
+        if (k < K) acc[k] /= ssum;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPredMaxK; k++)
+    if (k < K) out[(int64_t)k * out_stride + row] = acc[k];
+}
+
+int predict_rows_per_block(int F) {
+  // Stage the block's rows in LDS when F x rows x 4 B stays within 64 KiB.
+  for (int rb = 256; rb >= 64; rb >>= 1)
+    if ((int64_t)F * rb * 4 <= 65536) return rb;
+  return 0;  // too many features to stage: read the raw matrix directly
+}
+
+void launch_predict(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
+                    const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
+                    const uint32_t* cat_size, const uint32_t* cat_words, const float* base, int K,
+                    const float* X, int64_t n, int64_t x_stride, int F, int transform, float* out,
+                    int64_t out_stride, uint32_t* leaf_idx) {
+  if (n <= 0) return;
+  const int rb = predict_rows_per_block(F);
+  if (rb > 0) {
+    hipLaunchKernelGGL((k_predict<true, false>), dim3((unsigned)((n + rb - 1) / rb)), dim3(rb),
+                       (size_t)F * rb * 4, s, nodes, tree_off, tree_group, n_trees, cat_start,
+                       cat_size, cat_words, base, K, X, n, x_stride, F, transform, out, out_stride,
+                       leaf_idx);
+  } else {
+    hipLaunchKernelGGL((k_predict<false, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       nodes, tree_off, tree_group, n_trees, cat_start, cat_size, cat_words, base,
+                       K, X, n, x_stride, F, transform, out, out_stride, leaf_idx);
+  }
+}
+
+// out[k][row] += leaf of each listed tree, trees in order (training-time and
+// validation prediction updates, trainer.rs:331-347).
+void launch_predict_accum(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
+                          const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
+                          const uint32_t* cat_size, const uint32_t* cat_words, int K,
+                          const float* X, int64_t n, int64_t x_stride, int F, float* out,
+                          int64_t out_stride) {
+  if (n <= 0) return;
+  const int rb = predict_rows_per_block(F);
+  if (rb > 0) {
+    hipLaunchKernelGGL((k_predict<true, true>), dim3((unsigned)((n + rb - 1) / rb)), dim3(rb),
+                       (size_t)F * rb * 4, s, nodes, tree_off, tree_group, n_trees, cat_start,
+                       cat_size, cat_words, nullptr, K, X, n, x_stride, F, 0, out, out_stride,
+                       nullptr);
+  } else {
+    hipLaunchKernelGGL((k_predict<false, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       s, nodes, tree_off, tree_group, n_trees, cat_start, cat_size, cat_words,
+                       nullptr, K, X, n, x_stride, F, 0, out, out_stride, nullptr);
+  }
+}
+
+// Metric partial sums per block: (sum w*loss, sum w), f64.  RMSE gets one row
+// of partials per output (metrics/regression.rs:17-40); logloss and
+// multiclass logloss clamp the transformed probability to [1e-15, 1-1e-15]
+// (metrics/classification.rs:13-40, 332-360).
+__global__ __launch_bounds__(256) void k_metric(int metric, int transform,
+                                                const float* __restrict__ preds, int K,
+                                                const float* __restrict__ y,
+                                                const float* __restrict__ w, int64_t n,
+                                                double* __restrict__ partial) {
+  __shared__ double s_a[256 / WAVE], s_b[256 / WAVE];
+  const int k = blockIdx.y;
+  const double EPS = 1e-15;
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float wi = w ? w[i] : 1.0f;
+    if (metric == 1) {
+      const double d = (double)preds[(int64_t)k * n + i] - (double)y[i];
+      a += (double)wi * d * d;
+    } else if (metric == 2) {
+      const float p = transform == 1 ? sigmoid32(preds[i]) : preds[i];
+      const double q = fmin(fmax((double)p, EPS), 1.0 - EPS), l = (double)y[i];
+      a += (double)wi * (-(l * log(q) + (1.0 - l) * log(1.0 - q)));
+    } else {
+      float mx = -INFINITY;
+      for (int c = 0; c < K; c++) mx = fmaxf(mx, preds[(int64_t)c * n + i]);
+      float ssum = 0.0f;
+      for (int c = 0; c < K; c++) ssum += exp32(preds[(int64_t)c * n + i] - mx);
+      const int lab = (int)rint((double)y[i]);
+      float p = exp32(preds[(int64_t)lab * n + i] - mx);
+      if (ssum > 0.0f) p /= ssum;
+      const double q = fmin(fmax((double)p, EPS), 1.0 - EPS);
+      a += (double)wi * (-log(q));
+    }
+    b += (double)wi;
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    a += __shfl_xor(a, m, WAVE);
+    b += __shfl_xor(b, m, WAVE);
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    s_a[threadIdx.x / WAVE] = a;
+    s_b[threadIdx.x / WAVE] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0, tb = 0;
+    for (int i = 0; i < 256 / WAVE; i++) {
+      ta += s_a[i];
+      tb += s_b[i];
+    }
+    partial[((int64_t)k * gridDim.x + blockIdx.x) * 2] = ta;
+    partial[((int64_t)k * gridDim.x + blockIdx.x) * 2 + 1] = tb;
+  }
+}
+
+void launch_metric(hipStream_t s, int metric, int transform, const float* preds, int K,
+                   const float* y, const float* w, int64_t n, double* partial, int n_blocks) {
+  const int ky = metric == 1 ? K : 1;
+  hipLaunchKernelGGL(k_metric, dim3(n_blocks, ky), dim3(256), 0, s, metric, transform, preds, K,
+                     y, w, n, partial);
+}
+
+}  // namespace bst

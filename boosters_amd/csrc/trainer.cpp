@@ -1,0 +1,799 @@
+// trainer.cpp — the boosting loop and tree growth, driving the device kernels.
+//
+// Control flow restates (paths relative to crates/boosters/src/):
+//   GBDTTrainer::train          training/gbdt/trainer.rs:162-401
+//   TreeGrower::grow            training/gbdt/grower.rs:265-342
+//   process_candidate           grower.rs:346-481 (subtraction trick, child sums)
+//   finalize_remaining          grower.rs:484-514
+//   GrowthState                 training/gbdt/expansion.rs:112-231 (depth-wise level queue,
+//                               leaf-wise max-heap by (gain, partition id), n_leaves++ on pop)
+//   apply_split_to_builder      grower.rs:576-650 (next_up thresholds, categorical swap)
+//   compute_leaf_weight         split/gain.rs:151-168
+//   update_predictions          grower.rs:208-219 (partition path) / trainer.rs:331-339
+//
+// Schedules: PLAIN expands one node per launch sequence (partition -> sync ->
+// histogram/subtract/split -> sync); BATCHED expands every chosen node of all
+// trees grown together (a depth level, or one pop per class tree of a
+// softmax round) with one launch per kernel.  Both produce identical trees:
+// node processing order, partition ids and tie rules are the same.
+#include "trainer.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+
+namespace bst {
+
+namespace {
+
+constexpr uint32_t kHistTile = 8192;   // rows per histogram block
+constexpr uint32_t kPartTile = 4096;   // rows per partition block
+constexpr uint32_t kSegTile = 16384;   // rows per sums / leaf-update block
+constexpr uint32_t kDirect = 0x80000000u;
+
+struct Xoshiro {  // xoshiro256++ seeded by splitmix64
+  uint64_t s[4];
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  explicit Xoshiro(uint64_t seed) {
+    for (int i = 0; i < 4; i++) {
+      seed += 0x9e3779b97f4a7c15ULL;
+      uint64_t z = seed;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+      s[i] = z ^ (z >> 31);
+    }
+  }
+  uint64_t next() {
+    const uint64_t r = rotl(s[0] + s[3], 23) + s[0];
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  uint64_t range_incl(uint64_t lo, uint64_t hi) {
+    const uint64_t range = hi - lo + 1;
+    if (range == 0) return next();
+    const uint64_t zone = (range << __builtin_clzll(range)) - 1;
+    for (;;) {
+      const uint64_t v = next();
+      const unsigned __int128 m = (unsigned __int128)v * range;
+      if ((uint64_t)m <= zone) return lo + (uint64_t)(m >> 64);
+    }
+  }
+};
+
+DevSplit invalid_split() {
+  DevSplit s;
+  std::memset(&s, 0, sizeof(s));
+  s.gain = -std::numeric_limits<float>::infinity();
+  return s;
+}
+
+template <class T>
+void upload(hipStream_t s, DevBuf<T>& d, const std::vector<T>& h) {
+  if (h.empty()) return;
+  d.ensure(h.size());
+  h2d(s, d.p, h.data(), h.size());
+}
+
+int default_metric(int obj) {
+  return obj == BST_OBJ_LOGISTIC ? BST_METRIC_LOGLOSS
+                                 : (obj == BST_OBJ_SOFTMAX ? BST_METRIC_MLOGLOSS : BST_METRIC_RMSE);
+}
+
+}  // namespace
+
+// Device metric partial sums (metrics/{classification,regression}.rs), launched from here.
+void launch_metric(hipStream_t s, int metric, int transform, const float* preds, int K,
+                   const float* y, const float* w, int64_t n, double* partial, int n_blocks);
+void launch_predict_accum(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
+                          const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
+                          const uint32_t* cat_size, const uint32_t* cat_words, int K,
+                          const float* X, int64_t n, int64_t x_stride, int F, float* out,
+                          int64_t out_stride);
+
+struct Trainer::TreeState {
+  int t = 0;
+  int output = 0;
+  TreeBuild tb;
+  std::vector<uint32_t> pbegin, pcount, pslot;
+  uint32_t n_part = 1;
+  std::vector<uint32_t> free_slots;
+  std::vector<float> leaf_lr;
+  std::vector<Cand> level, next;
+  std::priority_queue<Cand, std::vector<Cand>, CandLess> heap;
+  uint32_t n_leaves = 1;
+  int64_t rows = 0;
+  bool sampled = false;
+};
+
+Trainer::Trainer(bst_ctx* ctx, bst_dataset* ds, const TrainOptions& opt)
+    : ctx_(ctx), ds_(ds), opt_(opt), s_(ctx->stream), n_(ds->d.n) {
+  const auto& c = opt_.cfg;
+  sched_ = opt_.ext.schedule;
+  lr_ = c.learning_rate;
+  const uint32_t max_leaves = c.growth == BST_GROW_DEPTHWISE ? (1u << c.max_depth) : c.max_leaves;
+  MN_ = (uint32_t)std::max<uint64_t>(2ull * max_leaves - 1, 63) + 2;
+  SPT_ = max_leaves + 2;
+  const bool col_sampling = !(std::fabs(c.colsample_bytree - 1.0f) < 1e-6f);
+  T_ = (sched_ != BST_SCHED_PLAIN && opt_.K > 1 && !col_sampling) ? opt_.K : 1;
+  slot_elems_ = (int64_t)ds->d.NG * 256 * ds->d.FG * 2;
+  rightmost_ = opt_.ext.n_threads != 1 && ds->d.E >= 16;
+  sp_.lambda = (double)c.lambda;
+  sp_.min_gain = (double)c.min_gain;
+  sp_.min_child_weight = (double)c.min_child_weight;
+  sp_.min_samples_leaf = c.min_samples_leaf;
+  sp_.max_onehot_cats = c.max_onehot_cats;
+  sp_.rightmost = rightmost_ ? 1 : 0;
+
+  const int K = opt_.K;
+  preds_.alloc((size_t)K * n_);
+  g_.alloc((size_t)K * n_);
+  h_.alloc((size_t)K * n_);
+  absmax_.alloc(2 * K);
+  qexp_.alloc(2 * K);
+  idx_.alloc((size_t)T_ * n_);
+  idx_tmp_.alloc((size_t)T_ * n_);
+  ogh_.alloc((size_t)T_ * n_);
+  ogh_tmp_.alloc((size_t)T_ * n_);
+  flag_.alloc((size_t)T_ * n_);
+  node_sum_.alloc((size_t)2 * T_ * MN_);
+  node_count_.alloc((size_t)T_ * MN_);
+  node_key_.alloc((size_t)T_ * MN_);
+  node_split_.alloc((size_t)T_ * MN_);
+  cand_.alloc((size_t)T_ * MN_ * ds->d.E);
+  hist_.alloc((size_t)T_ * SPT_ * slot_elems_);
+  leaf_val_.alloc((size_t)T_ * MN_);
+  tree_output_.alloc(T_);
+  feats_.alloc(ds->d.E);
+  BST_HIP(hipMemsetAsync(node_key_.p, 0, node_key_.bytes(), s_));
+}
+
+uint32_t Trainer::alloc_slot(TreeState& ts) {
+  if (ts.free_slots.empty()) raise(BST_E_OOM, "histogram slots exhausted");
+  const uint32_t s = ts.free_slots.back();
+  ts.free_slots.pop_back();
+  return s;
+}
+
+void Trainer::make_leaf(TreeState& ts, const Cand& c) {
+  const double G = std::ldexp((double)c.g, -qexp_h_[2 * ts.output]);
+  const double H = std::ldexp((double)c.h, -qexp_h_[2 * ts.output + 1]);
+  const auto& cfg = opt_.cfg;
+  float w;
+  const double lam = (double)cfg.lambda, al = (double)cfg.alpha;
+  if (al == 0.0) {
+    w = (float)(-G / (H + lam));
+  } else {
+    const double ag = std::fabs(G);
+    w = ag <= al ? 0.0f : (float)((G > 0.0 ? -1.0 : 1.0) * (ag - al) / (H + lam));
+  }
+  ts.tb.is_leaf[c.tree_node] = 1;
+  ts.tb.leaf[c.tree_node] = w;
+  ts.tb.cover[c.tree_node] = (float)H;
+  ts.leaf_lr[c.node] = w * lr_;
+  if (ts.pslot[c.node] != UINT32_MAX) {
+    ts.free_slots.push_back(ts.pslot[c.node]);
+    ts.pslot[c.node] = UINT32_MAX;
+  }
+}
+
+// Root histograms and splits for every tree grown this round.
+void Trainer::root(std::vector<TreeState>& trees) {
+  std::vector<HistItem> items;
+  std::vector<uint32_t> zero, nodes, slots;
+  std::vector<uint32_t> counts((size_t)T_ * MN_, 0);
+  bool gather = false;
+  for (auto& ts : trees) {
+    const uint32_t gnode = ts.t * MN_;
+    ts.pslot[0] = alloc_slot(ts);
+    const uint32_t gslot = ts.t * SPT_ + ts.pslot[0];
+    const uint32_t m = (uint32_t)ts.rows;
+    const uint32_t nt = std::max<uint32_t>(1, (m + kHistTile - 1) / kHistTile);
+    if (nt > 1 || m == 0) zero.push_back(gslot);
+    for (uint32_t k = 0; k < nt && m > 0; k++) {
+      const uint32_t b = k * kHistTile, cnt = std::min(kHistTile, m - b);
+      for (int gr = 0; gr < ds_->d.NG; gr++)
+        items.push_back({gnode, gslot, (uint32_t)(ts.t * n_) + b, cnt, (uint32_t)gr,
+                         (uint32_t)ts.t | (nt == 1 ? kDirect : 0u)});
+    }
+    counts[gnode] = m;
+    nodes.push_back(gnode);
+    slots.push_back(gslot);
+    gather = gather || ts.sampled;
+  }
+  upload(s_, d_hist_items_, items);
+  upload(s_, d_u32a_, zero);
+  upload(s_, d_u32b_, nodes);
+  upload(s_, d_u32c_, slots);
+  h2d(s_, node_count_.p, counts.data(), counts.size());
+  const int32_t* qexp = qexp_.p + 2 * trees[0].output;
+  launch_zero_slots(s_, d_u32a_.p, (int)zero.size(), hist_.p, slot_elems_);
+  double rows = 0;
+  for (auto& ts : trees) rows += (double)ts.rows;
+  int pi = ctx_->prof_begin(0, hist_bytes(rows, (int)trees.size(), gather));
+  launch_hist(s_, ds_->d, d_hist_items_.p, (int)items.size(), idx_.p, ogh_.p, 0, qexp, hist_.p,
+              slot_elems_, node_sum_.p, gather, 0);
+  ctx_->prof_end(pi);
+  pi = ctx_->prof_begin(1, split_bytes((int)nodes.size()));
+  launch_split(s_, ds_->d, d_u32b_.p, d_u32c_.p, (int)nodes.size(), feats_.p, (int)feats_h_.size(),
+               hist_.p, slot_elems_, node_sum_.p, node_count_.p, qexp, MN_, sp_, node_key_.p,
+               cand_.p, node_split_.p);
+  ctx_->prof_end(pi);
+  std::vector<long long> sums(2 * nodes.size());
+  std::vector<DevSplit> splits(nodes.size());
+  for (size_t i = 0; i < nodes.size(); i++) {
+    d2h(s_, &sums[2 * i], node_sum_.p + 2 * nodes[i], 2);
+    d2h(s_, &splits[i], node_split_.p + nodes[i], 1);
+  }
+  qexp_h_.resize(2 * opt_.K);
+  d2h(s_, qexp_h_.data(), qexp_.p, 2 * opt_.K);
+  BST_HIP(hipStreamSynchronize(s_));
+  ctx_->prof_drain();
+  stats.hist_rows += 0;
+  for (size_t i = 0; i < trees.size(); i++) {
+    auto& ts = trees[i];
+    stats.hist_rows += ts.rows;
+    const uint32_t root = ts.tb.alloc();
+    Cand c{0, root, 0, splits[i], sums[2 * i], sums[2 * i + 1], (uint32_t)ts.rows};
+    if (opt_.cfg.growth == BST_GROW_DEPTHWISE)
+      ts.level.push_back(c);
+    else
+      ts.heap.push(c);
+  }
+}
+
+namespace {
+struct Pending {
+  int t;
+  Cand c;
+  uint32_t lt, rt, right;
+};
+}  // namespace
+
+void Trainer::expand_batch(std::vector<TreeState>& trees,
+                           std::vector<std::pair<int, Cand>>& chosen) {
+  const auto& cfg = opt_.cfg;
+  std::vector<PartItem> pitems;
+  std::vector<PartTile> ptiles;
+  std::vector<Pending> pend;
+  for (auto& tc : chosen) {
+    TreeState& ts = trees[tc.first];
+    const Cand& c = tc.second;
+    if (!(c.split.gain > -std::numeric_limits<float>::infinity()) || c.split.gain <= cfg.min_gain) {
+      make_leaf(ts, c);
+      continue;
+    }
+    const double H = std::ldexp((double)c.h, -qexp_h_[2 * ts.output + 1]);
+    ts.tb.gain[c.tree_node] = c.split.gain;
+    ts.tb.cover[c.tree_node] = (float)H;
+    // apply_split_to_builder
+    const uint32_t e = c.split.feature;
+    const uint32_t of = ds_->eff_to_orig[e];
+    const uint32_t a = ts.tb.alloc(), b = ts.tb.alloc();
+    ts.tb.split_index[c.tree_node] = of;
+    ts.tb.left[c.tree_node] = a;
+    ts.tb.right[c.tree_node] = b;
+    uint32_t lt, rt;
+    if (!(c.split.flags & 2u)) {
+      ts.tb.thr[c.tree_node] = ds_->thr_up[(size_t)e * 256 + c.split.bin];
+      ts.tb.dleft[c.tree_node] = (uint8_t)(c.split.flags & 1u);
+      ts.tb.stype[c.tree_node] = 0;
+      lt = a;
+      rt = b;
+    } else {
+      int maxc = -1;
+      for (int k = 0; k < 256; k++)
+        if ((c.split.cats[k >> 5] >> (k & 31)) & 1u) maxc = k;
+      std::vector<uint32_t> words;
+      if (maxc >= 0) {
+        words.assign((size_t)(maxc >> 5) + 1, 0u);
+        for (int k = 0; k <= maxc; k++)
+          if ((c.split.cats[k >> 5] >> (k & 31)) & 1u) words[k >> 5] |= 1u << (k & 31);
+      }
+      ts.tb.thr[c.tree_node] = 0.0f;
+      ts.tb.dleft[c.tree_node] = (c.split.flags & 1u) ? 0 : 1;
+      ts.tb.stype[c.tree_node] = 1;
+      ts.tb.cat_nodes.push_back({c.tree_node, words});
+      lt = b;
+      rt = a;
+    }
+    const uint32_t right = ts.n_part++;
+    const uint32_t begin = (uint32_t)(ts.t * n_) + ts.pbegin[c.node];
+    const uint32_t nt = std::max<uint32_t>(1, (c.count + kPartTile - 1) / kPartTile);
+    pitems.push_back({ts.t * MN_ + c.node, begin, c.count, (uint32_t)ptiles.size(), nt,
+                      ts.t * MN_ + right, e, 0});
+    for (uint32_t k = 0; k < nt; k++) {
+      const uint32_t b0 = k * kPartTile;
+      ptiles.push_back({(uint32_t)pitems.size() - 1, begin + b0,
+                        c.count > b0 ? std::min(kPartTile, c.count - b0) : 0u, 0});
+    }
+    pend.push_back({tc.first, c, lt, rt, right});
+  }
+  if (pend.empty()) return;
+
+  upload(s_, d_part_items_, pitems);
+  upload(s_, d_part_tiles_, ptiles);
+  tile_left_.ensure(ptiles.size());
+  tile_off_.ensure(ptiles.size());
+  item_left_.ensure(pitems.size());
+  double prow = 0;
+  for (auto& it : pitems) prow += it.count;
+  int pi = ctx_->prof_begin(2, prow * (4 + 8 + 12 + 1));
+  launch_partition(s_, ds_->d, d_part_items_.p, (int)pitems.size(), d_part_tiles_.p,
+                   (int)ptiles.size(), node_split_.p, idx_.p, ogh_.p, idx_tmp_.p, ogh_tmp_.p,
+                   flag_.p, tile_left_.p, tile_off_.p, item_left_.p, node_count_.p, node_sum_.p);
+  ctx_->prof_end(pi);
+  std::vector<uint32_t> left(pitems.size());
+  d2h(s_, left.data(), item_left_.p, left.size());
+  BST_HIP(hipStreamSynchronize(s_));
+  ctx_->prof_drain();
+
+  std::vector<HistItem> hitems;
+  std::vector<uint32_t> zero, quads_full, quads_sums, snodes, sslots;
+  std::vector<Segment> segs;
+  bool leafwise = cfg.growth == BST_GROW_LEAFWISE;
+  std::vector<uint8_t> need(pend.size());
+  for (size_t i = 0; i < pend.size(); i++) {
+    Pending& p = pend[i];
+    TreeState& ts = trees[p.t];
+    const uint32_t node = p.c.node, right = p.right;
+    const uint32_t L = left[i], R = p.c.count - L;
+    ts.pbegin[right] = ts.pbegin[node] + L;
+    ts.pcount[right] = R;
+    ts.pcount[node] = L;
+    const bool left_small = L <= R;
+    const uint32_t small = left_small ? node : right, large = left_small ? right : node;
+    const uint32_t scount = left_small ? L : R;
+    ts.pslot[large] = ts.pslot[node];
+    if (large != node) ts.pslot[node] = UINT32_MAX;
+    const uint32_t child_depth = p.c.depth + 1;
+    need[i] = leafwise || child_depth < cfg.max_depth;
+    const uint32_t gsmall = ts.t * MN_ + small, glarge = ts.t * MN_ + large;
+    const uint32_t sbegin = (uint32_t)(ts.t * n_) + ts.pbegin[small];
+    stats.hist_rows += scount;
+    if (need[i]) {
+      ts.pslot[small] = alloc_slot(ts);
+      const uint32_t gss = ts.t * SPT_ + ts.pslot[small], gls = ts.t * SPT_ + ts.pslot[large];
+      const uint32_t nt = std::max<uint32_t>(1, (scount + kHistTile - 1) / kHistTile);
+      if (nt > 1 || scount == 0) zero.push_back(gss);
+      for (uint32_t k = 0; k < nt && scount > 0; k++) {
+        const uint32_t b0 = k * kHistTile, cnt = std::min(kHistTile, scount - b0);
+        for (int gr = 0; gr < ds_->d.NG; gr++)
+          hitems.push_back({gsmall, gss, sbegin + b0, cnt, (uint32_t)gr,
+                            (uint32_t)ts.t | (nt == 1 ? kDirect : 0u)});
+      }
+      quads_full.insert(quads_full.end(), {gls, gss, glarge, gsmall});
+      const uint32_t gl = ts.t * MN_ + node, grt = ts.t * MN_ + right;
+      snodes.push_back(gl);
+      sslots.push_back(ts.t * SPT_ + ts.pslot[node]);
+      snodes.push_back(grt);
+      sslots.push_back(ts.t * SPT_ + ts.pslot[right]);
+    } else {
+      ts.pslot[small] = UINT32_MAX;
+      for (uint32_t b0 = 0; b0 < scount; b0 += kSegTile)
+        segs.push_back({gsmall, sbegin + b0, std::min(kSegTile, scount - b0), (uint32_t)ts.t});
+      quads_sums.insert(quads_sums.end(), {0u, 0u, glarge, gsmall});
+    }
+  }
+  const int32_t* qexp = qexp_.p + 2 * trees[0].output;
+  upload(s_, d_u32a_, zero);
+  upload(s_, d_hist_items_, hitems);
+  upload(s_, d_segs_, segs);
+  upload(s_, d_u32b_, quads_full);
+  upload(s_, d_u32c_, quads_sums);
+  launch_zero_slots(s_, d_u32a_.p, (int)zero.size(), hist_.p, slot_elems_);
+  if (!hitems.empty()) {
+    double hrows = 0;
+    int hnodes = 0;
+    for (auto& it : hitems)
+      if (it.group == 0) hrows += it.count;
+    for (size_t k = 0; k < quads_full.size(); k += 4) hnodes++;
+    pi = ctx_->prof_begin(0, hist_bytes(hrows, hnodes, true));
+    launch_hist(s_, ds_->d, d_hist_items_.p, (int)hitems.size(), idx_.p, ogh_.p, 0, qexp, hist_.p,
+                slot_elems_, node_sum_.p, true, 0);
+    ctx_->prof_end(pi);
+  }
+  launch_node_sums(s_, d_segs_.p, (int)segs.size(), ogh_.p, 0, qexp, node_sum_.p);
+  launch_subtract(s_, d_u32b_.p, (int)quads_full.size() / 4, hist_.p, slot_elems_, node_sum_.p);
+  launch_subtract(s_, d_u32c_.p, (int)quads_sums.size() / 4, hist_.p, 0, node_sum_.p);
+  if (!snodes.empty()) {
+    std::vector<uint32_t> both(snodes);
+    both.insert(both.end(), sslots.begin(), sslots.end());
+    upload(s_, d_u32d_, both);
+    pi = ctx_->prof_begin(1, split_bytes((int)snodes.size()));
+    launch_split(s_, ds_->d, d_u32d_.p, d_u32d_.p + snodes.size(), (int)snodes.size(), feats_.p,
+                 (int)feats_h_.size(), hist_.p, slot_elems_, node_sum_.p, node_count_.p, qexp, MN_,
+                 sp_, node_key_.p, cand_.p, node_split_.p);
+    ctx_->prof_end(pi);
+  }
+  std::vector<long long> sums((size_t)2 * T_ * MN_);
+  std::vector<DevSplit> splits((size_t)T_ * MN_);
+  d2h(s_, sums.data(), node_sum_.p, sums.size());
+  if (!snodes.empty()) d2h(s_, splits.data(), node_split_.p, splits.size());
+  BST_HIP(hipStreamSynchronize(s_));
+  ctx_->prof_drain();
+  for (size_t i = 0; i < pend.size(); i++) {
+    Pending& p = pend[i];
+    TreeState& ts = trees[p.t];
+    const uint32_t node = p.c.node, right = p.right;
+    const uint32_t gl = ts.t * MN_ + node, grt = ts.t * MN_ + right;
+    Cand cl{node, p.lt, p.c.depth + 1, need[i] ? splits[gl] : invalid_split(), sums[2 * gl],
+            sums[2 * gl + 1], ts.pcount[node]};
+    Cand cr{right, p.rt, p.c.depth + 1, need[i] ? splits[grt] : invalid_split(), sums[2 * grt],
+            sums[2 * grt + 1], ts.pcount[right]};
+    if (cfg.growth == BST_GROW_DEPTHWISE) {
+      ts.next.push_back(cl);
+      ts.next.push_back(cr);
+    } else {
+      ts.heap.push(cl);
+      ts.heap.push(cr);
+    }
+  }
+}
+
+void Trainer::grow_round(std::vector<TreeState>& trees) {
+  const auto& cfg = opt_.cfg;
+  root(trees);
+  const bool plain = sched_ == BST_SCHED_PLAIN;
+  if (cfg.growth == BST_GROW_DEPTHWISE) {
+    for (uint32_t depth = 0; depth < cfg.max_depth; depth++) {
+      std::vector<std::pair<int, Cand>> chosen;
+      for (size_t i = 0; i < trees.size(); i++)
+        for (auto& c : trees[i].level) chosen.push_back({(int)i, c});
+      if (chosen.empty()) break;
+      for (auto& ts : trees) ts.next.clear();
+      if (plain) {
+        for (auto& one : chosen) {
+          std::vector<std::pair<int, Cand>> v{one};
+          expand_batch(trees, v);
+        }
+      } else {
+        expand_batch(trees, chosen);
+      }
+      for (auto& ts : trees) ts.level.swap(ts.next);
+    }
+    for (auto& ts : trees) {
+      for (auto& c : ts.level) make_leaf(ts, c);
+      ts.level.clear();
+    }
+  } else {
+    for (;;) {
+      std::vector<std::pair<int, Cand>> chosen;
+      for (size_t i = 0; i < trees.size(); i++) {
+        TreeState& ts = trees[i];
+        if (ts.n_leaves < cfg.max_leaves && !ts.heap.empty()) {
+          chosen.push_back({(int)i, ts.heap.top()});
+          ts.heap.pop();
+          ts.n_leaves++;
+        }
+      }
+      if (chosen.empty()) break;
+      if (plain) {
+        for (auto& one : chosen) {
+          std::vector<std::pair<int, Cand>> v{one};
+          expand_batch(trees, v);
+        }
+      } else {
+        expand_batch(trees, chosen);
+      }
+    }
+    for (auto& ts : trees)
+      while (!ts.heap.empty()) {
+        make_leaf(ts, ts.heap.top());
+        ts.heap.pop();
+      }
+  }
+  for (auto& ts : trees)
+    for (size_t i = 0; i < ts.tb.leaf.size(); i++)
+      if (ts.tb.is_leaf[i]) ts.tb.leaf[i] = ts.tb.leaf[i] * lr_;
+}
+
+double Trainer::eval_metric(const float* preds, const float* y, int64_t n) {
+  const auto& cfg = opt_.cfg;
+  const int metric = cfg.metric == BST_METRIC_DEFAULT ? default_metric(cfg.objective) : cfg.metric;
+  if (metric == BST_METRIC_NONE) return std::numeric_limits<double>::quiet_NaN();
+  const int transform = cfg.objective == BST_OBJ_LOGISTIC ? BST_TRANSFORM_SIGMOID
+                        : cfg.objective == BST_OBJ_SOFTMAX ? BST_TRANSFORM_SOFTMAX
+                                                           : BST_TRANSFORM_IDENTITY;
+  const int nb = 512;
+  d_metric_.ensure((size_t)2 * nb * opt_.K);
+  launch_metric(s_, metric, transform, preds, opt_.K, y, nullptr, n, d_metric_.p, nb);
+  std::vector<double> part((size_t)2 * nb * opt_.K);
+  d2h(s_, part.data(), d_metric_.p, part.size());
+  BST_HIP(hipStreamSynchronize(s_));
+  if (metric == BST_METRIC_RMSE) {
+    double acc = 0;
+    for (int k = 0; k < opt_.K; k++) {
+      double ss = 0, sw = 0;
+      for (int b = 0; b < nb; b++) {
+        ss += part[(size_t)(k * nb + b) * 2];
+        sw += part[(size_t)(k * nb + b) * 2 + 1];
+      }
+      acc += std::sqrt(ss / sw);
+    }
+    return acc / opt_.K;
+  }
+  double sl = 0, sw = 0;
+  for (int b = 0; b < nb; b++) {
+    sl += part[(size_t)b * 2];
+    sw += part[(size_t)b * 2 + 1];
+  }
+  return sw > 0 ? sl / sw : 0.0;
+}
+
+bst_model* Trainer::train(const float* y, const float* w, const float* vX, const float* vy,
+                          int64_t n_valid) {
+  const auto& cfg = opt_.cfg;
+  const int K = opt_.K;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto* m = new bst_model();
+  m->info.n_groups = K;
+  m->info.n_features = ds_->d.F;
+  m->info.objective = cfg.objective;
+  m->info.output_transform = cfg.objective == BST_OBJ_LOGISTIC ? BST_TRANSFORM_SIGMOID
+                             : cfg.objective == BST_OBJ_SOFTMAX ? BST_TRANSFORM_SOFTMAX
+                                                                : BST_TRANSFORM_IDENTITY;
+  // Base score (objectives: regression.rs:44-68, classification.rs:54-76,189-222),
+  // accumulated in f64 on the host from the device targets.
+  {
+    std::vector<float> yh(n_), wh;
+    d2h(s_, yh.data(), y, n_);
+    if (w) {
+      wh.resize(n_);
+      d2h(s_, wh.data(), w, n_);
+    }
+    BST_HIP(hipStreamSynchronize(s_));
+    auto wt = [&](int64_t i) { return w ? wh[i] : 1.0f; };
+    m->base_score.assign(K, 0.0f);
+    if (cfg.objective == BST_OBJ_SQUARED) {
+      double sw = 0, swy = 0;
+      for (int64_t i = 0; i < n_; i++) {
+        sw += (double)wt(i);
+        swy += (double)wt(i) * (double)yh[i];
+      }
+      m->base_score[0] = sw > 0 ? (float)(swy / sw) : 0.0f;
+    } else if (cfg.objective == BST_OBJ_LOGISTIC) {
+      double pos = 0, tot = 0;
+      for (int64_t i = 0; i < n_; i++) {
+        pos += (double)wt(i) * (double)yh[i];
+        tot += (double)wt(i);
+      }
+      const double p = std::min(std::max(pos / tot, 1e-7), 1.0 - 1e-7);
+      m->base_score[0] = (float)std::log(p / (1.0 - p));
+    } else {
+      std::vector<double> cw(K, 0.0);
+      double tot = 0;
+      for (int64_t i = 0; i < n_; i++) {
+        const size_t l = (size_t)yh[i];
+        if (l < (size_t)K) cw[l] += (double)wt(i);
+        tot += (double)wt(i);
+      }
+      for (int k = 0; k < K; k++) {
+        const double p = std::min(std::max(cw[k] / tot, 1e-7), 1.0 - 1e-7);
+        m->base_score[k] = (float)std::log(p);
+      }
+    }
+  }
+  {
+    std::vector<float> fill(n_);
+    for (int k = 0; k < K; k++) {
+      std::fill(fill.begin(), fill.end(), m->base_score[k]);
+      h2d(s_, preds_.p + (size_t)k * n_, fill.data(), n_);
+    }
+    if (vX && n_valid > 0) {
+      vpreds_.alloc((size_t)K * n_valid);
+      std::vector<float> vf(n_valid);
+      for (int k = 0; k < K; k++) {
+        std::fill(vf.begin(), vf.end(), m->base_score[k]);
+        h2d(s_, vpreds_.p + (size_t)k * n_valid, vf.data(), n_valid);
+      }
+    }
+    BST_HIP(hipStreamSynchronize(s_));
+  }
+  const uint32_t E = (uint32_t)ds_->d.E;
+  feats_h_.resize(E);
+  for (uint32_t i = 0; i < E; i++) feats_h_[i] = i;
+  h2d(s_, feats_.p, feats_h_.data(), E);
+  Xoshiro col_rng(cfg.seed), row_rng(cfg.seed);
+  const bool col_sampling = !(std::fabs(cfg.colsample_bytree - 1.0f) < 1e-6f);
+  const bool uniform = !(std::fabs(cfg.subsample - 1.0f) < 1e-6f);
+  if (opt_.ext.goss_top_rate > 0.0f) raise(BST_E_CONFIG, "GOSS row sampling is not supported by this engine build");
+
+  std::vector<std::vector<float>> leafv;
+  bool have_best = false;
+  double best_val = 0;
+  size_t best_round = 0, cur_round = 0, best_n_trees = 0;
+  std::vector<uint32_t> sampled;
+  for (uint32_t round = 0; round < cfg.n_trees; round++) {
+    BST_HIP(hipMemsetAsync(absmax_.p, 0, absmax_.bytes(), s_));
+    launch_gradients(s_, cfg.objective, preds_.p, y, w, n_, K, g_.p, h_.p, absmax_.p);
+    launch_qscale(s_, absmax_.p, K, n_, qexp_.p);
+    for (int k0 = 0; k0 < K; k0 += T_) {
+      std::vector<TreeState> trees(T_);
+      BST_HIP(hipMemsetAsync(node_sum_.p, 0, node_sum_.bytes(), s_));
+      std::vector<uint32_t> outs(T_);
+      for (int t = 0; t < T_; t++) {
+        TreeState& ts = trees[t];
+        ts.t = t;
+        ts.output = k0 + t;
+        outs[t] = ts.output;
+        ts.pbegin.assign(MN_, 0);
+        ts.pcount.assign(MN_, 0);
+        ts.pslot.assign(MN_, UINT32_MAX);
+        ts.leaf_lr.assign(MN_, std::numeric_limits<float>::quiet_NaN());
+        ts.free_slots.clear();
+        for (uint32_t s = SPT_; s-- > 0;) ts.free_slots.push_back(s);
+        // Column sampling per tree (column.rs:266-283, seeded stream).
+        if (col_sampling) {
+          feats_h_.resize(E);
+          for (uint32_t i = 0; i < E; i++) feats_h_[i] = i;
+          const uint32_t kk = std::max<uint32_t>(1, (uint32_t)std::ceil((double)E * cfg.colsample_bytree));
+          for (uint32_t i = 0; i < kk && i < E; i++) {
+            const uint32_t j = i + (uint32_t)col_rng.range_incl(0, E - 1 - i);
+            std::swap(feats_h_[i], feats_h_[j]);
+          }
+          feats_h_.resize(std::min(kk, E));
+          std::sort(feats_h_.begin(), feats_h_.end());
+          h2d(s_, feats_.p, feats_h_.data(), feats_h_.size());
+          // The split kernel's tie rule depends on the number of candidate features.
+          sp_.rightmost = (opt_.ext.n_threads != 1 && feats_h_.size() >= 16) ? 1 : 0;
+        } else {
+          sp_.rightmost = (opt_.ext.n_threads != 1 && E >= 16) ? 1 : 0;
+        }
+        // Uniform row sampling (row.rs:170-208): reservoir over row ids.
+        if (uniform) {
+          const size_t tc = std::min<size_t>(std::max<size_t>((size_t)((float)n_ * cfg.subsample), 1), n_);
+          sampled.resize(tc);
+          for (size_t i = 0; i < tc; i++) sampled[i] = (uint32_t)i;
+          for (size_t i = tc; i < (size_t)n_; i++) {
+            const size_t j = (size_t)row_rng.range_incl(0, i);
+            if (j < tc) sampled[j] = (uint32_t)i;
+          }
+          std::sort(sampled.begin(), sampled.end());
+          upload(s_, d_sampled_, sampled);
+          ts.rows = (int64_t)tc;
+          ts.sampled = true;
+          launch_init_rows(s_, idx_.p + t * n_, ogh_.p + t * n_, g_.p + (size_t)ts.output * n_,
+                           h_.p + (size_t)ts.output * n_, d_sampled_.p, ts.rows);
+        } else {
+          ts.rows = n_;
+          launch_init_rows(s_, idx_.p + t * n_, ogh_.p + t * n_, g_.p + (size_t)ts.output * n_,
+                           h_.p + (size_t)ts.output * n_, nullptr, n_);
+        }
+        ts.pcount[0] = (uint32_t)ts.rows;
+      }
+      sp_.rightmost = sp_.rightmost;
+      grow_round(trees);
+      // Prediction update and model append, trees in output order.
+      h2d(s_, tree_output_.p, outs.data(), outs.size());
+      for (auto& ts : trees) {
+        bst_model one;
+        one.info.n_trees = 1;
+        one.info.n_groups = K;
+        one.base_score.assign(K, 0.0f);
+        one.tree_offset = {0, (int64_t)ts.tb.split_index.size()};
+        one.tree_group = {(uint32_t)ts.output};
+        const size_t nn = ts.tb.split_index.size();
+        one.split_index = ts.tb.split_index;
+        one.split_threshold = ts.tb.thr;
+        one.left = ts.tb.left;
+        one.right = ts.tb.right;
+        one.default_left = ts.tb.dleft;
+        one.is_leaf = ts.tb.is_leaf;
+        one.split_type = ts.tb.stype;
+        one.leaf_value.assign(nn, 0.0f);
+        for (size_t i = 0; i < nn; i++)
+          if (ts.tb.is_leaf[i]) one.leaf_value[i] = ts.tb.leaf[i];
+        one.gain = ts.tb.gain;
+        one.cover = ts.tb.cover;
+        one.cat_start.assign(nn, 0);
+        one.cat_size.assign(nn, 0);
+        auto cn = ts.tb.cat_nodes;
+        std::sort(cn.begin(), cn.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (auto& c : cn) {
+          one.cat_start[c.first] = (uint32_t)one.cat_words.size();
+          one.cat_size[c.first] = (uint32_t)c.second.size();
+          one.cat_words.insert(one.cat_words.end(), c.second.begin(), c.second.end());
+        }
+        one.info.total_nodes = (int64_t)nn;
+        one.info.total_cat_words = (int64_t)one.cat_words.size();
+        const bool need_dev = ts.sampled || (vX && n_valid > 0);
+        if (need_dev) one.upload(s_);
+        if (!ts.sampled) {
+          std::vector<Segment> segs;
+          std::vector<float> lv((size_t)T_ * MN_, 0.0f);
+          for (uint32_t p = 0; p < ts.n_part; p++) {
+            const float v = ts.leaf_lr[p];
+            if (std::isnan(v)) continue;
+            lv[ts.t * MN_ + p] = v;
+            for (uint32_t b0 = 0; b0 < ts.pcount[p]; b0 += kSegTile)
+              segs.push_back({ts.t * MN_ + p, (uint32_t)(ts.t * n_) + ts.pbegin[p] + b0,
+                              std::min(kSegTile, ts.pcount[p] - b0), (uint32_t)ts.t});
+          }
+          h2d(s_, leaf_val_.p, lv.data(), lv.size());
+          upload(s_, d_segs_, segs);
+          launch_leaf_update(s_, d_segs_.p, (int)segs.size(), leaf_val_.p, idx_.p, 0, preds_.p,
+                             tree_output_.p, n_);
+          BST_HIP(hipStreamSynchronize(s_));
+        } else {
+          launch_predict_accum(s_, one.d_nodes.p, one.d_tree_off.p, one.d_tree_group.p, 1,
+                               one.d_cat_start.p, one.d_cat_size.p, one.d_cat_words.p, K,
+                               ds_->d.raw, n_, n_, ds_->d.F, preds_.p, n_);
+        }
+        if (vX && n_valid > 0)
+          launch_predict_accum(s_, one.d_nodes.p, one.d_tree_off.p, one.d_tree_group.p, 1,
+                               one.d_cat_start.p, one.d_cat_size.p, one.d_cat_words.p, K, vX,
+                               n_valid, n_valid, ds_->d.F, vpreds_.p, n_valid);
+        if (need_dev) BST_HIP(hipStreamSynchronize(s_));
+        // Append to the forest.
+        const int64_t off = m->tree_offset.empty() ? 0 : m->tree_offset.back();
+        if (m->tree_offset.empty()) m->tree_offset.push_back(0);
+        m->tree_offset.push_back(off + (int64_t)nn);
+        m->tree_group.push_back((uint32_t)ts.output);
+        const uint32_t woff = (uint32_t)m->cat_words.size();
+        for (size_t i = 0; i < nn; i++) {
+          m->split_index.push_back(one.split_index[i]);
+          m->split_threshold.push_back(one.split_threshold[i]);
+          m->left.push_back(one.left[i]);
+          m->right.push_back(one.right[i]);
+          m->default_left.push_back(one.default_left[i]);
+          m->is_leaf.push_back(one.is_leaf[i]);
+          m->split_type.push_back(one.split_type[i]);
+          m->leaf_value.push_back(one.leaf_value[i]);
+          m->gain.push_back(one.gain[i]);
+          m->cover.push_back(one.cover[i]);
+          m->cat_start.push_back(one.cat_size[i] ? one.cat_start[i] + woff : 0);
+          m->cat_size.push_back(one.cat_size[i]);
+        }
+        m->cat_words.insert(m->cat_words.end(), one.cat_words.begin(), one.cat_words.end());
+        stats.n_trees++;
+      }
+    }
+    // evaluate_round (eval.rs:218-260) and early stopping (callback.rs:78-118).
+    double value;
+    if (vX && vy && n_valid > 0)
+      value = eval_metric(vpreds_.p, vy, n_valid);
+    else
+      value = eval_metric(preds_.p, y, n_);
+    metric_history.push_back(value);
+    if (cfg.early_stopping_rounds > 0) {
+      const bool imp = !have_best || value < best_val;
+      if (imp) {
+        best_val = value;
+        have_best = true;
+        best_round = cur_round;
+        best_n_trees = m->tree_group.size();
+      }
+      cur_round++;
+      if (cur_round - best_round > cfg.early_stopping_rounds) break;
+    }
+  }
+  if (cfg.early_stopping_rounds > 0 && best_n_trees > 0 && best_n_trees < m->tree_group.size()) {
+    const int64_t keep = m->tree_offset[best_n_trees];
+    m->tree_offset.resize(best_n_trees + 1);
+    m->tree_group.resize(best_n_trees);
+    for (auto* v : {&m->split_index, &m->left, &m->right, &m->cat_start, &m->cat_size})
+      v->resize(keep);
+    for (auto* v : {&m->split_threshold, &m->leaf_value, &m->gain, &m->cover}) v->resize(keep);
+    for (auto* v : {&m->default_left, &m->is_leaf, &m->split_type}) v->resize(keep);
+    uint32_t words = 0;
+    for (int64_t i = 0; i < keep; i++)
+      if (m->cat_size[i]) words = std::max(words, m->cat_start[i] + m->cat_size[i]);
+    m->cat_words.resize(words);
+  }
+  if (m->tree_offset.empty()) m->tree_offset.push_back(0);
+  m->info.n_trees = (int64_t)m->tree_group.size();
+  m->info.total_nodes = m->tree_offset.back();
+  m->info.total_cat_words = (int64_t)m->cat_words.size();
+  stats.ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return m;
+}
+
+}  // namespace bst

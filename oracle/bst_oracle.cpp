@@ -1,0 +1,1435 @@
+// bst_oracle.cpp — TEST INFRASTRUCTURE ONLY (parity checker + CPU baseline).
+//
+// A CPU restatement of egordm/boosters' histogram GBDT path.  Paths in the
+// citations are relative to crates/boosters/src/.  Build flags matter for
+// parity: -O2 -ffp-contract=off (no FMA contraction, as rustc), no -ffast-math,
+// glibc expf/logf (what Rust's f32::exp lowers to on x86_64-linux-gnu).
+//
+// Deliberate differences from the reference, each numerically neutral or
+// documented:
+//   * No LRU histogram pool (pool.rs): every parent histogram is kept, so the
+//     larger child is always parent - smaller.  The reference rebuilds some
+//     children from rows when the 8-slot pool evicted the parent; that only
+//     changes f64 summation order.
+//   * Sparse-storage features are binned and histogrammed like dense ones
+//     (the reference skips sparse features in its gathered histogram path,
+//     ops.rs:158-160); no in-scope workload has a feature with <=10% nonzeros.
+//   * Column sampling uses xoshiro256++ seeded from the config seed; the
+//     reference seeds it from OS entropy (column.rs:137-142) so it has no
+//     reproducible stream to match.
+#include "bst_oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <queue>
+#include <string>
+#include <unordered_set>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_err;
+int g_threads = 0;
+// 0: glibc expf (what Rust's f32::exp calls on x86_64-linux-gnu);
+// 1: exp evaluated in f64 and rounded once to f32 (the engine's device exp).
+// glibc expf is off by one ulp from the correctly rounded value on ~7.6e-5 of
+// inputs, so mode 1 is used where tests need the engine's exact arithmetic.
+int g_exp_mode = 0;
+inline float exp32(float x) { return g_exp_mode ? (float)std::exp((double)x) : expf(x); }
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------------------
+// Binning (data/binned/)
+// ---------------------------------------------------------------------------
+struct Mapper {
+  bool categorical = false;
+  std::vector<double> bounds;  // numeric upper bounds (incl. f64::MAX sentinels)
+  std::vector<int32_t> cats;   // categorical: sorted categories
+  uint32_t n_bins = 0;
+  int missing = 0;  // 0 None, 2 NaN (MissingType, bin_mapper.rs:20-28)
+  uint32_t default_bin = 0;
+  bool trivial = false;
+};
+
+// BinMapper::value_to_bin + search_bin, bin_mapper.rs:210-258.
+uint32_t value_to_bin(const Mapper& m, double v) {
+  if (std::isnan(v)) return m.missing == 2 ? m.n_bins - 1 : m.default_bin;
+  if (m.categorical) {
+    // Rust `f64 as i32`: truncating, saturating; unknown category -> bin 0.
+    int32_t c;
+    if (v >= 2147483647.0)
+      c = 2147483647;
+    else if (v <= -2147483648.0)
+      c = (int32_t)-2147483648LL;
+    else
+      c = (int32_t)v;
+    auto it = std::lower_bound(m.cats.begin(), m.cats.end(), c);
+    if (it != m.cats.end() && *it == c) return (uint32_t)(it - m.cats.begin());
+    return 0;
+  }
+  size_t lo = 0, hi = m.bounds.empty() ? 0 : m.bounds.size() - 1;
+  if (m.missing == 2 && hi > 0) hi -= 1;
+  while (lo < hi) {
+    size_t mid = lo + (hi - lo) / 2;
+    if (v <= m.bounds[mid])
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return (uint32_t)lo;
+}
+
+// bin_numeric, data/binned/dataset.rs:683-750.
+Mapper bin_numeric(const std::vector<float>& data, uint32_t max_bins, bool has_nan) {
+  float min_v = std::numeric_limits<float>::max();
+  float max_v = -std::numeric_limits<float>::max();
+  std::vector<float> vals;
+  vals.reserve(data.size());
+  for (float x : data)
+    if (std::isfinite(x)) {
+      min_v = std::min(min_v, x);
+      max_v = std::max(max_v, x);
+      vals.push_back(x);
+    }
+  Mapper m;
+  if (vals.empty() || min_v >= max_v) {
+    m.bounds = {std::numeric_limits<double>::max()};
+    m.n_bins = 1;
+    return m;
+  }
+  uint32_t nb = std::min<uint64_t>(max_bins, vals.size());
+  std::stable_sort(vals.begin(), vals.end(), [](float a, float b) { return a < b; });
+  vals.erase(std::unique(vals.begin(), vals.end(), [](float a, float b) { return a == b; }),
+             vals.end());
+  size_t nu = vals.size();
+  if (nu <= nb) {
+    for (size_t i = 0; i + 1 < nu; i++) m.bounds.push_back(((double)vals[i] + (double)vals[i + 1]) / 2.0);
+  } else {
+    for (uint32_t i = 1; i < nb; i++) {
+      double q = (double)i / (double)nb;
+      size_t idx = std::min<size_t>((size_t)std::floor(q * (double)(nu - 1)), nu - 2);
+      double b = ((double)vals[idx] + (double)vals[idx + 1]) / 2.0;
+      if (m.bounds.empty() || b > m.bounds.back()) m.bounds.push_back(b);
+    }
+  }
+  m.bounds.push_back(std::numeric_limits<double>::max());
+  if (has_nan) m.bounds.push_back(std::numeric_limits<double>::max());
+  m.missing = has_nan ? 2 : 0;
+  m.n_bins = (uint32_t)m.bounds.size();
+  m.default_bin = has_nan ? m.n_bins - 1 : 0;
+  return m;
+}
+
+// Deterministic sample of row ids for edge computation, dataset.rs:598-629.
+std::vector<uint32_t> binning_sample(int64_t n, uint64_t cnt) {
+  std::vector<uint32_t> idx;
+  if ((uint64_t)n <= cnt) return idx;
+  uint64_t state = 0xD1B54A32D192ED03ULL ^ (uint64_t)n;
+  std::unordered_set<uint32_t> seen;
+  seen.reserve(cnt * 2);
+  idx.reserve(cnt);
+  while (idx.size() < cnt) {
+    state += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    uint32_t r = (uint32_t)(z % (uint64_t)n);
+    if (seen.insert(r).second) idx.push_back(r);
+  }
+  std::sort(idx.begin(), idx.end());
+  return idx;
+}
+
+}  // namespace
+
+struct ora_binned {
+  int64_t n = 0;
+  int32_t F = 0;
+  std::vector<Mapper> mappers;         // per original feature
+  std::vector<uint32_t> eff_to_orig;   // effective column -> original feature
+  std::vector<uint32_t> eff_nbins, eff_offset;
+  std::vector<uint8_t> eff_cat, eff_missing;
+  std::vector<std::vector<uint16_t>> bins;  // per effective column [n]
+  uint32_t total_bins = 0;
+};
+
+namespace {
+
+// FeatureStats::update / into_analysis, feature_analysis.rs:313-432 (the
+// fields that decide trivial / categorical / has-NaN).
+struct Analysis {
+  bool trivial, numeric;
+  int64_t n_nan;
+};
+
+Analysis analyze(const float* col, int64_t n, bool user_cat, const bst_binning_config& cfg) {
+  size_t max_unique = std::max<size_t>(cfg.max_categorical_cardinality, cfg.max_bins) + 10;
+  std::vector<float> uniq;
+  bool exceeded = false, all_int = true;
+  int64_t valid = 0, nan = 0;
+  for (int64_t i = 0; i < n; i++) {
+    float v = col[i];
+    if (std::isnan(v)) {
+      nan++;
+      continue;
+    }
+    valid++;
+    if (all_int && (v - std::trunc(v)) != 0.0f) all_int = false;
+    if (!exceeded && std::find(uniq.begin(), uniq.end(), v) == uniq.end()) {
+      if (uniq.size() >= max_unique)
+        exceeded = true;
+      else
+        uniq.push_back(v);
+    }
+  }
+  size_t nu = uniq.size();
+  Analysis a;
+  a.trivial = valid == 0 || (nu == 1 && uniq[0] == 0.0f) || nu == 0;
+  bool binary = nu == 2;
+  bool auto_cat = !user_cat && all_int && nu <= cfg.max_categorical_cardinality;
+  a.numeric = user_cat ? false : (binary ? true : !auto_cat);
+  a.n_nan = nan;
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// Objectives (training/objectives/)
+// ---------------------------------------------------------------------------
+inline float wt(const float* w, int64_t i) { return w ? w[i] : 1.0f; }
+
+void gradients(int obj, int K, const float* preds, const float* y, const float* w, int64_t n,
+               float* g, float* h) {
+  const float HESS_MIN = 1e-6f;
+  if (obj == BST_OBJ_SQUARED) {  // regression.rs:20-42
+    for (int k = 0; k < K; k++)
+      for (int64_t i = 0; i < n; i++) {
+        float wi = wt(w, i);
+        g[k * n + i] = wi * (preds[k * n + i] - y[i]);
+        h[k * n + i] = wi;
+      }
+  } else if (obj == BST_OBJ_LOGISTIC) {  // classification.rs:20-52
+    for (int k = 0; k < K; k++)
+      for (int64_t i = 0; i < n; i++) {
+        float wi = wt(w, i);
+        float p = 1.0f / (1.0f + exp32(-preds[k * n + i]));
+        g[k * n + i] = wi * (p - y[i]);
+        h[k * n + i] = fmaxf(wi * p * (1.0f - p), HESS_MIN);  // f32::max ignores NaN
+      }
+  } else if (obj == BST_OBJ_SOFTMAX) {  // classification.rs:141-187
+    for (int64_t i = 0; i < n; i++) {
+      float wi = wt(w, i);
+      size_t label = (size_t)y[i];
+      float mx = -std::numeric_limits<float>::infinity();
+      for (int k = 0; k < K; k++) mx = fmaxf(mx, preds[k * n + i]);
+      float s = 0.0f;
+      for (int k = 0; k < K; k++) s += exp32(preds[k * n + i] - mx);
+      for (int k = 0; k < K; k++) {
+        float p = exp32(preds[k * n + i] - mx) / s;
+        float ind = (size_t)k == label ? 1.0f : 0.0f;
+        g[k * n + i] = wi * (p - ind);
+        h[k * n + i] = fmaxf(wi * p * (1.0f - p), HESS_MIN);  // f32::max ignores NaN
+      }
+    }
+  }
+}
+
+std::vector<float> base_score(int obj, int K, const float* y, const float* w, int64_t n) {
+  if (obj == BST_OBJ_SQUARED) {  // regression.rs:44-68
+    if (n == 0) return {0.0f};
+    double sw = 0, swy = 0;
+    for (int64_t i = 0; i < n; i++) {
+      sw += (double)wt(w, i);
+      swy += (double)wt(w, i) * (double)y[i];
+    }
+    return {sw > 0 ? (float)(swy / sw) : 0.0f};
+  }
+  if (obj == BST_OBJ_LOGISTIC) {  // classification.rs:54-76
+    if (n == 0) return {0.0f};
+    double pos = 0, tot = 0;
+    for (int64_t i = 0; i < n; i++) {
+      pos += (double)wt(w, i) * (double)y[i];
+      tot += (double)wt(w, i);
+    }
+    double p = std::min(std::max(pos / tot, 1e-7), 1.0 - 1e-7);
+    return {(float)std::log(p / (1.0 - p))};
+  }
+  // softmax, classification.rs:189-222
+  std::vector<float> out(K, 0.0f);
+  if (n == 0) return out;
+  std::vector<double> cw(K, 0.0);
+  double tot = 0;
+  for (int64_t i = 0; i < n; i++) {
+    size_t l = (size_t)y[i];
+    if (l < (size_t)K) cw[l] += (double)wt(w, i);
+    tot += (double)wt(w, i);
+  }
+  for (int k = 0; k < K; k++) {
+    double p = std::min(std::max(cw[k] / tot, 1e-7), 1.0 - 1e-7);
+    out[k] = (float)std::log(p);
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Split finding (training/gbdt/split/)
+// ---------------------------------------------------------------------------
+struct Gain {  // NodeGainContext, gain.rs:16-52
+  double lambda, offset;
+  Gain(double pg, double ph, const bst_gbdt_config& c) {
+    lambda = (double)c.lambda;
+    double ps = pg * pg / (ph + lambda);
+    offset = 0.5 * ps + (double)c.min_gain;
+  }
+  float gain(double gl, double hl, double gr, double hr) const {
+    double sl = gl * gl / (hl + lambda);
+    double sr = gr * gr / (hr + lambda);
+    return (float)(0.5 * (sl + sr) - offset);
+  }
+};
+
+// GainParams::is_valid_split, gain.rs:123-137.
+inline bool valid_split(const bst_gbdt_config& c, double hl, double hr, uint32_t cl, uint32_t cr) {
+  double mw = (double)c.min_child_weight;
+  return hl >= mw && hr >= mw && cl >= c.min_samples_leaf && cr >= c.min_samples_leaf;
+}
+
+// GainParams::compute_leaf_weight, gain.rs:151-168.
+float leaf_weight(const bst_gbdt_config& c, double g, double h) {
+  double lam = c.lambda, al = c.alpha;
+  if (al == 0.0) return (float)(-g / (h + lam));
+  double ag = std::fabs(g);
+  if (ag <= al) return 0.0f;
+  double s = g > 0.0 ? -1.0 : 1.0;
+  return (float)(s * (ag - al) / (h + lam));
+}
+
+inline uint32_t sat_sub(uint32_t a, uint32_t b) { return a > b ? a - b : 0; }
+
+ora_split invalid_split() {
+  ora_split s;
+  memset(&s, 0, sizeof(s));
+  s.gain = -std::numeric_limits<float>::infinity();
+  return s;
+}
+
+// find_numerical_split, find.rs:286-407 (non-bundle columns).
+ora_split numeric_split(const double* bins, uint32_t nb, uint32_t feat, double pg, double ph,
+                        uint32_t pc, bool has_missing, const Gain& ctx,
+                        const bst_gbdt_config& c) {
+  ora_split best = invalid_split();
+  if (nb < 2) return best;
+  double lg = 0, lh = 0;
+  uint32_t lc = 0;
+  for (uint32_t b = 0; b + 1 < nb; b++) {
+    lg += bins[2 * b];
+    lh += bins[2 * b + 1];
+    lc += 1;
+    double rg = pg - lg, rh = ph - lh;
+    uint32_t rc = sat_sub(pc, lc);
+    if (valid_split(c, lh, rh, lc, rc)) {
+      float g = ctx.gain(lg, lh, rg, rh);
+      if (g > best.gain) {
+        best.feature = feat;
+        best.gain = g;
+        best.default_left = 0;
+        best.is_categorical = 0;
+        best.bin = b;
+      }
+    }
+  }
+  if (has_missing) {
+    double rg = 0, rh = 0;
+    uint32_t rc = 0;
+    uint32_t max_excl = nb - 1;  // the missing bin is the last one
+    for (uint32_t b = max_excl; b-- > 1;) {
+      rg += bins[2 * b];
+      rh += bins[2 * b + 1];
+      rc += 1;
+      double lg2 = pg - rg, lh2 = ph - rh;
+      uint32_t lc2 = sat_sub(pc, rc);
+      if (valid_split(c, lh2, rh, lc2, rc)) {
+        float g = ctx.gain(lg2, lh2, rg, rh);
+        if (g > best.gain) {
+          best.feature = feat;
+          best.gain = g;
+          best.default_left = 1;
+          best.is_categorical = 0;
+          best.bin = b - 1;
+        }
+      }
+    }
+  }
+  return best;
+}
+
+inline void cat_insert(ora_split& s, uint32_t c) { s.left_cats[c >> 5] |= 1u << (c & 31); }
+
+// find_categorical_split / find_onehot_split / find_sorted_split, find.rs:418-573.
+ora_split categorical_split(const double* bins, uint32_t nc, uint32_t feat, double pg, double ph,
+                            uint32_t pc, const Gain& ctx, const bst_gbdt_config& c) {
+  ora_split best = invalid_split();
+  if (nc < 2) return best;
+  if (nc <= c.max_onehot_cats) {
+    for (uint32_t k = 0; k < nc; k++) {
+      double lg = bins[2 * k], lh = bins[2 * k + 1];
+      uint32_t lc = 1;
+      double rg = pg - lg, rh = ph - lh;
+      uint32_t rc = sat_sub(pc, lc);
+      if (valid_split(c, lh, rh, lc, rc)) {
+        float g = ctx.gain(lg, lh, rg, rh);
+        if (g > best.gain) {
+          best = invalid_split();
+          best.feature = feat;
+          best.gain = g;
+          best.is_categorical = 1;
+          cat_insert(best, k);
+        }
+      }
+    }
+    return best;
+  }
+  std::vector<std::pair<uint32_t, double>> sc;
+  for (uint32_t k = 0; k < nc; k++) {
+    double g = bins[2 * k], h = bins[2 * k + 1];
+    if (h > 1e-10) sc.push_back({k, -g / (h + 1e-6)});
+  }
+  if (sc.size() < 2) return best;
+  std::stable_sort(sc.begin(), sc.end(),
+                   [](const std::pair<uint32_t, double>& a, const std::pair<uint32_t, double>& b) {
+                     return a.second < b.second;
+                   });
+  ora_split cur = invalid_split();
+  double lg = 0, lh = 0;
+  uint32_t lc = 0;
+  for (size_t i = 0; i + 1 < sc.size(); i++) {
+    uint32_t k = sc[i].first;
+    cat_insert(cur, k);
+    lg += bins[2 * k];
+    lh += bins[2 * k + 1];
+    lc += 1;
+    double rg = pg - lg, rh = ph - lh;
+    uint32_t rc = sat_sub(pc, lc);
+    if (valid_split(c, lh, rh, lc, rc)) {
+      float g = ctx.gain(lg, lh, rg, rh);
+      if (g > best.gain) {
+        best = cur;
+        best.feature = feat;
+        best.gain = g;
+        best.default_left = 0;
+        best.is_categorical = 1;
+      }
+    }
+  }
+  return best;
+}
+
+// GreedySplitter::find_split, find.rs:113-270.  parallel = rayon reduce
+// (a.gain > b.gain ? a : b), which returns the rightmost maximum; sequential
+// keeps the first maximum.
+ora_split find_split(const ora_binned& B, const double* hist, double pg, double ph, uint32_t pc,
+                     const bst_gbdt_config& c, int n_threads, const std::vector<uint32_t>& feats) {
+  Gain ctx(pg, ph, c);
+  bool parallel = n_threads != 1 && feats.size() >= 16;
+  std::vector<ora_split> per(feats.size());
+#pragma omp parallel for schedule(dynamic, 4) if (parallel)
+  for (size_t j = 0; j < feats.size(); j++) {
+    uint32_t f = feats[j];
+    const double* bins = hist + 2 * (size_t)B.eff_offset[f];
+    per[j] = B.eff_cat[f] ? categorical_split(bins, B.eff_nbins[f], f, pg, ph, pc, ctx, c)
+                          : numeric_split(bins, B.eff_nbins[f], f, pg, ph, pc,
+                                          B.eff_missing[f] != 0, ctx, c);
+  }
+  ora_split best = invalid_split();
+  for (auto& s : per) {
+    if (parallel ? !(best.gain > s.gain) : (s.gain > best.gain)) best = s;
+  }
+  return best;
+}
+
+// ---------------------------------------------------------------------------
+// Histograms (training/gbdt/histograms/ops.rs)
+// ---------------------------------------------------------------------------
+void build_hist(const ora_binned& B, const float* g, const float* h, const uint32_t* rows,
+                int64_t nr, double* out) {
+  int64_t total = (int64_t)B.total_bins * 2;
+  std::fill(out, out + total, 0.0);
+  int64_t F = (int64_t)B.eff_to_orig.size();
+  // Feature-parallel, one feature per task, like the rayon path (ops.rs:92-99).
+#pragma omp parallel for schedule(dynamic, 1) if (F >= 4 && nr * F >= 4096)
+  for (int64_t f = 0; f < F; f++) {
+    double* hs = out + 2 * (size_t)B.eff_offset[f];
+    const uint16_t* col = B.bins[f].data();
+    if (rows) {
+      for (int64_t i = 0; i < nr; i++) {
+        uint32_t r = rows[i];
+        uint32_t b = col[r];
+        hs[2 * b] += (double)g[r];
+        hs[2 * b + 1] += (double)h[r];
+      }
+    } else {
+      for (int64_t r = 0; r < nr; r++) {
+        uint32_t b = col[r];
+        hs[2 * b] += (double)g[r];
+        hs[2 * b + 1] += (double)h[r];
+      }
+    }
+  }
+}
+
+// Goes-left predicate, partition.rs:321-337.
+inline bool goes_left(const ora_binned& B, const ora_split& s, uint32_t bin) {
+  uint32_t f = s.feature;
+  const Mapper& m = B.mappers[B.eff_to_orig[f]];
+  bool has_missing = m.missing != 0;
+  if (has_missing && bin == m.default_bin) return s.default_left != 0;
+  if (s.is_categorical) return bin < 256 && ((s.left_cats[bin >> 5] >> (bin & 31)) & 1u);
+  return bin <= s.bin;
+}
+
+// stable_partition_range, partition.rs:73-145.
+int64_t stable_partition(const ora_binned& B, uint32_t* rows, int64_t n, const ora_split& s,
+                         std::vector<uint32_t>& scratch) {
+  scratch.resize(n);
+  const uint16_t* col = B.bins[s.feature].data();
+  int64_t lw = 0, rw = n;
+  for (int64_t i = 0; i < n; i++) {
+    uint32_t r = rows[i];
+    if (goes_left(B, s, col[r]))
+      scratch[lw++] = r;
+    else
+      scratch[--rw] = r;
+  }
+  std::reverse(scratch.begin() + lw, scratch.begin() + n);
+  std::copy(scratch.begin(), scratch.begin() + n, rows);
+  return lw;
+}
+
+// ---------------------------------------------------------------------------
+// RNG: xoshiro256++ with splitmix64 seeding (rand 0.8.5 SmallRng on 64-bit).
+// ---------------------------------------------------------------------------
+struct Xoshiro {
+  uint64_t s[4];
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  explicit Xoshiro(uint64_t seed) {
+    for (int i = 0; i < 4; i++) {
+      seed += 0x9e3779b97f4a7c15ULL;
+      uint64_t z = seed;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+      s[i] = z ^ (z >> 31);
+    }
+  }
+  uint64_t next() {
+    uint64_t r = rotl(s[0] + s[3], 23) + s[0];
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  float next_f32() { return (float)((uint32_t)(next() >> 32) >> 8) * (1.0f / 16777216.0f); }
+  // UniformInt::sample_single_inclusive for u64 (widening multiply + zone).
+  uint64_t range_incl(uint64_t lo, uint64_t hi) {
+    uint64_t range = hi - lo + 1;
+    if (range == 0) return next();
+    int lz = __builtin_clzll(range);
+    uint64_t zone = (range << lz) - 1;
+    for (;;) {
+      uint64_t v = next();
+      unsigned __int128 m = (unsigned __int128)v * range;
+      uint64_t mhi = (uint64_t)(m >> 64), mlo = (uint64_t)m;
+      if (mlo <= zone) return lo + mhi;
+    }
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Model
+// ---------------------------------------------------------------------------
+struct OTree {
+  std::vector<uint32_t> split_index, left, right;
+  std::vector<float> thr, leaf, gain, cover;
+  std::vector<uint8_t> dleft, is_leaf, stype;
+  std::vector<std::pair<uint32_t, std::vector<uint32_t>>> cat_nodes;
+  uint32_t alloc() {
+    split_index.push_back(0);
+    left.push_back(0);
+    right.push_back(0);
+    thr.push_back(0.0f);
+    leaf.push_back(0.0f);
+    gain.push_back(0.0f);
+    cover.push_back(0.0f);
+    dleft.push_back(0);
+    is_leaf.push_back(0);
+    stype.push_back(0);
+    return (uint32_t)split_index.size() - 1;
+  }
+};
+
+struct ora_model {
+  int32_t objective = 0, n_groups = 1, n_features = 0, transform = 0;
+  std::vector<float> base;
+  std::vector<OTree> trees;
+  std::vector<uint32_t> groups;
+  std::vector<double> metric;
+  int64_t hist_rows = 0;
+  double ms_total = 0;
+};
+
+namespace {
+
+// next_up_f32, grower.rs:543-556.
+float next_up(float x) {
+  if (std::isnan(x) || x == std::numeric_limits<float>::infinity()) return x;
+  if (x == 0.0f) {
+    uint32_t one = 1;
+    float r;
+    memcpy(&r, &one, 4);
+    return r;
+  }
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  b = x > 0 ? b + 1 : b - 1;
+  float r;
+  memcpy(&r, &b, 4);
+  return r;
+}
+
+struct Cand {  // NodeCandidate, expansion.rs:242-320
+  uint32_t node, tree_node;
+  uint32_t depth;
+  ora_split split;
+  double g, h;
+  uint32_t count;
+};
+struct CandLess {  // BinaryHeap max by (gain, partition node id)
+  bool operator()(const Cand& a, const Cand& b) const {
+    float ga = a.split.gain, gb = b.split.gain;
+    if (ga < gb) return true;
+    if (ga > gb) return false;
+    return a.node < b.node;
+  }
+};
+
+struct Grower {
+  const ora_binned& B;
+  const bst_gbdt_config& c;
+  int n_threads;
+  std::vector<uint32_t> idx, scratch;
+  std::vector<uint32_t> leaf_begin, leaf_count;
+  uint32_t n_part = 0;
+  std::vector<std::vector<double>> hist;  // by partition node id
+  std::vector<float> last_leaf;           // by partition node id (lr applied)
+  int64_t hist_rows = 0;
+  std::vector<uint32_t> feats;
+  Xoshiro* col_rng;
+
+  Grower(const ora_binned& b, const bst_gbdt_config& cfg, int nt, Xoshiro* rng)
+      : B(b), c(cfg), n_threads(nt), col_rng(rng) {}
+
+  void build(uint32_t node, const float* g, const float* h) {
+    auto& hs = hist[node];
+    hs.assign((size_t)B.total_bins * 2, 0.0);
+    uint32_t nr = leaf_count[node];
+    if (nr == 0) return;
+    build_hist(B, g, h, idx.data() + leaf_begin[node], nr, hs.data());
+    hist_rows += nr;
+  }
+  std::pair<double, double> sums(uint32_t node) {  // pool.rs:536-549 (feature 0)
+    double sg = 0, sh = 0;
+    const auto& hs = hist[node];
+    for (uint32_t b = 0; b < B.eff_nbins[0]; b++) {
+      sg += hs[2 * b];
+      sh += hs[2 * b + 1];
+    }
+    return {sg, sh};
+  }
+  ora_split split_of(uint32_t node, double g, double h, uint32_t cnt) {
+    return find_split(B, hist[node].data(), g, h, cnt, c, n_threads, feats);
+  }
+
+  // TreeGrower::grow, grower.rs:265-342 (+ process_candidate :346-481).
+  OTree grow(const float* g, const float* h, const std::vector<uint32_t>* sampled) {
+    int64_t n = B.n;
+    uint32_t max_leaves = c.growth == BST_GROW_DEPTHWISE ? (1u << c.max_depth) : c.max_leaves;
+    size_t max_nodes = std::max<size_t>(2 * (size_t)max_leaves - 1, 63);
+    if (sampled)
+      idx = *sampled;
+    else {
+      idx.resize(n);
+      for (int64_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+    }
+    leaf_begin.assign(max_nodes + 2, 0);
+    leaf_count.assign(max_nodes + 2, 0);
+    leaf_count[0] = (uint32_t)idx.size();
+    n_part = 1;
+    hist.assign(max_nodes + 2, {});
+    last_leaf.assign(max_nodes + 2, std::numeric_limits<float>::quiet_NaN());
+    sample_columns();
+
+    OTree t;
+    uint32_t root = t.alloc();
+    build(0, g, h);
+    auto [tg, th] = sums(0);
+    ora_split rs = split_of(0, tg, th, (uint32_t)n);
+    Cand rootc{0, root, 0, rs, tg, th, (uint32_t)n};
+
+    auto make_leaf = [&](const Cand& cd) {
+      float w = leaf_weight(c, cd.g, cd.h);
+      t.is_leaf[cd.tree_node] = 1;
+      t.leaf[cd.tree_node] = w;
+      t.cover[cd.tree_node] = (float)cd.h;
+      last_leaf[cd.node] = w * c.learning_rate;
+      hist[cd.node].clear();
+    };
+    auto expand = [&](const Cand& cd, std::vector<Cand>& out) {
+      if (!(cd.split.gain > -std::numeric_limits<float>::infinity()) ||
+          cd.split.gain <= c.min_gain) {
+        make_leaf(cd);
+        return;
+      }
+      t.gain[cd.tree_node] = cd.split.gain;
+      t.cover[cd.tree_node] = (float)cd.h;
+      uint32_t lt, rt;
+      apply_split(t, cd.tree_node, cd.split, lt, rt);
+      // partitioner.split: left keeps the id, right gets a new id.
+      uint32_t begin = leaf_begin[cd.node], cnt = leaf_count[cd.node];
+      int64_t lc = stable_partition(B, idx.data() + begin, cnt, cd.split, scratch);
+      uint32_t rnode = n_part++;
+      leaf_count[cd.node] = (uint32_t)lc;
+      leaf_begin[rnode] = begin + (uint32_t)lc;
+      leaf_count[rnode] = cnt - (uint32_t)lc;
+      uint32_t lnode = cd.node;
+      uint32_t lcount = (uint32_t)lc, rcount = cnt - (uint32_t)lc;
+      bool left_small = lcount <= rcount;
+      uint32_t small = left_small ? lnode : rnode, large = left_small ? rnode : lnode;
+      uint32_t scount = left_small ? lcount : rcount, gcount = left_small ? rcount : lcount;
+      std::vector<double> parent = std::move(hist[cd.node]);
+      build(small, g, h);
+      hist[large] = std::move(parent);
+      for (size_t k = 0; k < hist[large].size(); k++) hist[large][k] -= hist[small][k];
+      auto [sg, sh] = sums(small);
+      double lg_ = cd.g - sg, lh_ = cd.h - sh;
+      ora_split ss = split_of(small, sg, sh, scount);
+      ora_split ls = split_of(large, lg_, lh_, gcount);
+      Cand cl, cr;
+      if (left_small) {
+        cl = {lnode, lt, cd.depth + 1, ss, sg, sh, scount};
+        cr = {rnode, rt, cd.depth + 1, ls, lg_, lh_, gcount};
+      } else {
+        cl = {lnode, lt, cd.depth + 1, ls, lg_, lh_, gcount};
+        cr = {rnode, rt, cd.depth + 1, ss, sg, sh, scount};
+      }
+      out.push_back(cl);
+      out.push_back(cr);
+    };
+
+    if (c.growth == BST_GROW_DEPTHWISE) {
+      std::vector<Cand> level{rootc}, next;
+      uint32_t depth = 0;
+      while (depth < c.max_depth && !level.empty()) {
+        next.clear();
+        for (auto& cd : level) expand(cd, next);
+        level.swap(next);
+        depth++;
+      }
+      for (auto& cd : level) make_leaf(cd);
+    } else {
+      std::priority_queue<Cand, std::vector<Cand>, CandLess> heap;
+      heap.push(rootc);
+      uint32_t n_leaves = 1;
+      while (n_leaves < c.max_leaves && !heap.empty()) {
+        Cand cd = heap.top();
+        heap.pop();
+        n_leaves++;
+        std::vector<Cand> kids;
+        expand(cd, kids);
+        for (auto& k : kids) heap.push(k);
+      }
+      while (!heap.empty()) {
+        make_leaf(heap.top());
+        heap.pop();
+      }
+    }
+    for (size_t i = 0; i < t.leaf.size(); i++)  // apply_learning_rate, mutable_tree.rs:267-273
+      if (t.is_leaf[i]) t.leaf[i] = t.leaf[i] * c.learning_rate;
+    return t;
+  }
+
+  // apply_split_to_builder, grower.rs:576-650.
+  void apply_split(OTree& t, uint32_t node, const ora_split& s, uint32_t& lt, uint32_t& rt) {
+    uint32_t of = B.eff_to_orig[s.feature];
+    const Mapper& m = B.mappers[of];
+    uint32_t a = t.alloc(), b = t.alloc();
+    t.split_index[node] = of;
+    t.left[node] = a;
+    t.right[node] = b;
+    if (!s.is_categorical) {
+      float up = (float)m.bounds[s.bin];
+      t.thr[node] = next_up(up);
+      t.dleft[node] = (uint8_t)s.default_left;
+      t.stype[node] = 0;
+      lt = a;
+      rt = b;
+    } else {
+      std::vector<uint32_t> words;
+      int maxc = -1;
+      for (int k = 0; k < 256; k++)
+        if ((s.left_cats[k >> 5] >> (k & 31)) & 1u) maxc = k;
+      if (maxc >= 0) {
+        words.assign((size_t)(maxc >> 5) + 1, 0u);
+        for (int k = 0; k <= maxc; k++)
+          if ((s.left_cats[k >> 5] >> (k & 31)) & 1u) words[k >> 5] |= 1u << (k & 31);
+      }
+      t.thr[node] = 0.0f;
+      t.dleft[node] = s.default_left ? 0 : 1;
+      t.stype[node] = 1;
+      t.cat_nodes.push_back({node, words});
+      lt = b;  // training-left rows take the inference-right child
+      rt = a;
+    }
+  }
+
+  // ColSampler (column.rs) restated with a seeded stream: per tree, then per
+  // level (level sampling is applied on the tree subset).
+  void sample_columns() {
+    uint32_t F = (uint32_t)B.eff_to_orig.size();
+    feats.resize(F);
+    for (uint32_t i = 0; i < F; i++) feats[i] = i;
+    float rate = c.colsample_bytree;
+    if (!(std::fabs(rate - 1.0f) < 1e-6f) && col_rng) {
+      uint32_t k = std::max<uint32_t>(1, (uint32_t)std::ceil((double)F * rate));
+      for (uint32_t i = 0; i < k && i < F; i++) {
+        uint32_t j = i + (uint32_t)col_rng->range_incl(0, F - 1 - i);
+        std::swap(feats[i], feats[j]);
+      }
+      feats.resize(std::min(k, F));
+      std::sort(feats.begin(), feats.end());
+    }
+  }
+
+  // update_predictions_from_last_tree, grower.rs:208-219.
+  void update_preds(float* preds) {
+    for (uint32_t node = 0; node < n_part; node++) {
+      float v = last_leaf[node];
+      if (std::isnan(v)) continue;
+      for (uint32_t i = leaf_begin[node]; i < leaf_begin[node] + leaf_count[node]; i++)
+        preds[idx[i]] += v;
+    }
+  }
+};
+
+// Tree traversal, tree_view.rs:119-160.
+uint32_t traverse(const bst_forest_arrays* f, int64_t base, const float* X, int64_t n, int64_t row) {
+  uint32_t node = 0;
+  while (!f->is_leaf[base + node]) {
+    int64_t k = base + node;
+    float x = X[(int64_t)f->split_index[k] * n + row];
+    if (std::isnan(x)) {
+      node = f->default_left[k] ? f->left[k] : f->right[k];
+    } else if (f->split_type[k] == 0) {
+      node = x < f->split_threshold[k] ? f->left[k] : f->right[k];
+    } else {
+      // float_to_category: Rust `f32 as u32` saturates (negatives -> 0).
+      uint32_t cat = x <= 0.0f ? 0u : (x >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)x);
+      uint32_t size = f->cat_size[k], w = cat >> 5;
+      bool right = size != 0 && w < size && ((f->cat_words[f->cat_start[k] + w] >> (cat & 31)) & 1u);
+      node = right ? f->right[k] : f->left[k];
+    }
+  }
+  return node;
+}
+
+float sigmoid_t(float x) {  // transform.rs:106-115
+  float c = x;  // f32::clamp keeps NaN
+  if (c < -500.0f) c = -500.0f;
+  if (c > 500.0f) c = 500.0f;
+  if (c >= 0.0f) return 1.0f / (1.0f + exp32(-c));
+  float e = exp32(c);
+  return e / (1.0f + e);
+}
+
+void transform(int tr, float* out, int K, int64_t n) {
+  if (tr == BST_TRANSFORM_SIGMOID) {
+    for (int64_t i = 0; i < (int64_t)K * n; i++) out[i] = sigmoid_t(out[i]);
+  } else if (tr == BST_TRANSFORM_SOFTMAX) {  // transform.rs:83-103
+    for (int64_t i = 0; i < n; i++) {
+      float mx = -std::numeric_limits<float>::infinity();
+      for (int k = 0; k < K; k++) mx = fmaxf(mx, out[k * n + i]);
+      float s = 0.0f;
+      for (int k = 0; k < K; k++) {
+        float e = exp32(out[k * n + i] - mx);
+        out[k * n + i] = e;
+        s += e;
+      }
+      if (s > 0.0f)
+        for (int k = 0; k < K; k++) out[k * n + i] /= s;
+    }
+  }
+}
+
+double metric_value(int metric, const float* p, int K, const float* y, const float* w, int64_t n) {
+  if (n == 0) return 0.0;
+  const double EPS = 1e-15;
+  if (metric == BST_METRIC_RMSE) {  // regression.rs:17-40
+    double s = 0;
+    for (int k = 0; k < K; k++) {
+      double ss = 0, sw = 0;
+      for (int64_t i = 0; i < n; i++) {
+        double d = (double)p[k * n + i] - (double)y[i];
+        ss += (double)wt(w, i) * d * d;
+        sw += (double)wt(w, i);
+      }
+      s += std::sqrt(ss / sw);
+    }
+    return s / K;
+  }
+  if (metric == BST_METRIC_LOGLOSS) {  // classification.rs:13-40
+    double sl = 0, sw = 0;
+    for (int64_t i = 0; i < n; i++) {
+      double q = std::min(std::max((double)p[i], EPS), 1.0 - EPS), l = y[i];
+      sl += (double)wt(w, i) * (-(l * std::log(q) + (1.0 - l) * std::log(1.0 - q)));
+      sw += (double)wt(w, i);
+    }
+    return sw > 0 ? sl / sw : 0.0;
+  }
+  if (metric == BST_METRIC_MLOGLOSS) {  // classification.rs:332-360
+    double sl = 0, sw = 0;
+    for (int64_t i = 0; i < n; i++) {
+      size_t c = (size_t)std::llround((double)y[i]);
+      double q = std::min(std::max((double)p[c * n + i], EPS), 1.0 - EPS);
+      sl += (double)wt(w, i) * (-std::log(q));
+      sw += (double)wt(w, i);
+    }
+    return sw > 0 ? sl / sw : 0.0;
+  }
+  return std::numeric_limits<double>::quiet_NaN();
+}
+
+int default_metric(int obj) {
+  return obj == BST_OBJ_LOGISTIC ? BST_METRIC_LOGLOSS
+                                 : (obj == BST_OBJ_SOFTMAX ? BST_METRIC_MLOGLOSS : BST_METRIC_RMSE);
+}
+
+int transform_of(int obj) {
+  return obj == BST_OBJ_LOGISTIC ? BST_TRANSFORM_SIGMOID
+                                 : (obj == BST_OBJ_SOFTMAX ? BST_TRANSFORM_SOFTMAX
+                                                           : BST_TRANSFORM_IDENTITY);
+}
+
+void export_sizes(const ora_model& m, bst_model_info* info) {
+  int64_t nodes = 0, words = 0;
+  for (auto& t : m.trees) {
+    nodes += (int64_t)t.split_index.size();
+    for (auto& cn : t.cat_nodes) words += (int64_t)cn.second.size();
+  }
+  info->n_trees = (int64_t)m.trees.size();
+  info->n_groups = m.n_groups;
+  info->n_features = m.n_features;
+  info->total_nodes = nodes;
+  info->total_cat_words = words;
+  info->output_transform = m.transform;
+  info->objective = m.objective;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API
+// ===========================================================================
+extern "C" {
+
+const char* ora_last_error(void) { return g_err.c_str(); }
+
+int ora_num_threads(void) {
+#ifdef _OPENMP
+  return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void ora_set_exp_mode(int mode) { g_exp_mode = mode; }
+
+void ora_set_num_threads(int n) {
+  g_threads = n;
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#endif
+}
+
+int ora_bin(const float* X, int64_t n, int32_t F, const uint8_t* is_cat,
+            const bst_binning_config* cfgp, ora_binned** out) {
+  if (!X || !out) return fail(BST_E_INVALID, "null argument");
+  if (n <= 0 || F <= 0) return fail(BST_E_SHAPE, "empty dataset");
+  bst_binning_config cfg = cfgp ? *cfgp
+                                : bst_binning_config{256, 0.9f, 1, 0, 200000};
+  auto* B = new ora_binned();
+  B->n = n;
+  B->F = F;
+  B->mappers.resize(F);
+  std::vector<Analysis> an(F);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int32_t f = 0; f < F; f++) an[f] = analyze(X + (int64_t)f * n, n, is_cat && is_cat[f], cfg);
+  std::vector<uint32_t> sample = binning_sample(n, cfg.sample_cnt);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int32_t f = 0; f < F; f++) {
+    const float* col = X + (int64_t)f * n;
+    Mapper m;
+    if (an[f].numeric) {
+      std::vector<float> vals;
+      if (sample.empty())
+        vals.assign(col, col + n);
+      else {
+        vals.resize(sample.size());
+        for (size_t i = 0; i < sample.size(); i++) vals[i] = col[sample[i]];
+      }
+      m = bin_numeric(vals, cfg.max_bins, an[f].n_nan > 0);
+    } else {
+      std::unordered_set<int32_t> seen;
+      for (int64_t i = 0; i < n; i++)
+        if (std::isfinite(col[i])) {
+          double v = col[i];
+          int32_t c = v >= 2147483647.0 ? 2147483647
+                                        : (v <= -2147483648.0 ? (int32_t)-2147483648LL : (int32_t)v);
+          seen.insert(c);
+        }
+      m.categorical = true;
+      m.cats.assign(seen.begin(), seen.end());
+      std::sort(m.cats.begin(), m.cats.end());
+      if (m.cats.empty()) m.cats.push_back(0);
+      m.n_bins = (uint32_t)m.cats.size();
+    }
+    m.trivial = an[f].trivial;
+    B->mappers[f] = m;
+  }
+  uint32_t off = 0;
+  for (int32_t f = 0; f < F; f++) {
+    if (B->mappers[f].trivial) continue;
+    B->eff_to_orig.push_back((uint32_t)f);
+    B->eff_nbins.push_back(B->mappers[f].n_bins);
+    B->eff_offset.push_back(off);
+    B->eff_cat.push_back(B->mappers[f].categorical);
+    B->eff_missing.push_back(B->mappers[f].missing != 0);
+    off += B->mappers[f].n_bins;
+  }
+  B->total_bins = off;
+  size_t E = B->eff_to_orig.size();
+  B->bins.resize(E);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t e = 0; e < E; e++) {
+    const Mapper& m = B->mappers[B->eff_to_orig[e]];
+    const float* col = X + (int64_t)B->eff_to_orig[e] * n;
+    auto& v = B->bins[e];
+    v.resize(n);
+    for (int64_t i = 0; i < n; i++) v[i] = (uint16_t)value_to_bin(m, (double)col[i]);
+  }
+  if (E == 0) {
+    delete B;
+    return fail(BST_E_SHAPE, "all features are trivial");
+  }
+  *out = B;
+  return BST_OK;
+}
+
+void ora_binned_free(ora_binned* b) { delete b; }
+
+int ora_feature_info(const ora_binned* b, int32_t f, uint32_t* nb, int32_t* mt, uint32_t* db,
+                     int32_t* cat, int32_t* triv) {
+  if (!b || f < 0 || f >= b->F) return fail(BST_E_INVALID, "bad feature");
+  const Mapper& m = b->mappers[f];
+  if (nb) *nb = m.n_bins;
+  if (mt) *mt = m.missing;
+  if (db) *db = m.default_bin;
+  if (cat) *cat = m.categorical;
+  if (triv) *triv = m.trivial;
+  return BST_OK;
+}
+
+int ora_bin_bounds(const ora_binned* b, int32_t f, double* out, int64_t cap) {
+  if (!b || f < 0 || f >= b->F) return fail(BST_E_INVALID, "bad feature");
+  const Mapper& m = b->mappers[f];
+  int64_t k = 0;
+  if (m.categorical)
+    for (int32_t c : m.cats) {
+      if (k < cap) out[k] = c;
+      k++;
+    }
+  else
+    for (double v : m.bounds) {
+      if (k < cap) out[k] = v;
+      k++;
+    }
+  return (int)k;
+}
+
+int ora_get_bins(const ora_binned* b, uint16_t* out) {
+  if (!b || !out) return fail(BST_E_INVALID, "null");
+  std::fill(out, out + (size_t)b->F * b->n, 0);
+  for (size_t e = 0; e < b->eff_to_orig.size(); e++)
+    std::copy(b->bins[e].begin(), b->bins[e].end(), out + (size_t)b->eff_to_orig[e] * b->n);
+  return BST_OK;
+}
+
+int ora_hist_layout(const ora_binned* b, int32_t* ne, uint32_t* e2o, uint32_t* enb, uint32_t* eoff) {
+  if (!b) return fail(BST_E_INVALID, "null");
+  size_t E = b->eff_to_orig.size();
+  if (ne) *ne = (int32_t)E;
+  for (size_t e = 0; e < E; e++) {
+    if (e2o) e2o[e] = b->eff_to_orig[e];
+    if (enb) enb[e] = b->eff_nbins[e];
+    if (eoff) eoff[e] = b->eff_offset[e];
+  }
+  return (int)b->total_bins;
+}
+
+uint32_t ora_value_to_bin(const ora_binned* b, int32_t f, double v) {
+  if (!b || f < 0 || f >= b->F) return 0xFFFFFFFFu;
+  return value_to_bin(b->mappers[f], v);
+}
+
+float ora_leaf_weight(const bst_gbdt_config* cfg, double g, double h) {
+  return leaf_weight(*cfg, g, h);
+}
+
+int ora_gradients(int32_t obj, int32_t K, const float* p, const float* y, const float* w, int64_t n,
+                  float* g, float* h) {
+  if (!p || !y || !g || !h) return fail(BST_E_INVALID, "null");
+  gradients(obj, K, p, y, w, n, g, h);
+  return BST_OK;
+}
+
+int ora_base_score(int32_t obj, int32_t K, const float* y, const float* w, int64_t n, float* out) {
+  auto b = base_score(obj, K, y, w, n);
+  for (size_t i = 0; i < b.size(); i++) out[i] = b[i];
+  return BST_OK;
+}
+
+int ora_histogram(const ora_binned* b, const float* g, const float* h, const uint32_t* rows,
+                  int64_t nr, double* out) {
+  if (!b || !g || !h || !out) return fail(BST_E_INVALID, "null");
+  build_hist(*b, g, h, rows, rows ? nr : b->n, out);
+  return BST_OK;
+}
+
+int ora_find_split(const ora_binned* b, const double* hist, double pg, double ph, uint32_t pc,
+                   const bst_gbdt_config* cfg, int32_t nt, const uint32_t* feats, int32_t nf,
+                   ora_split* out) {
+  if (!b || !hist || !cfg || !out) return fail(BST_E_INVALID, "null");
+  std::vector<uint32_t> fs;
+  if (feats)
+    fs.assign(feats, feats + nf);
+  else
+    for (uint32_t i = 0; i < b->eff_to_orig.size(); i++) fs.push_back(i);
+  *out = find_split(*b, hist, pg, ph, pc, *cfg, nt, fs);
+  return BST_OK;
+}
+
+int ora_partition(const ora_binned* b, uint32_t* rows, int64_t n, const ora_split* s, int64_t* lc) {
+  if (!b || !rows || !s) return fail(BST_E_INVALID, "null");
+  std::vector<uint32_t> scratch;
+  int64_t l = stable_partition(*b, rows, n, *s, scratch);
+  if (lc) *lc = l;
+  return BST_OK;
+}
+
+int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, const float* w,
+              const float* VX, const float* vy, int64_t nv, const bst_gbdt_config* cfgp,
+              const bst_ext_config* extp, ora_model** out) {
+  if (!b || !y || !cfgp || !out) return fail(BST_E_INVALID, "null");
+  const bst_gbdt_config& c = *cfgp;
+  bst_ext_config ext;
+  ext.schedule = 0;
+  ext.n_threads = 0;
+  ext.goss_top_rate = 0;
+  ext.goss_other_rate = 0;
+  ext.query_group_sizes = nullptr;
+  ext.n_query_groups = 0;
+  ext.autotune = 0;
+  if (extp) ext = *extp;
+  int64_t n = b->n;
+  if (ny < n) return fail(BST_E_TARGETS, "fewer targets than rows");
+  int K = c.objective == BST_OBJ_SOFTMAX ? (int)c.n_classes : 1;
+  auto t0 = std::chrono::steady_clock::now();
+  auto* M = new ora_model();
+  M->objective = c.objective;
+  M->n_groups = K;
+  M->n_features = b->F;
+  M->transform = transform_of(c.objective);
+  M->base = base_score(c.objective, K, y, w, n);
+  std::vector<float> preds((size_t)K * n), g((size_t)K * n), h((size_t)K * n);
+  for (int k = 0; k < K; k++) std::fill(preds.begin() + (size_t)k * n, preds.begin() + (size_t)(k + 1) * n, M->base[k]);
+  int metric = c.metric == BST_METRIC_DEFAULT ? default_metric(c.objective) : c.metric;
+  std::vector<float> vpreds;
+  if (VX && vy && nv > 0) {
+    vpreds.resize((size_t)K * nv);
+    for (int k = 0; k < K; k++) std::fill(vpreds.begin() + (size_t)k * nv, vpreds.begin() + (size_t)(k + 1) * nv, M->base[k]);
+  }
+  Xoshiro col_rng(c.seed);
+  Xoshiro row_rng(c.seed);
+  Grower gr(*b, c, ext.n_threads, &col_rng);
+  bool goss = ext.goss_top_rate > 0.0f;
+  bool uniform = !goss && !(std::fabs(c.subsample - 1.0f) < 1e-6f);
+  size_t warmup = goss ? (size_t)(1.0f / c.learning_rate) : 0;
+  int es_patience = (int)c.early_stopping_rounds;
+  bool hib = false;
+  double best_val = 0;
+  bool have_best = false;
+  size_t best_round = 0, cur_round = 0, best_n_trees = 0;
+  for (uint32_t round = 0; round < c.n_trees; round++) {
+    gradients(c.objective, K, preds.data(), y, w, n, g.data(), h.data());
+    for (int k = 0; k < K; k++) {
+      float* gk = g.data() + (size_t)k * n;
+      float* hk = h.data() + (size_t)k * n;
+      std::vector<uint32_t> sampled;
+      bool use_sample = false;
+      if (uniform) {  // sample_uniform, row.rs:170-208
+        size_t tc = std::min<size_t>(std::max<size_t>((size_t)((float)n * c.subsample), 1), n);
+        sampled.resize(tc);
+        for (size_t i = 0; i < tc; i++) sampled[i] = (uint32_t)i;
+        for (size_t i = tc; i < (size_t)n; i++) {
+          size_t j = (size_t)row_rng.range_incl(0, i);
+          if (j < tc) sampled[j] = (uint32_t)i;
+        }
+        std::vector<uint8_t> mask(n, 0);
+        for (auto r : sampled) mask[r] = 1;
+        for (int64_t i = 0; i < n; i++)
+          if (!mask[i]) gk[i] = hk[i] = 0.0f;
+        std::sort(sampled.begin(), sampled.end());
+        use_sample = true;
+      } else if (goss && round >= warmup) {  // sample_goss, row.rs:211-270
+        std::vector<float> mag(n);
+        for (int64_t i = 0; i < n; i++) mag[i] = std::fabs(gk[i] * hk[i]);
+        size_t top_k = std::min<size_t>(std::max<size_t>((size_t)((float)n * ext.goss_top_rate), 1), n - 1);
+        size_t other_k = std::max<size_t>((size_t)((float)n * ext.goss_other_rate), 1);
+        std::vector<float> sm = mag;
+        std::nth_element(sm.begin(), sm.begin() + (n - top_k), sm.end());
+        float thr = sm[n - top_k];
+        float mult = (float)(n - top_k) / (float)other_k;
+        size_t big = 0, small = 0;
+        for (int64_t i = 0; i < n; i++) {
+          if (mag[i] >= thr) {
+            sampled.push_back((uint32_t)i);
+            big++;
+          } else {
+            size_t need = other_k > small ? other_k - small : 0;
+            size_t tk = top_k > big ? top_k - big : 0;
+            size_t all = (size_t)(n - i) > tk ? (size_t)(n - i) - tk : 0;
+            if (all > 0) {
+              float prob = (float)need / (float)all;
+              if (row_rng.next_f32() < prob) {
+                sampled.push_back((uint32_t)i);
+                small++;
+                gk[i] *= mult;
+                hk[i] *= mult;
+              }
+            }
+          }
+        }
+        use_sample = true;
+      }
+      OTree t = gr.grow(gk, hk, use_sample ? &sampled : nullptr);
+      M->trees.push_back(std::move(t));
+      M->groups.push_back((uint32_t)k);
+      OTree& tt = M->trees.back();
+      // Prediction updates: partition path unless sampled (trainer.rs:331-339).
+      if (!use_sample) {
+        gr.update_preds(preds.data() + (size_t)k * n);
+      } else {
+        // Traverse raw features of the training rows.
+        bst_model_info info;
+        std::vector<int64_t> off{0, (int64_t)tt.split_index.size()};
+        std::vector<uint32_t> cs(tt.split_index.size(), 0), csz(tt.split_index.size(), 0), words;
+        for (auto& cn : tt.cat_nodes) {
+          cs[cn.first] = (uint32_t)words.size();
+          csz[cn.first] = (uint32_t)cn.second.size();
+          words.insert(words.end(), cn.second.begin(), cn.second.end());
+        }
+        bst_forest_arrays fa{};
+        uint32_t grp = 0;
+        fa.tree_offset = off.data();
+        fa.tree_group = &grp;
+        fa.split_index = tt.split_index.data();
+        fa.split_threshold = tt.thr.data();
+        fa.left = tt.left.data();
+        fa.right = tt.right.data();
+        fa.default_left = tt.dleft.data();
+        fa.is_leaf = tt.is_leaf.data();
+        fa.split_type = tt.stype.data();
+        fa.leaf_value = tt.leaf.data();
+        fa.cat_start = cs.data();
+        fa.cat_size = csz.data();
+        fa.cat_words = words.data();
+        (void)info;
+        if (!X) {
+          delete M;
+          return fail(BST_E_INVALID, "row sampling needs raw features");
+        }
+        float* pk = preds.data() + (size_t)k * n;
+        for (int64_t i = 0; i < n; i++) pk[i] += tt.leaf[traverse(&fa, 0, X, n, i)];
+      }
+      if (!vpreds.empty()) {
+        std::vector<int64_t> off{0, (int64_t)tt.split_index.size()};
+        std::vector<uint32_t> cs(tt.split_index.size(), 0), csz(tt.split_index.size(), 0), words;
+        for (auto& cn : tt.cat_nodes) {
+          cs[cn.first] = (uint32_t)words.size();
+          csz[cn.first] = (uint32_t)cn.second.size();
+          words.insert(words.end(), cn.second.begin(), cn.second.end());
+        }
+        bst_forest_arrays fa{};
+        uint32_t grp = 0;
+        fa.tree_offset = off.data();
+        fa.tree_group = &grp;
+        fa.split_index = tt.split_index.data();
+        fa.split_threshold = tt.thr.data();
+        fa.left = tt.left.data();
+        fa.right = tt.right.data();
+        fa.default_left = tt.dleft.data();
+        fa.is_leaf = tt.is_leaf.data();
+        fa.split_type = tt.stype.data();
+        fa.leaf_value = tt.leaf.data();
+        fa.cat_start = cs.data();
+        fa.cat_size = csz.data();
+        fa.cat_words = words.data();
+        float* vk = vpreds.data() + (size_t)k * nv;
+        for (int64_t i = 0; i < nv; i++) vk[i] += tt.leaf[traverse(&fa, 0, VX, nv, i)];
+      }
+    }
+    // evaluate_round (eval.rs:218-260) on transformed predictions.
+    double value = std::numeric_limits<double>::quiet_NaN();
+    if (metric != BST_METRIC_NONE) {
+      if (!vpreds.empty()) {
+        std::vector<float> tp = vpreds;
+        transform(M->transform, tp.data(), K, nv);
+        value = metric_value(metric, tp.data(), K, vy, nullptr, nv);
+      } else {
+        std::vector<float> tp = preds;
+        transform(M->transform, tp.data(), K, n);
+        value = metric_value(metric, tp.data(), K, y, w, n);
+      }
+    }
+    M->metric.push_back(value);
+    if (es_patience > 0) {  // EarlyStopping::update, callback.rs:78-118
+      bool imp = !have_best || (hib ? value > best_val : value < best_val);
+      if (imp) {
+        best_val = value;
+        have_best = true;
+        best_round = cur_round;
+        best_n_trees = M->trees.size();
+      }
+      cur_round++;
+      if (cur_round - best_round > (size_t)es_patience) break;
+    }
+  }
+  if (es_patience > 0 && best_n_trees > 0 && best_n_trees < M->trees.size()) {
+    M->trees.resize(best_n_trees);
+    M->groups.resize(best_n_trees);
+  }
+  M->hist_rows = gr.hist_rows;
+  M->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = M;
+  return BST_OK;
+}
+
+int ora_model_info(const ora_model* m, bst_model_info* info) {
+  if (!m || !info) return fail(BST_E_INVALID, "null");
+  export_sizes(*m, info);
+  return BST_OK;
+}
+
+int ora_model_export(const ora_model* m, bst_forest_arrays* o) {
+  if (!m || !o) return fail(BST_E_INVALID, "null");
+  for (int k = 0; k < m->n_groups; k++) o->base_score[k] = m->base[k];
+  int64_t off = 0, woff = 0;
+  for (size_t t = 0; t < m->trees.size(); t++) {
+    const OTree& tr = m->trees[t];
+    o->tree_offset[t] = off;
+    o->tree_group[t] = m->groups[t];
+    size_t nn = tr.split_index.size();
+    std::vector<std::pair<uint32_t, std::vector<uint32_t>>> cn = tr.cat_nodes;
+    std::sort(cn.begin(), cn.end(), [](auto& a, auto& b) { return a.first < b.first; });
+    for (size_t i = 0; i < nn; i++) {
+      o->split_index[off + i] = tr.split_index[i];
+      o->split_threshold[off + i] = tr.thr[i];
+      o->left[off + i] = tr.left[i];
+      o->right[off + i] = tr.right[i];
+      o->default_left[off + i] = tr.dleft[i];
+      o->is_leaf[off + i] = tr.is_leaf[i];
+      o->split_type[off + i] = tr.stype[i];
+      o->leaf_value[off + i] = tr.is_leaf[i] ? tr.leaf[i] : 0.0f;
+      o->gain[off + i] = tr.gain[i];
+      o->cover[off + i] = tr.cover[i];
+      o->cat_start[off + i] = 0;
+      o->cat_size[off + i] = 0;
+    }
+    for (auto& c : cn) {
+      o->cat_start[off + c.first] = (uint32_t)woff;
+      o->cat_size[off + c.first] = (uint32_t)c.second.size();
+      for (uint32_t wv : c.second) o->cat_words[woff++] = wv;
+    }
+    off += (int64_t)nn;
+  }
+  o->tree_offset[m->trees.size()] = off;
+  return BST_OK;
+}
+
+int ora_model_metric(const ora_model* m, double* out, int64_t cap, int64_t* nr) {
+  if (!m) return fail(BST_E_INVALID, "null");
+  for (int64_t i = 0; i < (int64_t)m->metric.size() && i < cap; i++) out[i] = m->metric[i];
+  if (nr) *nr = (int64_t)m->metric.size();
+  return BST_OK;
+}
+
+int ora_model_stats(const ora_model* m, int64_t* hr, double* ms) {
+  if (!m) return fail(BST_E_INVALID, "null");
+  if (hr) *hr = m->hist_rows;
+  if (ms) *ms = m->ms_total;
+  return BST_OK;
+}
+
+void ora_model_free(ora_model* m) { delete m; }
+
+int ora_predict(const bst_model_info* info, const bst_forest_arrays* f, const float* X, int64_t n,
+                int32_t F, int32_t raw, float* out, uint32_t* leaf_idx) {
+  if (!info || !f || !X || !out) return fail(BST_E_INVALID, "null");
+  if (F < info->n_features) return fail(BST_E_SHAPE, "too few features");
+  int K = (int)info->n_groups;
+  // predict_into, predictor.rs:197-302: base score, then f32 += leaf in
+  // forest order per row.
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    for (int k = 0; k < K; k++) out[k * n + i] = f->base_score[k];
+    for (int64_t t = 0; t < info->n_trees; t++) {
+      int64_t base = f->tree_offset[t];
+      uint32_t leaf = traverse(f, base, X, n, i);
+      out[(int64_t)f->tree_group[t] * n + i] += f->leaf_value[base + leaf];
+      if (leaf_idx) leaf_idx[t * n + i] = leaf;
+    }
+  }
+  if (!raw) transform(info->output_transform, out, K, n);
+  return BST_OK;
+}
+
+double ora_metric(int32_t metric, const float* p, int32_t K, const float* y, const float* w,
+                  int64_t n) {
+  return metric_value(metric, p, K, y, w, n);
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+/*
+ * bst_oracle.h — TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of egordm/boosters' histogram-GBDT path, used as the parity
+ * checker for the HIP engine and as the CPU baseline timed beside it.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the
+ * engine (boosters_amd/) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * crates/boosters/src/).  Shared POD structs come from include/boosters_amd.h.
+ */
+#ifndef BST_ORACLE_H
+#define BST_ORACLE_H
+
+#include "../include/boosters_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_binned ora_binned;
+typedef struct ora_model ora_model;
+
+/* A split decision, SplitInfo (training/gbdt/split/types.rs:34-55). */
+typedef struct ora_split {
+  uint32_t feature; /* effective column */
+  float gain;       /* -inf = invalid   */
+  int32_t default_left;
+  int32_t is_categorical;
+  uint32_t bin;           /* numeric: rows with bin <= bin go left */
+  uint32_t left_cats[8];  /* categorical: bin ids going left (<= 256) */
+} ora_split;
+
+const char* ora_last_error(void);
+int ora_num_threads(void);
+void ora_set_num_threads(int n);
+/* 0 = glibc expf (reference-faithful), 1 = f64 exp rounded to f32 (engine). */
+void ora_set_exp_mode(int mode);
+
+/* Binning: BinnedDataset::from_dataset, data/binned/dataset.rs:207-309. */
+int ora_bin(const float* feat_major, int64_t n_rows, int32_t n_feat, const uint8_t* is_categorical,
+            const bst_binning_config* cfg, ora_binned** out);
+void ora_binned_free(ora_binned* b);
+int ora_feature_info(const ora_binned* b, int32_t feature, uint32_t* n_bins, int32_t* missing_type,
+                     uint32_t* default_bin, int32_t* is_categorical, int32_t* is_trivial);
+int ora_bin_bounds(const ora_binned* b, int32_t feature, double* out, int64_t cap);
+int ora_get_bins(const ora_binned* b, uint16_t* out);
+int ora_hist_layout(const ora_binned* b, int32_t* n_effective, uint32_t* eff_to_orig,
+                    uint32_t* eff_n_bins, uint32_t* eff_offset);
+
+/* BinMapper::value_to_bin on one feature's mapper, bin_mapper.rs:210-258. */
+uint32_t ora_value_to_bin(const ora_binned* b, int32_t feature, double value);
+/* GainParams::compute_leaf_weight, split/gain.rs:151-168. */
+float ora_leaf_weight(const bst_gbdt_config* cfg, double grad_sum, double hess_sum);
+
+/* Objectives: objectives/mod.rs:236-298. */
+int ora_gradients(int32_t objective, int32_t n_outputs, const float* preds, const float* targets,
+                  const float* weights, int64_t n, float* grad, float* hess);
+int ora_base_score(int32_t objective, int32_t n_outputs, const float* targets,
+                   const float* weights, int64_t n, float* out);
+
+/* Histogram of one node in partition order, (f64, f64) bins: ops.rs:79-283. */
+int ora_histogram(const ora_binned* b, const float* grad, const float* hess, const uint32_t* rows,
+                  int64_t n_rows, double* out_gh);
+
+/* GreedySplitter::find_split, split/find.rs:113-270. hist_gh in the layout of
+ * ora_hist_layout; features nullable = all effective columns. */
+int ora_find_split(const ora_binned* b, const double* hist_gh, double parent_grad,
+                   double parent_hess, uint32_t parent_count, const bst_gbdt_config* cfg,
+                   int32_t n_threads, const uint32_t* features, int32_t n_features,
+                   ora_split* out);
+
+/* RowPartitioner::split, partition.rs:296-382 (stable; returns left count). */
+int ora_partition(const ora_binned* b, uint32_t* rows, int64_t n_rows, const ora_split* split,
+                  int64_t* left_count);
+
+/* GBDTTrainer::train, training/gbdt/trainer.rs:162-401. */
+int ora_train(const ora_binned* b, const float* feat_major, const float* targets,
+              int64_t n_targets, const float* weights, const float* valid_feat_major,
+              const float* valid_targets, int64_t n_valid, const bst_gbdt_config* cfg,
+              const bst_ext_config* ext, ora_model** out);
+int ora_model_info(const ora_model* m, bst_model_info* info);
+int ora_model_export(const ora_model* m, bst_forest_arrays* out);
+int ora_model_metric(const ora_model* m, double* out, int64_t cap, int64_t* n_rounds);
+int ora_model_stats(const ora_model* m, int64_t* hist_rows_built, double* ms_total);
+void ora_model_free(ora_model* m);
+
+/* Prediction over a forest in exchange format: predictor.rs:197-302,
+ * tree_view.rs:119-160, transform.rs:23-115. */
+int ora_predict(const bst_model_info* info, const bst_forest_arrays* f, const float* feat_major,
+                int64_t n_rows, int32_t n_feat, int32_t raw, float* out, uint32_t* leaf_idx);
+
+/* Metrics over transformed predictions [K][n]: metrics/{classification,regression}.rs. */
+double ora_metric(int32_t metric, const float* preds, int32_t n_outputs, const float* targets,
+                  const float* weights, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
