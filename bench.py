@@ -228,6 +228,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--qps", type=float, default=1000.0)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run the plain copy kernel 5x on this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU row "
@@ -237,6 +239,13 @@ def main():
     L = A.lib()
     ctx = C.c_void_p()
     A.check(L.bst_ctx_create(0, C.byref(ctx)))
+    if args.copy_bytes:
+        nbytes = args.copy_bytes // 16 * 16
+        src, dst = Dev.empty(ctx, nbytes), Dev.empty(ctx, nbytes)
+        for _ in range(5):
+            A.check(L.bst_kernel_copy(ctx, dst.p, src.p, nbytes))
+        print(json.dumps({"copy_bytes": nbytes, "launches": 5}), flush=True)
+        return
     n, F, depth = args.rows, args.features, args.depth
     log(f"generating {n} x {F} binary-logistic data with 10% NaN")
     X, y = gen_binary_nan(n, F, seed=0)
@@ -247,22 +256,35 @@ def main():
     y_dev = Dev(ctx, y)
     log(f"binned in {bin_ms:.0f} ms")
 
-    # Warmup (untimed), then exactly `steps` timed boosting iterations.
-    h, _ = train_timed(ctx, ds, y_dev, n, cfg_c2(args.warmup, depth), ext(A.SCHED_BATCHED))
-    L.bst_model_destroy(h)
+    # Warmup (untimed), then exactly `steps` timed boosting iterations on the
+    # graph schedule (device-side growth, one hipGraph replay per iteration).
+    if args.warmup > 0:
+        h, _ = train_timed(ctx, ds, y_dev, n, cfg_c2(args.warmup, depth), ext(A.SCHED_GRAPH))
+        L.bst_model_destroy(h)
+    h_model, wall_ms = train_timed(ctx, ds, y_dev, n, cfg_c2(args.steps, depth), ext(A.SCHED_GRAPH))
+    ms_tree = wall_ms / args.steps
+    log(f"graph: {ms_tree:.3f} ms/tree ({args.steps} trees in {wall_ms:.1f} ms)")
+
+    # Kernel statistics: the same schedule launched eagerly with HIP events
+    # around each kernel on the training stream (a separate, untimed run).
     A.check(L.bst_ctx_set_profiling(ctx, 1))
-    h_model, wall_ms = train_timed(ctx, ds, y_dev, n, cfg_c2(args.steps, depth), ext(A.SCHED_BATCHED))
+    h_prof, _ = train_timed(ctx, ds, y_dev, n, cfg_c2(args.steps, depth), ext(A.SCHED_GRAPH))
     hist = kernel_stats(ctx, 0)
     split = kernel_stats(ctx, 1)
     part = kernel_stats(ctx, 2)
     A.check(L.bst_ctx_set_profiling(ctx, 0))
-    ms_tree = wall_ms / args.steps
-    log(f"batched: {ms_tree:.3f} ms/tree ({args.steps} trees in {wall_ms:.1f} ms)")
+    L.bst_model_destroy(h_prof)
 
-    h_plain, plain_ms = train_timed(ctx, ds, y_dev, n, cfg_c2(args.plain_steps, depth), ext(A.SCHED_PLAIN))
-    L.bst_model_destroy(h_plain)
-    plain_ms_tree = plain_ms / args.plain_steps
-    log(f"plain: {plain_ms_tree:.3f} ms/tree")
+    h_b, batched_ms = train_timed(ctx, ds, y_dev, n, cfg_c2(args.steps, depth), ext(A.SCHED_BATCHED))
+    L.bst_model_destroy(h_b)
+    batched_ms_tree = batched_ms / args.steps
+    log(f"batched: {batched_ms_tree:.3f} ms/tree")
+    plain_ms_tree = None
+    if args.plain_steps > 0:
+        h_plain, plain_ms = train_timed(ctx, ds, y_dev, n, cfg_c2(args.plain_steps, depth), ext(A.SCHED_PLAIN))
+        L.bst_model_destroy(h_plain)
+        plain_ms_tree = plain_ms / args.plain_steps
+        log(f"plain: {plain_ms_tree:.3f} ms/tree")
 
     # Held-out validation loss of the timed model (20% split drawn from the same generator).
     Xv, yv = gen_binary_nan(n // 4, F, seed=1)
@@ -303,9 +325,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"binary logistic {n} x {F}, 10% NaN, depth-wise {depth}, 256 bins, lr 0.3",
-            "schedule": "batched (level-batched hist/split/partition launches)",
-            "plain_ms_per_tree": round(plain_ms_tree, 4),
-            "batched_speedup_vs_plain": round(plain_ms_tree / ms_tree, 2),
+            "schedule": "graph (device-side growth, one hipGraph replay per boosting iteration)",
+            "batched_ms_per_tree": round(batched_ms_tree, 4),
+            "plain_ms_per_tree": None if plain_ms_tree is None else round(plain_ms_tree, 4),
+            "graph_speedup_vs_plain": None if plain_ms_tree is None else round(plain_ms_tree / ms_tree, 2),
             "binning_ms": round(bin_ms, 1),
             "val_logloss": round(val_logloss, 6),
             "kernel_ms_per_tree": {

@@ -209,6 +209,7 @@ def _declare(lib):
         "bst_ctx_stream": (vp, [vp]),
         "bst_ctx_set_profiling": (C.c_int, [vp, C.c_int32]),
         "bst_ctx_kernel_stats": (C.c_int, [vp, C.c_int32, P(C.c_int64), P(C.c_double), P(C.c_double)]),
+        "bst_kernel_copy": (C.c_int, [vp, vp, vp, C.c_int64]),
         "bst_device_alloc": (C.c_int, [vp, C.c_int64, P(vp)]),
         "bst_device_free": (C.c_int, [vp, vp]),
         "bst_memcpy_h2d": (C.c_int, [vp, vp, vp, C.c_int64]),
@@ -296,7 +297,8 @@ def exported_symbols():
     return [
         "bst_last_error", "bst_version", "bst_gbdt_config_default", "bst_ext_config_default",
         "bst_gbdt_config_validate", "bst_ctx_create", "bst_ctx_destroy", "bst_ctx_sync",
-        "bst_ctx_stream", "bst_ctx_set_profiling", "bst_ctx_kernel_stats", "bst_device_alloc",
+        "bst_ctx_stream", "bst_ctx_set_profiling", "bst_ctx_kernel_stats", "bst_kernel_copy",
+        "bst_device_alloc",
         "bst_device_free", "bst_memcpy_h2d", "bst_memcpy_d2h",
         "bst_dataset_create", "bst_dataset_create_device",
         "bst_dataset_destroy", "bst_dataset_feature_info", "bst_dataset_bin_bounds",

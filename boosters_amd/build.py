@@ -15,7 +15,15 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(os.path.dirname(HERE), "_build", "boosters_amd")
 OUT = os.path.join(HERE, "libboosters_amd.so")
 
-SOURCES = ["kernels.hip", "binning.cpp", "trainer.cpp", "api.cpp", "server.cpp"]
+SOURCES = [
+    "kernels.hip",
+    "grow_device.hip",
+    "binning.cpp",
+    "trainer.cpp",
+    "trainer_device.cpp",
+    "api.cpp",
+    "server.cpp",
+]
 
 # -ffp-contract=off: every f32/f64 operation rounds as written (no FMA
 # contraction), which the bit-exact objective and gain arithmetic rely on.

@@ -322,6 +322,17 @@ int bst_ctx_kernel_stats(bst_ctx* ctx, int32_t kind, int64_t* launches, double* 
   });
 }
 
+int bst_kernel_copy(bst_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  return guard([&] {
+    need(ctx && dst && src && bytes >= 0 && bytes % 16 == 0, BST_E_INVALID, "bad argument");
+    set_device(ctx);
+    launch_copy16(ctx->stream, dst, src, bytes);
+    BST_HIP(hipGetLastError());
+    BST_HIP(hipStreamSynchronize(ctx->stream));
+    return BST_OK;
+  });
+}
+
 int bst_device_alloc(bst_ctx* ctx, int64_t bytes, void** out) {
   return guard([&] {
     need(ctx && out && bytes > 0, BST_E_INVALID, "bad argument");
@@ -487,7 +498,6 @@ static int train_impl(bst_ctx* ctx, bst_dataset* ds, const float* y_dev, int64_t
   opt.cfg = *cfg;
   bst_ext_config_default(&opt.ext);
   if (ext) opt.ext = *ext;
-  if (opt.ext.schedule == BST_SCHED_GRAPH) opt.ext.schedule = BST_SCHED_BATCHED;
   opt.K = cfg->objective == BST_OBJ_SOFTMAX ? (int)cfg->n_classes : 1;
   Trainer tr(ctx, ds, opt);
   bst_model* m = tr.train(y_dev, w_dev, vX_dev, vy_dev, nv);
@@ -700,6 +710,9 @@ int bst_kernel_gradients(bst_ctx* ctx, int32_t obj, int32_t K, const float* pred
   return guard([&] {
     need(ctx && preds && y && g && h, BST_E_INVALID, "null argument");
     need(obj >= 0 && obj <= BST_OBJ_SOFTMAX, BST_E_CONFIG, "objective");
+    need(K >= 1 && K <= 16 && (obj == BST_OBJ_SOFTMAX || K == 1), BST_E_CONFIG,
+         "outputs must be 1, or 2..16 for softmax");
+    need(n >= 0, BST_E_INVALID, "negative row count");
     set_device(ctx);
     hipStream_t s = ctx->stream;
     DevBuf<float> dp((size_t)K * n), dy(n), dw, dg((size_t)K * n), dh((size_t)K * n);

@@ -116,13 +116,14 @@ void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g,
 void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
                  const uint32_t* idx, const float2* ogh, int64_t tree_stride_rows,
                  const int32_t* qexp, long long* hist, int64_t slot_elems, long long* node_sum,
-                 bool gather, int rows_per_block_hint);
+                 bool gather, int rows_per_block_hint, const uint32_t* dyn = nullptr);
 void launch_node_sums(hipStream_t s, const Segment* segs, int n_segs, const float2* ogh,
-                      int64_t tree_stride_rows, const int32_t* qexp, long long* node_sum);
-void launch_subtract(hipStream_t s, const uint32_t* triples, int n, long long* hist,
-                     int64_t slot_elems, long long* node_sum);
+                      int64_t tree_stride_rows, const int32_t* qexp, long long* node_sum,
+                      const uint32_t* dyn = nullptr);
+void launch_subtract(hipStream_t s, const uint32_t* quads, int n, long long* hist,
+                     int64_t slot_elems, long long* node_sum, const uint32_t* dyn = nullptr);
 void launch_zero_slots(hipStream_t s, const uint32_t* slots, int n, long long* hist,
-                       int64_t slot_elems);
+                       int64_t slot_elems, const uint32_t* dyn = nullptr);
 
 struct SplitParams {
   double lambda;
@@ -136,7 +137,8 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
                   int n_nodes, const uint32_t* feats, int n_feats, const long long* hist,
                   int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
                   const int32_t* qexp, uint32_t max_nodes, SplitParams p,
-                  unsigned long long* node_key, DevSplit* cand, DevSplit* node_split);
+                  unsigned long long* node_key, DevSplit* cand, DevSplit* node_split,
+                  const uint32_t* dyn = nullptr);
 
 // Stable partition of every listed node (partition.rs:73-145): count, scan,
 // scatter, copy back, then children counts/sums.
@@ -144,11 +146,13 @@ void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items,
                       const PartTile* tiles, int n_tiles, const DevSplit* node_split,
                       uint32_t* idx, float2* ogh, uint32_t* idx_tmp, float2* ogh_tmp,
                       uint8_t* flag, uint32_t* tile_left, uint32_t* tile_off,
-                      uint32_t* item_left, uint32_t* node_count, long long* node_sum);
+                      uint32_t* item_left, uint32_t* node_count, long long* node_sum,
+                      const uint32_t* dyn_items = nullptr, const uint32_t* dyn_tiles = nullptr);
 
 void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const float* leaf_val,
                         const uint32_t* idx, int64_t tree_stride_rows, float* preds,
-                        const uint32_t* tree_output, int64_t n_rows);
+                        const uint32_t* tree_output, int64_t n_rows,
+                        const uint32_t* dyn = nullptr);
 
 void launch_predict(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
                     const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
@@ -156,6 +160,7 @@ void launch_predict(hipStream_t s, const DevNode* nodes, const int64_t* tree_off
                     int K, const float* X, int64_t n, int64_t x_stride, int F, int transform,
                     float* out, int64_t out_stride, uint32_t* leaf_idx);
 int predict_rows_per_block(int F);
+void launch_copy16(hipStream_t s, void* dst, const void* src, int64_t bytes);
 constexpr int kPredMaxGroups = 16;
 
 }  // namespace bst

@@ -1,0 +1,86 @@
+// grow_device.h — device-side tree growth control (the graph schedule).
+//
+// The host grower (trainer.cpp) decides which nodes to expand, allocates
+// partition / tree node ids and builds the per-step work lists on the CPU,
+// with two stream synchronisations per step.  DeviceGrower moves exactly
+// that control onto the GPU: single-block "controller" kernels read the
+// split results, make the same decisions in the same order (depth-wise level
+// order, leaf-wise max (gain, partition id) with n_leaves++ on pop), assign
+// the same ids, and write the work lists and their counts to device memory.
+// The heavy kernels are launched with worst-case grids and early-exit on the
+// device-side counts, so one boosting round is a fixed launch sequence that is
+// captured once in a hipGraph and replayed every round.
+#pragma once
+#include "engine.h"
+
+namespace bst {
+
+struct CtlParams {
+  uint32_t T;           // trees grown together
+  uint32_t MN;          // partition ids per tree
+  uint32_t SPT;         // histogram slots per tree
+  uint32_t MNT;         // tree nodes per tree
+  uint32_t W;           // max candidates expanded per step per tree
+  uint32_t WO;          // open-set capacity per tree
+  uint32_t growth;      // 0 depth-wise, 1 leaf-wise
+  uint32_t max_depth, max_leaves;
+  uint32_t NG, E, FG;
+  uint32_t hist_tile, part_tile, seg_tile;
+  int64_t stride;       // positions per tree slot in idx/ogh
+  float lr;
+  double lambda, alpha, min_gain;
+  int32_t out0;         // output of tree slot 0 (qexp / gradient row)
+};
+
+struct TreeCtl {
+  uint32_t n_part, n_tnodes, n_leaves, n_open, n_next, n_free, depth, rows;
+};
+
+// Count slots in DeviceBufs::counts.
+// C_HROWS / C_HNODES accumulate over a whole round (rows and nodes
+// histogrammed), for the algorithmic-byte accounting.
+enum : int {
+  C_ITEMS = 0, C_TILES, C_HIST, C_ZERO, C_QUADS, C_SQUADS, C_SEGS, C_SPLIT, C_LSEGS,
+  C_HROWS, C_HNODES, C_NCOUNTS
+};
+
+struct DeviceBufs {
+  TreeCtl* ctl;
+  uint32_t *open_node, *open_tnode, *open_depth;  // [T][WO]
+  uint32_t *next_node, *next_tnode, *next_depth;  // [T][2W]
+  uint32_t *ch_node, *ch_tnode, *ch_depth;        // [T][W]
+  uint32_t* pbegin;                               // [T*MN] global positions
+  uint32_t* pslot;                                // [T*MN] tree-local slot, ~0 = none
+  uint32_t* free_slot;                            // [T][SPT] stack
+  float* leaf_val;                                // [T*MN]
+  // Tree arrays [T][MNT] (cats: [T][MNT][8]).
+  uint32_t *tr_split, *tr_left, *tr_right, *tr_cats;
+  float *tr_thr, *tr_leaf, *tr_gain, *tr_cover;
+  uint8_t *tr_dleft, *tr_isleaf, *tr_stype;
+  // Step work lists.
+  PartItem* items;
+  PartTile* tiles;
+  uint32_t* pend;  // [items][4]: tree, lt, rt, rank within tree
+  HistItem* hitems;
+  uint32_t *zero, *quads, *squads, *split_nodes, *split_slots;
+  Segment *segs, *leaf_segs;
+  uint32_t* counts;  // [C_NCOUNTS]
+  // Engine buffers shared with the trainer.
+  long long* node_sum;
+  uint32_t* node_count;
+  DevSplit* node_split;
+  const float* thr_up;         // [E][256]
+  const uint32_t* eff_to_orig; // [E]
+  const int32_t* qexp;         // [2K]
+  const uint32_t* item_left;
+};
+
+void launch_ctl_init(hipStream_t s, const CtlParams& p, const DeviceBufs& b,
+                     const uint32_t* rows_per_tree);
+void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b);
+void launch_ctl_children(hipStream_t s, const CtlParams& p, const DeviceBufs& b);
+void launch_ctl_advance(hipStream_t s, const CtlParams& p, const DeviceBufs& b);
+void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b);
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, int64_t n);
+
+}  // namespace bst

@@ -1,0 +1,526 @@
+// grow_device.hip — controller kernels of the device-side grower.
+//
+// These single-block kernels restate, on the GPU, the host control of
+// trainer.cpp (itself restating training/gbdt/{grower,expansion}.rs):
+//   plan      = GrowthState::pop_next + should_expand + apply_split_to_builder
+//               + RowPartitioner::split id allocation (grower.rs:346-394,
+//               530-650; expansion.rs:150-231)
+//   children  = the subtraction-trick bookkeeping of process_candidate
+//               (grower.rs:396-480): smaller child built, larger = parent - smaller
+//   finalize  = finalize_remaining + apply_learning_rate (grower.rs:484-514,
+//               mutable_tree.rs:267-273) and the partition-range prediction
+//               update list (grower.rs:208-219)
+// Decisions, orders and ids are the same as the host grower's, so the
+// forests are identical (tested).
+#include <math.h>
+
+#include "grow_device.h"
+
+namespace bst {
+
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kDirectFlag = 0x80000000u;
+constexpr int kCtlThreads = 1024;
+
+// Exclusive scan over the block (kCtlThreads threads); *total receives the sum.
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(x, d, 64);
+    if (lane >= d) x += o;
+  }
+  if (lane == 63) s_tmp[wv] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int i = 0; i < kCtlThreads / 64; i++) {
+      const uint32_t t = s_tmp[i];
+      s_tmp[i] = s;
+      s += t;
+    }
+    s_tmp[kCtlThreads / 64] = s;
+  }
+  __syncthreads();
+  const uint32_t r = x - v + s_tmp[wv];
+  *total = s_tmp[kCtlThreads / 64];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ bool expandable(const DevSplit& s, double min_gain) {
+  return s.gain > -INFINITY && (double)s.gain > min_gain;
+}
+
+__device__ void init_tree_node(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t tn) {
+  const uint64_t k = (uint64_t)t * P.MNT + tn;
+  B.tr_split[k] = 0;
+  B.tr_left[k] = 0;
+  B.tr_right[k] = 0;
+  B.tr_thr[k] = 0.0f;
+  B.tr_leaf[k] = 0.0f;
+  B.tr_gain[k] = 0.0f;
+  B.tr_cover[k] = 0.0f;
+  B.tr_dleft[k] = 0;
+  B.tr_isleaf[k] = 0;
+  B.tr_stype[k] = 0;
+  for (int w = 0; w < kMaxCatWords; w++) B.tr_cats[k * kMaxCatWords + w] = 0;
+}
+
+// compute_leaf_weight (split/gain.rs:151-168) from exact fixed-point sums.
+__device__ void make_leaf(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t p,
+                          uint32_t tn) {
+  const uint32_t gid = t * P.MN + p;
+  const int out = P.out0 + (int)t;
+  const double G = ldexp((double)B.node_sum[2 * gid], -B.qexp[2 * out]);
+  const double H = ldexp((double)B.node_sum[2 * gid + 1], -B.qexp[2 * out + 1]);
+  float w;
+  if (P.alpha == 0.0) {
+    w = (float)(-G / (H + P.lambda));
+  } else {
+    const double ag = fabs(G);
+    w = ag <= P.alpha ? 0.0f : (float)((G > 0.0 ? -1.0 : 1.0) * (ag - P.alpha) / (H + P.lambda));
+  }
+  const uint64_t k = (uint64_t)t * P.MNT + tn;
+  B.tr_isleaf[k] = 1;
+  B.tr_leaf[k] = w;
+  B.tr_cover[k] = (float)H;
+  B.leaf_val[gid] = w * P.lr;
+  const uint32_t slot = B.pslot[gid];
+  if (slot != kNone) {
+    const uint32_t pos = atomicAdd(&B.ctl[t].n_free, 1u);
+    B.free_slot[(uint64_t)t * P.SPT + pos] = slot;
+    B.pslot[gid] = kNone;
+  }
+}
+
+}  // namespace
+
+// Root state of every tree, root histogram work and the root split list.
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBufs B,
+                                                          const uint32_t* __restrict__ rows) {
+  __shared__ uint32_t s_hist, s_zero;
+  if (threadIdx.x == 0) {
+    s_hist = 0;
+    s_zero = 0;
+  }
+  __syncthreads();
+  for (uint32_t t = 0; t < P.T; t++) {
+    for (uint32_t p = threadIdx.x; p < P.MN; p += blockDim.x) {
+      B.pslot[t * P.MN + p] = kNone;
+      B.leaf_val[t * P.MN + p] = NAN;
+    }
+    for (uint32_t i = threadIdx.x; i + 1 < P.SPT; i += blockDim.x)
+      B.free_slot[(uint64_t)t * P.SPT + i] = P.SPT - 1 - i;  // slot 0 goes to the root
+    __syncthreads();
+    const uint32_t m = rows[t];
+    const uint32_t nt = (m + P.hist_tile - 1) / P.hist_tile;
+    if (threadIdx.x == 0) {
+      TreeCtl c;
+      c.n_part = 1;
+      c.n_tnodes = 1;
+      c.n_leaves = 1;
+      c.n_open = 1;
+      c.n_next = 0;
+      c.n_free = P.SPT - 1;
+      c.depth = 0;
+      c.rows = m;
+      B.ctl[t] = c;
+      const uint32_t gid = t * P.MN;
+      B.pslot[gid] = 0;
+      B.pbegin[gid] = (uint32_t)(t * P.stride);
+      B.node_count[gid] = m;
+      B.node_sum[2 * gid] = 0;
+      B.node_sum[2 * gid + 1] = 0;
+      init_tree_node(P, B, t, 0);
+      B.open_node[(uint64_t)t * P.WO] = 0;
+      B.open_tnode[(uint64_t)t * P.WO] = 0;
+      B.open_depth[(uint64_t)t * P.WO] = 0;
+      B.split_nodes[t] = gid;
+      B.split_slots[t] = t * P.SPT;
+      if (nt != 1) B.zero[s_zero++] = t * P.SPT;
+    }
+    __syncthreads();
+    const uint32_t base = s_hist;
+    for (uint32_t k = threadIdx.x; k < nt * P.NG; k += blockDim.x) {
+      const uint32_t tile = k / P.NG, g = k % P.NG;
+      const uint32_t b0 = tile * P.hist_tile;
+      HistItem it;
+      it.node = t * P.MN;
+      it.slot = t * P.SPT;
+      it.begin = (uint32_t)(t * P.stride) + b0;
+      it.count = min(P.hist_tile, m - b0);
+      it.group = g;
+      it.tree = t | (nt == 1 ? kDirectFlag : 0u);
+      B.hitems[base + k] = it;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_hist += nt * P.NG;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < C_NCOUNTS; i++) B.counts[i] = 0;
+    B.counts[C_HIST] = s_hist;
+    B.counts[C_ZERO] = s_zero;
+    B.counts[C_SPLIT] = P.T;
+    uint32_t r = 0;
+    for (uint32_t t = 0; t < P.T; t++) r += rows[t];
+    B.counts[C_HROWS] = r;
+    B.counts[C_HNODES] = P.T;
+  }
+}
+
+// Choose the nodes to expand this step, decide leaf / split, allocate ids,
+// write the tree nodes and the partition work list.
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBufs B) {
+  __shared__ uint32_t s_tmp[kCtlThreads / 64 + 1];
+  __shared__ uint32_t s_items, s_tiles, s_nch;
+  __shared__ uint32_t s_part, s_tnodes;
+  if (threadIdx.x == 0) {
+    s_items = 0;
+    s_tiles = 0;
+  }
+  __syncthreads();
+  for (uint32_t t = 0; t < P.T; t++) {
+    const uint64_t ob = (uint64_t)t * P.WO, cb = (uint64_t)t * P.W;
+    if (threadIdx.x == 0) {
+      TreeCtl& c = B.ctl[t];
+      uint32_t nch = 0;
+      if (P.growth == 0) {
+        if (c.depth < P.max_depth) nch = c.n_open;
+      } else if (c.n_leaves < P.max_leaves && c.n_open > 0) {
+        // Max (gain, partition id): BinaryHeap order of NodeCandidate.
+        uint32_t best = 0;
+        float bg = 0.0f;
+        uint32_t bp = 0;
+        for (uint32_t i = 0; i < c.n_open; i++) {
+          const uint32_t p = B.open_node[ob + i];
+          const float g = B.node_split[t * P.MN + p].gain;
+          if (i == 0 || g > bg || (g == bg && p > bp)) {
+            best = i;
+            bg = g;
+            bp = p;
+          }
+        }
+        B.ch_node[cb] = B.open_node[ob + best];
+        B.ch_tnode[cb] = B.open_tnode[ob + best];
+        B.ch_depth[cb] = B.open_depth[ob + best];
+        const uint32_t last = c.n_open - 1;
+        B.open_node[ob + best] = B.open_node[ob + last];
+        B.open_tnode[ob + best] = B.open_tnode[ob + last];
+        B.open_depth[ob + best] = B.open_depth[ob + last];
+        c.n_open = last;
+        c.n_leaves += 1;
+        nch = 1;
+      }
+      s_nch = nch;
+      s_part = c.n_part;
+      s_tnodes = c.n_tnodes;
+    }
+    __syncthreads();
+    const uint32_t nch = s_nch;
+    if (P.growth == 0) {
+      for (uint32_t i = threadIdx.x; i < nch; i += blockDim.x) {
+        B.ch_node[cb + i] = B.open_node[ob + i];
+        B.ch_tnode[cb + i] = B.open_tnode[ob + i];
+        B.ch_depth[cb + i] = B.open_depth[ob + i];
+      }
+      __syncthreads();
+    }
+    uint32_t carry_e = 0, carry_t = 0;
+    for (uint32_t base = 0; base < nch; base += blockDim.x) {
+      const uint32_t i = base + threadIdx.x;
+      const bool in = i < nch;
+      uint32_t p = 0, tn = 0, dep = 0, count = 0;
+      DevSplit sp;
+      bool ex = false;
+      if (in) {
+        p = B.ch_node[cb + i];
+        tn = B.ch_tnode[cb + i];
+        dep = B.ch_depth[cb + i];
+        sp = B.node_split[t * P.MN + p];
+        ex = expandable(sp, P.min_gain);
+        count = B.node_count[t * P.MN + p];
+      }
+      const uint32_t ntile = ex ? max(1u, (count + P.part_tile - 1) / P.part_tile) : 0u;
+      uint32_t tot_e, tot_t;
+      const uint32_t e = block_excl_scan(ex ? 1u : 0u, s_tmp, &tot_e) + carry_e;
+      const uint32_t toff = block_excl_scan(ntile, s_tmp, &tot_t) + carry_t;
+      if (in && !ex) make_leaf(P, B, t, p, tn);
+      if (in && ex) {
+        const uint32_t gid = t * P.MN + p;
+        const int out = P.out0 + (int)t;
+        const double H = ldexp((double)B.node_sum[2 * gid + 1], -B.qexp[2 * out + 1]);
+        const uint32_t right = s_part + e;
+        const uint32_t a = s_tnodes + 2 * e, b = a + 1;
+        const uint64_t k = (uint64_t)t * P.MNT + tn;
+        const uint32_t f = sp.feature;
+        B.tr_gain[k] = sp.gain;
+        B.tr_cover[k] = (float)H;
+        B.tr_split[k] = B.eff_to_orig[f];
+        B.tr_left[k] = a;
+        B.tr_right[k] = b;
+        uint32_t lt, rt;
+        if (!(sp.flags & 2u)) {
+          B.tr_thr[k] = B.thr_up[(uint64_t)f * 256 + sp.bin];
+          B.tr_dleft[k] = (uint8_t)(sp.flags & 1u);
+          B.tr_stype[k] = 0;
+          lt = a;
+          rt = b;
+        } else {
+          B.tr_thr[k] = 0.0f;
+          B.tr_dleft[k] = (sp.flags & 1u) ? 0 : 1;
+          B.tr_stype[k] = 1;
+          for (int w = 0; w < kMaxCatWords; w++) B.tr_cats[k * kMaxCatWords + w] = sp.cats[w];
+          lt = b;  // training-left rows take the inference-right child
+          rt = a;
+        }
+        init_tree_node(P, B, t, a);
+        init_tree_node(P, B, t, b);
+        const uint32_t g = s_items + e;
+        const uint32_t begin = B.pbegin[gid];
+        PartItem it;
+        it.node = gid;
+        it.begin = begin;
+        it.count = count;
+        it.tile0 = s_tiles + toff;
+        it.n_tiles = ntile;
+        it.right_node = t * P.MN + right;
+        it.feature = f;
+        it.pad = 0;
+        B.items[g] = it;
+        B.pend[5 * g + 0] = t;
+        B.pend[5 * g + 1] = lt;
+        B.pend[5 * g + 2] = rt;
+        B.pend[5 * g + 3] = e;
+        B.pend[5 * g + 4] = dep + 1;
+        for (uint32_t q = 0; q < ntile; q++) {
+          const uint32_t b0 = q * P.part_tile;
+          PartTile pt;
+          pt.item = g;
+          pt.begin = begin + b0;
+          pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
+          pt.pad = 0;
+          B.tiles[s_tiles + toff + q] = pt;
+        }
+      }
+      carry_e += tot_e;
+      carry_t += tot_t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      TreeCtl& c = B.ctl[t];
+      c.n_part += carry_e;
+      c.n_tnodes += 2 * carry_e;
+      if (P.growth == 0) {
+        c.n_open = 0;
+        c.n_next = 0;
+      }
+      s_items += carry_e;
+      s_tiles += carry_t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    B.counts[C_ITEMS] = s_items;
+    B.counts[C_TILES] = s_tiles;
+  }
+}
+
+// After the partition: slots, the smaller child's histogram / sums work, the
+// subtraction list, the split list, and the children pushed to the open set.
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, DeviceBufs B) {
+  if (threadIdx.x == 0) {
+    B.counts[C_HIST] = 0;
+    B.counts[C_ZERO] = 0;
+    B.counts[C_QUADS] = 0;
+    B.counts[C_SQUADS] = 0;
+    B.counts[C_SEGS] = 0;
+    B.counts[C_SPLIT] = 0;
+  }
+  __syncthreads();
+  const uint32_t n_items = B.counts[C_ITEMS];
+  for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) {
+    const uint32_t t = B.pend[5 * i], lt = B.pend[5 * i + 1], rt = B.pend[5 * i + 2];
+    const uint32_t e = B.pend[5 * i + 3], cdepth = B.pend[5 * i + 4];
+    const PartItem it = B.items[i];
+    const uint32_t L = B.item_left[i], R = it.count - L;
+    const uint32_t parent = it.node, right = it.right_node;
+    const uint32_t pb = B.pbegin[parent];
+    B.pbegin[right] = pb + L;
+    const bool left_small = L <= R;
+    const uint32_t small = left_small ? parent : right, large = left_small ? right : parent;
+    const uint32_t scount = left_small ? L : R;
+    const uint32_t sbegin = left_small ? pb : pb + L;
+    const uint32_t ps = B.pslot[parent];
+    B.pslot[parent] = kNone;
+    B.pslot[large] = ps;
+    const bool need = P.growth == 1 || cdepth < P.max_depth;
+    if (need) {
+      const uint32_t k = atomicSub(&B.ctl[t].n_free, 1u) - 1u;
+      const uint32_t ss = B.free_slot[(uint64_t)t * P.SPT + k];
+      B.pslot[small] = ss;
+      const uint32_t gss = t * P.SPT + ss, gls = t * P.SPT + ps;
+      const uint32_t nt = (scount + P.hist_tile - 1) / P.hist_tile;
+      atomicAdd(&B.counts[C_HROWS], scount);
+      atomicAdd(&B.counts[C_HNODES], 1u);
+      if (nt != 1) B.zero[atomicAdd(&B.counts[C_ZERO], 1u)] = gss;
+      if (nt > 0) {
+        const uint32_t h0 = atomicAdd(&B.counts[C_HIST], nt * P.NG);
+        for (uint32_t q = 0; q < nt; q++)
+          for (uint32_t g = 0; g < P.NG; g++) {
+            HistItem h;
+            h.node = small;
+            h.slot = gss;
+            h.begin = sbegin + q * P.hist_tile;
+            h.count = min(P.hist_tile, scount - q * P.hist_tile);
+            h.group = g;
+            h.tree = t | (nt == 1 ? kDirectFlag : 0u);
+            B.hitems[h0 + q * P.NG + g] = h;
+          }
+      }
+      const uint32_t qd = atomicAdd(&B.counts[C_QUADS], 1u);
+      B.quads[4 * qd + 0] = gls;
+      B.quads[4 * qd + 1] = gss;
+      B.quads[4 * qd + 2] = large;
+      B.quads[4 * qd + 3] = small;
+      const uint32_t sidx = atomicAdd(&B.counts[C_SPLIT], 2u);
+      B.split_nodes[sidx] = parent;
+      B.split_slots[sidx] = t * P.SPT + B.pslot[parent];
+      B.split_nodes[sidx + 1] = right;
+      B.split_slots[sidx + 1] = t * P.SPT + B.pslot[right];
+    } else {
+      B.pslot[small] = kNone;
+      const uint32_t ns = (scount + P.seg_tile - 1) / P.seg_tile;
+      if (ns > 0) {
+        const uint32_t s0 = atomicAdd(&B.counts[C_SEGS], ns);
+        for (uint32_t q = 0; q < ns; q++) {
+          Segment sg;
+          sg.node = small;
+          sg.begin = sbegin + q * P.seg_tile;
+          sg.count = min(P.seg_tile, scount - q * P.seg_tile);
+          sg.tree = t;
+          B.segs[s0 + q] = sg;
+        }
+      }
+      const uint32_t qd = atomicAdd(&B.counts[C_SQUADS], 1u);
+      B.squads[4 * qd + 0] = 0;
+      B.squads[4 * qd + 1] = 0;
+      B.squads[4 * qd + 2] = large;
+      B.squads[4 * qd + 3] = small;
+    }
+    const uint32_t lp = parent - t * P.MN, rp = right - t * P.MN;
+    if (P.growth == 0) {
+      const uint64_t nb = (uint64_t)t * 2 * P.W + 2 * e;
+      B.next_node[nb] = lp;
+      B.next_tnode[nb] = lt;
+      B.next_depth[nb] = cdepth;
+      B.next_node[nb + 1] = rp;
+      B.next_tnode[nb + 1] = rt;
+      B.next_depth[nb + 1] = cdepth;
+      atomicAdd(&B.ctl[t].n_next, 2u);
+    } else {
+      const uint32_t pos = atomicAdd(&B.ctl[t].n_open, 2u);
+      const uint64_t ob = (uint64_t)t * P.WO + pos;
+      B.open_node[ob] = lp;
+      B.open_tnode[ob] = lt;
+      B.open_depth[ob] = cdepth;
+      B.open_node[ob + 1] = rp;
+      B.open_tnode[ob + 1] = rt;
+      B.open_depth[ob + 1] = cdepth;
+    }
+  }
+}
+
+// Depth-wise: the next level becomes the open level.
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_advance(CtlParams P, DeviceBufs B) {
+  for (uint32_t t = 0; t < P.T; t++) {
+    const uint32_t nn = B.ctl[t].n_next;
+    for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) {
+      const uint64_t s = (uint64_t)t * 2 * P.W + i, d = (uint64_t)t * P.WO + i;
+      B.open_node[d] = B.next_node[s];
+      B.open_tnode[d] = B.next_tnode[s];
+      B.open_depth[d] = B.next_depth[s];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      TreeCtl& c = B.ctl[t];
+      c.n_open = nn;
+      c.n_next = 0;
+      c.depth += 1;
+    }
+    __syncthreads();
+  }
+}
+
+// Remaining candidates become leaves; learning rate; leaf-range update list.
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_finalize(CtlParams P, DeviceBufs B) {
+  if (threadIdx.x == 0) B.counts[C_LSEGS] = 0;
+  __syncthreads();
+  for (uint32_t t = 0; t < P.T; t++) {
+    const uint32_t no = B.ctl[t].n_open;
+    for (uint32_t i = threadIdx.x; i < no; i += blockDim.x) {
+      const uint64_t ob = (uint64_t)t * P.WO + i;
+      make_leaf(P, B, t, B.open_node[ob], B.open_tnode[ob]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) B.ctl[t].n_open = 0;
+    const uint32_t ntn = B.ctl[t].n_tnodes;
+    for (uint32_t i = threadIdx.x; i < ntn; i += blockDim.x) {
+      const uint64_t k = (uint64_t)t * P.MNT + i;
+      if (B.tr_isleaf[k]) B.tr_leaf[k] = B.tr_leaf[k] * P.lr;
+    }
+    const uint32_t np = B.ctl[t].n_part;
+    for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
+      const uint32_t gid = t * P.MN + p;
+      if (isnan(B.leaf_val[gid])) continue;
+      const uint32_t cnt = B.node_count[gid], b = B.pbegin[gid];
+      const uint32_t ns = (cnt + P.seg_tile - 1) / P.seg_tile;
+      if (ns == 0) continue;
+      const uint32_t s0 = atomicAdd(&B.counts[C_LSEGS], ns);
+      for (uint32_t q = 0; q < ns; q++) {
+        Segment sg;
+        sg.node = gid;
+        sg.begin = b + q * P.seg_tile;
+        sg.count = min(P.seg_tile, cnt - q * P.seg_tile);
+        sg.tree = t;
+        B.leaf_segs[s0 + q] = sg;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void launch_ctl_init(hipStream_t s, const CtlParams& p, const DeviceBufs& b,
+                     const uint32_t* rows_per_tree) {
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(kCtlThreads), 0, s, p, b, rows_per_tree);
+}
+void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
+  hipLaunchKernelGGL(k_ctl_plan, dim3(1), dim3(kCtlThreads), 0, s, p, b);
+}
+void launch_ctl_children(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
+  hipLaunchKernelGGL(k_ctl_children, dim3(1), dim3(kCtlThreads), 0, s, p, b);
+}
+void launch_ctl_advance(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
+  hipLaunchKernelGGL(k_ctl_advance, dim3(1), dim3(kCtlThreads), 0, s, p, b);
+}
+void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
+  hipLaunchKernelGGL(k_ctl_finalize, dim3(1), dim3(kCtlThreads), 0, s, p, b);
+}
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, int64_t n) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+}
+
+}  // namespace bst

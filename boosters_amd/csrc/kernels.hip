@@ -200,71 +200,106 @@ void launch_to_rowmajor(hipStream_t s, const DatasetDev& d) {
 // ===========================================================================
 // Objectives
 // ===========================================================================
-__device__ __forceinline__ void wave_atomic_max_abs(uint32_t* dst, float v) {
-  uint32_t u = __float_as_uint(fabsf(v));
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    uint32_t o = __shfl_xor(u, m, WAVE);
-    u = o > u ? o : u;
-  }
-  if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(dst, u);
-}
+constexpr int kGradMaxK = 16;   // outputs handled by one gradient launch
+constexpr int kGradThreads = 256;
+
+__device__ __forceinline__ uint32_t abs_bits(float v) { return __float_as_uint(fabsf(v)); }
 
 // grad/hess [K][n]; absmax[2k] = max|g|, absmax[2k+1] = max|h| (as f32 bits).
-__global__ __launch_bounds__(256) void k_gradients(int objective, const float* __restrict__ preds,
-                                                   const float* __restrict__ y,
-                                                   const float* __restrict__ w, int64_t n, int K,
-                                                   float* __restrict__ g, float* __restrict__ h,
-                                                   uint32_t* __restrict__ absmax) {
+// Grid-stride over rows with per-thread running maxima; one atomicMax per
+// block and value, so the maxima cost no global contention.
+__global__ __launch_bounds__(kGradThreads) void k_gradients(
+    int objective, const float* __restrict__ preds, const float* __restrict__ y,
+    const float* __restrict__ w, int64_t n, int K, float* __restrict__ g, float* __restrict__ h,
+    uint32_t* __restrict__ absmax) {
   const float HESS_MIN = 1e-6f;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = i < n;
-  const float wi = (in && w) ? w[i] : 1.0f;
-  const float yi = in ? y[i] : 0.0f;
-  if (objective == 2) {  // softmax, classification.rs:141-187
-    float mx = -INFINITY;
-    for (int k = 0; k < K; k++) mx = fmaxf(mx, in ? preds[(int64_t)k * n + i] : 0.0f);
-    float s = 0.0f;
-    for (int k = 0; k < K; k++) s += exp32((in ? preds[(int64_t)k * n + i] : 0.0f) - mx);
-    const int label = (int)yi;
-    for (int k = 0; k < K; k++) {
-      float gv = 0.0f, hv = 0.0f;
-      if (in) {
-        const float p = exp32(preds[(int64_t)k * n + i] - mx) / s;
-        const float ind = k == label ? 1.0f : 0.0f;
-        gv = wi * (p - ind);
-        hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
-        g[(int64_t)k * n + i] = gv;
-        h[(int64_t)k * n + i] = hv;
-      }
-      wave_atomic_max_abs(&absmax[2 * k], gv);
-      wave_atomic_max_abs(&absmax[2 * k + 1], hv);
-    }
-    return;
+  __shared__ uint32_t s_max[2 * kGradMaxK][kGradThreads / WAVE];
+  uint32_t mx_g[kGradMaxK], mx_h[kGradMaxK];
+#pragma unroll
+  for (int k = 0; k < kGradMaxK; k++) {
+    mx_g[k] = 0u;
+    mx_h[k] = 0u;
   }
-  for (int k = 0; k < K; k++) {
-    float gv = 0.0f, hv = 0.0f;
-    if (in) {
-      const float p0 = preds[(int64_t)k * n + i];
-      if (objective == 1) {  // logistic, classification.rs:24-52
-        const float p = 1.0f / (1.0f + exp32(-p0));
-        gv = wi * (p - yi);
-        hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
-      } else {  // squared, regression.rs:20-42
-        gv = wi * (p0 - yi);
-        hv = wi;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float wi = w ? w[i] : 1.0f;
+    const float yi = y[i];
+    if (objective == 2) {  // softmax, classification.rs:141-187
+      float mx = -INFINITY;
+#pragma unroll
+      for (int k = 0; k < kGradMaxK; k++)
+        if (k < K) mx = fmaxf(mx, preds[(int64_t)k * n + i]);
+      float s = 0.0f;
+#pragma unroll
+      for (int k = 0; k < kGradMaxK; k++)
+        if (k < K) s += exp32(preds[(int64_t)k * n + i] - mx);
+      const int label = (int)yi;
+#pragma unroll
+      for (int k = 0; k < kGradMaxK; k++) {
+        if (k < K) {
+          const float p = exp32(preds[(int64_t)k * n + i] - mx) / s;
+          const float ind = k == label ? 1.0f : 0.0f;
+          const float gv = wi * (p - ind);
+          const float hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
+          g[(int64_t)k * n + i] = gv;
+          h[(int64_t)k * n + i] = hv;
+          mx_g[k] = max(mx_g[k], abs_bits(gv));
+          mx_h[k] = max(mx_h[k], abs_bits(hv));
+        }
       }
-      g[(int64_t)k * n + i] = gv;
-      h[(int64_t)k * n + i] = hv;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kGradMaxK; k++) {
+        if (k < K) {
+          const float p0 = preds[(int64_t)k * n + i];
+          float gv, hv;
+          if (objective == 1) {  // logistic, classification.rs:24-52
+            const float p = 1.0f / (1.0f + exp32(-p0));
+            gv = wi * (p - yi);
+            hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
+          } else {  // squared, regression.rs:20-42
+            gv = wi * (p0 - yi);
+            hv = wi;
+          }
+          g[(int64_t)k * n + i] = gv;
+          h[(int64_t)k * n + i] = hv;
+          mx_g[k] = max(mx_g[k], abs_bits(gv));
+          mx_h[k] = max(mx_h[k], abs_bits(hv));
+        }
+      }
     }
-    wave_atomic_max_abs(&absmax[2 * k], gv);
-    wave_atomic_max_abs(&absmax[2 * k + 1], hv);
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+#pragma unroll
+  for (int k = 0; k < kGradMaxK; k++) {
+    if (k < K) {
+      uint32_t a = mx_g[k], b = mx_h[k];
+#pragma unroll
+      for (int m = WAVE / 2; m >= 1; m >>= 1) {
+        a = max(a, (uint32_t)__shfl_xor(a, m, WAVE));
+        b = max(b, (uint32_t)__shfl_xor(b, m, WAVE));
+      }
+      if (lane == 0) {
+        s_max[2 * k][wv] = a;
+        s_max[2 * k + 1][wv] = b;
+      }
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < 2 * K) {
+    uint32_t v = 0u;
+    for (int i = 0; i < kGradThreads / WAVE; i++) v = max(v, s_max[threadIdx.x][i]);
+    atomicMax(&absmax[threadIdx.x], v);
   }
 }
 
 void launch_gradients(hipStream_t s, int objective, const float* preds, const float* y,
                       const float* w, int64_t n, int K, float* g, float* h, uint32_t* absmax) {
-  hipLaunchKernelGGL(k_gradients, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, objective,
+  // Callers validate 1 <= K <= kGradMaxK (softmax allows at most 16 classes).
+  if (n <= 0 || K < 1 || K > kGradMaxK) return;
+  int64_t blocks = (n + kGradThreads - 1) / kGradThreads;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_gradients, dim3((unsigned)blocks), dim3(kGradThreads), 0, s, objective,
                      preds, y, w, n, K, g, h, absmax);
 }
 
@@ -313,134 +348,176 @@ void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g,
 // ===========================================================================
 // Histograms
 // ===========================================================================
-// One block = one HistItem: a tile of a node's rows x one group of FG
-// features.  Lane (threadIdx % FG) owns one feature; threadIdx / FG picks the
-// row, so the FG lanes of a row read FG consecutive bytes of that row.  The
-// LDS sub-histogram is laid out [bin][FG]: lanes of one row always hit FG
-// distinct 8-byte words, so ds_add_u64 is bank-conflict free whatever the
-// bins are, and hot bins (many rows in one bin) never collide inside a row.
+// One block-iteration = one HistItem: a tile of a node's rows x one group of
+// FG features.  Per chunk of kHistChunk rows the block first stages, one row
+// per thread, the row id and the row's fixed-point (g, h) pair in LDS (each
+// row is loaded and quantised once, not once per feature lane); then lane
+// (threadIdx % FG) owns one feature and threadIdx / FG picks the row, so the
+// FG lanes of a row read FG consecutive bytes of that row.  Each thread issues
+// its bin loads for kHistU rows before consuming any, so many loads are in
+// flight per wave.  The LDS sub-histogram is laid out [bin][FG]: lanes of one
+// row always hit FG distinct 8-byte words, so ds_add_u64 is bank-conflict
+// free whatever the bins are, and hot bins never collide inside a row.
 // Global layout per slot: [group][bin][FG] of (g, h) int64 pairs, which the
 // flush writes with unit stride.
+//
+// XCD-aware item order: work-groups are dispatched round-robin over the 8
+// XCDs (block b runs on XCD b % 8), each with its own L2.  Position p of the
+// grid-stride walk maps to tile (p / (8 NG)) * 8 + p % 8 and group
+// (p / 8) % NG, so the NG group blocks of one tile (which read disjoint
+// 32-byte pieces of the same 128-byte rows) run on one XCD and share its L2
+// instead of each fetching every row from HBM.  The grid is a multiple of 8.
+constexpr int kHistThreads = 1024;
+constexpr int kHistChunk = 1024;  // rows staged per chunk (one per thread)
+constexpr int kHistU = 8;         // rows whose bin loads are issued together
+
 template <int FG, bool GATHER, bool HAS_HI>
-__global__ __launch_bounds__(1024) void k_hist(const HistItem* __restrict__ items,
-                                               const uint32_t* __restrict__ idx,
-                                               const float2* __restrict__ ogh,
-                                               int64_t tree_stride,
-                                               const uint8_t* __restrict__ bins_rm, int S,
-                                               int hi_off, int E, const int32_t* __restrict__ qexp,
-                                               long long* __restrict__ hist, int64_t slot_elems,
-                                               long long* __restrict__ node_sum) {
+__global__ __launch_bounds__(kHistThreads) void k_hist(
+    const HistItem* __restrict__ items, uint32_t n_items_static, int NG,
+    const uint32_t* __restrict__ idx, const float2* __restrict__ ogh, int64_t tree_stride,
+    const uint8_t* __restrict__ bins_rm, int S, int hi_off, int E,
+    const int32_t* __restrict__ qexp, long long* __restrict__ hist, int64_t slot_elems,
+    long long* __restrict__ node_sum, const uint32_t* __restrict__ dyn) {
   __shared__ long long s_g[256 * FG];
   __shared__ long long s_h[256 * FG];
-  __shared__ long long s_red[2][1024 / WAVE];
-  const HistItem it = items[blockIdx.x];
-  const bool direct = (it.tree & 0x80000000u) != 0;  // this tile covers the whole node
-  const uint32_t tree = it.tree & 0x7FFFFFFFu;
-  for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
-    s_g[i] = 0;
-    s_h[i] = 0;
-  }
-  __syncthreads();
+  __shared__ uint32_t s_row[kHistChunk];
+  __shared__ longlong2 s_q[kHistChunk];
+  __shared__ long long s_red[2][kHistThreads / WAVE];
+  constexpr int RPP = kHistThreads / FG;         // rows per pass
+  constexpr int RPT = kHistChunk / RPP;          // rows per thread per chunk
+  static_assert(RPT % kHistU == 0, "chunk must split into load batches");
+  const uint32_t n_items = dyn ? *dyn : n_items_static;
+  const uint32_t n_tiles = n_items / (uint32_t)NG;
+  const uint32_t n_pos = ((n_tiles + 7u) / 8u) * 8u * (uint32_t)NG;
   const int lf = threadIdx.x % FG;
   const int rs = threadIdx.x / FG;
-  constexpr int RPP = 1024 / FG;
-  constexpr int U = 4;
-  const int f = (int)it.group * FG + lf;
-  const bool fvalid = f < E;
-  const double sg = ldexp(1.0, qexp[2 * tree]);
-  const double sh = ldexp(1.0, qexp[2 * tree + 1]);
-  const uint32_t* tidx = idx + (int64_t)tree * tree_stride;
-  const float2* togh = ogh + (int64_t)tree * tree_stride;
-  long long accg = 0, acch = 0;
-  const bool do_sum = it.group == 0 && lf == 0;
-  for (uint32_t p0 = rs; p0 < it.count; p0 += RPP * U) {
-    uint32_t rows[U];
-    bool ok[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint32_t p = p0 + u * RPP;
-      ok[u] = p < it.count;
-      const uint32_t pos = it.begin + (ok[u] ? p : 0);
-      rows[u] = GATHER ? tidx[pos] : pos;
+  for (uint32_t p = blockIdx.x; p < n_pos; p += gridDim.x) {
+    const uint32_t tile = (p / (8u * NG)) * 8u + (p & 7u);
+    const uint32_t grp = (p >> 3) % (uint32_t)NG;
+    if (tile >= n_tiles) continue;  // block-uniform
+    const HistItem it = items[tile * (uint32_t)NG + grp];
+    const bool direct = (it.tree & 0x80000000u) != 0;  // this tile covers the whole node
+    const uint32_t tree = it.tree & 0x7FFFFFFFu;
+    for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
+      s_g[i] = 0;
+      s_h[i] = 0;
     }
-    float2 gh[U];
-    uint32_t b[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint32_t pos = it.begin + (ok[u] ? p0 + u * RPP : 0);
-      gh[u] = togh[pos];
-      const uint8_t* rp = bins_rm + (int64_t)rows[u] * S;
-      b[u] = fvalid ? (uint32_t)rp[f] : 0u;
-      if (HAS_HI && fvalid) b[u] |= ((uint32_t)(rp[hi_off + (f >> 3)] >> (f & 7)) & 1u) << 8;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (!ok[u]) continue;
-      const long long gq = __double2ll_rn((double)gh[u].x * sg);
-      const long long hq = __double2ll_rn((double)gh[u].y * sh);
-      if (do_sum) {
+    const int f = (int)it.group * FG + lf;
+    const bool fvalid = f < E;
+    const int fs = fvalid ? f : 0;  // loads stay in the row for padding lanes
+    const double sg = ldexp(1.0, qexp[2 * tree]);
+    const double sh = ldexp(1.0, qexp[2 * tree + 1]);
+    const uint32_t* tidx = idx + (int64_t)tree * tree_stride;
+    const float2* togh = ogh + (int64_t)tree * tree_stride;
+    long long accg = 0, acch = 0;
+    for (uint32_t c0 = 0; c0 < it.count; c0 += kHistChunk) {
+      const uint32_t nrow = min((uint32_t)kHistChunk, it.count - c0);
+      // Stage: one row per thread.
+      if (threadIdx.x < nrow) {
+        const uint32_t pos = it.begin + c0 + threadIdx.x;
+        const uint32_t r = GATHER ? tidx[pos] : pos;
+        const float2 gh = togh[pos];
+        const long long gq = __double2ll_rn((double)gh.x * sg);
+        const long long hq = __double2ll_rn((double)gh.y * sh);
+        s_row[threadIdx.x] = r;
+        s_q[threadIdx.x] = make_longlong2(gq, hq);
         accg += gq;
         acch += hq;
       }
-      // bin 256 (NaN bin of a 257-bin feature) is not accumulated: it equals
-      // the node total minus the 256 real bins and is never read by the scan.
-      if (fvalid && b[u] < 256u) {
-        atomicAdd((unsigned long long*)&s_g[b[u] * FG + lf], (unsigned long long)gq);
-        atomicAdd((unsigned long long*)&s_h[b[u] * FG + lf], (unsigned long long)hq);
+      __syncthreads();  // staging visible; also orders the LDS zeroing
+#pragma unroll
+      for (int j0 = 0; j0 < RPT; j0 += kHistU) {
+        uint32_t b[kHistU];
+#pragma unroll
+        for (int u = 0; u < kHistU; u++) {
+          const uint32_t j = (uint32_t)rs + (uint32_t)(j0 + u) * RPP;
+          const uint32_t r = s_row[j < nrow ? j : 0u];
+          const uint8_t* rp = bins_rm + (int64_t)r * S;
+          uint32_t v = rp[fs];
+          if (HAS_HI) v |= ((uint32_t)(rp[hi_off + (fs >> 3)] >> (fs & 7)) & 1u) << 8;
+          b[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < kHistU; u++) {
+          const uint32_t j = (uint32_t)rs + (uint32_t)(j0 + u) * RPP;
+          // bin 256 (NaN bin of a 257-bin feature) is not accumulated: it
+          // equals the node total minus the 256 real bins and is never read.
+          if (j < nrow && fvalid && b[u] < 256u) {
+            const longlong2 q = s_q[j];
+            atomicAdd((unsigned long long*)&s_g[b[u] * FG + lf], (unsigned long long)q.x);
+            atomicAdd((unsigned long long*)&s_h[b[u] * FG + lf], (unsigned long long)q.y);
+          }
+        }
+      }
+      __syncthreads();  // staging is rewritten by the next chunk
+    }
+    if (it.group == 0) {
+      accg = wave_sum_ll(accg);
+      acch = wave_sum_ll(acch);
+      if ((threadIdx.x & (WAVE - 1)) == 0) {
+        s_red[0][threadIdx.x / WAVE] = accg;
+        s_red[1][threadIdx.x / WAVE] = acch;
       }
     }
-  }
-  if (it.group == 0) {
-    accg = wave_sum_ll(accg);
-    acch = wave_sum_ll(acch);
-    if ((threadIdx.x & (WAVE - 1)) == 0) {
-      s_red[0][threadIdx.x / WAVE] = accg;
-      s_red[1][threadIdx.x / WAVE] = acch;
+    __syncthreads();
+    if (it.group == 0 && threadIdx.x == 0) {
+      long long tg = 0, th = 0;
+      for (int wv = 0; wv < kHistThreads / WAVE; wv++) {
+        tg += s_red[0][wv];
+        th += s_red[1][wv];
+      }
+      if (direct) {
+        node_sum[2 * it.node] = tg;
+        node_sum[2 * it.node + 1] = th;
+      } else {
+        atomicAdd((unsigned long long*)&node_sum[2 * it.node], (unsigned long long)tg);
+        atomicAdd((unsigned long long*)&node_sum[2 * it.node + 1], (unsigned long long)th);
+      }
     }
-  }
-  __syncthreads();
-  if (it.group == 0 && threadIdx.x == 0) {
-    long long tg = 0, th = 0;
-    for (int wv = 0; wv < 1024 / WAVE; wv++) {
-      tg += s_red[0][wv];
-      th += s_red[1][wv];
-    }
+    long long* dst = hist + (int64_t)it.slot * slot_elems + (int64_t)it.group * 256 * FG * 2;
     if (direct) {
-      node_sum[2 * it.node] = tg;
-      node_sum[2 * it.node + 1] = th;
+      for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
+        longlong2 v;
+        v.x = s_g[i];
+        v.y = s_h[i];
+        ((longlong2*)dst)[i] = v;
+      }
     } else {
-      atomicAdd((unsigned long long*)&node_sum[2 * it.node], (unsigned long long)tg);
-      atomicAdd((unsigned long long*)&node_sum[2 * it.node + 1], (unsigned long long)th);
-    }
-  }
-  long long* dst = hist + (int64_t)it.slot * slot_elems + (int64_t)it.group * 256 * FG * 2;
-  if (direct) {
-    for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
-      longlong2 v;
-      v.x = s_g[i];
-      v.y = s_h[i];
-      ((longlong2*)dst)[i] = v;
-    }
-  } else {
-    for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
-      const long long gv = s_g[i], hv = s_h[i];
-      if (gv | hv) {
-        atomicAdd((unsigned long long*)&dst[2 * i], (unsigned long long)gv);
-        atomicAdd((unsigned long long*)&dst[2 * i + 1], (unsigned long long)hv);
+      for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
+        const long long gv = s_g[i], hv = s_h[i];
+        if (gv | hv) {
+          atomicAdd((unsigned long long*)&dst[2 * i], (unsigned long long)gv);
+          atomicAdd((unsigned long long*)&dst[2 * i + 1], (unsigned long long)hv);
+        }
       }
     }
+    __syncthreads();  // the LDS sub-histogram is reused by the next item
   }
 }
+
+// Grid for a histogram launch: a multiple of 8 (one slot per XCD), at most
+// kHistMaxGrid blocks; with a device-side count the caller's bound is used.
+constexpr int kHistMaxGrid = 2048;
 
 template <int FG>
 static void launch_hist_fg(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
                            const uint32_t* idx, const float2* ogh, int64_t tree_stride,
                            const int32_t* qexp, long long* hist, int64_t slot_elems,
-                           long long* node_sum, bool gather) {
+                           long long* node_sum, bool gather, const uint32_t* dyn) {
   const bool hi = d.hi_off >= 0;
-#define BST_HIST_LAUNCH(G, H)                                                                      \
-  hipLaunchKernelGGL((k_hist<FG, G, H>), dim3(n_items), dim3(1024), 0, s, items, idx, ogh,         \
-                     tree_stride, d.bins_rm, d.S, d.hi_off, d.E, qexp, hist, slot_elems, node_sum)
+  int64_t want;
+  if (dyn) {
+    want = n_items;  // capacity bound of the device-side item list
+  } else {
+    const int64_t n_tiles = n_items / d.NG;
+    want = (n_tiles + 7) / 8 * 8 * d.NG;
+  }
+  int64_t grid = want < kHistMaxGrid ? want : kHistMaxGrid;
+  grid = (grid + 7) / 8 * 8;
+#define BST_HIST_LAUNCH(G, H)                                                                   \
+  hipLaunchKernelGGL((k_hist<FG, G, H>), dim3((unsigned)grid), dim3(kHistThreads), 0, s, items, \
+                     (uint32_t)n_items, d.NG, idx, ogh, tree_stride, d.bins_rm, d.S, d.hi_off,  \
+                     d.E, qexp, hist, slot_elems, node_sum, dyn)
   if (gather) {
     if (hi)
       BST_HIST_LAUNCH(true, true);
@@ -458,14 +535,14 @@ static void launch_hist_fg(hipStream_t s, const DatasetDev& d, const HistItem* i
 void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
                  const uint32_t* idx, const float2* ogh, int64_t tree_stride, const int32_t* qexp,
                  long long* hist, int64_t slot_elems, long long* node_sum, bool gather,
-                 int /*rows_per_block_hint*/) {
+                 int /*rows_per_block_hint*/, const uint32_t* dyn) {
   if (n_items <= 0) return;
   if (d.FG == 16)
     launch_hist_fg<16>(s, d, items, n_items, idx, ogh, tree_stride, qexp, hist, slot_elems,
-                       node_sum, gather);
+                       node_sum, gather, dyn);
   else
     launch_hist_fg<32>(s, d, items, n_items, idx, ogh, tree_stride, qexp, hist, slot_elems,
-                       node_sum, gather);
+                       node_sum, gather, dyn);
 }
 
 // Exact node totals over row segments (used where no histogram is needed).
@@ -473,9 +550,12 @@ __global__ __launch_bounds__(256) void k_node_sums(const Segment* __restrict__ s
                                                    const float2* __restrict__ ogh,
                                                    int64_t tree_stride,
                                                    const int32_t* __restrict__ qexp,
-                                                   long long* __restrict__ node_sum) {
+                                                   long long* __restrict__ node_sum,
+                                                   const uint32_t* __restrict__ dyn) {
   __shared__ long long s_red[2][256 / WAVE];
-  const Segment sg = segs[blockIdx.x];
+  const uint32_t n_work = dyn ? *dyn : gridDim.x;
+  for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
+  const Segment sg = segs[wi];
   const double a = ldexp(1.0, qexp[2 * sg.tree]), b = ldexp(1.0, qexp[2 * sg.tree + 1]);
   const float2* t = ogh + (int64_t)sg.tree * tree_stride;
   long long g = 0, h = 0;
@@ -500,13 +580,16 @@ __global__ __launch_bounds__(256) void k_node_sums(const Segment* __restrict__ s
     atomicAdd((unsigned long long*)&node_sum[2 * sg.node], (unsigned long long)tg);
     atomicAdd((unsigned long long*)&node_sum[2 * sg.node + 1], (unsigned long long)th);
   }
+  __syncthreads();
+  }
 }
 
 void launch_node_sums(hipStream_t s, const Segment* segs, int n_segs, const float2* ogh,
-                      int64_t tree_stride, const int32_t* qexp, long long* node_sum) {
+                      int64_t tree_stride, const int32_t* qexp, long long* node_sum,
+                      const uint32_t* dyn) {
   if (n_segs <= 0) return;
   hipLaunchKernelGGL(k_node_sums, dim3(n_segs), dim3(256), 0, s, segs, ogh, tree_stride, qexp,
-                     node_sum);
+                     node_sum, dyn);
 }
 
 // Subtraction trick: for each (large_slot, small_slot, large_node, small_node)
@@ -514,49 +597,58 @@ void launch_node_sums(hipStream_t s, const Segment* segs, int n_segs, const floa
 __global__ __launch_bounds__(256) void k_subtract(const uint32_t* __restrict__ quads,
                                                   long long* __restrict__ hist,
                                                   int64_t slot_elems,
-                                                  long long* __restrict__ node_sum) {
-  const uint32_t* q = quads + 4 * blockIdx.y;
-  longlong2* dst = (longlong2*)(hist + (int64_t)q[0] * slot_elems);
-  const longlong2* src = (const longlong2*)(hist + (int64_t)q[1] * slot_elems);
-  const int64_t m = slot_elems / 2;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    longlong2 a = dst[i];
-    const longlong2 b = src[i];
-    a.x -= b.x;
-    a.y -= b.y;
-    dst[i] = a;
+                                                  long long* __restrict__ node_sum,
+                                                  const uint32_t* __restrict__ dyn) {
+  const uint32_t n_work = dyn ? *dyn : gridDim.y;
+  for (uint32_t wi = blockIdx.y; wi < n_work; wi += gridDim.y) {
+    const uint32_t* q = quads + 4 * wi;
+    longlong2* dst = (longlong2*)(hist + (int64_t)q[0] * slot_elems);
+    const longlong2* src = (const longlong2*)(hist + (int64_t)q[1] * slot_elems);
+    const int64_t m = slot_elems / 2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      longlong2 a = dst[i];
+      const longlong2 b = src[i];
+      a.x -= b.x;
+      a.y -= b.y;
+      dst[i] = a;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 2)
+      node_sum[2 * q[2] + threadIdx.x] -= node_sum[2 * q[3] + threadIdx.x];
   }
-  if (blockIdx.x == 0 && threadIdx.x < 2) node_sum[2 * q[2] + threadIdx.x] -= node_sum[2 * q[3] + threadIdx.x];
 }
 
 void launch_subtract(hipStream_t s, const uint32_t* quads, int n, long long* hist,
-                     int64_t slot_elems, long long* node_sum) {
+                     int64_t slot_elems, long long* node_sum, const uint32_t* dyn) {
   if (n <= 0) return;
   int64_t bx = (slot_elems / 2 + 255) / 256;
   if (bx > 64) bx = 64;
   if (bx < 1) bx = 1;
   hipLaunchKernelGGL(k_subtract, dim3((unsigned)bx, n), dim3(256), 0, s, quads, hist, slot_elems,
-                     node_sum);
+                     node_sum, dyn);
 }
 
 __global__ __launch_bounds__(256) void k_zero_slots(const uint32_t* __restrict__ slots,
                                                     long long* __restrict__ hist,
-                                                    int64_t slot_elems) {
-  longlong2* dst = (longlong2*)(hist + (int64_t)slots[blockIdx.y] * slot_elems);
-  const int64_t m = slot_elems / 2;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = make_longlong2(0, 0);
+                                                    int64_t slot_elems,
+                                                    const uint32_t* __restrict__ dyn) {
+  const uint32_t n_work = dyn ? *dyn : gridDim.y;
+  for (uint32_t wi = blockIdx.y; wi < n_work; wi += gridDim.y) {
+    longlong2* dst = (longlong2*)(hist + (int64_t)slots[wi] * slot_elems);
+    const int64_t m = slot_elems / 2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+      dst[i] = make_longlong2(0, 0);
+  }
 }
 
 void launch_zero_slots(hipStream_t s, const uint32_t* slots, int n, long long* hist,
-                       int64_t slot_elems) {
+                       int64_t slot_elems, const uint32_t* dyn) {
   if (n <= 0) return;
   int64_t bx = (slot_elems / 2 + 255) / 256;
   if (bx > 64) bx = 64;
   hipLaunchKernelGGL(k_zero_slots, dim3((unsigned)bx, n), dim3(256), 0, s, slots, hist,
-                     slot_elems);
+                     slot_elems, dyn);
 }
 
 // ===========================================================================
@@ -596,15 +688,18 @@ __global__ __launch_bounds__(256) void k_split(
     int NG, int E, const long long* __restrict__ hist, int64_t slot_elems,
     const long long* __restrict__ node_sum, const uint32_t* __restrict__ node_count,
     const int32_t* __restrict__ qexp, uint32_t max_nodes, SplitParams p,
-    unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand) {
+    unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand,
+    const uint32_t* __restrict__ dyn) {
   __shared__ double s_ratio[4][256];
   __shared__ uint16_t s_order[4][256];
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
   const int fi = blockIdx.x * 4 + wv;
-  const uint32_t node = nodes[blockIdx.y];
-  if (fi >= n_feats) return;
+  if (fi >= n_feats) return;  // whole wave; the kernel has no block barrier
+  const uint32_t n_work = dyn ? *dyn : gridDim.y;
+  for (uint32_t wy = blockIdx.y; wy < n_work; wy += gridDim.y) {
+  const uint32_t node = nodes[wy];
   const uint32_t f = feats[fi];
-  const uint32_t slot = slots[blockIdx.y];
+  const uint32_t slot = slots[wy];
   const uint32_t tree = node / max_nodes;
   const double ig = ldexp(1.0, -qexp[2 * tree]), ih = ldexp(1.0, -qexp[2 * tree + 1]);
   const long long G = node_sum[2 * node], H = node_sum[2 * node + 1];
@@ -823,14 +918,16 @@ __global__ __launch_bounds__(256) void k_split(
       atomicMax(&node_key[node], ((unsigned long long)ord_f32(out.gain) << 32) | tb);
     }
   }
+  }
 }
 
 __global__ void k_split_finalize(const uint32_t* __restrict__ nodes, int n_nodes, int E,
                                  int rightmost, unsigned long long* __restrict__ node_key,
                                  const DevSplit* __restrict__ cand,
-                                 DevSplit* __restrict__ node_split) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_nodes) return;
+                                 DevSplit* __restrict__ node_split,
+                                 const uint32_t* __restrict__ dyn) {
+  const int n_work = dyn ? (int)*dyn : n_nodes;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_work; i += gridDim.x * blockDim.x) {
   const uint32_t node = nodes[i];
   const unsigned long long k = node_key[node];
   node_key[node] = 0;
@@ -847,20 +944,22 @@ __global__ void k_split_finalize(const uint32_t* __restrict__ nodes, int n_nodes
     s = cand[(int64_t)node * E + f];
   }
   node_split[node] = s;
+  }
 }
 
 void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, const uint32_t* slots,
                   int n_nodes, const uint32_t* feats, int n_feats, const long long* hist,
                   int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
                   const int32_t* qexp, uint32_t max_nodes, SplitParams p,
-                  unsigned long long* node_key, DevSplit* cand, DevSplit* node_split) {
+                  unsigned long long* node_key, DevSplit* cand, DevSplit* node_split,
+                  const uint32_t* dyn) {
   if (n_nodes <= 0) return;
   dim3 grid((unsigned)((n_feats + 3) / 4), (unsigned)n_nodes);
   hipLaunchKernelGGL(k_split, grid, dim3(256), 0, s, nodes, slots, feats, n_feats, d.meta, d.FG,
                      d.NG, d.E, hist, slot_elems, node_sum, node_count, qexp, max_nodes, p,
-                     node_key, cand);
+                     node_key, cand, dyn);
   hipLaunchKernelGGL(k_split_finalize, dim3((n_nodes + 63) / 64), dim3(64), 0, s, nodes, n_nodes,
-                     d.E, p.rightmost, node_key, cand, node_split);
+                     d.E, p.rightmost, node_key, cand, node_split, dyn);
 }
 
 // ===========================================================================
@@ -880,38 +979,43 @@ __global__ __launch_bounds__(kPartThreads) void k_part_count(
     const DevSplit* __restrict__ node_split, const FeatMeta* __restrict__ fmeta,
     const uint8_t* __restrict__ bins_cm, const uint32_t* __restrict__ hi_cm, int64_t n,
     int64_t hi_words, const uint32_t* __restrict__ idx, uint8_t* __restrict__ flag,
-    uint32_t* __restrict__ tile_left) {
+    uint32_t* __restrict__ tile_left, const uint32_t* __restrict__ dyn_tiles) {
   __shared__ uint32_t s_cnt[kPartThreads / WAVE];
-  const PartTile t = tiles[blockIdx.x];
-  const PartItem it = items[t.item];
-  const DevSplit sp = node_split[it.node];
-  const FeatMeta fm = fmeta[it.feature];
-  const uint8_t* col = bins_cm + (int64_t)it.feature * n;
-  const uint32_t* hcol = hi_cm ? hi_cm + (int64_t)it.feature * hi_words : nullptr;
-  uint32_t c = 0;
-  for (uint32_t p = threadIdx.x; p < t.count; p += kPartThreads) {
-    const uint32_t pos = t.begin + p;
-    const uint32_t row = idx[pos];
-    uint32_t bin = col[row];
-    if (hcol) bin |= ((hcol[row >> 5] >> (row & 31)) & 1u) << 8;
-    const bool l = goes_left(sp, fm, bin);
-    flag[pos] = l ? 1 : 0;
-    c += l ? 1u : 0u;
-  }
-  for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, WAVE);
-  if ((threadIdx.x & (WAVE - 1)) == 0) s_cnt[threadIdx.x / WAVE] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t s = 0;
-    for (int i = 0; i < kPartThreads / WAVE; i++) s += s_cnt[i];
-    tile_left[blockIdx.x] = s;
+  const uint32_t n_work = dyn_tiles ? *dyn_tiles : gridDim.x;
+  for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
+    const PartTile t = tiles[wi];
+    const PartItem it = items[t.item];
+    const DevSplit sp = node_split[it.node];
+    const FeatMeta fm = fmeta[it.feature];
+    const uint8_t* col = bins_cm + (int64_t)it.feature * n;
+    const uint32_t* hcol = hi_cm ? hi_cm + (int64_t)it.feature * hi_words : nullptr;
+    uint32_t c = 0;
+    for (uint32_t p = threadIdx.x; p < t.count; p += kPartThreads) {
+      const uint32_t pos = t.begin + p;
+      const uint32_t row = idx[pos];
+      uint32_t bin = col[row];
+      if (hcol) bin |= ((hcol[row >> 5] >> (row & 31)) & 1u) << 8;
+      const bool l = goes_left(sp, fm, bin);
+      flag[pos] = l ? 1 : 0;
+      c += l ? 1u : 0u;
+    }
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) s_cnt[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t s = 0;
+      for (int i = 0; i < kPartThreads / WAVE; i++) s += s_cnt[i];
+      tile_left[wi] = s;
+    }
+    __syncthreads();
   }
 }
 
 // Pass 2: exclusive scan of left counts over the tiles of each node.
 __global__ void k_part_scan(const PartItem* __restrict__ items, int n_items,
                             const uint32_t* __restrict__ tile_left, uint32_t* __restrict__ tile_off,
-                            uint32_t* __restrict__ item_left) {
+                            uint32_t* __restrict__ item_left, const uint32_t* __restrict__ dyn_items) {
+  if (dyn_items) n_items = (int)*dyn_items;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += gridDim.x * blockDim.x) {
     const PartItem it = items[i];
     uint32_t s = 0;
@@ -930,51 +1034,58 @@ __global__ __launch_bounds__(kPartThreads) void k_part_scatter(
     const uint32_t* __restrict__ tile_off, const uint32_t* __restrict__ item_left,
     const uint8_t* __restrict__ flag, const uint32_t* __restrict__ idx,
     const float2* __restrict__ ogh, uint32_t* __restrict__ idx_tmp,
-    float2* __restrict__ ogh_tmp) {
+    float2* __restrict__ ogh_tmp, const uint32_t* __restrict__ dyn_tiles) {
   __shared__ uint32_t s_wave[kPartThreads / WAVE];
   __shared__ uint32_t s_carry;
-  const PartTile t = tiles[blockIdx.x];
-  const PartItem it = items[t.item];
-  const uint32_t L = item_left[t.item];
-  const uint32_t lbefore = tile_off[blockIdx.x];
-  const uint32_t rbefore = (t.begin - it.begin) - lbefore;
   const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (uint32_t p0 = 0; p0 < t.count; p0 += kPartThreads) {
-    const uint32_t p = p0 + threadIdx.x;
-    const bool in = p < t.count;
-    const bool l = in && flag[t.begin + p];
-    const unsigned long long bal = __ballot(l);
-    const uint32_t below = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) s_wave[wv] = __popcll(bal);
+  const uint32_t n_work = dyn_tiles ? *dyn_tiles : gridDim.x;
+  for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
+    const PartTile t = tiles[wi];
+    const PartItem it = items[t.item];
+    const uint32_t L = item_left[t.item];
+    const uint32_t lbefore = tile_off[wi];
+    const uint32_t rbefore = (t.begin - it.begin) - lbefore;
+    if (threadIdx.x == 0) s_carry = 0;
     __syncthreads();
-    uint32_t woff = s_carry;
-    for (int k = 0; k < wv; k++) woff += s_wave[k];
-    if (in) {
-      const uint32_t lrank = woff + below;          // lefts before p in this tile
-      const uint32_t rrank = p - lrank;             // rights before p in this tile
-      const uint32_t dst = l ? it.begin + lbefore + lrank : it.begin + L + rbefore + rrank;
-      idx_tmp[dst] = idx[t.begin + p];
-      ogh_tmp[dst] = ogh[t.begin + p];
+    for (uint32_t p0 = 0; p0 < t.count; p0 += kPartThreads) {
+      const uint32_t p = p0 + threadIdx.x;
+      const bool in = p < t.count;
+      const bool l = in && flag[t.begin + p];
+      const unsigned long long bal = __ballot(l);
+      const uint32_t below = __popcll(bal & ((1ull << lane) - 1ull));
+      if (lane == 0) s_wave[wv] = __popcll(bal);
+      __syncthreads();
+      uint32_t woff = s_carry;
+      for (int k = 0; k < wv; k++) woff += s_wave[k];
+      if (in) {
+        const uint32_t lrank = woff + below;  // lefts before p in this tile
+        const uint32_t rrank = p - lrank;     // rights before p in this tile
+        const uint32_t dst = l ? it.begin + lbefore + lrank : it.begin + L + rbefore + rrank;
+        idx_tmp[dst] = idx[t.begin + p];
+        ogh_tmp[dst] = ogh[t.begin + p];
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t s = s_carry;
+        for (int k = 0; k < kPartThreads / WAVE; k++) s += s_wave[k];
+        s_carry = s;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t s = s_carry;
-      for (int k = 0; k < kPartThreads / WAVE; k++) s += s_wave[k];
-      s_carry = s;
-    }
-    __syncthreads();
   }
 }
 
 __global__ __launch_bounds__(kPartThreads) void k_part_copyback(
     const PartTile* __restrict__ tiles, const uint32_t* __restrict__ idx_tmp,
-    const float2* __restrict__ ogh_tmp, uint32_t* __restrict__ idx, float2* __restrict__ ogh) {
-  const PartTile t = tiles[blockIdx.x];
-  for (uint32_t p = threadIdx.x; p < t.count; p += kPartThreads) {
-    idx[t.begin + p] = idx_tmp[t.begin + p];
-    ogh[t.begin + p] = ogh_tmp[t.begin + p];
+    const float2* __restrict__ ogh_tmp, uint32_t* __restrict__ idx, float2* __restrict__ ogh,
+    const uint32_t* __restrict__ dyn_tiles) {
+  const uint32_t n_work = dyn_tiles ? *dyn_tiles : gridDim.x;
+  for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
+    const PartTile t = tiles[wi];
+    for (uint32_t p = threadIdx.x; p < t.count; p += kPartThreads) {
+      idx[t.begin + p] = idx_tmp[t.begin + p];
+      ogh[t.begin + p] = ogh_tmp[t.begin + p];
+    }
   }
 }
 
@@ -984,38 +1095,41 @@ __global__ __launch_bounds__(kPartThreads) void k_part_copyback(
 __global__ void k_part_children(const PartItem* __restrict__ items, int n_items,
                                 const uint32_t* __restrict__ item_left,
                                 uint32_t* __restrict__ node_count,
-                                long long* __restrict__ node_sum) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_items) return;
-  const PartItem it = items[i];
-  const uint32_t L = item_left[i], R = it.count - L;
-  node_count[it.node] = L;
-  node_count[it.right_node] = R;
-  const long long g = node_sum[2 * it.node], h = node_sum[2 * it.node + 1];
-  node_sum[2 * it.right_node] = g;
-  node_sum[2 * it.right_node + 1] = h;
-  const uint32_t small = L <= R ? it.node : it.right_node;
-  node_sum[2 * small] = 0;
-  node_sum[2 * small + 1] = 0;
+                                long long* __restrict__ node_sum,
+                                const uint32_t* __restrict__ dyn_items) {
+  if (dyn_items) n_items = (int)*dyn_items;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += gridDim.x * blockDim.x) {
+    const PartItem it = items[i];
+    const uint32_t L = item_left[i], R = it.count - L;
+    node_count[it.node] = L;
+    node_count[it.right_node] = R;
+    const long long g = node_sum[2 * it.node], h = node_sum[2 * it.node + 1];
+    node_sum[2 * it.right_node] = g;
+    node_sum[2 * it.right_node + 1] = h;
+    const uint32_t small = L <= R ? it.node : it.right_node;
+    node_sum[2 * small] = 0;
+    node_sum[2 * small + 1] = 0;
+  }
 }
 
 void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items, int n_items,
                       const PartTile* tiles, int n_tiles, const DevSplit* node_split,
                       uint32_t* idx, float2* ogh, uint32_t* idx_tmp, float2* ogh_tmp,
                       uint8_t* flag, uint32_t* tile_left, uint32_t* tile_off,
-                      uint32_t* item_left, uint32_t* node_count, long long* node_sum) {
+                      uint32_t* item_left, uint32_t* node_count, long long* node_sum,
+                      const uint32_t* dyn_items, const uint32_t* dyn_tiles) {
   if (n_items <= 0) return;
   hipLaunchKernelGGL(k_part_count, dim3(n_tiles), dim3(kPartThreads), 0, s, items, tiles,
                      node_split, d.meta, d.bins_cm, d.hi_cm, d.n, d.hi_words, idx, flag,
-                     tile_left);
+                     tile_left, dyn_tiles);
   hipLaunchKernelGGL(k_part_scan, dim3((n_items + 255) / 256), dim3(256), 0, s, items, n_items,
-                     tile_left, tile_off, item_left);
+                     tile_left, tile_off, item_left, dyn_items);
   hipLaunchKernelGGL(k_part_scatter, dim3(n_tiles), dim3(kPartThreads), 0, s, items, tiles,
-                     tile_off, item_left, flag, idx, ogh, idx_tmp, ogh_tmp);
+                     tile_off, item_left, flag, idx, ogh, idx_tmp, ogh_tmp, dyn_tiles);
   hipLaunchKernelGGL(k_part_copyback, dim3(n_tiles), dim3(kPartThreads), 0, s, tiles, idx_tmp,
-                     ogh_tmp, idx, ogh);
+                     ogh_tmp, idx, ogh, dyn_tiles);
   hipLaunchKernelGGL(k_part_children, dim3((n_items + 63) / 64), dim3(64), 0, s, items, n_items,
-                     item_left, node_count, node_sum);
+                     item_left, node_count, node_sum, dyn_items);
 }
 
 // ===========================================================================
@@ -1026,20 +1140,24 @@ __global__ __launch_bounds__(256) void k_leaf_update(const Segment* __restrict__
                                                      const uint32_t* __restrict__ idx,
                                                      int64_t tree_stride, float* __restrict__ preds,
                                                      const uint32_t* __restrict__ tree_output,
-                                                     int64_t n_rows) {
-  const Segment sg = segs[blockIdx.x];
-  const float v = leaf_val[sg.node];
-  const uint32_t* t = idx + (int64_t)sg.tree * tree_stride;
-  float* pr = preds + (int64_t)tree_output[sg.tree] * n_rows;
-  for (uint32_t p = threadIdx.x; p < sg.count; p += blockDim.x) pr[t[sg.begin + p]] += v;
+                                                     int64_t n_rows,
+                                                     const uint32_t* __restrict__ dyn) {
+  const uint32_t n_work = dyn ? *dyn : gridDim.x;
+  for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
+    const Segment sg = segs[wi];
+    const float v = leaf_val[sg.node];
+    const uint32_t* t = idx + (int64_t)sg.tree * tree_stride;
+    float* pr = preds + (int64_t)tree_output[sg.tree] * n_rows;
+    for (uint32_t p = threadIdx.x; p < sg.count; p += blockDim.x) pr[t[sg.begin + p]] += v;
+  }
 }
 
 void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const float* leaf_val,
                         const uint32_t* idx, int64_t tree_stride, float* preds,
-                        const uint32_t* tree_output, int64_t n_rows) {
+                        const uint32_t* tree_output, int64_t n_rows, const uint32_t* dyn) {
   if (n_segs <= 0) return;
   hipLaunchKernelGGL(k_leaf_update, dim3(n_segs), dim3(256), 0, s, segs, leaf_val, idx,
-                     tree_stride, preds, tree_output, n_rows);
+                     tree_stride, preds, tree_output, n_rows, dyn);
 }
 
 __device__ __forceinline__ float sigmoid32(float x) {  // transform.rs:106-115
@@ -1175,6 +1293,23 @@ void launch_predict_accum(hipStream_t s, const DevNode* nodes, const int64_t* tr
                        s, nodes, tree_off, tree_group, n_trees, cat_start, cat_size, cat_words,
                        nullptr, K, X, n, x_stride, F, 0, out, out_stride, nullptr);
   }
+}
+
+// Plain 16-byte streaming copy, the calibration point for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
+                                                uint4* __restrict__ dst, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy16(hipStream_t s, void* dst, const void* src, int64_t bytes) {
+  const int64_t n16 = bytes / 16;
+  int64_t blocks = (n16 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s, (const uint4*)src,
+                     (uint4*)dst, n16);
 }
 
 // Metric partial sums per block: (sum w*loss, sum w), f64.  RMSE gets one row

@@ -205,7 +205,9 @@ void Trainer::root(std::vector<TreeState>& trees) {
     counts[gnode] = m;
     nodes.push_back(gnode);
     slots.push_back(gslot);
-    gather = gather || ts.sampled;
+    // Without gather the kernel takes the position as the row id, which only
+    // holds for tree slot 0 (slot t's positions start at t * n).
+    gather = gather || ts.sampled || ts.t > 0;
   }
   upload(s_, d_hist_items_, items);
   upload(s_, d_u32a_, zero);
@@ -605,12 +607,92 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   const bool uniform = !(std::fabs(cfg.subsample - 1.0f) < 1e-6f);
   if (opt_.ext.goss_top_rate > 0.0f) raise(BST_E_CONFIG, "GOSS row sampling is not supported by this engine build");
 
+  // One grown tree as a single-tree forest (leaf values already carry the learning rate).
+  auto single_tree = [&](const TreeBuild& tb, int output) {
+    bst_model one;
+    one.info.n_trees = 1;
+    one.info.n_groups = K;
+    one.base_score.assign(K, 0.0f);
+    const size_t nn = tb.split_index.size();
+    one.tree_offset = {0, (int64_t)nn};
+    one.tree_group = {(uint32_t)output};
+    one.split_index = tb.split_index;
+    one.split_threshold = tb.thr;
+    one.left = tb.left;
+    one.right = tb.right;
+    one.default_left = tb.dleft;
+    one.is_leaf = tb.is_leaf;
+    one.split_type = tb.stype;
+    one.leaf_value.assign(nn, 0.0f);
+    for (size_t i = 0; i < nn; i++)
+      if (tb.is_leaf[i]) one.leaf_value[i] = tb.leaf[i];
+    one.gain = tb.gain;
+    one.cover = tb.cover;
+    one.cat_start.assign(nn, 0);
+    one.cat_size.assign(nn, 0);
+    auto cn = tb.cat_nodes;
+    std::sort(cn.begin(), cn.end(), [](auto& a, auto& b) { return a.first < b.first; });
+    for (auto& c : cn) {
+      one.cat_start[c.first] = (uint32_t)one.cat_words.size();
+      one.cat_size[c.first] = (uint32_t)c.second.size();
+      one.cat_words.insert(one.cat_words.end(), c.second.begin(), c.second.end());
+    }
+    one.info.total_nodes = (int64_t)nn;
+    one.info.total_cat_words = (int64_t)one.cat_words.size();
+    return one;
+  };
+  // Validation predictions += the tree (needs one.upload first).
+  auto valid_update = [&](bst_model& one) {
+    launch_predict_accum(s_, one.d_nodes.p, one.d_tree_off.p, one.d_tree_group.p, 1,
+                         one.d_cat_start.p, one.d_cat_size.p, one.d_cat_words.p, K, vX, n_valid,
+                         n_valid, ds_->d.F, vpreds_.p, n_valid);
+  };
+  auto append_tree = [&](bst_model* fm, const bst_model& one, int output) {
+    const size_t nn = one.split_index.size();
+    const int64_t off = fm->tree_offset.empty() ? 0 : fm->tree_offset.back();
+    if (fm->tree_offset.empty()) fm->tree_offset.push_back(0);
+    fm->tree_offset.push_back(off + (int64_t)nn);
+    fm->tree_group.push_back((uint32_t)output);
+    const uint32_t woff = (uint32_t)fm->cat_words.size();
+    for (size_t i = 0; i < nn; i++) {
+      fm->split_index.push_back(one.split_index[i]);
+      fm->split_threshold.push_back(one.split_threshold[i]);
+      fm->left.push_back(one.left[i]);
+      fm->right.push_back(one.right[i]);
+      fm->default_left.push_back(one.default_left[i]);
+      fm->is_leaf.push_back(one.is_leaf[i]);
+      fm->split_type.push_back(one.split_type[i]);
+      fm->leaf_value.push_back(one.leaf_value[i]);
+      fm->gain.push_back(one.gain[i]);
+      fm->cover.push_back(one.cover[i]);
+      fm->cat_start.push_back(one.cat_size[i] ? one.cat_start[i] + woff : 0);
+      fm->cat_size.push_back(one.cat_size[i]);
+    }
+    fm->cat_words.insert(fm->cat_words.end(), one.cat_words.begin(), one.cat_words.end());
+    stats.n_trees++;
+  };
+  const bool on_device = device_schedule_ok();
+  if (on_device) sp_.rightmost = (opt_.ext.n_threads != 1 && E >= 16) ? 1 : 0;
+
   std::vector<std::vector<float>> leafv;
   bool have_best = false;
   double best_val = 0;
   size_t best_round = 0, cur_round = 0, best_n_trees = 0;
   std::vector<uint32_t> sampled;
   for (uint32_t round = 0; round < cfg.n_trees; round++) {
+    if (on_device) {
+      std::vector<TreeBuild> tbs;
+      device_round(y, w, tbs);
+      for (int t = 0; t < T_; t++) {
+        bst_model one = single_tree(tbs[t], t);
+        if (vX && n_valid > 0) {
+          one.upload(s_);
+          valid_update(one);
+          BST_HIP(hipStreamSynchronize(s_));
+        }
+        append_tree(m, one, t);
+      }
+    } else {
     BST_HIP(hipMemsetAsync(absmax_.p, 0, absmax_.bytes(), s_));
     launch_gradients(s_, cfg.objective, preds_.p, y, w, n_, K, g_.p, h_.p, absmax_.p);
     launch_qscale(s_, absmax_.p, K, n_, qexp_.p);
@@ -673,36 +755,7 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
       // Prediction update and model append, trees in output order.
       h2d(s_, tree_output_.p, outs.data(), outs.size());
       for (auto& ts : trees) {
-        bst_model one;
-        one.info.n_trees = 1;
-        one.info.n_groups = K;
-        one.base_score.assign(K, 0.0f);
-        one.tree_offset = {0, (int64_t)ts.tb.split_index.size()};
-        one.tree_group = {(uint32_t)ts.output};
-        const size_t nn = ts.tb.split_index.size();
-        one.split_index = ts.tb.split_index;
-        one.split_threshold = ts.tb.thr;
-        one.left = ts.tb.left;
-        one.right = ts.tb.right;
-        one.default_left = ts.tb.dleft;
-        one.is_leaf = ts.tb.is_leaf;
-        one.split_type = ts.tb.stype;
-        one.leaf_value.assign(nn, 0.0f);
-        for (size_t i = 0; i < nn; i++)
-          if (ts.tb.is_leaf[i]) one.leaf_value[i] = ts.tb.leaf[i];
-        one.gain = ts.tb.gain;
-        one.cover = ts.tb.cover;
-        one.cat_start.assign(nn, 0);
-        one.cat_size.assign(nn, 0);
-        auto cn = ts.tb.cat_nodes;
-        std::sort(cn.begin(), cn.end(), [](auto& a, auto& b) { return a.first < b.first; });
-        for (auto& c : cn) {
-          one.cat_start[c.first] = (uint32_t)one.cat_words.size();
-          one.cat_size[c.first] = (uint32_t)c.second.size();
-          one.cat_words.insert(one.cat_words.end(), c.second.begin(), c.second.end());
-        }
-        one.info.total_nodes = (int64_t)nn;
-        one.info.total_cat_words = (int64_t)one.cat_words.size();
+        bst_model one = single_tree(ts.tb, ts.output);
         const bool need_dev = ts.sampled || (vX && n_valid > 0);
         if (need_dev) one.upload(s_);
         if (!ts.sampled) {
@@ -726,34 +779,11 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
                                one.d_cat_start.p, one.d_cat_size.p, one.d_cat_words.p, K,
                                ds_->d.raw, n_, n_, ds_->d.F, preds_.p, n_);
         }
-        if (vX && n_valid > 0)
-          launch_predict_accum(s_, one.d_nodes.p, one.d_tree_off.p, one.d_tree_group.p, 1,
-                               one.d_cat_start.p, one.d_cat_size.p, one.d_cat_words.p, K, vX,
-                               n_valid, n_valid, ds_->d.F, vpreds_.p, n_valid);
+        if (vX && n_valid > 0) valid_update(one);
         if (need_dev) BST_HIP(hipStreamSynchronize(s_));
-        // Append to the forest.
-        const int64_t off = m->tree_offset.empty() ? 0 : m->tree_offset.back();
-        if (m->tree_offset.empty()) m->tree_offset.push_back(0);
-        m->tree_offset.push_back(off + (int64_t)nn);
-        m->tree_group.push_back((uint32_t)ts.output);
-        const uint32_t woff = (uint32_t)m->cat_words.size();
-        for (size_t i = 0; i < nn; i++) {
-          m->split_index.push_back(one.split_index[i]);
-          m->split_threshold.push_back(one.split_threshold[i]);
-          m->left.push_back(one.left[i]);
-          m->right.push_back(one.right[i]);
-          m->default_left.push_back(one.default_left[i]);
-          m->is_leaf.push_back(one.is_leaf[i]);
-          m->split_type.push_back(one.split_type[i]);
-          m->leaf_value.push_back(one.leaf_value[i]);
-          m->gain.push_back(one.gain[i]);
-          m->cover.push_back(one.cover[i]);
-          m->cat_start.push_back(one.cat_size[i] ? one.cat_start[i] + woff : 0);
-          m->cat_size.push_back(one.cat_size[i]);
-        }
-        m->cat_words.insert(m->cat_words.end(), one.cat_words.begin(), one.cat_words.end());
-        stats.n_trees++;
+        append_tree(m, one, ts.output);
       }
+    }
     }
     // evaluate_round (eval.rs:218-260) and early stopping (callback.rs:78-118).
     double value;

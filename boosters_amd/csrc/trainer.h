@@ -1,5 +1,6 @@
 // trainer.h — host orchestration of histogram GBDT training on the device.
 #pragma once
+#include <memory>
 #include <queue>
 
 #include "host_common.h"
@@ -74,6 +75,16 @@ class Trainer {
   void finish_trees(std::vector<TreeState>& trees, bst_model* m);
   double eval_metric(const float* preds, const float* y, int64_t n);
   uint32_t alloc_slot(TreeState& ts);
+  // Graph schedule (trainer_device.cpp): the whole round on the device.
+  struct DevState;
+  struct DevStateDelete {
+    void operator()(DevState* p) const;
+  };
+  std::unique_ptr<DevState, DevStateDelete> dev_;
+  bool device_schedule_ok() const;
+  void device_setup();
+  void device_enqueue_round(const float* y, const float* w);
+  void device_round(const float* y, const float* w, std::vector<TreeBuild>& out);
   // Algorithmic bytes: each histogrammed row reads its E bins (1 B, or
   // 1.125 B with the NaN hi plane) plus its gradient pair (8 B) and row id
   // (4 B when gathered); each built node writes E x 256 (g, h) int64 pairs.

@@ -179,6 +179,10 @@ void* bst_ctx_stream(bst_ctx* ctx);
 int bst_ctx_set_profiling(bst_ctx* ctx, int32_t enable);
 int bst_ctx_kernel_stats(bst_ctx* ctx, int32_t kind, int64_t* launches, double* ms,
                          double* alg_bytes);
+/* A plain streaming copy kernel (dst = src, bytes multiple of 16) on the ctx
+ * stream: the reference point that scales HBM byte counters (no counterpart
+ * in the reference). */
+int bst_kernel_copy(bst_ctx* ctx, void* dst, const void* src, int64_t bytes);
 /* Device buffers for callers without their own allocator (plumbing). */
 int bst_device_alloc(bst_ctx* ctx, int64_t bytes, void** out);
 int bst_device_free(bst_ctx* ctx, void* ptr);

@@ -84,7 +84,9 @@ def test_gradients(ctx, oracle, obj, K):
     oracle.set_exp_mode(0)
     rg, rh = oracle.gradients(obj, preds, y, w)
     ulp = np.abs(g.view(np.int32).astype(np.int64) - rg.view(np.int32).astype(np.int64))
-    assert ulp.max() <= 2
+    # glibc expf is within 1 ulp of the correctly rounded exp; softmax combines
+    # K exps, a sum and a division, so the gap can reach a few ulp.
+    assert ulp.max() <= (2 if obj != 2 else 6)
 
 
 def test_histogram_vs_f64(ctx, oracle):
@@ -251,6 +253,62 @@ def test_schedules_identical(ctx, oracle, n_threads):
     gb = GBDTModel(h, ctx).export()
     for k in ga:
         np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+
+
+GRAPH_CASES = {
+    # name: (data builder, config kwargs)
+    "squared_depth6": (lambda: regression_data(5000, 100, seed=0) + (None,),
+                       dict(n_trees=8, learning_rate=0.1, max_depth=6)),
+    "logistic_nan_depth8": (lambda: binary_nan_data(30_000, 24, seed=1) + (None,),
+                            dict(objective=1, n_trees=6, learning_rate=0.1, max_depth=8)),
+    "squared_leafwise": (lambda: regression_data(20_000, 30, seed=5) + (None,),
+                         dict(n_trees=6, growth=1, max_leaves=40, learning_rate=0.2)),
+    "softmax_leafwise_cat": (lambda: multiclass_cat_data(12_000, 12, 3, 4, card=16, seed=4),
+                             dict(objective=2, n_classes=4, n_trees=4, learning_rate=0.1, growth=1,
+                                  max_leaves=31)),
+    "softmax_depthwise_cat": (lambda: multiclass_cat_data(9000, 8, 2, 3, card=40, seed=13),
+                              dict(objective=2, n_classes=3, n_trees=4, max_depth=5,
+                                   max_onehot_cats=4)),
+    "min_gain_stops_early": (lambda: regression_data(3000, 10, seed=9) + (None,),
+                             dict(n_trees=5, max_depth=7, min_gain=5.0, min_samples_leaf=40)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(GRAPH_CASES))
+def test_graph_schedule_matches_oracle(ctx, oracle, case):
+    """Device-side growth replayed as a hipGraph grows the oracle's forest."""
+    build, kw = GRAPH_CASES[case]
+    X, y, is_cat = build()
+    cfg = make_cfg(**kw)
+    gm, om, b = _train_both(ctx, oracle, X, y, cfg, is_cat, schedule=2)
+    ga = _forests_equal(gm, om.info, om.arrays)
+    # Same forest as the host-driven batched schedule, leaf values included.
+    h = C.c_void_p()
+    A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0, C.byref(cfg),
+                              C.byref(make_ext(1)), C.byref(h)))
+    gb = GBDTModel(h, ctx).export()
+    for k in ga:
+        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+
+
+def test_graph_schedule_training_predictions(ctx, oracle):
+    """The in-graph prediction update leaves the same per-round training loss."""
+    X, y = binary_nan_data(20_000, 16, seed=23)
+    cfg = make_cfg(objective=1, n_trees=7, learning_rate=0.3, max_depth=6)
+    b = _binned(ctx, X, None)
+    hist = {}
+    for sched in (1, 2):
+        h = C.c_void_p()
+        A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0, C.byref(cfg),
+                                  C.byref(make_ext(sched)), C.byref(h)))
+        GBDTModel(h, ctx)
+        buf = np.zeros(64, np.float64)
+        nr = C.c_int64()
+        A.check(A.lib().bst_last_train_metric(ctx.h, A.ptr(buf), len(buf), C.byref(nr)))
+        hist[sched] = buf[: nr.value].copy()
+    assert len(hist[2]) == 7
+    np.testing.assert_array_equal(hist[1], hist[2])
+    assert hist[2][-1] < hist[2][0]
 
 
 def test_predict_and_leaf_indices(ctx, oracle):
