@@ -282,6 +282,8 @@ int bst_ctx_destroy(bst_ctx* ctx) {
   return guard([&] {
     if (!ctx) return BST_OK;
     (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->staging) (void)hipHostFree(ctx->staging);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BST_OK;
@@ -499,7 +501,14 @@ static int train_impl(bst_ctx* ctx, bst_dataset* ds, const float* y_dev, int64_t
   bst_ext_config_default(&opt.ext);
   if (ext) opt.ext = *ext;
   opt.K = cfg->objective == BST_OBJ_SOFTMAX ? (int)cfg->n_classes : 1;
+  const auto tc0 = std::chrono::steady_clock::now();
   Trainer tr(ctx, ds, opt);
+  if (std::getenv("BST_TRACE_SETUP")) {
+    BST_HIP(hipStreamSynchronize(ctx->stream));
+    std::fprintf(stderr, "[bst] trainer buffers %.3f ms\n",
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0)
+                     .count());
+  }
   bst_model* m = tr.train(y_dev, w_dev, vX_dev, vy_dev, nv);
   ctx->last_ms_total = tr.stats.ms_total;
   ctx->last_ms_per_tree = tr.stats.n_trees ? tr.stats.ms_total / (double)tr.stats.n_trees : 0.0;

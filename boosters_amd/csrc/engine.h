@@ -36,7 +36,9 @@ struct HistItem {
   uint32_t begin;    // first position in idx/ogh of this tile
   uint32_t count;    // rows in this tile
   uint32_t group;    // feature group
-  uint32_t tree;     // tree slot (selects the quantisation scale)
+  uint32_t tree;     // tree slot (selects the quantisation scale); bit 31: the
+                     // tile is the whole node (store directly); bit 30: the rows
+                     // live in the alternate idx/ogh buffer
 };
 
 // A split decision on device (SplitInfo, split/types.rs:34-55).
@@ -57,7 +59,7 @@ struct PartItem {
   uint32_t n_tiles;
   uint32_t right_node; // global id of the new right child
   uint32_t feature;    // effective column
-  uint32_t pad;
+  uint32_t pad;        // device-driven partition: bit 0 = rows are in the alternate buffer
 };
 
 struct PartTile {
@@ -72,7 +74,7 @@ struct Segment {
   uint32_t node;
   uint32_t begin;
   uint32_t count;
-  uint32_t tree;
+  uint32_t tree;  // tree slot; bit 31: rows in the alternate idx buffer
 };
 
 // Packed tree node for traversal (Tree SoA, repr/gbdt/tree.rs:32-47).
@@ -108,7 +110,8 @@ void launch_to_rowmajor(hipStream_t s, const DatasetDev& d);
 
 void launch_gradients(hipStream_t s, int objective, const float* preds, const float* y,
                       const float* w, int64_t n, int K, float* g, float* h, uint32_t* absmax);
-void launch_qscale(hipStream_t s, const uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp);
+// Reads absmax[0..2K) and resets it to zero for the next round.
+void launch_qscale(hipStream_t s, uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp);
 
 void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g, const float* h,
                       const uint32_t* sampled, int64_t m);
@@ -116,7 +119,18 @@ void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g,
 void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
                  const uint32_t* idx, const float2* ogh, int64_t tree_stride_rows,
                  const int32_t* qexp, long long* hist, int64_t slot_elems, long long* node_sum,
-                 bool gather, int rows_per_block_hint, const uint32_t* dyn = nullptr);
+                 bool gather, int rows_per_block_hint, const uint32_t* dyn = nullptr,
+                 long long* partial = nullptr, const uint32_t* idx_alt = nullptr,
+                 const float2* ogh_alt = nullptr);
+// node_sum may be null (no totals); partial non-null: multi-tile nodes store
+// one partial histogram per item there instead of global atomics, and
+// launch_hist_finish sums them (and applies the subtraction trick).
+void launch_hist_finish(hipStream_t s, const uint32_t* fin, int n, int NG, int FG,
+                        const long long* partial, long long* hist, int64_t slot_elems,
+                        const uint32_t* dyn);
+void launch_init_rows_sums(hipStream_t s, uint32_t* idx, float2* ogh, const float* g,
+                           const float* h, int64_t n, int T, const int32_t* qexp,
+                           uint32_t max_nodes, long long* node_sum);
 void launch_node_sums(hipStream_t s, const Segment* segs, int n_segs, const float2* ogh,
                       int64_t tree_stride_rows, const int32_t* qexp, long long* node_sum,
                       const uint32_t* dyn = nullptr);
@@ -138,7 +152,7 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
                   int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
                   const int32_t* qexp, uint32_t max_nodes, SplitParams p,
                   unsigned long long* node_key, DevSplit* cand, DevSplit* node_split,
-                  const uint32_t* dyn = nullptr);
+                  const uint32_t* dyn = nullptr, bool finalize = true);
 
 // Stable partition of every listed node (partition.rs:73-145): count, scan,
 // scatter, copy back, then children counts/sums.
@@ -149,10 +163,24 @@ void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items,
                       uint32_t* item_left, uint32_t* node_count, long long* node_sum,
                       const uint32_t* dyn_items = nullptr, const uint32_t* dyn_tiles = nullptr);
 
+// Device-driven partition, one pass: each tile reserves its left rows from
+// the front and its right rows from the back of the node's range in the other
+// buffer of the pair (cursor[2 item] / cursor[2 item + 1], zeroed by the
+// caller; the final left cursor is the left count) and stores the exact
+// totals of its left and right rows in tile_sums[4 tile ..].  Row order
+// inside a node is not kept (no result depends on it: all sums are exact
+// integers).  Counts read from dyn_tiles.
+constexpr uint32_t kFusedTileRows = 4096;  // rows per tile of launch_partition_fused
+void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
+                            const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
+                            uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,
+                            uint32_t* cursor, const int32_t* qexp, uint32_t max_nodes,
+                            long long* tile_sums, const uint32_t* dyn_tiles);
+
 void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const float* leaf_val,
                         const uint32_t* idx, int64_t tree_stride_rows, float* preds,
                         const uint32_t* tree_output, int64_t n_rows,
-                        const uint32_t* dyn = nullptr);
+                        const uint32_t* dyn = nullptr, const uint32_t* idx_alt = nullptr);
 
 void launch_predict(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
                     const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,

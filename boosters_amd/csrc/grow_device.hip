@@ -4,7 +4,9 @@
 // trainer.cpp (itself restating training/gbdt/{grower,expansion}.rs):
 //   plan      = GrowthState::pop_next + should_expand + apply_split_to_builder
 //               + RowPartitioner::split id allocation (grower.rs:346-394,
-//               530-650; expansion.rs:150-231)
+//               530-650; expansion.rs:150-231), preceded by the best-split
+//               selection of the previous step's nodes (find.rs per-feature
+//               bests reduced across features)
 //   children  = the subtraction-trick bookkeeping of process_candidate
 //               (grower.rs:396-480): smaller child built, larger = parent - smaller
 //   finalize  = finalize_remaining + apply_learning_rate (grower.rs:484-514,
@@ -21,7 +23,8 @@ namespace bst {
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr uint32_t kDirectFlag = 0x80000000u;
+constexpr uint32_t kDirectFlag = 0x80000000u;  // HistItem.tree: tile = whole node
+constexpr uint32_t kAltFlag = 0x40000000u;     // HistItem.tree: alternate buffer
 constexpr int kCtlThreads = 1024;
 
 // Exclusive scan over the block (kCtlThreads threads); *total receives the sum.
@@ -97,16 +100,35 @@ __device__ void make_leaf(const CtlParams& P, const DeviceBufs& B, uint32_t t, u
   }
 }
 
+// Depth-wise: the previous step's children become the level to plan.
+__device__ void advance_level(const CtlParams& P, const DeviceBufs& B) {
+  for (uint32_t t = 0; t < P.T; t++) {
+    const uint32_t nn = B.ctl[t].n_next;
+    for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) {
+      const uint64_t s = (uint64_t)t * 2 * P.W + i, d = (uint64_t)t * P.WO + i;
+      B.open_node[d] = B.next_node[s];
+      B.open_tnode[d] = B.next_tnode[s];
+      B.open_depth[d] = B.next_depth[s];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      TreeCtl& c = B.ctl[t];
+      c.n_open = nn;
+      c.n_next = 0;
+      c.depth += 1;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
-// Root state of every tree, root histogram work and the root split list.
+// Root state of every tree, root histogram work, its finish entry and the
+// root split list.  Root totals are added by the row initialisation.
 __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBufs B,
                                                           const uint32_t* __restrict__ rows) {
-  __shared__ uint32_t s_hist, s_zero;
-  if (threadIdx.x == 0) {
-    s_hist = 0;
-    s_zero = 0;
-  }
+  __shared__ uint32_t s_hist;
+  if (threadIdx.x == 0) s_hist = 0;
   __syncthreads();
   for (uint32_t t = 0; t < P.T; t++) {
     for (uint32_t p = threadIdx.x; p < P.MN; p += blockDim.x) {
@@ -131,6 +153,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
       B.ctl[t] = c;
       const uint32_t gid = t * P.MN;
       B.pslot[gid] = 0;
+      B.pbuf[gid] = 0;
       B.pbegin[gid] = (uint32_t)(t * P.stride);
       B.node_count[gid] = m;
       B.node_sum[2 * gid] = 0;
@@ -141,7 +164,10 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
       B.open_depth[(uint64_t)t * P.WO] = 0;
       B.split_nodes[t] = gid;
       B.split_slots[t] = t * P.SPT;
-      if (nt != 1) B.zero[s_zero++] = t * P.SPT;
+      B.fin[4 * t + 0] = t * P.SPT;
+      B.fin[4 * t + 1] = kNone;
+      B.fin[4 * t + 2] = s_hist;
+      B.fin[4 * t + 3] = nt;
     }
     __syncthreads();
     const uint32_t base = s_hist;
@@ -164,7 +190,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
   if (threadIdx.x == 0) {
     for (int i = 0; i < C_NCOUNTS; i++) B.counts[i] = 0;
     B.counts[C_HIST] = s_hist;
-    B.counts[C_ZERO] = s_zero;
+    B.counts[C_FIN] = P.T;
     B.counts[C_SPLIT] = P.T;
     uint32_t r = 0;
     for (uint32_t t = 0; t < P.T; t++) r += rows[t];
@@ -173,12 +199,36 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
   }
 }
 
-// Choose the nodes to expand this step, decide leaf / split, allocate ids,
-// write the tree nodes and the partition work list.
-__global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBufs B) {
+// Pick each listed node's best split from its per-feature candidates (the
+// packed key holds the best gain and the winning feature under the tie
+// rule), then choose the nodes to expand this step, decide leaf / split,
+// allocate ids, write the tree nodes and the partition work list.
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBufs B,
+                                                          int advance) {
   __shared__ uint32_t s_tmp[kCtlThreads / 64 + 1];
   __shared__ uint32_t s_items, s_tiles, s_nch;
   __shared__ uint32_t s_part, s_tnodes;
+  const uint32_t n_sp = B.counts[C_SPLIT];
+  for (uint32_t i = threadIdx.x; i < n_sp; i += blockDim.x) {
+    const uint32_t node = B.split_nodes[i];
+    const unsigned long long key = B.node_key[node];
+    B.node_key[node] = 0;
+    DevSplit sp;
+    if (key == 0) {
+      sp.gain = -INFINITY;
+      sp.feature = 0;
+      sp.bin = 0;
+      sp.flags = 0;
+      for (int w = 0; w < kMaxCatWords; w++) sp.cats[w] = 0;
+    } else {
+      const uint32_t lo = (uint32_t)(key & 0xFFFFFFFFu);
+      const uint32_t f = P.rightmost ? lo : (0xFFFFFFFFu - lo);
+      sp = B.cand[(uint64_t)node * P.E + f];
+    }
+    B.node_split[node] = sp;
+  }
+  __syncthreads();
+  if (advance && P.growth == 0) advance_level(P, B);
   if (threadIdx.x == 0) {
     s_items = 0;
     s_tiles = 0;
@@ -255,6 +305,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
         const int out = P.out0 + (int)t;
         const double H = ldexp((double)B.node_sum[2 * gid + 1], -B.qexp[2 * out + 1]);
         const uint32_t right = s_part + e;
+        const uint32_t rgid = t * P.MN + right;
         const uint32_t a = s_tnodes + 2 * e, b = a + 1;
         const uint64_t k = (uint64_t)t * P.MNT + tn;
         const uint32_t f = sp.feature;
@@ -288,10 +339,12 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
         it.count = count;
         it.tile0 = s_tiles + toff;
         it.n_tiles = ntile;
-        it.right_node = t * P.MN + right;
+        it.right_node = rgid;
         it.feature = f;
-        it.pad = 0;
+        it.pad = B.pbuf[gid];
         B.items[g] = it;
+        B.cursor[2 * g] = 0;
+        B.cursor[2 * g + 1] = 0;
         B.pend[5 * g + 0] = t;
         B.pend[5 * g + 1] = lt;
         B.pend[5 * g + 2] = rt;
@@ -330,25 +383,66 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   }
 }
 
-// After the partition: slots, the smaller child's histogram / sums work, the
-// subtraction list, the split list, and the children pushed to the open set.
+// After the partition: child counts, slots, the smaller child's histogram
+// items, the finish list (partials + subtraction), the split list, and the
+// children pushed to the open set (leaf-wise) or the next level (depth-wise).
+constexpr uint32_t kChildLdsItems = 1024;  // items whose tile totals are reduced in LDS
+
 __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, DeviceBufs B) {
+  __shared__ long long s_sum[kChildLdsItems][4];
   if (threadIdx.x == 0) {
     B.counts[C_HIST] = 0;
-    B.counts[C_ZERO] = 0;
-    B.counts[C_QUADS] = 0;
-    B.counts[C_SQUADS] = 0;
-    B.counts[C_SEGS] = 0;
+    B.counts[C_FIN] = 0;
     B.counts[C_SPLIT] = 0;
   }
-  __syncthreads();
   const uint32_t n_items = B.counts[C_ITEMS];
+  const bool lds_sums = n_items <= kChildLdsItems;
+  if (lds_sums) {
+    // Children's exact totals: every thread adds whole partition tiles.
+    for (uint32_t i = threadIdx.x; i < 4 * n_items; i += blockDim.x) (&s_sum[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t n_tiles = B.counts[C_TILES];
+    for (uint32_t k = threadIdx.x; k < n_tiles; k += blockDim.x) {
+      const uint32_t item = B.tiles[k].item;
+      const long long* ts = B.tile_sums + 4 * (uint64_t)k;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        atomicAdd((unsigned long long*)&s_sum[item][j], (unsigned long long)ts[j]);
+    }
+  }
+  __syncthreads();
   for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) {
     const uint32_t t = B.pend[5 * i], lt = B.pend[5 * i + 1], rt = B.pend[5 * i + 2];
     const uint32_t e = B.pend[5 * i + 3], cdepth = B.pend[5 * i + 4];
     const PartItem it = B.items[i];
-    const uint32_t L = B.item_left[i], R = it.count - L;
+    const uint32_t L = B.cursor[2 * i], R = it.count - L;
     const uint32_t parent = it.node, right = it.right_node;
+    B.node_count[parent] = L;
+    B.node_count[right] = R;
+    {  // children's exact totals from the partition tiles
+      long long lg = 0, lh = 0, rg = 0, rh = 0;
+      if (lds_sums) {
+        lg = s_sum[i][0];
+        lh = s_sum[i][1];
+        rg = s_sum[i][2];
+        rh = s_sum[i][3];
+      } else {
+        for (uint32_t k = 0; k < it.n_tiles; k++) {
+          const long long* ts = B.tile_sums + 4 * (uint64_t)(it.tile0 + k);
+          lg += ts[0];
+          lh += ts[1];
+          rg += ts[2];
+          rh += ts[3];
+        }
+      }
+      B.node_sum[2 * parent] = lg;
+      B.node_sum[2 * parent + 1] = lh;
+      B.node_sum[2 * right] = rg;
+      B.node_sum[2 * right + 1] = rh;
+    }
+    const uint8_t cbuf = (uint8_t)((it.pad & 1u) ^ 1u);  // the partition wrote the other buffer
+    B.pbuf[parent] = cbuf;
+    B.pbuf[right] = cbuf;
     const uint32_t pb = B.pbegin[parent];
     B.pbegin[right] = pb + L;
     const bool left_small = L <= R;
@@ -367,26 +461,23 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
       const uint32_t nt = (scount + P.hist_tile - 1) / P.hist_tile;
       atomicAdd(&B.counts[C_HROWS], scount);
       atomicAdd(&B.counts[C_HNODES], 1u);
-      if (nt != 1) B.zero[atomicAdd(&B.counts[C_ZERO], 1u)] = gss;
-      if (nt > 0) {
-        const uint32_t h0 = atomicAdd(&B.counts[C_HIST], nt * P.NG);
-        for (uint32_t q = 0; q < nt; q++)
-          for (uint32_t g = 0; g < P.NG; g++) {
-            HistItem h;
-            h.node = small;
-            h.slot = gss;
-            h.begin = sbegin + q * P.hist_tile;
-            h.count = min(P.hist_tile, scount - q * P.hist_tile);
-            h.group = g;
-            h.tree = t | (nt == 1 ? kDirectFlag : 0u);
-            B.hitems[h0 + q * P.NG + g] = h;
-          }
-      }
-      const uint32_t qd = atomicAdd(&B.counts[C_QUADS], 1u);
-      B.quads[4 * qd + 0] = gls;
-      B.quads[4 * qd + 1] = gss;
-      B.quads[4 * qd + 2] = large;
-      B.quads[4 * qd + 3] = small;
+      const uint32_t h0 = atomicAdd(&B.counts[C_HIST], nt * P.NG);
+      for (uint32_t q = 0; q < nt; q++)
+        for (uint32_t g = 0; g < P.NG; g++) {
+          HistItem h;
+          h.node = small;
+          h.slot = gss;
+          h.begin = sbegin + q * P.hist_tile;
+          h.count = min(P.hist_tile, scount - q * P.hist_tile);
+          h.group = g;
+          h.tree = t | (nt == 1 ? kDirectFlag : 0u) | (cbuf ? kAltFlag : 0u);
+          B.hitems[h0 + q * P.NG + g] = h;
+        }
+      const uint32_t fi = atomicAdd(&B.counts[C_FIN], 1u);
+      B.fin[4 * fi + 0] = gss;
+      B.fin[4 * fi + 1] = gls;
+      B.fin[4 * fi + 2] = h0;
+      B.fin[4 * fi + 3] = nt;
       const uint32_t sidx = atomicAdd(&B.counts[C_SPLIT], 2u);
       B.split_nodes[sidx] = parent;
       B.split_slots[sidx] = t * P.SPT + B.pslot[parent];
@@ -394,23 +485,6 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
       B.split_slots[sidx + 1] = t * P.SPT + B.pslot[right];
     } else {
       B.pslot[small] = kNone;
-      const uint32_t ns = (scount + P.seg_tile - 1) / P.seg_tile;
-      if (ns > 0) {
-        const uint32_t s0 = atomicAdd(&B.counts[C_SEGS], ns);
-        for (uint32_t q = 0; q < ns; q++) {
-          Segment sg;
-          sg.node = small;
-          sg.begin = sbegin + q * P.seg_tile;
-          sg.count = min(P.seg_tile, scount - q * P.seg_tile);
-          sg.tree = t;
-          B.segs[s0 + q] = sg;
-        }
-      }
-      const uint32_t qd = atomicAdd(&B.counts[C_SQUADS], 1u);
-      B.squads[4 * qd + 0] = 0;
-      B.squads[4 * qd + 1] = 0;
-      B.squads[4 * qd + 2] = large;
-      B.squads[4 * qd + 3] = small;
     }
     const uint32_t lp = parent - t * P.MN, rp = right - t * P.MN;
     if (P.growth == 0) {
@@ -435,29 +509,10 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
   }
 }
 
-// Depth-wise: the next level becomes the open level.
-__global__ __launch_bounds__(kCtlThreads) void k_ctl_advance(CtlParams P, DeviceBufs B) {
-  for (uint32_t t = 0; t < P.T; t++) {
-    const uint32_t nn = B.ctl[t].n_next;
-    for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) {
-      const uint64_t s = (uint64_t)t * 2 * P.W + i, d = (uint64_t)t * P.WO + i;
-      B.open_node[d] = B.next_node[s];
-      B.open_tnode[d] = B.next_tnode[s];
-      B.open_depth[d] = B.next_depth[s];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      TreeCtl& c = B.ctl[t];
-      c.n_open = nn;
-      c.n_next = 0;
-      c.depth += 1;
-    }
-    __syncthreads();
-  }
-}
-
 // Remaining candidates become leaves; learning rate; leaf-range update list.
-__global__ __launch_bounds__(kCtlThreads) void k_ctl_finalize(CtlParams P, DeviceBufs B) {
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_finalize(CtlParams P, DeviceBufs B,
+                                                              int advance) {
+  if (advance && P.growth == 0) advance_level(P, B);
   if (threadIdx.x == 0) B.counts[C_LSEGS] = 0;
   __syncthreads();
   for (uint32_t t = 0; t < P.T; t++) {
@@ -486,12 +541,19 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_finalize(CtlParams P, Devic
         sg.node = gid;
         sg.begin = b + q * P.seg_tile;
         sg.count = min(P.seg_tile, cnt - q * P.seg_tile);
-        sg.tree = t;
+        sg.tree = t | (B.pbuf[gid] ? 0x80000000u : 0u);
         B.leaf_segs[s0 + q] = sg;
       }
     }
     __syncthreads();
   }
+}
+
+__global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                           int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
@@ -500,21 +562,50 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
     p[i] = v;
 }
 
+size_t tree_out_layout(uint32_t T, uint32_t MNT, char* base, TreeOut* out) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  const size_t tn = (size_t)T * MNT;
+  TreeOut o;
+  o.ctl = (TreeCtl*)take(sizeof(TreeCtl) * T);
+  o.counts = (uint32_t*)take(sizeof(uint32_t) * C_NCOUNTS);
+  o.tr_split = (uint32_t*)take(4 * tn);
+  o.tr_left = (uint32_t*)take(4 * tn);
+  o.tr_right = (uint32_t*)take(4 * tn);
+  o.tr_cats = (uint32_t*)take(4 * tn * kMaxCatWords);
+  o.tr_thr = (float*)take(4 * tn);
+  o.tr_leaf = (float*)take(4 * tn);
+  o.tr_gain = (float*)take(4 * tn);
+  o.tr_cover = (float*)take(4 * tn);
+  o.tr_dleft = (uint8_t*)take(tn);
+  o.tr_isleaf = (uint8_t*)take(tn);
+  o.tr_stype = (uint8_t*)take(tn);
+  if (out) *out = o;
+  return off;
+}
+
 void launch_ctl_init(hipStream_t s, const CtlParams& p, const DeviceBufs& b,
                      const uint32_t* rows_per_tree) {
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(kCtlThreads), 0, s, p, b, rows_per_tree);
 }
-void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
-  hipLaunchKernelGGL(k_ctl_plan, dim3(1), dim3(kCtlThreads), 0, s, p, b);
+void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance) {
+  hipLaunchKernelGGL(k_ctl_plan, dim3(1), dim3(kCtlThreads), 0, s, p, b, advance ? 1 : 0);
 }
 void launch_ctl_children(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
   hipLaunchKernelGGL(k_ctl_children, dim3(1), dim3(kCtlThreads), 0, s, p, b);
 }
-void launch_ctl_advance(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
-  hipLaunchKernelGGL(k_ctl_advance, dim3(1), dim3(kCtlThreads), 0, s, p, b);
+void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance) {
+  hipLaunchKernelGGL(k_ctl_finalize, dim3(1), dim3(kCtlThreads), 0, s, p, b, advance ? 1 : 0);
 }
-void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
-  hipLaunchKernelGGL(k_ctl_finalize, dim3(1), dim3(kCtlThreads), 0, s, p, b);
+void launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, int64_t n) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
 }
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, int64_t n) {
   if (n <= 0) return;

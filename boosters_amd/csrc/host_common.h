@@ -122,6 +122,22 @@ struct bst_ctx {
   };
   bool profile = false;
   std::vector<Pending> pending;
+  // Grow-only pinned host buffer for device -> host staging.
+  void* staging = nullptr;
+  size_t staging_bytes = 0;
+  void* pinned_staging(size_t bytes) {
+    if (bytes > staging_bytes) {
+      if (staging) {
+        BST_HIP(hipStreamSynchronize(stream));
+        (void)hipHostFree(staging);
+        staging = nullptr;
+        staging_bytes = 0;
+      }
+      BST_HIP(hipHostMalloc(&staging, bytes, hipHostMallocDefault));
+      staging_bytes = bytes;
+    }
+    return staging;
+  }
   int64_t k_launches[3] = {0, 0, 0};
   double k_ms[3] = {0, 0, 0}, k_bytes[3] = {0, 0, 0};
   // Record the start of a launch of `kind`; returns an index for prof_end.

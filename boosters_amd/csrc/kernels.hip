@@ -304,7 +304,7 @@ void launch_gradients(hipStream_t s, int objective, const float* preds, const fl
 }
 
 // Scale exponent per (tree, g/h): largest e with n * max|x| * 2^e <= 2^62.
-__global__ void k_qscale(const uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
+__global__ void k_qscale(uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
   const int i = threadIdx.x;
   if (i >= 2 * K) return;
   const float m = __uint_as_float(absmax[i]);
@@ -318,9 +318,10 @@ __global__ void k_qscale(const uint32_t* absmax, int K, int64_t n_rows, int32_t*
     if (e > 1000) e = 1000;
   }
   qexp[i] = e;
+  absmax[i] = 0u;  // ready for the next round's gradients
 }
 
-void launch_qscale(hipStream_t s, const uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
+void launch_qscale(hipStream_t s, uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
   hipLaunchKernelGGL(k_qscale, dim3(1), dim3(64 * ((2 * K + 63) / 64)), 0, s, absmax, K, n_rows,
                      qexp);
 }
@@ -345,21 +346,74 @@ void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g,
                      h, sampled, m);
 }
 
+// Rows of every tree slot in order (slot t uses output out0 + t) and the
+// exact fixed-point root totals, added to node_sum[t * max_nodes] (zeroed
+// before).  Grid (row blocks, slots).
+__global__ __launch_bounds__(256) void k_init_rows_sums(uint32_t* __restrict__ idx,
+                                                        float2* __restrict__ ogh,
+                                                        const float* __restrict__ g,
+                                                        const float* __restrict__ h,
+                                                        int64_t n, const int32_t* __restrict__ qexp,
+                                                        uint32_t max_nodes,
+                                                        long long* __restrict__ node_sum) {
+  __shared__ long long s_red[256 / WAVE];
+  const int t = blockIdx.y;
+  const double sg = ldexp(1.0, qexp[2 * t]), sh = ldexp(1.0, qexp[2 * t + 1]);
+  long long ag = 0, ah = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 gh = make_float2(g[(int64_t)t * n + i], h[(int64_t)t * n + i]);
+    idx[(int64_t)t * n + i] = (uint32_t)i;
+    ogh[(int64_t)t * n + i] = gh;
+    ag += __double2ll_rn((double)gh.x * sg);
+    ah += __double2ll_rn((double)gh.y * sh);
+  }
+  ag = wave_sum_ll(ag);
+  ah = wave_sum_ll(ah);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_red[threadIdx.x / WAVE] = ag;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long tg = 0;
+    for (int i = 0; i < 256 / WAVE; i++) tg += s_red[i];
+    atomicAdd((unsigned long long*)&node_sum[2 * (int64_t)t * max_nodes], (unsigned long long)tg);
+  }
+  __syncthreads();
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_red[threadIdx.x / WAVE] = ah;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long th = 0;
+    for (int i = 0; i < 256 / WAVE; i++) th += s_red[i];
+    atomicAdd((unsigned long long*)&node_sum[2 * (int64_t)t * max_nodes + 1], (unsigned long long)th);
+  }
+}
+
+void launch_init_rows_sums(hipStream_t s, uint32_t* idx, float2* ogh, const float* g,
+                           const float* h, int64_t n, int T, const int32_t* qexp,
+                           uint32_t max_nodes, long long* node_sum) {
+  if (n <= 0 || T <= 0) return;
+  int64_t bx = (n + 255) / 256;
+  if (bx > 512) bx = 512;
+  hipLaunchKernelGGL(k_init_rows_sums, dim3((unsigned)bx, (unsigned)T), dim3(256), 0, s, idx, ogh,
+                     g, h, n, qexp, max_nodes, node_sum);
+}
+
 // ===========================================================================
 // Histograms
 // ===========================================================================
 // One block-iteration = one HistItem: a tile of a node's rows x one group of
-// FG features.  Per chunk of kHistChunk rows the block first stages, one row
-// per thread, the row id and the row's fixed-point (g, h) pair in LDS (each
-// row is loaded and quantised once, not once per feature lane); then lane
-// (threadIdx % FG) owns one feature and threadIdx / FG picks the row, so the
-// FG lanes of a row read FG consecutive bytes of that row.  Each thread issues
-// its bin loads for kHistU rows before consuming any, so many loads are in
-// flight per wave.  The LDS sub-histogram is laid out [bin][FG]: lanes of one
-// row always hit FG distinct 8-byte words, so ds_add_u64 is bank-conflict
-// free whatever the bins are, and hot bins never collide inside a row.
-// Global layout per slot: [group][bin][FG] of (g, h) int64 pairs, which the
-// flush writes with unit stride.
+// FG features.  The tile is walked in chunks of kHistChunk rows.  Staging:
+// one row per thread loads the row id and the row's (g, h), quantises the
+// pair to fixed point once (not once per feature lane) and stores both in
+// LDS; the next chunk's loads are issued before the current chunk is
+// histogrammed, so their latency overlaps that work (two staging buffers).
+// Histogramming: lane (threadIdx % FG) owns one feature and threadIdx / FG
+// picks a run of RPT consecutive staged rows, so the FG lanes of a row read
+// FG consecutive bytes of that row; each thread issues the bin (and hi-bit)
+// loads of all its RPT rows before consuming any.  The LDS sub-histogram is
+// laid out [bin][FG]: lanes of one row always hit FG distinct 8-byte words,
+// so ds_add_u64 is bank-conflict free whatever the bins are, and hot bins
+// never collide inside a row.  Global layout per slot: [group][bin][FG] of
+// (g, h) int64 pairs, which the flush writes with unit stride.
 //
 // XCD-aware item order: work-groups are dispatched round-robin over the 8
 // XCDs (block b runs on XCD b % 8), each with its own L2.  Position p of the
@@ -367,25 +421,29 @@ void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g,
 // (p / 8) % NG, so the NG group blocks of one tile (which read disjoint
 // 32-byte pieces of the same 128-byte rows) run on one XCD and share its L2
 // instead of each fetching every row from HBM.  The grid is a multiple of 8.
+//
+// OFF64: row byte offsets need 64 bits (n * S >= 2^32); otherwise they are
+// 32-bit offsets from the uniform base, which keeps address registers small.
 constexpr int kHistThreads = 1024;
-constexpr int kHistChunk = 1024;  // rows staged per chunk (one per thread)
-constexpr int kHistU = 8;         // rows whose bin loads are issued together
+constexpr int kHistChunk = 512;  // rows per staged chunk
 
-template <int FG, bool GATHER, bool HAS_HI>
+template <int FG, bool GATHER, bool HAS_HI, bool OFF64>
 __global__ __launch_bounds__(kHistThreads) void k_hist(
     const HistItem* __restrict__ items, uint32_t n_items_static, int NG,
     const uint32_t* __restrict__ idx, const float2* __restrict__ ogh, int64_t tree_stride,
     const uint8_t* __restrict__ bins_rm, int S, int hi_off, int E,
     const int32_t* __restrict__ qexp, long long* __restrict__ hist, int64_t slot_elems,
-    long long* __restrict__ node_sum, const uint32_t* __restrict__ dyn) {
+    long long* __restrict__ node_sum, long long* __restrict__ partial,
+    const uint32_t* __restrict__ dyn, const uint32_t* __restrict__ idx_alt,
+    const float2* __restrict__ ogh_alt) {
   __shared__ long long s_g[256 * FG];
   __shared__ long long s_h[256 * FG];
-  __shared__ uint32_t s_row[kHistChunk];
-  __shared__ longlong2 s_q[kHistChunk];
+  __shared__ uint32_t s_row[2][kHistChunk];
+  __shared__ longlong2 s_q[2][kHistChunk];
   __shared__ long long s_red[2][kHistThreads / WAVE];
-  constexpr int RPP = kHistThreads / FG;         // rows per pass
-  constexpr int RPT = kHistChunk / RPP;          // rows per thread per chunk
-  static_assert(RPT % kHistU == 0, "chunk must split into load batches");
+  constexpr int RPP = kHistThreads / FG;  // row runs per chunk
+  constexpr int RPT = kHistChunk / RPP;   // rows per thread per chunk
+  static_assert(RPT >= 1 && kHistChunk <= kHistThreads, "chunk shape");
   const uint32_t n_items = dyn ? *dyn : n_items_static;
   const uint32_t n_tiles = n_items / (uint32_t)NG;
   const uint32_t n_pos = ((n_tiles + 7u) / 8u) * 8u * (uint32_t)NG;
@@ -395,9 +453,11 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(
     const uint32_t tile = (p / (8u * NG)) * 8u + (p & 7u);
     const uint32_t grp = (p >> 3) % (uint32_t)NG;
     if (tile >= n_tiles) continue;  // block-uniform
-    const HistItem it = items[tile * (uint32_t)NG + grp];
+    const uint32_t item_idx = tile * (uint32_t)NG + grp;
+    const HistItem it = items[item_idx];
     const bool direct = (it.tree & 0x80000000u) != 0;  // this tile covers the whole node
-    const uint32_t tree = it.tree & 0x7FFFFFFFu;
+    const bool alt = (it.tree & 0x40000000u) != 0;     // rows in the alternate buffer
+    const uint32_t tree = it.tree & 0x3FFFFFFFu;
     for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
       s_g[i] = 0;
       s_h[i] = 0;
@@ -405,53 +465,77 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(
     const int f = (int)it.group * FG + lf;
     const bool fvalid = f < E;
     const int fs = fvalid ? f : 0;  // loads stay in the row for padding lanes
+    const int hb = hi_off + (fs >> 3), hs = fs & 7;
     const double sg = ldexp(1.0, qexp[2 * tree]);
     const double sh = ldexp(1.0, qexp[2 * tree + 1]);
-    const uint32_t* tidx = idx + (int64_t)tree * tree_stride;
-    const float2* togh = ogh + (int64_t)tree * tree_stride;
+    const uint32_t* tidx = (alt ? idx_alt : idx) + (int64_t)tree * tree_stride;
+    const float2* togh = (alt ? ogh_alt : ogh) + (int64_t)tree * tree_stride;
+    const uint32_t n_chunks = (it.count + kHistChunk - 1) / kHistChunk;
     long long accg = 0, acch = 0;
-    for (uint32_t c0 = 0; c0 < it.count; c0 += kHistChunk) {
-      const uint32_t nrow = min((uint32_t)kHistChunk, it.count - c0);
-      // Stage: one row per thread.
-      if (threadIdx.x < nrow) {
-        const uint32_t pos = it.begin + c0 + threadIdx.x;
-        const uint32_t r = GATHER ? tidx[pos] : pos;
-        const float2 gh = togh[pos];
-        const long long gq = __double2ll_rn((double)gh.x * sg);
-        const long long hq = __double2ll_rn((double)gh.y * sh);
-        s_row[threadIdx.x] = r;
-        s_q[threadIdx.x] = make_longlong2(gq, hq);
+    // Staging registers: this thread's row of the next chunk.
+    uint32_t st_r = 0;
+    float2 st_gh = make_float2(0.0f, 0.0f);
+    bool st_ok = false;
+    auto stage_load = [&](uint32_t c) {
+      const uint32_t q = c * kHistChunk + threadIdx.x;
+      st_ok = threadIdx.x < kHistChunk && q < it.count;
+      if (st_ok) {
+        const uint32_t pos = it.begin + q;
+        st_r = GATHER ? tidx[pos] : pos;
+        st_gh = togh[pos];
+      }
+    };
+    auto stage_store = [&](int buf) {
+      if (st_ok) {
+        const long long gq = __double2ll_rn((double)st_gh.x * sg);
+        const long long hq = __double2ll_rn((double)st_gh.y * sh);
+        s_row[buf][threadIdx.x] = st_r;
+        s_q[buf][threadIdx.x] = make_longlong2(gq, hq);
         accg += gq;
         acch += hq;
       }
-      __syncthreads();  // staging visible; also orders the LDS zeroing
+    };
+    if (n_chunks > 0) {
+      stage_load(0);
+      stage_store(0);
+    }
+    __syncthreads();  // staging visible; also orders the LDS zeroing
+    for (uint32_t c = 0; c < n_chunks; c++) {
+      const int cur = (int)(c & 1u);
+      if (c + 1 < n_chunks) stage_load(c + 1);
+      const uint32_t nrow = min((uint32_t)kHistChunk, it.count - c * kHistChunk);
+      uint32_t bv[RPT], hv[RPT];
 #pragma unroll
-      for (int j0 = 0; j0 < RPT; j0 += kHistU) {
-        uint32_t b[kHistU];
-#pragma unroll
-        for (int u = 0; u < kHistU; u++) {
-          const uint32_t j = (uint32_t)rs + (uint32_t)(j0 + u) * RPP;
-          const uint32_t r = s_row[j < nrow ? j : 0u];
-          const uint8_t* rp = bins_rm + (int64_t)r * S;
-          uint32_t v = rp[fs];
-          if (HAS_HI) v |= ((uint32_t)(rp[hi_off + (fs >> 3)] >> (fs & 7)) & 1u) << 8;
-          b[u] = v;
-        }
-#pragma unroll
-        for (int u = 0; u < kHistU; u++) {
-          const uint32_t j = (uint32_t)rs + (uint32_t)(j0 + u) * RPP;
-          // bin 256 (NaN bin of a 257-bin feature) is not accumulated: it
-          // equals the node total minus the 256 real bins and is never read.
-          if (j < nrow && fvalid && b[u] < 256u) {
-            const longlong2 q = s_q[j];
-            atomicAdd((unsigned long long*)&s_g[b[u] * FG + lf], (unsigned long long)q.x);
-            atomicAdd((unsigned long long*)&s_h[b[u] * FG + lf], (unsigned long long)q.y);
-          }
+      for (int u = 0; u < RPT; u++) {
+        const uint32_t j = (uint32_t)(rs * RPT + u);
+        const uint32_t r = s_row[cur][j < nrow ? j : 0u];
+        if (OFF64) {
+          const uint8_t* rp = bins_rm + (uint64_t)r * (uint64_t)S;
+          bv[u] = rp[fs];
+          hv[u] = HAS_HI ? rp[hb] : 0u;
+        } else {
+          const uint32_t off = r * (uint32_t)S;
+          bv[u] = bins_rm[off + (uint32_t)fs];
+          hv[u] = HAS_HI ? bins_rm[off + (uint32_t)hb] : 0u;
         }
       }
-      __syncthreads();  // staging is rewritten by the next chunk
+#pragma unroll
+      for (int u = 0; u < RPT; u++) {
+        const uint32_t j = (uint32_t)(rs * RPT + u);
+        // bin 256 (NaN bin of a 257-bin feature) is not accumulated: it
+        // equals the node total minus the 256 real bins and is never read.
+        const bool nan_bin = HAS_HI && ((hv[u] >> hs) & 1u);
+        if (j < nrow && fvalid && !nan_bin) {
+          const longlong2 q = s_q[cur][j];
+          atomicAdd((unsigned long long*)&s_g[bv[u] * FG + lf], (unsigned long long)q.x);
+          atomicAdd((unsigned long long*)&s_h[bv[u] * FG + lf], (unsigned long long)q.y);
+        }
+      }
+      if (c + 1 < n_chunks) stage_store(cur ^ 1);
+      __syncthreads();  // next chunk staged; this chunk's staging may be rewritten
     }
-    if (it.group == 0) {
+    const bool do_sum = node_sum != nullptr && it.group == 0;
+    if (do_sum) {
       accg = wave_sum_ll(accg);
       acch = wave_sum_ll(acch);
       if ((threadIdx.x & (WAVE - 1)) == 0) {
@@ -460,7 +544,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(
       }
     }
     __syncthreads();
-    if (it.group == 0 && threadIdx.x == 0) {
+    if (do_sum && threadIdx.x == 0) {
       long long tg = 0, th = 0;
       for (int wv = 0; wv < kHistThreads / WAVE; wv++) {
         tg += s_red[0][wv];
@@ -475,12 +559,16 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(
       }
     }
     long long* dst = hist + (int64_t)it.slot * slot_elems + (int64_t)it.group * 256 * FG * 2;
-    if (direct) {
+    if (direct || partial) {
+      // Whole node: the slot itself.  One tile of several: this item's own
+      // partial, summed by k_hist_finish (no global atomics).
+      longlong2* out = direct ? (longlong2*)dst
+                              : (longlong2*)(partial + (int64_t)item_idx * 256 * FG * 2);
       for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
         longlong2 v;
         v.x = s_g[i];
         v.y = s_h[i];
-        ((longlong2*)dst)[i] = v;
+        out[i] = v;
       }
     } else {
       for (int i = threadIdx.x; i < 256 * FG; i += blockDim.x) {
@@ -495,6 +583,80 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(
   }
 }
 
+// Finish the histograms of one step: for each built node (small child, or a
+// root) and group, the node histogram is the sum of its tiles' partials (or
+// was stored directly by a single tile), and the larger sibling, which holds
+// the parent's histogram in its slot, becomes parent - small
+// (histograms/pool.rs subtraction trick).  fin[4 i] = {small slot, large slot
+// or ~0, first item index, tile count}; work = (node, group, 1024-value chunk).
+// A node with one tile was stored directly; with none it is all zeros.
+constexpr int kFinThreads = 256;
+constexpr int kFinChunk = kFinThreads * 2;  // int64 values per work item
+
+__global__ __launch_bounds__(kFinThreads) void k_hist_finish(
+    const uint32_t* __restrict__ fin, uint32_t n_static, int NG, int FG,
+    const long long* __restrict__ partial, long long* __restrict__ hist, int64_t slot_elems,
+    const uint32_t* __restrict__ dyn) {
+  const uint32_t n = dyn ? *dyn : n_static;
+  const uint32_t V = 256u * (uint32_t)FG * 2u;  // int64 values per group section
+  const uint32_t chunks = V / kFinChunk;
+  const uint32_t per_node = (uint32_t)NG * chunks;
+  const uint32_t n_work = n * per_node;
+  for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const uint32_t fi = w / per_node, rem = w % per_node;
+    const uint32_t g = rem / chunks, c = rem % chunks;
+    const uint32_t small = fin[4 * fi], large = fin[4 * fi + 1];
+    const uint32_t h0 = fin[4 * fi + 2], nt = fin[4 * fi + 3];
+    const uint32_t v0 = c * kFinChunk + threadIdx.x * 2u;
+    longlong2* sd = (longlong2*)(hist + (int64_t)small * slot_elems + (int64_t)g * V + v0);
+    longlong2 a;
+    if (nt == 1) {
+      a = *sd;
+    } else {  // nt == 0 (no rows) stores zeros
+      long long x0 = 0, y0 = 0, x1 = 0, y1 = 0, x2 = 0, y2 = 0, x3 = 0, y3 = 0;
+      const longlong2* src = (const longlong2*)(partial + (int64_t)(h0 + g) * V + v0);
+      const int64_t step = (int64_t)NG * V / 2;  // longlong2 between a node's tiles
+      uint32_t k = 0;
+      for (; k + 4 <= nt; k += 4) {  // four independent loads in flight
+        const longlong2 p0 = src[(k + 0) * step], p1 = src[(k + 1) * step];
+        const longlong2 p2 = src[(k + 2) * step], p3 = src[(k + 3) * step];
+        x0 += p0.x;
+        y0 += p0.y;
+        x1 += p1.x;
+        y1 += p1.y;
+        x2 += p2.x;
+        y2 += p2.y;
+        x3 += p3.x;
+        y3 += p3.y;
+      }
+      for (; k < nt; k++) {
+        const longlong2 p0 = src[k * step];
+        x0 += p0.x;
+        y0 += p0.y;
+      }
+      a = make_longlong2(x0 + x1 + x2 + x3, y0 + y1 + y2 + y3);
+      *sd = a;
+    }
+    if (large != 0xFFFFFFFFu) {
+      longlong2* ld = (longlong2*)(hist + (int64_t)large * slot_elems + (int64_t)g * V + v0);
+      longlong2 p = *ld;
+      p.x -= a.x;
+      p.y -= a.y;
+      *ld = p;
+    }
+  }
+}
+
+void launch_hist_finish(hipStream_t s, const uint32_t* fin, int n, int NG, int FG,
+                        const long long* partial, long long* hist, int64_t slot_elems,
+                        const uint32_t* dyn) {
+  if (n <= 0) return;
+  int64_t grid = (int64_t)n * NG * (256 * FG * 2 / kFinChunk);
+  if (grid > 2048) grid = 2048;
+  hipLaunchKernelGGL(k_hist_finish, dim3((unsigned)grid), dim3(kFinThreads), 0, s, fin,
+                     (uint32_t)n, NG, FG, partial, hist, slot_elems, dyn);
+}
+
 // Grid for a histogram launch: a multiple of 8 (one slot per XCD), at most
 // kHistMaxGrid blocks; with a device-side count the caller's bound is used.
 constexpr int kHistMaxGrid = 2048;
@@ -503,7 +665,8 @@ template <int FG>
 static void launch_hist_fg(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
                            const uint32_t* idx, const float2* ogh, int64_t tree_stride,
                            const int32_t* qexp, long long* hist, int64_t slot_elems,
-                           long long* node_sum, bool gather, const uint32_t* dyn) {
+                           long long* node_sum, long long* partial, bool gather,
+                           const uint32_t* dyn, const uint32_t* idx_alt, const float2* ogh_alt) {
   const bool hi = d.hi_off >= 0;
   int64_t want;
   if (dyn) {
@@ -514,20 +677,23 @@ static void launch_hist_fg(hipStream_t s, const DatasetDev& d, const HistItem* i
   }
   int64_t grid = want < kHistMaxGrid ? want : kHistMaxGrid;
   grid = (grid + 7) / 8 * 8;
-#define BST_HIST_LAUNCH(G, H)                                                                   \
-  hipLaunchKernelGGL((k_hist<FG, G, H>), dim3((unsigned)grid), dim3(kHistThreads), 0, s, items, \
+#define BST_HIST_LAUNCH(G, H, O)                                                                \
+  hipLaunchKernelGGL((k_hist<FG, G, H, O>), dim3((unsigned)grid), dim3(kHistThreads), 0, s, items, \
                      (uint32_t)n_items, d.NG, idx, ogh, tree_stride, d.bins_rm, d.S, d.hi_off,  \
-                     d.E, qexp, hist, slot_elems, node_sum, dyn)
-  if (gather) {
-    if (hi)
-      BST_HIST_LAUNCH(true, true);
-    else
-      BST_HIST_LAUNCH(true, false);
+                     d.E, qexp, hist, slot_elems, node_sum, partial, dyn, idx_alt, ogh_alt)
+  const bool off64 = (uint64_t)d.n * (uint64_t)d.S >= (1ull << 32);
+  if (off64) {
+    if (gather) {
+      if (hi) BST_HIST_LAUNCH(true, true, true); else BST_HIST_LAUNCH(true, false, true);
+    } else {
+      if (hi) BST_HIST_LAUNCH(false, true, true); else BST_HIST_LAUNCH(false, false, true);
+    }
   } else {
-    if (hi)
-      BST_HIST_LAUNCH(false, true);
-    else
-      BST_HIST_LAUNCH(false, false);
+    if (gather) {
+      if (hi) BST_HIST_LAUNCH(true, true, false); else BST_HIST_LAUNCH(true, false, false);
+    } else {
+      if (hi) BST_HIST_LAUNCH(false, true, false); else BST_HIST_LAUNCH(false, false, false);
+    }
   }
 #undef BST_HIST_LAUNCH
 }
@@ -535,14 +701,15 @@ static void launch_hist_fg(hipStream_t s, const DatasetDev& d, const HistItem* i
 void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int n_items,
                  const uint32_t* idx, const float2* ogh, int64_t tree_stride, const int32_t* qexp,
                  long long* hist, int64_t slot_elems, long long* node_sum, bool gather,
-                 int /*rows_per_block_hint*/, const uint32_t* dyn) {
+                 int /*rows_per_block_hint*/, const uint32_t* dyn, long long* partial,
+                 const uint32_t* idx_alt, const float2* ogh_alt) {
   if (n_items <= 0) return;
   if (d.FG == 16)
     launch_hist_fg<16>(s, d, items, n_items, idx, ogh, tree_stride, qexp, hist, slot_elems,
-                       node_sum, gather, dyn);
+                       node_sum, partial, gather, dyn, idx_alt, ogh_alt);
   else
     launch_hist_fg<32>(s, d, items, n_items, idx, ogh, tree_stride, qexp, hist, slot_elems,
-                       node_sum, gather, dyn);
+                       node_sum, partial, gather, dyn, idx_alt, ogh_alt);
 }
 
 // Exact node totals over row segments (used where no histogram is needed).
@@ -682,7 +849,9 @@ __device__ __forceinline__ unsigned long long cand_key(float gain, uint32_t rank
   return ((unsigned long long)ord_f32(gain) << 32) | (0xFFFFFFFFu - rank);
 }
 
-__global__ __launch_bounds__(256) void k_split(
+constexpr int kSplitWaves = 8;  // features per block
+
+__global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     const uint32_t* __restrict__ nodes, const uint32_t* __restrict__ slots,
     const uint32_t* __restrict__ feats, int n_feats, const FeatMeta* __restrict__ fmeta, int FG,
     int NG, int E, const long long* __restrict__ hist, int64_t slot_elems,
@@ -690,10 +859,13 @@ __global__ __launch_bounds__(256) void k_split(
     const int32_t* __restrict__ qexp, uint32_t max_nodes, SplitParams p,
     unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand,
     const uint32_t* __restrict__ dyn) {
-  __shared__ double s_ratio[4][256];
-  __shared__ uint16_t s_order[4][256];
+  __shared__ double s_ratio[kSplitWaves][256];
+  __shared__ uint16_t s_order[kSplitWaves][256];
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  const int fi = blockIdx.x * 4 + wv;
+  // A block's kSplitWaves waves take consecutive features: with the
+  // [group][bin][FG] layout their (g, h) pairs of one bin share one 128-byte
+  // line, which the block then uses whole.
+  const int fi = blockIdx.x * kSplitWaves + wv;
   if (fi >= n_feats) return;  // whole wave; the kernel has no block barrier
   const uint32_t n_work = dyn ? *dyn : gridDim.y;
   for (uint32_t wy = blockIdx.y; wy < n_work; wy += gridDim.y) {
@@ -952,12 +1124,13 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
                   int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
                   const int32_t* qexp, uint32_t max_nodes, SplitParams p,
                   unsigned long long* node_key, DevSplit* cand, DevSplit* node_split,
-                  const uint32_t* dyn) {
+                  const uint32_t* dyn, bool finalize) {
   if (n_nodes <= 0) return;
-  dim3 grid((unsigned)((n_feats + 3) / 4), (unsigned)n_nodes);
-  hipLaunchKernelGGL(k_split, grid, dim3(256), 0, s, nodes, slots, feats, n_feats, d.meta, d.FG,
+  dim3 grid((unsigned)((n_feats + kSplitWaves - 1) / kSplitWaves), (unsigned)n_nodes);
+  hipLaunchKernelGGL(k_split, grid, dim3(kSplitWaves * WAVE), 0, s, nodes, slots, feats, n_feats, d.meta, d.FG,
                      d.NG, d.E, hist, slot_elems, node_sum, node_count, qexp, max_nodes, p,
                      node_key, cand, dyn);
+  if (!finalize) return;  // the caller reduces node_key / cand itself
   hipLaunchKernelGGL(k_split_finalize, dim3((n_nodes + 63) / 64), dim3(64), 0, s, nodes, n_nodes,
                      d.E, p.rightmost, node_key, cand, node_split, dyn);
 }
@@ -1112,6 +1285,161 @@ __global__ void k_part_children(const PartItem* __restrict__ items, int n_items,
   }
 }
 
+__device__ __forceinline__ long long block_sum_ll(long long v, long long* s_red) {
+  v = wave_sum_ll(v);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_red[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  long long t = 0;
+  for (int i = 0; i < (int)(blockDim.x / WAVE); i++) t += s_red[i];
+  __syncthreads();
+  return t;
+}
+
+// One-pass partition of a tile of a node's rows (device-driven schedule).
+// Rows are read strided (row j * kFusedThreads + thread), so every load is
+// coalesced and each thread has kFusedRows independent row loads in flight.
+// Left rows keep their tile order in the front region, right rows in the
+// back region; the tile's place in each region comes from one atomic per
+// region, so tiles of a node may land in any order.  Each tile stores the
+// exact fixed-point (g, h) totals of its left and right rows in
+// tile_sums[4 tile .. 4 tile + 3]; the caller adds a node's tiles.
+constexpr int kFusedThreads = 1024;
+constexpr int kFusedRows = kFusedTileRows / kFusedThreads;  // rows per thread
+constexpr int kFusedWaves = kFusedThreads / WAVE;
+static_assert(kFusedRows * kFusedWaves <= WAVE, "one prefix value per lane");
+
+__global__ __launch_bounds__(kFusedThreads) void k_part_fused(
+    const PartItem* __restrict__ items, const PartTile* __restrict__ tiles,
+    const DevSplit* __restrict__ node_split, const FeatMeta* __restrict__ fmeta,
+    const uint8_t* __restrict__ bins_cm, const uint32_t* __restrict__ hi_cm, int64_t n,
+    int64_t hi_words, uint32_t* __restrict__ idx0, float2* __restrict__ ogh0,
+    uint32_t* __restrict__ idx1, float2* __restrict__ ogh1, uint32_t* __restrict__ cursor,
+    const int32_t* __restrict__ qexp, uint32_t max_nodes, long long* __restrict__ tile_sums,
+    const uint32_t* __restrict__ dyn_tiles) {
+  __shared__ uint32_t s_l[kFusedRows * kFusedWaves];  // lefts per (row pass, wave)
+  __shared__ uint32_t s_v[kFusedRows * kFusedWaves];  // rows per (row pass, wave)
+  __shared__ uint32_t s_base[2];
+  __shared__ long long s_red[kFusedWaves];
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const uint32_t n_work = *dyn_tiles;
+  for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
+    const PartTile t = tiles[wi];
+    const PartItem it = items[t.item];
+    const bool src1 = (it.pad & 1u) != 0;
+    const uint32_t* sidx = src1 ? idx1 : idx0;
+    const float2* sogh = src1 ? ogh1 : ogh0;
+    uint32_t* didx = src1 ? idx0 : idx1;
+    float2* dogh = src1 ? ogh0 : ogh1;
+    const DevSplit sp = node_split[it.node];
+    const FeatMeta fm = fmeta[it.feature];
+    const uint8_t* col = bins_cm + (int64_t)it.feature * n;
+    const uint32_t* hcol = hi_cm ? hi_cm + (int64_t)it.feature * hi_words : nullptr;
+    uint32_t rid[kFusedRows];
+    float2 gh[kFusedRows];
+#pragma unroll
+    for (int j = 0; j < kFusedRows; j++) {
+      const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
+      const uint32_t pos = p < t.count ? t.begin + p : 0u;  // position 0 always exists
+      rid[j] = sidx[pos];
+      gh[j] = sogh[pos];
+    }
+    uint32_t bins[kFusedRows];
+#pragma unroll
+    for (int j = 0; j < kFusedRows; j++) {
+      uint32_t b = col[rid[j]];
+      if (hcol) b |= ((hcol[rid[j] >> 5] >> (rid[j] & 31)) & 1u) << 8;
+      bins[j] = b;
+    }
+    unsigned long long lbal[kFusedRows], vbal[kFusedRows];
+#pragma unroll
+    for (int j = 0; j < kFusedRows; j++) {
+      const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
+      const bool valid = p < t.count;
+      const bool l = valid && goes_left(sp, fm, bins[j]);
+      lbal[j] = __ballot(l);
+      vbal[j] = __ballot(valid);
+      if (lane == 0) {
+        s_l[j * kFusedWaves + wv] = (uint32_t)__popcll(lbal[j]);
+        s_v[j * kFusedWaves + wv] = (uint32_t)__popcll(vbal[j]);
+      }
+    }
+    __syncthreads();
+    // Exclusive prefix over (row pass, wave) in position order; one wave.
+    if (wv == 0) {
+      constexpr int NS = kFusedRows * kFusedWaves;  // 64 = one value per lane
+      uint32_t lv = lane < NS ? s_l[lane] : 0u, vv = lane < NS ? s_v[lane] : 0u;
+      uint32_t li = lv, vi = vv;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t a = __shfl_up(li, d, 64), b = __shfl_up(vi, d, 64);
+        if (lane >= d) {
+          li += a;
+          vi += b;
+        }
+      }
+      const uint32_t TL = __shfl(li, 63, 64), TV = __shfl(vi, 63, 64);
+      if (lane < NS) {
+        s_l[lane] = li - lv;
+        s_v[lane] = vi - vv;
+      }
+      if (lane == 0) {
+        s_base[0] = atomicAdd(&cursor[2 * t.item], TL);
+        // Right rows fill the node's range from its end backwards, tile by tile.
+        const uint32_t TR = TV - TL;
+        s_base[1] = it.count - atomicAdd(&cursor[2 * t.item + 1], TR) - TR;
+      }
+    }
+    __syncthreads();
+    const uint32_t lbase = s_base[0], rbase = s_base[1];
+    const uint32_t tree = it.node / max_nodes;
+    const double sg = ldexp(1.0, qexp[2 * tree]), sh = ldexp(1.0, qexp[2 * tree + 1]);
+    long long lg = 0, lh = 0, rg = 0, rh = 0;
+#pragma unroll
+    for (int j = 0; j < kFusedRows; j++) {
+      const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
+      if (p < t.count) {
+        const bool l = (lbal[j] >> lane) & 1ull;
+        const uint32_t lrank = s_l[j * kFusedWaves + wv] + (uint32_t)__popcll(lbal[j] & lt_mask);
+        const uint32_t vrank = s_v[j * kFusedWaves + wv] + (uint32_t)__popcll(vbal[j] & lt_mask);
+        const uint32_t dst = it.begin + (l ? lbase + lrank : rbase + (vrank - lrank));
+        didx[dst] = rid[j];
+        dogh[dst] = gh[j];
+        const long long gq = __double2ll_rn((double)gh[j].x * sg);
+        const long long hq = __double2ll_rn((double)gh[j].y * sh);
+        if (l) {
+          lg += gq;
+          lh += hq;
+        } else {
+          rg += gq;
+          rh += hq;
+        }
+      }
+    }
+    lg = block_sum_ll(lg, s_red);
+    lh = block_sum_ll(lh, s_red);
+    rg = block_sum_ll(rg, s_red);
+    rh = block_sum_ll(rh, s_red);
+    if (threadIdx.x == 0) {
+      tile_sums[4 * (int64_t)wi + 0] = lg;
+      tile_sums[4 * (int64_t)wi + 1] = lh;
+      tile_sums[4 * (int64_t)wi + 2] = rg;
+      tile_sums[4 * (int64_t)wi + 3] = rh;
+    }
+  }
+}
+
+void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
+                            const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
+                            uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,
+                            uint32_t* cursor, const int32_t* qexp, uint32_t max_nodes,
+                            long long* tile_sums, const uint32_t* dyn_tiles) {
+  if (grid_tiles <= 0) return;
+  hipLaunchKernelGGL(k_part_fused, dim3(grid_tiles), dim3(kFusedThreads), 0, s, items, tiles,
+                     node_split, d.meta, d.bins_cm, d.hi_cm, d.n, d.hi_words, idx0, ogh0, idx1,
+                     ogh1, cursor, qexp, max_nodes, tile_sums, dyn_tiles);
+}
+
 void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items, int n_items,
                       const PartTile* tiles, int n_tiles, const DevSplit* node_split,
                       uint32_t* idx, float2* ogh, uint32_t* idx_tmp, float2* ogh_tmp,
@@ -1141,23 +1469,26 @@ __global__ __launch_bounds__(256) void k_leaf_update(const Segment* __restrict__
                                                      int64_t tree_stride, float* __restrict__ preds,
                                                      const uint32_t* __restrict__ tree_output,
                                                      int64_t n_rows,
-                                                     const uint32_t* __restrict__ dyn) {
+                                                     const uint32_t* __restrict__ dyn,
+                                                     const uint32_t* __restrict__ idx_alt) {
   const uint32_t n_work = dyn ? *dyn : gridDim.x;
   for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
     const Segment sg = segs[wi];
     const float v = leaf_val[sg.node];
-    const uint32_t* t = idx + (int64_t)sg.tree * tree_stride;
-    float* pr = preds + (int64_t)tree_output[sg.tree] * n_rows;
+    const uint32_t tree = sg.tree & 0x7FFFFFFFu;
+    const uint32_t* t = ((sg.tree >> 31) ? idx_alt : idx) + (int64_t)tree * tree_stride;
+    float* pr = preds + (int64_t)tree_output[tree] * n_rows;
     for (uint32_t p = threadIdx.x; p < sg.count; p += blockDim.x) pr[t[sg.begin + p]] += v;
   }
 }
 
 void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const float* leaf_val,
                         const uint32_t* idx, int64_t tree_stride, float* preds,
-                        const uint32_t* tree_output, int64_t n_rows, const uint32_t* dyn) {
+                        const uint32_t* tree_output, int64_t n_rows, const uint32_t* dyn,
+                        const uint32_t* idx_alt) {
   if (n_segs <= 0) return;
   hipLaunchKernelGGL(k_leaf_update, dim3(n_segs), dim3(256), 0, s, segs, leaf_val, idx,
-                     tree_stride, preds, tree_output, n_rows, dyn);
+                     tree_stride, preds, tree_output, n_rows, dyn, idx_alt);
 }
 
 __device__ __forceinline__ float sigmoid32(float x) {  // transform.rs:106-115

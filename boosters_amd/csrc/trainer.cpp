@@ -18,9 +18,14 @@
 // node processing order, partition ids and tie rules are the same.
 #include "trainer.h"
 
+#include "grow_device.h"
+
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 
 namespace bst {
@@ -544,13 +549,21 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   // Base score (objectives: regression.rs:44-68, classification.rs:54-76,189-222),
   // accumulated in f64 on the host from the device targets.
   {
-    std::vector<float> yh(n_), wh;
-    d2h(s_, yh.data(), y, n_);
-    if (w) {
-      wh.resize(n_);
-      d2h(s_, wh.data(), w, n_);
-    }
+    // Targets come back through the context's pinned staging buffer (a
+    // pageable copy of 1M floats costs milliseconds).
+    float* yh = (float*)ctx_->pinned_staging((size_t)(w ? 2 : 1) * n_ * sizeof(float));
+    float* wh = w ? yh + n_ : nullptr;
+    // A kernel writes the mapped pinned buffer directly: after the GPU has
+    // been idle, DMA copies of this size have been seen to take milliseconds.
+    void* ydev = nullptr;
+    BST_HIP(hipHostGetDevicePointer(&ydev, yh, 0));
+    launch_copy_u32(s_, (uint32_t*)ydev, (const uint32_t*)y, n_);
+    if (w) launch_copy_u32(s_, (uint32_t*)ydev + n_, (const uint32_t*)w, n_);
     BST_HIP(hipStreamSynchronize(s_));
+    if (std::getenv("BST_TRACE_SETUP"))
+      std::fprintf(stderr, "[bst] targets to host %.3f ms\n",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                       .count());
     auto wt = [&](int64_t i) { return w ? wh[i] : 1.0f; };
     m->base_score.assign(K, 0.0f);
     if (cfg.objective == BST_OBJ_SQUARED) {
@@ -583,20 +596,19 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
     }
   }
   {
-    std::vector<float> fill(n_);
-    for (int k = 0; k < K; k++) {
-      std::fill(fill.begin(), fill.end(), m->base_score[k]);
-      h2d(s_, preds_.p + (size_t)k * n_, fill.data(), n_);
-    }
+    auto bits = [](float v) {
+      uint32_t u;
+      std::memcpy(&u, &v, 4);
+      return u;
+    };
+    for (int k = 0; k < K; k++)
+      launch_fill_u32(s_, (uint32_t*)preds_.p + (size_t)k * n_, bits(m->base_score[k]), n_);
     if (vX && n_valid > 0) {
       vpreds_.alloc((size_t)K * n_valid);
-      std::vector<float> vf(n_valid);
-      for (int k = 0; k < K; k++) {
-        std::fill(vf.begin(), vf.end(), m->base_score[k]);
-        h2d(s_, vpreds_.p + (size_t)k * n_valid, vf.data(), n_valid);
-      }
+      for (int k = 0; k < K; k++)
+        launch_fill_u32(s_, (uint32_t*)vpreds_.p + (size_t)k * n_valid, bits(m->base_score[k]),
+                        n_valid);
     }
-    BST_HIP(hipStreamSynchronize(s_));
   }
   const uint32_t E = (uint32_t)ds_->d.E;
   feats_h_.resize(E);
@@ -673,6 +685,10 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   };
   const bool on_device = device_schedule_ok();
   if (on_device) sp_.rightmost = (opt_.ext.n_threads != 1 && E >= 16) ? 1 : 0;
+  if (std::getenv("BST_TRACE_SETUP"))
+    std::fprintf(stderr, "[bst] train set-up before the first round %.3f ms\n",
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                     .count());
 
   std::vector<std::vector<float>> leafv;
   bool have_best = false;

@@ -1,14 +1,17 @@
 // trainer_device.cpp — the graph schedule: one boosting round as a fixed
 // launch sequence driven by the device-side grower, captured in a hipGraph.
 //
-// Round = fill(absmax) -> gradients -> qscale -> init rows (per tree) ->
-// controller init -> root histogram + split -> steps x (plan, partition,
-// children, zero, histogram, sums, subtract, split[, advance]) -> finalize
-// -> partition-range prediction update -> copy of the tree arrays to pinned
-// host memory.  No host synchronisation happens inside a round; the only one
-// is after the graph launch, to read the grown trees (and the metric).
+// Round = gradients -> qscale -> controller init -> rows + root totals ->
+// root histogram -> finish -> split -> steps x (plan, one-pass partition into
+// the other buffer of the idx/ogh pair, children, histogram, finish, split)
+// -> finalize -> partition-range prediction update -> one copy of the
+// round's trees to pinned host memory.  No host synchronisation happens
+// inside a round; the only one is after the graph launch, to read the trees.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 
 #include "grow_device.h"
@@ -18,51 +21,44 @@ namespace bst {
 
 namespace {
 constexpr uint32_t kHistTileD = 8192;
-constexpr uint32_t kPartTileD = 4096;
+constexpr uint32_t kPartTileD = kFusedTileRows;
 constexpr uint32_t kSegTileD = 16384;
+constexpr int kMaxGrid = 2048;
 
-template <class T>
-T* pinned(size_t n) {
-  T* p = nullptr;
-  BST_HIP(hipHostMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
-  return p;
+int grid_for(uint32_t cap, int max_grid) {
+  return (int)std::max<uint32_t>(1, std::min<uint32_t>(cap, (uint32_t)max_grid));
 }
 }  // namespace
 
 struct Trainer::DevState {
   CtlParams P{};
   DeviceBufs B{};
+  TreeOut O{};
   int steps = 0;
-  uint32_t cap_items = 0, cap_tiles = 0, cap_hist = 0, cap_zero = 0, cap_quads = 0,
-           cap_segs = 0, cap_split = 0, cap_lsegs = 0;
-  DevBuf<TreeCtl> ctl;
+  uint32_t cap_items = 0, cap_tiles = 0, cap_hist = 0, cap_fin = 0, cap_split = 0,
+           cap_lsegs = 0;
+  size_t out_bytes = 0;
   DevBuf<uint32_t> open_node, open_tnode, open_depth, next_node, next_tnode, next_depth, ch_node,
-      ch_tnode, ch_depth, pbegin, pslot, free_slot;
-  DevBuf<uint32_t> tr_split, tr_left, tr_right, tr_cats;
-  DevBuf<float> tr_thr, tr_leaf, tr_gain, tr_cover;
-  DevBuf<uint8_t> tr_dleft, tr_isleaf, tr_stype;
+      ch_tnode, ch_depth, pbegin, pslot, free_slot, cursor;
+  DevBuf<uint8_t> pbuf;
+  DevBuf<char> out;  // TreeOut arrays, back to back
   DevBuf<PartItem> items;
   DevBuf<PartTile> tiles;
   DevBuf<uint32_t> pend;
   DevBuf<HistItem> hitems;
-  DevBuf<uint32_t> zero, quads, squads, split_nodes, split_slots, counts, rows, eff_to_orig;
-  DevBuf<Segment> segs, leaf_segs;
+  DevBuf<uint32_t> fin, split_nodes, split_slots, rows, eff_to_orig;
+  DevBuf<Segment> leaf_segs;
   DevBuf<float> thr_up;
-  // Pinned mirrors of the round's results.
-  TreeCtl* h_ctl = nullptr;
-  uint32_t *h_split = nullptr, *h_left = nullptr, *h_right = nullptr, *h_cats = nullptr,
-           *h_counts = nullptr;
-  float *h_thr = nullptr, *h_leaf = nullptr, *h_gain = nullptr, *h_cover = nullptr;
-  uint8_t *h_dleft = nullptr, *h_isleaf = nullptr, *h_stype = nullptr;
+  DevBuf<long long> partial;    // one histogram group per multi-tile item
+  DevBuf<long long> tile_sums;  // [tiles][4] partition totals
+  char* h_out = nullptr;      // pinned mirror of `out`
+  TreeOut H{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~DevState() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (void* p : {(void*)h_ctl, (void*)h_split, (void*)h_left, (void*)h_right, (void*)h_cats,
-                    (void*)h_counts, (void*)h_thr, (void*)h_leaf, (void*)h_gain, (void*)h_cover,
-                    (void*)h_dleft, (void*)h_isleaf, (void*)h_stype})
-      if (p) (void)hipHostFree(p);
+    if (h_out) (void)hipHostFree(h_out);
   }
 };
 
@@ -95,8 +91,10 @@ void Trainer::device_setup() {
   P.E = (uint32_t)ds_->d.E;
   P.FG = (uint32_t)ds_->d.FG;
   P.hist_tile = kHistTileD;
+  if (const char* e = std::getenv("BST_HIST_TILE")) P.hist_tile = (uint32_t)std::max(512, std::atoi(e));
   P.part_tile = kPartTileD;
   P.seg_tile = kSegTileD;
+  P.rightmost = (uint32_t)sp_.rightmost;
   P.stride = n_;
   P.lr = lr_;
   P.lambda = (double)c.lambda;
@@ -106,55 +104,53 @@ void Trainer::device_setup() {
   st->steps = P.growth == 0 ? (int)c.max_depth : (int)(c.max_leaves > 0 ? c.max_leaves - 1 : 0);
   const uint32_t T = P.T;
   const uint32_t tiles_n = (uint32_t)((n_ + kPartTileD - 1) / kPartTileD);
-  const uint32_t hist_n = (uint32_t)((n_ + kHistTileD - 1) / kHistTileD);
+  const uint32_t hist_n = (uint32_t)((n_ + P.hist_tile - 1) / P.hist_tile);
   const uint32_t segs_n = (uint32_t)((n_ + kSegTileD - 1) / kSegTileD);
+  // Worst cases per step: every tree expands W nodes whose rows sum to n.
   st->cap_items = T * P.W;
   st->cap_tiles = T * (tiles_n + P.W);
   st->cap_hist = T * (hist_n + P.W) * P.NG;
-  st->cap_zero = T * (P.W + 1);
-  st->cap_quads = T * P.W;
-  st->cap_segs = T * (segs_n + P.W);
+  st->cap_fin = std::max<uint32_t>(T, T * P.W);
   st->cap_split = std::max<uint32_t>(T, 2 * T * P.W);
   st->cap_lsegs = T * (segs_n + MN_);
   auto& s = *st;
-  s.ctl.alloc(T);
   for (auto* b : {&s.open_node, &s.open_tnode, &s.open_depth}) b->alloc((size_t)T * P.WO);
   for (auto* b : {&s.next_node, &s.next_tnode, &s.next_depth}) b->alloc((size_t)T * 2 * P.W);
   for (auto* b : {&s.ch_node, &s.ch_tnode, &s.ch_depth}) b->alloc((size_t)T * P.W);
   s.pbegin.alloc((size_t)T * MN_);
   s.pslot.alloc((size_t)T * MN_);
   s.free_slot.alloc((size_t)T * SPT_);
-  for (auto* b : {&s.tr_split, &s.tr_left, &s.tr_right}) b->alloc((size_t)T * P.MNT);
-  s.tr_cats.alloc((size_t)T * P.MNT * kMaxCatWords);
-  for (auto* b : {&s.tr_thr, &s.tr_leaf, &s.tr_gain, &s.tr_cover}) b->alloc((size_t)T * P.MNT);
-  for (auto* b : {&s.tr_dleft, &s.tr_isleaf, &s.tr_stype}) b->alloc((size_t)T * P.MNT);
+  s.pbuf.alloc((size_t)T * MN_);
+  s.cursor.alloc((size_t)2 * s.cap_items);
+  s.tile_sums.alloc((size_t)4 * s.cap_tiles);
+  s.out_bytes = tree_out_layout(T, P.MNT, nullptr, nullptr);
+  s.out.alloc(s.out_bytes);
+  tree_out_layout(T, P.MNT, s.out.p, &s.O);
+  BST_HIP(hipHostMalloc((void**)&s.h_out, s.out_bytes, hipHostMallocDefault));
+  tree_out_layout(T, P.MNT, s.h_out, &s.H);
   s.items.alloc(s.cap_items);
   s.tiles.alloc(s.cap_tiles);
   s.pend.alloc((size_t)5 * s.cap_items);
   s.hitems.alloc(s.cap_hist);
-  s.zero.alloc(s.cap_zero);
-  s.quads.alloc((size_t)4 * s.cap_quads);
-  s.squads.alloc((size_t)4 * s.cap_quads);
+  s.fin.alloc((size_t)4 * s.cap_fin);
   s.split_nodes.alloc(s.cap_split);
   s.split_slots.alloc(s.cap_split);
-  s.counts.alloc(C_NCOUNTS);
   s.rows.alloc(T);
-  s.segs.alloc(s.cap_segs);
   s.leaf_segs.alloc(s.cap_lsegs);
+  s.partial.alloc((size_t)s.cap_hist * 256 * P.FG * 2);
   s.thr_up.alloc(ds_->thr_up.size());
   h2d(s_, s.thr_up.p, ds_->thr_up.data(), ds_->thr_up.size());
   s.eff_to_orig.alloc(ds_->eff_to_orig.size());
   h2d(s_, s.eff_to_orig.p, ds_->eff_to_orig.data(), ds_->eff_to_orig.size());
-  tile_left_.ensure(s.cap_tiles);
-  tile_off_.ensure(s.cap_tiles);
-  item_left_.ensure(s.cap_items);
   std::vector<uint32_t> rows(T, (uint32_t)n_);
   h2d(s_, s.rows.p, rows.data(), rows.size());
   std::vector<uint32_t> outs(T);
   for (uint32_t t = 0; t < T; t++) outs[t] = t;
   h2d(s_, tree_output_.p, outs.data(), outs.size());
+  // The captured round expects absmax zeroed (each round's qscale re-zeroes it).
+  launch_fill_u32(s_, absmax_.p, 0u, (int64_t)absmax_.n);
   DeviceBufs& B = s.B;
-  B.ctl = s.ctl.p;
+  B.ctl = s.O.ctl;
   B.open_node = s.open_node.p;
   B.open_tnode = s.open_tnode.p;
   B.open_depth = s.open_depth.p;
@@ -166,52 +162,39 @@ void Trainer::device_setup() {
   B.ch_depth = s.ch_depth.p;
   B.pbegin = s.pbegin.p;
   B.pslot = s.pslot.p;
+  B.pbuf = s.pbuf.p;
   B.free_slot = s.free_slot.p;
   B.leaf_val = leaf_val_.p;
-  B.tr_split = s.tr_split.p;
-  B.tr_left = s.tr_left.p;
-  B.tr_right = s.tr_right.p;
-  B.tr_cats = s.tr_cats.p;
-  B.tr_thr = s.tr_thr.p;
-  B.tr_leaf = s.tr_leaf.p;
-  B.tr_gain = s.tr_gain.p;
-  B.tr_cover = s.tr_cover.p;
-  B.tr_dleft = s.tr_dleft.p;
-  B.tr_isleaf = s.tr_isleaf.p;
-  B.tr_stype = s.tr_stype.p;
+  B.tr_split = s.O.tr_split;
+  B.tr_left = s.O.tr_left;
+  B.tr_right = s.O.tr_right;
+  B.tr_cats = s.O.tr_cats;
+  B.tr_thr = s.O.tr_thr;
+  B.tr_leaf = s.O.tr_leaf;
+  B.tr_gain = s.O.tr_gain;
+  B.tr_cover = s.O.tr_cover;
+  B.tr_dleft = s.O.tr_dleft;
+  B.tr_isleaf = s.O.tr_isleaf;
+  B.tr_stype = s.O.tr_stype;
   B.items = s.items.p;
   B.tiles = s.tiles.p;
   B.pend = s.pend.p;
   B.hitems = s.hitems.p;
-  B.zero = s.zero.p;
-  B.quads = s.quads.p;
-  B.squads = s.squads.p;
+  B.fin = s.fin.p;
   B.split_nodes = s.split_nodes.p;
   B.split_slots = s.split_slots.p;
-  B.segs = s.segs.p;
   B.leaf_segs = s.leaf_segs.p;
-  B.counts = s.counts.p;
+  B.counts = s.O.counts;
   B.node_sum = node_sum_.p;
   B.node_count = node_count_.p;
   B.node_split = node_split_.p;
+  B.node_key = node_key_.p;
+  B.cand = cand_.p;
   B.thr_up = s.thr_up.p;
   B.eff_to_orig = s.eff_to_orig.p;
   B.qexp = qexp_.p;
-  B.item_left = item_left_.p;
-  const size_t tn = (size_t)T * P.MNT;
-  s.h_ctl = pinned<TreeCtl>(T);
-  s.h_split = pinned<uint32_t>(tn);
-  s.h_left = pinned<uint32_t>(tn);
-  s.h_right = pinned<uint32_t>(tn);
-  s.h_cats = pinned<uint32_t>(tn * kMaxCatWords);
-  s.h_counts = pinned<uint32_t>(C_NCOUNTS);
-  s.h_thr = pinned<float>(tn);
-  s.h_leaf = pinned<float>(tn);
-  s.h_gain = pinned<float>(tn);
-  s.h_cover = pinned<float>(tn);
-  s.h_dleft = pinned<uint8_t>(tn);
-  s.h_isleaf = pinned<uint8_t>(tn);
-  s.h_stype = pinned<uint8_t>(tn);
+  B.cursor = s.cursor.p;
+  B.tile_sums = s.tile_sums.p;
   BST_HIP(hipStreamSynchronize(s_));
   dev_ = std::move(st);
 }
@@ -224,85 +207,72 @@ void Trainer::device_enqueue_round(const float* y, const float* w) {
   const auto& cfg = opt_.cfg;
   const int K = opt_.K;
   const auto& d = ds_->d;
-  launch_fill_u32(s_, absmax_.p, 0u, 2 * K);
+  const int g_hist = grid_for(S.cap_hist, kMaxGrid);
+  const int g_tiles = grid_for(S.cap_tiles, kMaxGrid);
+  const int g_split = grid_for(S.cap_split, 512);
+  const int g_lsegs = grid_for(S.cap_lsegs, kMaxGrid);
+  uint32_t* cnt = B.counts;
+  const int n_feats = (int)feats_h_.size();
   launch_gradients(s_, cfg.objective, preds_.p, y, w, n_, K, g_.p, h_.p, absmax_.p);
   launch_qscale(s_, absmax_.p, K, n_, qexp_.p);
-  for (uint32_t t = 0; t < P.T; t++)
-    launch_init_rows(s_, idx_.p + t * n_, ogh_.p + t * n_, g_.p + (size_t)t * n_,
-                     h_.p + (size_t)t * n_, nullptr, n_);
-  const int g_hist = (int)std::min<uint32_t>(S.cap_hist, 2048);
-  const int g_tiles = (int)std::min<uint32_t>(S.cap_tiles, 2048);
-  const int g_items = (int)std::min<uint32_t>(S.cap_items, 1024);
-  const int g_quads = (int)std::min<uint32_t>(std::max<uint32_t>(S.cap_quads, 1), 256);
-  const int g_zero = (int)std::min<uint32_t>(S.cap_zero, 256);
-  const int g_segs = (int)std::min<uint32_t>(std::max<uint32_t>(S.cap_segs, 1), 1024);
-  const int g_split = (int)std::min<uint32_t>(S.cap_split, 512);
-  const int g_lsegs = (int)std::min<uint32_t>(S.cap_lsegs, 2048);
-  uint32_t* cnt = S.counts.p;
   launch_ctl_init(s_, P, B, S.rows.p);
-  launch_zero_slots(s_, B.zero, g_zero, hist_.p, slot_elems_, cnt + C_ZERO);
-  int pi = ctx_->prof_begin(0, 0.0);
-  launch_hist(s_, d, B.hitems, g_hist, idx_.p, ogh_.p, 0, qexp_.p, hist_.p, slot_elems_,
-              node_sum_.p, P.T > 1, 0, cnt + C_HIST);
-  ctx_->prof_end(pi);
-  pi = ctx_->prof_begin(1, 0.0);
-  launch_split(s_, d, B.split_nodes, B.split_slots, g_split, feats_.p, (int)feats_h_.size(),
-               hist_.p, slot_elems_, node_sum_.p, node_count_.p, qexp_.p, MN_, sp_, node_key_.p,
-               cand_.p, node_split_.p, cnt + C_SPLIT);
-  ctx_->prof_end(pi);
+  launch_init_rows_sums(s_, idx_.p, ogh_.p, g_.p, h_.p, n_, (int)P.T, qexp_.p, MN_, node_sum_.p);
+  auto hist_step = [&](bool gather) {
+    int pi = ctx_->prof_begin(0, 0.0);
+    launch_hist(s_, d, B.hitems, g_hist, idx_.p, ogh_.p, 0, qexp_.p, hist_.p, slot_elems_,
+                nullptr, gather, 0, cnt + C_HIST, S.partial.p, idx_tmp_.p, ogh_tmp_.p);
+    ctx_->prof_end(pi);
+    launch_hist_finish(s_, B.fin, (int)S.cap_fin, (int)P.NG, (int)P.FG, S.partial.p, hist_.p,
+                       slot_elems_, cnt + C_FIN);
+    pi = ctx_->prof_begin(1, 0.0);
+    launch_split(s_, d, B.split_nodes, B.split_slots, g_split, feats_.p, n_feats, hist_.p,
+                 slot_elems_, node_sum_.p, node_count_.p, qexp_.p, MN_, sp_, node_key_.p,
+                 cand_.p, node_split_.p, cnt + C_SPLIT, /*finalize=*/false);
+    ctx_->prof_end(pi);
+  };
+  // Positions of tree slot t start at t * n, so only slot 0 may skip the gather.
+  hist_step(P.T > 1);
   for (int step = 0; step < S.steps; step++) {
-    launch_ctl_plan(s_, P, B);
-    pi = ctx_->prof_begin(2, 0.0);
-    launch_partition(s_, d, B.items, g_items, B.tiles, g_tiles, node_split_.p, idx_.p, ogh_.p,
-                     idx_tmp_.p, ogh_tmp_.p, flag_.p, tile_left_.p, tile_off_.p, item_left_.p,
-                     node_count_.p, node_sum_.p, cnt + C_ITEMS, cnt + C_TILES);
+    launch_ctl_plan(s_, P, B, /*advance=*/step > 0);
+    int pi = ctx_->prof_begin(2, 0.0);
+    launch_partition_fused(s_, d, B.items, B.tiles, g_tiles, node_split_.p, idx_.p, ogh_.p,
+                           idx_tmp_.p, ogh_tmp_.p, B.cursor, qexp_.p, MN_, S.tile_sums.p,
+                           cnt + C_TILES);
     ctx_->prof_end(pi);
     launch_ctl_children(s_, P, B);
-    launch_zero_slots(s_, B.zero, g_zero, hist_.p, slot_elems_, cnt + C_ZERO);
-    pi = ctx_->prof_begin(0, 0.0);
-    launch_hist(s_, d, B.hitems, g_hist, idx_.p, ogh_.p, 0, qexp_.p, hist_.p, slot_elems_,
-                node_sum_.p, true, 0, cnt + C_HIST);
-    ctx_->prof_end(pi);
-    launch_node_sums(s_, B.segs, g_segs, ogh_.p, 0, qexp_.p, node_sum_.p, cnt + C_SEGS);
-    launch_subtract(s_, B.quads, g_quads, hist_.p, slot_elems_, node_sum_.p, cnt + C_QUADS);
-    launch_subtract(s_, B.squads, g_quads, hist_.p, 0, node_sum_.p, cnt + C_SQUADS);
-    pi = ctx_->prof_begin(1, 0.0);
-    launch_split(s_, d, B.split_nodes, B.split_slots, g_split, feats_.p, (int)feats_h_.size(),
-                 hist_.p, slot_elems_, node_sum_.p, node_count_.p, qexp_.p, MN_, sp_,
-                 node_key_.p, cand_.p, node_split_.p, cnt + C_SPLIT);
-    ctx_->prof_end(pi);
-    if (P.growth == 0) launch_ctl_advance(s_, P, B);
+    // The last step's children are never split (depth limit, or the leaf
+    // budget is spent), so their histograms are not built.
+    if (step + 1 < S.steps) hist_step(true);
   }
-  launch_ctl_finalize(s_, P, B);
+  launch_ctl_finalize(s_, P, B, /*advance=*/S.steps > 0);
   launch_leaf_update(s_, B.leaf_segs, g_lsegs, leaf_val_.p, idx_.p, 0, preds_.p,
-                     tree_output_.p, n_, cnt + C_LSEGS);
-  const size_t tn = (size_t)P.T * P.MNT;
-  d2h(s_, S.h_ctl, S.ctl.p, P.T);
-  d2h(s_, S.h_counts, S.counts.p, C_NCOUNTS);
-  d2h(s_, S.h_split, S.tr_split.p, tn);
-  d2h(s_, S.h_left, S.tr_left.p, tn);
-  d2h(s_, S.h_right, S.tr_right.p, tn);
-  d2h(s_, S.h_cats, S.tr_cats.p, tn * kMaxCatWords);
-  d2h(s_, S.h_thr, S.tr_thr.p, tn);
-  d2h(s_, S.h_leaf, S.tr_leaf.p, tn);
-  d2h(s_, S.h_gain, S.tr_gain.p, tn);
-  d2h(s_, S.h_cover, S.tr_cover.p, tn);
-  d2h(s_, S.h_dleft, S.tr_dleft.p, tn);
-  d2h(s_, S.h_isleaf, S.tr_isleaf.p, tn);
-  d2h(s_, S.h_stype, S.tr_stype.p, tn);
+                     tree_output_.p, n_, cnt + C_LSEGS, idx_tmp_.p);
+  d2h(s_, S.h_out, S.out.p, S.out_bytes);
 }
 
 // Launch (or capture and launch) one round, wait, and return the trees.
 void Trainer::device_round(const float* y, const float* w, std::vector<TreeBuild>& out) {
+  using clk = std::chrono::steady_clock;
+  static const bool trace = std::getenv("BST_TRACE_SETUP") != nullptr;
+  const auto t0 = clk::now();
   if (!dev_) device_setup();
   DevState& S = *dev_;
+  const auto t1 = clk::now();
   const bool use_graph = !ctx_->profile;  // profiled runs time each launch with events
   if (use_graph) {
     if (!S.exec) {
       BST_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
       device_enqueue_round(y, w);
       BST_HIP(hipStreamEndCapture(s_, &S.graph));
+      const auto t2 = clk::now();
       BST_HIP(hipGraphInstantiate(&S.exec, S.graph, nullptr, nullptr, 0));
+      if (trace) {
+        const auto t3 = clk::now();
+        std::fprintf(stderr, "[bst] device setup %.3f ms, capture %.3f ms, instantiate %.3f ms\n",
+                     std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                     std::chrono::duration<double, std::milli>(t2 - t1).count(),
+                     std::chrono::duration<double, std::milli>(t3 - t2).count());
+      }
     }
     BST_HIP(hipGraphLaunch(S.exec, s_));
   } else {
@@ -310,10 +280,11 @@ void Trainer::device_round(const float* y, const float* w, std::vector<TreeBuild
   }
   BST_HIP(hipStreamSynchronize(s_));
   BST_HIP(hipGetLastError());
+  const TreeOut& H = S.H;
   if (ctx_->profile) {
     // Algorithmic bytes of the round's histogram launches, spread evenly.
-    const double rows = (double)S.h_counts[C_HROWS];
-    const int nodes = (int)S.h_counts[C_HNODES];
+    const double rows = (double)H.counts[C_HROWS];
+    const int nodes = (int)H.counts[C_HNODES];
     const double bytes = hist_bytes(rows, nodes, true);
     int nh = 0;
     for (auto& p : ctx_->pending)
@@ -322,27 +293,27 @@ void Trainer::device_round(const float* y, const float* w, std::vector<TreeBuild
       if (p.kind == 0) p.bytes = bytes / std::max(nh, 1);
     ctx_->prof_drain();
   }
-  stats.hist_rows += S.h_counts[C_HROWS];
+  stats.hist_rows += H.counts[C_HROWS];
   const CtlParams& P = S.P;
   out.assign(P.T, TreeBuild());
   for (uint32_t t = 0; t < P.T; t++) {
     TreeBuild& tb = out[t];
-    const uint32_t nn = S.h_ctl[t].n_tnodes;
+    const uint32_t nn = H.ctl[t].n_tnodes;
     const size_t o = (size_t)t * P.MNT;
     for (uint32_t i = 0; i < nn; i++) {
       tb.alloc();
-      tb.split_index[i] = S.h_split[o + i];
-      tb.left[i] = S.h_left[o + i];
-      tb.right[i] = S.h_right[o + i];
-      tb.thr[i] = S.h_thr[o + i];
-      tb.leaf[i] = S.h_isleaf[o + i] ? S.h_leaf[o + i] : 0.0f;
-      tb.gain[i] = S.h_gain[o + i];
-      tb.cover[i] = S.h_cover[o + i];
-      tb.dleft[i] = S.h_dleft[o + i];
-      tb.is_leaf[i] = S.h_isleaf[o + i];
-      tb.stype[i] = S.h_stype[o + i];
+      tb.split_index[i] = H.tr_split[o + i];
+      tb.left[i] = H.tr_left[o + i];
+      tb.right[i] = H.tr_right[o + i];
+      tb.thr[i] = H.tr_thr[o + i];
+      tb.leaf[i] = H.tr_isleaf[o + i] ? H.tr_leaf[o + i] : 0.0f;
+      tb.gain[i] = H.tr_gain[o + i];
+      tb.cover[i] = H.tr_cover[o + i];
+      tb.dleft[i] = H.tr_dleft[o + i];
+      tb.is_leaf[i] = H.tr_isleaf[o + i];
+      tb.stype[i] = H.tr_stype[o + i];
       if (!tb.is_leaf[i] && tb.stype[i] == 1) {
-        const uint32_t* c = S.h_cats + (o + i) * kMaxCatWords;
+        const uint32_t* c = H.tr_cats + (o + i) * kMaxCatWords;
         int maxc = -1;
         for (int k = 0; k < 256; k++)
           if ((c[k >> 5] >> (k & 31)) & 1u) maxc = k;
