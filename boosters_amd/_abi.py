@@ -266,6 +266,11 @@ def _declare(lib):
             [vp, C.c_int32, C.c_int32, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int64,
              P(C.c_float), P(C.c_float)],
         ),
+        "bst_kernel_lambdarank": (
+            C.c_int,
+            [vp, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int64, P(C.c_int32), C.c_int64,
+             P(C.c_float), P(C.c_float)],
+        ),
         "bst_kernel_histogram": (
             C.c_int,
             [vp, vp, P(C.c_float), P(C.c_float), P(C.c_uint32), C.c_int64, P(C.c_double), C.c_int64],
@@ -305,7 +310,7 @@ def exported_symbols():
         "bst_dataset_get_bins", "bst_train", "bst_train_device", "bst_last_train_stats",
         "bst_last_train_metric", "bst_model_info_get", "bst_model_export", "bst_model_import",
         "bst_model_destroy", "bst_predict", "bst_predict_device", "bst_server_create",
-        "bst_server_submit", "bst_server_wait", "bst_server_destroy", "bst_kernel_gradients",
+        "bst_server_submit", "bst_server_wait", "bst_server_destroy", "bst_kernel_gradients", "bst_kernel_lambdarank",
         "bst_kernel_histogram", "bst_dataset_hist_layout", "bst_kernel_partition",
         "bst_kernel_find_split",
     ]

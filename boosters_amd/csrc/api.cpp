@@ -158,6 +158,7 @@ bst_dataset* make_dataset(bst_ctx* ctx, const float* X, int64_t n, int32_t F,
 }
 
 int default_metric_of(int obj) {
+  if (obj == BST_OBJ_LAMBDARANK) return BST_METRIC_NONE;  // no ranking metric in the reference
   return obj == BST_OBJ_LOGISTIC ? BST_METRIC_LOGLOSS
                                  : (obj == BST_OBJ_SOFTMAX ? BST_METRIC_MLOGLOSS : BST_METRIC_RMSE);
 }
@@ -176,7 +177,7 @@ int validate(const bst_gbdt_config* c) {
   if (c->alpha < 0.0f) raise(BST_E_CONFIG, "alpha must be non-negative");
   if (c->min_child_weight < 0.0f) raise(BST_E_CONFIG, "min_child_weight must be non-negative");
   if (c->min_gain < 0.0f) raise(BST_E_CONFIG, "min_gain must be non-negative");
-  if (c->objective < 0 || c->objective > BST_OBJ_SOFTMAX)
+  if (c->objective < 0 || c->objective > BST_OBJ_LAMBDARANK)
     raise(BST_E_CONFIG, "objective not supported by this engine build");
   if (c->objective == BST_OBJ_SOFTMAX && (c->n_classes < 2 || c->n_classes > kPredMaxGroups))
     raise(BST_E_CONFIG, "softmax needs 2..16 classes");
@@ -501,6 +502,26 @@ static int train_impl(bst_ctx* ctx, bst_dataset* ds, const float* y_dev, int64_t
   bst_ext_config_default(&opt.ext);
   if (ext) opt.ext = *ext;
   opt.K = cfg->objective == BST_OBJ_SOFTMAX ? (int)cfg->n_classes : 1;
+  if (cfg->objective == BST_OBJ_LAMBDARANK) {
+    // Query groups: ragged, in row order, summing to the row count.
+    need(opt.ext.query_group_sizes && opt.ext.n_query_groups >= 1, BST_E_CONFIG,
+         "LambdaRank needs query_group_sizes");
+    opt.group_off.assign((size_t)opt.ext.n_query_groups + 1, 0);
+    int64_t max_group = 1;
+    for (int64_t q = 0; q < opt.ext.n_query_groups; q++) {
+      const int32_t sz = opt.ext.query_group_sizes[q];
+      need(sz >= 1, BST_E_CONFIG, "empty query group");
+      need(sz <= kRankMaxGroupRows, BST_E_CONFIG, "query group larger than 4096 rows");
+      opt.group_off[q + 1] = opt.group_off[q] + sz;
+      max_group = std::max<int64_t>(max_group, sz);
+    }
+    need(opt.group_off.back() == ds->d.n, BST_E_SHAPE,
+         "query group sizes must sum to the row count");
+    opt.rank_disc.resize((size_t)max_group);
+    for (size_t r = 0; r < opt.rank_disc.size(); r++)
+      opt.rank_disc[r] = 1.0 / std::log2(2.0 + (double)r);
+    opt.ext.query_group_sizes = nullptr;  // not kept past this call
+  }
   const auto tc0 = std::chrono::steady_clock::now();
   Trainer tr(ctx, ds, opt);
   if (std::getenv("BST_TRACE_SETUP")) {
@@ -714,6 +735,45 @@ int bst_predict_device(bst_ctx* ctx, const bst_model* m, const float* Xd, int64_
 }
 
 // ---- component entry points ------------------------------------------------
+int bst_kernel_lambdarank(bst_ctx* ctx, const float* preds, const float* y, const float* w,
+                          int64_t n, const int32_t* sizes, int64_t n_groups, float* g, float* h) {
+  return guard([&] {
+    need(ctx && preds && y && g && h && sizes, BST_E_INVALID, "null argument");
+    need(n_groups >= 1, BST_E_CONFIG, "no query groups");
+    std::vector<int64_t> off((size_t)n_groups + 1, 0);
+    int64_t max_group = 1;
+    for (int64_t q = 0; q < n_groups; q++) {
+      need(sizes[q] >= 1 && sizes[q] <= kRankMaxGroupRows, BST_E_CONFIG,
+           "query group sizes must be 1..4096");
+      off[q + 1] = off[q] + sizes[q];
+      max_group = std::max<int64_t>(max_group, sizes[q]);
+    }
+    need(off.back() == n, BST_E_SHAPE, "query group sizes must sum to n");
+    std::vector<double> disc((size_t)max_group);
+    for (size_t r = 0; r < disc.size(); r++) disc[r] = 1.0 / std::log2(2.0 + (double)r);
+    set_device(ctx);
+    hipStream_t s = ctx->stream;
+    DevBuf<float> dp(n), dy(n), dw, dg(n), dh(n);
+    DevBuf<int64_t> doff(off.size());
+    DevBuf<double> ddisc(disc.size());
+    DevBuf<uint32_t> am(2);
+    h2d(s, dp.p, preds, n);
+    h2d(s, dy.p, y, n);
+    if (w) {
+      dw.alloc(n);
+      h2d(s, dw.p, w, n);
+    }
+    h2d(s, doff.p, off.data(), off.size());
+    h2d(s, ddisc.p, disc.data(), disc.size());
+    BST_HIP(hipMemsetAsync(am.p, 0, am.bytes(), s));
+    launch_lambdarank(s, dp.p, dy.p, dw.p, doff.p, n_groups, ddisc.p, dg.p, dh.p, am.p);
+    d2h(s, g, dg.p, n);
+    d2h(s, h, dh.p, n);
+    BST_HIP(hipStreamSynchronize(s));
+    return BST_OK;
+  });
+}
+
 int bst_kernel_gradients(bst_ctx* ctx, int32_t obj, int32_t K, const float* preds,
                          const float* y, const float* w, int64_t n, float* g, float* h) {
   return guard([&] {

@@ -110,6 +110,13 @@ void launch_to_rowmajor(hipStream_t s, const DatasetDev& d);
 
 void launch_gradients(hipStream_t s, int objective, const float* preds, const float* y,
                       const float* w, int64_t n, int K, float* g, float* h, uint32_t* absmax);
+// LambdaRank gradients of one output over ragged query groups
+// (group_off[n_groups + 1] row offsets, each group <= 4096 rows); disc[r] =
+// 1 / log2(2 + r) for r below the largest group; absmax as launch_gradients.
+constexpr int kRankMaxGroupRows = 4096;
+void launch_lambdarank(hipStream_t s, const float* preds, const float* y, const float* w,
+                       const int64_t* group_off, int64_t n_groups, const double* disc, float* g,
+                       float* h, uint32_t* absmax);
 // Reads absmax[0..2K) and resets it to zero for the next round.
 void launch_qscale(hipStream_t s, uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp);
 

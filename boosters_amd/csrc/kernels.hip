@@ -303,6 +303,145 @@ void launch_gradients(hipStream_t s, int objective, const float* preds, const fl
                      preds, y, w, n, K, g, h, absmax);
 }
 
+// ---------------------------------------------------------------------------
+// LambdaRank over ragged query groups (no counterpart in the reference; the
+// published LambdaMART formulation with sigma = 1 and pair truncation):
+// one block per group.  Documents are ranked by score descending, ties by
+// index (bitonic sort of (score, index) keys in LDS); a pair of positions
+// (a, b) counts when the labels differ and min(a, b) < kRankTrunc;
+//   rho = 1 / (1 + exp(s_hi - s_lo))                         (f32)
+//   dNDCG = (2^l_hi - 2^l_lo) |disc[a] - disc[b]| / maxDCG    (f64)
+// the higher-labelled document gets -rho dNDCG, the other +rho dNDCG, both
+// rho (1 - rho) dNDCG.  Each document sums its pairs in f64 in position
+// order (one thread per position), so the result does not depend on the
+// launch shape; disc[r] = 1 / log2(2 + r) is a host-computed table.
+constexpr int kRankTrunc = 30;
+constexpr int kRankMaxGroup = 4096;
+constexpr int kRankThreads = 256;
+
+__device__ __forceinline__ uint32_t ord_f32_asc(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ __launch_bounds__(kRankThreads) void k_lambdarank(
+    const float* __restrict__ preds, const float* __restrict__ y, const float* __restrict__ w,
+    const int64_t* __restrict__ off, int64_t n_groups, const double* __restrict__ disc,
+    float* __restrict__ g, float* __restrict__ h, uint32_t* __restrict__ absmax) {
+  __shared__ unsigned long long s_key[kRankMaxGroup];
+  __shared__ float s_s[kRankMaxGroup];
+  __shared__ uint8_t s_l[kRankMaxGroup];
+  __shared__ uint32_t s_cnt[32];
+  __shared__ double s_inv;
+  __shared__ uint32_t s_max[2][kRankThreads / WAVE];
+  uint32_t mg = 0u, mh = 0u;
+  for (int64_t q = blockIdx.x; q < n_groups; q += gridDim.x) {
+    const int64_t b0 = off[q];
+    const int m = (int)(off[q + 1] - b0);
+    int P = 1;
+    while (P < m) P <<= 1;
+    if (threadIdx.x < 32) s_cnt[threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+      unsigned long long key = ~0ull;
+      if (i < m) {
+        const float sv = preds[b0 + i] + 0.0f;  // -0 ranks with +0
+        int l = (int)y[b0 + i];
+        l = l < 0 ? 0 : (l > 31 ? 31 : l);
+        s_s[i] = sv;
+        s_l[i] = (uint8_t)l;
+        key = ((unsigned long long)(~ord_f32_asc(sv)) << 32) | (uint32_t)i;
+      }
+      s_key[i] = key;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += blockDim.x) atomicAdd(&s_cnt[s_l[i]], 1u);
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+          const int ixj = i ^ j;
+          if (ixj > i) {
+            const unsigned long long a = s_key[i], b = s_key[ixj];
+            const bool up = (i & k) == 0;
+            if ((a > b) == up) {
+              s_key[i] = b;
+              s_key[ixj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      // maxDCG@T: labels in descending order, ranks ascending.
+      double md = 0.0;
+      int r = 0;
+      const int lim = m < kRankTrunc ? m : kRankTrunc;
+      for (int l = 31; l >= 0 && r < lim; l--)
+        for (uint32_t c = 0; c < s_cnt[l] && r < lim; c++, r++)
+          md += (ldexp(1.0, l) - 1.0) * disc[r];
+      s_inv = md > 0.0 ? 1.0 / md : 0.0;
+    }
+    __syncthreads();
+    const double inv = s_inv;
+    for (int a = threadIdx.x; a < m; a += blockDim.x) {
+      const uint32_t i = (uint32_t)(s_key[a] & 0xFFFFFFFFull);
+      double ga = 0.0, ha = 0.0;
+      if (inv > 0.0) {
+        const int li = s_l[i];
+        const float si = s_s[i];
+        const int bend = a < kRankTrunc ? m : (m < kRankTrunc ? m : kRankTrunc);
+        for (int b = 0; b < bend; b++) {
+          if (b == a) continue;
+          const uint32_t j = (uint32_t)(s_key[b] & 0xFFFFFFFFull);
+          const int lj = s_l[j];
+          if (li == lj) continue;
+          const bool ih = li > lj;
+          const float sj = s_s[j];
+          const float shi = ih ? si : sj, slo = ih ? sj : si;
+          const float rho = 1.0f / (1.0f + exp32(shi - slo));
+          const double gap = ldexp(1.0, ih ? li : lj) - ldexp(1.0, ih ? lj : li);
+          const double dn = gap * fabs(disc[a] - disc[b]) * inv;
+          const double lam = (double)rho * dn;
+          ga += ih ? -lam : lam;
+          ha += (double)rho * (1.0 - (double)rho) * dn;
+        }
+      }
+      const double wi = w ? (double)w[b0 + i] : 1.0;
+      const float gv = (float)(ga * wi), hv = (float)(ha * wi);
+      g[b0 + i] = gv;
+      h[b0 + i] = hv;
+      mg = max(mg, abs_bits(gv));
+      mh = max(mh, abs_bits(hv));
+    }
+    __syncthreads();  // LDS is reused by the next group
+  }
+#pragma unroll
+  for (int m2 = WAVE / 2; m2 >= 1; m2 >>= 1) {
+    mg = max(mg, (uint32_t)__shfl_xor(mg, m2, WAVE));
+    mh = max(mh, (uint32_t)__shfl_xor(mh, m2, WAVE));
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    s_max[0][threadIdx.x / WAVE] = mg;
+    s_max[1][threadIdx.x / WAVE] = mh;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint32_t v = 0u;
+    for (int i = 0; i < kRankThreads / WAVE; i++) v = max(v, s_max[threadIdx.x][i]);
+    atomicMax(&absmax[threadIdx.x], v);
+  }
+}
+
+void launch_lambdarank(hipStream_t s, const float* preds, const float* y, const float* w,
+                       const int64_t* group_off, int64_t n_groups, const double* disc, float* g,
+                       float* h, uint32_t* absmax) {
+  if (n_groups <= 0) return;
+  const int64_t grid = n_groups < 2048 ? n_groups : 2048;
+  hipLaunchKernelGGL(k_lambdarank, dim3((unsigned)grid), dim3(kRankThreads), 0, s, preds, y, w,
+                     group_off, n_groups, disc, g, h, absmax);
+}
+
 // Scale exponent per (tree, g/h): largest e with n * max|x| * 2^e <= 2^62.
 __global__ void k_qscale(uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
   const int i = threadIdx.x;

@@ -87,6 +87,7 @@ void upload(hipStream_t s, DevBuf<T>& d, const std::vector<T>& h) {
 }
 
 int default_metric(int obj) {
+  if (obj == BST_OBJ_LAMBDARANK) return BST_METRIC_NONE;  // no ranking metric in the reference
   return obj == BST_OBJ_LOGISTIC ? BST_METRIC_LOGLOSS
                                  : (obj == BST_OBJ_SOFTMAX ? BST_METRIC_MLOGLOSS : BST_METRIC_RMSE);
 }
@@ -157,6 +158,21 @@ Trainer::Trainer(bst_ctx* ctx, bst_dataset* ds, const TrainOptions& opt)
   tree_output_.alloc(T_);
   feats_.alloc(ds->d.E);
   BST_HIP(hipMemsetAsync(node_key_.p, 0, node_key_.bytes(), s_));
+  if (c.objective == BST_OBJ_LAMBDARANK) {
+    group_off_.alloc(opt_.group_off.size());
+    h2d(s_, group_off_.p, opt_.group_off.data(), opt_.group_off.size());
+    rank_disc_.alloc(opt_.rank_disc.size());
+    h2d(s_, rank_disc_.p, opt_.rank_disc.data(), opt_.rank_disc.size());
+  }
+}
+
+void Trainer::launch_objective(const float* y, const float* w) {
+  const auto& cfg = opt_.cfg;
+  if (cfg.objective == BST_OBJ_LAMBDARANK)
+    launch_lambdarank(s_, preds_.p, y, w, group_off_.p, (int64_t)opt_.group_off.size() - 1,
+                      rank_disc_.p, g_.p, h_.p, absmax_.p);
+  else
+    launch_gradients(s_, cfg.objective, preds_.p, y, w, n_, opt_.K, g_.p, h_.p, absmax_.p);
 }
 
 uint32_t Trainer::alloc_slot(TreeState& ts) {
@@ -566,7 +582,9 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
                        .count());
     auto wt = [&](int64_t i) { return w ? wh[i] : 1.0f; };
     m->base_score.assign(K, 0.0f);
-    if (cfg.objective == BST_OBJ_SQUARED) {
+    if (cfg.objective == BST_OBJ_LAMBDARANK) {
+      // Scores start at 0 (ranking has no prior).
+    } else if (cfg.objective == BST_OBJ_SQUARED) {
       double sw = 0, swy = 0;
       for (int64_t i = 0; i < n_; i++) {
         sw += (double)wt(i);
@@ -710,7 +728,7 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
       }
     } else {
     BST_HIP(hipMemsetAsync(absmax_.p, 0, absmax_.bytes(), s_));
-    launch_gradients(s_, cfg.objective, preds_.p, y, w, n_, K, g_.p, h_.p, absmax_.p);
+    launch_objective(y, w);
     launch_qscale(s_, absmax_.p, K, n_, qexp_.p);
     for (int k0 = 0; k0 < K; k0 += T_) {
       std::vector<TreeState> trees(T_);

@@ -49,6 +49,10 @@ struct TrainOptions {
   bst_gbdt_config cfg;
   bst_ext_config ext;
   int K = 1;  // outputs
+  // LambdaRank: query group row offsets [n_groups + 1] and rank discounts
+  // 1 / log2(2 + r) for r below the largest group.
+  std::vector<int64_t> group_off;
+  std::vector<double> rank_disc;
 };
 
 struct TrainStats {
@@ -110,6 +114,10 @@ class Trainer {
   std::vector<int32_t> qexp_h_;
 
   DevBuf<float> preds_, g_, h_, vpreds_;
+  DevBuf<int64_t> group_off_;  // LambdaRank
+  DevBuf<double> rank_disc_;
+  // The round's gradient launch (objective dispatch).
+  void launch_objective(const float* y, const float* w);
   DevBuf<uint32_t> absmax_;
   DevBuf<int32_t> qexp_;
   DevBuf<uint32_t> idx_, idx_tmp_;

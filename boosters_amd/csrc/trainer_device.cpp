@@ -213,7 +213,7 @@ void Trainer::device_enqueue_round(const float* y, const float* w) {
   const int g_lsegs = grid_for(S.cap_lsegs, kMaxGrid);
   uint32_t* cnt = B.counts;
   const int n_feats = (int)feats_h_.size();
-  launch_gradients(s_, cfg.objective, preds_.p, y, w, n_, K, g_.p, h_.p, absmax_.p);
+  launch_objective(y, w);
   launch_qscale(s_, absmax_.p, K, n_, qexp_.p);
   launch_ctl_init(s_, P, B, S.rows.p);
   launch_init_rows_sums(s_, idx_.p, ogh_.p, g_.p, h_.p, n_, (int)P.T, qexp_.p, MN_, node_sum_.p);

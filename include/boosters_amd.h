@@ -265,6 +265,14 @@ int bst_server_destroy(bst_server* srv);
 int bst_kernel_gradients(bst_ctx* ctx, int32_t objective, int32_t n_outputs, const float* preds,
                          const float* targets, const float* weights, int64_t n, float* grad,
                          float* hess);
+/* LambdaRank gradients over ragged query groups (group_sizes [n_groups],
+ * each 1..4096 rows, summing to n; targets are integer relevance labels
+ * 0..31).  No counterpart in the reference (its objectives stop at
+ * objectives/mod.rs:196-219): the published LambdaMART gradients with
+ * sigma = 1 and pairs truncated to the top 30 positions.  Host buffers. */
+int bst_kernel_lambdarank(bst_ctx* ctx, const float* preds, const float* targets,
+                          const float* weights, int64_t n, const int32_t* group_sizes,
+                          int64_t n_groups, float* grad, float* hess);
 /* TreeGrower::build_histogram (grower.rs:662-730) for one node: rows (nullable
  * = all rows in order) with per-row grad/hess indexed by row id.  Output is the
  * node histogram as f64 (g, h) pairs, [n_effective_features][max_bins_eff]

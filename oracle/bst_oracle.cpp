@@ -245,7 +245,78 @@ void gradients(int obj, int K, const float* preds, const float* y, const float* 
   }
 }
 
+// LambdaRank gradients over ragged query groups.  The reference has no
+// ranking objective (objectives/mod.rs:196-219), so this follows the
+// published LambdaMART formulation (Burges, "From RankNet to LambdaRank to
+// LambdaMART: An Overview", MSR-TR-2010-82) with sigma = 1 and the pair
+// truncation of LightGBM's lambdarank objective, as this engine defines it:
+//   - documents of a group ranked by score descending, ties by index;
+//   - pair (a, b) of positions counts when labels differ and min(a, b) < T;
+//   - rho = 1 / (1 + exp(s_hi - s_lo)) in f32 (s_hi: the higher label);
+//   - dNDCG = (2^l_hi - 2^l_lo) |disc[a] - disc[b]| / maxDCG@T, disc[r] = 1/log2(2 + r);
+//   - the higher-labelled document gets -rho dNDCG, the other +rho dNDCG,
+//     both get rho (1 - rho) dNDCG; each document sums its pairs in f64 in
+//     position order, then scales by its weight and rounds to f32.
+// No lambda normalisation.  Parity with any reference: unpinned.
+constexpr int kRankTrunc = 30;
+
+std::vector<double> rank_discounts(int64_t max_group) {
+  std::vector<double> d((size_t)std::max<int64_t>(max_group, 1));
+  for (size_t r = 0; r < d.size(); r++) d[r] = 1.0 / std::log2(2.0 + (double)r);
+  return d;
+}
+
+void lambdarank_gradients(const float* preds, const float* y, const float* w, int64_t n,
+                          const int32_t* sizes, int64_t n_groups, float* g, float* h) {
+  int64_t max_group = 1;
+  for (int64_t q = 0; q < n_groups; q++) max_group = std::max<int64_t>(max_group, sizes[q]);
+  const std::vector<double> disc = rank_discounts(max_group);
+  std::vector<int64_t> off((size_t)n_groups + 1, 0);
+  for (int64_t q = 0; q < n_groups; q++) off[q + 1] = off[q] + sizes[q];
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t q = 0; q < n_groups; q++) {
+    const int64_t b0 = off[q], m = sizes[q];
+    std::vector<int64_t> order((size_t)m);
+    for (int64_t i = 0; i < m; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return preds[b0 + a] > preds[b0 + b]; });
+    auto label = [&](int64_t i) { return (int)y[b0 + i]; };
+    std::vector<int> ls((size_t)m);
+    for (int64_t i = 0; i < m; i++) ls[i] = label(i);
+    std::sort(ls.begin(), ls.end(), std::greater<int>());
+    double maxdcg = 0.0;
+    for (int64_t r = 0; r < std::min<int64_t>(m, kRankTrunc); r++)
+      maxdcg += (std::ldexp(1.0, ls[r]) - 1.0) * disc[r];
+    for (int64_t a = 0; a < m; a++) {
+      const int64_t i = order[a];
+      double ga = 0.0, ha = 0.0;
+      if (maxdcg > 0.0) {
+        const double inv = 1.0 / maxdcg;
+        for (int64_t b = 0; b < m; b++) {
+          if (b == a || std::min(a, b) >= kRankTrunc) continue;
+          const int64_t j = order[b];
+          const int li = label(i), lj = label(j);
+          if (li == lj) continue;
+          const bool i_high = li > lj;
+          const float shi = i_high ? preds[b0 + i] : preds[b0 + j];
+          const float slo = i_high ? preds[b0 + j] : preds[b0 + i];
+          const float rho = 1.0f / (1.0f + exp32(shi - slo));
+          const double gap = std::ldexp(1.0, i_high ? li : lj) - std::ldexp(1.0, i_high ? lj : li);
+          const double dn = gap * std::fabs(disc[a] - disc[b]) * inv;
+          const double lam = (double)rho * dn;
+          ga += i_high ? -lam : lam;
+          ha += (double)rho * (1.0 - (double)rho) * dn;
+        }
+      }
+      const double wi = (double)wt(w, b0 + i);
+      g[b0 + i] = (float)(ga * wi);
+      h[b0 + i] = (float)(ha * wi);
+    }
+  }
+}
+
 std::vector<float> base_score(int obj, int K, const float* y, const float* w, int64_t n) {
+  if (obj == BST_OBJ_LAMBDARANK) return {0.0f};
   if (obj == BST_OBJ_SQUARED) {  // regression.rs:44-68
     if (n == 0) return {0.0f};
     double sw = 0, swy = 0;
@@ -924,6 +995,7 @@ double metric_value(int metric, const float* p, int K, const float* y, const flo
 }
 
 int default_metric(int obj) {
+  if (obj == BST_OBJ_LAMBDARANK) return BST_METRIC_NONE;  // no ranking metric in the reference
   return obj == BST_OBJ_LOGISTIC ? BST_METRIC_LOGLOSS
                                  : (obj == BST_OBJ_SOFTMAX ? BST_METRIC_MLOGLOSS : BST_METRIC_RMSE);
 }
@@ -1112,7 +1184,21 @@ float ora_leaf_weight(const bst_gbdt_config* cfg, double g, double h) {
 int ora_gradients(int32_t obj, int32_t K, const float* p, const float* y, const float* w, int64_t n,
                   float* g, float* h) {
   if (!p || !y || !g || !h) return fail(BST_E_INVALID, "null");
+  if (obj == BST_OBJ_LAMBDARANK) return fail(BST_E_CONFIG, "use ora_lambdarank_gradients");
   gradients(obj, K, p, y, w, n, g, h);
+  return BST_OK;
+}
+
+int ora_lambdarank_gradients(const float* p, const float* y, const float* w, int64_t n,
+                             const int32_t* sizes, int64_t n_groups, float* g, float* h) {
+  if (!p || !y || !g || !h || !sizes) return fail(BST_E_INVALID, "null");
+  int64_t tot = 0;
+  for (int64_t q = 0; q < n_groups; q++) {
+    if (sizes[q] < 1) return fail(BST_E_CONFIG, "empty query group");
+    tot += sizes[q];
+  }
+  if (tot != n) return fail(BST_E_SHAPE, "query group sizes must sum to the row count");
+  lambdarank_gradients(p, y, w, n, sizes, n_groups, g, h);
   return BST_OK;
 }
 
@@ -1167,6 +1253,16 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
   int64_t n = b->n;
   if (ny < n) return fail(BST_E_TARGETS, "fewer targets than rows");
   int K = c.objective == BST_OBJ_SOFTMAX ? (int)c.n_classes : 1;
+  if (c.objective == BST_OBJ_LAMBDARANK) {
+    if (!ext.query_group_sizes || ext.n_query_groups < 1)
+      return fail(BST_E_CONFIG, "LambdaRank needs query groups");
+    int64_t tot = 0;
+    for (int64_t q = 0; q < ext.n_query_groups; q++) {
+      if (ext.query_group_sizes[q] < 1) return fail(BST_E_CONFIG, "empty query group");
+      tot += ext.query_group_sizes[q];
+    }
+    if (tot != n) return fail(BST_E_SHAPE, "query group sizes must sum to the row count");
+  }
   auto t0 = std::chrono::steady_clock::now();
   auto* M = new ora_model();
   M->objective = c.objective;
@@ -1194,7 +1290,11 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
   bool have_best = false;
   size_t best_round = 0, cur_round = 0, best_n_trees = 0;
   for (uint32_t round = 0; round < c.n_trees; round++) {
-    gradients(c.objective, K, preds.data(), y, w, n, g.data(), h.data());
+    if (c.objective == BST_OBJ_LAMBDARANK)
+      lambdarank_gradients(preds.data(), y, w, n, ext.query_group_sizes, ext.n_query_groups,
+                           g.data(), h.data());
+    else
+      gradients(c.objective, K, preds.data(), y, w, n, g.data(), h.data());
     for (int k = 0; k < K; k++) {
       float* gk = g.data() + (size_t)k * n;
       float* hk = h.data() + (size_t)k * n;

@@ -56,6 +56,10 @@ float ora_leaf_weight(const bst_gbdt_config* cfg, double grad_sum, double hess_s
 /* Objectives: objectives/mod.rs:236-298. */
 int ora_gradients(int32_t objective, int32_t n_outputs, const float* preds, const float* targets,
                   const float* weights, int64_t n, float* grad, float* hess);
+/* LambdaRank (no reference counterpart; spec in bst_oracle.cpp). */
+int ora_lambdarank_gradients(const float* preds, const float* targets, const float* weights,
+                             int64_t n, const int32_t* group_sizes, int64_t n_groups,
+                             float* grad, float* hess);
 int ora_base_score(int32_t objective, int32_t n_outputs, const float* targets,
                    const float* weights, int64_t n, float* out);
 

@@ -48,6 +48,9 @@ def lib():
                                           P(C.c_uint32)]),
             "ora_gradients": (C.c_int, [C.c_int32, C.c_int32, P(C.c_float), P(C.c_float),
                                         P(C.c_float), C.c_int64, P(C.c_float), P(C.c_float)]),
+            "ora_lambdarank_gradients": (C.c_int, [P(C.c_float), P(C.c_float), P(C.c_float),
+                                                   C.c_int64, P(C.c_int32), C.c_int64,
+                                                   P(C.c_float), P(C.c_float)]),
             "ora_base_score": (C.c_int, [C.c_int32, C.c_int32, P(C.c_float), P(C.c_float),
                                          C.c_int64, P(C.c_float)]),
             "ora_histogram": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_uint32),
@@ -173,6 +176,18 @@ def gradients(objective, preds, targets, weights=None):
     _check(lib().ora_gradients(objective, K, A.ptr(preds),
                                A.ptr(np.ascontiguousarray(targets, np.float32)), A.ptr(w), n,
                                A.ptr(g), A.ptr(h)))
+    return g, h
+
+
+def lambdarank_gradients(preds, targets, group_sizes, weights=None):
+    preds = np.ascontiguousarray(preds, np.float32).reshape(-1)
+    n = len(preds)
+    g = np.zeros(n, np.float32)
+    h = np.zeros(n, np.float32)
+    sizes = np.ascontiguousarray(group_sizes, np.int32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    _check(lib().ora_lambdarank_gradients(A.ptr(preds), A.ptr(np.ascontiguousarray(targets, np.float32)),
+                                          A.ptr(w), n, A.ptr(sizes), len(sizes), A.ptr(g), A.ptr(h)))
     return g, h
 
 

@@ -111,3 +111,25 @@ def multiclass_cat_data(n, n_num, n_cat, K, card=32, seed=0):
     is_cat = np.zeros(n_num + n_cat, np.uint8)
     is_cat[n_num:] = 1
     return X, y, is_cat
+
+
+def ranking_data(n, F, seed=0, zero_p=0.3, group_lo=5, group_hi=500):
+    """Config-4 generator: X ~ U[0,1] with exact zeros (p = 0.3), query sizes
+    ~ U{group_lo..group_hi} until the rows are covered (last truncated), score
+    s = sum_f a_f [x_f > t_f], label = quintile of s (0..4)."""
+    rng = np.random.default_rng(seed)
+    X = rng.random((F, n), dtype=np.float32)
+    X[rng.random((F, n)) < zero_p] = 0.0
+    sizes = []
+    tot = 0
+    while tot < n:
+        sz = int(rng.integers(group_lo, group_hi + 1))
+        sz = min(sz, n - tot)
+        sizes.append(sz)
+        tot += sz
+    a = rng.standard_normal(F)
+    t = rng.random(F)
+    s = (a[:, None] * (X > t[:, None])).sum(axis=0)
+    q = np.quantile(s, [0.2, 0.4, 0.6, 0.8])
+    y = np.searchsorted(q, s, side="right").astype(np.float32)
+    return X, y, np.array(sizes, np.int32)

@@ -14,7 +14,8 @@ import pytest
 
 from boosters_amd import _abi as A
 from boosters_amd.model import BinnedDataset, Dataset, GBDTModel
-from tests.conftest import binary_nan_data, make_cfg, make_ext, multiclass_cat_data, regression_data
+from tests.conftest import (binary_nan_data, make_cfg, make_ext, multiclass_cat_data,
+                            ranking_data, regression_data)
 
 pytestmark = pytest.mark.gpu
 
@@ -87,6 +88,27 @@ def test_gradients(ctx, oracle, obj, K):
     # glibc expf is within 1 ulp of the correctly rounded exp; softmax combines
     # K exps, a sum and a division, so the gap can reach a few ulp.
     assert ulp.max() <= (2 if obj != 2 else 6)
+
+
+def test_lambdarank_gradients(ctx, oracle):
+    rng = np.random.default_rng(21)
+    sizes = np.concatenate([[1, 2, 600, 31, 30, 29], rng.integers(1, 200, 80)]).astype(np.int32)
+    n = int(sizes.sum())
+    y = rng.integers(0, 5, n).astype(np.float32)
+    p = (rng.standard_normal(n) * 2).astype(np.float32)
+    p[::7] = p[1::7][: len(p[::7])]  # ties inside groups
+    p[5] = -0.0
+    p[6] = 0.0
+    w = (rng.random(n) + 0.5).astype(np.float32)
+    for weights in (None, w):
+        g = np.zeros(n, np.float32)
+        h = np.zeros(n, np.float32)
+        A.check(A.lib().bst_kernel_lambdarank(ctx.h, A.ptr(p), A.ptr(y), A.ptr(weights), n,
+                                              A.ptr(sizes), len(sizes), A.ptr(g), A.ptr(h)))
+        oracle.set_exp_mode(1)
+        og, oh = oracle.lambdarank_gradients(p, y, sizes, weights)
+        np.testing.assert_array_equal(g, og)
+        np.testing.assert_array_equal(h, oh)
 
 
 def test_histogram_vs_f64(ctx, oracle):
@@ -309,6 +331,27 @@ def test_graph_schedule_training_predictions(ctx, oracle):
     assert len(hist[2]) == 7
     np.testing.assert_array_equal(hist[1], hist[2])
     assert hist[2][-1] < hist[2][0]
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_train_lambdarank(ctx, oracle, schedule):
+    X, y, sizes = ranking_data(6000, 24, seed=3, group_lo=5, group_hi=80)
+    cfg = make_cfg(objective=3, n_trees=5, learning_rate=0.3, growth=1, max_leaves=31)
+    oracle.set_exp_mode(1)
+    o = oracle.Binned(X, None)
+    e0 = make_ext(0)
+    e0.query_group_sizes = A.ptr(sizes)
+    e0.n_query_groups = len(sizes)
+    om = oracle.train(o, y, cfg, e0)
+    b = _binned(ctx, X, None)
+    e = make_ext(schedule)
+    e.query_group_sizes = A.ptr(sizes)
+    e.n_query_groups = len(sizes)
+    h = C.c_void_p()
+    A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0, C.byref(cfg),
+                              C.byref(e), C.byref(h)))
+    gm = GBDTModel(h, ctx)
+    _forests_equal(gm, om.info, om.arrays)
 
 
 def test_predict_and_leaf_indices(ctx, oracle):
