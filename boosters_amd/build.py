@@ -18,6 +18,7 @@ OUT = os.path.join(HERE, "libboosters_amd.so")
 SOURCES = [
     "kernels.hip",
     "grow_device.hip",
+    "sample_device.hip",
     "binning.cpp",
     "trainer.cpp",
     "trainer_device.cpp",

@@ -173,6 +173,8 @@ int validate(const bst_gbdt_config* c) {
     raise(BST_E_CONFIG, "colsample_bytree must be in (0, 1]");
   if (!(c->colsample_bylevel > 0.0f && c->colsample_bylevel <= 1.0f))
     raise(BST_E_CONFIG, "colsample_bylevel must be in (0, 1]");
+  if (!(std::fabs(c->colsample_bylevel - 1.0f) < 1e-6f))
+    raise(BST_E_CONFIG, "colsample_bylevel < 1 is not supported by this engine build");
   if (c->lambda < 0.0f) raise(BST_E_CONFIG, "lambda must be non-negative");
   if (c->alpha < 0.0f) raise(BST_E_CONFIG, "alpha must be non-negative");
   if (c->min_child_weight < 0.0f) raise(BST_E_CONFIG, "min_child_weight must be non-negative");
@@ -502,6 +504,13 @@ static int train_impl(bst_ctx* ctx, bst_dataset* ds, const float* y_dev, int64_t
   bst_ext_config_default(&opt.ext);
   if (ext) opt.ext = *ext;
   opt.K = cfg->objective == BST_OBJ_SOFTMAX ? (int)cfg->n_classes : 1;
+  if (opt.ext.goss_top_rate > 0.0f) {  // RowSamplingParams::Goss
+    need(opt.ext.goss_top_rate < 1.0f && opt.ext.goss_other_rate > 0.0f &&
+             opt.ext.goss_other_rate <= 1.0f &&
+             opt.ext.goss_top_rate + opt.ext.goss_other_rate <= 1.0f,
+         BST_E_CONFIG, "GOSS rates: 0 < top < 1, 0 < other <= 1, top + other <= 1");
+    need(ds->d.n >= 2, BST_E_CONFIG, "GOSS needs at least two rows");
+  }
   if (cfg->objective == BST_OBJ_LAMBDARANK) {
     // Query groups: ragged, in row order, summing to the row count.
     need(opt.ext.query_group_sizes && opt.ext.n_query_groups >= 1, BST_E_CONFIG,

@@ -997,7 +997,7 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     const long long* __restrict__ node_sum, const uint32_t* __restrict__ node_count,
     const int32_t* __restrict__ qexp, uint32_t max_nodes, SplitParams p,
     unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand,
-    const uint32_t* __restrict__ dyn) {
+    const uint32_t* __restrict__ dyn, int feat_stride) {
   __shared__ double s_ratio[kSplitWaves][256];
   __shared__ uint16_t s_order[kSplitWaves][256];
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
@@ -1009,9 +1009,9 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
   const uint32_t n_work = dyn ? *dyn : gridDim.y;
   for (uint32_t wy = blockIdx.y; wy < n_work; wy += gridDim.y) {
   const uint32_t node = nodes[wy];
-  const uint32_t f = feats[fi];
   const uint32_t slot = slots[wy];
   const uint32_t tree = node / max_nodes;
+  const uint32_t f = feats[(int64_t)tree * feat_stride + fi];
   const double ig = ldexp(1.0, -qexp[2 * tree]), ih = ldexp(1.0, -qexp[2 * tree + 1]);
   const long long G = node_sum[2 * node], H = node_sum[2 * node + 1];
   const uint32_t cnt = node_count[node];
@@ -1263,12 +1263,12 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
                   int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
                   const int32_t* qexp, uint32_t max_nodes, SplitParams p,
                   unsigned long long* node_key, DevSplit* cand, DevSplit* node_split,
-                  const uint32_t* dyn, bool finalize) {
+                  const uint32_t* dyn, bool finalize, int feat_stride) {
   if (n_nodes <= 0) return;
   dim3 grid((unsigned)((n_feats + kSplitWaves - 1) / kSplitWaves), (unsigned)n_nodes);
   hipLaunchKernelGGL(k_split, grid, dim3(kSplitWaves * WAVE), 0, s, nodes, slots, feats, n_feats, d.meta, d.FG,
                      d.NG, d.E, hist, slot_elems, node_sum, node_count, qexp, max_nodes, p,
-                     node_key, cand, dyn);
+                     node_key, cand, dyn, feat_stride);
   if (!finalize) return;  // the caller reduces node_key / cand itself
   hipLaunchKernelGGL(k_split_finalize, dim3((n_nodes + 63) / 64), dim3(64), 0, s, nodes, n_nodes,
                      d.E, p.rightmost, node_key, cand, node_split, dyn);

@@ -18,15 +18,16 @@
 // node processing order, partition ids and tie rules are the same.
 #include "trainer.h"
 
-#include "grow_device.h"
-
 #include <algorithm>
-#include <cstring>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <limits>
+
+#include "grow_device.h"
+#include "sample_keys.h"
 
 namespace bst {
 
@@ -37,40 +38,18 @@ constexpr uint32_t kPartTile = 4096;   // rows per partition block
 constexpr uint32_t kSegTile = 16384;   // rows per sums / leaf-update block
 constexpr uint32_t kDirect = 0x80000000u;
 
-struct Xoshiro {  // xoshiro256++ seeded by splitmix64
-  uint64_t s[4];
-  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-  explicit Xoshiro(uint64_t seed) {
-    for (int i = 0; i < 4; i++) {
-      seed += 0x9e3779b97f4a7c15ULL;
-      uint64_t z = seed;
-      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-      s[i] = z ^ (z >> 31);
-    }
-  }
-  uint64_t next() {
-    const uint64_t r = rotl(s[0] + s[3], 23) + s[0];
-    const uint64_t t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = rotl(s[3], 45);
-    return r;
-  }
-  uint64_t range_incl(uint64_t lo, uint64_t hi) {
-    const uint64_t range = hi - lo + 1;
-    if (range == 0) return next();
-    const uint64_t zone = (range << __builtin_clzll(range)) - 1;
-    for (;;) {
-      const uint64_t v = next();
-      const unsigned __int128 m = (unsigned __int128)v * range;
-      if ((uint64_t)m <= zone) return lo + (uint64_t)(m >> 64);
-    }
-  }
-};
+// The k candidates 0..n-1 with the smallest (sampling key, index), in index
+// order (the draw rule of every sampling decision; sample_keys.h).
+std::vector<uint32_t> smallest_keys(uint32_t n, uint64_t base, uint32_t k) {
+  std::vector<uint64_t> kv(n);
+  for (uint32_t i = 0; i < n; i++) kv[i] = ((uint64_t)sample_key(base, i) << 32) | i;
+  k = std::min(k, n);
+  if (k < n) std::nth_element(kv.begin(), kv.begin() + k, kv.end());
+  std::vector<uint32_t> out(k);
+  for (uint32_t j = 0; j < k; j++) out[j] = (uint32_t)(kv[j] & 0xFFFFFFFFu);
+  std::sort(out.begin(), out.end());
+  return out;
+}
 
 DevSplit invalid_split() {
   DevSplit s;
@@ -127,7 +106,9 @@ Trainer::Trainer(bst_ctx* ctx, bst_dataset* ds, const TrainOptions& opt)
   MN_ = (uint32_t)std::max<uint64_t>(2ull * max_leaves - 1, 63) + 2;
   SPT_ = max_leaves + 2;
   const bool col_sampling = !(std::fabs(c.colsample_bytree - 1.0f) < 1e-6f);
-  T_ = (sched_ != BST_SCHED_PLAIN && opt_.K > 1 && !col_sampling) ? opt_.K : 1;
+  // Class trees of a round grow together, except on the host-driven path with
+  // column sampling (its split launch takes one feature list).
+  T_ = (sched_ != BST_SCHED_PLAIN && opt_.K > 1 && (graph_capable() || !col_sampling)) ? opt_.K : 1;
   slot_elems_ = (int64_t)ds->d.NG * 256 * ds->d.FG * 2;
   rightmost_ = opt_.ext.n_threads != 1 && ds->d.E >= 16;
   sp_.lambda = (double)c.lambda;
@@ -632,10 +613,9 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   feats_h_.resize(E);
   for (uint32_t i = 0; i < E; i++) feats_h_[i] = i;
   h2d(s_, feats_.p, feats_h_.data(), E);
-  Xoshiro col_rng(cfg.seed), row_rng(cfg.seed);
   const bool col_sampling = !(std::fabs(cfg.colsample_bytree - 1.0f) < 1e-6f);
   const bool uniform = !(std::fabs(cfg.subsample - 1.0f) < 1e-6f);
-  if (opt_.ext.goss_top_rate > 0.0f) raise(BST_E_CONFIG, "GOSS row sampling is not supported by this engine build");
+
 
   // One grown tree as a single-tree forest (leaf values already carry the learning rate).
   auto single_tree = [&](const TreeBuild& tb, int output) {
@@ -702,7 +682,15 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
     stats.n_trees++;
   };
   const bool on_device = device_schedule_ok();
-  if (on_device) sp_.rightmost = (opt_.ext.n_threads != 1 && E >= 16) ? 1 : 0;
+  if (opt_.ext.goss_top_rate > 0.0f && !on_device)
+    raise(BST_E_CONFIG, "GOSS row sampling runs on the graph schedule");
+  if (on_device) {
+    // The split kernel's tie rule depends on the number of candidate features.
+    const uint32_t nf = col_sampling
+                            ? std::max<uint32_t>(1, (uint32_t)std::ceil((double)E * cfg.colsample_bytree))
+                            : E;
+    sp_.rightmost = (opt_.ext.n_threads != 1 && std::min(nf, E) >= 16) ? 1 : 0;
+  }
   if (std::getenv("BST_TRACE_SETUP"))
     std::fprintf(stderr, "[bst] train set-up before the first round %.3f ms\n",
                  std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
@@ -745,33 +733,22 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
         ts.leaf_lr.assign(MN_, std::numeric_limits<float>::quiet_NaN());
         ts.free_slots.clear();
         for (uint32_t s = SPT_; s-- > 0;) ts.free_slots.push_back(s);
-        // Column sampling per tree (column.rs:266-283, seeded stream).
+        // Column sampling per tree (column.rs:266-283): the k columns with
+        // the smallest sampling keys of (round, output).
         if (col_sampling) {
-          feats_h_.resize(E);
-          for (uint32_t i = 0; i < E; i++) feats_h_[i] = i;
           const uint32_t kk = std::max<uint32_t>(1, (uint32_t)std::ceil((double)E * cfg.colsample_bytree));
-          for (uint32_t i = 0; i < kk && i < E; i++) {
-            const uint32_t j = i + (uint32_t)col_rng.range_incl(0, E - 1 - i);
-            std::swap(feats_h_[i], feats_h_[j]);
-          }
-          feats_h_.resize(std::min(kk, E));
-          std::sort(feats_h_.begin(), feats_h_.end());
+          feats_h_ = smallest_keys(E, sample_key_base(cfg.seed, kStreamCols, round, (uint32_t)ts.output), kk);
           h2d(s_, feats_.p, feats_h_.data(), feats_h_.size());
           // The split kernel's tie rule depends on the number of candidate features.
           sp_.rightmost = (opt_.ext.n_threads != 1 && feats_h_.size() >= 16) ? 1 : 0;
         } else {
           sp_.rightmost = (opt_.ext.n_threads != 1 && E >= 16) ? 1 : 0;
         }
-        // Uniform row sampling (row.rs:170-208): reservoir over row ids.
+        // Uniform row sampling (row.rs:170-208): target_count rows, the
+        // smallest sampling keys of (round, output).
         if (uniform) {
           const size_t tc = std::min<size_t>(std::max<size_t>((size_t)((float)n_ * cfg.subsample), 1), n_);
-          sampled.resize(tc);
-          for (size_t i = 0; i < tc; i++) sampled[i] = (uint32_t)i;
-          for (size_t i = tc; i < (size_t)n_; i++) {
-            const size_t j = (size_t)row_rng.range_incl(0, i);
-            if (j < tc) sampled[j] = (uint32_t)i;
-          }
-          std::sort(sampled.begin(), sampled.end());
+          sampled = smallest_keys((uint32_t)n_, sample_key_base(cfg.seed, kStreamRows, round, (uint32_t)ts.output), (uint32_t)tc);
           upload(s_, d_sampled_, sampled);
           ts.rows = (int64_t)tc;
           ts.sampled = true;

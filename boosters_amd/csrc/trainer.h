@@ -85,9 +85,10 @@ class Trainer {
     void operator()(DevState* p) const;
   };
   std::unique_ptr<DevState, DevStateDelete> dev_;
+  bool graph_capable() const;  // the configuration can run on the graph schedule
   bool device_schedule_ok() const;
   void device_setup();
-  void device_enqueue_round(const float* y, const float* w);
+  void device_enqueue_round(const float* y, const float* w, bool sampled);
   void device_round(const float* y, const float* w, std::vector<TreeBuild>& out);
   // Algorithmic bytes: each histogrammed row reads its E bins (1 B, or
   // 1.125 B with the NaN hi plane) plus its gradient pair (8 B) and row id

@@ -15,6 +15,7 @@
 #include <limits>
 
 #include "grow_device.h"
+#include "sample_device.h"
 #include "trainer.h"
 
 namespace bst {
@@ -51,27 +52,39 @@ struct Trainer::DevState {
   DevBuf<float> thr_up;
   DevBuf<long long> partial;    // one histogram group per multi-tile item
   DevBuf<long long> tile_sums;  // [tiles][4] partition totals
+  // Sampling: row draws (uniform / GOSS) and per-tree column lists.
+  SampleParams SP{};
+  int sample_mode = kSampleNone;
+  uint32_t goss_warmup = 0;
+  int n_feats = 0;     // candidate columns per tree
+  bool col_sample = false;
+  uint32_t host_round = 0;
+  DevBuf<uint32_t> round;  // boosting round, read by the sampling kernels
+  DevBuf<SampleState> samp;
+  DevBuf<uint32_t> bcount, feats_dev;
   char* h_out = nullptr;      // pinned mirror of `out`
   TreeOut H{};
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // [0]: rounds without row sampling, [1]: rounds with it.
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   ~DevState() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (int i = 0; i < 2; i++) {
+      if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+      if (graph[i]) (void)hipGraphDestroy(graph[i]);
+    }
     if (h_out) (void)hipHostFree(h_out);
   }
 };
 
 void Trainer::DevStateDelete::operator()(DevState* p) const { delete p; }
 
-bool Trainer::device_schedule_ok() const {
+bool Trainer::graph_capable() const {
   const auto& c = opt_.cfg;
-  const bool sampling = !(std::fabs(c.subsample - 1.0f) < 1e-6f) ||
-                        !(std::fabs(c.colsample_bytree - 1.0f) < 1e-6f) ||
-                        opt_.ext.goss_top_rate > 0.0f;
   const bool depth_ok = c.growth != BST_GROW_DEPTHWISE || c.max_depth <= 12;
-  return sched_ == BST_SCHED_GRAPH && !sampling && depth_ok && T_ == opt_.K;
+  return sched_ == BST_SCHED_GRAPH && depth_ok;
 }
+
+bool Trainer::device_schedule_ok() const { return graph_capable() && T_ == opt_.K; }
 
 void Trainer::device_setup() {
   std::unique_ptr<DevState, DevStateDelete> st(new DevState());
@@ -123,6 +136,46 @@ void Trainer::device_setup() {
   s.pbuf.alloc((size_t)T * MN_);
   s.cursor.alloc((size_t)2 * s.cap_items);
   s.tile_sums.alloc((size_t)4 * s.cap_tiles);
+  // Row sampling (row.rs:141-270) and column sampling (column.rs:266-283).
+  {
+    SampleParams& SP = s.SP;
+    SP.T = T;
+    SP.n = n_;
+    SP.stride = n_;
+    SP.seed = c.seed;
+    const bool uniform = !(std::fabs(c.subsample - 1.0f) < 1e-6f);
+    if (opt_.ext.goss_top_rate > 0.0f) {
+      const size_t n = (size_t)n_;
+      const size_t top_k =
+          std::min<size_t>(std::max<size_t>((size_t)((float)n * opt_.ext.goss_top_rate), 1), n - 1);
+      const size_t other_k = std::max<size_t>((size_t)((float)n * opt_.ext.goss_other_rate), 1);
+      s.sample_mode = kSampleGoss;
+      SP.k_mag = (uint32_t)(n - top_k);
+      SP.k_key = (uint32_t)(other_k - 1);
+      SP.mult = (float)(n - top_k) / (float)other_k;
+      s.goss_warmup = (uint32_t)(1.0f / c.learning_rate);
+    } else if (uniform) {
+      const size_t tc =
+          std::min<size_t>(std::max<size_t>((size_t)((float)n_ * c.subsample), 1), (size_t)n_);
+      s.sample_mode = kSampleUniform;
+      SP.k_key = (uint32_t)(tc - 1);
+    }
+    SP.mode = s.sample_mode;
+    s.col_sample = !(std::fabs(c.colsample_bytree - 1.0f) < 1e-6f);
+    s.n_feats = (int)P.E;
+    if (s.col_sample) {
+      s.n_feats = (int)std::min<uint32_t>(
+          P.E, std::max<uint32_t>(1, (uint32_t)std::ceil((double)P.E * c.colsample_bytree)));
+      s.feats_dev.alloc((size_t)T * P.E);
+    }
+    s.round.alloc(1);
+    launch_fill_u32(s_, s.round.p, 0u, 1);
+    if (s.sample_mode != kSampleNone) {
+      s.samp.alloc(T);
+      const int64_t nb = (n_ + kSampleBlockRows - 1) / kSampleBlockRows;
+      s.bcount.alloc((size_t)T * 4 * nb);
+    }
+  }
   s.out_bytes = tree_out_layout(T, P.MNT, nullptr, nullptr);
   s.out.alloc(s.out_bytes);
   tree_out_layout(T, P.MNT, s.out.p, &s.O);
@@ -200,7 +253,7 @@ void Trainer::device_setup() {
 }
 
 // Every launch of one round (T trees = all K outputs), no host sync.
-void Trainer::device_enqueue_round(const float* y, const float* w) {
+void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled) {
   DevState& S = *dev_;
   const CtlParams& P = S.P;
   const DeviceBufs& B = S.B;
@@ -212,26 +265,44 @@ void Trainer::device_enqueue_round(const float* y, const float* w) {
   const int g_split = grid_for(S.cap_split, 512);
   const int g_lsegs = grid_for(S.cap_lsegs, kMaxGrid);
   uint32_t* cnt = B.counts;
-  const int n_feats = (int)feats_h_.size();
+  const int n_feats = S.n_feats;
+  const uint32_t* feats = S.col_sample ? S.feats_dev.p : feats_.p;
+  const int feat_stride = S.col_sample ? (int)P.E : 0;
   launch_objective(y, w);
+  if (sampled) {
+    // Sampled rows (index order) and their (g, h) go straight to slot
+    // positions; rows[t] and the quantisation maxima cover only them.
+    launch_sample_rows(s_, S.SP, S.round.p, g_.p, h_.p, S.samp.p, S.bcount.p, idx_.p, ogh_.p,
+                       S.rows.p, absmax_.p);
+  } else {
+    launch_fill_u32(s_, S.rows.p, (uint32_t)n_, P.T);
+  }
   launch_qscale(s_, absmax_.p, K, n_, qexp_.p);
   launch_ctl_init(s_, P, B, S.rows.p);
-  launch_init_rows_sums(s_, idx_.p, ogh_.p, g_.p, h_.p, n_, (int)P.T, qexp_.p, MN_, node_sum_.p);
-  auto hist_step = [&](bool gather) {
+  if (!sampled)
+    launch_init_rows_sums(s_, idx_.p, ogh_.p, g_.p, h_.p, n_, (int)P.T, qexp_.p, MN_,
+                          node_sum_.p);
+  if (S.col_sample)
+    launch_sample_columns(s_, opt_.cfg.seed, S.round.p, (int)P.T, (int)P.E, n_feats,
+                          S.feats_dev.p);
+  auto hist_step = [&](bool gather, bool root_sums) {
     int pi = ctx_->prof_begin(0, 0.0);
     launch_hist(s_, d, B.hitems, g_hist, idx_.p, ogh_.p, 0, qexp_.p, hist_.p, slot_elems_,
-                nullptr, gather, 0, cnt + C_HIST, S.partial.p, idx_tmp_.p, ogh_tmp_.p);
+                root_sums ? node_sum_.p : nullptr, gather, 0, cnt + C_HIST, S.partial.p,
+                idx_tmp_.p, ogh_tmp_.p);
     ctx_->prof_end(pi);
     launch_hist_finish(s_, B.fin, (int)S.cap_fin, (int)P.NG, (int)P.FG, S.partial.p, hist_.p,
                        slot_elems_, cnt + C_FIN);
     pi = ctx_->prof_begin(1, 0.0);
-    launch_split(s_, d, B.split_nodes, B.split_slots, g_split, feats_.p, n_feats, hist_.p,
+    launch_split(s_, d, B.split_nodes, B.split_slots, g_split, feats, n_feats, hist_.p,
                  slot_elems_, node_sum_.p, node_count_.p, qexp_.p, MN_, sp_, node_key_.p,
-                 cand_.p, node_split_.p, cnt + C_SPLIT, /*finalize=*/false);
+                 cand_.p, node_split_.p, cnt + C_SPLIT, /*finalize=*/false, feat_stride);
     ctx_->prof_end(pi);
   };
-  // Positions of tree slot t start at t * n, so only slot 0 may skip the gather.
-  hist_step(P.T > 1);
+  // Positions of tree slot t start at t * n, so only slot 0 of an unsampled
+  // round may skip the gather.  Sampled roots take their totals from the
+  // histogram (the rows were chosen after the quantisation scale).
+  hist_step(sampled || P.T > 1, sampled);
   for (int step = 0; step < S.steps; step++) {
     launch_ctl_plan(s_, P, B, /*advance=*/step > 0);
     int pi = ctx_->prof_begin(2, 0.0);
@@ -242,11 +313,20 @@ void Trainer::device_enqueue_round(const float* y, const float* w) {
     launch_ctl_children(s_, P, B);
     // The last step's children are never split (depth limit, or the leaf
     // budget is spent), so their histograms are not built.
-    if (step + 1 < S.steps) hist_step(true);
+    if (step + 1 < S.steps) hist_step(true, false);
   }
   launch_ctl_finalize(s_, P, B, /*advance=*/S.steps > 0);
-  launch_leaf_update(s_, B.leaf_segs, g_lsegs, leaf_val_.p, idx_.p, 0, preds_.p,
-                     tree_output_.p, n_, cnt + C_LSEGS, idx_tmp_.p);
+  if (sampled) {
+    // Rows outside the sample get the tree too: traverse the raw features
+    // (trainer.rs:331-339).
+    launch_tree_update_raw(s_, (int)P.T, P.MNT, B.tr_split, B.tr_thr, B.tr_left, B.tr_right,
+                           B.tr_dleft, B.tr_isleaf, B.tr_stype, B.tr_cats, B.tr_leaf, d.raw, n_,
+                           tree_output_.p, preds_.p);
+  } else {
+    launch_leaf_update(s_, B.leaf_segs, g_lsegs, leaf_val_.p, idx_.p, 0, preds_.p,
+                       tree_output_.p, n_, cnt + C_LSEGS, idx_tmp_.p);
+  }
+  launch_round_advance(s_, S.round.p);
   d2h(s_, S.h_out, S.out.p, S.out_bytes);
 }
 
@@ -258,14 +338,18 @@ void Trainer::device_round(const float* y, const float* w, std::vector<TreeBuild
   if (!dev_) device_setup();
   DevState& S = *dev_;
   const auto t1 = clk::now();
+  const bool sampled = S.sample_mode != kSampleNone &&
+                       !(S.sample_mode == kSampleGoss && S.host_round < S.goss_warmup);
+  S.host_round++;
+  const int gi = sampled ? 1 : 0;
   const bool use_graph = !ctx_->profile;  // profiled runs time each launch with events
   if (use_graph) {
-    if (!S.exec) {
+    if (!S.exec[gi]) {
       BST_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-      device_enqueue_round(y, w);
-      BST_HIP(hipStreamEndCapture(s_, &S.graph));
+      device_enqueue_round(y, w, sampled);
+      BST_HIP(hipStreamEndCapture(s_, &S.graph[gi]));
       const auto t2 = clk::now();
-      BST_HIP(hipGraphInstantiate(&S.exec, S.graph, nullptr, nullptr, 0));
+      BST_HIP(hipGraphInstantiate(&S.exec[gi], S.graph[gi], nullptr, nullptr, 0));
       if (trace) {
         const auto t3 = clk::now();
         std::fprintf(stderr, "[bst] device setup %.3f ms, capture %.3f ms, instantiate %.3f ms\n",
@@ -274,9 +358,9 @@ void Trainer::device_round(const float* y, const float* w, std::vector<TreeBuild
                      std::chrono::duration<double, std::milli>(t3 - t2).count());
       }
     }
-    BST_HIP(hipGraphLaunch(S.exec, s_));
+    BST_HIP(hipGraphLaunch(S.exec[gi], s_));
   } else {
-    device_enqueue_round(y, w);
+    device_enqueue_round(y, w, sampled);
   }
   BST_HIP(hipStreamSynchronize(s_));
   BST_HIP(hipGetLastError());

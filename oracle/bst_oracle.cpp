@@ -14,9 +14,14 @@
 //   * Sparse-storage features are binned and histogrammed like dense ones
 //     (the reference skips sparse features in its gathered histogram path,
 //     ops.rs:158-160); no in-scope workload has a feature with <=10% nonzeros.
-//   * Column sampling uses xoshiro256++ seeded from the config seed; the
-//     reference seeds it from OS entropy (column.rs:137-142) so it has no
-//     reproducible stream to match.
+//   * Row subsampling, GOSS small-gradient rows and column subsets are drawn
+//     as "the k candidates with the smallest counter-based key" (smallest_keys
+//     below) instead of the reference's rand 0.8.5 streams (row.rs:106,185,
+//     256; column seeded from OS entropy, column.rs:137-142): the same uniform
+//     k-subset distributions, reproducible by parallel hardware.  GOSS keeps
+//     the reference's threshold, warm-up and amplification.  Parity with the
+//     reference's draws is unpinned (no test or fixture fixes them).
+//   * LambdaRank has no reference counterpart; see lambdarank_gradients.
 #include "bst_oracle.h"
 
 #include <algorithm>
@@ -587,48 +592,6 @@ int64_t stable_partition(const ora_binned& B, uint32_t* rows, int64_t n, const o
   return lw;
 }
 
-// ---------------------------------------------------------------------------
-// RNG: xoshiro256++ with splitmix64 seeding (rand 0.8.5 SmallRng on 64-bit).
-// ---------------------------------------------------------------------------
-struct Xoshiro {
-  uint64_t s[4];
-  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-  explicit Xoshiro(uint64_t seed) {
-    for (int i = 0; i < 4; i++) {
-      seed += 0x9e3779b97f4a7c15ULL;
-      uint64_t z = seed;
-      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-      s[i] = z ^ (z >> 31);
-    }
-  }
-  uint64_t next() {
-    uint64_t r = rotl(s[0] + s[3], 23) + s[0];
-    uint64_t t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = rotl(s[3], 45);
-    return r;
-  }
-  float next_f32() { return (float)((uint32_t)(next() >> 32) >> 8) * (1.0f / 16777216.0f); }
-  // UniformInt::sample_single_inclusive for u64 (widening multiply + zone).
-  uint64_t range_incl(uint64_t lo, uint64_t hi) {
-    uint64_t range = hi - lo + 1;
-    if (range == 0) return next();
-    int lz = __builtin_clzll(range);
-    uint64_t zone = (range << lz) - 1;
-    for (;;) {
-      uint64_t v = next();
-      unsigned __int128 m = (unsigned __int128)v * range;
-      uint64_t mhi = (uint64_t)(m >> 64), mlo = (uint64_t)m;
-      if (mlo <= zone) return lo + mhi;
-    }
-  }
-};
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -699,6 +662,45 @@ struct CandLess {  // BinaryHeap max by (gain, partition node id)
   }
 };
 
+// Counter-based sampling keys.  The reference draws row subsamples, GOSS
+// small-gradient rows and column subsets from rand 0.8.5 streams (row.rs:106,
+// 185,256; column.rs:138-142), which no test pins and which column sampling
+// seeds from OS entropy.  This engine defines each draw as "the k candidates
+// with the smallest (key, index)", key = a 31-bit splitmix64 hash of (seed,
+// stream, round, output, candidate index): the same distributions (uniform
+// k-subsets) with draws that parallel hardware reproduces exactly.
+constexpr uint32_t kStreamRows = 1, kStreamGoss = 2, kStreamCols = 3;
+
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline uint64_t key_base(uint64_t seed, uint32_t stream, uint32_t round, uint32_t output) {
+  return mix64(seed ^ mix64(((uint64_t)round << 32) | ((uint64_t)output << 8) | stream));
+}
+inline uint32_t key_of(uint64_t base, uint32_t i) {
+  return (uint32_t)(mix64(base ^ ((uint64_t)i * 0x9E3779B97F4A7C15ull)) >> 33);
+}
+// The k candidates with the smallest (key, index), returned in index order.
+std::vector<uint32_t> smallest_keys(const std::vector<uint32_t>& cand, uint64_t base, size_t k) {
+  std::vector<std::pair<uint32_t, uint32_t>> kv(cand.size());
+  for (size_t j = 0; j < cand.size(); j++) kv[j] = {key_of(base, cand[j]), cand[j]};
+  k = std::min(k, kv.size());
+  std::nth_element(kv.begin(), kv.begin() + (k ? k - 1 : 0), kv.end());
+  std::vector<uint32_t> out;
+  if (k) {
+    // nth_element leaves the k smallest in [0, k) (unordered) only up to the
+    // pivot; take everything <= the k-th smallest pair.
+    const auto piv = kv[k - 1];
+    for (auto& p : kv)
+      if (p <= piv) out.push_back(p.second);
+  }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
 struct Grower {
   const ora_binned& B;
   const bst_gbdt_config& c;
@@ -710,10 +712,10 @@ struct Grower {
   std::vector<float> last_leaf;           // by partition node id (lr applied)
   int64_t hist_rows = 0;
   std::vector<uint32_t> feats;
-  Xoshiro* col_rng;
+  uint32_t round = 0, output = 0;  // identify the tree for column sampling
 
-  Grower(const ora_binned& b, const bst_gbdt_config& cfg, int nt, Xoshiro* rng)
-      : B(b), c(cfg), n_threads(nt), col_rng(rng) {}
+  Grower(const ora_binned& b, const bst_gbdt_config& cfg, int nt)
+      : B(b), c(cfg), n_threads(nt) {}
 
   void build(uint32_t node, const float* g, const float* h) {
     auto& hs = hist[node];
@@ -878,21 +880,16 @@ struct Grower {
     }
   }
 
-  // ColSampler (column.rs) restated with a seeded stream: per tree, then per
-  // level (level sampling is applied on the tree subset).
+  // ColSampler per tree (column.rs:266-283): k = ceil(F * colsample_bytree)
+  // effective columns, drawn as the k smallest sampling keys of this tree.
   void sample_columns() {
     uint32_t F = (uint32_t)B.eff_to_orig.size();
     feats.resize(F);
     for (uint32_t i = 0; i < F; i++) feats[i] = i;
     float rate = c.colsample_bytree;
-    if (!(std::fabs(rate - 1.0f) < 1e-6f) && col_rng) {
+    if (!(std::fabs(rate - 1.0f) < 1e-6f)) {
       uint32_t k = std::max<uint32_t>(1, (uint32_t)std::ceil((double)F * rate));
-      for (uint32_t i = 0; i < k && i < F; i++) {
-        uint32_t j = i + (uint32_t)col_rng->range_incl(0, F - 1 - i);
-        std::swap(feats[i], feats[j]);
-      }
-      feats.resize(std::min(k, F));
-      std::sort(feats.begin(), feats.end());
+      feats = smallest_keys(feats, key_base(c.seed, kStreamCols, round, output), std::min(k, F));
     }
   }
 
@@ -1278,9 +1275,7 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
     vpreds.resize((size_t)K * nv);
     for (int k = 0; k < K; k++) std::fill(vpreds.begin() + (size_t)k * nv, vpreds.begin() + (size_t)(k + 1) * nv, M->base[k]);
   }
-  Xoshiro col_rng(c.seed);
-  Xoshiro row_rng(c.seed);
-  Grower gr(*b, c, ext.n_threads, &col_rng);
+  Grower gr(*b, c, ext.n_threads);
   bool goss = ext.goss_top_rate > 0.0f;
   bool uniform = !goss && !(std::fabs(c.subsample - 1.0f) < 1e-6f);
   size_t warmup = goss ? (size_t)(1.0f / c.learning_rate) : 0;
@@ -1300,21 +1295,19 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
       float* hk = h.data() + (size_t)k * n;
       std::vector<uint32_t> sampled;
       bool use_sample = false;
-      if (uniform) {  // sample_uniform, row.rs:170-208
+      if (uniform) {  // sample_uniform, row.rs:170-208: target_count rows
         size_t tc = std::min<size_t>(std::max<size_t>((size_t)((float)n * c.subsample), 1), n);
-        sampled.resize(tc);
-        for (size_t i = 0; i < tc; i++) sampled[i] = (uint32_t)i;
-        for (size_t i = tc; i < (size_t)n; i++) {
-          size_t j = (size_t)row_rng.range_incl(0, i);
-          if (j < tc) sampled[j] = (uint32_t)i;
-        }
+        std::vector<uint32_t> all((size_t)n);
+        for (int64_t i = 0; i < n; i++) all[i] = (uint32_t)i;
+        sampled = smallest_keys(all, key_base(c.seed, kStreamRows, round, (uint32_t)k), tc);
         std::vector<uint8_t> mask(n, 0);
         for (auto r : sampled) mask[r] = 1;
         for (int64_t i = 0; i < n; i++)
           if (!mask[i]) gk[i] = hk[i] = 0.0f;
-        std::sort(sampled.begin(), sampled.end());
         use_sample = true;
       } else if (goss && round >= warmup) {  // sample_goss, row.rs:211-270
+        // Rows with |g h| >= the (n - top_k)-th smallest magnitude are kept;
+        // other_k of the rest (all of them if fewer) are drawn and amplified.
         std::vector<float> mag(n);
         for (int64_t i = 0; i < n; i++) mag[i] = std::fabs(gk[i] * hk[i]);
         size_t top_k = std::min<size_t>(std::max<size_t>((size_t)((float)n * ext.goss_top_rate), 1), n - 1);
@@ -1323,28 +1316,24 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
         std::nth_element(sm.begin(), sm.begin() + (n - top_k), sm.end());
         float thr = sm[n - top_k];
         float mult = (float)(n - top_k) / (float)other_k;
-        size_t big = 0, small = 0;
-        for (int64_t i = 0; i < n; i++) {
-          if (mag[i] >= thr) {
+        std::vector<uint32_t> small;
+        for (int64_t i = 0; i < n; i++)
+          if (mag[i] >= thr)
             sampled.push_back((uint32_t)i);
-            big++;
-          } else {
-            size_t need = other_k > small ? other_k - small : 0;
-            size_t tk = top_k > big ? top_k - big : 0;
-            size_t all = (size_t)(n - i) > tk ? (size_t)(n - i) - tk : 0;
-            if (all > 0) {
-              float prob = (float)need / (float)all;
-              if (row_rng.next_f32() < prob) {
-                sampled.push_back((uint32_t)i);
-                small++;
-                gk[i] *= mult;
-                hk[i] *= mult;
-              }
-            }
-          }
+          else
+            small.push_back((uint32_t)i);
+        std::vector<uint32_t> pick =
+            smallest_keys(small, key_base(c.seed, kStreamGoss, round, (uint32_t)k), other_k);
+        for (auto r : pick) {
+          gk[r] *= mult;
+          hk[r] *= mult;
         }
+        sampled.insert(sampled.end(), pick.begin(), pick.end());
+        std::sort(sampled.begin(), sampled.end());
         use_sample = true;
       }
+      gr.round = round;
+      gr.output = (uint32_t)k;
       OTree t = gr.grow(gk, hk, use_sample ? &sampled : nullptr);
       M->trees.push_back(std::move(t));
       M->groups.push_back((uint32_t)k);

@@ -354,6 +354,62 @@ def test_train_lambdarank(ctx, oracle, schedule):
     _forests_equal(gm, om.info, om.arrays)
 
 
+def _train_both_ext(ctx, oracle, X, y, cfg, is_cat, schedule, goss=None, sizes=None):
+    oracle.set_exp_mode(1)
+    o = oracle.Binned(X, is_cat)
+    exts = []
+    for sched in (0, schedule):
+        e = make_ext(sched)
+        if goss:
+            e.goss_top_rate, e.goss_other_rate = goss
+        if sizes is not None:
+            e.query_group_sizes = A.ptr(sizes)
+            e.n_query_groups = len(sizes)
+        exts.append(e)
+    om = oracle.train(o, y, cfg, exts[0])
+    b = _binned(ctx, X, is_cat)
+    h = C.c_void_p()
+    A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0, C.byref(cfg),
+                              C.byref(exts[1]), C.byref(h)))
+    return GBDTModel(h, ctx), om
+
+
+SAMPLING_CASES = {
+    # name: (data builder, config kwargs, goss rates or None, schedules)
+    "uniform_depthwise": (lambda: binary_nan_data(20_000, 16, seed=31) + (None,),
+                          dict(objective=1, n_trees=5, max_depth=6, subsample=0.6), None, (1, 2)),
+    "colsample_leafwise": (lambda: regression_data(15_000, 40, seed=32) + (None,),
+                           dict(n_trees=5, growth=1, max_leaves=24, colsample_bytree=0.5),
+                           None, (1, 2)),
+    "softmax_colsample_subsample": (lambda: multiclass_cat_data(9000, 10, 2, 3, card=12, seed=33),
+                                    dict(objective=2, n_classes=3, n_trees=3, max_depth=5,
+                                         colsample_bytree=0.6, subsample=0.7), None, (1, 2)),
+    "goss_logistic": (lambda: binary_nan_data(30_000, 20, seed=34) + (None,),
+                      dict(objective=1, n_trees=6, max_depth=6, learning_rate=0.5), (0.2, 0.1),
+                      (2,)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(SAMPLING_CASES))
+def test_sampling_matches_oracle(ctx, oracle, case):
+    build, kw, goss, schedules = SAMPLING_CASES[case]
+    X, y, is_cat = build()
+    cfg = make_cfg(**kw)
+    for sched in schedules:
+        gm, om = _train_both_ext(ctx, oracle, X, y, cfg, is_cat, sched, goss=goss)
+        _forests_equal(gm, om.info, om.arrays)
+
+
+def test_goss_colsample_lambdarank(ctx, oracle):
+    """The ranking workload's settings at a small size: GOSS 0.2/0.1, half the
+    columns per tree, leaf-wise, LambdaRank gradients."""
+    X, y, sizes = ranking_data(12_000, 30, seed=35, group_lo=5, group_hi=120)
+    cfg = make_cfg(objective=3, n_trees=6, learning_rate=0.3, growth=1, max_leaves=31,
+                   colsample_bytree=0.5)
+    gm, om = _train_both_ext(ctx, oracle, X, y, cfg, None, 2, goss=(0.2, 0.1), sizes=sizes)
+    _forests_equal(gm, om.info, om.arrays)
+
+
 def test_predict_and_leaf_indices(ctx, oracle):
     X, y, is_cat = multiclass_cat_data(9000, 8, 2, 3, card=10, seed=11)
     cfg = make_cfg(objective=2, n_classes=3, n_trees=6, growth=1, max_leaves=15)
