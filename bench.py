@@ -214,6 +214,189 @@ def cpu_baseline(X, y, n_trees, depth, threads):
     return ms / n_trees, t_bin
 
 
+# ---------------------------------------------------------------------------
+# Other workloads (C1 regression, C3 softmax with categoricals, C4 ranking).
+# Same contract: one JSON line; `value` = train ms per tree (graph schedule).
+# ---------------------------------------------------------------------------
+def gen_c1(n, F, seed):
+    """X ~ N(0,1); y = x0 + 0.5 x1^2 + N(0, 0.1^2)."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((F, n), dtype=np.float32)
+    y = (X[0] + 0.5 * X[1].astype(np.float64) ** 2 + rng.normal(0, 0.1, n)).astype(np.float32)
+    return X, y, None, None
+
+
+def gen_c3(n, n_num, n_cat, K, card, seed):
+    """Numeric N(0,1) + categorical ints {0..card-1}; class = argmax_k W_k x + sum_j C_kj[x_j] + Gumbel."""
+    rng = np.random.default_rng(seed)
+    Xn = rng.standard_normal((n_num, n), dtype=np.float32)
+    Xc = rng.integers(0, card, size=(n_cat, n), dtype=np.int32)
+    W = rng.normal(0, 1.0 / math.sqrt(n_num), size=(K, n_num))
+    Cm = rng.normal(0, 1.0 / math.sqrt(n_cat), size=(K, n_cat, card))
+    z = W @ Xn.astype(np.float64)
+    for j in range(n_cat):
+        z += Cm[:, j, Xc[j]]
+    z += rng.gumbel(size=(K, n))
+    y = np.argmax(z, axis=0).astype(np.float32)
+    X = np.concatenate([Xn, Xc.astype(np.float32)], axis=0)
+    is_cat = np.zeros(n_num + n_cat, np.uint8)
+    is_cat[n_num:] = 1
+    return X, y, is_cat, None
+
+
+def gen_c4(n, F, seed, zero_p=0.3):
+    """X ~ U[0,1] with exact zeros (p 0.3); query sizes U{5..500}; label = quintile of
+    s = sum_f a_f [x_f > t_f]."""
+    rng = np.random.default_rng(seed)
+    X = rng.random((F, n), dtype=np.float32)
+    X[rng.random((F, n), dtype=np.float32) < zero_p] = 0.0
+    sizes = []
+    tot = 0
+    while tot < n:
+        sz = min(int(rng.integers(5, 501)), n - tot)
+        sizes.append(sz)
+        tot += sz
+    a = rng.standard_normal(F).astype(np.float32)
+    t = rng.random(F, dtype=np.float32)
+    s = np.zeros(n, np.float32)
+    for f in range(F):
+        s += a[f] * (X[f] > t[f])
+    q = np.quantile(s, [0.2, 0.4, 0.6, 0.8])
+    y = np.searchsorted(q, s, side="right").astype(np.float32)
+    return X, y, None, np.array(sizes, np.int32)
+
+
+WORKLOADS = {
+    # name: (description, generator, config overrides, ext overrides, default trees)
+    "c1": ("regression 5000 x 100, depth-wise 6, lr 0.1",
+           lambda: gen_c1(5000, 100, 0),
+           dict(objective=A.OBJ_SQUARED, growth=A.GROW_DEPTHWISE, max_depth=6, learning_rate=0.1),
+           {}, 50),
+    "c3": ("softmax K=10, 4000000 x (180 numeric + 20 categorical), leaf-wise 255",
+           lambda: gen_c3(4_000_000, 180, 20, 10, 32, 0),
+           dict(objective=A.OBJ_SOFTMAX, n_classes=10, growth=A.GROW_LEAFWISE, max_leaves=255),
+           {}, 5),
+    "c4": ("LambdaRank 2000000 x 136, GOSS 0.2/0.1, colsample 0.5, leaf-wise 127",
+           lambda: gen_c4(2_000_000, 136, 0),
+           dict(objective=A.OBJ_LAMBDARANK, growth=A.GROW_LEAFWISE, max_leaves=127,
+                colsample_bytree=0.5),
+           dict(goss_top_rate=0.2, goss_other_rate=0.1), 10),
+}
+
+
+def run_workload(args, L, ctx):
+    desc, gen, cfg_kw, ext_kw, default_steps = WORKLOADS[args.workload]
+    steps = args.steps if args.steps_given else default_steps
+    log(f"{args.workload}: generating {desc}")
+    X, y, is_cat, sizes = gen()
+    F, n = X.shape
+    t0 = time.perf_counter()
+    ds = C.c_void_p()
+    A.check(L.bst_dataset_create(ctx, A.ptr(X), n, F, A.ptr(is_cat), None, C.byref(ds)))
+    bin_ms = (time.perf_counter() - t0) * 1e3
+    y_dev = Dev(ctx, y)
+    log(f"binned in {bin_ms:.0f} ms")
+
+    def cfg_of(n_trees):
+        c = A.GBDTConfigC()
+        L.bst_gbdt_config_default(C.byref(c))
+        for k, v in cfg_kw.items():
+            setattr(c, k, v)
+        c.n_trees = n_trees
+        return c
+
+    def ext_of():
+        e = ext(A.SCHED_GRAPH)
+        for k, v in ext_kw.items():
+            setattr(e, k, v)
+        if sizes is not None:
+            e.query_group_sizes = A.ptr(sizes)
+            e.n_query_groups = len(sizes)
+        return e
+
+    K = cfg_kw.get("n_classes", 1) if cfg_kw["objective"] == A.OBJ_SOFTMAX else 1
+    rounds = max(1, steps)
+    if args.warmup > 0:
+        h, _ = train_timed(ctx, ds, y_dev, n, cfg_of(args.warmup), ext_of())
+        L.bst_model_destroy(h)
+    h_model, wall_ms = train_timed(ctx, ds, y_dev, n, cfg_of(rounds), ext_of())
+    n_trees = rounds * K
+    ms_tree = wall_ms / n_trees
+    log(f"graph: {ms_tree:.3f} ms/tree ({n_trees} trees in {wall_ms:.1f} ms)")
+    A.check(L.bst_ctx_set_profiling(ctx, 1))
+    h_prof, _ = train_timed(ctx, ds, y_dev, n, cfg_of(rounds), ext_of())
+    hist = kernel_stats(ctx, 0)
+    split = kernel_stats(ctx, 1)
+    part = kernel_stats(ctx, 2)
+    A.check(L.bst_ctx_set_profiling(ctx, 0))
+    L.bst_model_destroy(h_prof)
+    launches, hms, hbytes = hist
+    achieved = hbytes / (hms * 1e-3) / 1e9 if launches else 0.0
+    result = {
+        "metric": f"train ms/tree ({desc})",
+        "value": round(ms_tree, 4),
+        "unit": "ms/tree",
+        "n_gpus": 1,
+        "steps": rounds,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms / rounds, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 grad/hess, int64 fixed-point histograms, u8 bins",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "schedule": "graph",
+            "trees_per_step": K,
+            "binning_ms": round(bin_ms, 1),
+            "kernel_ms_per_tree": {
+                "hist": round(hms / n_trees, 4),
+                "split": round(split[1] / n_trees, 4),
+                "partition": round(part[1] / n_trees, 4),
+            },
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_hist",
+            "launches": launches,
+            "avg_launch_us": round(hms / launches * 1e3, 2) if launches else None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        threads = min(os.cpu_count() or 1, 16)
+        O.set_threads(threads)
+        m = min(n, args.cpu_rows)
+        log(f"cpu baseline: oracle, {args.cpu_trees} round(s) on the first {m} rows, {threads} threads")
+        Xs = np.ascontiguousarray(X[:, :m])
+        ys = y[:m]
+        b = O.Binned(Xs, is_cat)
+        e = ext_of()
+        e.schedule = 0
+        if sizes is not None:
+            # whole query groups covering the first m rows
+            cs = np.cumsum(sizes)
+            ng = int(np.searchsorted(cs, m, side="left")) + 1
+            ss = sizes[:ng].copy()
+            ss[-1] -= int(cs[ng - 1] - m)
+            ss = ss[ss > 0]
+            e.query_group_sizes = A.ptr(ss)
+            e.n_query_groups = len(ss)
+        cm = O.train(b, ys, cfg_of(args.cpu_trees), e)
+        _, ms = cm.stats()
+        result["cpu_baseline"] = {
+            "value": round(ms / (args.cpu_trees * K), 2),
+            "unit": "ms/tree",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {m} of {n} rows, {args.cpu_trees} round(s) ({args.cpu_trees * K} trees)",
+        }
+    L.bst_model_destroy(h_model)
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,9 +411,14 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--qps", type=float, default=1000.0)
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4"], default="c2",
+                    help="c2 is the headline workload; the others print their own line")
+    ap.add_argument("--cpu-rows", type=int, default=200_000,
+                    help="rows of the CPU baseline sample for c1/c3/c4")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the plain copy kernel 5x on this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU row "
               "sharding is described in DESIGN.md, not built)", file=sys.stderr)
@@ -245,6 +433,9 @@ def main():
         for _ in range(5):
             A.check(L.bst_kernel_copy(ctx, dst.p, src.p, nbytes))
         print(json.dumps({"copy_bytes": nbytes, "launches": 5}), flush=True)
+        return
+    if args.workload != "c2":
+        run_workload(args, L, ctx)
         return
     n, F, depth = args.rows, args.features, args.depth
     log(f"generating {n} x {F} binary-logistic data with 10% NaN")

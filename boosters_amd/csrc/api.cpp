@@ -274,6 +274,7 @@ int bst_ctx_create(int device, bst_ctx** out) {
       raise(BST_E_DEVICE, std::string("engine is built for gfx950, device is ") + prop.gcnArchName);
     auto* c = new bst_ctx();
     c->device = device;
+    c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     BST_HIP(hipSetDevice(device));
     BST_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c;

@@ -54,6 +54,25 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total
   return r;
 }
 
+// Gain as an unsigned key with the float order (-inf lowest).
+__device__ __forceinline__ uint32_t ord_gain(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ unsigned long long block_max_u64(unsigned long long v, unsigned long long* s_w) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const unsigned long long o = __shfl_xor(v, m, 64);
+    v = o > v ? o : v;
+  }
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long r = 0ull;
+  for (int i = 0; i < kCtlThreads / 64; i++) r = s_w[i] > r ? s_w[i] : r;
+  __syncthreads();
+  return r;
+}
+
 __device__ __forceinline__ bool expandable(const DevSplit& s, double min_gain) {
   return s.gain > -INFINITY && (double)s.gain > min_gain;
 }
@@ -121,6 +140,27 @@ __device__ void advance_level(const CtlParams& P, const DeviceBufs& B) {
   }
 }
 
+// Histogram tile of a step: the power of two in [hist_tile_min, hist_tile]
+// that gives about hist_target (tile, group) items for `rows` rows, so small
+// steps still spread over the CUs and large ones do not make needless
+// partials.
+__device__ uint32_t step_tile(const CtlParams& P, uint64_t rows) {
+  const uint64_t want = (rows * P.NG + P.hist_target - 1) / P.hist_target;
+  uint32_t tile = P.hist_tile_min;
+  while (tile < want && tile < P.hist_tile) tile <<= 1;
+  return tile;
+}
+
+__device__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long* s_w) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long r = 0ull;
+  for (int i = 0; i < kCtlThreads / 64; i++) r += s_w[i];
+  __syncthreads();
+  return r;
+}
+
 }  // namespace
 
 // Root state of every tree, root histogram work, its finish entry and the
@@ -129,6 +169,9 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
                                                           const uint32_t* __restrict__ rows) {
   __shared__ uint32_t s_hist;
   if (threadIdx.x == 0) s_hist = 0;
+  uint64_t all_rows = 0;
+  for (uint32_t t = 0; t < P.T; t++) all_rows += rows[t];
+  const uint32_t tile = step_tile(P, all_rows);
   __syncthreads();
   for (uint32_t t = 0; t < P.T; t++) {
     for (uint32_t p = threadIdx.x; p < P.MN; p += blockDim.x) {
@@ -139,7 +182,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
       B.free_slot[(uint64_t)t * P.SPT + i] = P.SPT - 1 - i;  // slot 0 goes to the root
     __syncthreads();
     const uint32_t m = rows[t];
-    const uint32_t nt = (m + P.hist_tile - 1) / P.hist_tile;
+    const uint32_t nt = (m + tile - 1) / tile;
     if (threadIdx.x == 0) {
       TreeCtl c;
       c.n_part = 1;
@@ -172,13 +215,13 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
     __syncthreads();
     const uint32_t base = s_hist;
     for (uint32_t k = threadIdx.x; k < nt * P.NG; k += blockDim.x) {
-      const uint32_t tile = k / P.NG, g = k % P.NG;
-      const uint32_t b0 = tile * P.hist_tile;
+      const uint32_t q = k / P.NG, g = k % P.NG;
+      const uint32_t b0 = q * tile;
       HistItem it;
       it.node = t * P.MN;
       it.slot = t * P.SPT;
       it.begin = (uint32_t)(t * P.stride) + b0;
-      it.count = min(P.hist_tile, m - b0);
+      it.count = min(tile, m - b0);
       it.group = g;
       it.tree = t | (nt == 1 ? kDirectFlag : 0u);
       B.hitems[base + k] = it;
@@ -207,7 +250,8 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
                                                           int advance) {
   __shared__ uint32_t s_tmp[kCtlThreads / 64 + 1];
   __shared__ uint32_t s_items, s_tiles, s_nch;
-  __shared__ uint32_t s_part, s_tnodes;
+  __shared__ uint32_t s_part, s_tnodes, s_best;
+  __shared__ unsigned long long s_max64[kCtlThreads / 64];
   const uint32_t n_sp = B.counts[C_SPLIT];
   for (uint32_t i = threadIdx.x; i < n_sp; i += blockDim.x) {
     const uint32_t node = B.split_nodes[i];
@@ -236,25 +280,31 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   __syncthreads();
   for (uint32_t t = 0; t < P.T; t++) {
     const uint64_t ob = (uint64_t)t * P.WO, cb = (uint64_t)t * P.W;
+    if (P.growth == 1) {
+      // Block-parallel argmax of (gain, partition id) over the open set.
+      const uint32_t n_open = B.ctl[t].n_open;
+      // Partition ids are unique in the open set, so the key picks one node.
+      unsigned long long best = 0ull;
+      for (uint32_t i = threadIdx.x; i < n_open; i += blockDim.x) {
+        const uint32_t p = B.open_node[ob + i];
+        const float g = B.node_split[t * P.MN + p].gain + 0.0f;  // -0 ties with +0
+        const unsigned long long key = ((unsigned long long)ord_gain(g) << 32) | p;
+        best = key > best ? key : best;
+      }
+      best = block_max_u64(best, s_max64);
+      for (uint32_t i = threadIdx.x; i < n_open; i += blockDim.x)
+        if (B.open_node[ob + i] == (uint32_t)(best & 0xFFFFFFFFull)) s_best = i;
+      __syncthreads();
+    }
     if (threadIdx.x == 0) {
       TreeCtl& c = B.ctl[t];
       uint32_t nch = 0;
       if (P.growth == 0) {
         if (c.depth < P.max_depth) nch = c.n_open;
       } else if (c.n_leaves < P.max_leaves && c.n_open > 0) {
-        // Max (gain, partition id): BinaryHeap order of NodeCandidate.
-        uint32_t best = 0;
-        float bg = 0.0f;
-        uint32_t bp = 0;
-        for (uint32_t i = 0; i < c.n_open; i++) {
-          const uint32_t p = B.open_node[ob + i];
-          const float g = B.node_split[t * P.MN + p].gain;
-          if (i == 0 || g > bg || (g == bg && p > bp)) {
-            best = i;
-            bg = g;
-            bp = p;
-          }
-        }
+        // Max (gain, partition id): BinaryHeap order of NodeCandidate; the
+        // block found it (s_best) before this thread-0 section.
+        const uint32_t best = s_best;
         B.ch_node[cb] = B.open_node[ob + best];
         B.ch_tnode[cb] = B.open_tnode[ob + best];
         B.ch_depth[cb] = B.open_depth[ob + best];
@@ -395,8 +445,20 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
     B.counts[C_FIN] = 0;
     B.counts[C_SPLIT] = 0;
   }
+  __shared__ unsigned long long s_red64[kCtlThreads / 64];
   const uint32_t n_items = B.counts[C_ITEMS];
   const bool lds_sums = n_items <= kChildLdsItems;
+  // Rows this step histograms (smaller children that will be split), which
+  // sets the step's tile size.
+  unsigned long long step_rows = 0ull;
+  for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) {
+    const uint32_t cdepth = B.pend[5 * i + 4];
+    if (P.growth == 1 || cdepth < P.max_depth) {
+      const uint32_t L = B.cursor[2 * i], R = B.items[i].count - L;
+      step_rows += L <= R ? L : R;
+    }
+  }
+  const uint32_t tile = step_tile(P, block_sum_u64(step_rows, s_red64));
   if (lds_sums) {
     // Children's exact totals: every thread adds whole partition tiles.
     for (uint32_t i = threadIdx.x; i < 4 * n_items; i += blockDim.x) (&s_sum[0][0])[i] = 0;
@@ -458,7 +520,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
       const uint32_t ss = B.free_slot[(uint64_t)t * P.SPT + k];
       B.pslot[small] = ss;
       const uint32_t gss = t * P.SPT + ss, gls = t * P.SPT + ps;
-      const uint32_t nt = (scount + P.hist_tile - 1) / P.hist_tile;
+      const uint32_t nt = (scount + tile - 1) / tile;
       atomicAdd(&B.counts[C_HROWS], scount);
       atomicAdd(&B.counts[C_HNODES], 1u);
       const uint32_t h0 = atomicAdd(&B.counts[C_HIST], nt * P.NG);
@@ -467,8 +529,8 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
           HistItem h;
           h.node = small;
           h.slot = gss;
-          h.begin = sbegin + q * P.hist_tile;
-          h.count = min(P.hist_tile, scount - q * P.hist_tile);
+          h.begin = sbegin + q * tile;
+          h.count = min(tile, scount - q * tile);
           h.group = g;
           h.tree = t | (nt == 1 ? kDirectFlag : 0u) | (cbuf ? kAltFlag : 0u);
           B.hitems[h0 + q * P.NG + g] = h;

@@ -110,6 +110,7 @@ float round_down_f32(double b);
 
 struct bst_ctx {
   int device = 0;
+  int n_cu = 256;  // compute units (sizes grids of resident, grid-stride kernels)
   hipStream_t stream = nullptr;
   double last_ms_total = 0, last_ms_per_tree = 0;
   int64_t last_hist_rows = 0;

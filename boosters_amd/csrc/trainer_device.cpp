@@ -104,7 +104,12 @@ void Trainer::device_setup() {
   P.E = (uint32_t)ds_->d.E;
   P.FG = (uint32_t)ds_->d.FG;
   P.hist_tile = kHistTileD;
-  if (const char* e = std::getenv("BST_HIST_TILE")) P.hist_tile = (uint32_t)std::max(512, std::atoi(e));
+  P.hist_tile_min = 1024;
+  P.hist_target = (uint32_t)std::max(64, 2 * ctx_->n_cu);
+  if (const char* e = std::getenv("BST_HIST_TILE")) {
+    P.hist_tile = (uint32_t)std::max(512, std::atoi(e));
+    P.hist_tile_min = P.hist_tile;  // fixed tile
+  }
   P.part_tile = kPartTileD;
   P.seg_tile = kSegTileD;
   P.rightmost = (uint32_t)sp_.rightmost;
@@ -122,7 +127,9 @@ void Trainer::device_setup() {
   // Worst cases per step: every tree expands W nodes whose rows sum to n.
   st->cap_items = T * P.W;
   st->cap_tiles = T * (tiles_n + P.W);
-  st->cap_hist = T * (hist_n + P.W) * P.NG;
+  // Adaptive tiles give at most ~hist_target items for the rows, or the
+  // largest-tile count, plus one partial tile per node.
+  st->cap_hist = std::max<uint32_t>(P.hist_target + P.NG, T * hist_n * P.NG) + T * P.W * P.NG + P.NG;
   st->cap_fin = std::max<uint32_t>(T, T * P.W);
   st->cap_split = std::max<uint32_t>(T, 2 * T * P.W);
   st->cap_lsegs = T * (segs_n + MN_);
@@ -260,7 +267,10 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
   const auto& cfg = opt_.cfg;
   const int K = opt_.K;
   const auto& d = ds_->d;
-  const int g_hist = grid_for(S.cap_hist, kMaxGrid);
+  // The histogram kernel keeps one block per CU resident (LDS), so a grid of
+  // one block per CU walking the items is as fast as a larger one and costs
+  // nothing on steps with few items.  A multiple of 8 keeps the XCD mapping.
+  const int g_hist = grid_for(S.cap_hist, (ctx_->n_cu + 7) / 8 * 8);
   const int g_tiles = grid_for(S.cap_tiles, kMaxGrid);
   const int g_split = grid_for(S.cap_split, 512);
   const int g_lsegs = grid_for(S.cap_lsegs, kMaxGrid);
