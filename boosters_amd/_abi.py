@@ -246,6 +246,13 @@ def _declare(lib):
         "bst_model_export": (C.c_int, [vp, P(ForestArrays)]),
         "bst_model_import": (C.c_int, [vp, P(ModelInfo), P(ForestArrays), P(vp)]),
         "bst_model_destroy": (C.c_int, [vp]),
+        "bst_model_save_bstr": (C.c_int, [vp, C.c_int32, P(P(C.c_uint8)), P(C.c_int64)]),
+        "bst_model_load_bstr": (C.c_int, [vp, C.c_char_p, C.c_int64, P(vp)]),
+        "bst_model_save_bstr_file": (C.c_int, [vp, C.c_char_p, C.c_int32]),
+        "bst_model_load_bstr_file": (C.c_int, [vp, C.c_char_p, P(vp)]),
+        "bst_model_save_json": (C.c_int, [vp, C.c_int32, P(C.c_void_p), P(C.c_int64)]),
+        "bst_model_load_json": (C.c_int, [vp, C.c_char_p, C.c_int64, P(vp)]),
+        "bst_buffer_free": (None, [vp]),
         "bst_predict": (
             C.c_int,
             [vp, vp, P(C.c_float), C.c_int64, C.c_int32, C.c_int32, P(C.c_float), P(C.c_uint32)],
@@ -309,7 +316,9 @@ def exported_symbols():
         "bst_dataset_destroy", "bst_dataset_feature_info", "bst_dataset_bin_bounds",
         "bst_dataset_get_bins", "bst_train", "bst_train_device", "bst_last_train_stats",
         "bst_last_train_metric", "bst_model_info_get", "bst_model_export", "bst_model_import",
-        "bst_model_destroy", "bst_predict", "bst_predict_device", "bst_server_create",
+        "bst_model_destroy", "bst_model_save_bstr", "bst_model_load_bstr",
+        "bst_model_save_bstr_file", "bst_model_load_bstr_file", "bst_model_save_json",
+        "bst_model_load_json", "bst_buffer_free", "bst_predict", "bst_predict_device", "bst_server_create",
         "bst_server_submit", "bst_server_wait", "bst_server_destroy", "bst_kernel_gradients", "bst_kernel_lambdarank",
         "bst_kernel_histogram", "bst_dataset_hist_layout", "bst_kernel_partition",
         "bst_kernel_find_split",

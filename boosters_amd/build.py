@@ -24,6 +24,7 @@ SOURCES = [
     "trainer_device.cpp",
     "api.cpp",
     "server.cpp",
+    "persist.cpp",
 ]
 
 # -ffp-contract=off: every f32/f64 operation rounds as written (no FMA

@@ -23,29 +23,6 @@ namespace {
 
 #define g_err (::bst::last_error_ref())
 
-template <class Fn>
-int guard(Fn&& fn) {
-  try {
-    return fn();
-  } catch (const Error& e) {
-    g_err = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "host allocation failed";
-    return BST_E_OOM;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return BST_E_INVALID;
-  } catch (...) {
-    g_err = "unknown error";
-    return BST_E_INVALID;
-  }
-}
-
-void need(bool ok, int code, const char* msg) {
-  if (!ok) raise(code, msg);
-}
-
 bst_binning_config default_binning() {
   bst_binning_config b;
   b.max_bins = 256;
@@ -640,9 +617,10 @@ int bst_model_import(bst_ctx* ctx, const bst_model_info* info, const bst_forest_
                      bst_model** out) {
   return guard([&] {
     need(info && in && out, BST_E_INVALID, "null argument");
-    need(info->n_trees >= 0 && info->n_groups >= 1 && info->n_groups <= kPredMaxGroups,
+    need(info->n_trees >= 0 && info->n_groups >= 1 && info->n_groups <= kPredMaxGroups &&
+             info->total_nodes >= 0 && info->total_cat_words >= 0 && info->n_features >= 0,
          BST_E_FORMAT, "bad model shape");
-    auto* m = new bst_model();
+    std::unique_ptr<bst_model> m(new bst_model());
     m->info = *info;
     const int64_t T = info->n_trees, N = info->total_nodes, W = info->total_cat_words;
     m->base_score.assign(in->base_score, in->base_score + info->n_groups);
@@ -656,39 +634,15 @@ int bst_model_import(bst_ctx* ctx, const bst_model_info* info, const bst_forest_
     m->is_leaf.assign(in->is_leaf, in->is_leaf + N);
     m->split_type.assign(in->split_type, in->split_type + N);
     m->leaf_value.assign(in->leaf_value, in->leaf_value + N);
-    m->gain.assign(in->gain ? in->gain : in->leaf_value, (in->gain ? in->gain : in->leaf_value) + N);
-    m->cover.assign(in->cover ? in->cover : in->leaf_value, (in->cover ? in->cover : in->leaf_value) + N);
+    // Gains and covers are optional (tree.rs:146-161): absent means zeros.
+    if (in->gain) m->gain.assign(in->gain, in->gain + N); else m->gain.assign(N, 0.0f);
+    if (in->cover) m->cover.assign(in->cover, in->cover + N); else m->cover.assign(N, 0.0f);
     m->cat_start.assign(in->cat_start, in->cat_start + N);
     m->cat_size.assign(in->cat_size, in->cat_size + N);
     if (W > 0) m->cat_words.assign(in->cat_words, in->cat_words + W);
-    // Structural validation (Tree::validate-style): ids in range, groups in range.
-    for (int64_t t = 0; t < T; t++) {
-      const int64_t b = m->tree_offset[t], e = m->tree_offset[t + 1];
-      if (b < 0 || e > N || e <= b) {
-        delete m;
-        raise(BST_E_FORMAT, "bad tree offsets");
-      }
-      if (m->tree_group[t] >= (uint32_t)info->n_groups) {
-        delete m;
-        raise(BST_E_FORMAT, "tree group out of range");
-      }
-      for (int64_t i = b; i < e; i++) {
-        if (!m->is_leaf[i] && (m->left[i] >= (uint64_t)(e - b) || m->right[i] >= (uint64_t)(e - b))) {
-          delete m;
-          raise(BST_E_FORMAT, "child id out of range");
-        }
-        if (!m->is_leaf[i] && (int64_t)m->split_index[i] >= info->n_features) {
-          delete m;
-          raise(BST_E_FORMAT, "split feature out of range");
-        }
-        if (m->cat_size[i] && (int64_t)m->cat_start[i] + m->cat_size[i] > W) {
-          delete m;
-          raise(BST_E_FORMAT, "category bitset out of range");
-        }
-      }
-    }
+    check_forest(*m);
     (void)ctx;
-    *out = m;
+    *out = m.release();
     return BST_OK;
   });
 }

@@ -24,6 +24,31 @@ struct Error {
 // Thread-local message returned by bst_last_error() (defined in api.cpp).
 std::string& last_error_ref();
 
+// Runs one C-ABI entry point: exceptions become status codes and the message
+// of bst_last_error(); nothing throws across the boundary.
+template <class Fn>
+int guard(Fn&& fn) {
+  try {
+    return fn();
+  } catch (const Error& e) {
+    last_error_ref() = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    last_error_ref() = "host allocation failed";
+    return BST_E_OOM;
+  } catch (const std::exception& e) {
+    last_error_ref() = e.what();
+    return BST_E_INVALID;
+  } catch (...) {
+    last_error_ref() = "unknown error";
+    return BST_E_INVALID;
+  }
+}
+
+inline void need(bool ok, int code, const char* msg) {
+  if (!ok) raise(code, msg);
+}
+
 #define BST_HIP(x)                                                                   \
   do {                                                                               \
     hipError_t e_ = (x);                                                             \
@@ -190,6 +215,14 @@ struct bst_model {
   std::vector<uint32_t> tree_group, split_index, left, right, cat_start, cat_size, cat_words;
   std::vector<float> split_threshold, leaf_value, gain, cover;
   std::vector<uint8_t> default_left, is_leaf, split_type;
+  // Model metadata carried through .bstr files (persist/schema.rs:27-44):
+  // meta.num_groups (-1: same as info.n_groups), optional feature names,
+  // feature types (0 numeric, 1 categorical) and objective name.
+  int64_t meta_num_groups = -1;
+  bool has_feature_names = false, has_feature_types = false, has_objective_name = false;
+  std::vector<std::string> feature_names;
+  std::vector<uint8_t> feature_types;
+  std::string objective_name;
   // Device copy for traversal (built on first predict).
   int device_for = -1;
   bst::DevBuf<bst::DevNode> d_nodes;
@@ -198,3 +231,12 @@ struct bst_model {
   bst::DevBuf<float> d_base;
   void upload(hipStream_t s);
 };
+
+namespace bst {
+// Structural check of a forest before it is traversed on the device: array
+// lengths, groups, and per tree an acyclic walk from the root in which every
+// split node has in-range, distinct, unvisited children (the rules of
+// repr/gbdt/tree_view.rs:216-290), split features < n_features and < 2^24,
+// and category bitsets inside cat_words.  Raises BST_E_FORMAT.
+void check_forest(const bst_model& m);
+}  // namespace bst

@@ -10,6 +10,7 @@ Predictions are returned as (n_samples, n_outputs), the bindings' convention.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
@@ -354,11 +355,58 @@ class GBDTModel:
         A.check(A.lib().bst_model_import(ctx.h, C.byref(info), C.byref(fa), C.byref(h)))
         return GBDTModel(h, ctx)
 
+    # -- persistence (persist/mod.rs:88-160; bindings src/model/gbdt.rs:259-327) --
+    def to_bytes(self, compression_level: int = 3) -> bytes:
+        """The model as .bstr bytes (zstd level 3 by default, 0 = uncompressed)."""
+        buf, size = C.POINTER(C.c_uint8)(), C.c_int64()
+        A.check(A.lib().bst_model_save_bstr(self.h, int(compression_level), C.byref(buf), C.byref(size)))
+        try:
+            return C.string_at(buf, size.value)
+        finally:
+            A.lib().bst_buffer_free(buf)
+
+    def to_json_bytes(self, pretty: bool = False) -> bytes:
+        """The model as .bstr.json bytes (UTF-8)."""
+        buf, size = C.c_void_p(), C.c_int64()
+        A.check(A.lib().bst_model_save_json(self.h, int(pretty), C.byref(buf), C.byref(size)))
+        try:
+            return C.string_at(buf, size.value)
+        finally:
+            A.lib().bst_buffer_free(buf)
+
+    @staticmethod
+    def from_bytes(data: bytes, ctx: Optional[Context] = None) -> "GBDTModel":
+        """Load .bstr bytes; raises BoostersError (code BST_E_FORMAT) on bad input."""
+        h = C.c_void_p()
+        A.check(A.lib().bst_model_load_bstr(ctx.h if ctx else None, bytes(data), len(data), C.byref(h)))
+        return GBDTModel(h, ctx)
+
+    @staticmethod
+    def from_json_bytes(data: bytes, ctx: Optional[Context] = None) -> "GBDTModel":
+        h = C.c_void_p()
+        A.check(A.lib().bst_model_load_json(ctx.h if ctx else None, bytes(data), len(data), C.byref(h)))
+        return GBDTModel(h, ctx)
+
+    def save(self, path: str, compression_level: int = 3) -> None:
+        A.check(A.lib().bst_model_save_bstr_file(self.h, os.fsencode(path), int(compression_level)))
+
+    @staticmethod
+    def load(path: str, ctx: Optional[Context] = None) -> "GBDTModel":
+        h = C.c_void_p()
+        A.check(A.lib().bst_model_load_bstr_file(ctx.h if ctx else None, os.fsencode(path), C.byref(h)))
+        return GBDTModel(h, ctx)
+
+    def __reduce__(self):
+        """Pickle through the .bstr bytes (bindings src/model/gbdt.rs:180-188)."""
+        return (GBDTModel.from_bytes, (self.to_bytes(),))
+
     def _predict(self, data, raw: bool, leaf_index: bool = False):
         fm = data.feature_major if isinstance(data, Dataset) else np.ascontiguousarray(data, np.float32)
         F, n = fm.shape
         out = np.zeros((self.info.n_groups, n), np.float32)
         li = np.zeros((self.info.n_trees, n), np.uint32) if leaf_index else None
+        if self.ctx is None:  # loaded without a context: bind the default device now
+            self.ctx = Context.default()
         A.check(A.lib().bst_predict(self.ctx.h, self.h, A.ptr(fm), n, F, int(raw), A.ptr(out), A.ptr(li)))
         return (out, li) if leaf_index else out
 
@@ -385,6 +433,8 @@ class Server:
     def __init__(self, model: GBDTModel, hbm_budget_bytes: int = 1 << 30, max_inflight: int = 4):
         self.model = model
         self.h = C.c_void_p()
+        if model.ctx is None:
+            model.ctx = Context.default()
         A.check(A.lib().bst_server_create(model.ctx.h, model.h, int(hbm_budget_bytes),
                                           int(max_inflight), C.byref(self.h)))
         self._keep = {}

@@ -238,6 +238,34 @@ int bst_model_import(bst_ctx* ctx, const bst_model_info* info, const bst_forest_
                      bst_model** out);
 int bst_model_destroy(bst_model* model);
 
+/* ---- persistence: .bstr files ------------------------------------------ */
+/* GBDTModel::write_into (persist/mod.rs:105-111; binary.rs:52-99): the .bstr
+ * bytes of the model -- 32-byte header ("BSTR", version 2, format, model type
+ * 1), MessagePack of the model schema (persist/schema.rs:27-149), 16-byte
+ * trailer (payload length, CRC32C).  compression_level 0 = uncompressed
+ * (format byte 1); 1..22 = zstd level (format byte 2, needs libzstd.so.1);
+ * the reference's default is 3 (binary.rs:16).  *out is allocated by the
+ * library: release it with bst_buffer_free. */
+int bst_model_save_bstr(const bst_model* model, int32_t compression_level, uint8_t** out,
+                        int64_t* size);
+/* GBDTModel::read_from (persist/mod.rs:114-117; binary.rs:101-146;
+ * convert.rs:207-470): header, trailer length and CRC32C, optional zstd, the
+ * schema, then the forest checks of Tree::validate / Forest::validate.  Any
+ * defect returns BST_E_FORMAT.  ctx may be NULL (the model is host-side until
+ * its first prediction). */
+int bst_model_load_bstr(bst_ctx* ctx, const uint8_t* buf, int64_t size, bst_model** out);
+/* The same through a file path. */
+int bst_model_save_bstr_file(const bst_model* model, const char* path, int32_t compression_level);
+int bst_model_load_bstr_file(bst_ctx* ctx, const char* path, bst_model** out);
+/* write_json_into / read_json_from (persist/mod.rs:120-143; json.rs:50-75):
+ * {"bstr_version":2,"model_type":"gbdt","model":{schema}}.  pretty != 0
+ * indents by two spaces.  *out is NUL-terminated, *size excludes the NUL;
+ * release it with bst_buffer_free. */
+int bst_model_save_json(const bst_model* model, int32_t pretty, char** out, int64_t* size);
+int bst_model_load_json(bst_ctx* ctx, const char* buf, int64_t size, bst_model** out);
+/* Frees a buffer returned by bst_model_save_bstr / bst_model_save_json. */
+void bst_buffer_free(void* p);
+
 /* ---- prediction -------------------------------------------------------- */
 /* GBDTModel::predict / predict_raw (model.rs:217-258).  feat_major host
  * [n_feat][n_rows]; out [n_groups][n_rows] caller-allocated; leaf_idx

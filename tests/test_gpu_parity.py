@@ -423,6 +423,29 @@ def test_predict_and_leaf_indices(ctx, oracle):
         np.testing.assert_allclose(out, ref, rtol=1e-6, atol=1e-7)
 
 
+def test_bstr_round_trip_predictions(ctx, oracle):
+    """A trained forest written as .bstr (MessagePack, zstd) and .bstr.json and
+    read back (persist/mod.rs:88-143) has the same arrays and predicts the
+    same values and leaves on the device."""
+    X, y, is_cat = multiclass_cat_data(6000, 8, 2, 3, card=10, seed=13)
+    cfg = make_cfg(objective=2, n_classes=3, n_trees=4, growth=1, max_leaves=15)
+    gm, om, _ = _train_both(ctx, oracle, X, y, cfg, is_cat)
+    Xt, _, _ = multiclass_cat_data(3000, 8, 2, 3, card=10, seed=14)
+    Xt[1, :40] = np.nan
+    want, wli = gm.predict_feature_major(Xt, raw=False, leaf_index=True)
+    ga = gm.export()
+    assert ga["split_type"].any()
+    for m2 in (GBDTModel.from_bytes(gm.to_bytes(0), ctx), GBDTModel.from_bytes(gm.to_bytes(3), ctx),
+               GBDTModel.from_json_bytes(gm.to_json_bytes(), ctx)):
+        ea = m2.export()
+        for k in ("split_index", "split_threshold", "left", "right", "default_left", "is_leaf",
+                  "split_type", "leaf_value", "tree_group", "base_score"):
+            np.testing.assert_array_equal(ea[k], ga[k], err_msg=k)
+        got, li = m2.predict_feature_major(Xt, raw=False, leaf_index=True)
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(li, wli)
+
+
 def test_validation_loss_matches(ctx, oracle):
     X, y = binary_nan_data(20_000, 16, seed=21)
     cfg = make_cfg(objective=1, n_trees=10, learning_rate=0.1, max_depth=6)
