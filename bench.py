@@ -93,6 +93,15 @@ def train_timed(ctx, ds, y_dev, n, cfg, e):
     return h, (time.perf_counter() - t0) * 1e3
 
 
+def tuning(ctx):
+    """Histogram launch shape of the last train (autotuned or fixed)."""
+    v = [C.c_int32() for _ in range(4)]
+    ms = C.c_double()
+    A.check(A.lib().bst_last_train_tuning(ctx, *[C.byref(x) for x in v], C.byref(ms)))
+    return {"hist_fg": v[0].value, "hist_rows_min": v[1].value, "hist_blocks_per_cu": v[2].value,
+            "autotuned": bool(v[3].value)}
+
+
 def kernel_stats(ctx, kind):
     la, ms, by = C.c_int64(), C.c_double(), C.c_double()
     A.check(A.lib().bst_ctx_kernel_stats(ctx, kind, C.byref(la), C.byref(ms), C.byref(by)))
@@ -320,6 +329,7 @@ def run_workload(args, L, ctx):
         h, _ = train_timed(ctx, ds, y_dev, n, cfg_of(args.warmup), ext_of())
         L.bst_model_destroy(h)
     h_model, wall_ms = train_timed(ctx, ds, y_dev, n, cfg_of(rounds), ext_of())
+    tune = tuning(ctx)
     n_trees = rounds * K
     ms_tree = wall_ms / n_trees
     log(f"graph: {ms_tree:.3f} ms/tree ({n_trees} trees in {wall_ms:.1f} ms)")
@@ -348,6 +358,7 @@ def run_workload(args, L, ctx):
         "config": {
             "workload": desc,
             "schedule": "graph",
+            "hist_tuning": tune,
             "trees_per_step": K,
             "binning_ms": round(bin_ms, 1),
             "kernel_ms_per_tree": {
@@ -420,8 +431,8 @@ def main():
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU row "
-              "sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (row sharding "
+              "across GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     L = A.lib()
@@ -453,6 +464,7 @@ def main():
         h, _ = train_timed(ctx, ds, y_dev, n, cfg_c2(args.warmup, depth), ext(A.SCHED_GRAPH))
         L.bst_model_destroy(h)
     h_model, wall_ms = train_timed(ctx, ds, y_dev, n, cfg_c2(args.steps, depth), ext(A.SCHED_GRAPH))
+    tune = tuning(ctx)
     ms_tree = wall_ms / args.steps
     log(f"graph: {ms_tree:.3f} ms/tree ({args.steps} trees in {wall_ms:.1f} ms)")
 
@@ -517,6 +529,7 @@ def main():
         "config": {
             "workload": f"binary logistic {n} x {F}, 10% NaN, depth-wise {depth}, 256 bins, lr 0.3",
             "schedule": "graph (device-side growth, one hipGraph replay per boosting iteration)",
+            "hist_tuning": tune,
             "batched_ms_per_tree": round(batched_ms_tree, 4),
             "plain_ms_per_tree": None if plain_ms_tree is None else round(plain_ms_tree, 4),
             "graph_speedup_vs_plain": None if plain_ms_tree is None else round(plain_ms_tree / ms_tree, 2),

@@ -25,6 +25,7 @@ SOURCES = [
     "api.cpp",
     "server.cpp",
     "persist.cpp",
+    "autotune.cpp",
 ]
 
 # -ffp-contract=off: every f32/f64 operation rounds as written (no FMA

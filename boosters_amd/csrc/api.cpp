@@ -230,7 +230,7 @@ void bst_gbdt_config_default(bst_gbdt_config* c) {
 void bst_ext_config_default(bst_ext_config* e) {
   if (!e) return;
   std::memset(e, 0, sizeof(*e));
-  e->schedule = BST_SCHED_BATCHED;
+  e->schedule = BST_SCHED_GRAPH;
   e->n_threads = 0;
   e->autotune = 1;
 }
@@ -482,6 +482,17 @@ static int train_impl(bst_ctx* ctx, bst_dataset* ds, const float* y_dev, int64_t
   bst_ext_config_default(&opt.ext);
   if (ext) opt.ext = *ext;
   opt.K = cfg->objective == BST_OBJ_SOFTMAX ? (int)cfg->n_classes : 1;
+  need(opt.ext.schedule >= BST_SCHED_PLAIN && opt.ext.schedule <= BST_SCHED_GRAPH, BST_E_CONFIG,
+       "unknown schedule");
+  need(opt.ext.hist_fg == 0 || opt.ext.hist_fg == 16 || opt.ext.hist_fg == 32, BST_E_CONFIG,
+       "hist_fg must be 0, 16 or 32");
+  {
+    const int32_t r = opt.ext.hist_rows_min;
+    need(r == 0 || (r >= 512 && r <= 8192 && (r & (r - 1)) == 0), BST_E_CONFIG,
+         "hist_rows_min must be 0 or a power of two in 512..8192");
+  }
+  need(opt.ext.hist_blocks_per_cu >= 0 && opt.ext.hist_blocks_per_cu <= 8, BST_E_CONFIG,
+       "hist_blocks_per_cu must be in 0..8");
   if (opt.ext.goss_top_rate > 0.0f) {  // RowSamplingParams::Goss
     need(opt.ext.goss_top_rate < 1.0f && opt.ext.goss_other_rate > 0.0f &&
              opt.ext.goss_other_rate <= 1.0f &&
@@ -568,6 +579,20 @@ int bst_last_train_stats(bst_ctx* ctx, double* ms_total, double* ms_per_tree, in
     if (ms_total) *ms_total = ctx->last_ms_total;
     if (ms_per_tree) *ms_per_tree = ctx->last_ms_per_tree;
     if (hr) *hr = ctx->last_hist_rows;
+    return BST_OK;
+  });
+}
+
+int bst_last_train_tuning(bst_ctx* ctx, int32_t* hist_fg, int32_t* hist_rows_min,
+                          int32_t* hist_blocks_per_cu, int32_t* tuned, double* tune_ms) {
+  return guard([&] {
+    need(ctx, BST_E_INVALID, "null context");
+    const HistTuning& t = ctx->last_tuning;
+    if (hist_fg) *hist_fg = t.fg;
+    if (hist_rows_min) *hist_rows_min = (int32_t)t.tile_min;
+    if (hist_blocks_per_cu) *hist_blocks_per_cu = (int32_t)t.blocks_per_cu;
+    if (tuned) *tuned = t.tuned;
+    if (tune_ms) *tune_ms = t.tune_ms;
     return BST_OK;
   });
 }

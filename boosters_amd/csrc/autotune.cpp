@@ -1,0 +1,268 @@
+// autotune.cpp — measured choice of the histogram launch shape.
+//
+// No counterpart in the reference (its histograms are built on the CPU).
+// Three knobs decide how a step's histogram work is cut into blocks:
+//   fg            features per block: 32 (128 KB LDS sub-histogram, one block
+//                 per CU, 4 waves per SIMD) or 16 (64 KB, two blocks per CU,
+//                 8 waves per SIMD, each row read by twice as many blocks);
+//   tile_min      fewest rows per block; a step's tile is the power of two in
+//                 [tile_min, 8192] that spreads its rows over about
+//   blocks_per_cu x n_cu blocks (more blocks balance better, fewer make fewer
+//                 partial histograms for the finish pass to add).
+// Histogram sums are exact integers, so every shape gives the same
+// histograms: the choice changes time only, never the trees.
+//
+// The tuner times the histogram + finish launches of the steps a tree of this
+// configuration will run, on this dataset's own bins: for depth-wise growth
+// the root and every level's smaller children (balanced sizes), for
+// leaf-wise the root and single nodes of n/2 .. n/256 rows.  Rows are
+// gathered through a scattered permutation, as after partitioning.  Each
+// candidate runs once to warm up and is then timed twice (the faster run
+// counts); the cheapest weighted total wins.  Results are cached in the
+// context per data shape.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "host_common.h"
+
+namespace bst {
+
+namespace {
+
+constexpr uint32_t kTileMax = 8192;          // the device planner's largest tile
+constexpr uint64_t kMaxStepRows = 4u << 20;  // rows of the largest measured step
+constexpr uint32_t kSlots = 64;              // histogram slots the measurement cycles over
+
+struct Shape {
+  uint32_t nodes;      // built nodes in the step
+  uint32_t rows;       // rows per node
+  double weight;       // how many such steps a tree runs
+};
+
+uint32_t step_tile(uint32_t tile_min, uint32_t target, uint32_t NG, uint64_t rows) {
+  const uint64_t want = (rows * NG + target - 1) / target;
+  uint32_t tile = tile_min;
+  while (tile < want && tile < kTileMax) tile <<= 1;
+  return tile;
+}
+
+std::vector<Shape> tree_shapes(int64_t n, const bst_gbdt_config& cfg, int T) {
+  std::vector<Shape> sh;
+  const uint32_t N = (uint32_t)std::min<int64_t>(n, 0xFFFFFFFFll);
+  sh.push_back({(uint32_t)T, N, 1.0});
+  if (cfg.growth == BST_GROW_DEPTHWISE) {
+    for (int L = 1; L < cfg.max_depth; L++) {
+      const uint32_t nodes = 1u << std::min(L - 1, 10);
+      const uint32_t rows = (uint32_t)std::max<int64_t>(1, n >> L);
+      if (rows < 64) break;
+      sh.push_back({nodes * (uint32_t)T, rows, 1.0});
+    }
+  } else {
+    const double steps = std::max(1, (int)cfg.max_leaves - 1);
+    const int ks[4] = {1, 3, 5, 8};  // n/2, n/8, n/32, n/256
+    for (int k : ks) {
+      const uint32_t rows = (uint32_t)std::max<int64_t>(1, n >> k);
+      sh.push_back({(uint32_t)T, rows, steps / 4.0});
+    }
+  }
+  // Keep each measured step within kMaxStepRows rows (wide class-batched
+  // steps): fewer rows per node, the weight scaled to the rows it stands for.
+  for (Shape& x : sh) {
+    const uint64_t total = (uint64_t)x.nodes * x.rows;
+    if (total > kMaxStepRows) {
+      const uint32_t r = (uint32_t)std::max<uint64_t>(64, kMaxStepRows / x.nodes);
+      x.weight *= (double)x.rows / (double)r;
+      x.rows = r;
+    }
+  }
+  return sh;
+}
+
+// One step's work list: (tile, group) items and finish entries.
+struct Plan {
+  std::vector<HistItem> items;
+  std::vector<uint32_t> fin;
+  uint32_t nodes = 0;
+};
+
+Plan plan_step(const Shape& sh, uint32_t NG, uint32_t slots, int64_t n, uint32_t tile_min,
+               uint32_t target) {
+  Plan pl;
+  pl.nodes = sh.nodes;
+  const uint64_t total = (uint64_t)sh.nodes * sh.rows;
+  const uint32_t tile = step_tile(tile_min, target, NG, total);
+  const uint32_t nt = (sh.rows + tile - 1) / tile;
+  const uint64_t span = (uint64_t)std::max<int64_t>(1, n - (int64_t)sh.rows + 1);
+  uint64_t pos = 0;
+  for (uint32_t v = 0; v < sh.nodes; v++) {
+    const uint32_t slot = v % slots;
+    const uint32_t begin = (uint32_t)(pos % span);  // consecutive ranges, wrapped inside n
+    pos += sh.rows;
+    pl.fin.insert(pl.fin.end(), {slot, 0xFFFFFFFFu, (uint32_t)pl.items.size(), nt});
+    for (uint32_t k = 0; k < nt; k++) {
+      const uint32_t b0 = k * tile, cnt = std::min(tile, sh.rows - b0);
+      for (uint32_t g = 0; g < NG; g++)
+        pl.items.push_back({v, slot, begin + b0, cnt, g, nt == 1 ? 0x80000000u : 0u});
+    }
+  }
+  return pl;
+}
+
+struct Bench {
+  bst_ctx* ctx = nullptr;
+  DatasetDev d{};
+  DevBuf<uint32_t> idx;
+  DevBuf<float2> ogh;
+  DevBuf<int32_t> qexp;
+  DevBuf<long long> hist, partial;
+  DevBuf<HistItem> items;
+  DevBuf<uint32_t> fin, count;
+  hipEvent_t a = nullptr, b = nullptr;
+
+  ~Bench() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+
+  // Time (ms) of one step's histogram + finish launches: the faster of
+  // `reps` runs after one warm-up run.
+  float run(const Plan& pl, int fg, int reps) {
+    DatasetDev dd = d;
+    dd.FG = fg;
+    dd.NG = (d.E + fg - 1) / fg;
+    const int64_t slot_elems = (int64_t)dd.NG * 256 * fg * 2;
+    h2d(ctx->stream, items.p, pl.items.data(), pl.items.size());
+    h2d(ctx->stream, fin.p, pl.fin.data(), pl.fin.size());
+    const uint32_t n_items = (uint32_t)pl.items.size();
+    h2d(ctx->stream, count.p, &n_items, 1);
+    // The training launch: a resident grid walking a device-side item count.
+    const int grid = (hist_blocks_resident(fg) * ctx->n_cu + 7) / 8 * 8;
+    float best = 1e30f;
+    for (int r = 0; r < reps + 1; r++) {
+      BST_HIP(hipEventRecord(a, ctx->stream));
+      launch_hist(ctx->stream, dd, items.p, grid, idx.p, ogh.p, 0, qexp.p, hist.p, slot_elems,
+                  nullptr, true, 0, count.p, partial.p, nullptr, nullptr);
+      launch_hist_finish(ctx->stream, fin.p, (int)pl.nodes, dd.NG, fg, partial.p, hist.p,
+                         slot_elems, nullptr);
+      BST_HIP(hipEventRecord(b, ctx->stream));
+      BST_HIP(hipEventSynchronize(b));
+      float ms = 0.0f;
+      BST_HIP(hipEventElapsedTime(&ms, a, b));
+      if (r > 0) best = std::min(best, ms);
+    }
+    return best;
+  }
+};
+
+}  // namespace
+
+HistTuning choose_hist_tuning(bst_ctx* ctx, const DatasetDev& d, const bst_gbdt_config& cfg,
+                              const bst_ext_config& ext, int T) {
+  HistTuning t;
+  const bool fix_fg = ext.hist_fg != 0, fix_tile = ext.hist_rows_min != 0,
+             fix_bpc = ext.hist_blocks_per_cu != 0;
+  if (fix_fg) t.fg = ext.hist_fg;
+  if (fix_tile) t.tile_min = (uint32_t)ext.hist_rows_min;
+  if (fix_bpc) t.blocks_per_cu = (uint32_t)ext.hist_blocks_per_cu;
+  if (!ext.autotune || (fix_fg && fix_tile && fix_bpc) || d.n < 4096 || d.E <= 0) return t;
+
+  const std::vector<int64_t> key = {d.n, d.E, d.S, d.hi_off >= 0, T, cfg.growth,
+                                    cfg.growth == BST_GROW_DEPTHWISE ? cfg.max_depth : cfg.max_leaves,
+                                    ext.hist_fg, ext.hist_rows_min, ext.hist_blocks_per_cu,
+                                    ctx->n_cu};
+  auto hit = ctx->tune_cache.find(key);
+  if (hit != ctx->tune_cache.end()) {
+    HistTuning c = hit->second;
+    c.tune_ms = 0;
+    return c;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> fgs = fix_fg ? std::vector<int>{t.fg} : std::vector<int>{32, 16};
+  std::vector<uint32_t> tiles =
+      fix_tile ? std::vector<uint32_t>{t.tile_min} : std::vector<uint32_t>{512, 1024, 2048};
+  std::vector<uint32_t> bpcs =
+      fix_bpc ? std::vector<uint32_t>{t.blocks_per_cu} : std::vector<uint32_t>{1, 2, 4};
+  const std::vector<Shape> shapes = tree_shapes(d.n, cfg, T);
+
+  // Work lists of every candidate first, to size the buffers once.
+  struct Cand {
+    int fg;
+    uint32_t tile_min, bpc;
+    std::vector<Plan> plans;
+  };
+  std::vector<Cand> cands;
+  size_t max_items = 0, max_fin = 0, max_partial = 0;
+  for (int fg : fgs)
+    for (uint32_t tm : tiles)
+      for (uint32_t bpc : bpcs) {
+        Cand c{fg, tm, bpc, {}};
+        const uint32_t NG = (uint32_t)((d.E + fg - 1) / fg);
+        const uint32_t target = (uint32_t)std::max(64, (int)bpc * ctx->n_cu);
+        for (const Shape& sh : shapes) {
+          c.plans.push_back(plan_step(sh, NG, kSlots, d.n, tm, target));
+          const Plan& pl = c.plans.back();
+          max_items = std::max(max_items, pl.items.size());
+          max_fin = std::max(max_fin, pl.fin.size());
+          max_partial = std::max(max_partial, pl.items.size() * (size_t)256 * fg * 2);
+        }
+        cands.push_back(std::move(c));
+      }
+
+  Bench B;
+  B.ctx = ctx;
+  B.d = d;
+  const size_t n = (size_t)d.n;
+  // A scattered permutation of the rows (multiplier coprime to n), like the
+  // row order of a node after partitioning.
+  auto gcd = [](uint64_t x, uint64_t y) {
+    while (y) {
+      const uint64_t r = x % y;
+      x = y;
+      y = r;
+    }
+    return x;
+  };
+  uint64_t mul = 2654435761ull % n;
+  while (mul < 2 || gcd(mul, n) != 1) mul++;
+  std::vector<uint32_t> perm(n);
+  for (size_t i = 0; i < n; i++) perm[i] = (uint32_t)((i * mul) % n);
+  B.idx.alloc(n);
+  h2d(ctx->stream, B.idx.p, perm.data(), n);
+  std::vector<float2> gh(n, make_float2(0.25f, 0.125f));
+  B.ogh.alloc(n);
+  h2d(ctx->stream, B.ogh.p, gh.data(), n);
+  const int32_t qe[2] = {20, 20};
+  B.qexp.alloc(2);
+  h2d(ctx->stream, B.qexp.p, qe, 2);
+  const int64_t slot_elems = std::max<int64_t>((int64_t)((d.E + 15) / 16) * 256 * 16 * 2,
+                                               (int64_t)((d.E + 31) / 32) * 256 * 32 * 2);
+  B.hist.alloc((size_t)kSlots * slot_elems);
+  B.partial.alloc(std::max<size_t>(max_partial, 1));
+  B.items.alloc(std::max<size_t>(max_items, 1));
+  B.fin.alloc(std::max<size_t>(max_fin, 1));
+  B.count.alloc(1);
+  BST_HIP(hipEventCreate(&B.a));
+  BST_HIP(hipEventCreate(&B.b));
+
+  double best = 1e300;
+  HistTuning pick = t;
+  for (const Cand& c : cands) {
+    double total = 0;
+    for (size_t i = 0; i < shapes.size(); i++) total += shapes[i].weight * B.run(c.plans[i], c.fg, 2);
+    if (total < best) {
+      best = total;
+      pick.fg = c.fg;
+      pick.tile_min = c.tile_min;
+      pick.blocks_per_cu = c.bpc;
+    }
+  }
+  BST_HIP(hipStreamSynchronize(ctx->stream));
+  pick.tuned = 1;
+  pick.tune_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ctx->tune_cache[key] = pick;
+  return pick;
+}
+
+}  // namespace bst

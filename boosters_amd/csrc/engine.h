@@ -132,6 +132,9 @@ void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int 
 // node_sum may be null (no totals); partial non-null: multi-tile nodes store
 // one partial histogram per item there instead of global atomics, and
 // launch_hist_finish sums them (and applies the subtraction trick).
+// Histogram blocks one CU holds at once for a feature-group width (LDS).
+inline int hist_blocks_resident(int FG) { return FG == 16 ? 2 : 1; }
+
 void launch_hist_finish(hipStream_t s, const uint32_t* fin, int n, int NG, int FG,
                         const long long* partial, long long* hist, int64_t slot_elems,
                         const uint32_t* dyn);

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -133,6 +134,17 @@ float round_down_f32(double b);
 
 }  // namespace bst
 
+namespace bst {
+// Histogram launch shape of one training run.
+struct HistTuning {
+  int fg = 32;                 // features per histogram block: 32 (one block per CU) or 16 (two)
+  uint32_t tile_min = 1024;    // fewest rows per block in a step (power of two, 512..8192)
+  uint32_t blocks_per_cu = 2;  // blocks a step's rows are spread over, per CU
+  int tuned = 0;               // 1 = measured on this data shape
+  double tune_ms = 0;          // time spent measuring (0 when fixed or cached)
+};
+}  // namespace bst
+
 struct bst_ctx {
   int device = 0;
   int n_cu = 256;  // compute units (sizes grids of resident, grid-stride kernels)
@@ -164,6 +176,9 @@ struct bst_ctx {
     }
     return staging;
   }
+  // Autotuner results per data shape, and the shape the last train used.
+  std::map<std::vector<int64_t>, bst::HistTuning> tune_cache;
+  bst::HistTuning last_tuning;
   int64_t k_launches[3] = {0, 0, 0};
   double k_ms[3] = {0, 0, 0}, k_bytes[3] = {0, 0, 0};
   // Record the start of a launch of `kind`; returns an index for prof_end.
@@ -233,6 +248,12 @@ struct bst_model {
 };
 
 namespace bst {
+// The histogram launch shape for training `ds` with `cfg`/`ext` and T trees
+// grown together: fixed fields of `ext` are kept; with ext.autotune the free
+// ones are measured on this data (cached in ctx per shape).
+HistTuning choose_hist_tuning(bst_ctx* ctx, const DatasetDev& d, const bst_gbdt_config& cfg,
+                              const bst_ext_config& ext, int T);
+
 // Structural check of a forest before it is traversed on the device: array
 // lengths, groups, and per tree an acyclic walk from the root in which every
 // split node has in-range, distinct, unvisited children (the rules of

@@ -540,7 +540,7 @@ void launch_init_rows_sums(hipStream_t s, uint32_t* idx, float2* ogh, const floa
 // Histograms
 // ===========================================================================
 // One block-iteration = one HistItem: a tile of a node's rows x one group of
-// FG features.  The tile is walked in chunks of kHistChunk rows.  Staging:
+// FG features.  The tile is walked in chunks of hist_chunk<FG>() rows.  Staging:
 // one row per thread loads the row id and the row's (g, h), quantises the
 // pair to fixed point once (not once per feature lane) and stores both in
 // LDS; the next chunk's loads are issued before the current chunk is
@@ -564,10 +564,18 @@ void launch_init_rows_sums(hipStream_t s, uint32_t* idx, float2* ogh, const floa
 // OFF64: row byte offsets need 64 bits (n * S >= 2^32); otherwise they are
 // 32-bit offsets from the uniform base, which keeps address registers small.
 constexpr int kHistThreads = 1024;
-constexpr int kHistChunk = 512;  // rows per staged chunk
+// Rows per staged chunk.  FG = 32 keeps a 128 KB sub-histogram: one block per
+// CU, 16 rows per thread per chunk.  FG = 16 halves it and the staging, so
+// that two blocks fit a CU's 160 KB of LDS (74 KB each) at 8 waves per SIMD,
+// with 4 rows per thread per chunk.
+template <int FG>
+constexpr int hist_chunk() { return FG == 16 ? 256 : 512; }
+template <int FG>
+constexpr int hist_waves_per_eu() { return FG == 16 ? 8 : 4; }
 
 template <int FG, bool GATHER, bool HAS_HI, bool OFF64>
-__global__ __launch_bounds__(kHistThreads) void k_hist(
+__global__ __launch_bounds__(kHistThreads)
+__attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
     const HistItem* __restrict__ items, uint32_t n_items_static, int NG,
     const uint32_t* __restrict__ idx, const float2* __restrict__ ogh, int64_t tree_stride,
     const uint8_t* __restrict__ bins_rm, int S, int hi_off, int E,
@@ -577,6 +585,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(
     const float2* __restrict__ ogh_alt) {
   __shared__ long long s_g[256 * FG];
   __shared__ long long s_h[256 * FG];
+  constexpr int kHistChunk = hist_chunk<FG>();
   __shared__ uint32_t s_row[2][kHistChunk];
   __shared__ longlong2 s_q[2][kHistChunk];
   __shared__ long long s_red[2][kHistThreads / WAVE];

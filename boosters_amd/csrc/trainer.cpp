@@ -109,7 +109,12 @@ Trainer::Trainer(bst_ctx* ctx, bst_dataset* ds, const TrainOptions& opt)
   // Class trees of a round grow together, except on the host-driven path with
   // column sampling (its split launch takes one feature list).
   T_ = (sched_ != BST_SCHED_PLAIN && opt_.K > 1 && (graph_capable() || !col_sampling)) ? opt_.K : 1;
-  slot_elems_ = (int64_t)ds->d.NG * 256 * ds->d.FG * 2;
+  tune_ = choose_hist_tuning(ctx, ds->d, c, opt_.ext, T_);
+  ctx->last_tuning = tune_;
+  dd_ = ds->d;
+  dd_.FG = tune_.fg;
+  dd_.NG = (dd_.E + dd_.FG - 1) / dd_.FG;
+  slot_elems_ = (int64_t)dd_.NG * 256 * dd_.FG * 2;
   rightmost_ = opt_.ext.n_threads != 1 && ds->d.E >= 16;
   sp_.lambda = (double)c.lambda;
   sp_.min_gain = (double)c.min_gain;
@@ -200,7 +205,7 @@ void Trainer::root(std::vector<TreeState>& trees) {
     if (nt > 1 || m == 0) zero.push_back(gslot);
     for (uint32_t k = 0; k < nt && m > 0; k++) {
       const uint32_t b = k * kHistTile, cnt = std::min(kHistTile, m - b);
-      for (int gr = 0; gr < ds_->d.NG; gr++)
+      for (int gr = 0; gr < dd_.NG; gr++)
         items.push_back({gnode, gslot, (uint32_t)(ts.t * n_) + b, cnt, (uint32_t)gr,
                          (uint32_t)ts.t | (nt == 1 ? kDirect : 0u)});
     }
@@ -221,11 +226,11 @@ void Trainer::root(std::vector<TreeState>& trees) {
   double rows = 0;
   for (auto& ts : trees) rows += (double)ts.rows;
   int pi = ctx_->prof_begin(0, hist_bytes(rows, (int)trees.size(), gather));
-  launch_hist(s_, ds_->d, d_hist_items_.p, (int)items.size(), idx_.p, ogh_.p, 0, qexp, hist_.p,
+  launch_hist(s_, dd_, d_hist_items_.p, (int)items.size(), idx_.p, ogh_.p, 0, qexp, hist_.p,
               slot_elems_, node_sum_.p, gather, 0);
   ctx_->prof_end(pi);
   pi = ctx_->prof_begin(1, split_bytes((int)nodes.size()));
-  launch_split(s_, ds_->d, d_u32b_.p, d_u32c_.p, (int)nodes.size(), feats_.p, (int)feats_h_.size(),
+  launch_split(s_, dd_, d_u32b_.p, d_u32c_.p, (int)nodes.size(), feats_.p, (int)feats_h_.size(),
                hist_.p, slot_elems_, node_sum_.p, node_count_.p, qexp, MN_, sp_, node_key_.p,
                cand_.p, node_split_.p);
   ctx_->prof_end(pi);
@@ -329,7 +334,7 @@ void Trainer::expand_batch(std::vector<TreeState>& trees,
   double prow = 0;
   for (auto& it : pitems) prow += it.count;
   int pi = ctx_->prof_begin(2, prow * (4 + 8 + 12 + 1));
-  launch_partition(s_, ds_->d, d_part_items_.p, (int)pitems.size(), d_part_tiles_.p,
+  launch_partition(s_, dd_, d_part_items_.p, (int)pitems.size(), d_part_tiles_.p,
                    (int)ptiles.size(), node_split_.p, idx_.p, ogh_.p, idx_tmp_.p, ogh_tmp_.p,
                    flag_.p, tile_left_.p, tile_off_.p, item_left_.p, node_count_.p, node_sum_.p);
   ctx_->prof_end(pi);
@@ -368,7 +373,7 @@ void Trainer::expand_batch(std::vector<TreeState>& trees,
       if (nt > 1 || scount == 0) zero.push_back(gss);
       for (uint32_t k = 0; k < nt && scount > 0; k++) {
         const uint32_t b0 = k * kHistTile, cnt = std::min(kHistTile, scount - b0);
-        for (int gr = 0; gr < ds_->d.NG; gr++)
+        for (int gr = 0; gr < dd_.NG; gr++)
           hitems.push_back({gsmall, gss, sbegin + b0, cnt, (uint32_t)gr,
                             (uint32_t)ts.t | (nt == 1 ? kDirect : 0u)});
       }
@@ -399,7 +404,7 @@ void Trainer::expand_batch(std::vector<TreeState>& trees,
       if (it.group == 0) hrows += it.count;
     for (size_t k = 0; k < quads_full.size(); k += 4) hnodes++;
     pi = ctx_->prof_begin(0, hist_bytes(hrows, hnodes, true));
-    launch_hist(s_, ds_->d, d_hist_items_.p, (int)hitems.size(), idx_.p, ogh_.p, 0, qexp, hist_.p,
+    launch_hist(s_, dd_, d_hist_items_.p, (int)hitems.size(), idx_.p, ogh_.p, 0, qexp, hist_.p,
                 slot_elems_, node_sum_.p, true, 0);
     ctx_->prof_end(pi);
   }
@@ -411,7 +416,7 @@ void Trainer::expand_batch(std::vector<TreeState>& trees,
     both.insert(both.end(), sslots.begin(), sslots.end());
     upload(s_, d_u32d_, both);
     pi = ctx_->prof_begin(1, split_bytes((int)snodes.size()));
-    launch_split(s_, ds_->d, d_u32d_.p, d_u32d_.p + snodes.size(), (int)snodes.size(), feats_.p,
+    launch_split(s_, dd_, d_u32d_.p, d_u32d_.p + snodes.size(), (int)snodes.size(), feats_.p,
                  (int)feats_h_.size(), hist_.p, slot_elems_, node_sum_.p, node_count_.p, qexp, MN_,
                  sp_, node_key_.p, cand_.p, node_split_.p);
     ctx_->prof_end(pi);

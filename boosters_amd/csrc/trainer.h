@@ -101,6 +101,8 @@ class Trainer {
 
   bst_ctx* ctx_;
   bst_dataset* ds_;
+  HistTuning tune_;  // histogram launch shape (autotuned or fixed)
+  DatasetDev dd_;    // the dataset's device view with this run's feature grouping
   TrainOptions opt_;
   hipStream_t s_;
   int64_t n_;

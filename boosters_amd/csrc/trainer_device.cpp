@@ -100,12 +100,12 @@ void Trainer::device_setup() {
   P.WO = P.growth == 0 ? max_leaves : max_leaves + 1;
   P.max_depth = c.max_depth;
   P.max_leaves = c.max_leaves;
-  P.NG = (uint32_t)ds_->d.NG;
-  P.E = (uint32_t)ds_->d.E;
-  P.FG = (uint32_t)ds_->d.FG;
+  P.NG = (uint32_t)dd_.NG;
+  P.E = (uint32_t)dd_.E;
+  P.FG = (uint32_t)dd_.FG;
   P.hist_tile = kHistTileD;
-  P.hist_tile_min = 1024;
-  P.hist_target = (uint32_t)std::max(64, 2 * ctx_->n_cu);
+  P.hist_tile_min = tune_.tile_min;
+  P.hist_target = (uint32_t)std::max(64, (int)tune_.blocks_per_cu * ctx_->n_cu);
   if (const char* e = std::getenv("BST_HIST_TILE")) {
     P.hist_tile = (uint32_t)std::max(512, std::atoi(e));
     P.hist_tile_min = P.hist_tile;  // fixed tile
@@ -266,11 +266,12 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
   const DeviceBufs& B = S.B;
   const auto& cfg = opt_.cfg;
   const int K = opt_.K;
-  const auto& d = ds_->d;
-  // The histogram kernel keeps one block per CU resident (LDS), so a grid of
-  // one block per CU walking the items is as fast as a larger one and costs
-  // nothing on steps with few items.  A multiple of 8 keeps the XCD mapping.
-  const int g_hist = grid_for(S.cap_hist, (ctx_->n_cu + 7) / 8 * 8);
+  const auto& d = dd_;
+  // The histogram kernel keeps one block per CU resident (two with 16-feature
+  // groups, LDS-limited), so a grid of that many blocks walking the items is
+  // as fast as a larger one and costs nothing on steps with few items.  A
+  // multiple of 8 keeps the XCD mapping.
+  const int g_hist = grid_for(S.cap_hist, (hist_blocks_resident(d.FG) * ctx_->n_cu + 7) / 8 * 8);
   const int g_tiles = grid_for(S.cap_tiles, kMaxGrid);
   const int g_split = grid_for(S.cap_split, 512);
   const int g_lsegs = grid_for(S.cap_lsegs, kMaxGrid);

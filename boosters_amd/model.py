@@ -29,7 +29,8 @@ class GrowthStrategy:
 
 
 class Schedule:
-    """Engine schedule (extension): plain per-node launches, or batched launches."""
+    """Engine schedule (extension): plain per-node launches, batched launches, or
+    batched launches with device-side growth replayed as a hipGraph (default)."""
 
     Plain = A.SCHED_PLAIN
     Batched = A.SCHED_BATCHED
@@ -114,7 +115,7 @@ class GBDTConfig:
     seed: int = 42
     verbosity: int = 0
     # Extension knobs (no counterpart in the reference).
-    schedule: int = Schedule.Batched
+    schedule: int = Schedule.Graph
     goss_top_rate: float = 0.0
     goss_other_rate: float = 0.0
 

@@ -114,7 +114,15 @@ typedef struct bst_ext_config {
   float goss_other_rate;
   const int32_t* query_group_sizes; /* LambdaRank: ragged groups, sum == n_rows */
   int64_t n_query_groups;
-  int32_t autotune;        /* 1 = pick histogram launch shape per data shape  */
+  int32_t autotune;        /* 1 = measure histogram launch shapes on this data
+                              and keep the fastest (cached per ctx and data
+                              shape); 0 = fixed shape                         */
+  int32_t hist_fg;            /* 0 = tuned / default 32; else 16 or 32
+                                 features per histogram block                 */
+  int32_t hist_rows_min;      /* 0 = tuned / default 1024; else fewest rows per
+                                 histogram block, a power of two 512..8192    */
+  int32_t hist_blocks_per_cu; /* 0 = tuned / default 2; else 1..8 histogram
+                                 blocks per CU that a step's rows spread over */
 } bst_ext_config;
 
 /* ---- forest exchange format -------------------------------------------- */
@@ -228,6 +236,12 @@ int bst_train_device(bst_ctx* ctx, bst_dataset* ds, const float* targets_dev, in
 /* Timing of the last bst_train on this ctx: binning excluded. */
 int bst_last_train_stats(bst_ctx* ctx, double* ms_total, double* ms_per_tree,
                          int64_t* hist_rows_built);
+/* Histogram launch shape of the last train on this ctx (fixed or chosen by
+ * the autotuner); tuned = 1 if it was measured on this data shape, tune_ms =
+ * time the measurement took in that call (0 when cached).  No counterpart in
+ * the reference. */
+int bst_last_train_tuning(bst_ctx* ctx, int32_t* hist_fg, int32_t* hist_rows_min,
+                          int32_t* hist_blocks_per_cu, int32_t* tuned, double* tune_ms);
 /* Per-round metric values recorded by the last train ([n_rounds]). */
 int bst_last_train_metric(bst_ctx* ctx, double* out, int64_t cap, int64_t* n_rounds);
 

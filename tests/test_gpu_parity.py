@@ -313,6 +313,58 @@ def test_graph_schedule_matches_oracle(ctx, oracle, case):
         np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
 
 
+HIST_SHAPES = [(16, 512, 4), (32, 2048, 1), (16, 8192, 1), (32, 512, 8)]
+
+
+@pytest.mark.parametrize("case", ["logistic_nan_depth8", "softmax_leafwise_cat", "squared_leafwise"])
+def test_hist_launch_shapes_identical(ctx, oracle, case):
+    """Every histogram launch shape the autotuner can pick (16- or 32-feature
+    groups, tile sizes, blocks per CU) grows the oracle's forest: the sums
+    are exact, so the shape changes time only."""
+    build, kw = GRAPH_CASES[case]
+    X, y, is_cat = build()
+    cfg = make_cfg(**kw)
+    gm, om, b = _train_both(ctx, oracle, X, y, cfg, is_cat, schedule=2)
+    ga = _forests_equal(gm, om.info, om.arrays)
+    for sched in (2, 1):
+        for fg, rows_min, bpc in HIST_SHAPES:
+            ext = make_ext(sched)
+            ext.autotune = 0
+            ext.hist_fg, ext.hist_rows_min, ext.hist_blocks_per_cu = fg, rows_min, bpc
+            h = C.c_void_p()
+            A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0,
+                                      C.byref(cfg), C.byref(ext), C.byref(h)))
+            gb = GBDTModel(h, ctx).export()
+            for k in ga:
+                np.testing.assert_array_equal(ga[k], gb[k], err_msg=f"{k} {sched} {fg} {rows_min} {bpc}")
+            got = [C.c_int32() for _ in range(4)]
+            A.check(A.lib().bst_last_train_tuning(ctx.h, *[C.byref(v) for v in got], None))
+            assert [v.value for v in got] == [fg, rows_min, bpc, 0]
+
+
+def test_autotuner_choice_and_cache(ctx, oracle):
+    """With autotune on, the shape is measured once per data shape and cached
+    in the context; the forest is the oracle's whatever it picks."""
+    X, y = binary_nan_data(40_000, 24, seed=31)
+    cfg = make_cfg(objective=1, n_trees=3, max_depth=6)
+    gm, om, b = _train_both(ctx, oracle, X, y, cfg, None, schedule=2)
+    _forests_equal(gm, om.info, om.arrays)
+    fg, rows_min, bpc, tuned = (C.c_int32() for _ in range(4))
+    ms = C.c_double()
+    A.check(A.lib().bst_last_train_tuning(ctx.h, C.byref(fg), C.byref(rows_min), C.byref(bpc),
+                                          C.byref(tuned), C.byref(ms)))
+    assert tuned.value == 1 and fg.value in (16, 32) and rows_min.value in (512, 1024, 2048)
+    assert bpc.value in (1, 2, 4) and ms.value > 0
+    first = (fg.value, rows_min.value, bpc.value)
+    h = C.c_void_p()
+    A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0, C.byref(cfg),
+                              C.byref(make_ext(2)), C.byref(h)))
+    GBDTModel(h, ctx)
+    A.check(A.lib().bst_last_train_tuning(ctx.h, C.byref(fg), C.byref(rows_min), C.byref(bpc),
+                                          C.byref(tuned), C.byref(ms)))
+    assert (fg.value, rows_min.value, bpc.value) == first and ms.value == 0.0
+
+
 def test_graph_schedule_training_predictions(ctx, oracle):
     """The in-graph prediction update leaves the same per-round training loss."""
     X, y = binary_nan_data(20_000, 16, seed=23)
