@@ -54,10 +54,14 @@ def cfg_c2(n_trees, depth):
     return c
 
 
+HIST_SHAPE = (0, 0, 0)  # (features per block, rows per block, blocks per CU); 0 = autotuned
+
+
 def ext(schedule):
     e = A.ExtConfig()
     A.lib().bst_ext_config_default(C.byref(e))
     e.schedule = schedule
+    e.hist_fg, e.hist_rows_min, e.hist_blocks_per_cu = HIST_SHAPE
     return e
 
 
@@ -428,7 +432,13 @@ def main():
                     help="rows of the CPU baseline sample for c1/c3/c4")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the plain copy kernel 5x on this many bytes (HBM counter calibration)")
+    ap.add_argument("--hist-shape", default="",
+                    help="fixed histogram launch shape FG,ROWS_MIN,BLOCKS_PER_CU instead of the "
+                         "autotuner's choice (e.g. 32,2048,1)")
     args = ap.parse_args()
+    if args.hist_shape:
+        global HIST_SHAPE
+        HIST_SHAPE = tuple(int(v) for v in args.hist_shape.split(","))
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (row sharding "

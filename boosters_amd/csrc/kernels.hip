@@ -564,6 +564,8 @@ void launch_init_rows_sums(hipStream_t s, uint32_t* idx, float2* ogh, const floa
 // OFF64: row byte offsets need 64 bits (n * S >= 2^32); otherwise they are
 // 32-bit offsets from the uniform base, which keeps address registers small.
 constexpr int kHistThreads = 1024;
+constexpr int kHistRowBatch = 4;  // rows whose (g, h) are read from LDS together
+constexpr int kHistBufs = 3;      // staging buffers: chunks c (adds), c+1 (bin loads), c+2
 // Rows per staged chunk.  FG = 32 keeps a 128 KB sub-histogram: one block per
 // CU, 16 rows per thread per chunk.  FG = 16 halves it and the staging, so
 // that two blocks fit a CU's 160 KB of LDS (74 KB each) at 8 waves per SIMD,
@@ -583,15 +585,19 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
     long long* __restrict__ node_sum, long long* __restrict__ partial,
     const uint32_t* __restrict__ dyn, const uint32_t* __restrict__ idx_alt,
     const float2* __restrict__ ogh_alt) {
-  __shared__ long long s_g[256 * FG];
-  __shared__ long long s_h[256 * FG];
+  // Bins 0..255, plus row 256 that takes every add with nothing to count
+  // (the NaN bin, rows past the tile end, padding features) so that the
+  // adds need no branch; row 256 is never read.
+  __shared__ long long s_g[257 * FG];
+  __shared__ long long s_h[257 * FG];
   constexpr int kHistChunk = hist_chunk<FG>();
-  __shared__ uint32_t s_row[2][kHistChunk];
-  __shared__ longlong2 s_q[2][kHistChunk];
+  __shared__ uint32_t s_row[kHistBufs][kHistChunk];
+  __shared__ longlong2 s_q[kHistBufs][kHistChunk];
   __shared__ long long s_red[2][kHistThreads / WAVE];
   constexpr int RPP = kHistThreads / FG;  // row runs per chunk
   constexpr int RPT = kHistChunk / RPP;   // rows per thread per chunk
-  static_assert(RPT >= 1 && kHistChunk <= kHistThreads, "chunk shape");
+  static_assert(RPT >= 1 && kHistChunk <= kHistThreads && RPT % kHistRowBatch == 0,
+                "chunk shape");
   const uint32_t n_items = dyn ? *dyn : n_items_static;
   const uint32_t n_tiles = n_items / (uint32_t)NG;
   const uint32_t n_pos = ((n_tiles + 7u) / 8u) * 8u * (uint32_t)NG;
@@ -643,20 +649,15 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
         acch += hq;
       }
     };
-    if (n_chunks > 0) {
-      stage_load(0);
-      stage_store(0);
-    }
-    __syncthreads();  // staging visible; also orders the LDS zeroing
-    for (uint32_t c = 0; c < n_chunks; c++) {
-      const int cur = (int)(c & 1u);
-      if (c + 1 < n_chunks) stage_load(c + 1);
+    // Bin loads of chunk c (row ids from the staging buffer).  Only called for
+    // c < n_chunks, whose row 0 is staged, so clamped lanes read a valid id.
+    auto load_bins = [&](uint32_t c, uint32_t* bv, uint32_t* hv) {
+      const int buf = (int)(c % kHistBufs);
       const uint32_t nrow = min((uint32_t)kHistChunk, it.count - c * kHistChunk);
-      uint32_t bv[RPT], hv[RPT];
 #pragma unroll
       for (int u = 0; u < RPT; u++) {
         const uint32_t j = (uint32_t)(rs * RPT + u);
-        const uint32_t r = s_row[cur][j < nrow ? j : 0u];
+        const uint32_t r = s_row[buf][j < nrow ? j : 0u];
         if (OFF64) {
           const uint8_t* rp = bins_rm + (uint64_t)r * (uint64_t)S;
           bv[u] = rp[fs];
@@ -667,20 +668,60 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
           hv[u] = HAS_HI ? bins_rm[off + (uint32_t)hb] : 0u;
         }
       }
+    };
+    // Pipeline: chunk c+2 is staged (row ids, quantised (g, h)) and chunk
+    // c+1's bins are in flight while chunk c is added, so the latency of the
+    // gathered bin loads overlaps the LDS adds.  Three staging buffers.
+    if (n_chunks > 0) {
+      stage_load(0);
+      stage_store(0);
+    }
+    if (n_chunks > 1) {
+      stage_load(1);
+      stage_store(1);
+    }
+    __syncthreads();  // staging visible; also orders the LDS zeroing
+    uint32_t bv[RPT], hv[RPT];
 #pragma unroll
-      for (int u = 0; u < RPT; u++) {
-        const uint32_t j = (uint32_t)(rs * RPT + u);
-        // bin 256 (NaN bin of a 257-bin feature) is not accumulated: it
-        // equals the node total minus the 256 real bins and is never read.
-        const bool nan_bin = HAS_HI && ((hv[u] >> hs) & 1u);
-        if (j < nrow && fvalid && !nan_bin) {
-          const longlong2 q = s_q[cur][j];
-          atomicAdd((unsigned long long*)&s_g[bv[u] * FG + lf], (unsigned long long)q.x);
-          atomicAdd((unsigned long long*)&s_h[bv[u] * FG + lf], (unsigned long long)q.y);
+    for (int u = 0; u < RPT; u++) bv[u] = hv[u] = 0u;
+    if (n_chunks > 0) load_bins(0, bv, hv);
+    for (uint32_t c = 0; c < n_chunks; c++) {
+      const int cur = (int)(c % kHistBufs);
+      const uint32_t nrow = min((uint32_t)kHistChunk, it.count - c * kHistChunk);
+      if (c + 2 < n_chunks) stage_load(c + 2);
+      uint32_t bn[RPT], hn[RPT];
+#pragma unroll
+      for (int u = 0; u < RPT; u++) bn[u] = hn[u] = 0u;
+      if (c + 1 < n_chunks) load_bins(c + 1, bn, hn);
+      // Four rows at a time: their (g, h) pairs are read from LDS together,
+      // then added without branches.  The NaN bin of a 257-bin feature (hi
+      // bit set) is not accumulated: it is the node total minus the 256 real
+      // bins and goes to row 256 with the other adds that count nothing.
+#pragma unroll
+      for (int u0 = 0; u0 < RPT; u0 += kHistRowBatch) {
+        longlong2 q[kHistRowBatch];
+#pragma unroll
+        for (int k = 0; k < kHistRowBatch; k++) {
+          const uint32_t j = (uint32_t)(rs * RPT + u0 + k);
+          q[k] = s_q[cur][j < nrow ? j : 0u];
+        }
+#pragma unroll
+        for (int k = 0; k < kHistRowBatch; k++) {
+          const int u = u0 + k;
+          const uint32_t j = (uint32_t)(rs * RPT + u);
+          const bool nan_bin = HAS_HI && ((hv[u] >> hs) & 1u);
+          const uint32_t bin = (j < nrow && fvalid && !nan_bin) ? bv[u] : 256u;
+          atomicAdd((unsigned long long*)&s_g[bin * FG + lf], (unsigned long long)q[k].x);
+          atomicAdd((unsigned long long*)&s_h[bin * FG + lf], (unsigned long long)q[k].y);
         }
       }
-      if (c + 1 < n_chunks) stage_store(cur ^ 1);
-      __syncthreads();  // next chunk staged; this chunk's staging may be rewritten
+      if (c + 2 < n_chunks) stage_store((int)((c + 2) % kHistBufs));
+      __syncthreads();  // chunk c+2 staged; chunk c's buffer may be rewritten
+#pragma unroll
+      for (int u = 0; u < RPT; u++) {
+        bv[u] = bn[u];
+        hv[u] = hn[u];
+      }
     }
     const bool do_sum = node_sum != nullptr && it.group == 0;
     if (do_sum) {
