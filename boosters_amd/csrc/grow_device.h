@@ -27,6 +27,9 @@
 
 namespace bst {
 
+// Trees the controllers grow together (class batching) at most.
+constexpr uint32_t kCtlMaxTrees = 16;
+
 struct CtlParams {
   uint32_t T;           // trees grown together
   uint32_t MN;          // partition ids per tree

@@ -26,6 +26,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kDirectFlag = 0x80000000u;  // HistItem.tree: tile = whole node
 constexpr uint32_t kAltFlag = 0x40000000u;     // HistItem.tree: alternate buffer
 constexpr int kCtlThreads = 1024;
+constexpr uint32_t kPlanLdsItems = 1024;  // items whose partition tiles the plan fills block-wide
 
 // Exclusive scan over the block (kCtlThreads threads); *total receives the sum.
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
@@ -252,6 +253,11 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   __shared__ uint32_t s_items, s_tiles, s_nch;
   __shared__ uint32_t s_part, s_tnodes, s_best;
   __shared__ unsigned long long s_max64[kCtlThreads / 64];
+  // Partition tiles of the first kPlanLdsItems items, written by the whole
+  // block at the end: first tile, row range of each item.
+  __shared__ uint32_t s_tfirst[kPlanLdsItems], s_tbegin[kPlanLdsItems], s_tcount[kPlanLdsItems];
+  __shared__ uint32_t s_tlds_end;
+  if (threadIdx.x == 0) s_tlds_end = kNone;
   const uint32_t n_sp = B.counts[C_SPLIT];
   for (uint32_t i = threadIdx.x; i < n_sp; i += blockDim.x) {
     const uint32_t node = B.split_nodes[i];
@@ -400,14 +406,21 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
         B.pend[5 * g + 2] = rt;
         B.pend[5 * g + 3] = e;
         B.pend[5 * g + 4] = dep + 1;
-        for (uint32_t q = 0; q < ntile; q++) {
-          const uint32_t b0 = q * P.part_tile;
-          PartTile pt;
-          pt.item = g;
-          pt.begin = begin + b0;
-          pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
-          pt.pad = 0;
-          B.tiles[s_tiles + toff + q] = pt;
+        if (g < kPlanLdsItems) {
+          s_tfirst[g] = s_tiles + toff;
+          s_tbegin[g] = begin;
+          s_tcount[g] = count;
+        } else {
+          if (g == kPlanLdsItems) s_tlds_end = s_tiles + toff;
+          for (uint32_t q = 0; q < ntile; q++) {
+            const uint32_t b0 = q * P.part_tile;
+            PartTile pt;
+            pt.item = g;
+            pt.begin = begin + b0;
+            pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
+            pt.pad = 0;
+            B.tiles[s_tiles + toff + q] = pt;
+          }
         }
       }
       carry_e += tot_e;
@@ -427,6 +440,24 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
     }
     __syncthreads();
   }
+  // Tiles of the items recorded in LDS: tile k belongs to the last item g
+  // with s_tfirst[g] <= k (every item has at least one tile).
+  const uint32_t n_lds_items = min(s_items, kPlanLdsItems);
+  const uint32_t end_k = min(s_tiles, s_tlds_end);
+  for (uint32_t k = threadIdx.x; k < end_k; k += blockDim.x) {
+    uint32_t lo = 0, hi = n_lds_items - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (s_tfirst[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    const uint32_t b0 = (k - s_tfirst[lo]) * P.part_tile, count = s_tcount[lo];
+    PartTile pt;
+    pt.item = lo;
+    pt.begin = s_tbegin[lo] + b0;
+    pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
+    pt.pad = 0;
+    B.tiles[k] = pt;
+  }
   if (threadIdx.x == 0) {
     B.counts[C_ITEMS] = s_items;
     B.counts[C_TILES] = s_tiles;
@@ -440,24 +471,32 @@ constexpr uint32_t kChildLdsItems = 1024;  // items whose tile totals are reduce
 
 __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, DeviceBufs B) {
   __shared__ long long s_sum[kChildLdsItems][4];
-  if (threadIdx.x == 0) {
-    B.counts[C_HIST] = 0;
-    B.counts[C_FIN] = 0;
-    B.counts[C_SPLIT] = 0;
-  }
   __shared__ unsigned long long s_red64[kCtlThreads / 64];
+  __shared__ uint32_t s_tmp[kCtlThreads / 64 + 1];
+  // Per tree: items this step, free slots and open-set size at entry.
+  __shared__ uint32_t s_cnt[kCtlMaxTrees], s_free[kCtlMaxTrees], s_open[kCtlMaxTrees];
+  // Small child of item i, for the block-wide fill of the histogram items.
+  __shared__ uint32_t s_h0[kChildLdsItems], s_hnode[kChildLdsItems], s_hslot[kChildLdsItems],
+      s_hbegin[kChildLdsItems], s_hcount[kChildLdsItems], s_htree[kChildLdsItems];
   const uint32_t n_items = B.counts[C_ITEMS];
   const bool lds_sums = n_items <= kChildLdsItems;
+  if (threadIdx.x < P.T) {
+    s_cnt[threadIdx.x] = 0;
+    s_free[threadIdx.x] = B.ctl[threadIdx.x].n_free;
+    s_open[threadIdx.x] = B.ctl[threadIdx.x].n_open;
+  }
+  // Every item of a step is at the same depth (depth-wise levels advance
+  // together; leaf-wise always splits), so whether the children are split
+  // next is one decision for the step.
+  const bool need = n_items > 0 && (P.growth == 1 || B.pend[4] < P.max_depth);
   // Rows this step histograms (smaller children that will be split), which
   // sets the step's tile size.
   unsigned long long step_rows = 0ull;
-  for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) {
-    const uint32_t cdepth = B.pend[5 * i + 4];
-    if (P.growth == 1 || cdepth < P.max_depth) {
+  if (need)
+    for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) {
       const uint32_t L = B.cursor[2 * i], R = B.items[i].count - L;
       step_rows += L <= R ? L : R;
     }
-  }
   const uint32_t tile = step_tile(P, block_sum_u64(step_rows, s_red64));
   if (lds_sums) {
     // Children's exact totals: every thread adds whole partition tiles.
@@ -472,102 +511,165 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
         atomicAdd((unsigned long long*)&s_sum[item][j], (unsigned long long)ts[j]);
     }
   }
+  // Items per tree: the plan numbers a tree's items e = 0, 1, ... in order.
+  for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x)
+    atomicMax(&s_cnt[B.pend[5 * i]], B.pend[5 * i + 3] + 1u);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) {
-    const uint32_t t = B.pend[5 * i], lt = B.pend[5 * i + 1], rt = B.pend[5 * i + 2];
-    const uint32_t e = B.pend[5 * i + 3], cdepth = B.pend[5 * i + 4];
-    const PartItem it = B.items[i];
-    const uint32_t L = B.cursor[2 * i], R = it.count - L;
-    const uint32_t parent = it.node, right = it.right_node;
-    B.node_count[parent] = L;
-    B.node_count[right] = R;
-    {  // children's exact totals from the partition tiles
-      long long lg = 0, lh = 0, rg = 0, rh = 0;
-      if (lds_sums) {
-        lg = s_sum[i][0];
-        lh = s_sum[i][1];
-        rg = s_sum[i][2];
-        rh = s_sum[i][3];
-      } else {
-        for (uint32_t k = 0; k < it.n_tiles; k++) {
-          const long long* ts = B.tile_sums + 4 * (uint64_t)(it.tile0 + k);
-          lg += ts[0];
-          lh += ts[1];
-          rg += ts[2];
-          rh += ts[3];
+  // Work lists are laid out by item order: histogram items by an exclusive
+  // scan of each child's tile count, finish entry i and split entries 2i,
+  // 2i + 1 for item i; the e-th item of tree t takes that tree's e-th free
+  // slot from the top of its free list.
+  uint32_t carry_h = 0;
+  unsigned long long hrows = 0ull;
+  for (uint32_t base = 0; base < n_items; base += blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const bool in = i < n_items;
+    uint32_t t = 0, lt = 0, rt = 0, e = 0, cdepth = 0, L = 0, R = 0, scount = 0, nt = 0;
+    PartItem it{};
+    if (in) {
+      t = B.pend[5 * i];
+      lt = B.pend[5 * i + 1];
+      rt = B.pend[5 * i + 2];
+      e = B.pend[5 * i + 3];
+      cdepth = B.pend[5 * i + 4];
+      it = B.items[i];
+      L = B.cursor[2 * i];
+      R = it.count - L;
+      scount = L <= R ? L : R;
+      nt = need ? (scount + tile - 1) / tile : 0u;
+    }
+    uint32_t tot;
+    const uint32_t h0 = block_excl_scan(nt * P.NG, s_tmp, &tot) + carry_h;
+    carry_h += tot;
+    if (in) {
+      const uint32_t parent = it.node, right = it.right_node;
+      B.node_count[parent] = L;
+      B.node_count[right] = R;
+      {  // children's exact totals from the partition tiles
+        long long lg = 0, lh = 0, rg = 0, rh = 0;
+        if (lds_sums) {
+          lg = s_sum[i][0];
+          lh = s_sum[i][1];
+          rg = s_sum[i][2];
+          rh = s_sum[i][3];
+        } else {
+          for (uint32_t k = 0; k < it.n_tiles; k++) {
+            const long long* ts = B.tile_sums + 4 * (uint64_t)(it.tile0 + k);
+            lg += ts[0];
+            lh += ts[1];
+            rg += ts[2];
+            rh += ts[3];
+          }
         }
+        B.node_sum[2 * parent] = lg;
+        B.node_sum[2 * parent + 1] = lh;
+        B.node_sum[2 * right] = rg;
+        B.node_sum[2 * right + 1] = rh;
       }
-      B.node_sum[2 * parent] = lg;
-      B.node_sum[2 * parent + 1] = lh;
-      B.node_sum[2 * right] = rg;
-      B.node_sum[2 * right + 1] = rh;
-    }
-    const uint8_t cbuf = (uint8_t)((it.pad & 1u) ^ 1u);  // the partition wrote the other buffer
-    B.pbuf[parent] = cbuf;
-    B.pbuf[right] = cbuf;
-    const uint32_t pb = B.pbegin[parent];
-    B.pbegin[right] = pb + L;
-    const bool left_small = L <= R;
-    const uint32_t small = left_small ? parent : right, large = left_small ? right : parent;
-    const uint32_t scount = left_small ? L : R;
-    const uint32_t sbegin = left_small ? pb : pb + L;
-    const uint32_t ps = B.pslot[parent];
-    B.pslot[parent] = kNone;
-    B.pslot[large] = ps;
-    const bool need = P.growth == 1 || cdepth < P.max_depth;
-    if (need) {
-      const uint32_t k = atomicSub(&B.ctl[t].n_free, 1u) - 1u;
-      const uint32_t ss = B.free_slot[(uint64_t)t * P.SPT + k];
-      B.pslot[small] = ss;
-      const uint32_t gss = t * P.SPT + ss, gls = t * P.SPT + ps;
-      const uint32_t nt = (scount + tile - 1) / tile;
-      atomicAdd(&B.counts[C_HROWS], scount);
-      atomicAdd(&B.counts[C_HNODES], 1u);
-      const uint32_t h0 = atomicAdd(&B.counts[C_HIST], nt * P.NG);
-      for (uint32_t q = 0; q < nt; q++)
-        for (uint32_t g = 0; g < P.NG; g++) {
-          HistItem h;
-          h.node = small;
-          h.slot = gss;
-          h.begin = sbegin + q * tile;
-          h.count = min(tile, scount - q * tile);
-          h.group = g;
-          h.tree = t | (nt == 1 ? kDirectFlag : 0u) | (cbuf ? kAltFlag : 0u);
-          B.hitems[h0 + q * P.NG + g] = h;
+      const uint8_t cbuf = (uint8_t)((it.pad & 1u) ^ 1u);  // the partition wrote the other buffer
+      B.pbuf[parent] = cbuf;
+      B.pbuf[right] = cbuf;
+      const uint32_t pb = B.pbegin[parent];
+      B.pbegin[right] = pb + L;
+      const bool left_small = L <= R;
+      const uint32_t small = left_small ? parent : right, large = left_small ? right : parent;
+      const uint32_t sbegin = left_small ? pb : pb + L;
+      const uint32_t ps = B.pslot[parent];
+      B.pslot[large] = ps;
+      if (need) {
+        const uint32_t ss = B.free_slot[(uint64_t)t * P.SPT + (s_free[t] - 1u - e)];
+        B.pslot[small] = ss;
+        const uint32_t gss = t * P.SPT + ss, gls = t * P.SPT + ps;
+        hrows += scount;
+        const uint32_t hflags = t | (nt == 1 ? kDirectFlag : 0u) | (cbuf ? kAltFlag : 0u);
+        if (lds_sums) {  // written by the whole block below
+          s_h0[i] = h0;
+          s_hnode[i] = small;
+          s_hslot[i] = gss;
+          s_hbegin[i] = sbegin;
+          s_hcount[i] = scount;
+          s_htree[i] = hflags;
+        } else {
+          for (uint32_t q = 0; q < nt; q++)
+            for (uint32_t g = 0; g < P.NG; g++) {
+              HistItem h;
+              h.node = small;
+              h.slot = gss;
+              h.begin = sbegin + q * tile;
+              h.count = min(tile, scount - q * tile);
+              h.group = g;
+              h.tree = hflags;
+              B.hitems[h0 + q * P.NG + g] = h;
+            }
         }
-      const uint32_t fi = atomicAdd(&B.counts[C_FIN], 1u);
-      B.fin[4 * fi + 0] = gss;
-      B.fin[4 * fi + 1] = gls;
-      B.fin[4 * fi + 2] = h0;
-      B.fin[4 * fi + 3] = nt;
-      const uint32_t sidx = atomicAdd(&B.counts[C_SPLIT], 2u);
-      B.split_nodes[sidx] = parent;
-      B.split_slots[sidx] = t * P.SPT + B.pslot[parent];
-      B.split_nodes[sidx + 1] = right;
-      B.split_slots[sidx + 1] = t * P.SPT + B.pslot[right];
-    } else {
-      B.pslot[small] = kNone;
+        B.fin[4 * i + 0] = gss;
+        B.fin[4 * i + 1] = gls;
+        B.fin[4 * i + 2] = h0;
+        B.fin[4 * i + 3] = nt;
+        // The large child keeps the parent's slot (ps); the small one has ss.
+        B.split_nodes[2 * i] = parent;
+        B.split_slots[2 * i] = t * P.SPT + (left_small ? ss : ps);
+        B.split_nodes[2 * i + 1] = right;
+        B.split_slots[2 * i + 1] = t * P.SPT + (left_small ? ps : ss);
+      } else {
+        B.pslot[small] = kNone;
+      }
+      const uint32_t lp = parent - t * P.MN, rp = right - t * P.MN;
+      if (P.growth == 0) {
+        const uint64_t nb = (uint64_t)t * 2 * P.W + 2 * e;
+        B.next_node[nb] = lp;
+        B.next_tnode[nb] = lt;
+        B.next_depth[nb] = cdepth;
+        B.next_node[nb + 1] = rp;
+        B.next_tnode[nb + 1] = rt;
+        B.next_depth[nb + 1] = cdepth;
+      } else {
+        const uint64_t ob = (uint64_t)t * P.WO + s_open[t] + 2 * e;
+        B.open_node[ob] = lp;
+        B.open_tnode[ob] = lt;
+        B.open_depth[ob] = cdepth;
+        B.open_node[ob + 1] = rp;
+        B.open_tnode[ob + 1] = rt;
+        B.open_depth[ob + 1] = cdepth;
+      }
     }
-    const uint32_t lp = parent - t * P.MN, rp = right - t * P.MN;
-    if (P.growth == 0) {
-      const uint64_t nb = (uint64_t)t * 2 * P.W + 2 * e;
-      B.next_node[nb] = lp;
-      B.next_tnode[nb] = lt;
-      B.next_depth[nb] = cdepth;
-      B.next_node[nb + 1] = rp;
-      B.next_tnode[nb + 1] = rt;
-      B.next_depth[nb + 1] = cdepth;
-      atomicAdd(&B.ctl[t].n_next, 2u);
-    } else {
-      const uint32_t pos = atomicAdd(&B.ctl[t].n_open, 2u);
-      const uint64_t ob = (uint64_t)t * P.WO + pos;
-      B.open_node[ob] = lp;
-      B.open_tnode[ob] = lt;
-      B.open_depth[ob] = cdepth;
-      B.open_node[ob + 1] = rp;
-      B.open_tnode[ob + 1] = rt;
-      B.open_depth[ob + 1] = cdepth;
+  }
+  const unsigned long long rows_total = block_sum_u64(hrows, s_red64);  // also syncs s_h*
+  if (need && lds_sums) {
+    // Histogram item k belongs to the last item i with s_h0[i] <= k (every
+    // item has at least one tile, so s_h0 is strictly increasing).
+    for (uint32_t k = threadIdx.x; k < carry_h; k += blockDim.x) {
+      uint32_t lo = 0, hi = n_items - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (s_h0[mid] <= k) lo = mid; else hi = mid - 1;
+      }
+      const uint32_t r = k - s_h0[lo], q = r / P.NG, g = r % P.NG;
+      HistItem h;
+      h.node = s_hnode[lo];
+      h.slot = s_hslot[lo];
+      h.begin = s_hbegin[lo] + q * tile;
+      h.count = min(tile, s_hcount[lo] - q * tile);
+      h.group = g;
+      h.tree = s_htree[lo];
+      B.hitems[k] = h;
     }
+  }
+  if (threadIdx.x == 0) {
+    B.counts[C_HIST] = carry_h;
+    B.counts[C_FIN] = need ? n_items : 0u;
+    B.counts[C_SPLIT] = need ? 2u * n_items : 0u;
+    B.counts[C_HROWS] += (uint32_t)rows_total;
+    B.counts[C_HNODES] += need ? n_items : 0u;
+  }
+  if (threadIdx.x < P.T) {
+    TreeCtl& c = B.ctl[threadIdx.x];
+    const uint32_t cnt = s_cnt[threadIdx.x];
+    if (need) c.n_free = s_free[threadIdx.x] - cnt;
+    if (P.growth == 0)
+      c.n_next += 2u * cnt;
+    else
+      c.n_open = s_open[threadIdx.x] + 2u * cnt;
   }
 }
 

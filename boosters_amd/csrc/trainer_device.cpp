@@ -93,6 +93,7 @@ void Trainer::device_setup() {
   P.T = (uint32_t)T_;
   P.MN = MN_;
   P.SPT = SPT_;
+  if (P.T > kCtlMaxTrees) raise(BST_E_CONFIG, "more trees per round than the device grower holds");
   const uint32_t max_leaves = c.growth == BST_GROW_DEPTHWISE ? (1u << c.max_depth) : c.max_leaves;
   P.MNT = 2 * max_leaves + 1;
   P.growth = c.growth == BST_GROW_DEPTHWISE ? 0 : 1;
