@@ -97,6 +97,29 @@ def train_timed(ctx, ds, y_dev, n, cfg, e):
     return h, (time.perf_counter() - t0) * 1e3
 
 
+PROFILE_C2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                          "k_hist_logistic_1Mx100_depth8.md")
+
+
+def hist_traffic_per_launch():
+    """HBM bytes per k_hist launch measured by rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (calibrated against a copy kernel) in the committed C2 profile:
+    the launch-weighted mean over its root and gathered histogram launches."""
+    try:
+        text = open(PROFILE_C2).read()
+    except OSError:
+        return None
+    section = text.split("## HBM traffic", 1)[-1].split("\n## ", 1)[0]
+    rows = [l for l in section.splitlines() if l.startswith("| `k_hist<")]
+    n = tot = 0.0
+    for l in rows:
+        cells = [c.strip() for c in l.strip().strip("|").split("|")]
+        launches, fetch_mb, write_mb = int(cells[1]), float(cells[2]), float(cells[3])
+        n += launches
+        tot += launches * (fetch_mb + write_mb) * 1e6
+    return round(tot / n) if n else None
+
+
 def tuning(ctx):
     """Histogram launch shape of the last train (autotuned or fixed)."""
     v = [C.c_int32() for _ in range(4)]
@@ -516,7 +539,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": hist_traffic_per_launch(),
+            "traffic_source": "profiles/k_hist_logistic_1Mx100_depth8.md (PMC FETCH_SIZE + WRITE_SIZE)",
             "kernel": "k_hist",
             "launches": launches,
             "avg_launch_us": round(hms / launches * 1e3, 2),
