@@ -1040,46 +1040,28 @@ __device__ __forceinline__ unsigned long long cand_key(float gain, uint32_t rank
 
 constexpr int kSplitWaves = 8;  // features per block
 
-__global__ __launch_bounds__(kSplitWaves * 64) void k_split(
-    const uint32_t* __restrict__ nodes, const uint32_t* __restrict__ slots,
-    const uint32_t* __restrict__ feats, int n_feats, const FeatMeta* __restrict__ fmeta, int FG,
-    int NG, int E, const long long* __restrict__ hist, int64_t slot_elems,
-    const long long* __restrict__ node_sum, const uint32_t* __restrict__ node_count,
-    const int32_t* __restrict__ qexp, uint32_t max_nodes, SplitParams p,
-    unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand,
-    const uint32_t* __restrict__ dyn, int feat_stride) {
-  __shared__ double s_ratio[kSplitWaves][256];
-  __shared__ uint16_t s_order[kSplitWaves][256];
-  const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  // A block's kSplitWaves waves take consecutive features: with the
-  // [group][bin][FG] layout their (g, h) pairs of one bin share one 128-byte
-  // line, which the block then uses whole.
-  const int fi = blockIdx.x * kSplitWaves + wv;
-  if (fi >= n_feats) return;  // whole wave; the kernel has no block barrier
-  const uint32_t n_work = dyn ? *dyn : gridDim.y;
-  for (uint32_t wy = blockIdx.y; wy < n_work; wy += gridDim.y) {
-  const uint32_t node = nodes[wy];
-  const uint32_t slot = slots[wy];
-  const uint32_t tree = node / max_nodes;
-  const uint32_t f = feats[(int64_t)tree * feat_stride + fi];
-  const double ig = ldexp(1.0, -qexp[2 * tree]), ih = ldexp(1.0, -qexp[2 * tree + 1]);
-  const long long G = node_sum[2 * node], H = node_sum[2 * node + 1];
-  const uint32_t cnt = node_count[node];
-  const double pg = (double)G * ig, ph = (double)H * ih;
-  const GainCtx ctx(pg, ph, p);
-  const FeatMeta fm = fmeta[f];
+// Best split of one feature of one node, one wave (find.rs:286-573): bin b
+// comes from `load(b)` as its (g, h) fixed-point pair; `ratio` / `order` are
+// the wave's 256-entry scratch for the sorted categorical partition.  Lane 0
+// writes the feature's candidate; every lane returns the feature's packed
+// node key (gain, then the tie rule's feature order), 0 if nothing is valid.
+template <class BinLoad>
+__device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, uint32_t node,
+                                            uint32_t f, long long G, long long H, uint32_t cnt,
+                                            double ig, double ih, const SplitParams& p,
+                                            const GainCtx& ctx, double* ratio, uint16_t* order,
+                                            int E, DevSplit* __restrict__ cand) {
+  const int lane = threadIdx.x & (WAVE - 1);
   const uint32_t nb = fm.n_bins;
-  const long long* base = hist + (int64_t)slot * slot_elems;
-  const int grp = (int)f / FG, lf = (int)f % FG;
-  // Load bins 4*lane .. 4*lane+3.
+  // Bins 4*lane .. 4*lane+3.
   long long vg[4], vh[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const int b = 4 * lane + j;
-    const int64_t e = ((int64_t)(grp * 256 + b) * FG + lf) * 2;
     const bool in = (uint32_t)b < nb;
-    vg[j] = in ? base[e] : 0;
-    vh[j] = in ? base[e + 1] : 0;
+    const longlong2 v = in ? load((uint32_t)b) : make_longlong2(0, 0);
+    vg[j] = v.x;
+    vh[j] = v.y;
   }
   unsigned long long best = 0;  // 0 = no valid candidate
   DevSplit out;
@@ -1191,8 +1173,6 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     } else {
       // Sorted ratio partition (find.rs:511-573): categories with hess > 1e-10
       // sorted stably by -G/(H+1e-6), then a prefix scan in that order.
-      double* ratio = s_ratio[wv];
-      uint16_t* order = s_order[wv];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const uint32_t k = 4 * lane + j;
@@ -1233,12 +1213,9 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
           oh[j] = 0;
           if (q < m) {
             const uint32_t k = order[q];
-            const int src = (int)(k >> 2), jj = (int)(k & 3);
-            (void)src;
-            (void)jj;
-            const int64_t e = ((int64_t)(grp * 256 + (int)k) * FG + lf) * 2;
-            og[j] = base[e];
-            oh[j] = base[e + 1];
+            const longlong2 v = load(k);
+            og[j] = v.x;
+            oh[j] = v.y;
           }
         }
         const long long sgl = og[0] + og[1] + og[2] + og[3], shl = oh[0] + oh[1] + oh[2] + oh[3];
@@ -1272,13 +1249,76 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
       }
     }
   }
-  if (lane == 0) {
-    cand[(int64_t)node * E + f] = out;
-    if (best) {
-      const uint32_t tb = p.rightmost ? f : (0xFFFFFFFFu - f);
-      atomicMax(&node_key[node], ((unsigned long long)ord_f32(out.gain) << 32) | tb);
+  if (lane == 0) cand[(int64_t)node * E + f] = out;
+  if (!best) return 0ull;
+  const uint32_t tb = p.rightmost ? f : (0xFFFFFFFFu - f);
+  return ((unsigned long long)ord_f32(out.gain) << 32) | tb;
+}
+
+// One block = one node x kSplitWaves consecutive candidate features.  The
+// block first stages the features' 256 bins into LDS ([feature][bin]): with
+// the [group][bin][FG] histogram layout the features' pairs of one bin are
+// adjacent, so consecutive threads read consecutive 16-byte pairs and each
+// 128-byte line is fetched once per block instead of once per wave.  Then
+// wave w evaluates feature w from LDS.
+__global__ __launch_bounds__(kSplitWaves * 64) void k_split(
+    const uint32_t* __restrict__ nodes, const uint32_t* __restrict__ slots,
+    const uint32_t* __restrict__ feats, int n_feats, const FeatMeta* __restrict__ fmeta, int FG,
+    int NG, int E, const long long* __restrict__ hist, int64_t slot_elems,
+    const long long* __restrict__ node_sum, const uint32_t* __restrict__ node_count,
+    const int32_t* __restrict__ qexp, uint32_t max_nodes, SplitParams p,
+    unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand,
+    const uint32_t* __restrict__ dyn, int feat_stride) {
+  __shared__ double s_ratio[kSplitWaves][256];
+  __shared__ uint16_t s_order[kSplitWaves][256];
+  __shared__ longlong2 s_bins[kSplitWaves][256];
+  __shared__ unsigned long long s_key[kSplitWaves];
+  constexpr int kThreads = kSplitWaves * 64;
+  const int wv = threadIdx.x / WAVE;
+  const int fi = blockIdx.x * kSplitWaves + wv;
+  const uint32_t n_work = dyn ? *dyn : gridDim.y;
+  for (uint32_t wy = blockIdx.y; wy < n_work; wy += gridDim.y) {  // block-uniform
+    const uint32_t node = nodes[wy];
+    const uint32_t slot = slots[wy];
+    const uint32_t tree = node / max_nodes;
+    const long long* base = hist + (int64_t)slot * slot_elems;
+    const uint32_t* fl = feats + (int64_t)tree * feat_stride;
+    // Stage: thread t loads bin t / kSplitWaves + pass * (kThreads / kSplitWaves)
+    // of the block's feature t % kSplitWaves.
+    {
+      const int w = threadIdx.x % kSplitWaves;
+      const int fw = blockIdx.x * kSplitWaves + w;
+      if (fw < n_feats) {
+        const uint32_t f = fl[fw];
+        const int grp = (int)f / FG, lf = (int)f % FG;
+        for (int b = threadIdx.x / kSplitWaves; b < 256; b += kThreads / kSplitWaves) {
+          const int64_t e = ((int64_t)(grp * 256 + b) * FG + lf) * 2;
+          s_bins[w][b] = make_longlong2(base[e], base[e + 1]);
+        }
+      }
     }
-  }
+    __syncthreads();
+    unsigned long long key = 0ull;
+    if (fi < n_feats) {
+      const uint32_t f = fl[fi];
+      const double ig = ldexp(1.0, -qexp[2 * tree]), ih = ldexp(1.0, -qexp[2 * tree + 1]);
+      const long long G = node_sum[2 * node], H = node_sum[2 * node + 1];
+      const uint32_t cnt = node_count[node];
+      const GainCtx ctx((double)G * ig, (double)H * ih, p);
+      const FeatMeta fm = fmeta[f];
+      auto load = [&](uint32_t b) { return s_bins[wv][b]; };
+      key = split_feature(load, fm, node, f, G, H, cnt, ig, ih, p, ctx, s_ratio[wv],
+                          s_order[wv], E, cand);
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) s_key[wv] = key;
+    __syncthreads();  // also: s_bins is restaged for the block's next node
+    // One atomic per (node, block): the node's key word is shared by every
+    // feature block of the node, so per-feature atomics serialise on it.
+    if (threadIdx.x == 0) {
+      unsigned long long m = 0ull;
+      for (int w = 0; w < kSplitWaves; w++) m = s_key[w] > m ? s_key[w] : m;
+      if (m) atomicMax(&node_key[node], m);
+    }
   }
 }
 
