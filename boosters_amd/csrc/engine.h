@@ -48,6 +48,7 @@ struct DevSplit {
   uint32_t bin;
   uint32_t flags;  // bit0 default_left, bit1 categorical
   uint32_t cats[kMaxCatWords];
+  long long lg, lh;  // exact fixed-point (g, h) sums of the rows going left
 };
 
 // One node to split: rows [begin, begin+count) of idx/ogh.
@@ -178,16 +179,15 @@ void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items,
 // Device-driven partition, one pass: each tile reserves its left rows from
 // the front and its right rows from the back of the node's range in the other
 // buffer of the pair (cursor[2 item] / cursor[2 item + 1], zeroed by the
-// caller; the final left cursor is the left count) and stores the exact
-// totals of its left and right rows in tile_sums[4 tile ..].  Row order
-// inside a node is not kept (no result depends on it: all sums are exact
-// integers).  Counts read from dyn_tiles.
+// caller; the final left cursor is the left count).  Row order inside a node
+// is not kept (no result depends on it: all sums are exact integers; the
+// children's totals come from the split's left sums).  Counts read from
+// dyn_tiles.
 constexpr uint32_t kFusedTileRows = 4096;  // rows per tile of launch_partition_fused
 void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
                             const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
                             uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,
-                            uint32_t* cursor, const int32_t* qexp, uint32_t max_nodes,
-                            long long* tile_sums, const uint32_t* dyn_tiles);
+                            uint32_t* cursor, const uint32_t* dyn_tiles);
 
 void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const float* leaf_val,
                         const uint32_t* idx, int64_t tree_stride_rows, float* preds,

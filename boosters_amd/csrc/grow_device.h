@@ -105,7 +105,6 @@ struct DeviceBufs {
   const uint32_t* eff_to_orig;   // [E]
   const int32_t* qexp;           // [2K]
   uint32_t* cursor;              // [items][2] partition cursors; [2 i] ends as the left count
-  const long long* tile_sums;    // [tiles][4] left g, left h, right g, right h
 };
 
 void launch_ctl_init(hipStream_t s, const CtlParams& p, const DeviceBufs& b,

@@ -270,6 +270,8 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
       sp.bin = 0;
       sp.flags = 0;
       for (int w = 0; w < kMaxCatWords; w++) sp.cats[w] = 0;
+      sp.lg = 0;
+      sp.lh = 0;
     } else {
       const uint32_t lo = (uint32_t)(key & 0xFFFFFFFFu);
       const uint32_t f = P.rightmost ? lo : (0xFFFFFFFFu - lo);
@@ -467,10 +469,9 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
 // After the partition: child counts, slots, the smaller child's histogram
 // items, the finish list (partials + subtraction), the split list, and the
 // children pushed to the open set (leaf-wise) or the next level (depth-wise).
-constexpr uint32_t kChildLdsItems = 1024;  // items whose tile totals are reduced in LDS
+constexpr uint32_t kChildLdsItems = 1024;  // items whose histogram items are filled block-wide
 
 __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, DeviceBufs B) {
-  __shared__ long long s_sum[kChildLdsItems][4];
   __shared__ unsigned long long s_red64[kCtlThreads / 64];
   __shared__ uint32_t s_tmp[kCtlThreads / 64 + 1];
   // Per tree: items this step, free slots and open-set size at entry.
@@ -479,7 +480,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
   __shared__ uint32_t s_h0[kChildLdsItems], s_hnode[kChildLdsItems], s_hslot[kChildLdsItems],
       s_hbegin[kChildLdsItems], s_hcount[kChildLdsItems], s_htree[kChildLdsItems];
   const uint32_t n_items = B.counts[C_ITEMS];
-  const bool lds_sums = n_items <= kChildLdsItems;
+  const bool lds_items = n_items <= kChildLdsItems;
   if (threadIdx.x < P.T) {
     s_cnt[threadIdx.x] = 0;
     s_free[threadIdx.x] = B.ctl[threadIdx.x].n_free;
@@ -498,19 +499,6 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
       step_rows += L <= R ? L : R;
     }
   const uint32_t tile = step_tile(P, block_sum_u64(step_rows, s_red64));
-  if (lds_sums) {
-    // Children's exact totals: every thread adds whole partition tiles.
-    for (uint32_t i = threadIdx.x; i < 4 * n_items; i += blockDim.x) (&s_sum[0][0])[i] = 0;
-    __syncthreads();
-    const uint32_t n_tiles = B.counts[C_TILES];
-    for (uint32_t k = threadIdx.x; k < n_tiles; k += blockDim.x) {
-      const uint32_t item = B.tiles[k].item;
-      const long long* ts = B.tile_sums + 4 * (uint64_t)k;
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-        atomicAdd((unsigned long long*)&s_sum[item][j], (unsigned long long)ts[j]);
-    }
-  }
   // Items per tree: the plan numbers a tree's items e = 0, 1, ... in order.
   for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x)
     atomicMax(&s_cnt[B.pend[5 * i]], B.pend[5 * i + 3] + 1u);
@@ -545,26 +533,13 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
       const uint32_t parent = it.node, right = it.right_node;
       B.node_count[parent] = L;
       B.node_count[right] = R;
-      {  // children's exact totals from the partition tiles
-        long long lg = 0, lh = 0, rg = 0, rh = 0;
-        if (lds_sums) {
-          lg = s_sum[i][0];
-          lh = s_sum[i][1];
-          rg = s_sum[i][2];
-          rh = s_sum[i][3];
-        } else {
-          for (uint32_t k = 0; k < it.n_tiles; k++) {
-            const long long* ts = B.tile_sums + 4 * (uint64_t)(it.tile0 + k);
-            lg += ts[0];
-            lh += ts[1];
-            rg += ts[2];
-            rh += ts[3];
-          }
-        }
+      {  // children's exact totals: the split's left sums, and the rest
+        const long long lg = B.node_split[parent].lg, lh = B.node_split[parent].lh;
+        const long long pg = B.node_sum[2 * parent], ph = B.node_sum[2 * parent + 1];
         B.node_sum[2 * parent] = lg;
         B.node_sum[2 * parent + 1] = lh;
-        B.node_sum[2 * right] = rg;
-        B.node_sum[2 * right + 1] = rh;
+        B.node_sum[2 * right] = pg - lg;
+        B.node_sum[2 * right + 1] = ph - lh;
       }
       const uint8_t cbuf = (uint8_t)((it.pad & 1u) ^ 1u);  // the partition wrote the other buffer
       B.pbuf[parent] = cbuf;
@@ -582,7 +557,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
         const uint32_t gss = t * P.SPT + ss, gls = t * P.SPT + ps;
         hrows += scount;
         const uint32_t hflags = t | (nt == 1 ? kDirectFlag : 0u) | (cbuf ? kAltFlag : 0u);
-        if (lds_sums) {  // written by the whole block below
+        if (lds_items) {  // written by the whole block below
           s_h0[i] = h0;
           s_hnode[i] = small;
           s_hslot[i] = gss;
@@ -635,7 +610,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
     }
   }
   const unsigned long long rows_total = block_sum_u64(hrows, s_red64);  // also syncs s_h*
-  if (need && lds_sums) {
+  if (need && lds_items) {
     // Histogram item k belongs to the last item i with s_h0[i] <= k (every
     // item has at least one tile, so s_h0 is strictly increasing).
     for (uint32_t k = threadIdx.x; k < carry_h; k += blockDim.x) {

@@ -1040,6 +1040,20 @@ __device__ __forceinline__ unsigned long long cand_key(float gain, uint32_t rank
 
 constexpr int kSplitWaves = 8;  // features per block
 
+// Wave-wide best candidate key; the winning lane's left-child sums go to
+// out.lg / out.lh (keys are unique per candidate, so one lane matches).
+__device__ __forceinline__ unsigned long long wave_best(unsigned long long local, long long lg,
+                                                        long long lh, DevSplit& out) {
+  const unsigned long long wb = wave_max_ull(local);
+  const unsigned long long who = __ballot(wb != 0ull && local == wb);
+  const int src = who ? __ffsll((unsigned long long)who) - 1 : 0;
+  out.lg = (long long)(((unsigned long long)(unsigned)__shfl((int)(lg >> 32), src, WAVE) << 32) |
+                       (unsigned)__shfl((int)(lg & 0xFFFFFFFF), src, WAVE));
+  out.lh = (long long)(((unsigned long long)(unsigned)__shfl((int)(lh >> 32), src, WAVE) << 32) |
+                       (unsigned)__shfl((int)(lh & 0xFFFFFFFF), src, WAVE));
+  return wb;
+}
+
 // Best split of one feature of one node, one wave (find.rs:286-573): bin b
 // comes from `load(b)` as its (g, h) fixed-point pair; `ratio` / `order` are
 // the wave's 256-entry scratch for the sorted categorical partition.  Lane 0
@@ -1064,8 +1078,11 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
     vh[j] = v.y;
   }
   unsigned long long best = 0;  // 0 = no valid candidate
+  long long bl_g = 0, bl_h = 0;  // left-child sums of this lane's best candidate
   DevSplit out;
   out.gain = -INFINITY;
+  out.lg = 0;
+  out.lh = 0;
   out.feature = f;
   out.bin = 0;
   out.flags = 0;
@@ -1115,7 +1132,11 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
           if (valid_split(p, hl, hr, cl, cr)) {
             const float gn = ctx.gain(gl, hl, gr, hr);
             const unsigned long long k = cand_key(gn, b);
-            if (k > best) best = k;
+            if (k > best) {
+              best = k;
+              bl_g = GL;
+              bl_h = HL;
+            }
           }
         }
         // Backward: bins b..me right, split at b-1, missing left; b in [1, me].
@@ -1128,11 +1149,15 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
           if (valid_split(p, hl, hr, cl, cr)) {
             const float gn = ctx.gain(gl, hl, gr, hr);
             const unsigned long long k = cand_key(gn, 256u + (uint32_t)(me - (int)b));
-            if (k > best) best = k;
+            if (k > best) {
+              best = k;
+              bl_g = G - GR;
+              bl_h = H - HR;
+            }
           }
         }
       }
-      best = wave_max_ull(best);
+      best = wave_best(best, bl_g, bl_h, out);
       if (best) {
         const uint32_t rank = 0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFu);
         out.gain = unord_f32((uint32_t)(best >> 32));
@@ -1159,11 +1184,15 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
           if (valid_split(p, hl, hr, 1u, sat_sub(cnt, 1u))) {
             const float gn = ctx.gain(gl, hl, gr, hr);
             const unsigned long long key = cand_key(gn, k);
-            if (key > best) best = key;
+            if (key > best) {
+              best = key;
+              bl_g = vg[j];
+              bl_h = vh[j];
+            }
           }
         }
       }
-      best = wave_max_ull(best);
+      best = wave_best(best, bl_g, bl_h, out);
       if (best) {
         const uint32_t k = 0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFu);
         out.gain = unord_f32((uint32_t)(best >> 32));
@@ -1232,11 +1261,15 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
             if (valid_split(p, hl, hr, cl, cr)) {
               const float gn = ctx.gain(gl, hl, gr, hr);
               const unsigned long long key = cand_key(gn, q);
-              if (key > best) best = key;
+              if (key > best) {
+                best = key;
+                bl_g = pg_;
+                bl_h = ph_;
+              }
             }
           }
         }
-        best = wave_max_ull(best);
+        best = wave_best(best, bl_g, bl_h, out);
         if (best) {
           const uint32_t qb = 0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFu);
           out.gain = unord_f32((uint32_t)(best >> 32));
@@ -1339,6 +1372,8 @@ __global__ void k_split_finalize(const uint32_t* __restrict__ nodes, int n_nodes
     s.bin = 0;
     s.flags = 0;
     for (int w = 0; w < kMaxCatWords; w++) s.cats[w] = 0;
+    s.lg = 0;
+    s.lh = 0;
   } else {
     const uint32_t lo = (uint32_t)(k & 0xFFFFFFFFu);
     const uint32_t f = rightmost ? lo : (0xFFFFFFFFu - lo);
@@ -1529,9 +1564,8 @@ __device__ __forceinline__ long long block_sum_ll(long long v, long long* s_red)
 // coalesced and each thread has kFusedRows independent row loads in flight.
 // Left rows keep their tile order in the front region, right rows in the
 // back region; the tile's place in each region comes from one atomic per
-// region, so tiles of a node may land in any order.  Each tile stores the
-// exact fixed-point (g, h) totals of its left and right rows in
-// tile_sums[4 tile .. 4 tile + 3]; the caller adds a node's tiles.
+// region, so tiles of a node may land in any order.  The children's totals
+// are not summed here: the split recorded the exact left sums.
 constexpr int kFusedThreads = 1024;
 constexpr int kFusedRows = kFusedTileRows / kFusedThreads;  // rows per thread
 constexpr int kFusedWaves = kFusedThreads / WAVE;
@@ -1543,12 +1577,10 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
     const uint8_t* __restrict__ bins_cm, const uint32_t* __restrict__ hi_cm, int64_t n,
     int64_t hi_words, uint32_t* __restrict__ idx0, float2* __restrict__ ogh0,
     uint32_t* __restrict__ idx1, float2* __restrict__ ogh1, uint32_t* __restrict__ cursor,
-    const int32_t* __restrict__ qexp, uint32_t max_nodes, long long* __restrict__ tile_sums,
     const uint32_t* __restrict__ dyn_tiles) {
   __shared__ uint32_t s_l[kFusedRows * kFusedWaves];  // lefts per (row pass, wave)
   __shared__ uint32_t s_v[kFusedRows * kFusedWaves];  // rows per (row pass, wave)
   __shared__ uint32_t s_base[2];
-  __shared__ long long s_red[kFusedWaves];
   const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   const uint32_t n_work = *dyn_tiles;
@@ -1621,9 +1653,6 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
     }
     __syncthreads();
     const uint32_t lbase = s_base[0], rbase = s_base[1];
-    const uint32_t tree = it.node / max_nodes;
-    const double sg = ldexp(1.0, qexp[2 * tree]), sh = ldexp(1.0, qexp[2 * tree + 1]);
-    long long lg = 0, lh = 0, rg = 0, rh = 0;
 #pragma unroll
     for (int j = 0; j < kFusedRows; j++) {
       const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
@@ -1634,39 +1663,20 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
         const uint32_t dst = it.begin + (l ? lbase + lrank : rbase + (vrank - lrank));
         didx[dst] = rid[j];
         dogh[dst] = gh[j];
-        const long long gq = __double2ll_rn((double)gh[j].x * sg);
-        const long long hq = __double2ll_rn((double)gh[j].y * sh);
-        if (l) {
-          lg += gq;
-          lh += hq;
-        } else {
-          rg += gq;
-          rh += hq;
-        }
       }
     }
-    lg = block_sum_ll(lg, s_red);
-    lh = block_sum_ll(lh, s_red);
-    rg = block_sum_ll(rg, s_red);
-    rh = block_sum_ll(rh, s_red);
-    if (threadIdx.x == 0) {
-      tile_sums[4 * (int64_t)wi + 0] = lg;
-      tile_sums[4 * (int64_t)wi + 1] = lh;
-      tile_sums[4 * (int64_t)wi + 2] = rg;
-      tile_sums[4 * (int64_t)wi + 3] = rh;
-    }
+    __syncthreads();  // s_l, s_v, s_base are rewritten by the next tile
   }
 }
 
 void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
                             const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
                             uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,
-                            uint32_t* cursor, const int32_t* qexp, uint32_t max_nodes,
-                            long long* tile_sums, const uint32_t* dyn_tiles) {
+                            uint32_t* cursor, const uint32_t* dyn_tiles) {
   if (grid_tiles <= 0) return;
   hipLaunchKernelGGL(k_part_fused, dim3(grid_tiles), dim3(kFusedThreads), 0, s, items, tiles,
                      node_split, d.meta, d.bins_cm, d.hi_cm, d.n, d.hi_words, idx0, ogh0, idx1,
-                     ogh1, cursor, qexp, max_nodes, tile_sums, dyn_tiles);
+                     ogh1, cursor, dyn_tiles);
 }
 
 void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items, int n_items,

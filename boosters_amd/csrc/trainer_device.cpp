@@ -51,7 +51,6 @@ struct Trainer::DevState {
   DevBuf<Segment> leaf_segs;
   DevBuf<float> thr_up;
   DevBuf<long long> partial;    // one histogram group per multi-tile item
-  DevBuf<long long> tile_sums;  // [tiles][4] partition totals
   // Sampling: row draws (uniform / GOSS) and per-tree column lists.
   SampleParams SP{};
   int sample_mode = kSampleNone;
@@ -143,7 +142,6 @@ void Trainer::device_setup() {
   s.free_slot.alloc((size_t)T * SPT_);
   s.pbuf.alloc((size_t)T * MN_);
   s.cursor.alloc((size_t)2 * s.cap_items);
-  s.tile_sums.alloc((size_t)4 * s.cap_tiles);
   // Row sampling (row.rs:141-270) and column sampling (column.rs:266-283).
   {
     SampleParams& SP = s.SP;
@@ -255,7 +253,6 @@ void Trainer::device_setup() {
   B.eff_to_orig = s.eff_to_orig.p;
   B.qexp = qexp_.p;
   B.cursor = s.cursor.p;
-  B.tile_sums = s.tile_sums.p;
   BST_HIP(hipStreamSynchronize(s_));
   dev_ = std::move(st);
 }
@@ -319,8 +316,7 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
     launch_ctl_plan(s_, P, B, /*advance=*/step > 0);
     int pi = ctx_->prof_begin(2, 0.0);
     launch_partition_fused(s_, d, B.items, B.tiles, g_tiles, node_split_.p, idx_.p, ogh_.p,
-                           idx_tmp_.p, ogh_tmp_.p, B.cursor, qexp_.p, MN_, S.tile_sums.p,
-                           cnt + C_TILES);
+                           idx_tmp_.p, ogh_tmp_.p, B.cursor, cnt + C_TILES);
     ctx_->prof_end(pi);
     launch_ctl_children(s_, P, B);
     // The last step's children are never split (depth limit, or the leaf
