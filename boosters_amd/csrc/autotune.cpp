@@ -14,15 +14,19 @@
 //
 // The tuner times the histogram + finish launches of the steps a tree of this
 // configuration will run, on this dataset's own bins: for depth-wise growth
-// the root and every level's smaller children (balanced sizes), for
-// leaf-wise the root and single nodes of n/2 .. n/256 rows.  Rows are
-// gathered through a scattered permutation, as after partitioning.  Each
-// candidate runs once to warm up and is then timed twice (the faster run
-// counts); the cheapest weighted total wins.  Results are cached in the
-// context per data shape.
+// the root (rows in place when the round is unsampled with one tree) and
+// every level's smaller children (n/2 x 0.75^(L-1) rows at level L, node
+// sizes spread over 0.25x .. 1.75x of the mean), for leaf-wise the root and single
+// nodes of n/2 .. n/256 rows.  Gathered rows follow a scattered permutation,
+// as after partitioning.  Three rounds over all candidates (the first warms
+// clocks and caches and is discarded) keep each (candidate, step) pair's
+// fastest time; the cheapest weighted total wins.  Results are cached in the
+// context per data shape.  BST_TRACE_TUNE=1 prints every candidate's total.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "host_common.h"
 
@@ -36,8 +40,10 @@ constexpr uint32_t kSlots = 64;              // histogram slots the measurement 
 
 struct Shape {
   uint32_t nodes;      // built nodes in the step
-  uint32_t rows;       // rows per node
+  uint32_t rows;       // mean rows per node
   double weight;       // how many such steps a tree runs
+  bool gather = true;  // rows read through the row-id list
+  bool spread = false; // node sizes spread over 0.25x .. 1.75x of the mean
 };
 
 uint32_t step_tile(uint32_t tile_min, uint32_t target, uint32_t NG, uint64_t rows) {
@@ -47,23 +53,28 @@ uint32_t step_tile(uint32_t tile_min, uint32_t target, uint32_t NG, uint64_t row
   return tile;
 }
 
-std::vector<Shape> tree_shapes(int64_t n, const bst_gbdt_config& cfg, int T) {
+std::vector<Shape> tree_shapes(int64_t n, const bst_gbdt_config& cfg, int T, bool root_gather) {
   std::vector<Shape> sh;
   const uint32_t N = (uint32_t)std::min<int64_t>(n, 0xFFFFFFFFll);
-  sh.push_back({(uint32_t)T, N, 1.0});
+  sh.push_back({(uint32_t)T, N, 1.0, root_gather, false});
   if (cfg.growth == BST_GROW_DEPTHWISE) {
+    // Level L builds the smaller child of each of its 2^(L-1) parents, in
+    // nodes of uneven sizes.  Uneven splits make the smaller children hold
+    // less than half the rows, the more so the deeper the level: taken as
+    // n/2 x 0.75^(L-1).
     for (int L = 1; L < cfg.max_depth; L++) {
       const uint32_t nodes = 1u << std::min(L - 1, 10);
-      const uint32_t rows = (uint32_t)std::max<int64_t>(1, n >> L);
+      const double total = 0.5 * (double)n * std::pow(0.75, L - 1);
+      const uint32_t rows = (uint32_t)std::max(1.0, total / nodes);
       if (rows < 64) break;
-      sh.push_back({nodes * (uint32_t)T, rows, 1.0});
+      sh.push_back({nodes * (uint32_t)T, rows, 1.0, true, nodes > 1});
     }
   } else {
     const double steps = std::max(1, (int)cfg.max_leaves - 1);
     const int ks[4] = {1, 3, 5, 8};  // n/2, n/8, n/32, n/256
     for (int k : ks) {
       const uint32_t rows = (uint32_t)std::max<int64_t>(1, n >> k);
-      sh.push_back({(uint32_t)T, rows, steps / 4.0});
+      sh.push_back({(uint32_t)T, rows, steps / 4.0, true, false});
     }
   }
   // Keep each measured step within kMaxStepRows rows (wide class-batched
@@ -84,24 +95,30 @@ struct Plan {
   std::vector<HistItem> items;
   std::vector<uint32_t> fin;
   uint32_t nodes = 0;
+  bool gather = true;
 };
 
 Plan plan_step(const Shape& sh, uint32_t NG, uint32_t slots, int64_t n, uint32_t tile_min,
                uint32_t target) {
   Plan pl;
   pl.nodes = sh.nodes;
+  pl.gather = sh.gather;
   const uint64_t total = (uint64_t)sh.nodes * sh.rows;
   const uint32_t tile = step_tile(tile_min, target, NG, total);
-  const uint32_t nt = (sh.rows + tile - 1) / tile;
-  const uint64_t span = (uint64_t)std::max<int64_t>(1, n - (int64_t)sh.rows + 1);
   uint64_t pos = 0;
   for (uint32_t v = 0; v < sh.nodes; v++) {
+    // Spread sizes: 0.25x .. 1.75x of the mean, linear in the node index.
+    const double f = sh.spread ? 0.25 + 1.5 * (double)v / (double)(sh.nodes - 1) : 1.0;
+    const uint32_t rows =
+        (uint32_t)std::min<double>((double)n, std::max(1.0, std::floor(f * (double)sh.rows)));
+    const uint32_t nt = (rows + tile - 1) / tile;
     const uint32_t slot = v % slots;
+    const uint64_t span = (uint64_t)std::max<int64_t>(1, n - (int64_t)rows + 1);
     const uint32_t begin = (uint32_t)(pos % span);  // consecutive ranges, wrapped inside n
-    pos += sh.rows;
+    pos += rows;
     pl.fin.insert(pl.fin.end(), {slot, 0xFFFFFFFFu, (uint32_t)pl.items.size(), nt});
     for (uint32_t k = 0; k < nt; k++) {
-      const uint32_t b0 = k * tile, cnt = std::min(tile, sh.rows - b0);
+      const uint32_t b0 = k * tile, cnt = std::min(tile, rows - b0);
       for (uint32_t g = 0; g < NG; g++)
         pl.items.push_back({v, slot, begin + b0, cnt, g, nt == 1 ? 0x80000000u : 0u});
     }
@@ -125,9 +142,9 @@ struct Bench {
     if (b) (void)hipEventDestroy(b);
   }
 
-  // Time (ms) of one step's histogram + finish launches: the faster of
-  // `reps` runs after one warm-up run.
-  float run(const Plan& pl, int fg, int reps) {
+  // Time (ms) of one step's histogram + finish launches: the fastest of
+  // `runs` runs.
+  float run(const Plan& pl, int fg, int runs) {
     DatasetDev dd = d;
     dd.FG = fg;
     dd.NG = (d.E + fg - 1) / fg;
@@ -139,17 +156,17 @@ struct Bench {
     // The training launch: a resident grid walking a device-side item count.
     const int grid = (hist_blocks_resident(fg) * ctx->n_cu + 7) / 8 * 8;
     float best = 1e30f;
-    for (int r = 0; r < reps + 1; r++) {
+    for (int r = 0; r < runs; r++) {
       BST_HIP(hipEventRecord(a, ctx->stream));
       launch_hist(ctx->stream, dd, items.p, grid, idx.p, ogh.p, 0, qexp.p, hist.p, slot_elems,
-                  nullptr, true, 0, count.p, partial.p, nullptr, nullptr);
+                  nullptr, pl.gather, 0, count.p, partial.p, nullptr, nullptr);
       launch_hist_finish(ctx->stream, fin.p, (int)pl.nodes, dd.NG, fg, partial.p, hist.p,
                          slot_elems, nullptr);
       BST_HIP(hipEventRecord(b, ctx->stream));
       BST_HIP(hipEventSynchronize(b));
       float ms = 0.0f;
       BST_HIP(hipEventElapsedTime(&ms, a, b));
-      if (r > 0) best = std::min(best, ms);
+      best = std::min(best, ms);
     }
     return best;
   }
@@ -183,7 +200,9 @@ HistTuning choose_hist_tuning(bst_ctx* ctx, const DatasetDev& d, const bst_gbdt_
       fix_tile ? std::vector<uint32_t>{t.tile_min} : std::vector<uint32_t>{512, 1024, 2048};
   std::vector<uint32_t> bpcs =
       fix_bpc ? std::vector<uint32_t>{t.blocks_per_cu} : std::vector<uint32_t>{1, 2, 4};
-  const std::vector<Shape> shapes = tree_shapes(d.n, cfg, T);
+  // Unsampled single-tree rounds read the root's rows in place (no gather).
+  const bool sampled = !(std::fabs(cfg.subsample - 1.0f) < 1e-6f) || ext.goss_top_rate > 0.0f;
+  const std::vector<Shape> shapes = tree_shapes(d.n, cfg, T, T > 1 || sampled);
 
   // Work lists of every candidate first, to size the buffers once.
   struct Cand {
@@ -245,11 +264,27 @@ HistTuning choose_hist_tuning(bst_ctx* ctx, const DatasetDev& d, const bst_gbdt_
   BST_HIP(hipEventCreate(&B.a));
   BST_HIP(hipEventCreate(&B.b));
 
+  // Rounds over all candidates (the first one warms the clocks and caches),
+  // keeping each (candidate, step) pair's fastest time: interleaving the
+  // repeats spreads clock drift over every candidate alike.
+  constexpr int kRounds = 3;
+  std::vector<std::vector<float>> tmin(cands.size(), std::vector<float>(shapes.size(), 1e30f));
+  for (int r = 0; r < kRounds; r++)
+    for (size_t ci = 0; ci < cands.size(); ci++)
+      for (size_t i = 0; i < shapes.size(); i++) {
+        const float ms = B.run(cands[ci].plans[i], cands[ci].fg, 1);
+        if (r > 0) tmin[ci][i] = std::min(tmin[ci][i], ms);
+      }
+  const bool trace = std::getenv("BST_TRACE_TUNE") != nullptr;
   double best = 1e300;
   HistTuning pick = t;
-  for (const Cand& c : cands) {
+  for (size_t ci = 0; ci < cands.size(); ci++) {
+    const Cand& c = cands[ci];
     double total = 0;
-    for (size_t i = 0; i < shapes.size(); i++) total += shapes[i].weight * B.run(c.plans[i], c.fg, 2);
+    for (size_t i = 0; i < shapes.size(); i++) total += shapes[i].weight * tmin[ci][i];
+    if (trace)
+      std::fprintf(stderr, "[tune] fg %d rows_min %u blocks/cu %u: %.3f ms\n", c.fg, c.tile_min,
+                   c.bpc, total);
     if (total < best) {
       best = total;
       pick.fg = c.fg;
