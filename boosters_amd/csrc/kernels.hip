@@ -201,7 +201,8 @@ void launch_to_rowmajor(hipStream_t s, const DatasetDev& d) {
 // Objectives
 // ===========================================================================
 constexpr int kGradMaxK = 16;   // outputs handled by one gradient launch
-constexpr int kGradThreads = 256;
+constexpr int kGradThreads = 1024;
+constexpr int kRowBatch = 4;  // rows per thread per pass in the streaming row kernels
 
 __device__ __forceinline__ uint32_t abs_bits(float v) { return __float_as_uint(fabsf(v)); }
 
@@ -220,8 +221,42 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
     mx_g[k] = 0u;
     mx_h[k] = 0u;
   }
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i_start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (objective != 2 && K == 1) {
+    // One output: kRowBatch rows per thread per pass, loads issued together.
+    for (; i_start < n; i_start += kRowBatch * stride) {
+      float p0[kRowBatch], yv[kRowBatch], wv_[kRowBatch];
+#pragma unroll
+      for (int k = 0; k < kRowBatch; k++) {
+        const int64_t i = i_start + k * stride;
+        const bool in = i < n;
+        p0[k] = in ? preds[i] : 0.0f;
+        yv[k] = in ? y[i] : 0.0f;
+        wv_[k] = in && w ? w[i] : 1.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < kRowBatch; k++) {
+        const int64_t i = i_start + k * stride;
+        if (i < n) {
+          float gv, hv;
+          if (objective == 1) {  // logistic, classification.rs:24-52
+            const float p = 1.0f / (1.0f + exp32(-p0[k]));
+            gv = wv_[k] * (p - yv[k]);
+            hv = fmaxf(wv_[k] * p * (1.0f - p), HESS_MIN);
+          } else {  // squared, regression.rs:20-42
+            gv = wv_[k] * (p0[k] - yv[k]);
+            hv = wv_[k];
+          }
+          g[i] = gv;
+          h[i] = hv;
+          mx_g[0] = max(mx_g[0], abs_bits(gv));
+          mx_h[0] = max(mx_h[0], abs_bits(hv));
+        }
+      }
+    }
+  }
+  for (int64_t i = i_start; i < n; i += stride) {
     const float wi = w ? w[i] : 1.0f;
     const float yi = y[i];
     if (objective == 2) {  // softmax, classification.rs:141-187
@@ -297,8 +332,10 @@ void launch_gradients(hipStream_t s, int objective, const float* preds, const fl
                       const float* w, int64_t n, int K, float* g, float* h, uint32_t* absmax) {
   // Callers validate 1 <= K <= kGradMaxK (softmax allows at most 16 classes).
   if (n <= 0 || K < 1 || K > kGradMaxK) return;
+  // At most 256 blocks: each ends with one atomicMax per value on the same
+  // words, and same-address atomics serialise in L2.
   int64_t blocks = (n + kGradThreads - 1) / kGradThreads;
-  if (blocks > 1024) blocks = 1024;
+  if (blocks > 256) blocks = 256;
   hipLaunchKernelGGL(k_gradients, dim3((unsigned)blocks), dim3(kGradThreads), 0, s, objective,
                      preds, y, w, n, K, g, h, absmax);
 }
@@ -488,24 +525,40 @@ void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g,
 // Rows of every tree slot in order (slot t uses output out0 + t) and the
 // exact fixed-point root totals, added to node_sum[t * max_nodes] (zeroed
 // before).  Grid (row blocks, slots).
-__global__ __launch_bounds__(256) void k_init_rows_sums(uint32_t* __restrict__ idx,
+constexpr int kInitThreads = 1024;
+
+__global__ __launch_bounds__(kInitThreads) void k_init_rows_sums(uint32_t* __restrict__ idx,
                                                         float2* __restrict__ ogh,
                                                         const float* __restrict__ g,
                                                         const float* __restrict__ h,
                                                         int64_t n, const int32_t* __restrict__ qexp,
                                                         uint32_t max_nodes,
                                                         long long* __restrict__ node_sum) {
-  __shared__ long long s_red[256 / WAVE];
+  __shared__ long long s_red[kInitThreads / WAVE];
   const int t = blockIdx.y;
   const double sg = ldexp(1.0, qexp[2 * t]), sh = ldexp(1.0, qexp[2 * t + 1]);
   long long ag = 0, ah = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const float2 gh = make_float2(g[(int64_t)t * n + i], h[(int64_t)t * n + i]);
-    idx[(int64_t)t * n + i] = (uint32_t)i;
-    ogh[(int64_t)t * n + i] = gh;
-    ag += __double2ll_rn((double)gh.x * sg);
-    ah += __double2ll_rn((double)gh.y * sh);
+  // kRowBatch rows per thread per pass, their loads issued together.
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n;
+       i0 += kRowBatch * stride) {
+    float2 gh[kRowBatch];
+#pragma unroll
+    for (int k = 0; k < kRowBatch; k++) {
+      const int64_t i = i0 + k * stride;
+      gh[k] = i < n ? make_float2(g[(int64_t)t * n + i], h[(int64_t)t * n + i])
+                    : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int k = 0; k < kRowBatch; k++) {
+      const int64_t i = i0 + k * stride;
+      if (i < n) {
+        idx[(int64_t)t * n + i] = (uint32_t)i;
+        ogh[(int64_t)t * n + i] = gh[k];
+        ag += __double2ll_rn((double)gh[k].x * sg);
+        ah += __double2ll_rn((double)gh[k].y * sh);
+      }
+    }
   }
   ag = wave_sum_ll(ag);
   ah = wave_sum_ll(ah);
@@ -513,7 +566,7 @@ __global__ __launch_bounds__(256) void k_init_rows_sums(uint32_t* __restrict__ i
   __syncthreads();
   if (threadIdx.x == 0) {
     long long tg = 0;
-    for (int i = 0; i < 256 / WAVE; i++) tg += s_red[i];
+    for (int i = 0; i < kInitThreads / WAVE; i++) tg += s_red[i];
     atomicAdd((unsigned long long*)&node_sum[2 * (int64_t)t * max_nodes], (unsigned long long)tg);
   }
   __syncthreads();
@@ -521,7 +574,7 @@ __global__ __launch_bounds__(256) void k_init_rows_sums(uint32_t* __restrict__ i
   __syncthreads();
   if (threadIdx.x == 0) {
     long long th = 0;
-    for (int i = 0; i < 256 / WAVE; i++) th += s_red[i];
+    for (int i = 0; i < kInitThreads / WAVE; i++) th += s_red[i];
     atomicAdd((unsigned long long*)&node_sum[2 * (int64_t)t * max_nodes + 1], (unsigned long long)th);
   }
 }
@@ -530,10 +583,12 @@ void launch_init_rows_sums(hipStream_t s, uint32_t* idx, float2* ogh, const floa
                            const float* h, int64_t n, int T, const int32_t* qexp,
                            uint32_t max_nodes, long long* node_sum) {
   if (n <= 0 || T <= 0) return;
-  int64_t bx = (n + 255) / 256;
-  if (bx > 512) bx = 512;
-  hipLaunchKernelGGL(k_init_rows_sums, dim3((unsigned)bx, (unsigned)T), dim3(256), 0, s, idx, ogh,
-                     g, h, n, qexp, max_nodes, node_sum);
+  // At most 128 blocks per tree: each adds its totals with one atomic per
+  // value on the root's words (same-address atomics serialise in L2).
+  int64_t bx = (n + kInitThreads - 1) / kInitThreads;
+  if (bx > 128) bx = 128;
+  hipLaunchKernelGGL(k_init_rows_sums, dim3((unsigned)bx, (unsigned)T), dim3(kInitThreads), 0, s,
+                     idx, ogh, g, h, n, qexp, max_nodes, node_sum);
 }
 
 // ===========================================================================
@@ -1717,7 +1772,24 @@ __global__ __launch_bounds__(256) void k_leaf_update(const Segment* __restrict__
     const uint32_t tree = sg.tree & 0x7FFFFFFFu;
     const uint32_t* t = ((sg.tree >> 31) ? idx_alt : idx) + (int64_t)tree * tree_stride;
     float* pr = preds + (int64_t)tree_output[tree] * n_rows;
-    for (uint32_t p = threadIdx.x; p < sg.count; p += blockDim.x) pr[t[sg.begin + p]] += v;
+    // A segment's rows are distinct, so the read-modify-writes of a batch
+    // are independent: load kLeafBatch row ids, then their predictions,
+    // then store, keeping the batch's gathers in flight together.
+    constexpr int kLeafBatch = 8;
+    for (uint32_t p0 = 0; p0 < sg.count; p0 += blockDim.x * kLeafBatch) {
+      uint32_t r[kLeafBatch];
+      float x[kLeafBatch];
+#pragma unroll
+      for (int k = 0; k < kLeafBatch; k++) {
+        const uint32_t p = p0 + (uint32_t)k * blockDim.x + threadIdx.x;
+        r[k] = p < sg.count ? t[sg.begin + p] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int k = 0; k < kLeafBatch; k++) x[k] = r[k] != 0xFFFFFFFFu ? pr[r[k]] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < kLeafBatch; k++)
+        if (r[k] != 0xFFFFFFFFu) pr[r[k]] = x[k] + v;
+    }
   }
 }
 
@@ -1895,8 +1967,36 @@ __global__ __launch_bounds__(256) void k_metric(int metric, int transform,
   const int k = blockIdx.y;
   const double EPS = 1e-15;
   double a = 0.0, b = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i_start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (metric == 1 || metric == 2) {
+    // kRowBatch rows per thread per pass, loads issued together.
+    for (; i_start < n; i_start += kRowBatch * stride) {
+      float pv[kRowBatch], yv[kRowBatch], wv_[kRowBatch];
+#pragma unroll
+      for (int q = 0; q < kRowBatch; q++) {
+        const int64_t i = i_start + q * stride;
+        const bool in = i < n;
+        pv[q] = in ? preds[(metric == 1 ? (int64_t)k * n : 0) + i] : 0.0f;
+        yv[q] = in ? y[i] : 0.0f;
+        wv_[q] = in ? (w ? w[i] : 1.0f) : 0.0f;
+      }
+#pragma unroll
+      for (int q = 0; q < kRowBatch; q++) {
+        if (i_start + q * stride >= n) continue;
+        if (metric == 1) {
+          const double d = (double)pv[q] - (double)yv[q];
+          a += (double)wv_[q] * d * d;
+        } else {
+          const float p = transform == 1 ? sigmoid32(pv[q]) : pv[q];
+          const double qq = fmin(fmax((double)p, EPS), 1.0 - EPS), l = (double)yv[q];
+          a += (double)wv_[q] * (-(l * log(qq) + (1.0 - l) * log(1.0 - qq)));
+        }
+        b += (double)wv_[q];
+      }
+    }
+  }
+  for (int64_t i = i_start; i < n; i += stride) {
     const float wi = w ? w[i] : 1.0f;
     if (metric == 1) {
       const double d = (double)preds[(int64_t)k * n + i] - (double)y[i];

@@ -503,20 +503,25 @@ void Trainer::grow_round(std::vector<TreeState>& trees) {
       if (ts.tb.is_leaf[i]) ts.tb.leaf[i] = ts.tb.leaf[i] * lr_;
 }
 
-double Trainer::eval_metric(const float* preds, const float* y, int64_t n) {
+int Trainer::metric_kind() const {
   const auto& cfg = opt_.cfg;
-  const int metric = cfg.metric == BST_METRIC_DEFAULT ? default_metric(cfg.objective) : cfg.metric;
-  if (metric == BST_METRIC_NONE) return std::numeric_limits<double>::quiet_NaN();
+  return cfg.metric == BST_METRIC_DEFAULT ? default_metric(cfg.objective) : cfg.metric;
+}
+
+void Trainer::metric_launch(const float* preds, const float* y, int64_t n, double* host_part) {
+  const auto& cfg = opt_.cfg;
   const int transform = cfg.objective == BST_OBJ_LOGISTIC ? BST_TRANSFORM_SIGMOID
                         : cfg.objective == BST_OBJ_SOFTMAX ? BST_TRANSFORM_SOFTMAX
                                                            : BST_TRANSFORM_IDENTITY;
-  const int nb = 512;
+  const int nb = kMetricBlocks;
   d_metric_.ensure((size_t)2 * nb * opt_.K);
-  launch_metric(s_, metric, transform, preds, opt_.K, y, nullptr, n, d_metric_.p, nb);
-  std::vector<double> part((size_t)2 * nb * opt_.K);
-  d2h(s_, part.data(), d_metric_.p, part.size());
-  BST_HIP(hipStreamSynchronize(s_));
-  if (metric == BST_METRIC_RMSE) {
+  launch_metric(s_, metric_kind(), transform, preds, opt_.K, y, nullptr, n, d_metric_.p, nb);
+  d2h(s_, host_part, d_metric_.p, (size_t)2 * nb * opt_.K);
+}
+
+double Trainer::metric_finish(const double* part) const {
+  const int nb = kMetricBlocks;
+  if (metric_kind() == BST_METRIC_RMSE) {
     double acc = 0;
     for (int k = 0; k < opt_.K; k++) {
       double ss = 0, sw = 0;
@@ -534,6 +539,14 @@ double Trainer::eval_metric(const float* preds, const float* y, int64_t n) {
     sw += part[(size_t)b * 2 + 1];
   }
   return sw > 0 ? sl / sw : 0.0;
+}
+
+double Trainer::eval_metric(const float* preds, const float* y, int64_t n) {
+  if (metric_kind() == BST_METRIC_NONE) return std::numeric_limits<double>::quiet_NaN();
+  std::vector<double> part((size_t)2 * kMetricBlocks * opt_.K);
+  metric_launch(preds, y, n, part.data());
+  BST_HIP(hipStreamSynchronize(s_));
+  return metric_finish(part.data());
 }
 
 bst_model* Trainer::train(const float* y, const float* w, const float* vX, const float* vy,
@@ -706,10 +719,25 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   double best_val = 0;
   size_t best_round = 0, cur_round = 0, best_n_trees = 0;
   std::vector<uint32_t> sampled;
+  // Device rounds are pipelined: round r+1 is queued before the host reads
+  // round r, so the host's tree assembly overlaps the device (without a
+  // validation set, whose updates the host issues per round).  The training
+  // metric then runs in the queue right after its round.
+  // Profiled runs read each round's kernel events right after it, unpipelined.
+  const bool pipelined = on_device && !(vX && n_valid > 0) && !ctx_->profile;
+  const bool queue_metric = pipelined && metric_kind() != BST_METRIC_NONE;
+  if (pipelined && cfg.n_trees > 0) device_launch(y, w, queue_metric);
   for (uint32_t round = 0; round < cfg.n_trees; round++) {
+    double queued_value = std::numeric_limits<double>::quiet_NaN();
     if (on_device) {
       std::vector<TreeBuild> tbs;
-      device_round(y, w, tbs);
+      if (pipelined) {
+        if (round + 1 < cfg.n_trees) device_launch(y, w, queue_metric);
+        device_collect(tbs, &queued_value);
+      } else {
+        device_launch(y, w, false);
+        device_collect(tbs, nullptr);
+      }
       for (int t = 0; t < T_; t++) {
         bst_model one = single_tree(tbs[t], t);
         if (vX && n_valid > 0) {
@@ -805,6 +833,8 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
     double value;
     if (vX && vy && n_valid > 0)
       value = eval_metric(vpreds_.p, vy, n_valid);
+    else if (pipelined)
+      value = queued_value;
     else
       value = eval_metric(preds_.p, y, n_);
     metric_history.push_back(value);
@@ -820,6 +850,7 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
       if (cur_round - best_round > cfg.early_stopping_rounds) break;
     }
   }
+  if (pipelined) BST_HIP(hipStreamSynchronize(s_));  // a round queued past an early stop
   if (cfg.early_stopping_rounds > 0 && best_n_trees > 0 && best_n_trees < m->tree_group.size()) {
     const int64_t keep = m->tree_offset[best_n_trees];
     m->tree_offset.resize(best_n_trees + 1);

@@ -89,7 +89,19 @@ class Trainer {
   bool device_schedule_ok() const;
   void device_setup();
   void device_enqueue_round(const float* y, const float* w, bool sampled);
-  void device_round(const float* y, const float* w, std::vector<TreeBuild>& out);
+  // Queue one boosting round (graph replay, then the training metric's
+  // partial sums if `metric`, then the trees' copy to a host buffer of its
+  // own); rounds queued back to back run while the host reads earlier ones.
+  void device_launch(const float* y, const float* w, bool metric);
+  // Wait for the oldest queued round and read its trees (and metric partials).
+  void device_collect(std::vector<TreeBuild>& out, double* metric_value);
+  // Training metric (eval.rs:218-260): partial sums on the device into
+  // `host_part` (2 doubles per block and output, nb blocks), and the value
+  // from them.
+  int metric_kind() const;
+  void metric_launch(const float* preds, const float* y, int64_t n, double* host_part);
+  double metric_finish(const double* part) const;
+  static constexpr int kMetricBlocks = 1024;
   // Algorithmic bytes: each histogrammed row reads its E bins (1 B, or
   // 1.125 B with the NaN hi plane) plus its gradient pair (8 B) and row id
   // (4 B when gathered); each built node writes E x 256 (g, h) int64 pairs.

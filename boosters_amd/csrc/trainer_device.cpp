@@ -23,7 +23,7 @@ namespace bst {
 namespace {
 constexpr uint32_t kHistTileD = 8192;
 constexpr uint32_t kPartTileD = kFusedTileRows;
-constexpr uint32_t kSegTileD = 16384;
+constexpr uint32_t kSegTileD = 4096;
 constexpr int kMaxGrid = 2048;
 
 int grid_for(uint32_t cap, int max_grid) {
@@ -61,8 +61,13 @@ struct Trainer::DevState {
   DevBuf<uint32_t> round;  // boosting round, read by the sampling kernels
   DevBuf<SampleState> samp;
   DevBuf<uint32_t> bcount, feats_dev;
-  char* h_out = nullptr;      // pinned mirror of `out`
-  TreeOut H{};
+  // Two pinned mirrors of `out` (and of the metric partials): round r reads
+  // slot r % 2 while round r + 1 may be in flight.
+  char* h_out[2] = {nullptr, nullptr};
+  TreeOut H[2]{};
+  double* h_metric[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  uint64_t launched = 0, collected = 0;  // rounds queued / read
   // [0]: rounds without row sampling, [1]: rounds with it.
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -71,7 +76,11 @@ struct Trainer::DevState {
       if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
     }
-    if (h_out) (void)hipHostFree(h_out);
+    for (int i = 0; i < 2; i++) {
+      if (h_out[i]) (void)hipHostFree(h_out[i]);
+      if (h_metric[i]) (void)hipHostFree(h_metric[i]);
+      if (done[i]) (void)hipEventDestroy(done[i]);
+    }
   }
 };
 
@@ -185,8 +194,13 @@ void Trainer::device_setup() {
   s.out_bytes = tree_out_layout(T, P.MNT, nullptr, nullptr);
   s.out.alloc(s.out_bytes);
   tree_out_layout(T, P.MNT, s.out.p, &s.O);
-  BST_HIP(hipHostMalloc((void**)&s.h_out, s.out_bytes, hipHostMallocDefault));
-  tree_out_layout(T, P.MNT, s.h_out, &s.H);
+  for (int i = 0; i < 2; i++) {
+    BST_HIP(hipHostMalloc((void**)&s.h_out[i], s.out_bytes, hipHostMallocDefault));
+    tree_out_layout(T, P.MNT, s.h_out[i], &s.H[i]);
+    BST_HIP(hipHostMalloc((void**)&s.h_metric[i],
+                          sizeof(double) * 2 * kMetricBlocks * opt_.K, hipHostMallocDefault));
+    BST_HIP(hipEventCreateWithFlags(&s.done[i], hipEventDisableTiming));
+  }
   s.items.alloc(s.cap_items);
   s.tiles.alloc(s.cap_tiles);
   s.pend.alloc((size_t)5 * s.cap_items);
@@ -335,11 +349,10 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
                        tree_output_.p, n_, cnt + C_LSEGS, idx_tmp_.p);
   }
   launch_round_advance(s_, S.round.p);
-  d2h(s_, S.h_out, S.out.p, S.out_bytes);
 }
 
 // Launch (or capture and launch) one round, wait, and return the trees.
-void Trainer::device_round(const float* y, const float* w, std::vector<TreeBuild>& out) {
+void Trainer::device_launch(const float* y, const float* w, bool metric) {
   using clk = std::chrono::steady_clock;
   static const bool trace = std::getenv("BST_TRACE_SETUP") != nullptr;
   const auto t0 = clk::now();
@@ -370,9 +383,22 @@ void Trainer::device_round(const float* y, const float* w, std::vector<TreeBuild
   } else {
     device_enqueue_round(y, w, sampled);
   }
-  BST_HIP(hipStreamSynchronize(s_));
+  const int slot = (int)(S.launched & 1u);
+  if (metric) metric_launch(preds_.p, y, n_, S.h_metric[slot]);
+  d2h(s_, S.h_out[slot], S.out.p, S.out_bytes);
+  BST_HIP(hipEventRecord(S.done[slot], s_));
+  S.launched++;
+}
+
+void Trainer::device_collect(std::vector<TreeBuild>& out, double* metric_value) {
+  DevState& S = *dev_;
+  if (S.collected >= S.launched) raise(BST_E_INVALID, "no device round in flight");
+  const int slot = (int)(S.collected & 1u);
+  BST_HIP(hipEventSynchronize(S.done[slot]));
   BST_HIP(hipGetLastError());
-  const TreeOut& H = S.H;
+  S.collected++;
+  const TreeOut& H = S.H[slot];
+  if (metric_value) *metric_value = metric_finish(S.h_metric[slot]);
   if (ctx_->profile) {
     // Algorithmic bytes of the round's histogram launches, spread evenly.
     const double rows = (double)H.counts[C_HROWS];
