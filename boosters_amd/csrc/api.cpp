@@ -73,7 +73,17 @@ bst_dataset* make_dataset(bst_ctx* ctx, const float* X, int64_t n, int32_t F,
   d.F = F;
   d.E = E;
   d.hi_off = hi ? E : -1;
-  d.S = ((E + (hi ? (E + 7) / 8 : 0)) + 15) / 16 * 16;
+  {
+    // Row: E bin bytes, then the NaN bits (one per feature) when any feature
+    // has 257 bins, padded to 16 bytes and so that the bytes of the padding
+    // features of the last 32-feature group (bins and NaN bits, all zero)
+    // lie inside the row: histogram lanes of padding features read them
+    // instead of testing their feature index.
+    const int32_t ng32 = (E + 31) / 32;
+    const int32_t used =
+        std::max(E + (hi ? (E + 7) / 8 : 0), std::max(32 * ng32, hi ? E + 4 * ng32 : 0));
+    d.S = (used + 15) / 16 * 16;
+  }
   d.FG = 32;
   d.NG = (E + d.FG - 1) / d.FG;
   d.hi_words = (n + 31) / 32;

@@ -135,6 +135,12 @@ void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int 
 // launch_hist_finish sums them (and applies the subtraction trick).
 // Histogram blocks one CU holds at once for a feature-group width (LDS).
 inline int hist_blocks_resident(int FG) { return FG == 16 ? 2 : 1; }
+// Histogram lanes of padding features (index E .. FG * NG - 1) read their
+// own bin byte and NaN bit: the row must hold them (as zeros).
+inline bool hist_layout_padded(const DatasetDev& d, int FG) {
+  const int ng = (d.E + FG - 1) / FG;
+  return d.E > 0 && FG * ng <= d.S && (d.hi_off < 0 || d.hi_off + FG / 8 * ng <= d.S);
+}
 
 void launch_hist_finish(hipStream_t s, const uint32_t* fin, int n, int NG, int FG,
                         const long long* partial, long long* hist, int64_t slot_elems,

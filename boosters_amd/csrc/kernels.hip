@@ -19,6 +19,9 @@
 #include <float.h>
 #include <math.h>
 
+#include <stdexcept>
+#include <type_traits>
+
 namespace bst {
 
 constexpr int WAVE = 64;
@@ -597,13 +600,17 @@ void launch_init_rows_sums(hipStream_t s, uint32_t* idx, float2* ogh, const floa
 // One block-iteration = one HistItem: a tile of a node's rows x one group of
 // FG features.  The tile is walked in chunks of hist_chunk<FG>() rows.  Staging:
 // one row per thread loads the row id and the row's (g, h), quantises the
-// pair to fixed point once (not once per feature lane) and stores both in
-// LDS; the next chunk's loads are issued before the current chunk is
-// histogrammed, so their latency overlaps that work (two staging buffers).
-// Histogramming: lane (threadIdx % FG) owns one feature and threadIdx / FG
-// picks a run of RPT consecutive staged rows, so the FG lanes of a row read
-// FG consecutive bytes of that row; each thread issues the bin (and hi-bit)
-// loads of all its RPT rows before consuming any.  The LDS sub-histogram is
+// pair to fixed point once (not once per feature lane) and stores the row's
+// byte offset and the pair in LDS, two chunks ahead of the adds (three
+// staging buffers).  Histogramming: lane (threadIdx % FG) owns one feature
+// and threadIdx / FG picks a run of RPT consecutive staged rows, so the FG
+// lanes of a row read FG consecutive bytes of that row.  The bin (and
+// hi-bit) loads of chunk c+1 are issued before chunk c is added, into a
+// second register set, so their latency overlaps the adds.  The add loop
+// is about five VALU instructions per row and lane: no bounds tests (rows
+// past the tile's end are staged as row 0 with (g, h) = 0, padding
+// features read zero bytes of the padded row) and no select for the NaN
+// bin (bin | hi << 8 is 256 for it).  The LDS sub-histogram is
 // laid out [bin][FG]: lanes of one row always hit FG distinct 8-byte words,
 // so ds_add_u64 is bank-conflict free whatever the bins are, and hot bins
 // never collide inside a row.  Global layout per slot: [group][bin][FG] of
@@ -671,23 +678,28 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
       s_g[i] = 0;
       s_h[i] = 0;
     }
+    // Padding lanes (f >= E) read their own bytes: the row stride covers
+    // every group's FG bin bytes and FG / 8 NaN-bit bytes, which are zero
+    // there, so they add into columns nobody reads.
     const int f = (int)it.group * FG + lf;
-    const bool fvalid = f < E;
-    const int fs = fvalid ? f : 0;  // loads stay in the row for padding lanes
-    const int hb = hi_off + (fs >> 3), hs = fs & 7;
+    const int hb = hi_off + (f >> 3), hs = f & 7;
     const double sg = ldexp(1.0, qexp[2 * tree]);
     const double sh = ldexp(1.0, qexp[2 * tree + 1]);
     const uint32_t* tidx = (alt ? idx_alt : idx) + (int64_t)tree * tree_stride;
     const float2* togh = (alt ? ogh_alt : ogh) + (int64_t)tree * tree_stride;
     const uint32_t n_chunks = (it.count + kHistChunk - 1) / kHistChunk;
     long long accg = 0, acch = 0;
-    // Staging registers: this thread's row of the next chunk.
+    // Staging registers: this thread's row of the next chunk.  A staged row
+    // is its byte offset in bins_rm (its id when offsets need 64 bits) and
+    // its quantised (g, h); rows past the tile's end stage row 0 with (0, 0),
+    // so the adds need no bounds checks (they add zeros).
     uint32_t st_r = 0;
     float2 st_gh = make_float2(0.0f, 0.0f);
     bool st_ok = false;
+    const bool stager = threadIdx.x < kHistChunk;
     auto stage_load = [&](uint32_t c) {
       const uint32_t q = c * kHistChunk + threadIdx.x;
-      st_ok = threadIdx.x < kHistChunk && q < it.count;
+      st_ok = stager && q < it.count;
       if (st_ok) {
         const uint32_t pos = it.begin + q;
         st_r = GATHER ? tidx[pos] : pos;
@@ -695,32 +707,31 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
       }
     };
     auto stage_store = [&](int buf) {
-      if (st_ok) {
-        const long long gq = __double2ll_rn((double)st_gh.x * sg);
-        const long long hq = __double2ll_rn((double)st_gh.y * sh);
-        s_row[buf][threadIdx.x] = st_r;
+      if (stager) {
+        const long long gq = st_ok ? __double2ll_rn((double)st_gh.x * sg) : 0ll;
+        const long long hq = st_ok ? __double2ll_rn((double)st_gh.y * sh) : 0ll;
+        s_row[buf][threadIdx.x] = st_ok ? (OFF64 ? st_r : st_r * (uint32_t)S) : 0u;
         s_q[buf][threadIdx.x] = make_longlong2(gq, hq);
         accg += gq;
         acch += hq;
       }
     };
-    // Bin loads of chunk c (row ids from the staging buffer).  Only called for
-    // c < n_chunks, whose row 0 is staged, so clamped lanes read a valid id.
-    auto load_bins = [&](uint32_t c, uint32_t* bv, uint32_t* hv) {
+    // Bin loads of chunk c from the staged row offsets.  The loaded bytes stay
+    // uint8_t until they are used: widened where they are loaded, the
+    // compiler would widen them in another block and wait for the loads
+    // there.
+    auto load_bins = [&](uint32_t c, uint8_t* bv, uint8_t* hv) {
       const int buf = (int)(c % kHistBufs);
-      const uint32_t nrow = min((uint32_t)kHistChunk, it.count - c * kHistChunk);
 #pragma unroll
       for (int u = 0; u < RPT; u++) {
-        const uint32_t j = (uint32_t)(rs * RPT + u);
-        const uint32_t r = s_row[buf][j < nrow ? j : 0u];
+        const uint32_t r = s_row[buf][rs * RPT + u];
         if (OFF64) {
           const uint8_t* rp = bins_rm + (uint64_t)r * (uint64_t)S;
-          bv[u] = rp[fs];
+          bv[u] = rp[f];
           hv[u] = HAS_HI ? rp[hb] : 0u;
         } else {
-          const uint32_t off = r * (uint32_t)S;
-          bv[u] = bins_rm[off + (uint32_t)fs];
-          hv[u] = HAS_HI ? bins_rm[off + (uint32_t)hb] : 0u;
+          bv[u] = bins_rm[r + (uint32_t)f];
+          hv[u] = HAS_HI ? bins_rm[r + (uint32_t)hb] : 0u;
         }
       }
     };
@@ -736,47 +747,55 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
       stage_store(1);
     }
     __syncthreads();  // staging visible; also orders the LDS zeroing
-    uint32_t bv[RPT], hv[RPT];
+    // Bins of even chunks live in (av, ah), of odd chunks in (bv, bh): the
+    // loop below is unrolled by two so that no loaded register is copied
+    // (a copy would wait for the next chunk's loads).
+    uint8_t av[RPT], ah[RPT], bv[RPT], bh[RPT];
 #pragma unroll
-    for (int u = 0; u < RPT; u++) bv[u] = hv[u] = 0u;
-    if (n_chunks > 0) load_bins(0, bv, hv);
-    for (uint32_t c = 0; c < n_chunks; c++) {
+    for (int u = 0; u < RPT; u++) av[u] = ah[u] = bv[u] = bh[u] = 0;
+    if (n_chunks > 0) load_bins(0, av, ah);
+    // One chunk: issue the next chunk's bin loads (PF) into (nb, nh), add
+    // this one from (cb, ch).  The last chunk is peeled (PF false) so that
+    // inside the loop every path issues the same loads: the compiler then
+    // waits for exactly this chunk's bins, never for the next chunk's.
+    auto chunk = [&](uint32_t c, auto pf, const uint8_t (&cb)[RPT], const uint8_t (&ch)[RPT],
+                     uint8_t (&nb)[RPT], uint8_t (&nh)[RPT]) {
+      constexpr bool PF = decltype(pf)::value;
       const int cur = (int)(c % kHistBufs);
-      const uint32_t nrow = min((uint32_t)kHistChunk, it.count - c * kHistChunk);
       if (c + 2 < n_chunks) stage_load(c + 2);
-      uint32_t bn[RPT], hn[RPT];
-#pragma unroll
-      for (int u = 0; u < RPT; u++) bn[u] = hn[u] = 0u;
-      if (c + 1 < n_chunks) load_bins(c + 1, bn, hn);
+      if constexpr (PF) load_bins(c + 1, nb, nh);
       // Four rows at a time: their (g, h) pairs are read from LDS together,
-      // then added without branches.  The NaN bin of a 257-bin feature (hi
-      // bit set) is not accumulated: it is the node total minus the 256 real
-      // bins and goes to row 256 with the other adds that count nothing.
+      // then added without branches.  The NaN bin of a 257-bin feature is
+      // stored as byte 0 with its hi bit set, so bin | hi << 8 sends it to
+      // row 256: it is not accumulated (it is the node total minus the 256
+      // real bins).
 #pragma unroll
       for (int u0 = 0; u0 < RPT; u0 += kHistRowBatch) {
         longlong2 q[kHistRowBatch];
 #pragma unroll
-        for (int k = 0; k < kHistRowBatch; k++) {
-          const uint32_t j = (uint32_t)(rs * RPT + u0 + k);
-          q[k] = s_q[cur][j < nrow ? j : 0u];
-        }
+        for (int k = 0; k < kHistRowBatch; k++) q[k] = s_q[cur][rs * RPT + u0 + k];
 #pragma unroll
         for (int k = 0; k < kHistRowBatch; k++) {
           const int u = u0 + k;
-          const uint32_t j = (uint32_t)(rs * RPT + u);
-          const bool nan_bin = HAS_HI && ((hv[u] >> hs) & 1u);
-          const uint32_t bin = (j < nrow && fvalid && !nan_bin) ? bv[u] : 256u;
+          const uint32_t bin =
+              HAS_HI ? ((uint32_t)cb[u] | ((((uint32_t)ch[u] >> hs) & 1u) << 8)) : (uint32_t)cb[u];
           atomicAdd((unsigned long long*)&s_g[bin * FG + lf], (unsigned long long)q[k].x);
           atomicAdd((unsigned long long*)&s_h[bin * FG + lf], (unsigned long long)q[k].y);
         }
       }
       if (c + 2 < n_chunks) stage_store((int)((c + 2) % kHistBufs));
       __syncthreads();  // chunk c+2 staged; chunk c's buffer may be rewritten
-#pragma unroll
-      for (int u = 0; u < RPT; u++) {
-        bv[u] = bn[u];
-        hv[u] = hn[u];
-      }
+    };
+    uint32_t c = 0;
+    for (; c + 2 < n_chunks; c += 2) {
+      chunk(c, std::true_type{}, av, ah, bv, bh);
+      chunk(c + 1, std::true_type{}, bv, bh, av, ah);
+    }
+    if (c + 1 < n_chunks) {
+      chunk(c, std::true_type{}, av, ah, bv, bh);
+      chunk(c + 1, std::false_type{}, bv, bh, av, ah);
+    } else if (c < n_chunks) {
+      chunk(c, std::false_type{}, av, ah, bv, bh);
     }
     const bool do_sum = node_sum != nullptr && it.group == 0;
     if (do_sum) {
@@ -948,6 +967,8 @@ void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int 
                  int /*rows_per_block_hint*/, const uint32_t* dyn, long long* partial,
                  const uint32_t* idx_alt, const float2* ogh_alt) {
   if (n_items <= 0) return;
+  if (!hist_layout_padded(d, d.FG))
+    throw std::runtime_error("histogram rows must hold every group's padding features");
   if (d.FG == 16)
     launch_hist_fg<16>(s, d, items, n_items, idx, ogh, tree_stride, qexp, hist, slot_elems,
                        node_sum, partial, gather, dyn, idx_alt, ogh_alt);
