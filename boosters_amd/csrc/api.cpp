@@ -64,6 +64,7 @@ bst_dataset* make_dataset(bst_ctx* ctx, const float* X, int64_t n, int32_t F,
     fm.has_missing = m.missing != 0;
     ds->fmeta.push_back(fm);
     hi = hi || m.n_bins > 256;
+    if (m.categorical) ds->d.max_cat_bins = std::max<int32_t>(ds->d.max_cat_bins, (int32_t)m.n_bins);
   }
   const int32_t E = (int32_t)ds->eff_to_orig.size();
   need(E > 0, BST_E_SHAPE, "all features are trivial");

@@ -95,6 +95,7 @@ struct DatasetDev {
   int32_t hi_off = -1;  // byte offset of the hi-bit mask in a row, -1 = none
   int32_t FG = 32;      // features per histogram group (lanes of one row)
   int32_t NG = 0;       // number of groups
+  int32_t max_cat_bins = 0;  // most categories of a categorical feature (0: none)
   int64_t hi_words = 0; // u32 words per feature in hi_cm
   uint8_t* bins_rm = nullptr;   // [n][S]
   uint8_t* bins_cm = nullptr;   // [E][n]
@@ -171,7 +172,8 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
                   unsigned long long* node_key, DevSplit* cand, DevSplit* node_split,
                   const uint32_t* dyn = nullptr, bool finalize = true, int feat_stride = 0);
 // feats: n_feats effective columns, shared by all trees (feat_stride 0) or
-// one list per tree slot at feats + slot * feat_stride.
+// one list per tree slot at feats + slot * feat_stride; null = columns
+// 0 .. n_feats-1.
 
 // Stable partition of every listed node (partition.rs:73-145): count, scan,
 // scatter, copy back, then children counts/sums.

@@ -1135,7 +1135,7 @@ __device__ __forceinline__ unsigned long long wave_best(unsigned long long local
 // the wave's 256-entry scratch for the sorted categorical partition.  Lane 0
 // writes the feature's candidate; every lane returns the feature's packed
 // node key (gain, then the tie rule's feature order), 0 if nothing is valid.
-template <class BinLoad>
+template <bool SORTED, class BinLoad>
 __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, uint32_t node,
                                             uint32_t f, long long G, long long H, uint32_t cnt,
                                             double ig, double ih, const SplitParams& p,
@@ -1182,18 +1182,19 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
       // (last) bin stays left.
       const uint32_t max_excl = fm.has_missing ? nb - 1 : nb;
       const int me = (int)max_excl - 1;
-      // Totals up to bin me (inclusive) from the lane holding it.
-      long long Pme = 0, Qme = 0;
-      if (me >= 0) {
-        const int src = me >> 2, j = me & 3;
-        long long tg = P[0], th = Q[0];
-        if (j == 1) { tg = P[1]; th = Q[1]; }
-        if (j == 2) { tg = P[2]; th = Q[2]; }
-        if (j == 3) { tg = P[3]; th = Q[3]; }
-        Pme = (long long)__shfl((int)(tg & 0xFFFFFFFF), src, WAVE) & 0xFFFFFFFFLL;
-        Pme |= (long long)((unsigned long long)(unsigned)__shfl((int)(tg >> 32), src, WAVE) << 32);
-        Qme = (long long)__shfl((int)(th & 0xFFFFFFFF), src, WAVE) & 0xFFFFFFFFLL;
-        Qme |= (long long)((unsigned long long)(unsigned)__shfl((int)(th >> 32), src, WAVE) << 32);
+      // Totals over bins 0..me: every loaded bin (bins past nb, and bin 256,
+      // are zero: the NaN bin of a 257-bin feature is never accumulated)
+      // minus the missing bin when it is one of bins 0..255.  (Selecting
+      // P[me & 3] from the lane holding bin me would index registers at run
+      // time, which the compiler does in scratch.)
+      long long Pme = (long long)((unsigned long long)(unsigned)__shfl((int)(ig_incl >> 32), WAVE - 1, WAVE) << 32) |
+                      (long long)(unsigned)__shfl((int)(ig_incl & 0xFFFFFFFF), WAVE - 1, WAVE);
+      long long Qme = (long long)((unsigned long long)(unsigned)__shfl((int)(ih_incl >> 32), WAVE - 1, WAVE) << 32) |
+                      (long long)(unsigned)__shfl((int)(ih_incl & 0xFFFFFFFF), WAVE - 1, WAVE);
+      if (fm.has_missing && nb - 1 < 256u) {
+        const longlong2 mv = load(nb - 1);
+        Pme -= mv.x;
+        Qme -= mv.y;
       }
       // Exclusive prefix (bins < b) for the backward candidates.
 #pragma unroll
@@ -1273,11 +1274,16 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
         const uint32_t k = 0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFu);
         out.gain = unord_f32((uint32_t)(best >> 32));
         out.flags = 2;
-        out.cats[k >> 5] = 1u << (k & 31);
+        // Word index known at run time: written word by word so that `out`
+        // stays in registers.
+#pragma unroll
+        for (int w = 0; w < kMaxCatWords; w++) out.cats[w] = (uint32_t)w == (k >> 5) ? 1u << (k & 31) : 0u;
       }
-    } else {
+    } else if constexpr (SORTED) {
       // Sorted ratio partition (find.rs:511-573): categories with hess > 1e-10
-      // sorted stably by -G/(H+1e-6), then a prefix scan in that order.
+      // sorted stably by -G/(H+1e-6), then a prefix scan in that order.  (A
+      // launch without SORTED has no feature with more than max_onehot_cats
+      // categories.)
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const uint32_t k = 4 * lane + j;
@@ -1352,7 +1358,9 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
           out.flags = 2;
           for (uint32_t q = 0; q <= qb; q++) {
             const uint32_t k = order[q];
-            out.cats[k >> 5] |= 1u << (k & 31);
+#pragma unroll
+            for (int w = 0; w < kMaxCatWords; w++)
+              out.cats[w] |= (uint32_t)w == (k >> 5) ? 1u << (k & 31) : 0u;
           }
         }
       }
@@ -1369,7 +1377,13 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
 // the [group][bin][FG] histogram layout the features' pairs of one bin are
 // adjacent, so consecutive threads read consecutive 16-byte pairs and each
 // 128-byte line is fetched once per block instead of once per wave.  Then
-// wave w evaluates feature w from LDS.
+// wave w evaluates feature w from LDS.  Each wave's own inputs (feature
+// meta, node totals and count, scales) are loaded before the staging so
+// that their latency overlaps it.  SORTED: some categorical feature has more
+// categories than max_onehot_cats, so waves need the 256-entry ratio / order
+// scratch of the sorted partition; without it the block's LDS is the staged
+// bins only and more blocks fit a CU.
+template <bool SORTED>
 __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     const uint32_t* __restrict__ nodes, const uint32_t* __restrict__ slots,
     const uint32_t* __restrict__ feats, int n_feats, const FeatMeta* __restrict__ fmeta, int FG,
@@ -1378,8 +1392,9 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     const int32_t* __restrict__ qexp, uint32_t max_nodes, SplitParams p,
     unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand,
     const uint32_t* __restrict__ dyn, int feat_stride) {
-  __shared__ double s_ratio[kSplitWaves][256];
-  __shared__ uint16_t s_order[kSplitWaves][256];
+  constexpr int kScratch = SORTED ? 256 : 1;
+  __shared__ double s_ratio[kSplitWaves][kScratch];
+  __shared__ uint16_t s_order[kSplitWaves][kScratch];
   __shared__ longlong2 s_bins[kSplitWaves][256];
   __shared__ unsigned long long s_key[kSplitWaves];
   constexpr int kThreads = kSplitWaves * 64;
@@ -1391,33 +1406,36 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     const uint32_t slot = slots[wy];
     const uint32_t tree = node / max_nodes;
     const long long* base = hist + (int64_t)slot * slot_elems;
-    const uint32_t* fl = feats + (int64_t)tree * feat_stride;
+    const uint32_t* fl = feats ? feats + (int64_t)tree * feat_stride : nullptr;
+    // This wave's feature and inputs.
+    const bool fok = fi < n_feats;
+    const uint32_t f = fok ? (fl ? fl[fi] : (uint32_t)fi) : 0u;
+    const FeatMeta fm = fmeta[f];
+    const int32_t qg = qexp[2 * tree], qh = qexp[2 * tree + 1];
+    const long long G = node_sum[2 * node], H = node_sum[2 * node + 1];
+    const uint32_t cnt = node_count[node];
     // Stage: thread t loads bin t / kSplitWaves + pass * (kThreads / kSplitWaves)
     // of the block's feature t % kSplitWaves.
     {
       const int w = threadIdx.x % kSplitWaves;
       const int fw = blockIdx.x * kSplitWaves + w;
       if (fw < n_feats) {
-        const uint32_t f = fl[fw];
-        const int grp = (int)f / FG, lf = (int)f % FG;
+        const uint32_t fs = fl ? fl[fw] : (uint32_t)fw;
+        const int grp = (int)fs / FG, lf = (int)fs % FG;
         for (int b = threadIdx.x / kSplitWaves; b < 256; b += kThreads / kSplitWaves) {
           const int64_t e = ((int64_t)(grp * 256 + b) * FG + lf) * 2;
-          s_bins[w][b] = make_longlong2(base[e], base[e + 1]);
+          s_bins[w][b] = *(const longlong2*)(base + e);
         }
       }
     }
     __syncthreads();
     unsigned long long key = 0ull;
-    if (fi < n_feats) {
-      const uint32_t f = fl[fi];
-      const double ig = ldexp(1.0, -qexp[2 * tree]), ih = ldexp(1.0, -qexp[2 * tree + 1]);
-      const long long G = node_sum[2 * node], H = node_sum[2 * node + 1];
-      const uint32_t cnt = node_count[node];
+    if (fok) {
+      const double ig = ldexp(1.0, -qg), ih = ldexp(1.0, -qh);
       const GainCtx ctx((double)G * ig, (double)H * ih, p);
-      const FeatMeta fm = fmeta[f];
       auto load = [&](uint32_t b) { return s_bins[wv][b]; };
-      key = split_feature(load, fm, node, f, G, H, cnt, ig, ih, p, ctx, s_ratio[wv],
-                          s_order[wv], E, cand);
+      key = split_feature<SORTED>(load, fm, node, f, G, H, cnt, ig, ih, p, ctx, s_ratio[wv],
+                                  s_order[wv], E, cand);
     }
     if ((threadIdx.x & (WAVE - 1)) == 0) s_key[wv] = key;
     __syncthreads();  // also: s_bins is restaged for the block's next node
@@ -1467,9 +1485,14 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
                   const uint32_t* dyn, bool finalize, int feat_stride) {
   if (n_nodes <= 0) return;
   dim3 grid((unsigned)((n_feats + kSplitWaves - 1) / kSplitWaves), (unsigned)n_nodes);
-  hipLaunchKernelGGL(k_split, grid, dim3(kSplitWaves * WAVE), 0, s, nodes, slots, feats, n_feats, d.meta, d.FG,
-                     d.NG, d.E, hist, slot_elems, node_sum, node_count, qexp, max_nodes, p,
-                     node_key, cand, dyn, feat_stride);
+  if (d.max_cat_bins > (int32_t)p.max_onehot_cats)
+    hipLaunchKernelGGL(k_split<true>, grid, dim3(kSplitWaves * WAVE), 0, s, nodes, slots, feats,
+                       n_feats, d.meta, d.FG, d.NG, d.E, hist, slot_elems, node_sum, node_count,
+                       qexp, max_nodes, p, node_key, cand, dyn, feat_stride);
+  else
+    hipLaunchKernelGGL(k_split<false>, grid, dim3(kSplitWaves * WAVE), 0, s, nodes, slots, feats,
+                       n_feats, d.meta, d.FG, d.NG, d.E, hist, slot_elems, node_sum, node_count,
+                       qexp, max_nodes, p, node_key, cand, dyn, feat_stride);
   if (!finalize) return;  // the caller reduces node_key / cand itself
   hipLaunchKernelGGL(k_split_finalize, dim3((n_nodes + 63) / 64), dim3(64), 0, s, nodes, n_nodes,
                      d.E, p.rightmost, node_key, cand, node_split, dyn);

@@ -289,7 +289,8 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
   const int g_lsegs = grid_for(S.cap_lsegs, kMaxGrid);
   uint32_t* cnt = B.counts;
   const int n_feats = S.n_feats;
-  const uint32_t* feats = S.col_sample ? S.feats_dev.p : feats_.p;
+  // Without column sampling every tree scans columns 0 .. E-1 (no list).
+  const uint32_t* feats = S.col_sample ? S.feats_dev.p : nullptr;
   const int feat_stride = S.col_sample ? (int)P.E : 0;
   launch_objective(y, w);
   if (sampled) {
