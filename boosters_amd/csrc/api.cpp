@@ -275,6 +275,7 @@ int bst_ctx_destroy(bst_ctx* ctx) {
     if (!ctx) return BST_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->pool.release();
     if (ctx->staging) (void)hipHostFree(ctx->staging);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -531,19 +532,31 @@ static int train_impl(bst_ctx* ctx, bst_dataset* ds, const float* y_dev, int64_t
       opt.rank_disc[r] = 1.0 / std::log2(2.0 + (double)r);
     opt.ext.query_group_sizes = nullptr;  // not kept past this call
   }
+  const bool trace = std::getenv("BST_TRACE_SETUP") != nullptr;
+  auto ms_since = [](std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+  };
   const auto tc0 = std::chrono::steady_clock::now();
-  Trainer tr(ctx, ds, opt);
-  if (std::getenv("BST_TRACE_SETUP")) {
-    BST_HIP(hipStreamSynchronize(ctx->stream));
-    std::fprintf(stderr, "[bst] trainer buffers %.3f ms\n",
-                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0)
-                     .count());
+  bst_model* m = nullptr;
+  std::chrono::steady_clock::time_point td;
+  {
+    // The trainer's buffers come from (and go back to) the context's pool.
+    PoolScope pool_scope(&ctx->pool);
+    Trainer tr(ctx, ds, opt);
+    if (trace) {
+      BST_HIP(hipStreamSynchronize(ctx->stream));
+      std::fprintf(stderr, "[bst] trainer buffers %.3f ms\n", ms_since(tc0));
+    }
+    const auto tt = std::chrono::steady_clock::now();
+    m = tr.train(y_dev, w_dev, vX_dev, vy_dev, nv);
+    if (trace) std::fprintf(stderr, "[bst] train %.3f ms\n", ms_since(tt));
+    ctx->last_ms_total = tr.stats.ms_total;
+    ctx->last_ms_per_tree = tr.stats.n_trees ? tr.stats.ms_total / (double)tr.stats.n_trees : 0.0;
+    ctx->last_hist_rows = tr.stats.hist_rows;
+    ctx->last_metric = tr.metric_history;
+    td = std::chrono::steady_clock::now();
   }
-  bst_model* m = tr.train(y_dev, w_dev, vX_dev, vy_dev, nv);
-  ctx->last_ms_total = tr.stats.ms_total;
-  ctx->last_ms_per_tree = tr.stats.n_trees ? tr.stats.ms_total / (double)tr.stats.n_trees : 0.0;
-  ctx->last_hist_rows = tr.stats.hist_rows;
-  ctx->last_metric = tr.metric_history;
+  if (trace) std::fprintf(stderr, "[bst] trainer teardown %.3f ms\n", ms_since(td));
   *out = m;
   return BST_OK;
 }

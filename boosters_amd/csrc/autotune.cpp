@@ -5,8 +5,8 @@
 //   fg            features per block: 32 (128 KB LDS sub-histogram, one block
 //                 per CU, 4 waves per SIMD) or 16 (64 KB, two blocks per CU,
 //                 8 waves per SIMD, each row read by twice as many blocks);
-//   tile_min      fewest rows per block; a step's tile is the power of two in
-//                 [tile_min, 8192] that spreads its rows over about
+//   tile_min      fewest rows per block; a step's tile (hist_step_tile) is
+//                 the 512-row multiple that spreads its rows over at most
 //   blocks_per_cu x n_cu blocks (more blocks balance better, fewer make fewer
 //                 partial histograms for the finish pass to add).
 // Histogram sums are exact integers, so every shape gives the same
@@ -34,7 +34,7 @@ namespace bst {
 
 namespace {
 
-constexpr uint32_t kTileMax = 8192;          // the device planner's largest tile
+constexpr uint32_t kTileMax = 65536;         // the device planner's largest tile
 constexpr uint64_t kMaxStepRows = 4u << 20;  // rows of the largest measured step
 constexpr uint32_t kSlots = 64;              // histogram slots the measurement cycles over
 
@@ -47,10 +47,7 @@ struct Shape {
 };
 
 uint32_t step_tile(uint32_t tile_min, uint32_t target, uint32_t NG, uint64_t rows) {
-  const uint64_t want = (rows * NG + target - 1) / target;
-  uint32_t tile = tile_min;
-  while (tile < want && tile < kTileMax) tile <<= 1;
-  return tile;
+  return hist_step_tile(tile_min, kTileMax, target, NG, rows);
 }
 
 std::vector<Shape> tree_shapes(int64_t n, const bst_gbdt_config& cfg, int T, bool root_gather) {

@@ -134,6 +134,22 @@ void launch_hist(hipStream_t s, const DatasetDev& d, const HistItem* items, int 
 // node_sum may be null (no totals); partial non-null: multi-tile nodes store
 // one partial histogram per item there instead of global atomics, and
 // launch_hist_finish sums them (and applies the subtraction trick).
+// Histogram tile (rows per (tile, group) item) of a step over `rows` rows:
+// as many tiles per group as fit target items (target / NG), so that the
+// step's items fill the resident grid once and no block takes two, rounded
+// up to whole 512-row chunks, within [tile_min, tile_max].  Nodes round up
+// to whole tiles, so a step of several nodes gets up to one extra item per
+// node and group.
+__host__ __device__ inline uint32_t hist_step_tile(uint32_t tile_min, uint32_t tile_max,
+                                                   uint32_t target, uint32_t NG, uint64_t rows) {
+  const uint64_t per_group = target / NG > 0 ? target / NG : 1;
+  uint64_t tile = (rows + per_group - 1) / per_group;
+  tile = (tile + 511) / 512 * 512;
+  if (tile < tile_min) tile = tile_min;
+  if (tile > tile_max) tile = tile_max;
+  return (uint32_t)tile;
+}
+
 // Histogram blocks one CU holds at once for a feature-group width (LDS).
 inline int hist_blocks_resident(int FG) { return FG == 16 ? 2 : 1; }
 // Histogram lanes of padding features (index E .. FG * NG - 1) read their

@@ -141,15 +141,11 @@ __device__ void advance_level(const CtlParams& P, const DeviceBufs& B) {
   }
 }
 
-// Histogram tile of a step: the power of two in [hist_tile_min, hist_tile]
-// that gives about hist_target (tile, group) items for `rows` rows, so small
-// steps still spread over the CUs and large ones do not make needless
-// partials.
+// Histogram tile of a step: about hist_target (tile, group) items for `rows`
+// rows, so small steps still spread over the CUs and large ones do not make
+// needless partials.
 __device__ uint32_t step_tile(const CtlParams& P, uint64_t rows) {
-  const uint64_t want = (rows * P.NG + P.hist_target - 1) / P.hist_target;
-  uint32_t tile = P.hist_tile_min;
-  while (tile < want && tile < P.hist_tile) tile <<= 1;
-  return tile;
+  return hist_step_tile(P.hist_tile_min, P.hist_tile, P.hist_target, P.NG, rows);
 }
 
 __device__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long* s_w) {

@@ -57,26 +57,98 @@ inline void need(bool ok, int code, const char* msg) {
       ::bst::raise(BST_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));    \
   } while (0)
 
-// Device buffer owned by the engine.
+// Cache of one context's device and pinned host blocks, reused by size
+// between training runs: a run of the same shape then allocates and frees
+// nothing (hipFree and hipHostFree synchronise and cost tens of microseconds
+// each).  Blocks are only handed out inside a PoolScope on the context's
+// thread, and all training work runs on the context's one stream, so a
+// reused block's earlier work is ordered before its new work.  A scope ends
+// by freeing the blocks the run did not reuse, so the cache holds at most
+// one run's working set.
+struct BufPool {
+  std::multimap<size_t, void*> dev, host;              // free, returned by this run
+  std::multimap<size_t, void*> dev_prev, host_prev;    // free, left by the previous run
+  static void* take(std::multimap<size_t, void*>& a, std::multimap<size_t, void*>& b,
+                    size_t bytes) {
+    for (auto* m : {&a, &b}) {
+      auto it = m->find(bytes);
+      if (it != m->end()) {
+        void* q = it->second;
+        m->erase(it);
+        return q;
+      }
+    }
+    return nullptr;
+  }
+  void* take_dev(size_t bytes) { return take(dev, dev_prev, bytes); }
+  void* take_host(size_t bytes) { return take(host, host_prev, bytes); }
+  void put_dev(void* q, size_t bytes) { dev.emplace(bytes, q); }
+  void put_host(void* q, size_t bytes) { host.emplace(bytes, q); }
+  static void free_all(std::multimap<size_t, void*>& m, bool pinned) {
+    for (auto& kv : m) (void)(pinned ? hipHostFree(kv.second) : hipFree(kv.second));
+    m.clear();
+  }
+  // Start of a run: what is free now is the previous run's.
+  void begin_run() {
+    dev_prev.merge(dev);
+    host_prev.merge(host);
+  }
+  // End of a run: free the previous run's blocks this run did not take.
+  void end_run() {
+    free_all(dev_prev, false);
+    free_all(host_prev, true);
+  }
+  void release() {
+    end_run();
+    free_all(dev, false);
+    free_all(host, true);
+  }
+  ~BufPool() { release(); }
+};
+
+// The pool that device buffers allocated on this thread come from (null:
+// plain hipMalloc / hipFree).
+inline thread_local BufPool* tls_pool = nullptr;
+
+struct PoolScope {
+  BufPool* prev;
+  explicit PoolScope(BufPool* p) : prev(tls_pool) {
+    p->begin_run();
+    tls_pool = p;
+  }
+  ~PoolScope() {
+    tls_pool->end_run();
+    tls_pool = prev;
+  }
+  PoolScope(const PoolScope&) = delete;
+  PoolScope& operator=(const PoolScope&) = delete;
+};
+
+// Device buffer owned by the engine.  Allocated inside a PoolScope, its block
+// comes from (and returns to) that pool.
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  BufPool* pool = nullptr;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), pool(o.pool) {
     o.p = nullptr;
     o.n = 0;
+    o.pool = nullptr;
   }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       reset();
       p = o.p;
       n = o.n;
+      pool = o.pool;
       o.p = nullptr;
       o.n = 0;
+      o.pool = nullptr;
     }
     return *this;
   }
@@ -84,9 +156,18 @@ struct DevBuf {
   void alloc(size_t count) {
     reset();
     if (count == 0) return;
+    pool = tls_pool;
+    if (pool) {
+      if (void* q = pool->take_dev(count * sizeof(T))) {
+        p = (T*)q;
+        n = count;
+        return;
+      }
+    }
     hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
     if (e != hipSuccess) {
       p = nullptr;
+      pool = nullptr;
       raise(BST_E_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
                            " bytes failed: " + hipGetErrorString(e));
     }
@@ -97,9 +178,15 @@ struct DevBuf {
     if (count > n) alloc(count);
   }
   void reset() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pool)
+        pool->put_dev(p, n * sizeof(T));
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    pool = nullptr;
   }
   size_t bytes() const { return n * sizeof(T); }
 };
@@ -178,6 +265,8 @@ struct bst_ctx {
   }
   // Autotuner results per data shape, and the shape the last train used.
   std::map<std::vector<int64_t>, bst::HistTuning> tune_cache;
+  // Device and pinned blocks of the last training run, reused by the next.
+  bst::BufPool pool;
   bst::HistTuning last_tuning;
   int64_t k_launches[3] = {0, 0, 0};
   double k_ms[3] = {0, 0, 0}, k_bytes[3] = {0, 0, 0};

@@ -21,7 +21,7 @@
 namespace bst {
 
 namespace {
-constexpr uint32_t kHistTileD = 8192;
+constexpr uint32_t kHistTileD = 65536;  // largest histogram tile (rows per item)
 constexpr uint32_t kPartTileD = kFusedTileRows;
 constexpr uint32_t kSegTileD = 4096;
 constexpr int kMaxGrid = 2048;
@@ -71,14 +71,30 @@ struct Trainer::DevState {
   // [0]: rounds without row sampling, [1]: rounds with it.
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
+  BufPool* pool = nullptr;  // where the pinned mirrors came from (null: hipHostMalloc)
+  size_t metric_bytes = 0;
+  void* pinned(size_t bytes) {
+    if (pool)
+      if (void* q = pool->take_host(bytes)) return q;
+    void* q = nullptr;
+    BST_HIP(hipHostMalloc(&q, bytes, hipHostMallocDefault));
+    return q;
+  }
+  void unpin(void* q, size_t bytes) {
+    if (!q) return;
+    if (pool)
+      pool->put_host(q, bytes);
+    else
+      (void)hipHostFree(q);
+  }
   ~DevState() {
     for (int i = 0; i < 2; i++) {
       if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
     }
     for (int i = 0; i < 2; i++) {
-      if (h_out[i]) (void)hipHostFree(h_out[i]);
-      if (h_metric[i]) (void)hipHostFree(h_metric[i]);
+      unpin(h_out[i], out_bytes);
+      unpin(h_metric[i], metric_bytes);
       if (done[i]) (void)hipEventDestroy(done[i]);
     }
   }
@@ -194,11 +210,12 @@ void Trainer::device_setup() {
   s.out_bytes = tree_out_layout(T, P.MNT, nullptr, nullptr);
   s.out.alloc(s.out_bytes);
   tree_out_layout(T, P.MNT, s.out.p, &s.O);
+  s.pool = tls_pool;
+  s.metric_bytes = sizeof(double) * 2 * kMetricBlocks * opt_.K;
   for (int i = 0; i < 2; i++) {
-    BST_HIP(hipHostMalloc((void**)&s.h_out[i], s.out_bytes, hipHostMallocDefault));
+    s.h_out[i] = (char*)s.pinned(s.out_bytes);
     tree_out_layout(T, P.MNT, s.h_out[i], &s.H[i]);
-    BST_HIP(hipHostMalloc((void**)&s.h_metric[i],
-                          sizeof(double) * 2 * kMetricBlocks * opt_.K, hipHostMallocDefault));
+    s.h_metric[i] = (double*)s.pinned(s.metric_bytes);
     BST_HIP(hipEventCreateWithFlags(&s.done[i], hipEventDisableTiming));
   }
   s.items.alloc(s.cap_items);
