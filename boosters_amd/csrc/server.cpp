@@ -8,9 +8,12 @@
 // traversal launch for the whole batch.  Up to max_inflight batches are on the
 // device at once, each on its own stream, so the copies of one batch overlap
 // the traversal of another and new requests are admitted as soon as a batch
-// completes and frees its budget.  Every row is traversed by the same kernel
-// with the same per-row summation order, so a result is bit-identical whether
-// its request ran alone or packed with others.
+// completes and frees its budget.  The worker only decides under the server's
+// lock (retire, admit, budget); packing requests into the pinned batch (split
+// over a few threads when large), the launches and unpacking results run
+// outside it, so submit and wait never queue behind a copy.  Every row is
+// traversed by the same kernel with the same per-row summation order, so a
+// result is bit-identical whether its request ran alone or packed with others.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -51,8 +54,34 @@ struct Batch {
   DevBuf<float> d_in, d_out;
   std::vector<Request*> reqs;
   int64_t rows = 0, bytes = 0;
-  bool busy = false;
+  bool busy = false;      // admitted, until its requests are done
+  bool retiring = false;  // its results are being unpacked
 };
+
+// Run fn(i) for i in [0, n) on up to `threads` threads (the caller's included).
+template <class Fn>
+void parallel_for(int n, int threads, Fn&& fn) {
+  threads = std::max(1, std::min(threads, n));
+  if (threads == 1) {
+    for (int i = 0; i < n; i++) fn(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve((size_t)threads - 1);
+  auto part = [&](int t) {
+    for (int i = t; i < n; i += threads) fn(i);
+  };
+  for (int t = 1; t < threads; t++) pool.emplace_back(part, t);
+  part(0);
+  for (auto& th : pool) th.join();
+}
+
+// Threads for packing a batch: packs below 8 MB are copied by the worker alone.
+int pack_threads(size_t bytes) {
+  if (bytes < ((size_t)8 << 20)) return 1;
+  const unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(8u, hw / 2));
+}
 
 }  // namespace
 
@@ -92,13 +121,21 @@ void ensure_batch(Batch& b, int64_t rows, int32_t F, int32_t K) {
 void launch_batch(bst_server* srv, Batch& b) {
   const int32_t F = srv->F, K = srv->K;
   ensure_batch(b, b.rows, F, K);
-  // Pack requests side by side in the row dimension: [F][rows].
+  // Pack requests side by side in the row dimension: [F][rows], one task per
+  // feature row.
+  std::vector<int64_t> offs(b.reqs.size());
   int64_t off = 0;
-  for (Request* r : b.reqs) {
-    for (int32_t f = 0; f < F; f++)
-      std::memcpy(b.h_in + (size_t)f * b.rows + off, r->X + (size_t)f * r->n, (size_t)r->n * 4);
-    off += r->n;
+  for (size_t i = 0; i < b.reqs.size(); i++) {
+    offs[i] = off;
+    off += b.reqs[i]->n;
   }
+  parallel_for(F, pack_threads((size_t)b.rows * F * 4), [&](int f) {
+    for (size_t i = 0; i < b.reqs.size(); i++) {
+      const Request* r = b.reqs[i];
+      std::memcpy(b.h_in + (size_t)f * b.rows + offs[i], r->X + (size_t)f * r->n,
+                  (size_t)r->n * 4);
+    }
+  });
   BST_HIP(hipMemcpyAsync(b.d_in.p, b.h_in, (size_t)b.rows * F * 4, hipMemcpyHostToDevice, b.stream));
   bst_model* m = const_cast<bst_model*>(srv->model);
   launch_predict(b.stream, m->d_nodes.p, m->d_tree_off.p, m->d_tree_group.p, m->info.n_trees,
@@ -110,19 +147,27 @@ void launch_batch(bst_server* srv, Batch& b) {
   b.busy = true;
 }
 
-void complete_batch(bst_server* srv, Batch& b) {
+// Copy a finished batch's results to its requests (outside the lock).
+void unpack_batch(bst_server* srv, Batch& b) {
   const int32_t K = srv->K;
   int64_t off = 0;
-  const auto now = Clock::now();
   for (Request* r : b.reqs) {
     for (int32_t k = 0; k < K; k++)
       std::memcpy(r->out + (size_t)k * r->n, b.h_out + (size_t)k * b.rows + off, (size_t)r->n * 4);
     off += r->n;
+  }
+}
+
+// Mark a batch's requests done and free its budget (under the lock).
+void retire_batch(bst_server* srv, Batch& b) {
+  const auto now = Clock::now();
+  for (Request* r : b.reqs) {
     r->latency_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(now - r->t0).count();
     r->done = true;
   }
   b.reqs.clear();
   b.busy = false;
+  b.retiring = false;
   srv->used -= b.bytes;
   b.bytes = 0;
   b.rows = 0;
@@ -132,18 +177,19 @@ void worker_loop(bst_server* srv) {
   try {
     BST_HIP(hipSetDevice(srv->ctx->device));
     const int64_t max_batch_rows = 1 << 20;
+    std::vector<Batch*> finished, admitted;
     for (;;) {
-      bool progressed = false;
+      finished.clear();
+      admitted.clear();
       {
         std::unique_lock<std::mutex> lk(srv->mu);
-        // Retire finished batches.
+        // Batches whose device work is done (results unpacked below).
         for (auto& b : srv->batches) {
-          if (b.busy && hipEventQuery(b.ev) == hipSuccess) {
-            complete_batch(srv, b);
-            progressed = true;
+          if (b.busy && !b.retiring && hipEventQuery(b.ev) == hipSuccess) {
+            b.retiring = true;
+            finished.push_back(&b);
           }
         }
-        if (progressed) srv->cv_done.notify_all();
         // Admit queued requests into a free batch while the budget allows.
         for (auto& b : srv->batches) {
           if (b.busy || srv->queue.empty()) continue;
@@ -158,17 +204,30 @@ void worker_loop(bst_server* srv) {
             srv->used += r->bytes;
           }
           if (!b.reqs.empty()) {
-            launch_batch(srv, b);
-            progressed = true;
+            b.busy = true;
+            admitted.push_back(&b);
           }
         }
         bool any_busy = false;
         for (auto& b : srv->batches) any_busy = any_busy || b.busy;
         if (srv->stop && srv->queue.empty() && !any_busy) return;
-        if (!progressed && !any_busy)
+        if (finished.empty() && admitted.empty() && !any_busy)
           srv->cv_work.wait(lk, [&] { return srv->stop || !srv->queue.empty(); });
       }
-      if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+      // Copies and launches without the lock: only this thread touches an
+      // admitted or finished batch, and its requests' buffers belong to it
+      // until they are marked done.
+      for (Batch* b : admitted) launch_batch(srv, *b);
+      for (Batch* b : finished) unpack_batch(srv, *b);
+      if (!finished.empty()) {
+        {
+          std::lock_guard<std::mutex> lk(srv->mu);
+          for (Batch* b : finished) retire_batch(srv, *b);
+        }
+        srv->cv_done.notify_all();
+      }
+      if (finished.empty() && admitted.empty())
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
   } catch (const Error& e) {
     std::lock_guard<std::mutex> lk(srv->mu);
