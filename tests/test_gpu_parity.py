@@ -342,6 +342,32 @@ def test_hist_launch_shapes_identical(ctx, oracle, case):
             assert [v.value for v in got] == [fg, rows_min, bpc, 0]
 
 
+def test_repeated_trains_reuse_buffers(ctx, oracle):
+    """Training runs in one context reuse its device and pinned blocks (a pool
+    keyed by size): a run after runs of another shape grows the oracle's
+    forest, the same forest as the first run, so no state is carried from one
+    run to the next through a reused block."""
+    X, y = binary_nan_data(20_000, 24, seed=41)
+    cfg = make_cfg(objective=1, n_trees=4, max_depth=6)
+    gm, om, b = _train_both(ctx, oracle, X, y, cfg, None, schedule=2)
+    ga = _forests_equal(gm, om.info, om.arrays)
+    # Another shape in between: more rows, other features, leaf-wise.
+    X2, y2 = regression_data(30_000, 17, seed=42)
+    b2 = _binned(ctx, X2)
+    h = C.c_void_p()
+    A.check(A.lib().bst_train(ctx.h, b2.h, A.ptr(y2), len(y2), None, None, None, 0,
+                              C.byref(make_cfg(n_trees=3, growth=1, max_leaves=20)),
+                              C.byref(make_ext(2)), C.byref(h)))
+    GBDTModel(h, ctx)
+    for _ in range(2):
+        h = C.c_void_p()
+        A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0, C.byref(cfg),
+                                  C.byref(make_ext(2)), C.byref(h)))
+        gb = GBDTModel(h, ctx).export()
+        for k in ga:
+            np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+
+
 def test_autotuner_choice_and_cache(ctx, oracle):
     """With autotune on, the shape is measured once per data shape and cached
     in the context; the forest is the oracle's whatever it picks."""
