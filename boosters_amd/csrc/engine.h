@@ -121,6 +121,21 @@ void launch_lambdarank(hipStream_t s, const float* preds, const float* y, const 
                        float* h, uint32_t* absmax);
 // Reads absmax[0..2K) and resets it to zero for the next round.
 void launch_qscale(hipStream_t s, uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp);
+// Fixed-point exponent of one gradient or hessian column from its max |x|: the
+// e with max|x| * 2^e * n_rows < 2^62, so any sum over the rows fits (0 for
+// an all-zero or non-finite column).
+__device__ inline int32_t qscale_exponent(float m, int64_t n_rows) {
+  int lg = 0;
+  while ((1LL << lg) < n_rows && lg < 62) lg++;
+  int e = 0;
+  if (m > 0.0f && isfinite(m)) {
+    int x;
+    frexpf(m, &x);  // m < 2^x
+    e = 62 - lg - x;
+    if (e > 1000) e = 1000;
+  }
+  return e;
+}
 
 void launch_init_rows(hipStream_t s, uint32_t* idx, float2* ogh, const float* g, const float* h,
                       const uint32_t* sampled, int64_t m);

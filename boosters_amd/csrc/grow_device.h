@@ -107,12 +107,17 @@ struct DeviceBufs {
   uint32_t* cursor;              // [items][2] partition cursors; [2 i] ends as the left count
 };
 
+// Also the round's quantisation scales (what launch_qscale does: qexp from
+// absmax[0..n_q), absmax reset).  rows_per_tree null: every tree has all rows.
 void launch_ctl_init(hipStream_t s, const CtlParams& p, const DeviceBufs& b,
-                     const uint32_t* rows_per_tree);
+                     const uint32_t* rows_per_tree, uint32_t* absmax, int n_q, int64_t n_rows,
+                     int32_t* qexp);
 // advance: depth-wise, the previous step's children form the level to plan.
 void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance);
 void launch_ctl_children(hipStream_t s, const CtlParams& p, const DeviceBufs& b);
-void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance);
+// Also advances the boosting-round counter read by the sampling kernels.
+void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance,
+                         uint32_t* round);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, int64_t n);
 // Element copy by a kernel; dst may be mapped pinned host memory (device
 // pointer from hipHostGetDevicePointer).

@@ -163,11 +163,19 @@ __device__ unsigned long long block_sum_u64(unsigned long long v, unsigned long 
 // Root state of every tree, root histogram work, its finish entry and the
 // root split list.  Root totals are added by the row initialisation.
 __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBufs B,
-                                                          const uint32_t* __restrict__ rows) {
+                                                          const uint32_t* __restrict__ rows,
+                                                          uint32_t* __restrict__ absmax, int n_q,
+                                                          int64_t n_rows, int32_t* __restrict__ qexp) {
   __shared__ uint32_t s_hist;
   if (threadIdx.x == 0) s_hist = 0;
+  // The round's quantisation scales; absmax is then ready for the next round.
+  if ((int)threadIdx.x < n_q) {
+    qexp[threadIdx.x] = qscale_exponent(__uint_as_float(absmax[threadIdx.x]), n_rows);
+    absmax[threadIdx.x] = 0u;
+  }
+  auto rows_of = [&](uint32_t t) { return rows ? rows[t] : (uint32_t)P.stride; };
   uint64_t all_rows = 0;
-  for (uint32_t t = 0; t < P.T; t++) all_rows += rows[t];
+  for (uint32_t t = 0; t < P.T; t++) all_rows += rows_of(t);
   const uint32_t tile = step_tile(P, all_rows);
   __syncthreads();
   for (uint32_t t = 0; t < P.T; t++) {
@@ -178,7 +186,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
     for (uint32_t i = threadIdx.x; i + 1 < P.SPT; i += blockDim.x)
       B.free_slot[(uint64_t)t * P.SPT + i] = P.SPT - 1 - i;  // slot 0 goes to the root
     __syncthreads();
-    const uint32_t m = rows[t];
+    const uint32_t m = rows_of(t);
     const uint32_t nt = (m + tile - 1) / tile;
     if (threadIdx.x == 0) {
       TreeCtl c;
@@ -233,7 +241,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
     B.counts[C_FIN] = P.T;
     B.counts[C_SPLIT] = P.T;
     uint32_t r = 0;
-    for (uint32_t t = 0; t < P.T; t++) r += rows[t];
+    for (uint32_t t = 0; t < P.T; t++) r += rows_of(t);
     B.counts[C_HROWS] = r;
     B.counts[C_HNODES] = P.T;
   }
@@ -646,9 +654,13 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
 
 // Remaining candidates become leaves; learning rate; leaf-range update list.
 __global__ __launch_bounds__(kCtlThreads) void k_ctl_finalize(CtlParams P, DeviceBufs B,
-                                                              int advance) {
+                                                              int advance,
+                                                              uint32_t* __restrict__ round) {
   if (advance && P.growth == 0) advance_level(P, B);
-  if (threadIdx.x == 0) B.counts[C_LSEGS] = 0;
+  if (threadIdx.x == 0) {
+    B.counts[C_LSEGS] = 0;
+    if (round) *round += 1u;  // nothing after this kernel in the round reads it
+  }
   __syncthreads();
   for (uint32_t t = 0; t < P.T; t++) {
     const uint32_t no = B.ctl[t].n_open;
@@ -724,8 +736,11 @@ size_t tree_out_layout(uint32_t T, uint32_t MNT, char* base, TreeOut* out) {
 }
 
 void launch_ctl_init(hipStream_t s, const CtlParams& p, const DeviceBufs& b,
-                     const uint32_t* rows_per_tree) {
-  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(kCtlThreads), 0, s, p, b, rows_per_tree);
+                     const uint32_t* rows_per_tree, uint32_t* absmax, int n_q, int64_t n_rows,
+                     int32_t* qexp) {
+  if (n_q > kCtlThreads) throw std::runtime_error("too many quantisation scales for one block");
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(kCtlThreads), 0, s, p, b, rows_per_tree, absmax,
+                     n_q, n_rows, qexp);
 }
 void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance) {
   hipLaunchKernelGGL(k_ctl_plan, dim3(1), dim3(kCtlThreads), 0, s, p, b, advance ? 1 : 0);
@@ -733,8 +748,10 @@ void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b, boo
 void launch_ctl_children(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
   hipLaunchKernelGGL(k_ctl_children, dim3(1), dim3(kCtlThreads), 0, s, p, b);
 }
-void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance) {
-  hipLaunchKernelGGL(k_ctl_finalize, dim3(1), dim3(kCtlThreads), 0, s, p, b, advance ? 1 : 0);
+void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance,
+                         uint32_t* round) {
+  hipLaunchKernelGGL(k_ctl_finalize, dim3(1), dim3(kCtlThreads), 0, s, p, b, advance ? 1 : 0,
+                     round);
 }
 void launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, int64_t n) {
   if (n <= 0) return;

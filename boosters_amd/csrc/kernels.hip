@@ -486,17 +486,7 @@ void launch_lambdarank(hipStream_t s, const float* preds, const float* y, const 
 __global__ void k_qscale(uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
   const int i = threadIdx.x;
   if (i >= 2 * K) return;
-  const float m = __uint_as_float(absmax[i]);
-  int lg = 0;
-  while ((1LL << lg) < n_rows && lg < 62) lg++;
-  int e = 0;
-  if (m > 0.0f && isfinite(m)) {
-    int x;
-    frexpf(m, &x);  // m < 2^x
-    e = 62 - lg - x;
-    if (e > 1000) e = 1000;
-  }
-  qexp[i] = e;
+  qexp[i] = qscale_exponent(__uint_as_float(absmax[i]), n_rows);
   absmax[i] = 0u;  // ready for the next round's gradients
 }
 
@@ -1170,6 +1160,8 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
       long long lg = vg[0] + vg[1] + vg[2] + vg[3], lh = vh[0] + vh[1] + vh[2] + vh[3];
       const long long ig_incl = wave_incl_scan(lg), ih_incl = wave_incl_scan(lh);
       long long pgv = ig_incl - lg, phv = ih_incl - lh;  // exclusive prefix before bin 4*lane
+      // X[j]: prefix before bin 4*lane+j; P[j]: through it (X[j+1] = P[j]).
+      const long long X0 = pgv, Y0 = phv;
       long long P[4], Q[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -1218,7 +1210,7 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
         }
         // Backward: bins b..me right, split at b-1, missing left; b in [1, me].
         if (fm.has_missing && b >= 1 && (int)b <= me) {
-          const long long excl_g = P[j] - vg[j], excl_h = Q[j] - vh[j];
+          const long long excl_g = j == 0 ? X0 : P[j - 1], excl_h = j == 0 ? Y0 : Q[j - 1];
           const long long GR = Pme - excl_g, HR = Qme - excl_h;
           const uint32_t cr = max_excl - b, cl = sat_sub(cnt, cr);
           const double gr = (double)GR * ig, hr = (double)HR * ih;

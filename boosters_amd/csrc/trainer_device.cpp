@@ -315,11 +315,9 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
     // positions; rows[t] and the quantisation maxima cover only them.
     launch_sample_rows(s_, S.SP, S.round.p, g_.p, h_.p, S.samp.p, S.bcount.p, idx_.p, ogh_.p,
                        S.rows.p, absmax_.p);
-  } else {
-    launch_fill_u32(s_, S.rows.p, (uint32_t)n_, P.T);
   }
-  launch_qscale(s_, absmax_.p, K, n_, qexp_.p);
-  launch_ctl_init(s_, P, B, S.rows.p);
+  // Unsampled trees take every row (no row counts to read).
+  launch_ctl_init(s_, P, B, sampled ? S.rows.p : nullptr, absmax_.p, 2 * K, n_, qexp_.p);
   if (!sampled)
     launch_init_rows_sums(s_, idx_.p, ogh_.p, g_.p, h_.p, n_, (int)P.T, qexp_.p, MN_,
                           node_sum_.p);
@@ -355,7 +353,7 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
     // budget is spent), so their histograms are not built.
     if (step + 1 < S.steps) hist_step(true, false);
   }
-  launch_ctl_finalize(s_, P, B, /*advance=*/S.steps > 0);
+  launch_ctl_finalize(s_, P, B, /*advance=*/S.steps > 0, S.round.p);
   if (sampled) {
     // Rows outside the sample get the tree too: traverse the raw features
     // (trainer.rs:331-339).
@@ -366,7 +364,6 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
     launch_leaf_update(s_, B.leaf_segs, g_lsegs, leaf_val_.p, idx_.p, 0, preds_.p,
                        tree_output_.p, n_, cnt + C_LSEGS, idx_tmp_.p);
   }
-  launch_round_advance(s_, S.round.p);
 }
 
 // Launch (or capture and launch) one round, wait, and return the trees.
