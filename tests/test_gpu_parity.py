@@ -342,6 +342,30 @@ def test_hist_launch_shapes_identical(ctx, oracle, case):
             assert [v.value for v in got] == [fg, rows_min, bpc, 0]
 
 
+@pytest.mark.parametrize("E,nan", [(113, True), (40, False), (40, True), (7, True)])
+def test_padded_rows_feature_counts(ctx, oracle, E, nan):
+    """Histogram lanes of padding features (past the last real feature of the
+    last 16- or 32-feature group) read their own zero bytes of the padded
+    row instead of testing their index.  Feature counts whose padding needs
+    a longer row than the bins and NaN bits alone (113 with NaN bits: the
+    16-feature groups' NaN bytes; 40: the 32-feature groups' bins) grow the
+    oracle's forest with either group width."""
+    X, y = binary_nan_data(20_000, E, seed=E) if nan else regression_data(20_000, E, seed=E)
+    cfg = make_cfg(objective=1 if nan else 0, n_trees=3, max_depth=6)
+    gm, om, b = _train_both(ctx, oracle, X, y, cfg, None, schedule=2)
+    ga = _forests_equal(gm, om.info, om.arrays)
+    for fg in (16, 32):
+        ext = make_ext(2)
+        ext.autotune = 0
+        ext.hist_fg, ext.hist_rows_min, ext.hist_blocks_per_cu = fg, 512, 2
+        h = C.c_void_p()
+        A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0, C.byref(cfg),
+                                  C.byref(ext), C.byref(h)))
+        gb = GBDTModel(h, ctx).export()
+        for k in ga:
+            np.testing.assert_array_equal(ga[k], gb[k], err_msg=f"{k} fg {fg}")
+
+
 def test_repeated_trains_reuse_buffers(ctx, oracle):
     """Training runs in one context reuse its device and pinned blocks (a pool
     keyed by size): a run after runs of another shape grows the oracle's
