@@ -201,10 +201,14 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
                   int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
                   const int32_t* qexp, uint32_t max_nodes, SplitParams p,
                   unsigned long long* node_key, DevSplit* cand, DevSplit* node_split,
-                  const uint32_t* dyn = nullptr, bool finalize = true, int feat_stride = 0);
+                  const uint32_t* dyn = nullptr, bool finalize = true, int feat_stride = 0,
+                  const uint32_t* minus = nullptr);
 // feats: n_feats effective columns, shared by all trees (feat_stride 0) or
 // one list per tree slot at feats + slot * feat_stride; null = columns
-// 0 .. n_feats-1.
+// 0 .. n_feats-1.  minus (optional, per node entry): a slot whose histogram
+// the split subtracts from the entry's slot before scanning, writing the
+// difference back into the entry's slot (for its candidate features), or
+// ~0 for none.
 
 // Stable partition of every listed node (partition.rs:73-145): count, scan,
 // scatter, copy back, then children counts/sums.

@@ -93,6 +93,11 @@ struct DeviceBufs {
   HistItem* hitems;
   uint32_t* fin;   // [n][4]: small slot, large slot or ~0, first item, tiles
   uint32_t *split_nodes, *split_slots;
+  // Per split entry: the slot holding the smaller sibling's histogram, which
+  // the split subtracts from the entry's slot (the parent's histogram, kept
+  // by the larger child) and writes back; kNone for a histogram that is
+  // complete.
+  uint32_t* split_minus;
   Segment* leaf_segs;
   uint32_t* counts;  // [C_NCOUNTS]
   // Engine buffers shared with the trainer.

@@ -212,6 +212,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
       B.open_depth[(uint64_t)t * P.WO] = 0;
       B.split_nodes[t] = gid;
       B.split_slots[t] = t * P.SPT;
+      B.split_minus[t] = kNone;
       B.fin[4 * t + 0] = t * P.SPT;
       B.fin[4 * t + 1] = kNone;
       B.fin[4 * t + 2] = s_hist;
@@ -508,10 +509,10 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
     atomicMax(&s_cnt[B.pend[5 * i]], B.pend[5 * i + 3] + 1u);
   __syncthreads();
   // Work lists are laid out by item order: histogram items by an exclusive
-  // scan of each child's tile count, finish entry i and split entries 2i,
-  // 2i + 1 for item i; the e-th item of tree t takes that tree's e-th free
-  // slot from the top of its free list.
-  uint32_t carry_h = 0;
+  // scan of each child's tile count, finish entries by a scan over the items
+  // that need one, split entries 2i, 2i + 1 for item i; the e-th item of tree
+  // t takes that tree's e-th free slot from the top of its free list.
+  uint32_t carry_h = 0, carry_f = 0;
   unsigned long long hrows = 0ull;
   for (uint32_t base = 0; base < n_items; base += blockDim.x) {
     const uint32_t i = base + threadIdx.x;
@@ -533,6 +534,11 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
     uint32_t tot;
     const uint32_t h0 = block_excl_scan(nt * P.NG, s_tmp, &tot) + carry_h;
     carry_h += tot;
+    // Finish entries only for small children whose histogram is not already
+    // whole in its slot: several tiles (partials to add) or none (zeros).
+    const uint32_t fneed = (in && need && nt != 1) ? 1u : 0u;
+    const uint32_t fi = block_excl_scan(fneed, s_tmp, &tot) + carry_f;
+    carry_f += tot;
     if (in) {
       const uint32_t parent = it.node, right = it.right_node;
       B.node_count[parent] = L;
@@ -581,15 +587,22 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
               B.hitems[h0 + q * P.NG + g] = h;
             }
         }
-        B.fin[4 * i + 0] = gss;
-        B.fin[4 * i + 1] = gls;
-        B.fin[4 * i + 2] = h0;
-        B.fin[4 * i + 3] = nt;
-        // The large child keeps the parent's slot (ps); the small one has ss.
+        // The finish only adds the small child's partials; the split of the
+        // large child subtracts the small one from the parent's histogram,
+        // which the large child keeps in the parent's slot (ps), and writes
+        // the result back there.
+        if (fneed) {
+          B.fin[4 * fi + 0] = gss;
+          B.fin[4 * fi + 1] = kNone;
+          B.fin[4 * fi + 2] = h0;
+          B.fin[4 * fi + 3] = nt;
+        }
         B.split_nodes[2 * i] = parent;
         B.split_slots[2 * i] = t * P.SPT + (left_small ? ss : ps);
+        B.split_minus[2 * i] = left_small ? kNone : gss;
         B.split_nodes[2 * i + 1] = right;
         B.split_slots[2 * i + 1] = t * P.SPT + (left_small ? ps : ss);
+        B.split_minus[2 * i + 1] = left_small ? gss : kNone;
       } else {
         B.pslot[small] = kNone;
       }
@@ -636,7 +649,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
   }
   if (threadIdx.x == 0) {
     B.counts[C_HIST] = carry_h;
-    B.counts[C_FIN] = need ? n_items : 0u;
+    B.counts[C_FIN] = carry_f;
     B.counts[C_SPLIT] = need ? 2u * n_items : 0u;
     B.counts[C_HROWS] += (uint32_t)rows_total;
     B.counts[C_HNODES] += need ? n_items : 0u;

@@ -842,7 +842,9 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
 // the parent's histogram in its slot, becomes parent - small
 // (histograms/pool.rs subtraction trick).  fin[4 i] = {small slot, large slot
 // or ~0, first item index, tile count}; work = (node, group, 1024-value chunk).
-// A node with one tile was stored directly; with none it is all zeros.
+// A node with one tile was stored directly; with none it is all zeros.  The
+// device-driven schedule passes no large slot: its split does the
+// subtraction while it stages the bins (launch_split's `minus`).
 constexpr int kFinThreads = 256;
 constexpr int kFinChunk = kFinThreads * 2;  // int64 values per work item
 
@@ -860,6 +862,7 @@ __global__ __launch_bounds__(kFinThreads) void k_hist_finish(
     const uint32_t g = rem / chunks, c = rem % chunks;
     const uint32_t small = fin[4 * fi], large = fin[4 * fi + 1];
     const uint32_t h0 = fin[4 * fi + 2], nt = fin[4 * fi + 3];
+    if (nt == 1 && large == 0xFFFFFFFFu) continue;  // stored directly, nothing to subtract
     const uint32_t v0 = c * kFinChunk + threadIdx.x * 2u;
     longlong2* sd = (longlong2*)(hist + (int64_t)small * slot_elems + (int64_t)g * V + v0);
     longlong2 a;
@@ -904,8 +907,12 @@ void launch_hist_finish(hipStream_t s, const uint32_t* fin, int n, int NG, int F
                         const long long* partial, long long* hist, int64_t slot_elems,
                         const uint32_t* dyn) {
   if (n <= 0) return;
+  // Work items are (node, group, chunk): NG x 16 per node.  A grid sized for
+  // the worst case would mostly start blocks that find no work in the
+  // device-side count (a cost at every step); 512 covers the steps that have
+  // real work (nodes of several tiles).
   int64_t grid = (int64_t)n * NG * (256 * FG * 2 / kFinChunk);
-  if (grid > 2048) grid = 2048;
+  if (grid > 512) grid = 512;
   hipLaunchKernelGGL(k_hist_finish, dim3((unsigned)grid), dim3(kFinThreads), 0, s, fin,
                      (uint32_t)n, NG, FG, partial, hist, slot_elems, dyn);
 }
@@ -1379,11 +1386,11 @@ template <bool SORTED>
 __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     const uint32_t* __restrict__ nodes, const uint32_t* __restrict__ slots,
     const uint32_t* __restrict__ feats, int n_feats, const FeatMeta* __restrict__ fmeta, int FG,
-    int NG, int E, const long long* __restrict__ hist, int64_t slot_elems,
+    int NG, int E, long long* __restrict__ hist, int64_t slot_elems,
     const long long* __restrict__ node_sum, const uint32_t* __restrict__ node_count,
     const int32_t* __restrict__ qexp, uint32_t max_nodes, SplitParams p,
     unsigned long long* __restrict__ node_key, DevSplit* __restrict__ cand,
-    const uint32_t* __restrict__ dyn, int feat_stride) {
+    const uint32_t* __restrict__ dyn, int feat_stride, const uint32_t* __restrict__ minus) {
   constexpr int kScratch = SORTED ? 256 : 1;
   __shared__ double s_ratio[kSplitWaves][kScratch];
   __shared__ uint16_t s_order[kSplitWaves][kScratch];
@@ -1396,8 +1403,10 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
   for (uint32_t wy = blockIdx.y; wy < n_work; wy += gridDim.y) {  // block-uniform
     const uint32_t node = nodes[wy];
     const uint32_t slot = slots[wy];
+    const uint32_t mslot = minus ? minus[wy] : 0xFFFFFFFFu;
     const uint32_t tree = node / max_nodes;
-    const long long* base = hist + (int64_t)slot * slot_elems;
+    long long* base = hist + (int64_t)slot * slot_elems;
+    const long long* mbase = mslot != 0xFFFFFFFFu ? hist + (int64_t)mslot * slot_elems : nullptr;
     const uint32_t* fl = feats ? feats + (int64_t)tree * feat_stride : nullptr;
     // This wave's feature and inputs.
     const bool fok = fi < n_feats;
@@ -1407,16 +1416,31 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     const long long G = node_sum[2 * node], H = node_sum[2 * node + 1];
     const uint32_t cnt = node_count[node];
     // Stage: thread t loads bin t / kSplitWaves + pass * (kThreads / kSplitWaves)
-    // of the block's feature t % kSplitWaves.
+    // of the block's feature t % kSplitWaves.  With a sibling to subtract,
+    // the slot holds the parent's histogram: stage parent - sibling and
+    // write it back, so the slot holds this node's histogram from now on
+    // (the block's features only; no other block touches them).
     {
       const int w = threadIdx.x % kSplitWaves;
       const int fw = blockIdx.x * kSplitWaves + w;
       if (fw < n_feats) {
         const uint32_t fs = fl ? fl[fw] : (uint32_t)fw;
         const int grp = (int)fs / FG, lf = (int)fs % FG;
-        for (int b = threadIdx.x / kSplitWaves; b < 256; b += kThreads / kSplitWaves) {
-          const int64_t e = ((int64_t)(grp * 256 + b) * FG + lf) * 2;
-          s_bins[w][b] = *(const longlong2*)(base + e);
+        if (mbase) {
+          for (int b = threadIdx.x / kSplitWaves; b < 256; b += kThreads / kSplitWaves) {
+            const int64_t e = ((int64_t)(grp * 256 + b) * FG + lf) * 2;
+            longlong2 v = *(const longlong2*)(base + e);
+            const longlong2 m = *(const longlong2*)(mbase + e);
+            v.x -= m.x;
+            v.y -= m.y;
+            *(longlong2*)(base + e) = v;
+            s_bins[w][b] = v;
+          }
+        } else {
+          for (int b = threadIdx.x / kSplitWaves; b < 256; b += kThreads / kSplitWaves) {
+            const int64_t e = ((int64_t)(grp * 256 + b) * FG + lf) * 2;
+            s_bins[w][b] = *(const longlong2*)(base + e);
+          }
         }
       }
     }
@@ -1474,17 +1498,18 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
                   int64_t slot_elems, const long long* node_sum, const uint32_t* node_count,
                   const int32_t* qexp, uint32_t max_nodes, SplitParams p,
                   unsigned long long* node_key, DevSplit* cand, DevSplit* node_split,
-                  const uint32_t* dyn, bool finalize, int feat_stride) {
+                  const uint32_t* dyn, bool finalize, int feat_stride, const uint32_t* minus) {
+  long long* hw = const_cast<long long*>(hist);  // written only through `minus` entries
   if (n_nodes <= 0) return;
   dim3 grid((unsigned)((n_feats + kSplitWaves - 1) / kSplitWaves), (unsigned)n_nodes);
   if (d.max_cat_bins > (int32_t)p.max_onehot_cats)
     hipLaunchKernelGGL(k_split<true>, grid, dim3(kSplitWaves * WAVE), 0, s, nodes, slots, feats,
-                       n_feats, d.meta, d.FG, d.NG, d.E, hist, slot_elems, node_sum, node_count,
-                       qexp, max_nodes, p, node_key, cand, dyn, feat_stride);
+                       n_feats, d.meta, d.FG, d.NG, d.E, hw, slot_elems, node_sum, node_count,
+                       qexp, max_nodes, p, node_key, cand, dyn, feat_stride, minus);
   else
     hipLaunchKernelGGL(k_split<false>, grid, dim3(kSplitWaves * WAVE), 0, s, nodes, slots, feats,
-                       n_feats, d.meta, d.FG, d.NG, d.E, hist, slot_elems, node_sum, node_count,
-                       qexp, max_nodes, p, node_key, cand, dyn, feat_stride);
+                       n_feats, d.meta, d.FG, d.NG, d.E, hw, slot_elems, node_sum, node_count,
+                       qexp, max_nodes, p, node_key, cand, dyn, feat_stride, minus);
   if (!finalize) return;  // the caller reduces node_key / cand itself
   hipLaunchKernelGGL(k_split_finalize, dim3((n_nodes + 63) / 64), dim3(64), 0, s, nodes, n_nodes,
                      d.E, p.rightmost, node_key, cand, node_split, dyn);

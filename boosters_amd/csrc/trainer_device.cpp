@@ -47,7 +47,7 @@ struct Trainer::DevState {
   DevBuf<PartTile> tiles;
   DevBuf<uint32_t> pend;
   DevBuf<HistItem> hitems;
-  DevBuf<uint32_t> fin, split_nodes, split_slots, rows, eff_to_orig;
+  DevBuf<uint32_t> fin, split_nodes, split_slots, split_minus, rows, eff_to_orig;
   DevBuf<Segment> leaf_segs;
   DevBuf<float> thr_up;
   DevBuf<long long> partial;    // one histogram group per multi-tile item
@@ -225,6 +225,7 @@ void Trainer::device_setup() {
   s.fin.alloc((size_t)4 * s.cap_fin);
   s.split_nodes.alloc(s.cap_split);
   s.split_slots.alloc(s.cap_split);
+  s.split_minus.alloc(s.cap_split);
   s.rows.alloc(T);
   s.leaf_segs.alloc(s.cap_lsegs);
   s.partial.alloc((size_t)s.cap_hist * 256 * P.FG * 2);
@@ -273,6 +274,7 @@ void Trainer::device_setup() {
   B.fin = s.fin.p;
   B.split_nodes = s.split_nodes.p;
   B.split_slots = s.split_slots.p;
+  B.split_minus = s.split_minus.p;
   B.leaf_segs = s.leaf_segs.p;
   B.counts = s.O.counts;
   B.node_sum = node_sum_.p;
@@ -335,7 +337,8 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
     pi = ctx_->prof_begin(1, 0.0);
     launch_split(s_, d, B.split_nodes, B.split_slots, g_split, feats, n_feats, hist_.p,
                  slot_elems_, node_sum_.p, node_count_.p, qexp_.p, MN_, sp_, node_key_.p,
-                 cand_.p, node_split_.p, cnt + C_SPLIT, /*finalize=*/false, feat_stride);
+                 cand_.p, node_split_.p, cnt + C_SPLIT, /*finalize=*/false, feat_stride,
+                 B.split_minus);
     ctx_->prof_end(pi);
   };
   // Positions of tree slot t start at t * n, so only slot 0 of an unsampled
