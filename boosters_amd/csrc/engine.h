@@ -61,6 +61,12 @@ struct PartItem {
   uint32_t right_node; // global id of the new right child
   uint32_t feature;    // effective column
   uint32_t pad;        // device-driven partition: bit 0 = rows are in the alternate buffer
+  // Device-driven partition: the node's split decision, copied by the planner
+  // so that the partition need not load the split and the feature meta:
+  uint32_t split_bin;    // left: bin <= split_bin (numerical)
+  uint32_t split_flags;  // DevSplit::flags (bit 0 default left, bit 1 categorical)
+  uint32_t miss_bin;     // the feature's missing bin, or ~0 without one
+  uint32_t pad2;
 };
 
 struct PartTile {

@@ -108,6 +108,7 @@ struct DeviceBufs {
   const DevSplit* cand;          // [node][E] per-feature best
   const float* thr_up;           // [E][256]
   const uint32_t* eff_to_orig;   // [E]
+  const FeatMeta* fmeta;         // [E]
   const int32_t* qexp;           // [2K]
   uint32_t* cursor;              // [items][2] partition cursors; [2 i] ends as the left count
 };

@@ -405,6 +405,13 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
         it.right_node = rgid;
         it.feature = f;
         it.pad = B.pbuf[gid];
+        {
+          const FeatMeta fm = B.fmeta[f];
+          it.split_bin = sp.bin;
+          it.split_flags = sp.flags;
+          it.miss_bin = fm.has_missing ? fm.default_bin : 0xFFFFFFFFu;
+          it.pad2 = 0;
+        }
         B.items[g] = it;
         B.cursor[2 * g] = 0;
         B.cursor[2 * g + 1] = 0;

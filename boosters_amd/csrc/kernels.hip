@@ -1708,8 +1708,9 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
     const float2* sogh = src1 ? ogh1 : ogh0;
     uint32_t* didx = src1 ? idx0 : idx1;
     float2* dogh = src1 ? ogh0 : ogh1;
-    const DevSplit sp = node_split[it.node];
-    const FeatMeta fm = fmeta[it.feature];
+    // The split decision comes with the item (goes_left on it): only a
+    // categorical split reads its category words from the node's split.
+    const uint32_t* cats = (it.split_flags & 2u) ? node_split[it.node].cats : nullptr;
     const uint8_t* col = bins_cm + (int64_t)it.feature * n;
     const uint32_t* hcol = hi_cm ? hi_cm + (int64_t)it.feature * hi_words : nullptr;
     uint32_t rid[kFusedRows];
@@ -1733,7 +1734,11 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
     for (int j = 0; j < kFusedRows; j++) {
       const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
       const bool valid = p < t.count;
-      const bool l = valid && goes_left(sp, fm, bins[j]);
+      const uint32_t bn = bins[j];
+      const bool l =
+          valid && (bn == it.miss_bin ? (it.split_flags & 1u) != 0
+                    : cats ? bn < 256u && ((cats[bn >> 5] >> (bn & 31)) & 1u)
+                           : bn <= it.split_bin);
       lbal[j] = __ballot(l);
       vbal[j] = __ballot(valid);
       if (lane == 0) {

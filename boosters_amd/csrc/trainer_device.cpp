@@ -284,6 +284,7 @@ void Trainer::device_setup() {
   B.cand = cand_.p;
   B.thr_up = s.thr_up.p;
   B.eff_to_orig = s.eff_to_orig.p;
+  B.fmeta = dd_.meta;
   B.qexp = qexp_.p;
   B.cursor = s.cursor.p;
   BST_HIP(hipStreamSynchronize(s_));
