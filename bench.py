@@ -35,16 +35,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gen_binary_nan(n, F, seed):
-    """1M x 100 N(0,1), 10% NaN, label ~ Bernoulli(sigmoid(w . x[0:10])) on pre-mask x."""
+def gen_binary_nan(n, F, seed, w=None):
+    """1M x 100 N(0,1), 10% NaN, label ~ Bernoulli(sigmoid(w . x[0:10])) on pre-mask x.
+
+    `w` given: rows from the same labelling model (a held-out set of another
+    seed); otherwise the seed's own draw is used."""
     rng = np.random.default_rng(seed)
     X = rng.standard_normal((F, n), dtype=np.float32)
     mask = rng.random((F, n), dtype=np.float32) < 0.1
-    w = rng.standard_normal(10)
+    w_seed = rng.standard_normal(10)
+    w = w_seed if w is None else w
     score = w @ X[:10].astype(np.float64)
     y = (rng.random(n) < 1.0 / (1.0 + np.exp(-score))).astype(np.float32)
     X[mask] = np.nan
-    return X, y
+    return X, y, w
 
 
 def cfg_c2(n_trees, depth):
@@ -487,7 +491,7 @@ def main():
         return
     n, F, depth = args.rows, args.features, args.depth
     log(f"generating {n} x {F} binary-logistic data with 10% NaN")
-    X, y = gen_binary_nan(n, F, seed=0)
+    X, y, w_true = gen_binary_nan(n, F, seed=0)
     t0 = time.perf_counter()
     ds = C.c_void_p()
     A.check(L.bst_dataset_create(ctx, A.ptr(X), n, F, None, None, C.byref(ds)))
@@ -526,8 +530,9 @@ def main():
         plain_ms_tree = plain_ms / args.plain_steps
         log(f"plain: {plain_ms_tree:.3f} ms/tree")
 
-    # Held-out validation loss of the timed model (20% split drawn from the same generator).
-    Xv, yv = gen_binary_nan(n // 4, F, seed=1)
+    # Held-out validation loss of the timed model: n/4 new rows (another seed)
+    # labelled by the training data's model.
+    Xv, yv, _ = gen_binary_nan(n // 4, F, seed=1, w=w_true)
     pv = np.zeros(len(yv), np.float32)
     A.check(L.bst_predict(ctx, h_model, A.ptr(Xv), len(yv), F, 0, A.ptr(pv), None))
     p = np.clip(pv.astype(np.float64), 1e-15, 1 - 1e-15)

@@ -27,6 +27,7 @@ constexpr uint32_t kDirectFlag = 0x80000000u;  // HistItem.tree: tile = whole no
 constexpr uint32_t kAltFlag = 0x40000000u;     // HistItem.tree: alternate buffer
 constexpr int kCtlThreads = 1024;
 constexpr uint32_t kPlanLdsItems = 1024;  // items whose partition tiles the plan fills block-wide
+static_assert(kCtlThreads <= kPlanLdsItems, "a paired level records every item in LDS");
 
 // Exclusive scan over the block (kCtlThreads threads); *total receives the sum.
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
@@ -94,12 +95,13 @@ __device__ void init_tree_node(const CtlParams& P, const DeviceBufs& B, uint32_t
 }
 
 // compute_leaf_weight (split/gain.rs:151-168) from exact fixed-point sums.
-__device__ void make_leaf(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t p,
-                          uint32_t tn) {
-  const uint32_t gid = t * P.MN + p;
-  const int out = P.out0 + (int)t;
-  const double G = ldexp((double)B.node_sum[2 * gid], -B.qexp[2 * out]);
-  const double H = ldexp((double)B.node_sum[2 * gid + 1], -B.qexp[2 * out + 1]);
+// The node's sums (sg, sh), the round's scale exponents (qg, qh) and its
+// histogram slot are passed in already loaded.
+__device__ void make_leaf_from(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t gid,
+                               uint32_t tn, long long sg, long long sh, int qg, int qh,
+                               uint32_t slot) {
+  const double G = ldexp((double)sg, -qg);
+  const double H = ldexp((double)sh, -qh);
   float w;
   if (P.alpha == 0.0) {
     w = (float)(-G / (H + P.lambda));
@@ -112,12 +114,19 @@ __device__ void make_leaf(const CtlParams& P, const DeviceBufs& B, uint32_t t, u
   B.tr_leaf[k] = w;
   B.tr_cover[k] = (float)H;
   B.leaf_val[gid] = w * P.lr;
-  const uint32_t slot = B.pslot[gid];
   if (slot != kNone) {
     const uint32_t pos = atomicAdd(&B.ctl[t].n_free, 1u);
     B.free_slot[(uint64_t)t * P.SPT + pos] = slot;
     B.pslot[gid] = kNone;
   }
+}
+
+__device__ void make_leaf(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t p,
+                          uint32_t tn) {
+  const uint32_t gid = t * P.MN + p;
+  const int out = P.out0 + (int)t;
+  make_leaf_from(P, B, t, gid, tn, B.node_sum[2 * gid], B.node_sum[2 * gid + 1],
+                 B.qexp[2 * out], B.qexp[2 * out + 1], B.pslot[gid]);
 }
 
 // Depth-wise: the previous step's children become the level to plan.
@@ -248,6 +257,174 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
   }
 }
 
+// An expanded node (global id gid, tree node tn at depth dep): its tree
+// record, both children's tree nodes a and a + 1, partition item g (tiles
+// tile0 .. tile0 + ntile - 1, right child's partition id `right`) and the
+// record the children step reads back (tree, children's tree nodes, e, depth).
+// The split feature's original index, threshold (numerical splits) and
+// missing-value bin (kNone without missing values) are passed in loaded.
+__device__ void emit_expand(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t gid,
+                            uint32_t tn, uint32_t dep, const DevSplit& sp, double H,
+                            uint32_t count, uint32_t begin, uint32_t pbuf, uint32_t right,
+                            uint32_t a, uint32_t e, uint32_t g, uint32_t tile0, uint32_t ntile,
+                            uint32_t orig, float thr, uint32_t miss_bin) {
+  const uint32_t b = a + 1;
+  const uint64_t k = (uint64_t)t * P.MNT + tn;
+  B.tr_gain[k] = sp.gain;
+  B.tr_cover[k] = (float)H;
+  B.tr_split[k] = orig;
+  B.tr_left[k] = a;
+  B.tr_right[k] = b;
+  uint32_t lt, rt;
+  if (!(sp.flags & 2u)) {
+    B.tr_thr[k] = thr;
+    B.tr_dleft[k] = (uint8_t)(sp.flags & 1u);
+    B.tr_stype[k] = 0;
+    lt = a;
+    rt = b;
+  } else {
+    B.tr_thr[k] = 0.0f;
+    B.tr_dleft[k] = (sp.flags & 1u) ? 0 : 1;
+    B.tr_stype[k] = 1;
+    for (int w = 0; w < kMaxCatWords; w++) B.tr_cats[k * kMaxCatWords + w] = sp.cats[w];
+    lt = b;  // training-left rows take the inference-right child
+    rt = a;
+  }
+  init_tree_node(P, B, t, a);
+  init_tree_node(P, B, t, b);
+  PartItem it;
+  it.node = gid;
+  it.begin = begin;
+  it.count = count;
+  it.tile0 = tile0;
+  it.n_tiles = ntile;
+  it.right_node = t * P.MN + right;
+  it.feature = sp.feature;
+  it.pad = pbuf;
+  it.split_bin = sp.bin;
+  it.split_flags = sp.flags;
+  it.miss_bin = miss_bin;
+  it.pad2 = 0;
+  B.items[g] = it;
+  B.cursor[2 * g] = 0;
+  B.cursor[2 * g + 1] = 0;
+  B.pend[5 * g + 0] = t;
+  B.pend[5 * g + 1] = lt;
+  B.pend[5 * g + 2] = rt;
+  B.pend[5 * g + 3] = e;
+  B.pend[5 * g + 4] = dep + 1;
+}
+
+// Original index, threshold (numerical bins only; any bin reads in bounds)
+// and missing-value bin of the split feature.
+__device__ __forceinline__ void split_feature_info(const DeviceBufs& B, const DevSplit& sp,
+                                                   uint32_t* orig, float* thr,
+                                                   uint32_t* miss_bin) {
+  const uint32_t f = sp.feature;
+  *orig = B.eff_to_orig[f];
+  *thr = B.thr_up[(uint64_t)f * 256 + (sp.bin & 255u)];
+  const FeatMeta fm = B.fmeta[f];
+  *miss_bin = fm.has_missing ? fm.default_bin : kNone;
+}
+
+// The best split of a listed node from its packed key (best gain and winning
+// feature under the tie rule); no candidate (key 0) gives an invalid split.
+__device__ __forceinline__ DevSplit split_from_key(const CtlParams& P, const DeviceBufs& B,
+                                                   uint32_t node, unsigned long long key) {
+  DevSplit sp;
+  if (key == 0) {
+    sp.gain = -INFINITY;
+    sp.feature = 0;
+    sp.bin = 0;
+    sp.flags = 0;
+    for (int w = 0; w < kMaxCatWords; w++) sp.cats[w] = 0;
+    sp.lg = 0;
+    sp.lh = 0;
+  } else {
+    const uint32_t lo = (uint32_t)(key & 0xFFFFFFFFu);
+    const uint32_t f = P.rightmost ? lo : (0xFFFFFFFFu - lo);
+    sp = B.cand[(uint64_t)node * P.E + f];
+  }
+  return sp;
+}
+
+// Depth-wise growth of one tree.  The children step lists a level's nodes in
+// the same order in the split list and in the next-level list (entries 2i,
+// 2i + 1 are the children of item i in both), so thread i carries node i from
+// its split candidates to its tree nodes and partition item in registers,
+// with every load that does not depend on the split issued together.
+// Returns false, having written nothing, when the two lists do not pair up
+// or do not fit one block; the general path then runs.
+__device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int advance,
+                                  uint32_t n_sp, uint32_t* s_tmp, uint32_t* s_any,
+                                  uint32_t* s_tfirst, uint32_t* s_tbegin, uint32_t* s_tcount,
+                                  uint32_t* s_items, uint32_t* s_tiles) {
+  const uint32_t i = threadIdx.x;
+  const TreeCtl c0 = B.ctl[0];
+  const int qg = B.qexp[2 * P.out0], qh = B.qexp[2 * P.out0 + 1];
+  const uint32_t nn = advance ? c0.n_next : c0.n_open;
+  if (nn > (uint32_t)kCtlThreads || n_sp > (uint32_t)kCtlThreads) return false;
+  // Every thread loads (entry 0 past the lists; each list holds at least
+  // one entry) so that the loads issue together.
+  const bool in = i < nn, ins = i < n_sp;
+  const uint32_t li = in ? i : 0u;
+  const uint32_t p0 = (advance ? B.next_node : B.open_node)[li];
+  const uint32_t tn = (advance ? B.next_tnode : B.open_tnode)[li];
+  const uint32_t dep = (advance ? B.next_depth : B.open_depth)[li];
+  const uint32_t sn = B.split_nodes[ins ? i : 0u];
+  // One tree: partition ids are global ids.
+  const unsigned long long wb = __ballot(in != ins || (in && sn != p0));
+  if ((i & 63) == 0) s_any[i >> 6] = wb != 0ull ? 1u : 0u;
+  __syncthreads();
+  uint32_t any = 0;
+#pragma unroll
+  for (int w = 0; w < kCtlThreads / 64; w++) any |= s_any[w];
+  if (any) return false;
+  const uint32_t p = in ? p0 : 0u;
+  const unsigned long long key = B.node_key[p];
+  const uint32_t count = B.node_count[p], begin = B.pbegin[p], pbuf = B.pbuf[p];
+  const uint32_t slot = B.pslot[p];
+  const long long sg = B.node_sum[2 * p], sh = B.node_sum[2 * p + 1];
+  const DevSplit sp = split_from_key(P, B, p, key);
+  // Split-dependent loads ahead of the stores and the scans' barriers.
+  uint32_t orig, miss;
+  float thr;
+  split_feature_info(B, sp, &orig, &thr, &miss);
+  if (in) {
+    B.node_key[p] = 0;
+    B.node_split[p] = sp;
+  }
+  const uint32_t depth = c0.depth + (advance ? 1u : 0u);
+  const bool act = in && depth < P.max_depth;
+  const bool ex = act && expandable(sp, P.min_gain);
+  const uint32_t ntile = ex ? max(1u, (count + P.part_tile - 1) / P.part_tile) : 0u;
+  uint32_t tot_e, tot_t;
+  const uint32_t e = block_excl_scan(ex ? 1u : 0u, s_tmp, &tot_e);
+  const uint32_t toff = block_excl_scan(ntile, s_tmp, &tot_t);
+  if (act && !ex) make_leaf_from(P, B, 0, p, tn, sg, sh, qg, qh, slot);
+  if (ex) {
+    emit_expand(P, B, 0, p, tn, dep, sp, ldexp((double)sh, -qh), count, begin, pbuf,
+                c0.n_part + e, c0.n_tnodes + 2 * e, e, e, toff, ntile, orig, thr, miss);
+    s_tfirst[e] = toff;  // e < kCtlThreads <= kPlanLdsItems
+    s_tbegin[e] = begin;
+    s_tcount[e] = count;
+  }
+  if (i == 0) {
+    // The level is consumed (advance_level + the step's updates); n_free is
+    // left to the leaves' atomics.
+    TreeCtl& c = B.ctl[0];
+    c.depth = depth;
+    c.n_open = 0;
+    c.n_next = 0;
+    c.n_part = c0.n_part + tot_e;
+    c.n_tnodes = c0.n_tnodes + 2 * tot_e;
+    *s_items = tot_e;
+    *s_tiles = tot_t;
+  }
+  __syncthreads();
+  return true;
+}
+
 // Pick each listed node's best split from its per-feature candidates (the
 // packed key holds the best gain and the winning feature under the tie
 // rule), then choose the nodes to expand this step, decide leaf / split,
@@ -262,197 +439,143 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   // block at the end: first tile, row range of each item.
   __shared__ uint32_t s_tfirst[kPlanLdsItems], s_tbegin[kPlanLdsItems], s_tcount[kPlanLdsItems];
   __shared__ uint32_t s_tlds_end;
+  __shared__ uint32_t s_any[kCtlThreads / 64];
   if (threadIdx.x == 0) s_tlds_end = kNone;
   const uint32_t n_sp = B.counts[C_SPLIT];
-  for (uint32_t i = threadIdx.x; i < n_sp; i += blockDim.x) {
-    const uint32_t node = B.split_nodes[i];
-    const unsigned long long key = B.node_key[node];
-    B.node_key[node] = 0;
-    DevSplit sp;
-    if (key == 0) {
-      sp.gain = -INFINITY;
-      sp.feature = 0;
-      sp.bin = 0;
-      sp.flags = 0;
-      for (int w = 0; w < kMaxCatWords; w++) sp.cats[w] = 0;
-      sp.lg = 0;
-      sp.lh = 0;
-    } else {
-      const uint32_t lo = (uint32_t)(key & 0xFFFFFFFFu);
-      const uint32_t f = P.rightmost ? lo : (0xFFFFFFFFu - lo);
-      sp = B.cand[(uint64_t)node * P.E + f];
-    }
-    B.node_split[node] = sp;
-  }
-  __syncthreads();
-  if (advance && P.growth == 0) advance_level(P, B);
-  if (threadIdx.x == 0) {
-    s_items = 0;
-    s_tiles = 0;
-  }
-  __syncthreads();
-  for (uint32_t t = 0; t < P.T; t++) {
-    const uint64_t ob = (uint64_t)t * P.WO, cb = (uint64_t)t * P.W;
-    if (P.growth == 1) {
-      // Block-parallel argmax of (gain, partition id) over the open set.
-      const uint32_t n_open = B.ctl[t].n_open;
-      // Partition ids are unique in the open set, so the key picks one node.
-      unsigned long long best = 0ull;
-      for (uint32_t i = threadIdx.x; i < n_open; i += blockDim.x) {
-        const uint32_t p = B.open_node[ob + i];
-        const float g = B.node_split[t * P.MN + p].gain + 0.0f;  // -0 ties with +0
-        const unsigned long long key = ((unsigned long long)ord_gain(g) << 32) | p;
-        best = key > best ? key : best;
-      }
-      best = block_max_u64(best, s_max64);
-      for (uint32_t i = threadIdx.x; i < n_open; i += blockDim.x)
-        if (B.open_node[ob + i] == (uint32_t)(best & 0xFFFFFFFFull)) s_best = i;
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      TreeCtl& c = B.ctl[t];
-      uint32_t nch = 0;
-      if (P.growth == 0) {
-        if (c.depth < P.max_depth) nch = c.n_open;
-      } else if (c.n_leaves < P.max_leaves && c.n_open > 0) {
-        // Max (gain, partition id): BinaryHeap order of NodeCandidate; the
-        // block found it (s_best) before this thread-0 section.
-        const uint32_t best = s_best;
-        B.ch_node[cb] = B.open_node[ob + best];
-        B.ch_tnode[cb] = B.open_tnode[ob + best];
-        B.ch_depth[cb] = B.open_depth[ob + best];
-        const uint32_t last = c.n_open - 1;
-        B.open_node[ob + best] = B.open_node[ob + last];
-        B.open_tnode[ob + best] = B.open_tnode[ob + last];
-        B.open_depth[ob + best] = B.open_depth[ob + last];
-        c.n_open = last;
-        c.n_leaves += 1;
-        nch = 1;
-      }
-      s_nch = nch;
-      s_part = c.n_part;
-      s_tnodes = c.n_tnodes;
+  const bool paired = P.growth == 0 && P.T == 1 &&
+                      plan_level_paired(P, B, advance, n_sp, s_tmp, s_any, s_tfirst,
+                                        s_tbegin, s_tcount, &s_items, &s_tiles);
+  if (!paired) {
+    for (uint32_t i = threadIdx.x; i < n_sp; i += blockDim.x) {
+      const uint32_t node = B.split_nodes[i];
+      const unsigned long long key = B.node_key[node];
+      B.node_key[node] = 0;
+      B.node_split[node] = split_from_key(P, B, node, key);
     }
     __syncthreads();
-    const uint32_t nch = s_nch;
-    if (P.growth == 0) {
-      for (uint32_t i = threadIdx.x; i < nch; i += blockDim.x) {
-        B.ch_node[cb + i] = B.open_node[ob + i];
-        B.ch_tnode[cb + i] = B.open_tnode[ob + i];
-        B.ch_depth[cb + i] = B.open_depth[ob + i];
+    if (advance && P.growth == 0) advance_level(P, B);
+    if (threadIdx.x == 0) {
+      s_items = 0;
+      s_tiles = 0;
+    }
+    __syncthreads();
+    for (uint32_t t = 0; t < P.T; t++) {
+      const uint64_t ob = (uint64_t)t * P.WO, cb = (uint64_t)t * P.W;
+      if (P.growth == 1) {
+        // Block-parallel argmax of (gain, partition id) over the open set.
+        const uint32_t n_open = B.ctl[t].n_open;
+        // Partition ids are unique in the open set, so the key picks one node.
+        unsigned long long best = 0ull;
+        for (uint32_t i = threadIdx.x; i < n_open; i += blockDim.x) {
+          const uint32_t p = B.open_node[ob + i];
+          const float g = B.node_split[t * P.MN + p].gain + 0.0f;  // -0 ties with +0
+          const unsigned long long key = ((unsigned long long)ord_gain(g) << 32) | p;
+          best = key > best ? key : best;
+        }
+        best = block_max_u64(best, s_max64);
+        for (uint32_t i = threadIdx.x; i < n_open; i += blockDim.x)
+          if (B.open_node[ob + i] == (uint32_t)(best & 0xFFFFFFFFull)) s_best = i;
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        TreeCtl& c = B.ctl[t];
+        uint32_t nch = 0;
+        if (P.growth == 0) {
+          if (c.depth < P.max_depth) nch = c.n_open;
+        } else if (c.n_leaves < P.max_leaves && c.n_open > 0) {
+          // Max (gain, partition id): BinaryHeap order of NodeCandidate; the
+          // block found it (s_best) before this thread-0 section.
+          const uint32_t best = s_best;
+          B.ch_node[cb] = B.open_node[ob + best];
+          B.ch_tnode[cb] = B.open_tnode[ob + best];
+          B.ch_depth[cb] = B.open_depth[ob + best];
+          const uint32_t last = c.n_open - 1;
+          B.open_node[ob + best] = B.open_node[ob + last];
+          B.open_tnode[ob + best] = B.open_tnode[ob + last];
+          B.open_depth[ob + best] = B.open_depth[ob + last];
+          c.n_open = last;
+          c.n_leaves += 1;
+          nch = 1;
+        }
+        s_nch = nch;
+        s_part = c.n_part;
+        s_tnodes = c.n_tnodes;
       }
       __syncthreads();
-    }
-    uint32_t carry_e = 0, carry_t = 0;
-    for (uint32_t base = 0; base < nch; base += blockDim.x) {
-      const uint32_t i = base + threadIdx.x;
-      const bool in = i < nch;
-      uint32_t p = 0, tn = 0, dep = 0, count = 0;
-      DevSplit sp;
-      bool ex = false;
-      if (in) {
-        p = B.ch_node[cb + i];
-        tn = B.ch_tnode[cb + i];
-        dep = B.ch_depth[cb + i];
-        sp = B.node_split[t * P.MN + p];
-        ex = expandable(sp, P.min_gain);
-        count = B.node_count[t * P.MN + p];
+      const uint32_t nch = s_nch;
+      if (P.growth == 0) {
+        for (uint32_t i = threadIdx.x; i < nch; i += blockDim.x) {
+          B.ch_node[cb + i] = B.open_node[ob + i];
+          B.ch_tnode[cb + i] = B.open_tnode[ob + i];
+          B.ch_depth[cb + i] = B.open_depth[ob + i];
+        }
+        __syncthreads();
       }
-      const uint32_t ntile = ex ? max(1u, (count + P.part_tile - 1) / P.part_tile) : 0u;
-      uint32_t tot_e, tot_t;
-      const uint32_t e = block_excl_scan(ex ? 1u : 0u, s_tmp, &tot_e) + carry_e;
-      const uint32_t toff = block_excl_scan(ntile, s_tmp, &tot_t) + carry_t;
-      if (in && !ex) make_leaf(P, B, t, p, tn);
-      if (in && ex) {
-        const uint32_t gid = t * P.MN + p;
-        const int out = P.out0 + (int)t;
-        const double H = ldexp((double)B.node_sum[2 * gid + 1], -B.qexp[2 * out + 1]);
-        const uint32_t right = s_part + e;
-        const uint32_t rgid = t * P.MN + right;
-        const uint32_t a = s_tnodes + 2 * e, b = a + 1;
-        const uint64_t k = (uint64_t)t * P.MNT + tn;
-        const uint32_t f = sp.feature;
-        B.tr_gain[k] = sp.gain;
-        B.tr_cover[k] = (float)H;
-        B.tr_split[k] = B.eff_to_orig[f];
-        B.tr_left[k] = a;
-        B.tr_right[k] = b;
-        uint32_t lt, rt;
-        if (!(sp.flags & 2u)) {
-          B.tr_thr[k] = B.thr_up[(uint64_t)f * 256 + sp.bin];
-          B.tr_dleft[k] = (uint8_t)(sp.flags & 1u);
-          B.tr_stype[k] = 0;
-          lt = a;
-          rt = b;
-        } else {
-          B.tr_thr[k] = 0.0f;
-          B.tr_dleft[k] = (sp.flags & 1u) ? 0 : 1;
-          B.tr_stype[k] = 1;
-          for (int w = 0; w < kMaxCatWords; w++) B.tr_cats[k * kMaxCatWords + w] = sp.cats[w];
-          lt = b;  // training-left rows take the inference-right child
-          rt = a;
+      uint32_t carry_e = 0, carry_t = 0;
+      for (uint32_t base = 0; base < nch; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const bool in = i < nch;
+        uint32_t p = 0, tn = 0, dep = 0, count = 0;
+        DevSplit sp;
+        bool ex = false;
+        if (in) {
+          p = B.ch_node[cb + i];
+          tn = B.ch_tnode[cb + i];
+          dep = B.ch_depth[cb + i];
+          sp = B.node_split[t * P.MN + p];
+          ex = expandable(sp, P.min_gain);
+          count = B.node_count[t * P.MN + p];
         }
-        init_tree_node(P, B, t, a);
-        init_tree_node(P, B, t, b);
-        const uint32_t g = s_items + e;
-        const uint32_t begin = B.pbegin[gid];
-        PartItem it;
-        it.node = gid;
-        it.begin = begin;
-        it.count = count;
-        it.tile0 = s_tiles + toff;
-        it.n_tiles = ntile;
-        it.right_node = rgid;
-        it.feature = f;
-        it.pad = B.pbuf[gid];
-        {
-          const FeatMeta fm = B.fmeta[f];
-          it.split_bin = sp.bin;
-          it.split_flags = sp.flags;
-          it.miss_bin = fm.has_missing ? fm.default_bin : 0xFFFFFFFFu;
-          it.pad2 = 0;
-        }
-        B.items[g] = it;
-        B.cursor[2 * g] = 0;
-        B.cursor[2 * g + 1] = 0;
-        B.pend[5 * g + 0] = t;
-        B.pend[5 * g + 1] = lt;
-        B.pend[5 * g + 2] = rt;
-        B.pend[5 * g + 3] = e;
-        B.pend[5 * g + 4] = dep + 1;
-        if (g < kPlanLdsItems) {
-          s_tfirst[g] = s_tiles + toff;
-          s_tbegin[g] = begin;
-          s_tcount[g] = count;
-        } else {
-          if (g == kPlanLdsItems) s_tlds_end = s_tiles + toff;
-          for (uint32_t q = 0; q < ntile; q++) {
-            const uint32_t b0 = q * P.part_tile;
-            PartTile pt;
-            pt.item = g;
-            pt.begin = begin + b0;
-            pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
-            pt.pad = 0;
-            B.tiles[s_tiles + toff + q] = pt;
+        const uint32_t ntile = ex ? max(1u, (count + P.part_tile - 1) / P.part_tile) : 0u;
+        uint32_t tot_e, tot_t;
+        const uint32_t e = block_excl_scan(ex ? 1u : 0u, s_tmp, &tot_e) + carry_e;
+        const uint32_t toff = block_excl_scan(ntile, s_tmp, &tot_t) + carry_t;
+        if (in && !ex) make_leaf(P, B, t, p, tn);
+        if (in && ex) {
+          const uint32_t gid = t * P.MN + p;
+          const int out = P.out0 + (int)t;
+          const double H = ldexp((double)B.node_sum[2 * gid + 1], -B.qexp[2 * out + 1]);
+          const uint32_t g = s_items + e;
+          const uint32_t begin = B.pbegin[gid];
+          uint32_t orig, miss;
+          float thr;
+          split_feature_info(B, sp, &orig, &thr, &miss);
+          emit_expand(P, B, t, gid, tn, dep, sp, H, count, begin, B.pbuf[gid], s_part + e,
+                      s_tnodes + 2 * e, e, g, s_tiles + toff, ntile, orig, thr, miss);
+          if (g < kPlanLdsItems) {
+            s_tfirst[g] = s_tiles + toff;
+            s_tbegin[g] = begin;
+            s_tcount[g] = count;
+          } else {
+            if (g == kPlanLdsItems) s_tlds_end = s_tiles + toff;
+            for (uint32_t q = 0; q < ntile; q++) {
+              const uint32_t b0 = q * P.part_tile;
+              PartTile pt;
+              pt.item = g;
+              pt.begin = begin + b0;
+              pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
+              pt.pad = 0;
+              B.tiles[s_tiles + toff + q] = pt;
+            }
           }
         }
+        carry_e += tot_e;
+        carry_t += tot_t;
       }
-      carry_e += tot_e;
-      carry_t += tot_t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      TreeCtl& c = B.ctl[t];
-      c.n_part += carry_e;
-      c.n_tnodes += 2 * carry_e;
-      if (P.growth == 0) {
-        c.n_open = 0;
-        c.n_next = 0;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        TreeCtl& c = B.ctl[t];
+        c.n_part += carry_e;
+        c.n_tnodes += 2 * carry_e;
+        if (P.growth == 0) {
+          c.n_open = 0;
+          c.n_next = 0;
+        }
+        s_items += carry_e;
+        s_tiles += carry_t;
       }
-      s_items += carry_e;
-      s_tiles += carry_t;
+      __syncthreads();
     }
-    __syncthreads();
   }
   // Tiles of the items recorded in LDS: tile k belongs to the last item g
   // with s_tfirst[g] <= k (every item has at least one tile).
