@@ -601,6 +601,40 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   }
 }
 
+// One item of a children step and its parent's state: pending record (tree,
+// children's tree nodes, index in the tree, children's depth), the item, its
+// left count, and the parent's left sums, totals, first row and slot.
+struct ChildIn {
+  PartItem it;
+  uint32_t t, lt, rt, e, cdepth, L, R, pb, ps;
+  long long lg, lh, pg, ph;
+};
+
+// Nothing is loaded for an invalid item (all fields zero).  Loading the
+// parent's state early is safe: an item's writes touch only its own parent
+// and right child, and right children are fresh ids.
+__device__ __forceinline__ ChildIn child_load(const DeviceBufs& B, uint32_t i, bool valid) {
+  ChildIn c{};
+  if (valid) {
+    c.t = B.pend[5 * i];
+    c.lt = B.pend[5 * i + 1];
+    c.rt = B.pend[5 * i + 2];
+    c.e = B.pend[5 * i + 3];
+    c.cdepth = B.pend[5 * i + 4];
+    c.it = B.items[i];
+    c.L = B.cursor[2 * i];
+    c.R = c.it.count - c.L;
+    const uint32_t parent = c.it.node;
+    c.lg = B.node_split[parent].lg;
+    c.lh = B.node_split[parent].lh;
+    c.pg = B.node_sum[2 * parent];
+    c.ph = B.node_sum[2 * parent + 1];
+    c.pb = B.pbegin[parent];
+    c.ps = B.pslot[parent];
+  }
+  return c;
+}
+
 // After the partition: child counts, slots, the smaller child's histogram
 // items, the finish list (partials + subtraction), the split list, and the
 // children pushed to the open set (leaf-wise) or the next level (depth-wise).
@@ -616,6 +650,11 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
       s_hbegin[kChildLdsItems], s_hcount[kChildLdsItems], s_htree[kChildLdsItems];
   const uint32_t n_items = B.counts[C_ITEMS];
   const bool lds_items = n_items <= kChildLdsItems;
+  // One pass covers every item (the usual case): each thread loads its
+  // item's state once, first.
+  const bool one = n_items <= (uint32_t)kCtlThreads;
+  ChildIn pre{};
+  if (one) pre = child_load(B, threadIdx.x, threadIdx.x < n_items);
   if (threadIdx.x < P.T) {
     s_cnt[threadIdx.x] = 0;
     s_free[threadIdx.x] = B.ctl[threadIdx.x].n_free;
@@ -628,15 +667,24 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
   // Rows this step histograms (smaller children that will be split), which
   // sets the step's tile size.
   unsigned long long step_rows = 0ull;
-  if (need)
-    for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) {
-      const uint32_t L = B.cursor[2 * i], R = B.items[i].count - L;
-      step_rows += L <= R ? L : R;
+  if (need) {
+    if (one) {
+      if (threadIdx.x < n_items) step_rows = min(pre.L, pre.R);
+    } else {
+      for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) {
+        const uint32_t L = B.cursor[2 * i], R = B.items[i].count - L;
+        step_rows += L <= R ? L : R;
+      }
     }
+  }
   const uint32_t tile = step_tile(P, block_sum_u64(step_rows, s_red64));
   // Items per tree: the plan numbers a tree's items e = 0, 1, ... in order.
-  for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x)
-    atomicMax(&s_cnt[B.pend[5 * i]], B.pend[5 * i + 3] + 1u);
+  if (one) {
+    if (threadIdx.x < n_items) atomicMax(&s_cnt[pre.t], pre.e + 1u);
+  } else {
+    for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x)
+      atomicMax(&s_cnt[B.pend[5 * i]], B.pend[5 * i + 3] + 1u);
+  }
   __syncthreads();
   // Work lists are laid out by item order: histogram items by an exclusive
   // scan of each child's tile count, finish entries by a scan over the items
@@ -647,20 +695,14 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
   for (uint32_t base = 0; base < n_items; base += blockDim.x) {
     const uint32_t i = base + threadIdx.x;
     const bool in = i < n_items;
-    uint32_t t = 0, lt = 0, rt = 0, e = 0, cdepth = 0, L = 0, R = 0, scount = 0, nt = 0;
-    PartItem it{};
-    if (in) {
-      t = B.pend[5 * i];
-      lt = B.pend[5 * i + 1];
-      rt = B.pend[5 * i + 2];
-      e = B.pend[5 * i + 3];
-      cdepth = B.pend[5 * i + 4];
-      it = B.items[i];
-      L = B.cursor[2 * i];
-      R = it.count - L;
-      scount = L <= R ? L : R;
-      nt = need ? (scount + tile - 1) / tile : 0u;
-    }
+    const ChildIn c = one ? pre : child_load(B, i, in);
+    const uint32_t t = c.t, lt = c.lt, rt = c.rt, e = c.e, cdepth = c.cdepth, L = c.L, R = c.R;
+    const PartItem& it = c.it;
+    const uint32_t scount = L <= R ? L : R;
+    const uint32_t nt = in && need ? (scount + tile - 1) / tile : 0u;
+    // The small child's histogram slot, loaded ahead of the scans.
+    const uint32_t ss =
+        in && need ? B.free_slot[(uint64_t)t * P.SPT + (s_free[t] - 1u - e)] : kNone;
     uint32_t tot;
     const uint32_t h0 = block_excl_scan(nt * P.NG, s_tmp, &tot) + carry_h;
     carry_h += tot;
@@ -673,28 +715,24 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
       const uint32_t parent = it.node, right = it.right_node;
       B.node_count[parent] = L;
       B.node_count[right] = R;
-      {  // children's exact totals: the split's left sums, and the rest
-        const long long lg = B.node_split[parent].lg, lh = B.node_split[parent].lh;
-        const long long pg = B.node_sum[2 * parent], ph = B.node_sum[2 * parent + 1];
-        B.node_sum[2 * parent] = lg;
-        B.node_sum[2 * parent + 1] = lh;
-        B.node_sum[2 * right] = pg - lg;
-        B.node_sum[2 * right + 1] = ph - lh;
-      }
+      // Children's exact totals: the split's left sums, and the rest.
+      B.node_sum[2 * parent] = c.lg;
+      B.node_sum[2 * parent + 1] = c.lh;
+      B.node_sum[2 * right] = c.pg - c.lg;
+      B.node_sum[2 * right + 1] = c.ph - c.lh;
       const uint8_t cbuf = (uint8_t)((it.pad & 1u) ^ 1u);  // the partition wrote the other buffer
       B.pbuf[parent] = cbuf;
       B.pbuf[right] = cbuf;
-      const uint32_t pb = B.pbegin[parent];
+      const uint32_t pb = c.pb;
       B.pbegin[right] = pb + L;
       const bool left_small = L <= R;
       const uint32_t small = left_small ? parent : right, large = left_small ? right : parent;
       const uint32_t sbegin = left_small ? pb : pb + L;
-      const uint32_t ps = B.pslot[parent];
+      const uint32_t ps = c.ps;
       B.pslot[large] = ps;
       if (need) {
-        const uint32_t ss = B.free_slot[(uint64_t)t * P.SPT + (s_free[t] - 1u - e)];
         B.pslot[small] = ss;
-        const uint32_t gss = t * P.SPT + ss, gls = t * P.SPT + ps;
+        const uint32_t gss = t * P.SPT + ss;
         hrows += scount;
         const uint32_t hflags = t | (nt == 1 ? kDirectFlag : 0u) | (cbuf ? kAltFlag : 0u);
         if (lds_items) {  // written by the whole block below
