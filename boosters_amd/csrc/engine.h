@@ -73,7 +73,7 @@ struct PartTile {
   uint32_t item;   // index into PartItem list
   uint32_t begin;  // absolute first position
   uint32_t count;
-  uint32_t pad;
+  uint32_t src;    // device planner: the item's source buffer (PartItem.pad bit 0); host: 0
 };
 
 // Per-node segment for sums / leaf updates.
@@ -231,7 +231,9 @@ void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items,
 // caller; the final left cursor is the left count).  Row order inside a node
 // is not kept (no result depends on it: all sums are exact integers; the
 // children's totals come from the split's left sums).  Counts read from
-// dyn_tiles.
+// dyn_tiles; tiles[k].src names the source buffer; grid_tiles must not
+// exceed the tile buffer's capacity (each block reads its first tile ahead
+// of the count).
 constexpr uint32_t kFusedTileRows = 4096;  // rows per tile of launch_partition_fused
 void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
                             const PartTile* tiles, int grid_tiles, const DevSplit* node_split,

@@ -358,6 +358,7 @@ __device__ __forceinline__ DevSplit split_from_key(const CtlParams& P, const Dev
 __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int advance,
                                   uint32_t n_sp, uint32_t* s_tmp, uint32_t* s_any,
                                   uint32_t* s_tfirst, uint32_t* s_tbegin, uint32_t* s_tcount,
+                                  uint8_t* s_tsrc,
                                   uint32_t* s_items, uint32_t* s_tiles) {
   const uint32_t i = threadIdx.x;
   const TreeCtl c0 = B.ctl[0];
@@ -408,6 +409,7 @@ __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int a
     s_tfirst[e] = toff;  // e < kCtlThreads <= kPlanLdsItems
     s_tbegin[e] = begin;
     s_tcount[e] = count;
+    s_tsrc[e] = (uint8_t)pbuf;
   }
   if (i == 0) {
     // The level is consumed (advance_level + the step's updates); n_free is
@@ -438,13 +440,14 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   // Partition tiles of the first kPlanLdsItems items, written by the whole
   // block at the end: first tile, row range of each item.
   __shared__ uint32_t s_tfirst[kPlanLdsItems], s_tbegin[kPlanLdsItems], s_tcount[kPlanLdsItems];
+  __shared__ uint8_t s_tsrc[kPlanLdsItems];  // source buffer of the item's rows
   __shared__ uint32_t s_tlds_end;
   __shared__ uint32_t s_any[kCtlThreads / 64];
   if (threadIdx.x == 0) s_tlds_end = kNone;
   const uint32_t n_sp = B.counts[C_SPLIT];
   const bool paired = P.growth == 0 && P.T == 1 &&
                       plan_level_paired(P, B, advance, n_sp, s_tmp, s_any, s_tfirst,
-                                        s_tbegin, s_tcount, &s_items, &s_tiles);
+                                        s_tbegin, s_tcount, s_tsrc, &s_items, &s_tiles);
   if (!paired) {
     for (uint32_t i = threadIdx.x; i < n_sp; i += blockDim.x) {
       const uint32_t node = B.split_nodes[i];
@@ -536,16 +539,17 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
           const int out = P.out0 + (int)t;
           const double H = ldexp((double)B.node_sum[2 * gid + 1], -B.qexp[2 * out + 1]);
           const uint32_t g = s_items + e;
-          const uint32_t begin = B.pbegin[gid];
+          const uint32_t begin = B.pbegin[gid], pb = B.pbuf[gid];
           uint32_t orig, miss;
           float thr;
           split_feature_info(B, sp, &orig, &thr, &miss);
-          emit_expand(P, B, t, gid, tn, dep, sp, H, count, begin, B.pbuf[gid], s_part + e,
+          emit_expand(P, B, t, gid, tn, dep, sp, H, count, begin, pb, s_part + e,
                       s_tnodes + 2 * e, e, g, s_tiles + toff, ntile, orig, thr, miss);
           if (g < kPlanLdsItems) {
             s_tfirst[g] = s_tiles + toff;
             s_tbegin[g] = begin;
             s_tcount[g] = count;
+            s_tsrc[g] = (uint8_t)pb;
           } else {
             if (g == kPlanLdsItems) s_tlds_end = s_tiles + toff;
             for (uint32_t q = 0; q < ntile; q++) {
@@ -554,7 +558,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
               pt.item = g;
               pt.begin = begin + b0;
               pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
-              pt.pad = 0;
+              pt.src = pb;
               B.tiles[s_tiles + toff + q] = pt;
             }
           }
@@ -592,7 +596,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
     pt.item = lo;
     pt.begin = s_tbegin[lo] + b0;
     pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
-    pt.pad = 0;
+    pt.src = s_tsrc[lo];
     B.tiles[k] = pt;
   }
   if (threadIdx.x == 0) {

@@ -1699,20 +1699,17 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
   __shared__ uint32_t s_base[2];
   const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  // The first tile is read with the count (the grid does not exceed the tile
+  // buffer), and the rows with the item: the tile names the source buffer.
+  const PartTile first = tiles[blockIdx.x];
   const uint32_t n_work = *dyn_tiles;
   for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
-    const PartTile t = tiles[wi];
-    const PartItem it = items[t.item];
-    const bool src1 = (it.pad & 1u) != 0;
+    const PartTile t = wi == blockIdx.x ? first : tiles[wi];
+    const bool src1 = (t.src & 1u) != 0;
     const uint32_t* sidx = src1 ? idx1 : idx0;
     const float2* sogh = src1 ? ogh1 : ogh0;
     uint32_t* didx = src1 ? idx0 : idx1;
     float2* dogh = src1 ? ogh0 : ogh1;
-    // The split decision comes with the item (goes_left on it): only a
-    // categorical split reads its category words from the node's split.
-    const uint32_t* cats = (it.split_flags & 2u) ? node_split[it.node].cats : nullptr;
-    const uint8_t* col = bins_cm + (int64_t)it.feature * n;
-    const uint32_t* hcol = hi_cm ? hi_cm + (int64_t)it.feature * hi_words : nullptr;
     uint32_t rid[kFusedRows];
     float2 gh[kFusedRows];
 #pragma unroll
@@ -1722,6 +1719,12 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
       rid[j] = sidx[pos];
       gh[j] = sogh[pos];
     }
+    const PartItem it = items[t.item];
+    // The split decision comes with the item (goes_left on it): only a
+    // categorical split reads its category words from the node's split.
+    const uint32_t* cats = (it.split_flags & 2u) ? node_split[it.node].cats : nullptr;
+    const uint8_t* col = bins_cm + (int64_t)it.feature * n;
+    const uint32_t* hcol = hi_cm ? hi_cm + (int64_t)it.feature * hi_words : nullptr;
     uint32_t bins[kFusedRows];
 #pragma unroll
     for (int j = 0; j < kFusedRows; j++) {
