@@ -214,7 +214,9 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
 // 0 .. n_feats-1.  minus (optional, per node entry): a slot whose histogram
 // the split subtracts from the entry's slot before scanning, writing the
 // difference back into the entry's slot (for its candidate features), or
-// ~0 for none.
+// ~0 for none; with kMinusNoWriteback set the difference is only scanned
+// (no later step reads the node's histogram).
+constexpr uint32_t kMinusNoWriteback = 0x80000000u;
 
 // Stable partition of every listed node (partition.rs:73-145): count, scan,
 // scatter, copy back, then children counts/sums.

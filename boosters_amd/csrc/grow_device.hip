@@ -769,12 +769,16 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
           B.fin[4 * fi + 2] = h0;
           B.fin[4 * fi + 3] = nt;
         }
+        // Depth-wise, the children's children at the depth limit get no
+        // histograms, so the large child's histogram is used only by its split.
+        const uint32_t msub =
+            gss | (P.growth == 0 && cdepth + 1u >= P.max_depth ? kMinusNoWriteback : 0u);
         B.split_nodes[2 * i] = parent;
         B.split_slots[2 * i] = t * P.SPT + (left_small ? ss : ps);
-        B.split_minus[2 * i] = left_small ? kNone : gss;
+        B.split_minus[2 * i] = left_small ? kNone : msub;
         B.split_nodes[2 * i + 1] = right;
         B.split_slots[2 * i + 1] = t * P.SPT + (left_small ? ps : ss);
-        B.split_minus[2 * i + 1] = left_small ? gss : kNone;
+        B.split_minus[2 * i + 1] = left_small ? msub : kNone;
       } else {
         B.pslot[small] = kNone;
       }

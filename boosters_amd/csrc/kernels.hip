@@ -1403,10 +1403,13 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
   for (uint32_t wy = blockIdx.y; wy < n_work; wy += gridDim.y) {  // block-uniform
     const uint32_t node = nodes[wy];
     const uint32_t slot = slots[wy];
-    const uint32_t mslot = minus ? minus[wy] : 0xFFFFFFFFu;
+    const uint32_t mraw = minus ? minus[wy] : 0xFFFFFFFFu;
+    const uint32_t mslot = mraw & ~kMinusNoWriteback;
+    const bool wback = (mraw & kMinusNoWriteback) == 0u;
     const uint32_t tree = node / max_nodes;
     long long* base = hist + (int64_t)slot * slot_elems;
-    const long long* mbase = mslot != 0xFFFFFFFFu ? hist + (int64_t)mslot * slot_elems : nullptr;
+    const long long* mbase =
+        mraw != 0xFFFFFFFFu ? hist + (int64_t)mslot * slot_elems : nullptr;
     const uint32_t* fl = feats ? feats + (int64_t)tree * feat_stride : nullptr;
     // This wave's feature and inputs.
     const bool fok = fi < n_feats;
@@ -1419,7 +1422,8 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     // of the block's feature t % kSplitWaves.  With a sibling to subtract,
     // the slot holds the parent's histogram: stage parent - sibling and
     // write it back, so the slot holds this node's histogram from now on
-    // (the block's features only; no other block touches them).
+    // (the block's features only; no other block touches them), unless no
+    // later step reads it.
     {
       const int w = threadIdx.x % kSplitWaves;
       const int fw = blockIdx.x * kSplitWaves + w;
@@ -1433,7 +1437,7 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
             const longlong2 m = *(const longlong2*)(mbase + e);
             v.x -= m.x;
             v.y -= m.y;
-            *(longlong2*)(base + e) = v;
+            if (wback) *(longlong2*)(base + e) = v;
             s_bins[w][b] = v;
           }
         } else {
