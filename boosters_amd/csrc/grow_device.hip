@@ -109,11 +109,14 @@ __device__ void make_leaf_from(const CtlParams& P, const DeviceBufs& B, uint32_t
     const double ag = fabs(G);
     w = ag <= P.alpha ? 0.0f : (float)((G > 0.0 ? -1.0 : 1.0) * (ag - P.alpha) / (H + P.lambda));
   }
+  // The learning rate is applied here (apply_learning_rate scales every
+  // leaf by the same float product once the tree is done).
+  const float wl = w * P.lr;
   const uint64_t k = (uint64_t)t * P.MNT + tn;
   B.tr_isleaf[k] = 1;
-  B.tr_leaf[k] = w;
+  B.tr_leaf[k] = wl;
   B.tr_cover[k] = (float)H;
-  B.leaf_val[gid] = w * P.lr;
+  B.leaf_val[gid] = wl;
   if (slot != kNone) {
     const uint32_t pos = atomicAdd(&B.ctl[t].n_free, 1u);
     B.free_slot[(uint64_t)t * P.SPT + pos] = slot;
@@ -845,43 +848,63 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
 __global__ __launch_bounds__(kCtlThreads) void k_ctl_finalize(CtlParams P, DeviceBufs B,
                                                               int advance,
                                                               uint32_t* __restrict__ round) {
-  if (advance && P.growth == 0) advance_level(P, B);
-  if (threadIdx.x == 0) {
-    B.counts[C_LSEGS] = 0;
-    if (round) *round += 1u;  // nothing after this kernel in the round reads it
-  }
-  __syncthreads();
+  __shared__ uint32_t s_tmp[kCtlThreads / 64 + 1];
+  // Depth-wise, the last step's children (the next level) are what remains;
+  // they are read from the next-level list in place.
+  const bool lvl = advance && P.growth == 0;
+  uint32_t carry_s = 0;
   for (uint32_t t = 0; t < P.T; t++) {
-    const uint32_t no = B.ctl[t].n_open;
+    const TreeCtl c = B.ctl[t];
+    const uint32_t no = lvl ? c.n_next : c.n_open;
+    const uint64_t lb = lvl ? (uint64_t)t * 2 * P.W : (uint64_t)t * P.WO;
+    const uint32_t* onode = (lvl ? B.next_node : B.open_node) + lb;
+    const uint32_t* otn = (lvl ? B.next_tnode : B.open_tnode) + lb;
+    const int out = P.out0 + (int)t;
+    const int qg = B.qexp[2 * out], qh = B.qexp[2 * out + 1];
     for (uint32_t i = threadIdx.x; i < no; i += blockDim.x) {
-      const uint64_t ob = (uint64_t)t * P.WO + i;
-      make_leaf(P, B, t, B.open_node[ob], B.open_tnode[ob]);
+      const uint32_t gid = t * P.MN + onode[i];
+      // The tree is done: the next round's start resets every slot, so no
+      // slot is returned to the free list.
+      make_leaf_from(P, B, t, gid, otn[i], B.node_sum[2 * gid], B.node_sum[2 * gid + 1], qg, qh,
+                     kNone);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) B.ctl[t].n_open = 0;
-    const uint32_t ntn = B.ctl[t].n_tnodes;
-    for (uint32_t i = threadIdx.x; i < ntn; i += blockDim.x) {
-      const uint64_t k = (uint64_t)t * P.MNT + i;
-      if (B.tr_isleaf[k]) B.tr_leaf[k] = B.tr_leaf[k] * P.lr;
+    __syncthreads();  // leaf values of the leaves made above are visible
+    if (threadIdx.x == 0) {
+      TreeCtl& cw = B.ctl[t];
+      cw.n_open = 0;
+      if (lvl) {
+        cw.n_next = 0;
+        cw.depth = c.depth + 1;
+      }
     }
-    const uint32_t np = B.ctl[t].n_part;
-    for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
-      const uint32_t gid = t * P.MN + p;
-      if (isnan(B.leaf_val[gid])) continue;
+    // Leaf-range update list: every leaf's rows in seg_tile segments, laid
+    // out by a scan over the partition ids.
+    const uint32_t np = c.n_part;
+    for (uint32_t base = 0; base < np; base += kCtlThreads) {
+      const uint32_t p = base + threadIdx.x;
+      const bool in = p < np;
+      const uint32_t gid = t * P.MN + (in ? p : 0u);
+      const float lv = B.leaf_val[gid];
       const uint32_t cnt = B.node_count[gid], b = B.pbegin[gid];
-      const uint32_t ns = (cnt + P.seg_tile - 1) / P.seg_tile;
-      if (ns == 0) continue;
-      const uint32_t s0 = atomicAdd(&B.counts[C_LSEGS], ns);
+      const uint32_t alt = B.pbuf[gid] ? 0x80000000u : 0u;
+      const uint32_t ns = in && !isnan(lv) ? (cnt + P.seg_tile - 1) / P.seg_tile : 0u;
+      uint32_t tot;
+      const uint32_t s0 = block_excl_scan(ns, s_tmp, &tot) + carry_s;
+      carry_s += tot;
       for (uint32_t q = 0; q < ns; q++) {
         Segment sg;
         sg.node = gid;
         sg.begin = b + q * P.seg_tile;
         sg.count = min(P.seg_tile, cnt - q * P.seg_tile);
-        sg.tree = t | (B.pbuf[gid] ? 0x80000000u : 0u);
+        sg.tree = t | alt;
         B.leaf_segs[s0 + q] = sg;
       }
     }
     __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    B.counts[C_LSEGS] = carry_s;
+    if (round) *round += 1u;  // nothing after this kernel in the round reads it
   }
 }
 
