@@ -64,16 +64,26 @@ struct PartItem {
   // Device-driven partition: the node's split decision, copied by the planner
   // so that the partition need not load the split and the feature meta:
   uint32_t split_bin;    // left: bin <= split_bin (numerical)
-  uint32_t split_flags;  // DevSplit::flags (bit 0 default left, bit 1 categorical)
+  uint32_t split_flags;  // DevSplit::flags (bit 0 default left, bit 1 categorical), bit 2 below
   uint32_t miss_bin;     // the feature's missing bin, or ~0 without one
   uint32_t pad2;
+  // split_flags bit 2 (kItemLeaves): the children are leaves (depth-wise last
+  // step) and the partition adds their values to the predictions instead of
+  // moving the rows; values of the left (kept) and right child, learning rate
+  // applied.
+  float leaf_l, leaf_r;
+  uint32_t pad3, pad4;
 };
+constexpr uint32_t kItemLeaves = 4u;
+// PartTile::src bits: 1 = rows in the alternate buffer, 2 = the item applies
+// leaf values (kItemLeaves), so the tile needs no (g, h).
+constexpr uint32_t kTileAlt = 1u, kTileLeaves = 2u;
 
 struct PartTile {
   uint32_t item;   // index into PartItem list
   uint32_t begin;  // absolute first position
   uint32_t count;
-  uint32_t src;    // device planner: the item's source buffer (PartItem.pad bit 0); host: 0
+  uint32_t src;    // device planner: kTileAlt / kTileLeaves bits; host: 0
 };
 
 // Per-node segment for sums / leaf updates.
@@ -235,12 +245,16 @@ void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items,
 // children's totals come from the split's left sums).  Counts read from
 // dyn_tiles; tiles[k].src names the source buffer; grid_tiles must not
 // exceed the tile buffer's capacity (each block reads its first tile ahead
-// of the count).
+// of the count).  Items flagged kItemLeaves instead add leaf_l / leaf_r to
+// preds[tree_output[node / max_nodes] * n_rows + row] for each of their rows
+// (rows stay in place, cursors untouched).
 constexpr uint32_t kFusedTileRows = 4096;  // rows per tile of launch_partition_fused
 void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
                             const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
                             uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,
-                            uint32_t* cursor, const uint32_t* dyn_tiles);
+                            uint32_t* cursor, const uint32_t* dyn_tiles,
+                            const uint32_t* tree_output, float* preds, int64_t n_rows,
+                            uint32_t max_nodes);
 
 void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const float* leaf_val,
                         const uint32_t* idx, int64_t tree_stride_rows, float* preds,

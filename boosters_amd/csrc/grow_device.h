@@ -47,6 +47,9 @@ struct CtlParams {
   float lr;
   double lambda, alpha, min_gain;
   int32_t out0;         // output of tree slot 0 (qexp / gradient row)
+  // Depth-wise rounds over every row: the last step's partition adds the
+  // children's leaf values to the predictions (no leaf-range update for them).
+  uint32_t leaf_apply;
 };
 
 struct TreeCtl {

@@ -94,12 +94,10 @@ __device__ void init_tree_node(const CtlParams& P, const DeviceBufs& B, uint32_t
   for (int w = 0; w < kMaxCatWords; w++) B.tr_cats[k * kMaxCatWords + w] = 0;
 }
 
-// compute_leaf_weight (split/gain.rs:151-168) from exact fixed-point sums.
-// The node's sums (sg, sh), the round's scale exponents (qg, qh) and its
-// histogram slot are passed in already loaded.
-__device__ void make_leaf_from(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t gid,
-                               uint32_t tn, long long sg, long long sh, int qg, int qh,
-                               uint32_t slot) {
+// compute_leaf_weight (split/gain.rs:151-168) from exact fixed-point sums,
+// with the learning rate applied (apply_learning_rate scales every leaf by
+// the same float product once the tree is done).
+__device__ float leaf_weight(const CtlParams& P, long long sg, long long sh, int qg, int qh) {
   const double G = ldexp((double)sg, -qg);
   const double H = ldexp((double)sh, -qh);
   float w;
@@ -109,9 +107,16 @@ __device__ void make_leaf_from(const CtlParams& P, const DeviceBufs& B, uint32_t
     const double ag = fabs(G);
     w = ag <= P.alpha ? 0.0f : (float)((G > 0.0 ? -1.0 : 1.0) * (ag - P.alpha) / (H + P.lambda));
   }
-  // The learning rate is applied here (apply_learning_rate scales every
-  // leaf by the same float product once the tree is done).
-  const float wl = w * P.lr;
+  return w * P.lr;
+}
+
+// The node's sums (sg, sh), the round's scale exponents (qg, qh) and its
+// histogram slot are passed in already loaded.
+__device__ void make_leaf_from(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t gid,
+                               uint32_t tn, long long sg, long long sh, int qg, int qh,
+                               uint32_t slot) {
+  const float wl = leaf_weight(P, sg, sh, qg, qh);
+  const double H = ldexp((double)sh, -qh);
   const uint64_t k = (uint64_t)t * P.MNT + tn;
   B.tr_isleaf[k] = 1;
   B.tr_leaf[k] = wl;
@@ -266,11 +271,16 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
 // record the children step reads back (tree, children's tree nodes, e, depth).
 // The split feature's original index, threshold (numerical splits) and
 // missing-value bin (kNone without missing values) are passed in loaded.
+// With `leaves` (the children are leaves and the partition applies them),
+// the children's values come from the split's left sums and the parent's
+// totals (sg, sh), as the leaves made at the end of the tree would compute
+// them from the children's sums.
 __device__ void emit_expand(const CtlParams& P, const DeviceBufs& B, uint32_t t, uint32_t gid,
                             uint32_t tn, uint32_t dep, const DevSplit& sp, double H,
                             uint32_t count, uint32_t begin, uint32_t pbuf, uint32_t right,
                             uint32_t a, uint32_t e, uint32_t g, uint32_t tile0, uint32_t ntile,
-                            uint32_t orig, float thr, uint32_t miss_bin) {
+                            uint32_t orig, float thr, uint32_t miss_bin, bool leaves,
+                            long long sg, long long sh, int qg, int qh) {
   const uint32_t b = a + 1;
   const uint64_t k = (uint64_t)t * P.MNT + tn;
   B.tr_gain[k] = sp.gain;
@@ -305,9 +315,13 @@ __device__ void emit_expand(const CtlParams& P, const DeviceBufs& B, uint32_t t,
   it.feature = sp.feature;
   it.pad = pbuf;
   it.split_bin = sp.bin;
-  it.split_flags = sp.flags;
+  it.split_flags = sp.flags | (leaves ? kItemLeaves : 0u);
   it.miss_bin = miss_bin;
   it.pad2 = 0;
+  it.leaf_l = leaves ? leaf_weight(P, sp.lg, sp.lh, qg, qh) : 0.0f;
+  it.leaf_r = leaves ? leaf_weight(P, sg - sp.lg, sh - sp.lh, qg, qh) : 0.0f;
+  it.pad3 = 0;
+  it.pad4 = 0;
   B.items[g] = it;
   B.cursor[2 * g] = 0;
   B.cursor[2 * g + 1] = 0;
@@ -407,12 +421,14 @@ __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int a
   const uint32_t toff = block_excl_scan(ntile, s_tmp, &tot_t);
   if (act && !ex) make_leaf_from(P, B, 0, p, tn, sg, sh, qg, qh, slot);
   if (ex) {
+    const bool lv = P.leaf_apply && dep + 1u >= P.max_depth;
     emit_expand(P, B, 0, p, tn, dep, sp, ldexp((double)sh, -qh), count, begin, pbuf,
-                c0.n_part + e, c0.n_tnodes + 2 * e, e, e, toff, ntile, orig, thr, miss);
+                c0.n_part + e, c0.n_tnodes + 2 * e, e, e, toff, ntile, orig, thr, miss, lv, sg,
+                sh, qg, qh);
     s_tfirst[e] = toff;  // e < kCtlThreads <= kPlanLdsItems
     s_tbegin[e] = begin;
     s_tcount[e] = count;
-    s_tsrc[e] = (uint8_t)pbuf;
+    s_tsrc[e] = (uint8_t)((pbuf ? kTileAlt : 0u) | (lv ? kTileLeaves : 0u));
   }
   if (i == 0) {
     // The level is consumed (advance_level + the step's updates); n_free is
@@ -443,7 +459,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   // Partition tiles of the first kPlanLdsItems items, written by the whole
   // block at the end: first tile, row range of each item.
   __shared__ uint32_t s_tfirst[kPlanLdsItems], s_tbegin[kPlanLdsItems], s_tcount[kPlanLdsItems];
-  __shared__ uint8_t s_tsrc[kPlanLdsItems];  // source buffer of the item's rows
+  __shared__ uint8_t s_tsrc[kPlanLdsItems];  // PartTile::src bits of the item's tiles
   __shared__ uint32_t s_tlds_end;
   __shared__ uint32_t s_any[kCtlThreads / 64];
   if (threadIdx.x == 0) s_tlds_end = kNone;
@@ -540,19 +556,24 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
         if (in && ex) {
           const uint32_t gid = t * P.MN + p;
           const int out = P.out0 + (int)t;
-          const double H = ldexp((double)B.node_sum[2 * gid + 1], -B.qexp[2 * out + 1]);
+          const long long sg = B.node_sum[2 * gid], sh = B.node_sum[2 * gid + 1];
+          const int qg = B.qexp[2 * out], qh = B.qexp[2 * out + 1];
+          const double H = ldexp((double)sh, -qh);
           const uint32_t g = s_items + e;
           const uint32_t begin = B.pbegin[gid], pb = B.pbuf[gid];
+          const bool lv = P.leaf_apply && P.growth == 0 && dep + 1u >= P.max_depth;
+          const uint32_t src = (pb ? kTileAlt : 0u) | (lv ? kTileLeaves : 0u);
           uint32_t orig, miss;
           float thr;
           split_feature_info(B, sp, &orig, &thr, &miss);
           emit_expand(P, B, t, gid, tn, dep, sp, H, count, begin, pb, s_part + e,
-                      s_tnodes + 2 * e, e, g, s_tiles + toff, ntile, orig, thr, miss);
+                      s_tnodes + 2 * e, e, g, s_tiles + toff, ntile, orig, thr, miss, lv, sg, sh,
+                      qg, qh);
           if (g < kPlanLdsItems) {
             s_tfirst[g] = s_tiles + toff;
             s_tbegin[g] = begin;
             s_tcount[g] = count;
-            s_tsrc[g] = (uint8_t)pb;
+            s_tsrc[g] = (uint8_t)src;
           } else {
             if (g == kPlanLdsItems) s_tlds_end = s_tiles + toff;
             for (uint32_t q = 0; q < ntile; q++) {
@@ -561,7 +582,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
               pt.item = g;
               pt.begin = begin + b0;
               pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
-              pt.src = pb;
+              pt.src = src;
               B.tiles[s_tiles + toff + q] = pt;
             }
           }
@@ -720,8 +741,11 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
     carry_f += tot;
     if (in) {
       const uint32_t parent = it.node, right = it.right_node;
-      B.node_count[parent] = L;
-      B.node_count[right] = R;
+      // Children whose values the partition applied (the rows did not move)
+      // take no rows, so the end of the tree lists no leaf range for them.
+      const bool applied = (it.split_flags & kItemLeaves) != 0u;
+      B.node_count[parent] = applied ? 0u : L;
+      B.node_count[right] = applied ? 0u : R;
       // Children's exact totals: the split's left sums, and the rest.
       B.node_sum[2 * parent] = c.lg;
       B.node_sum[2 * parent + 1] = c.lh;

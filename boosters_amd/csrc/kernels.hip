@@ -1697,7 +1697,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
     const uint8_t* __restrict__ bins_cm, const uint32_t* __restrict__ hi_cm, int64_t n,
     int64_t hi_words, uint32_t* __restrict__ idx0, float2* __restrict__ ogh0,
     uint32_t* __restrict__ idx1, float2* __restrict__ ogh1, uint32_t* __restrict__ cursor,
-    const uint32_t* __restrict__ dyn_tiles) {
+    const uint32_t* __restrict__ dyn_tiles, const uint32_t* __restrict__ tree_output,
+    float* __restrict__ preds, int64_t n_rows, uint32_t max_nodes) {
   __shared__ uint32_t s_l[kFusedRows * kFusedWaves];  // lefts per (row pass, wave)
   __shared__ uint32_t s_v[kFusedRows * kFusedWaves];  // rows per (row pass, wave)
   __shared__ uint32_t s_base[2];
@@ -1709,7 +1710,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
   const uint32_t n_work = *dyn_tiles;
   for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
     const PartTile t = wi == blockIdx.x ? first : tiles[wi];
-    const bool src1 = (t.src & 1u) != 0;
+    const bool src1 = (t.src & kTileAlt) != 0;
+    const bool leaves = (t.src & kTileLeaves) != 0;  // block-uniform
     const uint32_t* sidx = src1 ? idx1 : idx0;
     const float2* sogh = src1 ? ogh1 : ogh0;
     uint32_t* didx = src1 ? idx0 : idx1;
@@ -1721,7 +1723,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
       const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
       const uint32_t pos = p < t.count ? t.begin + p : 0u;  // position 0 always exists
       rid[j] = sidx[pos];
-      gh[j] = sogh[pos];
+      gh[j] = leaves ? make_float2(0.0f, 0.0f) : sogh[pos];
     }
     const PartItem it = items[t.item];
     // The split decision comes with the item (goes_left on it): only a
@@ -1735,6 +1737,29 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
       uint32_t b = col[rid[j]];
       if (hcol) b |= ((hcol[rid[j] >> 5] >> (rid[j] & 31)) & 1u) << 8;
       bins[j] = b;
+    }
+    if (leaves) {
+      // The children are leaves: add their values to the rows' predictions
+      // (the rows of one tree are distinct, so the updates are independent;
+      // the same float add as the leaf-range update).
+      float* pr = preds + (int64_t)tree_output[it.node / max_nodes] * n_rows;
+      float x[kFusedRows];
+      bool l[kFusedRows];
+#pragma unroll
+      for (int j = 0; j < kFusedRows; j++) {
+        const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
+        const uint32_t bn = bins[j];
+        l[j] = bn == it.miss_bin ? (it.split_flags & 1u) != 0
+               : cats ? bn < 256u && ((cats[bn >> 5] >> (bn & 31)) & 1u)
+                      : bn <= it.split_bin;
+        x[j] = p < t.count ? pr[rid[j]] : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < kFusedRows; j++) {
+        const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
+        if (p < t.count) pr[rid[j]] = x[j] + (l[j] ? it.leaf_l : it.leaf_r);
+      }
+      continue;  // block-uniform; no shared state used
     }
     unsigned long long lbal[kFusedRows], vbal[kFusedRows];
 #pragma unroll
@@ -1800,11 +1825,13 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
 void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
                             const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
                             uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,
-                            uint32_t* cursor, const uint32_t* dyn_tiles) {
+                            uint32_t* cursor, const uint32_t* dyn_tiles,
+                            const uint32_t* tree_output, float* preds, int64_t n_rows,
+                            uint32_t max_nodes) {
   if (grid_tiles <= 0) return;
   hipLaunchKernelGGL(k_part_fused, dim3(grid_tiles), dim3(kFusedThreads), 0, s, items, tiles,
                      node_split, d.meta, d.bins_cm, d.hi_cm, d.n, d.hi_words, idx0, ogh0, idx1,
-                     ogh1, cursor, dyn_tiles);
+                     ogh1, cursor, dyn_tiles, tree_output, preds, n_rows, max_nodes);
 }
 
 void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items, int n_items,

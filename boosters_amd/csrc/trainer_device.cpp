@@ -294,7 +294,10 @@ void Trainer::device_setup() {
 // Every launch of one round (T trees = all K outputs), no host sync.
 void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled) {
   DevState& S = *dev_;
-  const CtlParams& P = S.P;
+  // Depth-wise rounds over every row apply the last level's leaf values in
+  // its partition; sampled rounds update every row by traversal instead.
+  CtlParams P = S.P;
+  P.leaf_apply = !sampled && P.growth == 0 ? 1u : 0u;
   const DeviceBufs& B = S.B;
   const auto& cfg = opt_.cfg;
   const int K = opt_.K;
@@ -350,7 +353,8 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
     launch_ctl_plan(s_, P, B, /*advance=*/step > 0);
     int pi = ctx_->prof_begin(2, 0.0);
     launch_partition_fused(s_, d, B.items, B.tiles, g_tiles, node_split_.p, idx_.p, ogh_.p,
-                           idx_tmp_.p, ogh_tmp_.p, B.cursor, cnt + C_TILES);
+                           idx_tmp_.p, ogh_tmp_.p, B.cursor, cnt + C_TILES, tree_output_.p,
+                           preds_.p, n_, P.MN);
     ctx_->prof_end(pi);
     launch_ctl_children(s_, P, B);
     // The last step's children are never split (depth limit, or the leaf
