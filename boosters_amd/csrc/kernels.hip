@@ -640,8 +640,11 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
   // Bins 0..255, plus row 256 that takes every add with nothing to count
   // (the NaN bin, rows past the tile end, padding features) so that the
   // adds need no branch; row 256 is never read.
-  __shared__ long long s_g[257 * FG];
-  __shared__ long long s_h[257 * FG];
+  // One array, g plane then h plane, so that an h add reaches its column
+  // with the g address plus an immediate offset.
+  __shared__ long long s_gh[2 * 257 * FG];
+  long long* const s_g = s_gh;
+  long long* const s_h = s_gh + 257 * FG;
   constexpr int kHistChunk = hist_chunk<FG>();
   __shared__ uint32_t s_row[kHistBufs][kHistChunk];
   __shared__ longlong2 s_q[kHistBufs][kHistChunk];
@@ -655,6 +658,8 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
   const uint32_t n_pos = ((n_tiles + 7u) / 8u) * 8u * (uint32_t)NG;
   const int lf = threadIdx.x % FG;
   const int rs = threadIdx.x / FG;
+  constexpr uint32_t kPlaneBytes = 257u * FG * 8u;  // g plane to h plane
+  char* const lane_g = reinterpret_cast<char*>(s_g + lf);
   for (uint32_t p = blockIdx.x; p < n_pos; p += gridDim.x) {
     const uint32_t tile = (p / (8u * NG)) * 8u + (p & 7u);
     const uint32_t grp = (p >> 3) % (uint32_t)NG;
@@ -673,6 +678,11 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
     // there, so they add into columns nobody reads.
     const int f = (int)it.group * FG + lf;
     const int hb = hi_off + (f >> 3), hs = f & 7;
+  // The NaN bit (bit hs of its byte) shifted to the bin-256 bit of the
+  // byte offset below: bit log2(256 * FG * 8).
+  constexpr int kHiBit = FG == 16 ? 15 : 16;
+  static_assert((1 << kHiBit) == 256 * FG * 8, "hi bit position");
+  const uint32_t hsh = (uint32_t)(kHiBit - hs);
     const double sg = ldexp(1.0, qexp[2 * tree]);
     const double sh = ldexp(1.0, qexp[2 * tree + 1]);
     const uint32_t* tidx = (alt ? idx_alt : idx) + (int64_t)tree * tree_stride;
@@ -767,10 +777,13 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
 #pragma unroll
         for (int k = 0; k < kHistRowBatch; k++) {
           const int u = u0 + k;
-          const uint32_t bin =
-              HAS_HI ? ((uint32_t)cb[u] | ((((uint32_t)ch[u] >> hs) & 1u) << 8)) : (uint32_t)cb[u];
-          atomicAdd((unsigned long long*)&s_g[bin * FG + lf], (unsigned long long)q[k].x);
-          atomicAdd((unsigned long long*)&s_h[bin * FG + lf], (unsigned long long)q[k].y);
+          // Byte offset of (bin, lf) in the g plane: bin * FG * 8 with
+          // bin = byte | NaN bit << 8, formed as two shifted adds.
+          uint32_t off = (uint32_t)cb[u] * (uint32_t)(FG * 8);
+          if (HAS_HI) off |= ((uint32_t)ch[u] << hsh) & (256u * FG * 8u);
+          char* const gp = lane_g + off;
+          atomicAdd((unsigned long long*)gp, (unsigned long long)q[k].x);
+          atomicAdd((unsigned long long*)(gp + kPlaneBytes), (unsigned long long)q[k].y);
         }
       }
       if (c + 2 < n_chunks) stage_store((int)((c + 2) % kHistBufs));
