@@ -108,14 +108,15 @@ Plan plan_step(const Shape& sh, uint32_t NG, uint32_t slots, int64_t n, uint32_t
     const double f = sh.spread ? 0.25 + 1.5 * (double)v / (double)(sh.nodes - 1) : 1.0;
     const uint32_t rows =
         (uint32_t)std::min<double>((double)n, std::max(1.0, std::floor(f * (double)sh.rows)));
-    const uint32_t nt = (rows + tile - 1) / tile;
+    const uint32_t nt = hist_node_tiles(rows, tile, kTileMax);
+    const uint32_t ts = hist_node_tile_rows(rows, nt);
     const uint32_t slot = v % slots;
     const uint64_t span = (uint64_t)std::max<int64_t>(1, n - (int64_t)rows + 1);
     const uint32_t begin = (uint32_t)(pos % span);  // consecutive ranges, wrapped inside n
     pos += rows;
     pl.fin.insert(pl.fin.end(), {slot, 0xFFFFFFFFu, (uint32_t)pl.items.size(), nt});
     for (uint32_t k = 0; k < nt; k++) {
-      const uint32_t b0 = k * tile, cnt = std::min(tile, rows - b0);
+      const uint32_t b0 = k * ts, cnt = std::min(ts, rows - b0);
       for (uint32_t g = 0; g < NG; g++)
         pl.items.push_back({v, slot, begin + b0, cnt, g, nt == 1 ? 0x80000000u : 0u});
     }

@@ -181,6 +181,24 @@ __host__ __device__ inline uint32_t hist_step_tile(uint32_t tile_min, uint32_t t
   return (uint32_t)tile;
 }
 
+// Tiles of one node of `rows` rows at a step's tile size: the nearest whole
+// number of tiles (one at least for a non-empty node; tiles of at most
+// tile_max rows), so that no node ends in a sliver tile that costs a block,
+// a partial histogram and its finish for a few rows.  Never more than
+// ceil(rows / tile), so work-list capacities sized for that hold.
+__host__ __device__ inline uint32_t hist_node_tiles(uint32_t rows, uint32_t tile,
+                                                    uint32_t tile_max) {
+  if (rows == 0) return 0;
+  uint32_t nt = (uint32_t)(((uint64_t)rows + tile / 2) / tile);
+  const uint32_t lo = (uint32_t)(((uint64_t)rows + tile_max - 1) / tile_max);
+  if (nt < lo) nt = lo;
+  return nt < 1 ? 1 : nt;
+}
+// Rows per tile of a node split into nt tiles (the last may be shorter).
+__host__ __device__ inline uint32_t hist_node_tile_rows(uint32_t rows, uint32_t nt) {
+  return nt ? (uint32_t)(((uint64_t)rows + nt - 1) / nt) : 0;
+}
+
 // Histogram blocks one CU holds at once for a feature-group width (LDS).
 inline int hist_blocks_resident(int FG) { return FG == 16 ? 2 : 1; }
 // Histogram lanes of padding features (index E .. FG * NG - 1) read their

@@ -204,7 +204,8 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
       B.free_slot[(uint64_t)t * P.SPT + i] = P.SPT - 1 - i;  // slot 0 goes to the root
     __syncthreads();
     const uint32_t m = rows_of(t);
-    const uint32_t nt = (m + tile - 1) / tile;
+    const uint32_t nt = hist_node_tiles(m, tile, P.hist_tile);
+    const uint32_t ts = hist_node_tile_rows(m, nt);
     if (threadIdx.x == 0) {
       TreeCtl c;
       c.n_part = 1;
@@ -239,12 +240,12 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_init(CtlParams P, DeviceBuf
     const uint32_t base = s_hist;
     for (uint32_t k = threadIdx.x; k < nt * P.NG; k += blockDim.x) {
       const uint32_t q = k / P.NG, g = k % P.NG;
-      const uint32_t b0 = q * tile;
+      const uint32_t b0 = q * ts;
       HistItem it;
       it.node = t * P.MN;
       it.slot = t * P.SPT;
       it.begin = (uint32_t)(t * P.stride) + b0;
-      it.count = min(tile, m - b0);
+      it.count = min(ts, m - b0);
       it.group = g;
       it.tree = t | (nt == 1 ? kDirectFlag : 0u);
       B.hitems[base + k] = it;
@@ -675,7 +676,8 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
   __shared__ uint32_t s_cnt[kCtlMaxTrees], s_free[kCtlMaxTrees], s_open[kCtlMaxTrees];
   // Small child of item i, for the block-wide fill of the histogram items.
   __shared__ uint32_t s_h0[kChildLdsItems], s_hnode[kChildLdsItems], s_hslot[kChildLdsItems],
-      s_hbegin[kChildLdsItems], s_hcount[kChildLdsItems], s_htree[kChildLdsItems];
+      s_hbegin[kChildLdsItems], s_hcount[kChildLdsItems], s_hts[kChildLdsItems],
+      s_htree[kChildLdsItems];
   const uint32_t n_items = B.counts[C_ITEMS];
   const bool lds_items = n_items <= kChildLdsItems;
   // One pass covers every item (the usual case): each thread loads its
@@ -727,7 +729,8 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
     const uint32_t t = c.t, lt = c.lt, rt = c.rt, e = c.e, cdepth = c.cdepth, L = c.L, R = c.R;
     const PartItem& it = c.it;
     const uint32_t scount = L <= R ? L : R;
-    const uint32_t nt = in && need ? (scount + tile - 1) / tile : 0u;
+    const uint32_t nt = in && need ? hist_node_tiles(scount, tile, P.hist_tile) : 0u;
+    const uint32_t ts = hist_node_tile_rows(scount, nt);
     // The small child's histogram slot, loaded ahead of the scans.
     const uint32_t ss =
         in && need ? B.free_slot[(uint64_t)t * P.SPT + (s_free[t] - 1u - e)] : kNone;
@@ -772,6 +775,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
           s_hslot[i] = gss;
           s_hbegin[i] = sbegin;
           s_hcount[i] = scount;
+          s_hts[i] = ts;
           s_htree[i] = hflags;
         } else {
           for (uint32_t q = 0; q < nt; q++)
@@ -779,8 +783,8 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
               HistItem h;
               h.node = small;
               h.slot = gss;
-              h.begin = sbegin + q * tile;
-              h.count = min(tile, scount - q * tile);
+              h.begin = sbegin + q * ts;
+              h.count = min(ts, scount - q * ts);
               h.group = g;
               h.tree = hflags;
               B.hitems[h0 + q * P.NG + g] = h;
@@ -840,11 +844,12 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
         if (s_h0[mid] <= k) lo = mid; else hi = mid - 1;
       }
       const uint32_t r = k - s_h0[lo], q = r / P.NG, g = r % P.NG;
+      const uint32_t ts = s_hts[lo];
       HistItem h;
       h.node = s_hnode[lo];
       h.slot = s_hslot[lo];
-      h.begin = s_hbegin[lo] + q * tile;
-      h.count = min(tile, s_hcount[lo] - q * tile);
+      h.begin = s_hbegin[lo] + q * ts;
+      h.count = min(ts, s_hcount[lo] - q * ts);
       h.group = g;
       h.tree = s_htree[lo];
       B.hitems[k] = h;
