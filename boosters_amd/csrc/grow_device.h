@@ -125,7 +125,11 @@ void launch_ctl_init(hipStream_t s, const CtlParams& p, const DeviceBufs& b,
 void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance);
 void launch_ctl_children(hipStream_t s, const CtlParams& p, const DeviceBufs& b);
 // Also advances the boosting-round counter read by the sampling kernels.
-void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance,
+// advance: 0 the open set remains; 1 depth-wise, the last step's children
+// (next level) remain; 2 the last step's partition applied its children's
+// values (leaf_apply) and its children step was skipped: the items'
+// children become leaves here.
+void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, int advance,
                          uint32_t* round);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, int64_t n);
 // Element copy by a kernel; dst may be mapped pinned host memory (device

@@ -878,13 +878,34 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_finalize(CtlParams P, Devic
                                                               int advance,
                                                               uint32_t* __restrict__ round) {
   __shared__ uint32_t s_tmp[kCtlThreads / 64 + 1];
-  // Depth-wise, the last step's children (the next level) are what remains;
-  // they are read from the next-level list in place.
-  const bool lvl = advance && P.growth == 0;
+  // advance 1, depth-wise: the last step's children (the next level) are
+  // what remains; they are read from the next-level list in place.
+  // advance 2: the last step's partition applied its children's values and
+  // its children step did not run: each item's two children become leaves
+  // here, from the split's left sums and the parent's totals (what that step
+  // would have given them), with no rows for the leaf-range list.
+  const bool from_items = advance == 2;
+  const bool lvl = (advance == 1 && P.growth == 0) || from_items;
+  if (from_items) {
+    const uint32_t ni = B.counts[C_ITEMS];
+    for (uint32_t i = threadIdx.x; i < ni; i += blockDim.x) {
+      const uint32_t t = B.pend[5 * i], lt = B.pend[5 * i + 1], rt = B.pend[5 * i + 2];
+      const PartItem it = B.items[i];
+      const long long lg = B.node_split[it.node].lg, lh = B.node_split[it.node].lh;
+      const long long pg = B.node_sum[2 * it.node], ph = B.node_sum[2 * it.node + 1];
+      const int out = P.out0 + (int)t;
+      const int qg = B.qexp[2 * out], qh = B.qexp[2 * out + 1];
+      make_leaf_from(P, B, t, it.node, lt, lg, lh, qg, qh, kNone);
+      make_leaf_from(P, B, t, it.right_node, rt, pg - lg, ph - lh, qg, qh, kNone);
+      B.node_count[it.node] = 0;
+      B.node_count[it.right_node] = 0;
+    }
+    __syncthreads();
+  }
   uint32_t carry_s = 0;
   for (uint32_t t = 0; t < P.T; t++) {
     const TreeCtl c = B.ctl[t];
-    const uint32_t no = lvl ? c.n_next : c.n_open;
+    const uint32_t no = from_items ? 0u : lvl ? c.n_next : c.n_open;
     const uint64_t lb = lvl ? (uint64_t)t * 2 * P.W : (uint64_t)t * P.WO;
     const uint32_t* onode = (lvl ? B.next_node : B.open_node) + lb;
     const uint32_t* otn = (lvl ? B.next_tnode : B.open_tnode) + lb;
@@ -989,10 +1010,9 @@ void launch_ctl_plan(hipStream_t s, const CtlParams& p, const DeviceBufs& b, boo
 void launch_ctl_children(hipStream_t s, const CtlParams& p, const DeviceBufs& b) {
   hipLaunchKernelGGL(k_ctl_children, dim3(1), dim3(kCtlThreads), 0, s, p, b);
 }
-void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, bool advance,
+void launch_ctl_finalize(hipStream_t s, const CtlParams& p, const DeviceBufs& b, int advance,
                          uint32_t* round) {
-  hipLaunchKernelGGL(k_ctl_finalize, dim3(1), dim3(kCtlThreads), 0, s, p, b, advance ? 1 : 0,
-                     round);
+  hipLaunchKernelGGL(k_ctl_finalize, dim3(1), dim3(kCtlThreads), 0, s, p, b, advance, round);
 }
 void launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, int64_t n) {
   if (n <= 0) return;

@@ -356,12 +356,15 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
                            idx_tmp_.p, ogh_tmp_.p, B.cursor, cnt + C_TILES, tree_output_.p,
                            preds_.p, n_, P.MN);
     ctx_->prof_end(pi);
-    launch_ctl_children(s_, P, B);
     // The last step's children are never split (depth limit, or the leaf
-    // budget is spent), so their histograms are not built.
-    if (step + 1 < S.steps) hist_step(true, false);
+    // budget is spent), so their histograms are not built; when its
+    // partition applied their values, the finalize makes them leaves and
+    // this step's children controller has nothing left to do.
+    const bool last = step + 1 == S.steps;
+    if (!(last && P.leaf_apply)) launch_ctl_children(s_, P, B);
+    if (!last) hist_step(true, false);
   }
-  launch_ctl_finalize(s_, P, B, /*advance=*/S.steps > 0, S.round.p);
+  launch_ctl_finalize(s_, P, B, S.steps == 0 ? 0 : P.leaf_apply ? 2 : 1, S.round.p);
   if (sampled) {
     // Rows outside the sample get the tree too: traverse the raw features
     // (trainer.rs:331-339).
