@@ -243,7 +243,9 @@ void launch_split(hipStream_t s, const DatasetDev& d, const uint32_t* nodes, con
 // the split subtracts from the entry's slot before scanning, writing the
 // difference back into the entry's slot (for its candidate features), or
 // ~0 for none; with kMinusNoWriteback set the difference is only scanned
-// (no later step reads the node's histogram).
+// (no later step reads the node's histogram).  With dyn, n_nodes (the grid)
+// must not exceed the entry lists: each block reads its first entry ahead
+// of the count.
 constexpr uint32_t kMinusNoWriteback = 0x80000000u;
 
 // Stable partition of every listed node (partition.rs:73-145): count, scan,

@@ -865,16 +865,23 @@ __global__ __launch_bounds__(kFinThreads) void k_hist_finish(
     const uint32_t* __restrict__ fin, uint32_t n_static, int NG, int FG,
     const long long* __restrict__ partial, long long* __restrict__ hist, int64_t slot_elems,
     const uint32_t* __restrict__ dyn) {
-  const uint32_t n = dyn ? *dyn : n_static;
   const uint32_t V = 256u * (uint32_t)FG * 2u;  // int64 values per group section
   const uint32_t chunks = V / kFinChunk;
   const uint32_t per_node = (uint32_t)NG * chunks;
+  // The block's first entry is read with the count when it lies inside the
+  // list (n_static is the list's length, or its capacity with dyn).
+  const uint32_t fi0 = blockIdx.x / per_node;
+  uint4 e0 = make_uint4(0u, 0u, 0u, 0u);
+  if (fi0 < n_static)
+    e0 = make_uint4(fin[4 * fi0], fin[4 * fi0 + 1], fin[4 * fi0 + 2], fin[4 * fi0 + 3]);
+  const uint32_t n = dyn ? *dyn : n_static;
   const uint32_t n_work = n * per_node;
   for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
     const uint32_t fi = w / per_node, rem = w % per_node;
     const uint32_t g = rem / chunks, c = rem % chunks;
-    const uint32_t small = fin[4 * fi], large = fin[4 * fi + 1];
-    const uint32_t h0 = fin[4 * fi + 2], nt = fin[4 * fi + 3];
+    const bool first = w == blockIdx.x;
+    const uint32_t small = first ? e0.x : fin[4 * fi], large = first ? e0.y : fin[4 * fi + 1];
+    const uint32_t h0 = first ? e0.z : fin[4 * fi + 2], nt = first ? e0.w : fin[4 * fi + 3];
     if (nt == 1 && large == 0xFFFFFFFFu) continue;  // stored directly, nothing to subtract
     const uint32_t v0 = c * kFinChunk + threadIdx.x * 2u;
     longlong2* sd = (longlong2*)(hist + (int64_t)small * slot_elems + (int64_t)g * V + v0);
@@ -1412,11 +1419,16 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
   constexpr int kThreads = kSplitWaves * 64;
   const int wv = threadIdx.x / WAVE;
   const int fi = blockIdx.x * kSplitWaves + wv;
+  // The first entry is read with the count (the grid never exceeds the
+  // entry lists).
+  const uint32_t node0 = nodes[blockIdx.y], slot0 = slots[blockIdx.y];
+  const uint32_t mraw0 = minus ? minus[blockIdx.y] : 0xFFFFFFFFu;
   const uint32_t n_work = dyn ? *dyn : gridDim.y;
   for (uint32_t wy = blockIdx.y; wy < n_work; wy += gridDim.y) {  // block-uniform
-    const uint32_t node = nodes[wy];
-    const uint32_t slot = slots[wy];
-    const uint32_t mraw = minus ? minus[wy] : 0xFFFFFFFFu;
+    const bool first = wy == blockIdx.y;
+    const uint32_t node = first ? node0 : nodes[wy];
+    const uint32_t slot = first ? slot0 : slots[wy];
+    const uint32_t mraw = first ? mraw0 : minus ? minus[wy] : 0xFFFFFFFFu;
     const uint32_t mslot = mraw & ~kMinusNoWriteback;
     const bool wback = (mraw & kMinusNoWriteback) == 0u;
     const uint32_t tree = node / max_nodes;
