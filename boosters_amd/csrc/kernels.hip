@@ -2079,6 +2079,16 @@ void launch_copy16(hipStream_t s, void* dst, const void* src, int64_t bytes) {
                      (uint4*)dst, n16);
 }
 
+// l * log(q) + (1 - l) * log(1 - q).  For a label of exactly 0 or 1 one
+// product is a signed zero, which leaves the other term's bits unchanged, so
+// one log of the selected argument gives the same value (and a wave of such
+// labels takes one log per lane instead of two).
+__device__ __forceinline__ double logloss_term(double q, double l) {
+  const bool hard = l == 0.0 || l == 1.0;
+  const double lx = log(l == 0.0 ? 1.0 - q : q);
+  return hard ? lx : l * lx + (1.0 - l) * log(1.0 - q);
+}
+
 // Metric partial sums per block: (sum w*loss, sum w), f64.  RMSE gets one row
 // of partials per output (metrics/regression.rs:17-40); logloss and
 // multiclass logloss clamp the transformed probability to [1e-15, 1-1e-15]
@@ -2114,8 +2124,8 @@ __global__ __launch_bounds__(256) void k_metric(int metric, int transform,
           a += (double)wv_[q] * d * d;
         } else {
           const float p = transform == 1 ? sigmoid32(pv[q]) : pv[q];
-          const double qq = fmin(fmax((double)p, EPS), 1.0 - EPS), l = (double)yv[q];
-          a += (double)wv_[q] * (-(l * log(qq) + (1.0 - l) * log(1.0 - qq)));
+          const double qq = fmin(fmax((double)p, EPS), 1.0 - EPS);
+          a += (double)wv_[q] * -logloss_term(qq, (double)yv[q]);
         }
         b += (double)wv_[q];
       }
@@ -2128,8 +2138,8 @@ __global__ __launch_bounds__(256) void k_metric(int metric, int transform,
       a += (double)wi * d * d;
     } else if (metric == 2) {
       const float p = transform == 1 ? sigmoid32(preds[i]) : preds[i];
-      const double q = fmin(fmax((double)p, EPS), 1.0 - EPS), l = (double)y[i];
-      a += (double)wi * (-(l * log(q) + (1.0 - l) * log(1.0 - q)));
+      const double q = fmin(fmax((double)p, EPS), 1.0 - EPS);
+      a += (double)wi * -logloss_term(q, (double)y[i]);
     } else {
       float mx = -INFINITY;
       for (int c = 0; c < K; c++) mx = fmaxf(mx, preds[(int64_t)c * n + i]);
