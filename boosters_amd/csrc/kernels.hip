@@ -212,15 +212,18 @@ __device__ __forceinline__ uint32_t abs_bits(float v) { return __float_as_uint(f
 // grad/hess [K][n]; absmax[2k] = max|g|, absmax[2k+1] = max|h| (as f32 bits).
 // Grid-stride over rows with per-thread running maxima; one atomicMax per
 // block and value, so the maxima cost no global contention.
+// KM: outputs the launch handles at most (1: one output and not softmax,
+// so the per-output maxima and their reduction hold one entry).
+template <int KM>
 __global__ __launch_bounds__(kGradThreads) void k_gradients(
     int objective, const float* __restrict__ preds, const float* __restrict__ y,
     const float* __restrict__ w, int64_t n, int K, float* __restrict__ g, float* __restrict__ h,
     uint32_t* __restrict__ absmax) {
   const float HESS_MIN = 1e-6f;
-  __shared__ uint32_t s_max[2 * kGradMaxK][kGradThreads / WAVE];
-  uint32_t mx_g[kGradMaxK], mx_h[kGradMaxK];
+  __shared__ uint32_t s_max[2 * KM][kGradThreads / WAVE];
+  uint32_t mx_g[KM], mx_h[KM];
 #pragma unroll
-  for (int k = 0; k < kGradMaxK; k++) {
+  for (int k = 0; k < KM; k++) {
     mx_g[k] = 0u;
     mx_h[k] = 0u;
   }
@@ -265,15 +268,15 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
     if (objective == 2) {  // softmax, classification.rs:141-187
       float mx = -INFINITY;
 #pragma unroll
-      for (int k = 0; k < kGradMaxK; k++)
+      for (int k = 0; k < KM; k++)
         if (k < K) mx = fmaxf(mx, preds[(int64_t)k * n + i]);
       float s = 0.0f;
 #pragma unroll
-      for (int k = 0; k < kGradMaxK; k++)
+      for (int k = 0; k < KM; k++)
         if (k < K) s += exp32(preds[(int64_t)k * n + i] - mx);
       const int label = (int)yi;
 #pragma unroll
-      for (int k = 0; k < kGradMaxK; k++) {
+      for (int k = 0; k < KM; k++) {
         if (k < K) {
           const float p = exp32(preds[(int64_t)k * n + i] - mx) / s;
           const float ind = k == label ? 1.0f : 0.0f;
@@ -287,7 +290,7 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < kGradMaxK; k++) {
+      for (int k = 0; k < KM; k++) {
         if (k < K) {
           const float p0 = preds[(int64_t)k * n + i];
           float gv, hv;
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
   }
   const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
 #pragma unroll
-  for (int k = 0; k < kGradMaxK; k++) {
+  for (int k = 0; k < KM; k++) {
     if (k < K) {
       uint32_t a = mx_g[k], b = mx_h[k];
 #pragma unroll
@@ -339,8 +342,12 @@ void launch_gradients(hipStream_t s, int objective, const float* preds, const fl
   // words, and same-address atomics serialise in L2.
   int64_t blocks = (n + kGradThreads - 1) / kGradThreads;
   if (blocks > 256) blocks = 256;
-  hipLaunchKernelGGL(k_gradients, dim3((unsigned)blocks), dim3(kGradThreads), 0, s, objective,
-                     preds, y, w, n, K, g, h, absmax);
+  if (K == 1 && objective != 2)
+    hipLaunchKernelGGL(k_gradients<1>, dim3((unsigned)blocks), dim3(kGradThreads), 0, s,
+                       objective, preds, y, w, n, K, g, h, absmax);
+  else
+    hipLaunchKernelGGL(k_gradients<kGradMaxK>, dim3((unsigned)blocks), dim3(kGradThreads), 0, s,
+                       objective, preds, y, w, n, K, g, h, absmax);
 }
 
 // ---------------------------------------------------------------------------
