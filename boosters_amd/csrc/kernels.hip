@@ -1830,10 +1830,16 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
         s_v[lane] = vi - vv;
       }
       if (lane == 0) {
-        s_base[0] = atomicAdd(&cursor[2 * t.item], TL);
-        // Right rows fill the node's range from its end backwards, tile by tile.
+        // One 64-bit add reserves both sides (left count in the low word,
+        // right count in the high word; neither can carry past a node's
+        // count), so the tiles of a node serialise on half as many atomics.
+        // Right rows fill the node's range from its end backwards.
         const uint32_t TR = TV - TL;
-        s_base[1] = it.count - atomicAdd(&cursor[2 * t.item + 1], TR) - TR;
+        const unsigned long long old = atomicAdd(
+            reinterpret_cast<unsigned long long*>(&cursor[2 * t.item]),
+            (unsigned long long)TL | ((unsigned long long)TR << 32));
+        s_base[0] = (uint32_t)old;
+        s_base[1] = it.count - (uint32_t)(old >> 32) - TR;
       }
     }
     __syncthreads();
