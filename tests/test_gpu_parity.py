@@ -415,11 +415,19 @@ def test_autotuner_choice_and_cache(ctx, oracle):
     assert (fg.value, rows_min.value, bpc.value) == first and ms.value == 0.0
 
 
-def test_graph_schedule_training_predictions(ctx, oracle):
-    """The in-graph prediction update leaves the same per-round training loss."""
-    X, y = binary_nan_data(20_000, 16, seed=23)
-    cfg = make_cfg(objective=1, n_trees=7, learning_rate=0.3, max_depth=6)
-    b = _binned(ctx, X, None)
+@pytest.mark.parametrize("case", ["logistic", "softmax_cat"])
+def test_graph_schedule_training_predictions(ctx, oracle, case):
+    """The in-graph prediction update (the last level's leaves applied by its
+    partition) leaves the same per-round training loss, the last round's
+    included; softmax grows three class trees per round."""
+    if case == "logistic":
+        X, y = binary_nan_data(20_000, 16, seed=23)
+        is_cat = None
+        cfg = make_cfg(objective=1, n_trees=7, learning_rate=0.3, max_depth=6)
+    else:
+        X, y, is_cat = multiclass_cat_data(9000, 8, 2, 3, card=40, seed=13)
+        cfg = make_cfg(objective=2, n_classes=3, n_trees=7, max_depth=5, max_onehot_cats=4)
+    b = _binned(ctx, X, is_cat)
     hist = {}
     for sched in (1, 2):
         h = C.c_void_p()
