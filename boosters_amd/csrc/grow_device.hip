@@ -377,7 +377,7 @@ __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int a
                                   uint32_t n_sp, uint32_t* s_tmp, uint32_t* s_any,
                                   uint32_t* s_tfirst, uint32_t* s_tbegin, uint32_t* s_tcount,
                                   uint8_t* s_tsrc,
-                                  uint32_t* s_items, uint32_t* s_tiles) {
+                                  uint32_t* s_items, uint32_t* s_tiles, uint32_t* s_ptile) {
   const uint32_t i = threadIdx.x;
   const TreeCtl c0 = B.ctl[0];
   const int qg = B.qexp[2 * P.out0], qh = B.qexp[2 * P.out0 + 1];
@@ -416,7 +416,8 @@ __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int a
   const uint32_t depth = c0.depth + (advance ? 1u : 0u);
   const bool act = in && depth < P.max_depth;
   const bool ex = act && expandable(sp, P.min_gain);
-  const uint32_t ntile = ex ? max(1u, (count + P.part_tile - 1) / P.part_tile) : 0u;
+  const uint32_t ptile = nn >= kFusedSmallTileNodes ? P.part_tile_small : P.part_tile;
+  const uint32_t ntile = ex ? max(1u, (count + ptile - 1) / ptile) : 0u;
   uint32_t tot_e, tot_t;
   const uint32_t e = block_excl_scan(ex ? 1u : 0u, s_tmp, &tot_e);
   const uint32_t toff = block_excl_scan(ntile, s_tmp, &tot_t);
@@ -442,6 +443,7 @@ __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int a
     c.n_tnodes = c0.n_tnodes + 2 * tot_e;
     *s_items = tot_e;
     *s_tiles = tot_t;
+    *s_ptile = ptile;
   }
   __syncthreads();
   return true;
@@ -463,11 +465,16 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   __shared__ uint8_t s_tsrc[kPlanLdsItems];  // PartTile::src bits of the item's tiles
   __shared__ uint32_t s_tlds_end;
   __shared__ uint32_t s_any[kCtlThreads / 64];
-  if (threadIdx.x == 0) s_tlds_end = kNone;
+  __shared__ uint32_t s_ptile;  // the step's partition tile
+  if (threadIdx.x == 0) {
+    s_tlds_end = kNone;
+    s_ptile = P.part_tile;
+  }
   const uint32_t n_sp = B.counts[C_SPLIT];
   const bool paired = P.growth == 0 && P.T == 1 &&
                       plan_level_paired(P, B, advance, n_sp, s_tmp, s_any, s_tfirst,
-                                        s_tbegin, s_tcount, s_tsrc, &s_items, &s_tiles);
+                                        s_tbegin, s_tcount, s_tsrc, &s_items, &s_tiles,
+                                        &s_ptile);
   if (!paired) {
     for (uint32_t i = threadIdx.x; i < n_sp; i += blockDim.x) {
       const uint32_t node = B.split_nodes[i];
@@ -616,11 +623,12 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
       const uint32_t mid = (lo + hi + 1) >> 1;
       if (s_tfirst[mid] <= k) lo = mid; else hi = mid - 1;
     }
-    const uint32_t b0 = (k - s_tfirst[lo]) * P.part_tile, count = s_tcount[lo];
+    const uint32_t ptile = s_ptile;
+    const uint32_t b0 = (k - s_tfirst[lo]) * ptile, count = s_tcount[lo];
     PartTile pt;
     pt.item = lo;
     pt.begin = s_tbegin[lo] + b0;
-    pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
+    pt.count = count > b0 ? min(ptile, count - b0) : 0u;
     pt.src = s_tsrc[lo];
     B.tiles[k] = pt;
   }

@@ -136,6 +136,7 @@ void Trainer::device_setup() {
     P.hist_tile_min = P.hist_tile;  // fixed tile
   }
   P.part_tile = kPartTileD;
+  P.part_tile_small = kFusedTileRowsSmall;
   P.seg_tile = kSegTileD;
   P.rightmost = (uint32_t)sp_.rightmost;
   P.stride = n_;
@@ -146,7 +147,8 @@ void Trainer::device_setup() {
   P.out0 = 0;
   st->steps = P.growth == 0 ? (int)c.max_depth : (int)(c.max_leaves > 0 ? c.max_leaves - 1 : 0);
   const uint32_t T = P.T;
-  const uint32_t tiles_n = (uint32_t)((n_ + kPartTileD - 1) / kPartTileD);
+  // Tiles of a step: at most one per kFusedTileRowsSmall rows plus one per node.
+  const uint32_t tiles_n = (uint32_t)((n_ + kFusedTileRowsSmall - 1) / kFusedTileRowsSmall);
   const uint32_t hist_n = (uint32_t)((n_ + P.hist_tile - 1) / P.hist_tile);
   const uint32_t segs_n = (uint32_t)((n_ + kSegTileD - 1) / kSegTileD);
   // Worst cases per step: every tree expands W nodes whose rows sum to n.
