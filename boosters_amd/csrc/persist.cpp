@@ -1117,7 +1117,12 @@ void read_schema(R& r, ModelSchema& ms) {
 
 // Appends one schema tree with scalar leaves to the model (TryFrom<TreeSchema>,
 // convert.rs:207-410).
-void append_tree(bst_model& m, const TreeSchema& t, const std::vector<double>& leaf, uint32_t group) {
+// Largest category id a model file may hold (exclusive).
+constexpr uint32_t kMaxCategoryId = 1u << 24;
+
+// cat_entries: category entries of the file seen so far (bounds the bitsets).
+void append_tree(bst_model& m, const TreeSchema& t, const std::vector<double>& leaf, uint32_t group,
+                 size_t& cat_entries) {
   const size_t n = (size_t)t.num_nodes;
   const size_t b = m.split_index.size();
   m.tree_offset.push_back((int64_t)(b + n));
@@ -1148,7 +1153,17 @@ void append_tree(bst_model& m, const TreeSchema& t, const std::vector<double>& l
       continue;
     }
     const uint32_t mx = *std::max_element(cats.begin(), cats.end());
+    // Bounds before allocating: a category id sizes its node's bitset, so a
+    // hostile file could ask for 512 MiB per node with one entry.  Ids are
+    // capped at 2^24 (trained models hold bin indices, below 2^16), and the
+    // bitsets of a whole file at 2^16 words plus 64 words per listed entry.
+    if (mx >= kMaxCategoryId)
+      bad("category value " + std::to_string(mx) + " exceeds the supported maximum " +
+          std::to_string(kMaxCategoryId - 1));
     const uint32_t nw = (mx >> 5) + 1;
+    cat_entries += cats.size();
+    if (m.cat_words.size() + nw > 65536 + 64 * cat_entries)
+      bad("category bitsets are too large for the number of categories listed");
     const size_t w0 = m.cat_words.size();
     m.cat_words.resize(w0 + nw, 0u);
     for (uint32_t c : cats) m.cat_words[w0 + (c >> 5)] |= 1u << (c & 31);
@@ -1168,6 +1183,7 @@ std::unique_ptr<bst_model> model_from_schema(ModelSchema& ms) {
   m->base_score.reserve(K);
   for (double v : ms.base_score) m->base_score.push_back((float)v);
   m->tree_offset.push_back(0);
+  size_t cat_entries = 0;
   for (size_t ti = 0; ti < ms.trees.size(); ti++) {
     TreeSchema& t = ms.trees[ti];
     const uint64_t group = ms.has_tree_groups ? ms.tree_groups[ti] : 0;
@@ -1194,7 +1210,7 @@ std::unique_ptr<bst_model> model_from_schema(ModelSchema& ms) {
       raise(BST_E_FORMAT, "linear-in-leaves trees are not supported by this engine");
     if (t.leaf_kind == 1) {
       if (t.leaf.size() != n) bad("Tree schema leaf_values must have length num_nodes");
-      append_tree(*m, t, t.leaf, (uint32_t)group);
+      append_tree(*m, t, t.leaf, (uint32_t)group, cat_entries);
     } else {
       if (t.vleaf.size() != n) bad("Tree schema leaf_values must have length num_nodes");
       const size_t k = t.vleaf.empty() ? 0 : t.vleaf[0].size();
@@ -1203,11 +1219,11 @@ std::unique_ptr<bst_model> model_from_schema(ModelSchema& ms) {
       std::vector<double> col(n);
       if (k == 1) {
         for (size_t i = 0; i < n; i++) col[i] = t.vleaf[i][0];
-        append_tree(*m, t, col, (uint32_t)group);
+        append_tree(*m, t, col, (uint32_t)group, cat_entries);
       } else if (k == K) {
         for (uint32_t g = 0; g < K; g++) {
           for (size_t i = 0; i < n; i++) col[i] = t.vleaf[i][g];
-          append_tree(*m, t, col, g);
+          append_tree(*m, t, col, g, cat_entries);
         }
       } else {
         bad("VectorLeaf length must be 1 or equal to forest n_groups");

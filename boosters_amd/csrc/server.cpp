@@ -41,6 +41,7 @@ struct Request {
   int64_t bytes = 0;
   Clock::time_point t0;
   bool done = false;
+  bool claimed = false;  // a bst_server_wait holds this ticket
   int status = BST_OK;
   int64_t latency_ns = 0;
 };
@@ -173,6 +174,24 @@ void retire_batch(bst_server* srv, Batch& b) {
   b.rows = 0;
 }
 
+// The worker stopped on an error: every queued and admitted request fails
+// with its code, and later submits are refused with its message.
+void fail_pending(bst_server* srv, int code, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->worker_error = msg;
+  for (auto* r : srv->queue) {
+    r->status = code;
+    r->done = true;
+  }
+  srv->queue.clear();
+  for (auto& b : srv->batches)
+    for (auto* r : b.reqs) {
+      r->status = code;
+      r->done = true;
+    }
+  srv->cv_done.notify_all();
+}
+
 void worker_loop(bst_server* srv) {
   try {
     BST_HIP(hipSetDevice(srv->ctx->device));
@@ -230,19 +249,13 @@ void worker_loop(bst_server* srv) {
         std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
   } catch (const Error& e) {
-    std::lock_guard<std::mutex> lk(srv->mu);
-    srv->worker_error = e.msg;
-    for (auto* r : srv->queue) {
-      r->status = e.code;
-      r->done = true;
-    }
-    srv->queue.clear();
-    for (auto& b : srv->batches)
-      for (auto* r : b.reqs) {
-        r->status = e.code;
-        r->done = true;
-      }
-    srv->cv_done.notify_all();
+    fail_pending(srv, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    fail_pending(srv, BST_E_OOM, "serving worker: host allocation failed");
+  } catch (const std::exception& e) {
+    fail_pending(srv, BST_E_DEVICE, std::string("serving worker: ") + e.what());
+  } catch (...) {
+    fail_pending(srv, BST_E_DEVICE, "serving worker: unknown failure");
   }
 }
 
@@ -332,10 +345,14 @@ int bst_server_wait(bst_server* srv, uint64_t ticket, int64_t* latency_ns) {
     auto it = srv->reqs.find(ticket);
     if (it == srv->reqs.end()) raise(BST_E_INVALID, "unknown ticket");
     Request* r = it->second;
+    // One waiter per ticket: a second concurrent wait on the same ticket is
+    // refused instead of racing the first one's erase and delete.
+    if (r->claimed) raise(BST_E_INVALID, "ticket is already being waited on");
+    r->claimed = true;
     srv->cv_done.wait(lk, [&] { return r->done; });
     const int st = r->status;
     if (latency_ns) *latency_ns = r->latency_ns;
-    srv->reqs.erase(it);
+    srv->reqs.erase(ticket);
     delete r;
     if (st != BST_OK) raise(st, srv->worker_error.empty() ? "request failed" : srv->worker_error);
     return BST_OK;

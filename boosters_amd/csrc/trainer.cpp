@@ -16,6 +16,7 @@
 // trees grown together (a depth level, or one pop per class tree of a
 // softmax round) with one launch per kernel.  Both produce identical trees:
 // node processing order, partition ids and tie rules are the same.
+#include "limits.h"
 #include "trainer.h"
 
 #include <algorithm>
@@ -108,7 +109,10 @@ Trainer::Trainer(bst_ctx* ctx, bst_dataset* ds, const TrainOptions& opt)
   const bool col_sampling = !(std::fabs(c.colsample_bytree - 1.0f) < 1e-6f);
   // Class trees of a round grow together, except on the host-driven path with
   // column sampling (its split launch takes one feature list).
-  T_ = (sched_ != BST_SCHED_PLAIN && opt_.K > 1 && (graph_capable() || !col_sampling)) ? opt_.K : 1;
+  // Positions of the trees of a round are 32-bit (t * n + i): a round whose
+  // K * n does not fit grows its class trees one at a time.
+  T_ = class_batch_trees(opt_.K, n_,
+                         sched_ != BST_SCHED_PLAIN && (graph_capable() || !col_sampling));
   tune_ = choose_hist_tuning(ctx, ds->d, c, opt_.ext, T_);
   ctx->last_tuning = tune_;
   dd_ = ds->d;
@@ -508,14 +512,15 @@ int Trainer::metric_kind() const {
   return cfg.metric == BST_METRIC_DEFAULT ? default_metric(cfg.objective) : cfg.metric;
 }
 
-void Trainer::metric_launch(const float* preds, const float* y, int64_t n, double* host_part) {
+void Trainer::metric_launch(const float* preds, const float* y, const float* w, int64_t n,
+                            double* host_part) {
   const auto& cfg = opt_.cfg;
   const int transform = cfg.objective == BST_OBJ_LOGISTIC ? BST_TRANSFORM_SIGMOID
                         : cfg.objective == BST_OBJ_SOFTMAX ? BST_TRANSFORM_SOFTMAX
                                                            : BST_TRANSFORM_IDENTITY;
   const int nb = kMetricBlocks;
   d_metric_.ensure((size_t)2 * nb * opt_.K);
-  launch_metric(s_, metric_kind(), transform, preds, opt_.K, y, nullptr, n, d_metric_.p, nb);
+  launch_metric(s_, metric_kind(), transform, preds, opt_.K, y, w, n, d_metric_.p, nb);
   d2h(s_, host_part, d_metric_.p, (size_t)2 * nb * opt_.K);
 }
 
@@ -529,7 +534,7 @@ double Trainer::metric_finish(const double* part) const {
         ss += part[(size_t)(k * nb + b) * 2];
         sw += part[(size_t)(k * nb + b) * 2 + 1];
       }
-      acc += std::sqrt(ss / sw);
+      acc += sw > 0.0 ? std::sqrt(ss / sw) : 0.0;
     }
     return acc / opt_.K;
   }
@@ -541,10 +546,10 @@ double Trainer::metric_finish(const double* part) const {
   return sw > 0 ? sl / sw : 0.0;
 }
 
-double Trainer::eval_metric(const float* preds, const float* y, int64_t n) {
+double Trainer::eval_metric(const float* preds, const float* y, const float* w, int64_t n) {
   if (metric_kind() == BST_METRIC_NONE) return std::numeric_limits<double>::quiet_NaN();
   std::vector<double> part((size_t)2 * kMetricBlocks * opt_.K);
-  metric_launch(preds, y, n, part.data());
+  metric_launch(preds, y, w, n, part.data());
   BST_HIP(hipStreamSynchronize(s_));
   return metric_finish(part.data());
 }
@@ -705,7 +710,7 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   if (on_device) {
     // The split kernel's tie rule depends on the number of candidate features.
     const uint32_t nf = col_sampling
-                            ? std::max<uint32_t>(1, (uint32_t)std::ceil((double)E * cfg.colsample_bytree))
+                            ? colsample_count(E, cfg.colsample_bytree)
                             : E;
     sp_.rightmost = (opt_.ext.n_threads != 1 && std::min(nf, E) >= 16) ? 1 : 0;
   }
@@ -769,7 +774,7 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
         // Column sampling per tree (column.rs:266-283): the k columns with
         // the smallest sampling keys of (round, output).
         if (col_sampling) {
-          const uint32_t kk = std::max<uint32_t>(1, (uint32_t)std::ceil((double)E * cfg.colsample_bytree));
+          const uint32_t kk = colsample_count(E, cfg.colsample_bytree);
           feats_h_ = smallest_keys(E, sample_key_base(cfg.seed, kStreamCols, round, (uint32_t)ts.output), kk);
           h2d(s_, feats_.p, feats_h_.data(), feats_h_.size());
           // The split kernel's tie rule depends on the number of candidate features.
@@ -832,11 +837,11 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
     // evaluate_round (eval.rs:218-260) and early stopping (callback.rs:78-118).
     double value;
     if (vX && vy && n_valid > 0)
-      value = eval_metric(vpreds_.p, vy, n_valid);
+      value = eval_metric(vpreds_.p, vy, nullptr, n_valid);
     else if (pipelined)
       value = queued_value;
     else
-      value = eval_metric(preds_.p, y, n_);
+      value = eval_metric(preds_.p, y, w, n_);
     metric_history.push_back(value);
     if (cfg.early_stopping_rounds > 0) {
       const bool imp = !have_best || value < best_val;

@@ -77,7 +77,8 @@ class Trainer {
   void root(std::vector<TreeState>& trees);
   void make_leaf(TreeState& ts, const Cand& c);
   void finish_trees(std::vector<TreeState>& trees, bst_model* m);
-  double eval_metric(const float* preds, const float* y, int64_t n);
+  // Training metric: weights w (nullable); validation metric: unweighted.
+  double eval_metric(const float* preds, const float* y, const float* w, int64_t n);
   uint32_t alloc_slot(TreeState& ts);
   // Graph schedule (trainer_device.cpp): the whole round on the device.
   struct DevState;
@@ -99,7 +100,8 @@ class Trainer {
   // `host_part` (2 doubles per block and output, nb blocks), and the value
   // from them.
   int metric_kind() const;
-  void metric_launch(const float* preds, const float* y, int64_t n, double* host_part);
+  void metric_launch(const float* preds, const float* y, const float* w, int64_t n,
+                     double* host_part);
   double metric_finish(const double* part) const;
   static constexpr int kMetricBlocks = 1024;
   // Algorithmic bytes: each histogrammed row reads its E bins (1 B, or

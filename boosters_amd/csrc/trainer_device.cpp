@@ -15,6 +15,7 @@
 #include <limits>
 
 #include "grow_device.h"
+#include "limits.h"
 #include "sample_device.h"
 #include "trainer.h"
 
@@ -198,7 +199,7 @@ void Trainer::device_setup() {
     s.n_feats = (int)P.E;
     if (s.col_sample) {
       s.n_feats = (int)std::min<uint32_t>(
-          P.E, std::max<uint32_t>(1, (uint32_t)std::ceil((double)P.E * c.colsample_bytree)));
+          P.E, colsample_count(P.E, c.colsample_bytree));
       s.feats_dev.alloc((size_t)T * P.E);
     }
     s.round.alloc(1);
@@ -412,7 +413,7 @@ void Trainer::device_launch(const float* y, const float* w, bool metric) {
     device_enqueue_round(y, w, sampled);
   }
   const int slot = (int)(S.launched & 1u);
-  if (metric) metric_launch(preds_.p, y, n_, S.h_metric[slot]);
+  if (metric) metric_launch(preds_.p, y, w, n_, S.h_metric[slot]);
   d2h(s_, S.h_out[slot], S.out.p, S.out_bytes);
   BST_HIP(hipEventRecord(S.done[slot], s_));
   S.launched++;

@@ -136,7 +136,20 @@ Mapper bin_numeric(const std::vector<float>& data, uint32_t max_bins, bool has_n
   return m;
 }
 
-// Deterministic sample of row ids for edge computation, dataset.rs:598-629.
+// One step of the SplitMix64 generator (Steele, Lea and Flood, "Fast
+// splittable pseudorandom number generators", OOPSLA 2014; the 64-bit
+// finaliser constants of Vigna's splitmix64.c): advance the state by the
+// golden gamma and return the mixed state.  dataset.rs:598-629 draws with it.
+uint64_t splitmix64_next(uint64_t& state) {
+  state += 0x9E3779B97F4A7C15ULL;
+  uint64_t z = state;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// Deterministic sample of row ids for edge computation, dataset.rs:598-629:
+// state 0xD1B54A32D192ED03 ^ n, draws z % n, duplicates rejected, sorted.
 std::vector<uint32_t> binning_sample(int64_t n, uint64_t cnt) {
   std::vector<uint32_t> idx;
   if ((uint64_t)n <= cnt) return idx;
@@ -145,11 +158,7 @@ std::vector<uint32_t> binning_sample(int64_t n, uint64_t cnt) {
   seen.reserve(cnt * 2);
   idx.reserve(cnt);
   while (idx.size() < cnt) {
-    state += 0x9E3779B97F4A7C15ULL;
-    uint64_t z = state;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    z ^= z >> 31;
+    const uint64_t z = splitmix64_next(state);
     uint32_t r = (uint32_t)(z % (uint64_t)n);
     if (seen.insert(r).second) idx.push_back(r);
   }
@@ -701,6 +710,69 @@ std::vector<uint32_t> smallest_keys(const std::vector<uint32_t>& cand, uint64_t 
   return out;
 }
 
+// Row and column sampling rules (training/sampling/{row,column}.rs); the
+// draws are the smallest keys of the stream, the counts and the gradient
+// edits are the reference's.
+
+// sample_uniform, row.rs:170-208: target_count = (n as f32 * subsample) as
+// usize, clamped to [1, n]; every row not drawn gets (g, h) = (0, 0).
+// Returns the drawn rows in index order.
+std::vector<uint32_t> sample_uniform(float* g, float* h, int64_t n, float subsample, uint64_t base) {
+  const size_t tc = std::min<size_t>(std::max<size_t>((size_t)((float)n * subsample), 1), (size_t)n);
+  std::vector<uint32_t> all((size_t)n);
+  for (int64_t i = 0; i < n; i++) all[i] = (uint32_t)i;
+  std::vector<uint32_t> sampled = smallest_keys(all, base, tc);
+  std::vector<uint8_t> mask((size_t)n, 0);
+  for (auto r : sampled) mask[r] = 1;
+  for (int64_t i = 0; i < n; i++)
+    if (!mask[i]) g[i] = h[i] = 0.0f;
+  return sampled;
+}
+
+// sample_goss, row.rs:211-270.  top_k = (n as f32 * top_rate) as usize
+// clamped to [1, n - 1]; other_k = max((n as f32 * other_rate) as usize, 1);
+// the threshold is the (n - top_k)-th smallest |g h| (select_nth_unstable);
+// rows with |g h| >= threshold are kept as they are; other_k rows of the rest
+// (all of them if fewer) are drawn and their g and h multiplied by
+// (n - top_k) / other_k in f32.  Returns the kept rows in index order.
+std::vector<uint32_t> sample_goss(float* g, float* h, int64_t n, float top_rate, float other_rate,
+                                  uint64_t base) {
+  std::vector<float> mag((size_t)n);
+  for (int64_t i = 0; i < n; i++) mag[i] = std::fabs(g[i] * h[i]);
+  const size_t top_k = std::min<size_t>(std::max<size_t>((size_t)((float)n * top_rate), 1), (size_t)n - 1);
+  const size_t other_k = std::max<size_t>((size_t)((float)n * other_rate), 1);
+  std::vector<float> sm = mag;
+  std::nth_element(sm.begin(), sm.begin() + (n - top_k), sm.end());
+  const float thr = sm[n - top_k];
+  const float mult = (float)(n - top_k) / (float)other_k;
+  std::vector<uint32_t> sampled, small;
+  for (int64_t i = 0; i < n; i++)
+    if (mag[i] >= thr)
+      sampled.push_back((uint32_t)i);
+    else
+      small.push_back((uint32_t)i);
+  std::vector<uint32_t> pick = smallest_keys(small, base, other_k);
+  for (auto r : pick) {
+    g[r] *= mult;
+    h[r] *= mult;
+  }
+  sampled.insert(sampled.end(), pick.begin(), pick.end());
+  std::sort(sampled.begin(), sampled.end());
+  return sampled;
+}
+
+// GOSS warm-up, row.rs:109-112: no sampling before round (1.0 / lr) as usize
+// (an f32 division).
+size_t goss_warmup(float learning_rate) { return (size_t)(1.0f / learning_rate); }
+
+// ColSampler::sample_tree, column.rs:180-186: k = ceil(F as f32 * rate),
+// clamped to [1, F]; the product is an f32 product.
+uint32_t colsample_k(uint32_t F, float rate) {
+  const float prod = (float)F * rate;
+  uint64_t k = (uint64_t)std::ceil(prod);
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(k, 1), F);
+}
+
 struct Grower {
   const ora_binned& B;
   const bst_gbdt_config& c;
@@ -880,17 +952,15 @@ struct Grower {
     }
   }
 
-  // ColSampler per tree (column.rs:266-283): k = ceil(F * colsample_bytree)
-  // effective columns, drawn as the k smallest sampling keys of this tree.
+  // ColSampler per tree (column.rs:180-186,266-283): colsample_k effective
+  // columns, drawn as the k smallest sampling keys of this tree.
   void sample_columns() {
     uint32_t F = (uint32_t)B.eff_to_orig.size();
     feats.resize(F);
     for (uint32_t i = 0; i < F; i++) feats[i] = i;
     float rate = c.colsample_bytree;
-    if (!(std::fabs(rate - 1.0f) < 1e-6f)) {
-      uint32_t k = std::max<uint32_t>(1, (uint32_t)std::ceil((double)F * rate));
-      feats = smallest_keys(feats, key_base(c.seed, kStreamCols, round, output), std::min(k, F));
-    }
+    if (!(std::fabs(rate - 1.0f) < 1e-6f))
+      feats = smallest_keys(feats, key_base(c.seed, kStreamCols, round, output), colsample_k(F, rate));
   }
 
   // update_predictions_from_last_tree, grower.rs:208-219.
@@ -965,7 +1035,7 @@ double metric_value(int metric, const float* p, int K, const float* y, const flo
         ss += (double)wt(w, i) * d * d;
         sw += (double)wt(w, i);
       }
-      s += std::sqrt(ss / sw);
+      s += sw > 0.0 ? std::sqrt(ss / sw) : 0.0;
     }
     return s / K;
   }
@@ -1278,7 +1348,7 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
   Grower gr(*b, c, ext.n_threads);
   bool goss = ext.goss_top_rate > 0.0f;
   bool uniform = !goss && !(std::fabs(c.subsample - 1.0f) < 1e-6f);
-  size_t warmup = goss ? (size_t)(1.0f / c.learning_rate) : 0;
+  size_t warmup = goss ? goss_warmup(c.learning_rate) : 0;
   int es_patience = (int)c.early_stopping_rounds;
   bool hib = false;
   double best_val = 0;
@@ -1295,41 +1365,12 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
       float* hk = h.data() + (size_t)k * n;
       std::vector<uint32_t> sampled;
       bool use_sample = false;
-      if (uniform) {  // sample_uniform, row.rs:170-208: target_count rows
-        size_t tc = std::min<size_t>(std::max<size_t>((size_t)((float)n * c.subsample), 1), n);
-        std::vector<uint32_t> all((size_t)n);
-        for (int64_t i = 0; i < n; i++) all[i] = (uint32_t)i;
-        sampled = smallest_keys(all, key_base(c.seed, kStreamRows, round, (uint32_t)k), tc);
-        std::vector<uint8_t> mask(n, 0);
-        for (auto r : sampled) mask[r] = 1;
-        for (int64_t i = 0; i < n; i++)
-          if (!mask[i]) gk[i] = hk[i] = 0.0f;
+      if (uniform) {  // sample_uniform, row.rs:170-208
+        sampled = sample_uniform(gk, hk, n, c.subsample, key_base(c.seed, kStreamRows, round, (uint32_t)k));
         use_sample = true;
       } else if (goss && round >= warmup) {  // sample_goss, row.rs:211-270
-        // Rows with |g h| >= the (n - top_k)-th smallest magnitude are kept;
-        // other_k of the rest (all of them if fewer) are drawn and amplified.
-        std::vector<float> mag(n);
-        for (int64_t i = 0; i < n; i++) mag[i] = std::fabs(gk[i] * hk[i]);
-        size_t top_k = std::min<size_t>(std::max<size_t>((size_t)((float)n * ext.goss_top_rate), 1), n - 1);
-        size_t other_k = std::max<size_t>((size_t)((float)n * ext.goss_other_rate), 1);
-        std::vector<float> sm = mag;
-        std::nth_element(sm.begin(), sm.begin() + (n - top_k), sm.end());
-        float thr = sm[n - top_k];
-        float mult = (float)(n - top_k) / (float)other_k;
-        std::vector<uint32_t> small;
-        for (int64_t i = 0; i < n; i++)
-          if (mag[i] >= thr)
-            sampled.push_back((uint32_t)i);
-          else
-            small.push_back((uint32_t)i);
-        std::vector<uint32_t> pick =
-            smallest_keys(small, key_base(c.seed, kStreamGoss, round, (uint32_t)k), other_k);
-        for (auto r : pick) {
-          gk[r] *= mult;
-          hk[r] *= mult;
-        }
-        sampled.insert(sampled.end(), pick.begin(), pick.end());
-        std::sort(sampled.begin(), sampled.end());
+        sampled = sample_goss(gk, hk, n, ext.goss_top_rate, ext.goss_other_rate,
+                              key_base(c.seed, kStreamGoss, round, (uint32_t)k));
         use_sample = true;
       }
       gr.round = round;
@@ -1513,6 +1554,53 @@ int ora_predict(const bst_model_info* info, const bst_forest_arrays* f, const fl
     }
   }
   if (!raw) transform(info->output_transform, out, K, n);
+  return BST_OK;
+}
+
+void ora_splitmix64_stream(uint64_t state, int64_t count, uint64_t* out) {
+  for (int64_t i = 0; i < count; i++) out[i] = splitmix64_next(state);
+}
+
+int64_t ora_binning_sample(int64_t n, uint64_t cnt, uint32_t* out) {
+  std::vector<uint32_t> idx = binning_sample(n, cnt);
+  if (out) std::copy(idx.begin(), idx.end(), out);
+  return (int64_t)idx.size();
+}
+
+int ora_sample_rows(const float* g, const float* h, int64_t n, const bst_gbdt_config* cfg,
+                    const bst_ext_config* ext, uint32_t round, uint32_t output, float* g_out,
+                    float* h_out, uint32_t* rows_out, int64_t* n_rows_out) {
+  if (!g || !h || !cfg || !g_out || !h_out || !n_rows_out || n < 1) return fail(BST_E_INVALID, "bad argument");
+  std::copy(g, g + n, g_out);
+  std::copy(h, h + n, h_out);
+  const bool goss = ext && ext->goss_top_rate > 0.0f;
+  const bool uniform = !goss && !(std::fabs(cfg->subsample - 1.0f) < 1e-6f);
+  std::vector<uint32_t> rows;
+  if (uniform) {
+    rows = sample_uniform(g_out, h_out, n, cfg->subsample, key_base(cfg->seed, kStreamRows, round, output));
+  } else if (goss && round >= goss_warmup(cfg->learning_rate)) {
+    if (n < 2) return fail(BST_E_CONFIG, "GOSS needs at least two rows");
+    rows = sample_goss(g_out, h_out, n, ext->goss_top_rate, ext->goss_other_rate,
+                       key_base(cfg->seed, kStreamGoss, round, output));
+  } else {
+    *n_rows_out = -1;  // this round trains on every row
+    return BST_OK;
+  }
+  if (rows_out) std::copy(rows.begin(), rows.end(), rows_out);
+  *n_rows_out = (int64_t)rows.size();
+  return BST_OK;
+}
+
+int ora_sample_cols(uint32_t n_feat, const bst_gbdt_config* cfg, uint32_t round, uint32_t output,
+                    uint32_t* out, int32_t* k) {
+  if (!cfg || !out || !k || n_feat < 1) return fail(BST_E_INVALID, "bad argument");
+  std::vector<uint32_t> feats(n_feat);
+  for (uint32_t i = 0; i < n_feat; i++) feats[i] = i;
+  if (!(std::fabs(cfg->colsample_bytree - 1.0f) < 1e-6f))
+    feats = smallest_keys(feats, key_base(cfg->seed, kStreamCols, round, output),
+                          colsample_k(n_feat, cfg->colsample_bytree));
+  std::copy(feats.begin(), feats.end(), out);
+  *k = (int32_t)feats.size();
   return BST_OK;
 }
 

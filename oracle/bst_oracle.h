@@ -94,6 +94,21 @@ void ora_model_free(ora_model* m);
 int ora_predict(const bst_model_info* info, const bst_forest_arrays* f, const float* feat_major,
                 int64_t n_rows, int32_t n_feat, int32_t raw, float* out, uint32_t* leaf_idx);
 
+/* Sampling rules, exposed for their pins.  SplitMix64 steps from a state
+ * (the mixer of dataset.rs:598-629's edge sample); the edge sample itself. */
+void ora_splitmix64_stream(uint64_t state, int64_t count, uint64_t* out);
+int64_t ora_binning_sample(int64_t n, uint64_t cnt, uint32_t* out);
+/* The row sample of (round, output) as ora_train draws it: uniform
+ * (row.rs:170-208) or GOSS (row.rs:211-270, after the warm-up of :109-112).
+ * g_out/h_out: the edited pairs; rows_out [n] the kept rows in order;
+ * *n_rows_out = -1 when the round uses every row. */
+int ora_sample_rows(const float* g, const float* h, int64_t n, const bst_gbdt_config* cfg,
+                    const bst_ext_config* ext, uint32_t round, uint32_t output, float* g_out,
+                    float* h_out, uint32_t* rows_out, int64_t* n_rows_out);
+/* The columns of tree (round, output), column.rs:180-186,266-283. */
+int ora_sample_cols(uint32_t n_feat, const bst_gbdt_config* cfg, uint32_t round, uint32_t output,
+                    uint32_t* out, int32_t* k);
+
 /* Metrics over transformed predictions [K][n]: metrics/{classification,regression}.rs. */
 double ora_metric(int32_t metric, const float* preds, int32_t n_outputs, const float* targets,
                   const float* weights, int64_t n);

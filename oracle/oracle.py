@@ -73,6 +73,13 @@ def lib():
                                       P(C.c_uint32)]),
             "ora_metric": (C.c_double, [C.c_int32, P(C.c_float), C.c_int32, P(C.c_float),
                                         P(C.c_float), C.c_int64]),
+            "ora_splitmix64_stream": (None, [C.c_uint64, C.c_int64, P(C.c_uint64)]),
+            "ora_binning_sample": (C.c_int64, [C.c_int64, C.c_uint64, P(C.c_uint32)]),
+            "ora_sample_rows": (C.c_int, [P(C.c_float), P(C.c_float), C.c_int64, P(A.GBDTConfigC),
+                                          P(A.ExtConfig), C.c_uint32, C.c_uint32, P(C.c_float),
+                                          P(C.c_float), P(C.c_uint32), P(C.c_int64)]),
+            "ora_sample_cols": (C.c_int, [C.c_uint32, P(A.GBDTConfigC), C.c_uint32, C.c_uint32,
+                                          P(C.c_uint32), P(C.c_int32)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -261,3 +268,37 @@ def metric(metric_id, preds, targets, weights=None):
     w = None if weights is None else np.ascontiguousarray(weights, np.float32)
     return lib().ora_metric(metric_id, A.ptr(preds), K,
                             A.ptr(np.ascontiguousarray(targets, np.float32)), A.ptr(w), n)
+
+
+def splitmix64_stream(state, count):
+    out = np.zeros(count, np.uint64)
+    lib().ora_splitmix64_stream(C.c_uint64(state), count, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
+
+
+def binning_sample(n, cnt):
+    out = np.zeros(max(min(int(n), int(cnt)), 1), np.uint32)
+    k = lib().ora_binning_sample(int(n), int(cnt), A.ptr(out))
+    return out[:k].copy()
+
+
+def sample_rows(grad, hess, cfg, ext=None, round_=0, output=0):
+    """The row sample of tree (round, output): (rows or None, grad, hess)."""
+    g = np.ascontiguousarray(grad, np.float32)
+    h = np.ascontiguousarray(hess, np.float32)
+    n = len(g)
+    go = np.zeros(n, np.float32)
+    ho = np.zeros(n, np.float32)
+    rows = np.zeros(n, np.uint32)
+    nr = C.c_int64()
+    _check(lib().ora_sample_rows(A.ptr(g), A.ptr(h), n, C.byref(cfg),
+                                 C.byref(ext) if ext is not None else None, round_, output,
+                                 A.ptr(go), A.ptr(ho), A.ptr(rows), C.byref(nr)))
+    return (None if nr.value < 0 else rows[: nr.value].copy()), go, ho
+
+
+def sample_cols(n_feat, cfg, round_=0, output=0):
+    out = np.zeros(n_feat, np.uint32)
+    k = C.c_int32()
+    _check(lib().ora_sample_cols(n_feat, C.byref(cfg), round_, output, A.ptr(out), C.byref(k)))
+    return out[: k.value].copy()

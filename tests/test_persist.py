@@ -571,6 +571,30 @@ def test_schema_rejections(edit):
     assert msg
 
 
+def test_category_ids_bounded_before_allocation():
+    # A category id sizes its node's bitset ((id >> 5) + 1 words): ids at or
+    # above 2^24 are refused before anything is allocated, and so are bitsets
+    # far larger than the categories the file lists.
+    def cat_model(sets):
+        s = base_schema()
+        t = s["forest"]["trees"][0]
+        t["categories"] = {"node_indices": [0], "category_sets": sets}
+        return s
+    ok = load(envelope(msgpack.packb(cat_model([[1, 3, 2**16 - 1]]))))
+    e = export(ok)
+    assert list(e["split_type"]) == [1, 0, 0] and list(e["cat_size"])[0] == 2**11
+    for bad_id in (2**24, 2**32 - 1):
+        code, msg = load_err(envelope(msgpack.packb(cat_model([[bad_id]]))))
+        assert code == A.BST_E_FORMAT and "category value" in msg
+    # One tree per node with a single large id: the bitsets would need 2^19
+    # words per node but the file lists one category per node.
+    s = base_schema()
+    t = s["forest"]["trees"][0]
+    t["categories"] = {"node_indices": [0], "category_sets": [[2**24 - 1]]}
+    code, msg = load_err(envelope(msgpack.packb(s)))
+    assert code == A.BST_E_FORMAT and "too large" in msg
+
+
 def test_import_rejects_cycles():
     # bst_model_import runs the same walk: a loop would never reach a leaf.
     with pytest.raises(A.BoostersError) as ei:
