@@ -245,6 +245,8 @@ def _declare(lib):
         ),
         "bst_last_train_stats": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_int64)]),
         "bst_last_train_metric": (C.c_int, [vp, P(C.c_double), C.c_int64, P(C.c_int64)]),
+        "bst_ctx_set_keep_predictions": (C.c_int, [vp, C.c_int32]),
+        "bst_last_train_predictions": (C.c_int, [vp, P(C.c_float), C.c_int64, P(C.c_int64)]),
         "bst_last_train_tuning": (
             C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_double)]),
         "bst_model_info_get": (C.c_int, [vp, P(ModelInfo)]),
@@ -320,7 +322,8 @@ def exported_symbols():
         "bst_dataset_create", "bst_dataset_create_device",
         "bst_dataset_destroy", "bst_dataset_feature_info", "bst_dataset_bin_bounds",
         "bst_dataset_get_bins", "bst_train", "bst_train_device", "bst_last_train_stats",
-        "bst_last_train_metric", "bst_last_train_tuning", "bst_model_info_get", "bst_model_export", "bst_model_import",
+        "bst_last_train_metric", "bst_ctx_set_keep_predictions", "bst_last_train_predictions",
+        "bst_last_train_tuning", "bst_model_info_get", "bst_model_export", "bst_model_import",
         "bst_model_destroy", "bst_model_save_bstr", "bst_model_load_bstr",
         "bst_model_save_bstr_file", "bst_model_load_bstr_file", "bst_model_save_json",
         "bst_model_load_json", "bst_buffer_free", "bst_predict", "bst_predict_device", "bst_server_create",

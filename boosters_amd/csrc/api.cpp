@@ -597,6 +597,28 @@ int bst_train_device(bst_ctx* ctx, bst_dataset* ds, const float* y, int64_t n_ta
   });
 }
 
+int bst_ctx_set_keep_predictions(bst_ctx* ctx, int32_t enable) {
+  return guard([&] {
+    need(ctx != nullptr, BST_E_INVALID, "null ctx");
+    ctx->keep_preds = enable != 0;
+    if (!ctx->keep_preds) ctx->last_preds.clear();
+    return BST_OK;
+  });
+}
+
+int bst_last_train_predictions(bst_ctx* ctx, float* out, int64_t cap, int64_t* count) {
+  return guard([&] {
+    need(ctx != nullptr, BST_E_INVALID, "null ctx");
+    const int64_t k = (int64_t)ctx->last_preds.size();
+    if (count) *count = k;
+    if (out) {
+      need(cap >= k, BST_E_SHAPE, "output buffer smaller than the kept predictions");
+      std::memcpy(out, ctx->last_preds.data(), (size_t)k * sizeof(float));
+    }
+    return BST_OK;
+  });
+}
+
 int bst_last_train_stats(bst_ctx* ctx, double* ms_total, double* ms_per_tree, int64_t* hr) {
   return guard([&] {
     need(ctx != nullptr, BST_E_INVALID, "null ctx");

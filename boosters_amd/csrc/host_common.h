@@ -239,6 +239,10 @@ struct bst_ctx {
   double last_ms_total = 0, last_ms_per_tree = 0;
   int64_t last_hist_rows = 0;
   std::vector<double> last_metric;
+  // Training predictions [K][n] after the last round of the last train, kept
+  // only when keep_preds is set (bst_ctx_set_keep_predictions).
+  bool keep_preds = false;
+  std::vector<float> last_preds;
   // Kernel profiling: events bracketing launches on `stream`.
   struct Pending {
     int kind;

@@ -856,6 +856,14 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
     }
   }
   if (pipelined) BST_HIP(hipStreamSynchronize(s_));  // a round queued past an early stop
+  if (ctx_->keep_preds) {
+    // Training predictions after the last round trained (before any
+    // early-stopping truncation): base score plus every tree's leaf, added
+    // by the partition's row ranges (or by traversal for sampled rounds).
+    ctx_->last_preds.resize((size_t)K * n_);
+    d2h(s_, ctx_->last_preds.data(), preds_.p, ctx_->last_preds.size());
+    BST_HIP(hipStreamSynchronize(s_));
+  }
   if (cfg.early_stopping_rounds > 0 && best_n_trees > 0 && best_n_trees < m->tree_group.size()) {
     const int64_t keep = m->tree_offset[best_n_trees];
     m->tree_offset.resize(best_n_trees + 1);

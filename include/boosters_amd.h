@@ -244,6 +244,13 @@ int bst_last_train_tuning(bst_ctx* ctx, int32_t* hist_fg, int32_t* hist_rows_min
                           int32_t* hist_blocks_per_cu, int32_t* tuned, double* tune_ms);
 /* Per-round metric values recorded by the last train ([n_rounds]). */
 int bst_last_train_metric(bst_ctx* ctx, double* out, int64_t cap, int64_t* n_rounds);
+/* Training predictions (raw margins, [n_outputs][n_rows]) after the last
+ * round of the last bst_train on this ctx, kept only while enabled: the
+ * reference keeps them inside GBDTTrainer::train (trainer.rs:232-339,
+ * updated from the partitioner's leaf ranges, grower.rs:208-219) and never
+ * returns them.  *count = number of floats; out may be NULL to query it. */
+int bst_ctx_set_keep_predictions(bst_ctx* ctx, int32_t enable);
+int bst_last_train_predictions(bst_ctx* ctx, float* out, int64_t cap, int64_t* count);
 
 /* ---- model ------------------------------------------------------------- */
 int bst_model_info_get(const bst_model* model, bst_model_info* info);

@@ -1557,6 +1557,75 @@ int ora_predict(const bst_model_info* info, const bst_forest_arrays* f, const fl
   return BST_OK;
 }
 
+int ora_partition_predict(const ora_binned* b, const bst_model_info* info,
+                          const bst_forest_arrays* f, const uint8_t* by_threshold,
+                          const float* X, float* out) {
+  if (!b || !info || !f || !out) return fail(BST_E_INVALID, "null");
+  const int64_t n = b->n;
+  const int K = (int)info->n_groups;
+  std::vector<int32_t> orig_to_eff(b->F, -1);
+  for (size_t e = 0; e < b->eff_to_orig.size(); e++) orig_to_eff[b->eff_to_orig[e]] = (int32_t)e;
+  // Per node: the split bin a numeric threshold came from (the bin b with
+  // next_up((f32)bound_b) == threshold, grower.rs:543-556,603-604).
+  std::vector<int32_t> split_bin((size_t)info->total_nodes, -1);
+  for (int64_t t = 0; t < info->n_trees; t++) {
+    if (by_threshold && by_threshold[t]) continue;
+    for (int64_t k = f->tree_offset[t]; k < f->tree_offset[t + 1]; k++) {
+      if (f->is_leaf[k] || f->split_type[k]) continue;
+      const uint32_t of = f->split_index[k];
+      if (of >= (uint32_t)b->F || orig_to_eff[of] < 0) return fail(BST_E_SHAPE, "split on a feature without bins");
+      const Mapper& m = b->mappers[of];
+      const size_t nb = m.bounds.size() - (m.missing == 2 ? 1 : 0);
+      int32_t found = -1;
+      for (size_t i = 0; i < nb; i++)
+        if (next_up((float)m.bounds[i]) == f->split_threshold[k]) {
+          if (found >= 0) return fail(BST_E_SHAPE, "two bins give the same threshold");
+          found = (int32_t)i;
+        }
+      if (found < 0) return fail(BST_E_SHAPE, "threshold is not a bin boundary");
+      split_bin[k] = found;
+    }
+  }
+  const bool any_thr = by_threshold && std::any_of(by_threshold, by_threshold + info->n_trees,
+                                                   [](uint8_t v) { return v != 0; });
+  if (any_thr && !X) return fail(BST_E_INVALID, "threshold traversal needs the raw features");
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; r++) {
+    for (int k = 0; k < K; k++) out[k * n + r] = f->base_score[k];
+    for (int64_t t = 0; t < info->n_trees; t++) {
+      const int64_t base = f->tree_offset[t];
+      uint32_t node = 0;
+      if (by_threshold && by_threshold[t]) {
+        node = traverse(f, base, X, n, r);
+      } else {
+        // RowPartitioner::split's rule (partition.rs:321-337) on the row's bins.
+        while (!f->is_leaf[base + node]) {
+          const int64_t k = base + node;
+          const uint32_t of = f->split_index[k];
+          const Mapper& m = b->mappers[of];
+          const uint32_t bin = b->bins[orig_to_eff[of]][r];
+          const bool missing = m.missing != 0 && bin == m.default_bin;
+          bool go_left;
+          if (!f->split_type[k]) {
+            go_left = missing ? f->default_left[k] != 0 : bin <= (uint32_t)split_bin[k];
+          } else {
+            // Categorical: training-left rows take the inference-right child
+            // and the stored default direction is inverted (grower.rs:612-648).
+            const uint32_t size = f->cat_size[k], w = bin >> 5;
+            const bool in_set = size != 0 && w < size &&
+                                ((f->cat_words[f->cat_start[k] + w] >> (bin & 31)) & 1u);
+            const bool train_left = missing ? f->default_left[k] == 0 : in_set;
+            go_left = !train_left;
+          }
+          node = go_left ? f->left[k] : f->right[k];
+        }
+      }
+      out[(int64_t)f->tree_group[t] * n + r] += f->leaf_value[base + node];
+    }
+  }
+  return BST_OK;
+}
+
 void ora_splitmix64_stream(uint64_t state, int64_t count, uint64_t* out) {
   for (int64_t i = 0; i < count; i++) out[i] = splitmix64_next(state);
 }

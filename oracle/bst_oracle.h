@@ -94,6 +94,16 @@ void ora_model_free(ora_model* m);
 int ora_predict(const bst_model_info* info, const bst_forest_arrays* f, const float* feat_major,
                 int64_t n_rows, int32_t n_feat, int32_t raw, float* out, uint32_t* leaf_idx);
 
+/* Training predictions as the reference's trainer forms them (trainer.rs:
+ * 323-339): base score plus, tree by tree in forest order, the leaf each
+ * training row of `b` reaches by the partitioner's rule on its bins
+ * (partition.rs:321-337; grower.rs:208-219), or, for trees with
+ * by_threshold[t] != 0 (rounds trained on a row sample), by traversing the
+ * raw features X with the thresholds.  out: [n_groups][n_rows]. */
+int ora_partition_predict(const ora_binned* b, const bst_model_info* info,
+                          const bst_forest_arrays* f, const uint8_t* by_threshold,
+                          const float* X, float* out);
+
 /* Sampling rules, exposed for their pins.  SplitMix64 steps from a state
  * (the mixer of dataset.rs:598-629's edge sample); the edge sample itself. */
 void ora_splitmix64_stream(uint64_t state, int64_t count, uint64_t* out);

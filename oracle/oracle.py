@@ -74,6 +74,8 @@ def lib():
             "ora_metric": (C.c_double, [C.c_int32, P(C.c_float), C.c_int32, P(C.c_float),
                                         P(C.c_float), C.c_int64]),
             "ora_splitmix64_stream": (None, [C.c_uint64, C.c_int64, P(C.c_uint64)]),
+            "ora_partition_predict": (C.c_int, [vp, P(A.ModelInfo), P(A.ForestArrays),
+                                                P(C.c_uint8), P(C.c_float), P(C.c_float)]),
             "ora_binning_sample": (C.c_int64, [C.c_int64, C.c_uint64, P(C.c_uint32)]),
             "ora_sample_rows": (C.c_int, [P(C.c_float), P(C.c_float), C.c_int64, P(A.GBDTConfigC),
                                           P(A.ExtConfig), C.c_uint32, C.c_uint32, P(C.c_float),
@@ -165,6 +167,17 @@ class Binned:
         fs = None if features is None else np.ascontiguousarray(features, np.uint32)
         _check(lib().ora_find_split(self.h, A.ptr(h), pg, ph, pc, C.byref(cfg), n_threads,
                                     A.ptr(fs), 0 if fs is None else len(fs), C.byref(out)))
+        return out
+
+    def training_predictions(self, info, arrays, by_threshold=None):
+        """The reference trainer's training predictions of a forest over these
+        rows: partition routing on the bins, or threshold traversal of the raw
+        features for trees flagged in `by_threshold`."""
+        out = np.zeros((info.n_groups, self.n_rows), np.float32)
+        bt = None if by_threshold is None else np.ascontiguousarray(by_threshold, np.uint8)
+        fa = A.forest_struct(arrays)
+        _check(lib().ora_partition_predict(self.h, C.byref(info), C.byref(fa), A.ptr(bt),
+                                           A.ptr(self.X), A.ptr(out)))
         return out
 
     def partition(self, rows, split):

@@ -260,3 +260,40 @@ def test_rmse_closed_forms(oracle):
     assert v == pytest.approx(math.sqrt(7 / 4), rel=1e-12)
     # Zero total weight gives 0, not NaN.
     assert oracle.metric(A.METRIC_RMSE, np.array([[1.0, 2.0]], F32), np.zeros(2, F32), np.zeros(2, F32)) == 0.0
+
+
+# --------------------------------------------- training-prediction routing ---
+def test_partition_predictions_match_trainer_metric(oracle):
+    """The trainer's training predictions (partition routing, grower.rs:208-219)
+    reproduce its own per-round training RMSE exactly."""
+    from tests.conftest import regression_data
+
+    X, y = regression_data(6000, 12, seed=61)
+    X[3, ::7] = np.nan
+    ob = oracle.Binned(X)
+    m = oracle.train(ob, y, make_cfg(n_trees=7, learning_rate=0.3, max_depth=5))
+    p = ob.training_predictions(m.info, m.arrays)
+    assert oracle.metric(A.METRIC_RMSE, p, y) == m.metric_history()[-1]
+
+
+def test_partition_routing_differs_from_threshold_at_rounded_up_bounds(oracle):
+    """A bin boundary halfway between two adjacent f32 values rounds (to even)
+    up to the larger one; the threshold next_up((f32)bound) then sends that
+    value left in traversal while the partitioner, on bins, sends it right
+    (grower.rs:543-556 vs partition.rs:321-337).  The trainer's own
+    predictions follow the partition."""
+    a = np.nextafter(F32(1.0), F32(2.0))          # odd mantissa
+    b = np.nextafter(a, F32(2.0))                 # even mantissa: the midpoint rounds to b
+    x = np.array([a] * 50 + [b] * 50, F32)[None]
+    y = np.array([0.0] * 50 + [1.0] * 50, F32)
+    m = oracle.train(oracle.Binned(x), y, make_cfg(n_trees=1, max_depth=1, learning_rate=1.0,
+                                                   min_child_weight=0.0, lambda_=0.0))
+    assert m.info.total_nodes == 3
+    thr = m.arrays["split_threshold"][0]
+    assert thr == np.nextafter(b, F32(2.0))
+    part = oracle.Binned(x).training_predictions(m.info, m.arrays)[0]
+    trav = oracle.predict(m.info, m.arrays, x, raw=True)[0]
+    np.testing.assert_array_equal(part[:50], trav[:50])
+    assert np.all(part[50:] != trav[50:])          # b: right by bins, left by threshold
+    assert np.all(part[50:] > part[:50])
+    assert oracle.metric(A.METRIC_RMSE, part[None], y) == m.metric_history()[-1]
