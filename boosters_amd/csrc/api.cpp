@@ -203,6 +203,100 @@ void bst_model::upload(hipStream_t s) {
   h2d(s, d_cat_size.p, cat_size.data(), nn);
   h2d(s, d_cat_words.p, cat_words.data(), cat_words.size());
   h2d(s, d_base.p, base_score.data(), base_score.size());
+  // Predictor layout: trees of at most kPredChunkNodes nodes testing features
+  // below 2^16 get a compact copy, renumbered breadth first so that a node's
+  // children are adjacent (left, left + 1), and are grouped into LDS chunks of
+  // consecutive trees; any other tree is a chunk of its own, traversed in the
+  // 16-byte layout above.
+  const size_t T = tree_offset.empty() ? 0 : tree_offset.size() - 1;
+  need(T == 0 || tree_offset.back() < (int64_t)UINT32_MAX, BST_E_SHAPE,
+       "forest has more than 2^32 nodes");
+  auto fbits = [](float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    return u;
+  };
+  std::vector<uint2> cn;
+  std::vector<uint32_t> ctoff(std::max<size_t>(T, 1), 0), cref;
+  std::vector<PredChunk> ch;
+  std::vector<uint32_t> new_id, order;
+  auto compactable = [&](size_t t) {
+    const int64_t b = tree_offset[t], e = tree_offset[t + 1];
+    if (e - b > kPredChunkNodes) return false;
+    for (int64_t k = b; k < e; k++)
+      if (!is_leaf[k] && split_index[k] >= 65536u) return false;
+    return true;
+  };
+  auto append_compact = [&](size_t t) {
+    const int64_t b = tree_offset[t];
+    const uint32_t nn = (uint32_t)(tree_offset[t + 1] - b);
+    new_id.assign(nn, UINT32_MAX);
+    order.clear();
+    new_id[0] = 0;
+    order.push_back(0);
+    uint32_t next = 1;
+    for (size_t i = 0; i < order.size(); i++) {
+      const uint32_t o = order[i];
+      if (is_leaf[b + o]) continue;
+      const uint32_t l = left[b + o], r = right[b + o];
+      new_id[l] = next;
+      new_id[r] = next + 1;
+      next += 2;
+      order.push_back(l);
+      order.push_back(r);
+    }
+    const size_t c0 = cn.size();
+    ctoff[t] = (uint32_t)c0;
+    cn.resize(c0 + next, make_uint2(0u, 0u));
+    for (uint32_t o : order) {
+      const int64_t k = b + o;
+      uint2 w;
+      if (is_leaf[k]) {
+        w.x = kCLeaf | (o << kCShift);
+        w.y = fbits(leaf_value[k]);
+      } else {
+        w.x = split_index[k] | (default_left[k] ? kCDefLeft : 0u) |
+              (new_id[left[k]] << kCShift);
+        if (split_type[k]) {
+          w.x |= kCCat;
+          w.y = (uint32_t)(cref.size() / 2);
+          cref.push_back(cat_start[k]);
+          cref.push_back(cat_size[k]);
+        } else {
+          w.y = fbits(split_threshold[k]);
+        }
+      }
+      cn[c0 + new_id[o]] = w;
+    }
+  };
+  for (size_t t = 0; t < T;) {
+    if (!compactable(t)) {
+      ch.push_back({(uint32_t)t, (uint32_t)(t + 1), 0u, 0u});
+      t++;
+      continue;
+    }
+    if (cn.size() & 1) cn.push_back(make_uint2(0u, 0u));  // chunks start 16-byte aligned
+    const uint32_t begin = (uint32_t)cn.size();
+    size_t e = t;
+    while (e < T && compactable(e) &&
+           cn.size() - begin + (size_t)(tree_offset[e + 1] - tree_offset[e]) <= (size_t)kPredChunkNodes)
+      append_compact(e++);
+    ch.push_back({(uint32_t)t, (uint32_t)e, begin, (uint32_t)(cn.size() - begin)});
+    t = e;
+  }
+  if (cn.size() & 1) cn.push_back(make_uint2(0u, 0u));
+  if (cn.empty()) cn.assign(2, make_uint2(0u, 0u));
+  pred_has_cat = cref.empty() ? 0 : 1;
+  if (cref.empty()) cref.assign(2, 0u);
+  n_chunks = (int)ch.size();
+  d_cnodes.alloc(cn.size());
+  d_ctree_off.alloc(ctoff.size());
+  d_cat_ref.alloc(cref.size());
+  d_chunks.alloc(std::max<size_t>(ch.size(), 1));
+  h2d(s, d_cnodes.p, cn.data(), cn.size());
+  h2d(s, d_ctree_off.p, ctoff.data(), ctoff.size());
+  h2d(s, d_cat_ref.p, cref.data(), cref.size());
+  h2d(s, d_chunks.p, ch.data(), ch.size());
   BST_HIP(hipStreamSynchronize(s));
 }
 
@@ -263,9 +357,11 @@ int bst_ctx_create(int device, bst_ctx** out) {
     auto* c = new bst_ctx();
     c->device = device;
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (prop.sharedMemPerBlock > 0) c->lds_per_block = prop.sharedMemPerBlock;
     BST_HIP(hipSetDevice(device));
+    std::unique_ptr<bst_ctx> guard_c(c);
     BST_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    *out = c;
+    *out = guard_c.release();
     return BST_OK;
   });
 }
@@ -276,6 +372,7 @@ int bst_ctx_destroy(bst_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->pool.release();
+    ctx->pred_leaf.reset();
     if (ctx->staging) (void)hipHostFree(ctx->staging);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -733,10 +830,20 @@ static void predict_dev(bst_ctx* ctx, const bst_model* mc, const float* Xd, int6
     m->upload(ctx->stream);
     m->device_for = ctx->device;
   }
-  launch_predict(ctx->stream, m->d_nodes.p, m->d_tree_off.p, m->d_tree_group.p, m->info.n_trees,
-                 m->d_cat_start.p, m->d_cat_size.p, m->d_cat_words.p, m->d_base.p,
-                 (int)m->info.n_groups, Xd, n, n, F, raw ? 0 : m->info.output_transform, outd, n,
-                 leafd);
+  const PredModel pm = m->view();
+  int64_t scratch_floats = 0;
+  float* scratch_p = nullptr;
+  if (pred_two_phase(pm, n, ctx->n_cu, ctx->lds_per_block, &scratch_floats)) {
+    // The context's grow-only leaf buffer; earlier work on the stream may
+    // still read the old one, so the stream drains before it is replaced.
+    if ((size_t)scratch_floats > ctx->pred_leaf.n) {
+      BST_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->pred_leaf.alloc((size_t)scratch_floats);
+    }
+    scratch_p = ctx->pred_leaf.p;
+  }
+  launch_predict(ctx->stream, pm, Xd, n, n, F, raw ? 0 : m->info.output_transform, outd, n, leafd,
+                 scratch_p, ctx->lds_per_block, ctx->n_cu);
   BST_HIP(hipGetLastError());
 }
 

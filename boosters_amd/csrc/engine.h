@@ -1,6 +1,8 @@
 // engine.h — internal interfaces of the gfx950 GBDT engine (host <-> kernels).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <vector>
 #include <stdint.h>
 
 namespace bst {
@@ -285,12 +287,73 @@ void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const fl
                         const uint32_t* tree_output, int64_t n_rows,
                         const uint32_t* dyn = nullptr, const uint32_t* idx_alt = nullptr);
 
-void launch_predict(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
-                    const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
-                    const uint32_t* cat_size, const uint32_t* cat_words, const float* base,
-                    int K, const float* X, int64_t n, int64_t x_stride, int F, int transform,
-                    float* out, int64_t out_stride, uint32_t* leaf_idx);
-int predict_rows_per_block(int F);
+// ---- prediction (inference/gbdt/predictor.rs:197-302) ----------------------
+// The forest is cut into chunks of consecutive whole trees.  A tree of at
+// most kPredChunkNodes nodes whose features fit 16 bits is stored a second
+// time in a compact 8-byte layout (PredCNode) and belongs to an LDS chunk of
+// at most kPredChunkNodes nodes: a block stages one chunk at a time in LDS
+// (the next chunk's loads in flight while the current one is traversed) and
+// its lanes traverse from there.  Any other tree forms a chunk of its own
+// that is traversed from global memory in the 16-byte layout (n_nodes = 0).
+constexpr int kPredChunkNodes = 3584;  // 28 KiB of 8-byte nodes per LDS buffer
+constexpr int kPredMaxRows = 256;      // rows per block at most, one per lane
+// Compact node: x = feature (bits 0..15) | leaf (16) | default_left (17) |
+// categorical (18) | bits 19..31: the left child's id in the compact tree
+// (its right child is the next id), or for a leaf its original tree-local id
+// (leaf indices report original ids).  y = threshold bits, leaf value bits,
+// or for a categorical split its index in PredModel::cat_ref.
+constexpr uint32_t kCLeaf = 1u << 16, kCDefLeft = 1u << 17, kCCat = 1u << 18;
+constexpr int kCShift = 19;
+struct PredChunk {
+  uint32_t first_tree, end_tree;  // trees [first_tree, end_tree)
+  uint32_t node_begin;            // LDS chunk: first compact node (even); else unused
+  uint32_t n_nodes;               // compact nodes staged; 0: one tree read from global memory
+};
+struct PredModel {
+  const DevNode* nodes;        // 16-byte nodes, original ids (global-memory trees)
+  const int64_t* tree_off;     // [n_trees + 1] into nodes
+  const uint2* cnodes;         // compact nodes of the LDS chunks' trees
+  const uint32_t* ctree_off;   // [n_trees] first compact node of each compact tree
+  const uint32_t* cat_ref;     // [2 x categorical compact nodes]: cat_start, cat_size
+  const uint32_t* tree_group;
+  const uint32_t *cat_start, *cat_size, *cat_words;
+  const PredChunk* chunks;
+  int n_chunks;
+  int n_features;  // features the trees test (rows staged per block)
+  int has_cat;     // some compact node is a categorical split
+  int64_t n_trees;
+  const float* base;
+  int K;
+};
+// Rows per block and LDS bytes of a predict launch for a model testing F
+// features: rows are staged feature-major when F x rows x 4 B fits beside the
+// two node buffers (256, 128 or 64 rows); otherwise *stage_rows = false and
+// the kernel reads features from global memory (256 rows per block).
+size_t pred_lds_bytes(int F, size_t lds_limit, bool* stage_rows, int* rows_per_block);
+// out[k][row] = transform(base[k] + leaves of the trees of group k, in forest
+// order, f32).  leaf_idx nullable: [n_trees][n] tree-local leaf ids.
+// leaf_scratch non-null: two-phase launch for batches too small to fill the
+// device -- (row tile, chunk range) blocks write every tree's leaf value to
+// leaf_scratch [n_trees][n], then one pass per row adds them in forest order
+// (the same f32 sums, bit for bit, as the one-phase launch).
+void launch_predict(hipStream_t s, const PredModel& m, const float* X, int64_t n,
+                    int64_t x_stride, int F, int transform, float* out, int64_t out_stride,
+                    uint32_t* leaf_idx, float* leaf_scratch, size_t lds_limit, int n_cu);
+// The same over rows [0, n) of a larger call: X, out and leaf_idx rows are
+// addressed with their own strides (x_stride, out_stride, leaf_stride);
+// leaf_scratch holds [n_trees][n] for this launch.
+void launch_predict_rows(hipStream_t s, const PredModel& m, const float* X, int64_t n,
+                         int64_t x_stride, int F, int transform, float* out, int64_t out_stride,
+                         uint32_t* leaf_idx, int64_t leaf_stride, float* leaf_scratch,
+                         size_t lds_limit, int n_cu);
+// out[k][row] += the listed trees' leaves, in order (training-time updates).
+void launch_predict_accum(hipStream_t s, const PredModel& m, const float* X, int64_t n,
+                          int64_t x_stride, int F, float* out, int64_t out_stride,
+                          size_t lds_limit);
+// Whether launch_predict takes the two-phase path for n rows (and the scratch
+// it then needs, in floats).
+bool pred_two_phase(const PredModel& m, int64_t n, int n_cu, size_t lds_limit,
+                    int64_t* scratch_floats);
 void launch_copy16(hipStream_t s, void* dst, const void* src, int64_t bytes);
 constexpr int kPredMaxGroups = 16;
 

@@ -235,6 +235,7 @@ struct HistTuning {
 struct bst_ctx {
   int device = 0;
   int n_cu = 256;  // compute units (sizes grids of resident, grid-stride kernels)
+  size_t lds_per_block = 65536;  // LDS a workgroup may use (hipDeviceProp sharedMemPerBlock)
   hipStream_t stream = nullptr;
   double last_ms_total = 0, last_ms_per_tree = 0;
   int64_t last_hist_rows = 0;
@@ -267,6 +268,8 @@ struct bst_ctx {
     }
     return staging;
   }
+  // Per-tree leaf values of two-phase predictions (grow-only, freed with the ctx).
+  bst::DevBuf<float> pred_leaf;
   // Autotuner results per data shape, and the shape the last train used.
   std::map<std::vector<int64_t>, bst::HistTuning> tune_cache;
   // Device and pinned blocks of the last training run, reused by the next.
@@ -337,7 +340,35 @@ struct bst_model {
   bst::DevBuf<int64_t> d_tree_off;
   bst::DevBuf<uint32_t> d_tree_group, d_cat_start, d_cat_size, d_cat_words;
   bst::DevBuf<float> d_base;
+  // Predictor layout (engine.h): compact nodes of the LDS chunks' trees, their
+  // offsets, the categorical references and the chunk table.
+  bst::DevBuf<uint2> d_cnodes;
+  bst::DevBuf<uint32_t> d_ctree_off, d_cat_ref;
+  bst::DevBuf<bst::PredChunk> d_chunks;
+  int n_chunks = 0;
+  int pred_has_cat = 0;
   void upload(hipStream_t s);
+  // The device copy as the predictor's view (after upload).
+  bst::PredModel view() const {
+    bst::PredModel v{};
+    v.nodes = d_nodes.p;
+    v.tree_off = d_tree_off.p;
+    v.tree_group = d_tree_group.p;
+    v.cat_start = d_cat_start.p;
+    v.cat_size = d_cat_size.p;
+    v.cat_words = d_cat_words.p;
+    v.cnodes = d_cnodes.p;
+    v.ctree_off = d_ctree_off.p;
+    v.cat_ref = d_cat_ref.p;
+    v.chunks = d_chunks.p;
+    v.n_chunks = n_chunks;
+    v.n_features = (int)info.n_features;
+    v.has_cat = pred_has_cat;
+    v.n_trees = info.n_trees;
+    v.base = d_base.p;
+    v.K = (int)info.n_groups;
+    return v;
+  }
 };
 
 namespace bst {

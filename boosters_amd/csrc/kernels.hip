@@ -1951,61 +1951,26 @@ __device__ __forceinline__ float sigmoid32(float x) {  // transform.rs:106-115
 }
 
 constexpr int kPredMaxK = 16;
+constexpr int kPredILP = 8;  // trees a lane traverses together (independent chains)
+constexpr int kPredChunk16 = kPredChunkNodes / 2;          // uint4 per node buffer
+constexpr int kPredPer = (kPredChunk16 + kPredMaxRows - 1) / kPredMaxRows;  // staged uint4 per lane
 
-// One lane per row; the block's rows are staged feature-major in LDS so node
-// tests read s_x[f][lane] (consecutive lanes, no bank conflicts).  Trees are
-// visited in forest order and each group accumulates in f32 from its base
-// score, the reference's exact summation order.
-template <bool STAGE, bool ACCUM>
-__global__ __launch_bounds__(256) void k_predict(
-    const DevNode* __restrict__ nodes, const int64_t* __restrict__ tree_off,
-    const uint32_t* __restrict__ tree_group, int64_t n_trees, const uint32_t* __restrict__ cat_start,
-    const uint32_t* __restrict__ cat_size, const uint32_t* __restrict__ cat_words,
-    const float* __restrict__ base, int K, const float* __restrict__ X, int64_t n,
-    int64_t x_stride, int F, int transform, float* __restrict__ out, int64_t out_stride,
-    uint32_t* __restrict__ leaf_idx) {
-  extern __shared__ float s_x[];  // [F][blockDim.x]
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = row < n;
-  if (STAGE) {
-    for (int f = 0; f < F; f++) s_x[f * blockDim.x + threadIdx.x] = in ? X[(int64_t)f * x_stride + row] : 0.0f;
-    __syncthreads();
-  }
-  if (!in) return;
-  float acc[kPredMaxK];
-#pragma unroll
-  for (int k = 0; k < kPredMaxK; k++)
-    acc[k] = k < K ? (ACCUM ? out[(int64_t)k * out_stride + row] : base[k]) : 0.0f;
-  for (int64_t t = 0; t < n_trees; t++) {
-    const int64_t b = tree_off[t];
-    uint32_t node = 0;
-    DevNode nd = nodes[b];
-    while (!(nd.meta & kNodeLeaf)) {
-      const uint32_t f = nd.meta & 0xFFFFFFu;
-      const float x = STAGE ? s_x[f * blockDim.x + threadIdx.x] : X[(int64_t)f * x_stride + row];
-      if (isnan(x)) {
-        node = (nd.meta & kNodeDefLeft) ? nd.left : nd.right;
-      } else if (!(nd.meta & kNodeCat)) {
-        node = x < nd.value ? nd.left : nd.right;
-      } else {
-        const uint32_t cat = x <= 0.0f ? 0u : (x >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)x);
-        const uint32_t sz = cat_size[b + node], w = cat >> 5;
-        const bool right = sz != 0 && w < sz && ((cat_words[cat_start[b + node] + w] >> (cat & 31)) & 1u);
-        node = right ? nd.right : nd.left;
-      }
-      nd = nodes[b + node];
-    }
-    const uint32_t g = tree_group[t];
-#pragma unroll
-    for (int k = 0; k < kPredMaxK; k++)
-      if ((uint32_t)k == g) acc[k] += nd.value;
-    if (leaf_idx) leaf_idx[t * n + row] = node;
-  }
+// Categorical test of one raw value (tree_view.rs:131-146): (u32)x, Rust's
+// saturating cast, in the bitset -> right child.
+__device__ __forceinline__ bool pred_cat_right(float x, uint32_t start, uint32_t size,
+                                               const uint32_t* __restrict__ cat_words) {
+  const uint32_t cat = x <= 0.0f ? 0u : (x >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)x);
+  const uint32_t w = cat >> 5;
+  return size != 0 && w < size && ((cat_words[start + w] >> (cat & 31)) & 1u);
+}
+
+// Output transform of one row's accumulators (transform.rs:83-115).
+__device__ __forceinline__ void pred_transform(float* acc, int K, int transform) {
   if (transform == 1) {
 #pragma unroll
     for (int k = 0; k < kPredMaxK; k++)
       if (k < K) acc[k] = sigmoid32(acc[k]);
-  } else if (transform == 2) {  // softmax, transform.rs:83-103
+  } else if (transform == 2) {  // softmax, max-subtracted
     float mx = -INFINITY;
 #pragma unroll
     for (int k = 0; k < kPredMaxK; k++)
@@ -2023,56 +1988,363 @@ __global__ __launch_bounds__(256) void k_predict(
         if (k < K) acc[k] /= ssum;
     }
   }
+}
+
+// Traversal with the forest staged in LDS, one lane per row.
+//
+// A block owns blockDim.x rows (staged feature-major in LDS when STAGE_X, so
+// a node test reads s_x[f][lane]: consecutive lanes, conflict free) and walks
+// the chunks [c0, c1) of the forest in order.  An LDS chunk's compact nodes
+// are copied into one of two LDS buffers: the next chunk's global loads are
+// issued before the current chunk is traversed and written to the other
+// buffer after it, one barrier per chunk.  A lane follows kPredILP trees of
+// a chunk at once (independent chains of dependent LDS loads in flight) and
+// adds their leaves to its accumulators in forest order, so every row's f32
+// sum is the reference's (predictor.rs:227-229,280-301).  Trees kept in the
+// 16-byte layout are traversed from global memory the same way.
+//
+// MODE 0: out = transform(base + sum); MODE 1: out += sum (training updates);
+// MODE 2: each tree's leaf value to leaf_out[t][row] (two-phase launch; the
+// block's chunk range is blockIdx.y's).
+template <bool STAGE_X, int MODE, bool CAT>
+__global__ __launch_bounds__(kPredMaxRows) void k_predict_lds(
+    PredModel m, const float* __restrict__ X, int64_t n, int64_t x_stride, int transform,
+    float* __restrict__ out, int64_t out_stride, uint32_t* __restrict__ leaf_idx,
+    int64_t leaf_stride, float* __restrict__ leaf_out, int chunks_per_block) {
+  extern __shared__ uint4 s_mem[];  // [2][kPredChunk16] node buffers, then [F][rows] floats
+  const int R = blockDim.x;
+  float* s_x = reinterpret_cast<float*>(s_mem + 2 * kPredChunk16);
+  const int lane = threadIdx.x;
+  const int64_t row = (int64_t)blockIdx.x * R + lane;
+  const bool in = row < n;
+  const uint4* __restrict__ gn16 = reinterpret_cast<const uint4*>(m.nodes);
+  const uint4* __restrict__ gc16 = reinterpret_cast<const uint4*>(m.cnodes);
+  const int64_t* __restrict__ tree_off = m.tree_off;
+  const uint32_t* __restrict__ ctree_off = m.ctree_off;
+  const uint32_t* __restrict__ cat_ref = m.cat_ref;
+  const uint32_t* __restrict__ tree_group = m.tree_group;
+  const uint32_t* __restrict__ cat_start = m.cat_start;
+  const uint32_t* __restrict__ cat_size = m.cat_size;
+  const uint32_t* __restrict__ cat_words = m.cat_words;
+  const PredChunk* __restrict__ chunks = m.chunks;
+  const int K = m.K;
+  // Rows are read from X only when the model tests features (a forest of
+  // leaves may come with no feature rows at all).
+  const bool xok = in && m.n_features > 0;
+  if (STAGE_X) {
+    // Eight feature rows' loads in flight at a time.
+    const int nf = m.n_features;
+    const int64_t r = in ? row : 0;
+    int f = 0;
+    for (; f + 8 <= nf; f += 8) {
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = X[(int64_t)(f + q) * x_stride + r];
+#pragma unroll
+      for (int q = 0; q < 8; q++) s_x[(f + q) * R + lane] = in ? v[q] : 0.0f;
+    }
+    for (; f < nf; f++) s_x[f * R + lane] = in ? X[(int64_t)f * x_stride + r] : 0.0f;
+  }
+  const int c0 = MODE == 2 ? (int)blockIdx.y * chunks_per_block : 0;
+  const int c1 = MODE == 2 ? min(c0 + chunks_per_block, m.n_chunks) : m.n_chunks;
+  // The next LDS chunk's nodes, in flight during the current chunk.  Loads
+  // past the chunk's end read its first pair instead (always in range);
+  // only the chunk's own pairs are stored.
+  // Seven 16-byte pairs per lane cover a 1792-pair buffer at 256 lanes; with
+  // fewer lanes (R = 128 or 64 rows) the pairs past 7 R are copied when the
+  // buffer is written.
+  static_assert(kPredPer == 7, "seven staged pairs per lane");
+  uint4 p0, p1, p2, p3, p4, p5, p6;
+#define BST_PRED_L1(pj, j)                                                          \
+  {                                                                                 \
+    const uint32_t i_ = (uint32_t)(lane + (j) * R);                                 \
+    pj = gc16[b16_ + (i_ < n16_ ? i_ : 0u)];                                        \
+  }
+#define BST_PRED_S1(pj, j, dst)                                                     \
+  {                                                                                 \
+    const uint32_t i_ = (uint32_t)(lane + (j) * R);                                 \
+    if (i_ < n16_) (dst)[i_] = pj;                                                  \
+  }
+  // Loads of a chunk's pairs (a global-memory chunk, n_nodes 0, loads its
+  // first pair and stores nothing); loads past the end read the first pair.
+#define BST_PRED_LOAD(ch)                                                           \
+  do {                                                                              \
+    const uint32_t n16_ = ((ch).n_nodes + 1) / 2, b16_ = (ch).node_begin / 2;       \
+    BST_PRED_L1(p0, 0) BST_PRED_L1(p1, 1) BST_PRED_L1(p2, 2) BST_PRED_L1(p3, 3)     \
+    BST_PRED_L1(p4, 4) BST_PRED_L1(p5, 5) BST_PRED_L1(p6, 6)                         \
+  } while (0)
+#define BST_PRED_STORE(ch, dst)                                                     \
+  do {                                                                              \
+    const uint32_t n16_ = ((ch).n_nodes + 1) / 2, b16_ = (ch).node_begin / 2;       \
+    BST_PRED_S1(p0, 0, dst) BST_PRED_S1(p1, 1, dst) BST_PRED_S1(p2, 2, dst)         \
+    BST_PRED_S1(p3, 3, dst) BST_PRED_S1(p4, 4, dst) BST_PRED_S1(p5, 5, dst)         \
+    BST_PRED_S1(p6, 6, dst)                                                         \
+    for (uint32_t i_ = (uint32_t)(lane + kPredPer * R); i_ < n16_; i_ += (uint32_t)R) \
+      (dst)[i_] = gc16[b16_ + i_];                                                  \
+  } while (0)
+  float acc[kPredMaxK];
+  if (MODE != 2) {
+#pragma unroll
+    for (int k = 0; k < kPredMaxK; k++)
+      acc[k] = (k < K && in) ? (MODE == 1 ? out[(int64_t)k * out_stride + row] : m.base[k]) : 0.0f;
+  }
+  if (c0 < c1) {
+    const PredChunk ch = chunks[c0];
+    BST_PRED_LOAD(ch);
+    BST_PRED_STORE(ch, s_mem);
+  }
+  __syncthreads();
+  for (int c = c0; c < c1; c++) {
+    const PredChunk ch = chunks[c];
+    const bool next = c + 1 < c1;
+    const PredChunk chn = chunks[next ? c + 1 : c];
+    if (next) BST_PRED_LOAD(chn);
+    const uint2* sn = reinterpret_cast<const uint2*>(s_mem + ((c - c0) & 1) * kPredChunk16);
+    for (uint32_t t0 = ch.first_tree; t0 < ch.end_tree; t0 += kPredILP) {
+      // Chain j follows tree t0 + j from its root; a chain past the chunk's
+      // last tree starts (and stays) on a leaf.
+      uint32_t node[kPredILP], base[kPredILP], leafid[kPredILP];
+      float leafv[kPredILP];
+      bool act[kPredILP];
+      if (ch.n_nodes) {
+#pragma unroll
+        for (int j = 0; j < kPredILP; j++) {
+          const uint32_t t = t0 + j;
+          act[j] = t < ch.end_tree;
+          base[j] = act[j] ? ctree_off[t] - ch.node_begin : 0u;
+          node[j] = 0;
+          leafv[j] = 0.0f;
+          leafid[j] = 0;
+        }
+        bool any = true;
+        while (any) {
+          any = false;
+          uint2 v[kPredILP];
+#pragma unroll
+          for (int j = 0; j < kPredILP; j++) v[j] = sn[base[j] + node[j]];
+          // Branch-free step: every chain reads its feature value (a leaf's
+          // feature field is 0, a valid row of s_x) and the results are
+          // selected; only categorical forests (CAT) test bitsets, in a
+          // branch taken when some lane of the wave is on such a node.
+          float xv[kPredILP];
+#pragma unroll
+          for (int j = 0; j < kPredILP; j++) {
+            const uint32_t f = v[j].x & 0xFFFFu;
+            xv[j] = STAGE_X ? s_x[f * R + lane] : (xok ? X[(int64_t)f * x_stride + row] : 0.0f);
+          }
+#pragma unroll
+          for (int j = 0; j < kPredILP; j++) {
+            const bool leaf = (v[j].x & kCLeaf) != 0;
+            const float x = xv[j];
+            // NaN compares false: NaN goes left only with default_left.
+            bool left = (x < __uint_as_float(v[j].y)) || (isnan(x) && (v[j].x & kCDefLeft));
+            if (CAT && (v[j].x & kCCat) && !isnan(x))
+              left = !pred_cat_right(x, cat_ref[2 * v[j].y], cat_ref[2 * v[j].y + 1], cat_words);
+            const bool step = act[j] && !leaf;
+            leafv[j] = (act[j] && leaf) ? __uint_as_float(v[j].y) : leafv[j];
+            leafid[j] = (act[j] && leaf) ? (v[j].x >> kCShift) : leafid[j];
+            node[j] = step ? (v[j].x >> kCShift) + (left ? 0u : 1u) : node[j];
+            act[j] = step;
+            any = any || step;
+          }
+        }
+      } else {
+        // One tree in the 16-byte layout, from global memory.
+        const int64_t b = tree_off[t0];
+        uint32_t nd = 0;
+        uint4 v = gn16[b];
+        while (!(v.x & kNodeLeaf)) {
+          const uint32_t f = v.x & 0xFFFFFFu;
+          const float x = STAGE_X ? s_x[f * R + lane] : (xok ? X[(int64_t)f * x_stride + row] : 0.0f);
+          bool left;
+          if (isnan(x))
+            left = (v.x & kNodeDefLeft) != 0;
+          else if (!(v.x & kNodeCat))
+            left = x < __uint_as_float(v.y);
+          else
+            left = !pred_cat_right(x, cat_start[b + nd], cat_size[b + nd], cat_words);
+          nd = left ? v.z : v.w;
+          v = gn16[b + nd];
+        }
+#pragma unroll
+        for (int j = 0; j < kPredILP; j++) {
+          act[j] = false;
+          leafv[j] = 0.0f;
+          leafid[j] = 0;
+        }
+        leafv[0] = __uint_as_float(v.y);
+        leafid[0] = nd;
+      }
+      const uint32_t t_end = ch.n_nodes ? ch.end_tree : t0 + 1;
+#pragma unroll
+      for (int j = 0; j < kPredILP; j++) {
+        const uint32_t t = t0 + j;
+        if (t < t_end) {
+          if (in && leaf_idx) leaf_idx[(int64_t)t * leaf_stride + row] = leafid[j];
+          if (MODE == 2) {
+            if (in) leaf_out[(int64_t)t * n + row] = leafv[j];
+          } else if (K == 1) {
+            acc[0] += leafv[j];
+          } else {
+            const uint32_t g = tree_group[t];
+#pragma unroll
+            for (int k = 0; k < kPredMaxK; k++)
+              if ((uint32_t)k == g) acc[k] += leafv[j];
+          }
+        }
+      }
+      if (!ch.n_nodes) break;  // a global-memory chunk holds one tree
+    }
+    if (next) BST_PRED_STORE(chn, s_mem + ((c + 1 - c0) & 1) * kPredChunk16);
+    __syncthreads();
+  }
+#undef BST_PRED_L1
+#undef BST_PRED_S1
+#undef BST_PRED_LOAD
+#undef BST_PRED_STORE
+  if (MODE == 2 || !in) return;
+  if (MODE == 0) pred_transform(acc, K, transform);
 #pragma unroll
   for (int k = 0; k < kPredMaxK; k++)
     if (k < K) out[(int64_t)k * out_stride + row] = acc[k];
 }
 
-int predict_rows_per_block(int F) {
-  // Stage the block's rows in LDS when F x rows x 4 B stays within 64 KiB.
-  for (int rb = 256; rb >= 64; rb >>= 1)
-    if ((int64_t)F * rb * 4 <= 65536) return rb;
-  return 0;  // too many features to stage: read the raw matrix directly
+// Second phase of the two-phase launch: per row, base + every tree's leaf in
+// forest order (the one-phase kernel's exact f32 sum), then the transform.
+__global__ __launch_bounds__(256) void k_predict_sum(PredModel m, const float* __restrict__ leaf_in,
+                                                     int64_t n, int transform,
+                                                     float* __restrict__ out, int64_t out_stride) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  float acc[kPredMaxK];
+#pragma unroll
+  for (int k = 0; k < kPredMaxK; k++) acc[k] = k < m.K ? m.base[k] : 0.0f;
+  if (m.K == 1) {
+    for (int64_t t = 0; t < m.n_trees; t++) acc[0] += leaf_in[t * n + row];
+  } else {
+    for (int64_t t = 0; t < m.n_trees; t++) {
+      const uint32_t g = m.tree_group[t];
+      const float v = leaf_in[t * n + row];
+#pragma unroll
+      for (int k = 0; k < kPredMaxK; k++)
+        if ((uint32_t)k == g) acc[k] += v;
+    }
+  }
+  pred_transform(acc, m.K, transform);
+#pragma unroll
+  for (int k = 0; k < kPredMaxK; k++)
+    if (k < m.K) out[(int64_t)k * out_stride + row] = acc[k];
 }
 
-void launch_predict(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
-                    const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
-                    const uint32_t* cat_size, const uint32_t* cat_words, const float* base, int K,
-                    const float* X, int64_t n, int64_t x_stride, int F, int transform, float* out,
-                    int64_t out_stride, uint32_t* leaf_idx) {
-  if (n <= 0) return;
-  const int rb = predict_rows_per_block(F);
-  if (rb > 0) {
-    hipLaunchKernelGGL((k_predict<true, false>), dim3((unsigned)((n + rb - 1) / rb)), dim3(rb),
-                       (size_t)F * rb * 4, s, nodes, tree_off, tree_group, n_trees, cat_start,
-                       cat_size, cat_words, base, K, X, n, x_stride, F, transform, out, out_stride,
-                       leaf_idx);
-  } else {
-    hipLaunchKernelGGL((k_predict<false, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       nodes, tree_off, tree_group, n_trees, cat_start, cat_size, cat_words, base,
-                       K, X, n, x_stride, F, transform, out, out_stride, leaf_idx);
+size_t pred_lds_bytes(int F, size_t lds_limit, bool* stage_rows, int* rows_per_block) {
+  const size_t nodes = (size_t)2 * kPredChunk16 * sizeof(uint4);
+  F = std::max(F, 1);  // a leaf's feature field (0) always reads a staged row
+  for (int r = kPredMaxRows; r >= 64; r >>= 1) {
+    const size_t rows = (size_t)F * r * sizeof(float);
+    if (nodes + rows <= lds_limit) {
+      if (stage_rows) *stage_rows = true;
+      if (rows_per_block) *rows_per_block = r;
+      return nodes + rows;
+    }
   }
+  if (stage_rows) *stage_rows = false;
+  if (rows_per_block) *rows_per_block = kPredMaxRows;
+  return nodes;
 }
 
-// out[k][row] += leaf of each listed tree, trees in order (training-time and
-// validation prediction updates, trainer.rs:331-347).
-void launch_predict_accum(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
-                          const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
-                          const uint32_t* cat_size, const uint32_t* cat_words, int K,
-                          const float* X, int64_t n, int64_t x_stride, int F, float* out,
-                          int64_t out_stride) {
+namespace {
+// Raise a kernel's dynamic-LDS limit (on the current device) when a launch
+// needs more than the default 64 KiB.
+template <class Kern>
+void pred_allow_lds(Kern k, size_t bytes) {
+  if (bytes <= 65536) return;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <bool STAGE_X, int MODE, bool CAT>
+void launch_pred_lds3(hipStream_t s, const PredModel& m, dim3 grid, int R, size_t lds,
+                      const float* X, int64_t n, int64_t x_stride, int transform, float* out,
+                      int64_t out_stride, uint32_t* leaf_idx, int64_t leaf_stride,
+                      float* leaf_out, int chunks_per_block) {
+  pred_allow_lds(k_predict_lds<STAGE_X, MODE, CAT>, lds);
+  hipLaunchKernelGGL((k_predict_lds<STAGE_X, MODE, CAT>), grid, dim3(R), lds, s, m, X, n,
+                     x_stride, transform, out, out_stride, leaf_idx, leaf_stride, leaf_out,
+                     chunks_per_block);
+}
+
+template <int MODE>
+void launch_pred_lds(hipStream_t s, const PredModel& m, unsigned gy, const float* X, int64_t n,
+                     int64_t x_stride, int transform, float* out, int64_t out_stride,
+                     uint32_t* leaf_idx, int64_t leaf_stride, float* leaf_out,
+                     int chunks_per_block, size_t lds_limit) {
+  bool stage = false;
+  int R = kPredMaxRows;
+  const size_t lds = pred_lds_bytes(m.n_features, lds_limit, &stage, &R);
+  const dim3 grid((unsigned)((n + R - 1) / R), gy);
+  const bool cat = m.has_cat != 0;
+#define BST_PRED_GO(ST, CT)                                                                   \
+  launch_pred_lds3<ST, MODE, CT>(s, m, grid, R, lds, X, n, x_stride, transform, out,          \
+                                 out_stride, leaf_idx, leaf_stride, leaf_out, chunks_per_block)
+  if (stage && cat) BST_PRED_GO(true, true);
+  else if (stage) BST_PRED_GO(true, false);
+  else if (cat) BST_PRED_GO(false, true);
+  else BST_PRED_GO(false, false);
+#undef BST_PRED_GO
+}
+
+// Blocks that keep every CU busy on the two-phase path.
+int pred_target_blocks(int n_cu) { return 4 * std::max(n_cu, 1); }
+}  // namespace
+
+bool pred_two_phase(const PredModel& m, int64_t n, int n_cu, size_t lds_limit,
+                    int64_t* scratch_floats) {
+  int R = kPredMaxRows;
+  (void)pred_lds_bytes(m.n_features, lds_limit, nullptr, &R);
+  const int64_t tiles = (n + R - 1) / R;
+  const bool two = m.n_chunks > 1 && tiles < (int64_t)std::max(n_cu, 1);
+  if (scratch_floats) *scratch_floats = two ? m.n_trees * n : 0;
+  return two;
+}
+
+void launch_predict_rows(hipStream_t s, const PredModel& m, const float* X, int64_t n,
+                         int64_t x_stride, int F, int transform, float* out, int64_t out_stride,
+                         uint32_t* leaf_idx, int64_t leaf_stride, float* leaf_scratch,
+                         size_t lds_limit, int n_cu) {
+  (void)F;  // rows stage the model's features (F >= m.n_features is checked by the caller)
   if (n <= 0) return;
-  const int rb = predict_rows_per_block(F);
-  if (rb > 0) {
-    hipLaunchKernelGGL((k_predict<true, true>), dim3((unsigned)((n + rb - 1) / rb)), dim3(rb),
-                       (size_t)F * rb * 4, s, nodes, tree_off, tree_group, n_trees, cat_start,
-                       cat_size, cat_words, nullptr, K, X, n, x_stride, F, 0, out, out_stride,
-                       nullptr);
-  } else {
-    hipLaunchKernelGGL((k_predict<false, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       s, nodes, tree_off, tree_group, n_trees, cat_start, cat_size, cat_words,
-                       nullptr, K, X, n, x_stride, F, 0, out, out_stride, nullptr);
+  if (leaf_scratch && m.n_chunks > 1) {
+    int R = kPredMaxRows;
+    (void)pred_lds_bytes(m.n_features, lds_limit, nullptr, &R);
+    const int64_t tiles = (n + R - 1) / R;
+    const int want = (int)std::min<int64_t>(m.n_chunks,
+                                            (pred_target_blocks(n_cu) + tiles - 1) / tiles);
+    const int per = (m.n_chunks + want - 1) / want;
+    const unsigned groups = (unsigned)((m.n_chunks + per - 1) / per);
+    launch_pred_lds<2>(s, m, groups, X, n, x_stride, transform, nullptr, 0, leaf_idx, leaf_stride,
+                       leaf_scratch, per, lds_limit);
+    hipLaunchKernelGGL(k_predict_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m,
+                       leaf_scratch, n, transform, out, out_stride);
+    return;
   }
+  launch_pred_lds<0>(s, m, 1, X, n, x_stride, transform, out, out_stride, leaf_idx, leaf_stride,
+                     nullptr, m.n_chunks, lds_limit);
+}
+
+void launch_predict(hipStream_t s, const PredModel& m, const float* X, int64_t n,
+                    int64_t x_stride, int F, int transform, float* out, int64_t out_stride,
+                    uint32_t* leaf_idx, float* leaf_scratch, size_t lds_limit, int n_cu) {
+  launch_predict_rows(s, m, X, n, x_stride, F, transform, out, out_stride, leaf_idx, n,
+                      leaf_scratch, lds_limit, n_cu);
+}
+
+void launch_predict_accum(hipStream_t s, const PredModel& m, const float* X, int64_t n,
+                          int64_t x_stride, int F, float* out, int64_t out_stride,
+                          size_t lds_limit) {
+  (void)F;
+  if (n <= 0) return;
+  launch_pred_lds<1>(s, m, 1, X, n, x_stride, 0, out, out_stride, nullptr, 0, nullptr,
+                     m.n_chunks, lds_limit);
 }
 
 // Plain 16-byte streaming copy, the calibration point for HBM byte counters.

@@ -138,10 +138,9 @@ void launch_batch(bst_server* srv, Batch& b) {
     }
   });
   BST_HIP(hipMemcpyAsync(b.d_in.p, b.h_in, (size_t)b.rows * F * 4, hipMemcpyHostToDevice, b.stream));
-  bst_model* m = const_cast<bst_model*>(srv->model);
-  launch_predict(b.stream, m->d_nodes.p, m->d_tree_off.p, m->d_tree_group.p, m->info.n_trees,
-                 m->d_cat_start.p, m->d_cat_size.p, m->d_cat_words.p, m->d_base.p, K, b.d_in.p,
-                 b.rows, b.rows, F, 0, b.d_out.p, b.rows, nullptr);
+  const PredModel pm = srv->model->view();
+  launch_predict(b.stream, pm, b.d_in.p, b.rows, b.rows, F, 0, b.d_out.p, b.rows, nullptr,
+                 nullptr, srv->ctx->lds_per_block, srv->ctx->n_cu);
   BST_HIP(hipGetLastError());
   BST_HIP(hipMemcpyAsync(b.h_out, b.d_out.p, (size_t)b.rows * K * 4, hipMemcpyDeviceToHost, b.stream));
   BST_HIP(hipEventRecord(b.ev, b.stream));

@@ -77,11 +77,6 @@ int default_metric(int obj) {
 // Device metric partial sums (metrics/{classification,regression}.rs), launched from here.
 void launch_metric(hipStream_t s, int metric, int transform, const float* preds, int K,
                    const float* y, const float* w, int64_t n, double* partial, int n_blocks);
-void launch_predict_accum(hipStream_t s, const DevNode* nodes, const int64_t* tree_off,
-                          const uint32_t* tree_group, int64_t n_trees, const uint32_t* cat_start,
-                          const uint32_t* cat_size, const uint32_t* cat_words, int K,
-                          const float* X, int64_t n, int64_t x_stride, int F, float* out,
-                          int64_t out_stride);
 
 struct Trainer::TreeState {
   int t = 0;
@@ -645,6 +640,7 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
     bst_model one;
     one.info.n_trees = 1;
     one.info.n_groups = K;
+    one.info.n_features = ds_->d.F;
     one.base_score.assign(K, 0.0f);
     const size_t nn = tb.split_index.size();
     one.tree_offset = {0, (int64_t)nn};
@@ -676,9 +672,8 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   };
   // Validation predictions += the tree (needs one.upload first).
   auto valid_update = [&](bst_model& one) {
-    launch_predict_accum(s_, one.d_nodes.p, one.d_tree_off.p, one.d_tree_group.p, 1,
-                         one.d_cat_start.p, one.d_cat_size.p, one.d_cat_words.p, K, vX, n_valid,
-                         n_valid, ds_->d.F, vpreds_.p, n_valid);
+    launch_predict_accum(s_, one.view(), vX, n_valid, n_valid, ds_->d.F, vpreds_.p, n_valid,
+                         ctx_->lds_per_block);
   };
   auto append_tree = [&](bst_model* fm, const bst_model& one, int output) {
     const size_t nn = one.split_index.size();
@@ -824,9 +819,8 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
                              tree_output_.p, n_);
           BST_HIP(hipStreamSynchronize(s_));
         } else {
-          launch_predict_accum(s_, one.d_nodes.p, one.d_tree_off.p, one.d_tree_group.p, 1,
-                               one.d_cat_start.p, one.d_cat_size.p, one.d_cat_words.p, K,
-                               ds_->d.raw, n_, n_, ds_->d.F, preds_.p, n_);
+          launch_predict_accum(s_, one.view(), ds_->d.raw, n_, n_, ds_->d.F, preds_.p, n_,
+                               ctx_->lds_per_block);
         }
         if (vX && n_valid > 0) valid_update(one);
         if (need_dev) BST_HIP(hipStreamSynchronize(s_));
