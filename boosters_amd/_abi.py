@@ -268,6 +268,9 @@ def _declare(lib):
             C.c_int,
             [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, vp],
         ),
+        "bst_predict_device_noalloc": (
+            C.c_int, [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, vp],
+        ),
         "bst_server_create": (C.c_int, [vp, vp, C.c_int64, C.c_int32, P(vp)]),
         "bst_server_submit": (
             C.c_int,
@@ -326,7 +329,7 @@ def exported_symbols():
         "bst_last_train_tuning", "bst_model_info_get", "bst_model_export", "bst_model_import",
         "bst_model_destroy", "bst_model_save_bstr", "bst_model_load_bstr",
         "bst_model_save_bstr_file", "bst_model_load_bstr_file", "bst_model_save_json",
-        "bst_model_load_json", "bst_buffer_free", "bst_predict", "bst_predict_device", "bst_server_create",
+        "bst_model_load_json", "bst_buffer_free", "bst_predict", "bst_predict_device", "bst_predict_device_noalloc", "bst_server_create",
         "bst_server_submit", "bst_server_wait", "bst_server_destroy", "bst_kernel_gradients", "bst_kernel_lambdarank",
         "bst_kernel_histogram", "bst_dataset_hist_layout", "bst_kernel_partition",
         "bst_kernel_find_split",

@@ -6,11 +6,41 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
+#include <mutex>
 
 #include "host_common.h"
 #include "trainer.h"
 
 using namespace bst;
+
+// ---- static scratch of the allocation-free predict entry point ------------
+// bst_predict_device_noalloc's two-phase launches keep every tree's leaf value
+// per row here instead of in an allocated buffer.  One copy per GPU, for the
+// life of the process; its device address is looked up once per device
+// (bst_ctx_create, device current) and only that address is used from host
+// code.  Calls that overlap on one GPU share it and race.
+constexpr size_t kNoAllocScratchFloats = (size_t)8 << 20;  // 32 MiB
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current).
+float* scratch_on_device(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) raise(BST_E_DEVICE, "device index out of range");
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      raise(BST_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<float*>(p);
+  }
+  return g_scratch_dev[device];
+}
+}  // namespace
 
 namespace bst {
 std::string& last_error_ref() {
@@ -360,6 +390,7 @@ int bst_ctx_create(int device, bst_ctx** out) {
     if (prop.sharedMemPerBlock > 0) c->lds_per_block = prop.sharedMemPerBlock;
     BST_HIP(hipSetDevice(device));
     std::unique_ptr<bst_ctx> guard_c(c);
+    c->noalloc_scratch = scratch_on_device(device);
     BST_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = guard_c.release();
     return BST_OK;
@@ -872,6 +903,45 @@ int bst_predict_device(bst_ctx* ctx, const bst_model* m, const float* Xd, int64_
     set_device(ctx);
     if (n == 0) return BST_OK;
     predict_dev(ctx, m, Xd, n, F, raw, outd, leafd);
+    return BST_OK;
+  });
+}
+
+int bst_predict_device_noalloc(bst_ctx* ctx, const bst_model* m, const float* Xd, int64_t n,
+                               int32_t F, int32_t raw, float* outd, uint32_t* leafd) {
+  return guard([&] {
+    need(ctx && m && Xd && outd, BST_E_INVALID, "null argument");
+    need(n >= 0, BST_E_SHAPE, "negative row count");
+    need(m->device_for == ctx->device && m->d_nodes.p, BST_E_INVALID,
+         "model is not resident on this device: predict with it once through bst_predict or "
+         "bst_predict_device (which uploads it) before the allocation-free call");
+    need(F >= m->info.n_features, BST_E_SHAPE, "fewer features than the model was trained on");
+    need(ctx->noalloc_scratch != nullptr, BST_E_DEVICE, "scratch not set up on this device");
+    // A row of the two-phase launch needs one float per tree.
+    need(m->info.n_trees <= (int64_t)kNoAllocScratchFloats, BST_E_OOM,
+         "forest has more trees than the allocation-free scratch holds leaf values for "
+         "(8388608 per row)");
+    BST_HIP(hipSetDevice(ctx->device));
+    if (n == 0) return BST_OK;
+    const PredModel pm = m->view();
+    const int transform = raw ? 0 : m->info.output_transform;
+    if (!pred_two_phase(pm, n, ctx->n_cu, ctx->lds_per_block, nullptr)) {
+      launch_predict(ctx->stream, pm, Xd, n, n, F, transform, outd, n, leafd, nullptr,
+                     ctx->lds_per_block, ctx->n_cu);
+    } else {
+      // Row batches whose leaf values fit the scratch, one after another on
+      // the ctx stream (each reuses the scratch after the previous one's
+      // reads); every row's sum is the same as in one launch.
+      const int64_t per = std::max<int64_t>(
+          1, std::min<int64_t>(n, (int64_t)kNoAllocScratchFloats / std::max<int64_t>(pm.n_trees, 1)));
+      for (int64_t r0 = 0; r0 < n; r0 += per) {
+        const int64_t nb = std::min(per, n - r0);
+        launch_predict_rows(ctx->stream, pm, Xd + r0, nb, n, F, transform, outd + r0, n,
+                            leafd ? leafd + r0 : nullptr, n, ctx->noalloc_scratch,
+                            ctx->lds_per_block, ctx->n_cu);
+      }
+    }
+    BST_HIP(hipGetLastError());
     return BST_OK;
   });
 }

@@ -236,6 +236,9 @@ struct bst_ctx {
   int device = 0;
   int n_cu = 256;  // compute units (sizes grids of resident, grid-stride kernels)
   size_t lds_per_block = 65536;  // LDS a workgroup may use (hipDeviceProp sharedMemPerBlock)
+  // Device address of this device's static predict scratch (the
+  // allocation-free entry point's; looked up at creation).
+  float* noalloc_scratch = nullptr;
   hipStream_t stream = nullptr;
   double last_ms_total = 0, last_ms_per_tree = 0;
   int64_t last_hist_rows = 0;

@@ -298,6 +298,23 @@ int bst_predict_device(bst_ctx* ctx, const bst_model* model, const float* feat_m
                        int64_t n_rows, int32_t n_feat, int32_t raw, float* out_dev,
                        uint32_t* leaf_idx_dev);
 
+/* Allocation-free variant of bst_predict_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, same results bit for bit (leaf indices too).
+ * It calls no hipMalloc / hipFree: the model must already be resident on the
+ * ctx's device (any earlier bst_predict / bst_predict_device /
+ * bst_server_create with it uploads it; otherwise BST_E_INVALID), and the
+ * two-phase launch that small batches take keeps its per-row leaf values in
+ * a static 32 MiB device array (8,388,608 floats, one copy per GPU for the
+ * life of the process), processing rows in batches of 8388608 / n_trees.
+ * Forests with more trees than that are rejected (BST_E_OOM).  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU (two ctxs of
+ * the same device on different threads or streams) race. */
+int bst_predict_device_noalloc(bst_ctx* ctx, const bst_model* model, const float* feat_major_dev,
+                               int64_t n_rows, int32_t n_feat, int32_t raw, float* out_dev,
+                               uint32_t* leaf_idx_dev);
+
 /* ---- serving loop (extension: continuous batching) --------------------- */
 int bst_server_create(bst_ctx* ctx, const bst_model* model, int64_t hbm_budget_bytes,
                       int32_t max_inflight, bst_server** out);
