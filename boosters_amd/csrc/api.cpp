@@ -252,7 +252,7 @@ void bst_model::upload(hipStream_t s) {
   std::vector<uint32_t> new_id, order;
   auto compactable = [&](size_t t) {
     const int64_t b = tree_offset[t], e = tree_offset[t + 1];
-    if (e - b > kPredChunkNodes) return false;
+    if (e - b + 1 > kPredChunkNodes) return false;  // with the chunk's sentinel leaf
     for (int64_t k = b; k < e; k++)
       if (!is_leaf[k] && split_index[k] >= 65536u) return false;
     return true;
@@ -282,7 +282,7 @@ void bst_model::upload(hipStream_t s) {
       const int64_t k = b + o;
       uint2 w;
       if (is_leaf[k]) {
-        w.x = kCLeaf | (o << kCShift);
+        w.x = o | kCLeaf | (new_id[o] << kCShift);
         w.y = fbits(leaf_value[k]);
       } else {
         w.x = split_index[k] | (default_left[k] ? kCDefLeft : 0u) |
@@ -308,9 +308,12 @@ void bst_model::upload(hipStream_t s) {
     if (cn.size() & 1) cn.push_back(make_uint2(0u, 0u));  // chunks start 16-byte aligned
     const uint32_t begin = (uint32_t)cn.size();
     size_t e = t;
+    // The chunk's trees and its sentinel leaf fit the buffer.
     while (e < T && compactable(e) &&
-           cn.size() - begin + (size_t)(tree_offset[e + 1] - tree_offset[e]) <= (size_t)kPredChunkNodes)
+           cn.size() - begin + (size_t)(tree_offset[e + 1] - tree_offset[e]) + 1 <=
+               (size_t)kPredChunkNodes)
       append_compact(e++);
+    cn.push_back(make_uint2(kCLeaf, 0u));  // sentinel leaf
     ch.push_back({(uint32_t)t, (uint32_t)e, begin, (uint32_t)(cn.size() - begin)});
     t = e;
   }

@@ -297,11 +297,14 @@ void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const fl
 // that is traversed from global memory in the 16-byte layout (n_nodes = 0).
 constexpr int kPredChunkNodes = 3584;  // 28 KiB of 8-byte nodes per LDS buffer
 constexpr int kPredMaxRows = 256;      // rows per block at most, one per lane
-// Compact node: x = feature (bits 0..15) | leaf (16) | default_left (17) |
-// categorical (18) | bits 19..31: the left child's id in the compact tree
-// (its right child is the next id), or for a leaf its original tree-local id
-// (leaf indices report original ids).  y = threshold bits, leaf value bits,
-// or for a categorical split its index in PredModel::cat_ref.
+// Compact node, x: split: feature (bits 0..15) | default_left (17) |
+// categorical (18) | the left child's id in the compact tree (bits 19..31;
+// the right child is the next id); leaf: its original tree-local id (bits
+// 0..15, leaf indices report original ids) | leaf (16) | its own compact id
+// (bits 19..31), so that a traversal step on a leaf stays on it.  y:
+// threshold bits, leaf value bits, or for a categorical split its index in
+// PredModel::cat_ref.  Every LDS chunk ends with a sentinel leaf (x = leaf
+// flag only) that chains past the chunk's last tree sit on.
 constexpr uint32_t kCLeaf = 1u << 16, kCDefLeft = 1u << 17, kCCat = 1u << 18;
 constexpr int kCShift = 19;
 struct PredChunk {

@@ -2105,48 +2105,50 @@ __global__ __launch_bounds__(kPredMaxRows) void k_predict_lds(
       // last tree starts (and stays) on a leaf.
       uint32_t node[kPredILP], base[kPredILP], leafid[kPredILP];
       float leafv[kPredILP];
-      bool act[kPredILP];
       if (ch.n_nodes) {
+        // A chain past the chunk's last tree sits on the sentinel leaf.
 #pragma unroll
         for (int j = 0; j < kPredILP; j++) {
           const uint32_t t = t0 + j;
-          act[j] = t < ch.end_tree;
-          base[j] = act[j] ? ctree_off[t] - ch.node_begin : 0u;
+          base[j] = t < ch.end_tree ? ctree_off[t] - ch.node_begin : ch.n_nodes - 1;
           node[j] = 0;
-          leafv[j] = 0.0f;
-          leafid[j] = 0;
         }
-        bool any = true;
+        // Branch-free steps: every chain reads its node and its feature value
+        // (a leaf reads feature 0) and moves to left + (go right); a leaf
+        // moves to itself.  The loop ends when every chain is on a leaf,
+        // whose word is then in v.
+        uint2 v[kPredILP];
+        uint32_t any = 1;
         while (any) {
-          any = false;
-          uint2 v[kPredILP];
+          any = 0;
 #pragma unroll
           for (int j = 0; j < kPredILP; j++) v[j] = sn[base[j] + node[j]];
-          // Branch-free step: every chain reads its feature value (a leaf's
-          // feature field is 0, a valid row of s_x) and the results are
-          // selected; only categorical forests (CAT) test bitsets, in a
-          // branch taken when some lane of the wave is on such a node.
           float xv[kPredILP];
 #pragma unroll
           for (int j = 0; j < kPredILP; j++) {
-            const uint32_t f = v[j].x & 0xFFFFu;
+            const uint32_t leaf = (v[j].x >> 16) & 1u;
+            const uint32_t f = (v[j].x & 0xFFFFu) & (leaf - 1u);
             xv[j] = STAGE_X ? s_x[f * R + lane] : (xok ? X[(int64_t)f * x_stride + row] : 0.0f);
           }
 #pragma unroll
           for (int j = 0; j < kPredILP; j++) {
-            const bool leaf = (v[j].x & kCLeaf) != 0;
+            const uint32_t w = v[j].x;
+            const uint32_t leaf = (w >> 16) & 1u;
             const float x = xv[j];
             // NaN compares false: NaN goes left only with default_left.
-            bool left = (x < __uint_as_float(v[j].y)) || (isnan(x) && (v[j].x & kCDefLeft));
-            if (CAT && (v[j].x & kCCat) && !isnan(x))
-              left = !pred_cat_right(x, cat_ref[2 * v[j].y], cat_ref[2 * v[j].y + 1], cat_words);
-            const bool step = act[j] && !leaf;
-            leafv[j] = (act[j] && leaf) ? __uint_as_float(v[j].y) : leafv[j];
-            leafid[j] = (act[j] && leaf) ? (v[j].x >> kCShift) : leafid[j];
-            node[j] = step ? (v[j].x >> kCShift) + (left ? 0u : 1u) : node[j];
-            act[j] = step;
-            any = any || step;
+            uint32_t left = (x < __uint_as_float(v[j].y) ? 1u : 0u) |
+                            (isnan(x) ? ((w >> 17) & 1u) : 0u) | leaf;
+            if (CAT && (w & kCCat) && !isnan(x))
+              left = pred_cat_right(x, cat_ref[2 * v[j].y], cat_ref[2 * v[j].y + 1], cat_words)
+                         ? 0u : 1u;
+            node[j] = (w >> kCShift) + (left ^ 1u);
+            any |= leaf ^ 1u;
           }
+        }
+#pragma unroll
+        for (int j = 0; j < kPredILP; j++) {
+          leafv[j] = __uint_as_float(v[j].y);
+          leafid[j] = v[j].x & 0xFFFFu;
         }
       } else {
         // One tree in the 16-byte layout, from global memory.
@@ -2168,7 +2170,6 @@ __global__ __launch_bounds__(kPredMaxRows) void k_predict_lds(
         }
 #pragma unroll
         for (int j = 0; j < kPredILP; j++) {
-          act[j] = false;
           leafv[j] = 0.0f;
           leafid[j] = 0;
         }
