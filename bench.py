@@ -168,27 +168,97 @@ def random_forest(n_trees, depth, F, seed):
     return info, a
 
 
-def serving_bench(ctx, model, F, n_req, qps, seed, budget, inflight):
-    """Poisson arrivals at `qps`, rows log-uniform in [1, 100000]."""
+class HipEvents:
+    """Kernel time on a given stream, from HIP events recorded on it."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(self.a)) == 0
+        assert self.hip.hipEventCreate(C.byref(self.b)) == 0
+
+    def time_ms(self, stream, fn):
+        assert self.hip.hipEventRecord(self.a, stream) == 0
+        fn()
+        assert self.hip.hipEventRecord(self.b, stream) == 0
+        assert self.hip.hipEventSynchronize(self.b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), self.a, self.b) == 0
+        return ms.value
+
+    def close(self):
+        self.hip.hipEventDestroy(self.a)
+        self.hip.hipEventDestroy(self.b)
+
+
+def predict_roofline(ctx, model, F, K, n_trees, depth, n_rows=1 << 20, reps=5):
+    """bst_predict_device on n_rows device-resident rows, timed with HIP
+    events on the context's stream: rows/s, node visits per clock per CU and
+    the HBM fraction of the algorithmic bytes (F x 4 B in + K x 4 B out per
+    row)."""
+    L = A.lib()
+    Xb = np.random.default_rng(5).standard_normal((F, n_rows), dtype=np.float32)
+    xd = Dev(ctx, Xb)
+    od = Dev.empty(ctx, n_rows * 4 * K)
+    run = lambda: A.check(L.bst_predict_device(ctx, model, xd.p, n_rows, F, 1, od.p, None))  # noqa: E731
+    run()
+    A.check(L.bst_ctx_sync(ctx))
+    ev = HipEvents()
+    stream = L.bst_ctx_stream(ctx)
+    ms = ev.time_ms(stream, lambda: [run() for _ in range(reps)]) / reps
+    ev.close()
+    xd.free()
+    od.free()
+    rows_s = n_rows / (ms * 1e-3)
+    alg = n_rows * (F + K) * 4.0
+    achieved = alg / (ms * 1e-3) / 1e9
+    return {
+        "kernel": "k_predict_lds", "rows": n_rows, "avg_launch_us": round(ms * 1e3, 1),
+        "rows_per_s": rows_s, "node_visits_per_clk_per_cu": round(rows_s * n_trees * depth / (256 * 2.4e9), 3),
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4),
+    }
+
+
+def serving_requests(F, n_req, seed):
+    """Request row counts log-uniform in [1, 100000], features N(0,1)."""
     rng = np.random.default_rng(seed)
     rows = np.floor(np.exp(rng.uniform(0, math.log(100001), n_req))).astype(np.int64)
+    reqs = [rng.standard_normal((F, int(r)), dtype=np.float32) for r in rows]
+    return rows, reqs
+
+
+def serving_bench(ctx, model, F, n_req, qps, seed, budget, inflight, reqs=None, plain=True):
+    """Poisson arrivals at `qps`, rows log-uniform in [1, 100000]."""
+    if reqs is None:
+        rows, reqs = serving_requests(F, n_req, seed)
+    else:
+        rows, reqs = reqs
+    rng = np.random.default_rng(seed + 1)
     gaps = rng.exponential(1.0 / qps, n_req)
     arrivals = np.cumsum(gaps) - gaps[0]
-    reqs = [rng.standard_normal((F, int(r)), dtype=np.float32) for r in rows]
     outs = [np.zeros(int(r), np.float32) for r in rows]
     total_rows = int(rows.sum())
 
     # Plain: one synchronous predict launch per request, in arrival order.
     lat_plain = np.zeros(n_req)
-    t0 = time.perf_counter()
-    for i, X in enumerate(reqs):
-        now = time.perf_counter() - t0
-        if now < arrivals[i]:
-            time.sleep(arrivals[i] - now)
-        A.check(A.lib().bst_predict(ctx, model, A.ptr(X), X.shape[1], F, 1, A.ptr(outs[i]), None))
-        lat_plain[i] = time.perf_counter() - t0 - arrivals[i]
-    span_plain = time.perf_counter() - t0
-    plain_outs = [o.copy() for o in outs]
+    span_plain = float("nan")
+    plain_outs = None
+    if plain:
+        t0 = time.perf_counter()
+        for i, X in enumerate(reqs):
+            now = time.perf_counter() - t0
+            if now < arrivals[i]:
+                time.sleep(arrivals[i] - now)
+            A.check(A.lib().bst_predict(ctx, model, A.ptr(X), X.shape[1], F, 1, A.ptr(outs[i]), None))
+            lat_plain[i] = time.perf_counter() - t0 - arrivals[i]
+        span_plain = time.perf_counter() - t0
+        plain_outs = [o.copy() for o in outs]
 
     # Server: continuous batching.
     srv = C.c_void_p()
@@ -224,37 +294,98 @@ def serving_bench(ctx, model, F, n_req, qps, seed, budget, inflight):
     th.join()
     span_srv = t_end[0] - t0
     A.check(A.lib().bst_server_destroy(srv))
-    identical = all(np.array_equal(a, b) for a, b in zip(plain_outs, outs))
-    return {
+    res = {
         "requests": n_req,
         "qps_offered": qps,
         "rows_total": total_rows,
+        "rows_per_s_offered": total_rows / max(arrivals[-1], 1e-9),
         "server": {
             "rows_per_s": total_rows / span_srv,
             "p50_ms": float(np.percentile(lat_srv, 50) * 1e3),
             "p99_ms": float(np.percentile(lat_srv, 99) * 1e3),
         },
-        "plain": {
+    }
+    if plain:
+        res["plain"] = {
             "rows_per_s": total_rows / span_plain,
             "p50_ms": float(np.percentile(lat_plain, 50) * 1e3),
             "p99_ms": float(np.percentile(lat_plain, 99) * 1e3),
-        },
-        "results_identical": identical,
+        }
+        res["results_identical"] = all(np.array_equal(a, b) for a, b in zip(plain_outs, outs))
+    return res
+
+
+def serving_sweep(ctx, model, F, n_req, qps_list, seed, budget, inflight, slo_ms):
+    """The same requests offered at increasing Poisson rates.  Capacity of a
+    mode = the highest delivered rows/s among the rates at which its p99
+    latency stays within slo_ms (the same latency bound for both modes)."""
+    reqs = serving_requests(F, n_req, seed)
+    points = []
+    for q in qps_list:
+        r = serving_bench(ctx, model, F, n_req, q, seed, budget, inflight, reqs=reqs)
+        log(f"serving at {q:.0f} QPS: server {r['server']['rows_per_s'] / 1e6:.1f} M rows/s "
+            f"p99 {r['server']['p99_ms']:.1f} ms | plain {r['plain']['rows_per_s'] / 1e6:.1f} M rows/s "
+            f"p99 {r['plain']['p99_ms']:.1f} ms")
+        points.append(r)
+    cap = {}
+    for mode in ("server", "plain"):
+        ok = [p[mode]["rows_per_s"] for p in points if p[mode]["p99_ms"] <= slo_ms]
+        cap[mode] = max(ok) if ok else None
+    return {
+        "slo_p99_ms": slo_ms,
+        "capacity_rows_per_s": cap,
+        "points": [{"qps": p["qps_offered"], "server": p["server"], "plain": p["plain"],
+                    "identical": p["results_identical"]} for p in points],
     }
 
 
-def cpu_baseline(X, y, n_trees, depth, threads):
-    """The oracle (C++ restatement of the reference CPU path) on the same data."""
+def cpu_setup():
+    """The CPU baseline's oracle build and thread count on this host.
+
+    The oracle (a C++ restatement of the reference CPU path, the parity
+    checker) is recompiled here with -O3 -march=native for timing; the
+    threads are this process's CPU share (OMP_NUM_THREADS when set, else the
+    affinity mask), and the host's core count and CPU model are recorded."""
     from oracle import oracle as O
 
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity_cpus"] = os.cpu_count()
+    try:
+        with open("/proc/cpuinfo") as f:
+            info["cpu_model"] = next((ln.split(":", 1)[1].strip() for ln in f
+                                      if ln.startswith("model name")), "unknown")
+    except OSError:
+        info["cpu_model"] = "unknown"
+    try:
+        import tempfile
+        O.use_library(O.build_native(tempfile.mkdtemp(prefix="bst_oracle_")))
+        info["build"] = "g++ -O3 -march=native -fopenmp"
+    except Exception as e:  # noqa: BLE001 - fall back to the shipped checker build
+        log(f"native oracle build failed ({e}); timing the -O2 build")
+        info["build"] = "g++ -O2 -fopenmp (native build failed)"
+    share = int(os.environ.get("OMP_NUM_THREADS") or 0) or info["affinity_cpus"]
+    threads = max(1, min(share, info["affinity_cpus"]))
+    return O, threads, info
+
+
+def cpu_baseline(X, y, n_trees, depth):
+    """The oracle on the same data: ms/tree with the CPU share's threads
+    (n_trees trees) and with one thread (one tree); binning timed apart."""
+    O, threads, info = cpu_setup()
     O.set_threads(threads)
     t0 = time.perf_counter()
     b = O.Binned(X)
     t_bin = time.perf_counter() - t0
-    cfg = cfg_c2(n_trees, depth)
-    m = O.train(b, y, cfg)
+    m = O.train(b, y, cfg_c2(n_trees, depth))
     _, ms = m.stats()
-    return ms / n_trees, t_bin
+    O.set_threads(1)
+    m1 = O.train(b, y, cfg_c2(1, depth))
+    _, ms1 = m1.stats()
+    O.set_threads(threads)
+    return ms / n_trees, ms1, t_bin, threads, info
 
 
 # ---------------------------------------------------------------------------
@@ -409,9 +540,7 @@ def run_workload(args, L, ctx):
         },
     }
     if not args.no_cpu_baseline:
-        from oracle import oracle as O
-
-        threads = min(os.cpu_count() or 1, 16)
+        O, threads, cinfo = cpu_setup()
         O.set_threads(threads)
         m = min(n, args.cpu_rows)
         log(f"cpu baseline: oracle, {args.cpu_trees} round(s) on the first {m} rows, {threads} threads")
@@ -437,6 +566,7 @@ def run_workload(args, L, ctx):
             "cores": threads,
             "kind": "port",
             "sample": f"first {m} of {n} rows, {args.cpu_trees} round(s) ({args.cpu_trees * K} trees)",
+            **cinfo,
         }
     L.bst_model_destroy(h_model)
     print(json.dumps(result), flush=True)
@@ -457,6 +587,10 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--qps", type=float, default=1000.0)
+    ap.add_argument("--qps-sweep", default="500,1000,2000,4000,8000",
+                    help="offered request rates of the serving capacity sweep ('' = none)")
+    ap.add_argument("--slo-ms", type=float, default=50.0,
+                    help="p99 latency bound at which serving capacity is compared")
     ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4"], default="c2",
                     help="c2 is the headline workload; the others print their own line")
     ap.add_argument("--cpu-rows", type=int, default=200_000,
@@ -593,36 +727,32 @@ def main():
         fa = A.forest_struct(arrays)
         mh = C.c_void_p()
         A.check(L.bst_model_import(ctx, C.byref(info), C.byref(fa), C.byref(mh)))
-        # Device-resident batch throughput.
-        nb = 1 << 20
-        Xb = np.random.default_rng(5).standard_normal((100, nb), dtype=np.float32)
-        xd = Dev(ctx, Xb)
-        od = Dev.empty(ctx, nb * 4)
-        A.check(L.bst_predict_device(ctx, mh, xd.p, nb, 100, 1, od.p, None))
-        A.check(L.bst_ctx_sync(ctx))
-        t0 = time.perf_counter()
-        reps = 5
-        for _ in range(reps):
-            A.check(L.bst_predict_device(ctx, mh, xd.p, nb, 100, 1, od.p, None))
-        A.check(L.bst_ctx_sync(ctx))
-        pred_rows_s = reps * nb / (time.perf_counter() - t0)
-        xd.free()
-        od.free()
+        # Device-resident batch throughput, timed with HIP events.
+        pr = predict_roofline(ctx, mh, 100, 1, 500, 8)
+        log(f"predict: {pr['rows_per_s'] / 1e6:.1f} M rows/s, "
+            f"{pr['node_visits_per_clk_per_cu']} node visits/clk/CU")
         sv = serving_bench(ctx, mh, 100, args.requests, args.qps, seed=7, budget=4 << 30, inflight=4)
-        sv["device_batch_rows_per_s"] = pred_rows_s
+        sv["device_batch_rows_per_s"] = pr["rows_per_s"]
+        sv["roofline"] = pr
+        if args.qps_sweep:
+            sv["sweep"] = serving_sweep(ctx, mh, 100, args.requests,
+                                        [float(q) for q in args.qps_sweep.split(",")], seed=7,
+                                        budget=4 << 30, inflight=4, slo_ms=args.slo_ms)
         result["predict"] = sv
         L.bst_model_destroy(mh)
 
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        log(f"cpu baseline: oracle, {args.cpu_trees} trees on the same data, {threads} threads")
-        cpu_ms_tree, cpu_bin_s = cpu_baseline(X, y, args.cpu_trees, depth, threads)
+        log(f"cpu baseline: oracle, {args.cpu_trees} trees on the same data (+1 tree on one thread)")
+        cpu_ms_tree, cpu_ms1, cpu_bin_s, threads, cinfo = cpu_baseline(X, y, args.cpu_trees, depth)
         result["cpu_baseline"] = {
             "value": round(cpu_ms_tree, 2),
             "unit": "ms/tree",
             "cores": threads,
             "kind": "port",
-            "sample": f"same {n} x {F} matrix, {args.cpu_trees} trees (binning {cpu_bin_s:.1f} s excluded)",
+            "sample": f"same {n} x {F} matrix, {args.cpu_trees} trees on {threads} threads and 1 tree "
+                      f"on 1 thread (binning {cpu_bin_s:.1f} s excluded)",
+            "single_thread_ms_per_tree": round(cpu_ms1, 2),
+            **cinfo,
         }
     L.bst_model_destroy(h_model)
     print(json.dumps(result), flush=True)

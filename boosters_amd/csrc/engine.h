@@ -328,6 +328,45 @@ struct PredModel {
   const float* base;
   int K;
 };
+// ---- serving: one persistent traversal launch --------------------------------
+// The serving kernel keeps one block per CU resident and takes 256-row tiles
+// (fewer rows per tile when the model's features need more LDS) of the
+// batches the host publishes, in publication order, until the host asks it
+// to stop or no tile has come for its idle time.  Batches live in pinned,
+// coherent host memory (inputs [F][rows], outputs [K][rows]) that the kernel
+// reads and writes directly; the descriptors and counters below are the
+// whole protocol.
+constexpr int kServeRing = 16;  // batches in flight at most
+struct ServeDesc {
+  // The batch's tiles, one 64-bit word so that a reader never sees half of a
+  // rewrite: first tile number (tiles are numbered globally) << 16 | count.
+  uint64_t range;
+  uint64_t in_off, out_off;  // float offsets of its rows and outputs in the arenas
+  uint64_t done_target;  // ServeDevState::tiles_done[slot] once its last tile is done
+  uint32_t rows, seq;
+};
+constexpr uint32_t kServeMaxTiles = 65535;  // tiles of one batch (16 bits of range)
+struct ServeHostState {  // pinned coherent host memory
+  uint64_t published;    // tiles published so far (host; a release store)
+  uint32_t stop;         // host: exit now
+  uint32_t error;        // kernel: a claimed tile matched no descriptor (never expected)
+  ServeDesc desc[kServeRing];
+  uint32_t done_seq[kServeRing];  // kernel: seq of the slot's finished batch (a release store)
+};
+struct ServeDevState {   // device memory, zeroed before the first launch
+  uint64_t next_tile;    // the next tile to claim
+  uint64_t tiles_done[kServeRing];
+};
+// Tile rows of the serving kernel for a model testing F features (0 when the
+// rows cannot be staged in LDS).
+int serve_tile_rows(int F, size_t lds_limit);
+// One persistent launch of n_cu blocks; `h` is the device address of the
+// pinned state; the kernel exits when h->stop is set or after idle_ticks of
+// the device's constant-rate clock with nothing to claim.
+void launch_serve(hipStream_t s, const PredModel& m, ServeHostState* h, ServeDevState* d,
+                  const float* in_base, float* out_base, uint64_t idle_ticks, size_t lds_limit,
+                  int n_cu);
+
 // Rows per block and LDS bytes of a predict launch for a model testing F
 // features: rows are staged feature-major when F x rows x 4 B fits beside the
 // two node buffers (256, 128 or 64 rows); otherwise *stage_rows = false and

@@ -2006,16 +2006,20 @@ __device__ __forceinline__ void pred_transform(float* acc, int K, int transform)
 // MODE 0: out = transform(base + sum); MODE 1: out += sum (training updates);
 // MODE 2: each tree's leaf value to leaf_out[t][row] (two-phase launch; the
 // block's chunk range is blockIdx.y's).
+extern __shared__ uint4 s_pred_mem[];  // [2][kPredChunk16] node buffers, then [F][rows] floats
+
+// The rows [row0, row0 + blockDim.x) of a block through the chunks [c0, c1):
+// the body of k_predict_lds and of the serving kernel's tiles.
 template <bool STAGE_X, int MODE, bool CAT>
-__global__ __launch_bounds__(kPredMaxRows) void k_predict_lds(
-    PredModel m, const float* __restrict__ X, int64_t n, int64_t x_stride, int transform,
-    float* __restrict__ out, int64_t out_stride, uint32_t* __restrict__ leaf_idx,
-    int64_t leaf_stride, float* __restrict__ leaf_out, int chunks_per_block) {
-  extern __shared__ uint4 s_mem[];  // [2][kPredChunk16] node buffers, then [F][rows] floats
+__device__ __forceinline__ void pred_block(
+    PredModel m, const float* __restrict__ X, int64_t n, int64_t x_stride, int64_t row0, int c0,
+    int c1, int transform, float* __restrict__ out, int64_t out_stride,
+    uint32_t* __restrict__ leaf_idx, int64_t leaf_stride, float* __restrict__ leaf_out) {
+  uint4* s_mem = s_pred_mem;
   const int R = blockDim.x;
   float* s_x = reinterpret_cast<float*>(s_mem + 2 * kPredChunk16);
   const int lane = threadIdx.x;
-  const int64_t row = (int64_t)blockIdx.x * R + lane;
+  const int64_t row = row0 + lane;
   const bool in = row < n;
   const uint4* __restrict__ gn16 = reinterpret_cast<const uint4*>(m.nodes);
   const uint4* __restrict__ gc16 = reinterpret_cast<const uint4*>(m.cnodes);
@@ -2045,8 +2049,6 @@ __global__ __launch_bounds__(kPredMaxRows) void k_predict_lds(
     }
     for (; f < nf; f++) s_x[f * R + lane] = in ? X[(int64_t)f * x_stride + r] : 0.0f;
   }
-  const int c0 = MODE == 2 ? (int)blockIdx.y * chunks_per_block : 0;
-  const int c1 = MODE == 2 ? min(c0 + chunks_per_block, m.n_chunks) : m.n_chunks;
   // The next LDS chunk's nodes, in flight during the current chunk.  Loads
   // past the chunk's end read its first pair instead (always in range);
   // only the chunk's own pairs are stored.
@@ -2210,6 +2212,113 @@ __global__ __launch_bounds__(kPredMaxRows) void k_predict_lds(
     if (k < K) out[(int64_t)k * out_stride + row] = acc[k];
 }
 
+template <bool STAGE_X, int MODE, bool CAT>
+__global__ __launch_bounds__(kPredMaxRows) void k_predict_lds(
+    PredModel m, const float* __restrict__ X, int64_t n, int64_t x_stride, int transform,
+    float* __restrict__ out, int64_t out_stride, uint32_t* __restrict__ leaf_idx,
+    int64_t leaf_stride, float* __restrict__ leaf_out, int chunks_per_block) {
+  const int c0 = MODE == 2 ? (int)blockIdx.y * chunks_per_block : 0;
+  const int c1 = MODE == 2 ? min(c0 + chunks_per_block, m.n_chunks) : m.n_chunks;
+  pred_block<STAGE_X, MODE, CAT>(m, X, n, x_stride, (int64_t)blockIdx.x * blockDim.x, c0, c1,
+                                 transform, out, out_stride, leaf_idx, leaf_stride, leaf_out);
+}
+
+// ---- serving kernel --------------------------------------------------------
+// Claims tiles in order (a compare-and-swap on next_tile, only below the
+// published count, so a block never holds a tile that is not yet described),
+// finds the tile's batch among the ring's descriptors, traverses its rows
+// with the prediction body (raw margins), makes the outputs visible to the
+// host, and counts the tile done; the block that finishes a batch's last
+// tile publishes the batch's seq.  Every wave leaves the loop on the stop
+// flag, on idle_ticks of the constant-rate clock without work, or after a
+// bounded number of polls.
+template <bool CAT>
+__global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostState* h,
+                                                        ServeDevState* d,
+                                                        const float* __restrict__ in_base,
+                                                        float* __restrict__ out_base,
+                                                        uint64_t idle_ticks) {
+  __shared__ uint64_t s_tile;
+  __shared__ uint32_t s_slot;
+  constexpr uint32_t kMaxPolls = 1u << 20;  // 2^20 polls of several us: a bound on any wait
+  uint32_t cursor = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      uint64_t got = ~0ull;
+      const uint64_t t0 = wall_clock64();
+      for (uint32_t poll = 0; poll < kMaxPolls; poll++) {
+        const uint64_t pub = __hip_atomic_load(&h->published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        uint64_t cur = __hip_atomic_load(&d->next_tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur < pub) {
+          if (__hip_atomic_compare_exchange_strong(&d->next_tile, &cur, cur + 1, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            got = cur;
+            break;
+          }
+          continue;
+        }
+        if (__hip_atomic_load(&h->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
+        if (wall_clock64() - t0 > idle_ticks) break;
+        __builtin_amdgcn_s_sleep(127);  // ~8k cycles between polls of host memory
+      }
+      uint32_t slot = kServeRing;
+      if (got != ~0ull) {
+        for (uint32_t k = 0; k < (uint32_t)kServeRing; k++) {
+          const uint32_t i = (cursor + k) % kServeRing;
+          const uint64_t rg = __hip_atomic_load(&h->desc[i].range, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          const uint64_t b = rg >> 16, nt = rg & 0xFFFFu;
+          if (got >= b && got < b + nt) {
+            slot = i;
+            break;
+          }
+        }
+        if (slot == kServeRing) {
+          __hip_atomic_store(&h->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          got = ~0ull;
+        } else {
+          cursor = slot;
+        }
+      }
+      s_tile = got;
+      s_slot = slot;
+    }
+    __syncthreads();
+    const uint64_t g = s_tile;
+    const uint32_t slot = s_slot;
+    if (g == ~0ull) return;
+    const ServeDesc* dp = &h->desc[slot];
+    const uint64_t tile_begin =
+        __hip_atomic_load(&dp->range, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >> 16;
+    const uint64_t in_off = __hip_atomic_load(&dp->in_off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint64_t out_off = __hip_atomic_load(&dp->out_off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint64_t target = __hip_atomic_load(&dp->done_target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint32_t rows = __hip_atomic_load(&dp->rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint32_t seq = __hip_atomic_load(&dp->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const int64_t row0 = (int64_t)(g - tile_begin) * blockDim.x;
+    pred_block<true, 0, CAT>(m, in_base + in_off, rows, rows, row0, 0, m.n_chunks, 0,
+                             out_base + out_off, rows, nullptr, 0, nullptr);
+    __threadfence_system();  // this thread's outputs reach the host before the count
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t old = __hip_atomic_fetch_add(&d->tiles_done[slot], 1ull, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_SYSTEM);
+      if (old + 1 == target)
+        __hip_atomic_store(&h->done_seq[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
+
+// The serving kernel's LDS: the prediction body's, within the limit less
+// room for its own claim words.
+constexpr size_t kServeLdsReserve = 256;
+
+int serve_tile_rows(int F, size_t lds_limit) {
+  bool stage = false;
+  int R = kPredMaxRows;
+  (void)pred_lds_bytes(F, lds_limit - kServeLdsReserve, &stage, &R);
+  return stage ? R : 0;
+}
+
 // Second phase of the two-phase launch: per row, base + every tree's leaf in
 // forest order (the one-phase kernel's exact f32 sum), then the transform.
 __global__ __launch_bounds__(256) void k_predict_sum(PredModel m, const float* __restrict__ leaf_in,
@@ -2337,6 +2446,23 @@ void launch_predict(hipStream_t s, const PredModel& m, const float* X, int64_t n
                     uint32_t* leaf_idx, float* leaf_scratch, size_t lds_limit, int n_cu) {
   launch_predict_rows(s, m, X, n, x_stride, F, transform, out, out_stride, leaf_idx, n,
                       leaf_scratch, lds_limit, n_cu);
+}
+
+void launch_serve(hipStream_t s, const PredModel& m, ServeHostState* h, ServeDevState* d,
+                  const float* in_base, float* out_base, uint64_t idle_ticks, size_t lds_limit,
+                  int n_cu) {
+  bool stage = false;
+  int R = kPredMaxRows;
+  const size_t lds = pred_lds_bytes(m.n_features, lds_limit - kServeLdsReserve, &stage, &R);
+  if (m.has_cat) {
+    pred_allow_lds(k_serve<true>, lds);
+    hipLaunchKernelGGL(k_serve<true>, dim3((unsigned)std::max(n_cu, 1)), dim3(R), lds, s, m, h, d,
+                       in_base, out_base, idle_ticks);
+  } else {
+    pred_allow_lds(k_serve<false>, lds);
+    hipLaunchKernelGGL(k_serve<false>, dim3((unsigned)std::max(n_cu, 1)), dim3(R), lds, s, m, h, d,
+                       in_base, out_base, idle_ticks);
+  }
 }
 
 void launch_predict_accum(hipStream_t s, const PredModel& m, const float* X, int64_t n,

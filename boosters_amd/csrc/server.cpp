@@ -1,19 +1,28 @@
-// server.cpp — device-resident prediction serving with continuous batching.
+// server.cpp — device-resident prediction serving: continuous batching into
+// one persistent traversal launch.
 //
 // No counterpart in the reference (its predict path is GBDTModel::predict,
 // model/gbdt/model.rs:217-258, one call per batch).  Requests of any row count
-// are queued by bst_server_submit (thread-safe).  A worker thread admits
-// queued requests while their device footprint fits the free part of the HBM
-// budget, packs them side by side into one [F][rows] batch, and runs one
-// traversal launch for the whole batch.  Up to max_inflight batches are on the
-// device at once, each on its own stream, so the copies of one batch overlap
-// the traversal of another and new requests are admitted as soon as a batch
-// completes and frees its budget.  The worker only decides under the server's
-// lock (retire, admit, budget); packing requests into the pinned batch (split
-// over a few threads when large), the launches and unpacking results run
-// outside it, so submit and wait never queue behind a copy.  Every row is
-// traversed by the same kernel with the same per-row summation order, so a
-// result is bit-identical whether its request ran alone or packed with others.
+// are queued by bst_server_submit (thread-safe).  A worker thread coalesces
+// queued requests into batches, one per free slot of a ring of kServeRing
+// slots, packs each batch's rows side by side into the slot's pinned input
+// arena ([F][rows]), and publishes it to the serving kernel (k_serve): one
+// block per CU stays resident and takes the published batches' 256-row tiles
+// in order, so batches that arrive while others are on the GPU are traversed
+// alongside them.  The kernel reads the rows and writes the raw margins
+// directly in pinned coherent host memory; the block that finishes a batch's
+// last tile publishes the batch's sequence number, and the worker copies the
+// outputs to the requests and frees the slot.
+//
+// Admission: the slot arenas are sized at creation from the caller's budget,
+// capped by the device's free HBM (hipMemGetInfo), and allocated then; a
+// request is admitted when a slot is free and its rows fit the slot beside
+// the requests already packed there.  The kernel leaves after 50 ms without
+// work (or on destroy) and the worker relaunches it when a batch is published
+// and no launch is live.  Every row is traversed by the prediction kernel's
+// own tile code with the same per-row summation order, so a result is
+// bit-identical whether its request ran alone or packed with others, and
+// equal to bst_predict_device's.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -38,7 +47,6 @@ struct Request {
   int64_t n = 0;
   int32_t F = 0;
   float* out = nullptr;
-  int64_t bytes = 0;
   Clock::time_point t0;
   bool done = false;
   bool claimed = false;  // a bst_server_wait holds this ticket
@@ -46,17 +54,11 @@ struct Request {
   int64_t latency_ns = 0;
 };
 
-struct Batch {
-  hipStream_t stream = nullptr;
-  hipEvent_t ev = nullptr;
-  float* h_in = nullptr;   // pinned [F][cap]
-  float* h_out = nullptr;  // pinned [K][cap]
-  int64_t cap = 0;
-  DevBuf<float> d_in, d_out;
+struct Slot {
   std::vector<Request*> reqs;
-  int64_t rows = 0, bytes = 0;
-  bool busy = false;      // admitted, until its requests are done
-  bool retiring = false;  // its results are being unpacked
+  int64_t rows = 0;
+  uint32_t seq = 0;  // of the batch in flight
+  bool busy = false;
 };
 
 // Run fn(i) for i in [0, n) on up to `threads` threads (the caller's included).
@@ -84,16 +86,35 @@ int pack_threads(size_t bytes) {
   return (int)std::max(1u, std::min(8u, hw / 2));
 }
 
-}  // namespace
+constexpr int64_t kMaxSlotRows = 1 << 20;
+constexpr double kIdleMs = 50.0;
 
+}  // namespace
 
 struct bst_server {
   bst_ctx* ctx = nullptr;
   const bst_model* model = nullptr;
-  int64_t budget = 0;
-  int64_t used = 0;
+  int64_t budget = 0;      // admission budget after the free-HBM cap
   int32_t F = 0, K = 1;
-  std::vector<Batch> batches;
+  int tile_rows = 256;
+  int64_t slot_rows = 0;   // rows one batch holds
+  std::vector<Slot> slots;
+  // Pinned coherent host memory: state, input and output arenas, and their
+  // device addresses.
+  ServeHostState* hs = nullptr;
+  ServeHostState* hs_dev = nullptr;
+  float* in_arena = nullptr;
+  float* in_dev = nullptr;
+  float* out_arena = nullptr;
+  float* out_dev = nullptr;
+  DevBuf<ServeDevState> dstate;
+  hipStream_t stream = nullptr;
+  hipEvent_t kernel_done = nullptr;
+  bool launched = false;
+  uint64_t idle_ticks = 0;
+  uint64_t next_tile = 0;                // tiles published so far
+  uint64_t cum_done[kServeRing] = {};    // tiles_done[slot] once its batches finish
+  uint32_t next_seq = 1;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Request*> queue;
@@ -106,71 +127,65 @@ struct bst_server {
 
 namespace {
 
-void ensure_batch(Batch& b, int64_t rows, int32_t F, int32_t K) {
-  if (rows <= b.cap) return;
-  int64_t cap = std::max<int64_t>(rows, b.cap * 2);
-  if (b.h_in) (void)hipHostFree(b.h_in);
-  if (b.h_out) (void)hipHostFree(b.h_out);
-  b.h_in = b.h_out = nullptr;
-  BST_HIP(hipHostMalloc((void**)&b.h_in, (size_t)cap * F * sizeof(float), hipHostMallocDefault));
-  BST_HIP(hipHostMalloc((void**)&b.h_out, (size_t)cap * K * sizeof(float), hipHostMallocDefault));
-  b.d_in.alloc((size_t)cap * F);
-  b.d_out.alloc((size_t)cap * K);
-  b.cap = cap;
+// The persistent kernel is live, or (re)launched now.
+void ensure_kernel(bst_server* srv) {
+  if (srv->launched) {
+    const hipError_t q = hipEventQuery(srv->kernel_done);
+    if (q == hipErrorNotReady) return;
+    if (q != hipSuccess) raise(BST_E_DEVICE, std::string("serving kernel: ") + hipGetErrorString(q));
+  }
+  const bst_model* m = srv->model;
+  launch_serve(srv->stream, m->view(), srv->hs_dev, srv->dstate.p, srv->in_dev, srv->out_dev,
+               srv->idle_ticks, srv->ctx->lds_per_block, srv->ctx->n_cu);
+  BST_HIP(hipGetLastError());
+  BST_HIP(hipEventRecord(srv->kernel_done, srv->stream));
+  srv->launched = true;
 }
 
-void launch_batch(bst_server* srv, Batch& b) {
-  const int32_t F = srv->F, K = srv->K;
-  ensure_batch(b, b.rows, F, K);
-  // Pack requests side by side in the row dimension: [F][rows], one task per
-  // feature row.
-  std::vector<int64_t> offs(b.reqs.size());
+// Pack a slot's requests into its input arena and publish the batch.
+void publish(bst_server* srv, int si) {
+  Slot& s = srv->slots[si];
+  const int32_t F = srv->F;
+  const int64_t rows = s.rows;
+  float* base = srv->in_arena + (size_t)si * srv->slot_rows * F;
+  std::vector<int64_t> offs(s.reqs.size());
   int64_t off = 0;
-  for (size_t i = 0; i < b.reqs.size(); i++) {
+  for (size_t i = 0; i < s.reqs.size(); i++) {
     offs[i] = off;
-    off += b.reqs[i]->n;
+    off += s.reqs[i]->n;
   }
-  parallel_for(F, pack_threads((size_t)b.rows * F * 4), [&](int f) {
-    for (size_t i = 0; i < b.reqs.size(); i++) {
-      const Request* r = b.reqs[i];
-      std::memcpy(b.h_in + (size_t)f * b.rows + offs[i], r->X + (size_t)f * r->n,
-                  (size_t)r->n * 4);
+  parallel_for(F, pack_threads((size_t)rows * F * 4), [&](int f) {
+    for (size_t i = 0; i < s.reqs.size(); i++) {
+      const Request* r = s.reqs[i];
+      std::memcpy(base + (size_t)f * rows + offs[i], r->X + (size_t)f * r->n, (size_t)r->n * 4);
     }
   });
-  BST_HIP(hipMemcpyAsync(b.d_in.p, b.h_in, (size_t)b.rows * F * 4, hipMemcpyHostToDevice, b.stream));
-  const PredModel pm = srv->model->view();
-  launch_predict(b.stream, pm, b.d_in.p, b.rows, b.rows, F, 0, b.d_out.p, b.rows, nullptr,
-                 nullptr, srv->ctx->lds_per_block, srv->ctx->n_cu);
-  BST_HIP(hipGetLastError());
-  BST_HIP(hipMemcpyAsync(b.h_out, b.d_out.p, (size_t)b.rows * K * 4, hipMemcpyDeviceToHost, b.stream));
-  BST_HIP(hipEventRecord(b.ev, b.stream));
-  b.busy = true;
+  const uint32_t n_tiles = (uint32_t)((rows + srv->tile_rows - 1) / srv->tile_rows);
+  srv->cum_done[si] += n_tiles;
+  s.seq = srv->next_seq++;
+  ServeDesc& d = srv->hs->desc[si];
+  d.in_off = (uint64_t)si * srv->slot_rows * F;
+  d.out_off = (uint64_t)si * srv->slot_rows * srv->K;
+  d.done_target = srv->cum_done[si];
+  d.rows = (uint32_t)rows;
+  d.seq = s.seq;
+  __atomic_store_n(&d.range, (srv->next_tile << 16) | n_tiles, __ATOMIC_RELAXED);
+  srv->next_tile += n_tiles;
+  // The rows and the descriptor are written before the tiles are published.
+  __atomic_store_n(&srv->hs->published, srv->next_tile, __ATOMIC_RELEASE);
 }
 
-// Copy a finished batch's results to its requests (outside the lock).
-void unpack_batch(bst_server* srv, Batch& b) {
+// Copy a finished slot's outputs to its requests (outside the lock).
+void unpack(bst_server* srv, int si) {
+  const Slot& s = srv->slots[si];
   const int32_t K = srv->K;
+  const float* base = srv->out_arena + (size_t)si * srv->slot_rows * K;
   int64_t off = 0;
-  for (Request* r : b.reqs) {
+  for (Request* r : s.reqs) {
     for (int32_t k = 0; k < K; k++)
-      std::memcpy(r->out + (size_t)k * r->n, b.h_out + (size_t)k * b.rows + off, (size_t)r->n * 4);
+      std::memcpy(r->out + (size_t)k * r->n, base + (size_t)k * s.rows + off, (size_t)r->n * 4);
     off += r->n;
   }
-}
-
-// Mark a batch's requests done and free its budget (under the lock).
-void retire_batch(bst_server* srv, Batch& b) {
-  const auto now = Clock::now();
-  for (Request* r : b.reqs) {
-    r->latency_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(now - r->t0).count();
-    r->done = true;
-  }
-  b.reqs.clear();
-  b.busy = false;
-  b.retiring = false;
-  srv->used -= b.bytes;
-  b.bytes = 0;
-  b.rows = 0;
 }
 
 // The worker stopped on an error: every queued and admitted request fails
@@ -183,69 +198,84 @@ void fail_pending(bst_server* srv, int code, const std::string& msg) {
     r->done = true;
   }
   srv->queue.clear();
-  for (auto& b : srv->batches)
-    for (auto* r : b.reqs) {
+  for (auto& s : srv->slots)
+    for (auto* r : s.reqs) {
       r->status = code;
       r->done = true;
     }
   srv->cv_done.notify_all();
 }
 
+// Ask the kernel to leave and wait for it (bounded: it polls the flag).
+void stop_kernel(bst_server* srv) {
+  __atomic_store_n(&srv->hs->stop, 1u, __ATOMIC_RELEASE);
+  if (srv->launched) (void)hipEventSynchronize(srv->kernel_done);
+  srv->launched = false;
+}
+
 void worker_loop(bst_server* srv) {
   try {
     BST_HIP(hipSetDevice(srv->ctx->device));
-    const int64_t max_batch_rows = 1 << 20;
-    std::vector<Batch*> finished, admitted;
+    std::vector<int> finished, admitted;
     for (;;) {
       finished.clear();
       admitted.clear();
+      // Batches whose last tile is done (their seq published by the kernel).
+      for (int si = 0; si < (int)srv->slots.size(); si++) {
+        const Slot& s = srv->slots[si];
+        if (s.busy && __atomic_load_n(&srv->hs->done_seq[si], __ATOMIC_ACQUIRE) == s.seq)
+          finished.push_back(si);
+      }
+      if (__atomic_load_n(&srv->hs->error, __ATOMIC_ACQUIRE))
+        raise(BST_E_DEVICE, "serving kernel found a tile without a batch");
+      for (int si : finished) unpack(srv, si);
+      bool any_busy = false;
       {
         std::unique_lock<std::mutex> lk(srv->mu);
-        // Batches whose device work is done (results unpacked below).
-        for (auto& b : srv->batches) {
-          if (b.busy && !b.retiring && hipEventQuery(b.ev) == hipSuccess) {
-            b.retiring = true;
-            finished.push_back(&b);
+        const auto now = Clock::now();
+        for (int si : finished) {
+          Slot& s = srv->slots[si];
+          for (Request* r : s.reqs) {
+            r->latency_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(now - r->t0).count();
+            r->done = true;
           }
+          s.reqs.clear();
+          s.rows = 0;
+          s.busy = false;
         }
-        // Admit queued requests into a free batch while the budget allows.
-        for (auto& b : srv->batches) {
-          if (b.busy || srv->queue.empty()) continue;
-          while (!srv->queue.empty()) {
+        // Admit queued requests into free slots, coalescing while they fit.
+        for (int si = 0; si < (int)srv->slots.size() && !srv->queue.empty(); si++) {
+          Slot& s = srv->slots[si];
+          if (s.busy) continue;
+          while (!srv->queue.empty() && s.rows + srv->queue.front()->n <= srv->slot_rows) {
             Request* r = srv->queue.front();
-            if (srv->used + r->bytes > srv->budget) break;
-            if (b.rows > 0 && b.rows + r->n > max_batch_rows) break;
             srv->queue.pop_front();
-            b.reqs.push_back(r);
-            b.rows += r->n;
-            b.bytes += r->bytes;
-            srv->used += r->bytes;
+            s.reqs.push_back(r);
+            s.rows += r->n;
           }
-          if (!b.reqs.empty()) {
-            b.busy = true;
-            admitted.push_back(&b);
+          if (!s.reqs.empty()) {
+            s.busy = true;
+            admitted.push_back(si);
           }
         }
-        bool any_busy = false;
-        for (auto& b : srv->batches) any_busy = any_busy || b.busy;
-        if (srv->stop && srv->queue.empty() && !any_busy) return;
+        for (auto& s : srv->slots) any_busy = any_busy || s.busy;
+        if (srv->stop && srv->queue.empty() && !any_busy) {
+          lk.unlock();
+          if (!finished.empty()) srv->cv_done.notify_all();
+          stop_kernel(srv);
+          return;
+        }
         if (finished.empty() && admitted.empty() && !any_busy)
-          srv->cv_work.wait(lk, [&] { return srv->stop || !srv->queue.empty(); });
+          srv->cv_work.wait_for(lk, std::chrono::milliseconds(20),
+                                [&] { return srv->stop || !srv->queue.empty(); });
       }
-      // Copies and launches without the lock: only this thread touches an
-      // admitted or finished batch, and its requests' buffers belong to it
-      // until they are marked done.
-      for (Batch* b : admitted) launch_batch(srv, *b);
-      for (Batch* b : finished) unpack_batch(srv, *b);
-      if (!finished.empty()) {
-        {
-          std::lock_guard<std::mutex> lk(srv->mu);
-          for (Batch* b : finished) retire_batch(srv, *b);
-        }
-        srv->cv_done.notify_all();
-      }
-      if (finished.empty() && admitted.empty())
-        std::this_thread::sleep_for(std::chrono::microseconds(20));
+      if (!finished.empty()) srv->cv_done.notify_all();
+      // Packing and publishing outside the lock: only this thread touches an
+      // admitted slot, and its requests' buffers belong to it until done.
+      for (int si : admitted) publish(srv, si);
+      if (any_busy || !admitted.empty()) ensure_kernel(srv);
+      if (finished.empty() && admitted.empty() && any_busy)
+        std::this_thread::sleep_for(std::chrono::microseconds(10));
     }
   } catch (const Error& e) {
     fail_pending(srv, e.code, e.msg);
@@ -256,6 +286,7 @@ void worker_loop(bst_server* srv) {
   } catch (...) {
     fail_pending(srv, BST_E_DEVICE, "serving worker: unknown failure");
   }
+  stop_kernel(srv);
 }
 
 template <class Fn>
@@ -269,6 +300,31 @@ int sguard(Fn&& fn) {
     ::bst::last_error_ref() = e.what();
     return BST_E_INVALID;
   }
+}
+
+void release(bst_server* srv) {
+  if (srv->stream) (void)hipStreamSynchronize(srv->stream);
+  if (srv->kernel_done) (void)hipEventDestroy(srv->kernel_done);
+  if (srv->stream) (void)hipStreamDestroy(srv->stream);
+  if (srv->hs) (void)hipHostFree(srv->hs);
+  if (srv->in_arena) (void)hipHostFree(srv->in_arena);
+  if (srv->out_arena) (void)hipHostFree(srv->out_arena);
+  for (auto& kv : srv->reqs) delete kv.second;
+  delete srv;
+}
+
+template <class T>
+T* pinned(size_t bytes, T** dev) {
+  void* p = nullptr;
+  BST_HIP(hipHostMalloc(&p, bytes, hipHostMallocCoherent | hipHostMallocMapped));
+  void* d = nullptr;
+  const hipError_t e = hipHostGetDevicePointer(&d, p, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(p);
+    raise(BST_E_DEVICE, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+  }
+  *dev = static_cast<T*>(d);
+  return static_cast<T*>(p);
 }
 
 }  // namespace
@@ -287,19 +343,40 @@ int bst_server_create(bst_ctx* ctx, const bst_model* model, int64_t budget, int3
       m->upload(ctx->stream);
       m->device_for = ctx->device;
     }
+    const int32_t F = (int32_t)model->info.n_features, K = (int32_t)model->info.n_groups;
+    const int R = serve_tile_rows(F, ctx->lds_per_block);
+    if (R == 0) raise(BST_E_CONFIG, "the model tests too many features for the serving kernel's LDS tile");
     auto* srv = new bst_server();
+    std::unique_ptr<bst_server, void (*)(bst_server*)> guard_srv(srv, release);
     srv->ctx = ctx;
     srv->model = model;
-    srv->budget = budget;
-    srv->F = (int32_t)model->info.n_features;
-    srv->K = (int32_t)model->info.n_groups;
-    srv->batches.resize(std::max(1, std::min(max_inflight, 8)));
-    for (auto& b : srv->batches) {
-      BST_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
-      BST_HIP(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
-    }
+    srv->F = F;
+    srv->K = K;
+    srv->tile_rows = R;
+    // Budget: the caller's, capped by the device's free HBM now.
+    size_t free_b = 0, total_b = 0;
+    BST_HIP(hipMemGetInfo(&free_b, &total_b));
+    srv->budget = std::min<int64_t>(budget, (int64_t)(free_b / 10 * 9));
+    const int n_slots = std::max(1, std::min<int>(max_inflight, kServeRing));
+    const int64_t row_bytes = (int64_t)(std::max(F, 1) + K) * 4;
+    srv->slot_rows = std::min<int64_t>(kMaxSlotRows, srv->budget / (n_slots * row_bytes));
+    srv->slot_rows = std::min<int64_t>(srv->slot_rows / R * R, (int64_t)kServeMaxTiles * R);
+    if (srv->slot_rows < R) raise(BST_E_OOM, "serving budget holds less than one tile per slot");
+    srv->slots.resize(n_slots);
+    srv->hs = pinned<ServeHostState>(sizeof(ServeHostState), &srv->hs_dev);
+    std::memset(srv->hs, 0, sizeof(ServeHostState));
+    srv->in_arena = pinned<float>((size_t)n_slots * srv->slot_rows * std::max(F, 1) * 4, &srv->in_dev);
+    srv->out_arena = pinned<float>((size_t)n_slots * srv->slot_rows * K * 4, &srv->out_dev);
+    srv->dstate.alloc(1);
+    BST_HIP(hipStreamCreateWithFlags(&srv->stream, hipStreamNonBlocking));
+    BST_HIP(hipEventCreateWithFlags(&srv->kernel_done, hipEventDisableTiming));
+    BST_HIP(hipMemsetAsync(srv->dstate.p, 0, sizeof(ServeDevState), srv->stream));
+    BST_HIP(hipStreamSynchronize(srv->stream));
+    int rate_khz = 0;
+    BST_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+    srv->idle_ticks = (uint64_t)(std::max(rate_khz, 1) * kIdleMs);
     srv->worker = std::thread(worker_loop, srv);
-    *out = srv;
+    *out = guard_srv.release();
     return BST_OK;
   });
 }
@@ -310,16 +387,14 @@ int bst_server_submit(bst_server* srv, const float* X, int64_t n, int32_t F, flo
     if (!srv || !X || !out || !ticket) raise(BST_E_INVALID, "null argument");
     if (F != srv->F) raise(BST_E_SHAPE, "request feature count differs from the model's");
     if (n <= 0) raise(BST_E_SHAPE, "empty request");
+    if (n > srv->slot_rows)
+      raise(BST_E_OOM, "request has more rows than a serving batch holds (" +
+                           std::to_string(srv->slot_rows) + " under this budget)");
     auto* r = new Request();
     r->X = X;
     r->n = n;
     r->F = F;
     r->out = out;
-    r->bytes = n * (int64_t)(F + srv->K) * 4;
-    if (r->bytes > srv->budget) {
-      delete r;
-      raise(BST_E_OOM, "request exceeds the serving HBM budget");
-    }
     r->t0 = Clock::now();
     {
       std::lock_guard<std::mutex> lk(srv->mu);
@@ -368,15 +443,7 @@ int bst_server_destroy(bst_server* srv) {
     srv->cv_work.notify_all();
     if (srv->worker.joinable()) srv->worker.join();
     (void)hipSetDevice(srv->ctx->device);
-    for (auto& b : srv->batches) {
-      if (b.stream) (void)hipStreamSynchronize(b.stream);
-      if (b.h_in) (void)hipHostFree(b.h_in);
-      if (b.h_out) (void)hipHostFree(b.h_out);
-      if (b.ev) (void)hipEventDestroy(b.ev);
-      if (b.stream) (void)hipStreamDestroy(b.stream);
-    }
-    for (auto& kv : srv->reqs) delete kv.second;
-    delete srv;
+    release(srv);
     return BST_OK;
   });
 }

@@ -27,68 +27,85 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        P = C.POINTER
-        vp = C.c_void_p
-        sig = {
-            "ora_last_error": (C.c_char_p, []),
-            "ora_num_threads": (C.c_int, []),
-            "ora_set_num_threads": (None, [C.c_int]),
-            "ora_set_exp_mode": (None, [C.c_int]),
-            "ora_bin": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, P(C.c_uint8),
-                                  P(A.BinningConfig), P(vp)]),
-            "ora_binned_free": (None, [vp]),
-            "ora_feature_info": (C.c_int, [vp, C.c_int32, P(C.c_uint32), P(C.c_int32),
-                                           P(C.c_uint32), P(C.c_int32), P(C.c_int32)]),
-            "ora_bin_bounds": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_int64]),
-            "ora_get_bins": (C.c_int, [vp, P(C.c_uint16)]),
-            "ora_value_to_bin": (C.c_uint32, [vp, C.c_int32, C.c_double]),
-            "ora_leaf_weight": (C.c_float, [P(A.GBDTConfigC), C.c_double, C.c_double]),
-            "ora_hist_layout": (C.c_int, [vp, P(C.c_int32), P(C.c_uint32), P(C.c_uint32),
-                                          P(C.c_uint32)]),
-            "ora_gradients": (C.c_int, [C.c_int32, C.c_int32, P(C.c_float), P(C.c_float),
-                                        P(C.c_float), C.c_int64, P(C.c_float), P(C.c_float)]),
-            "ora_lambdarank_gradients": (C.c_int, [P(C.c_float), P(C.c_float), P(C.c_float),
-                                                   C.c_int64, P(C.c_int32), C.c_int64,
-                                                   P(C.c_float), P(C.c_float)]),
-            "ora_base_score": (C.c_int, [C.c_int32, C.c_int32, P(C.c_float), P(C.c_float),
-                                         C.c_int64, P(C.c_float)]),
-            "ora_histogram": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_uint32),
-                                        C.c_int64, P(C.c_double)]),
-            "ora_find_split": (C.c_int, [vp, P(C.c_double), C.c_double, C.c_double, C.c_uint32,
-                                         P(A.GBDTConfigC), C.c_int32, P(C.c_uint32), C.c_int32,
-                                         P(A.SplitInfoC)]),
-            "ora_partition": (C.c_int, [vp, P(C.c_uint32), C.c_int64, P(A.SplitInfoC),
-                                        P(C.c_int64)]),
-            "ora_train": (C.c_int, [vp, P(C.c_float), P(C.c_float), C.c_int64, P(C.c_float),
-                                    P(C.c_float), P(C.c_float), C.c_int64, P(A.GBDTConfigC),
-                                    P(A.ExtConfig), P(vp)]),
-            "ora_model_info": (C.c_int, [vp, P(A.ModelInfo)]),
-            "ora_model_export": (C.c_int, [vp, P(A.ForestArrays)]),
-            "ora_model_metric": (C.c_int, [vp, P(C.c_double), C.c_int64, P(C.c_int64)]),
-            "ora_model_stats": (C.c_int, [vp, P(C.c_int64), P(C.c_double)]),
-            "ora_model_free": (None, [vp]),
-            "ora_predict": (C.c_int, [P(A.ModelInfo), P(A.ForestArrays), P(C.c_float),
-                                      C.c_int64, C.c_int32, C.c_int32, P(C.c_float),
-                                      P(C.c_uint32)]),
-            "ora_metric": (C.c_double, [C.c_int32, P(C.c_float), C.c_int32, P(C.c_float),
-                                        P(C.c_float), C.c_int64]),
-            "ora_splitmix64_stream": (None, [C.c_uint64, C.c_int64, P(C.c_uint64)]),
-            "ora_partition_predict": (C.c_int, [vp, P(A.ModelInfo), P(A.ForestArrays),
-                                                P(C.c_uint8), P(C.c_float), P(C.c_float)]),
-            "ora_binning_sample": (C.c_int64, [C.c_int64, C.c_uint64, P(C.c_uint32)]),
-            "ora_sample_rows": (C.c_int, [P(C.c_float), P(C.c_float), C.c_int64, P(A.GBDTConfigC),
-                                          P(A.ExtConfig), C.c_uint32, C.c_uint32, P(C.c_float),
-                                          P(C.c_float), P(C.c_uint32), P(C.c_int64)]),
-            "ora_sample_cols": (C.c_int, [C.c_uint32, P(A.GBDTConfigC), C.c_uint32, C.c_uint32,
-                                          P(C.c_uint32), P(C.c_int32)]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _LIB = L
+        _LIB = _load(LIB_PATH)
     return _LIB
+
+
+def build_native(out_dir="/tmp"):
+    """Compile the -O3 -march=native timing build of the same restatement on
+    this machine (for the CPU baseline); returns its path."""
+    subprocess.run(["make", "-s", "-C", HERE, "native", f"OUT_DIR={out_dir}"], check=True)
+    return os.path.join(out_dir, "libbst_oracle_native.so")
+
+
+def use_library(path):
+    """Route every oracle call through another build of bst_oracle.cpp."""
+    global _LIB
+    _LIB = _load(path)
+
+
+def _load(path):
+    L = C.CDLL(path)
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "ora_last_error": (C.c_char_p, []),
+        "ora_num_threads": (C.c_int, []),
+        "ora_set_num_threads": (None, [C.c_int]),
+        "ora_set_exp_mode": (None, [C.c_int]),
+        "ora_bin": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, P(C.c_uint8),
+                              P(A.BinningConfig), P(vp)]),
+        "ora_binned_free": (None, [vp]),
+        "ora_feature_info": (C.c_int, [vp, C.c_int32, P(C.c_uint32), P(C.c_int32),
+                                       P(C.c_uint32), P(C.c_int32), P(C.c_int32)]),
+        "ora_bin_bounds": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_int64]),
+        "ora_get_bins": (C.c_int, [vp, P(C.c_uint16)]),
+        "ora_value_to_bin": (C.c_uint32, [vp, C.c_int32, C.c_double]),
+        "ora_leaf_weight": (C.c_float, [P(A.GBDTConfigC), C.c_double, C.c_double]),
+        "ora_hist_layout": (C.c_int, [vp, P(C.c_int32), P(C.c_uint32), P(C.c_uint32),
+                                      P(C.c_uint32)]),
+        "ora_gradients": (C.c_int, [C.c_int32, C.c_int32, P(C.c_float), P(C.c_float),
+                                    P(C.c_float), C.c_int64, P(C.c_float), P(C.c_float)]),
+        "ora_lambdarank_gradients": (C.c_int, [P(C.c_float), P(C.c_float), P(C.c_float),
+                                               C.c_int64, P(C.c_int32), C.c_int64,
+                                               P(C.c_float), P(C.c_float)]),
+        "ora_base_score": (C.c_int, [C.c_int32, C.c_int32, P(C.c_float), P(C.c_float),
+                                     C.c_int64, P(C.c_float)]),
+        "ora_histogram": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_uint32),
+                                    C.c_int64, P(C.c_double)]),
+        "ora_find_split": (C.c_int, [vp, P(C.c_double), C.c_double, C.c_double, C.c_uint32,
+                                     P(A.GBDTConfigC), C.c_int32, P(C.c_uint32), C.c_int32,
+                                     P(A.SplitInfoC)]),
+        "ora_partition": (C.c_int, [vp, P(C.c_uint32), C.c_int64, P(A.SplitInfoC),
+                                    P(C.c_int64)]),
+        "ora_train": (C.c_int, [vp, P(C.c_float), P(C.c_float), C.c_int64, P(C.c_float),
+                                P(C.c_float), P(C.c_float), C.c_int64, P(A.GBDTConfigC),
+                                P(A.ExtConfig), P(vp)]),
+        "ora_model_info": (C.c_int, [vp, P(A.ModelInfo)]),
+        "ora_model_export": (C.c_int, [vp, P(A.ForestArrays)]),
+        "ora_model_metric": (C.c_int, [vp, P(C.c_double), C.c_int64, P(C.c_int64)]),
+        "ora_model_stats": (C.c_int, [vp, P(C.c_int64), P(C.c_double)]),
+        "ora_model_free": (None, [vp]),
+        "ora_predict": (C.c_int, [P(A.ModelInfo), P(A.ForestArrays), P(C.c_float),
+                                  C.c_int64, C.c_int32, C.c_int32, P(C.c_float),
+                                  P(C.c_uint32)]),
+        "ora_metric": (C.c_double, [C.c_int32, P(C.c_float), C.c_int32, P(C.c_float),
+                                    P(C.c_float), C.c_int64]),
+        "ora_splitmix64_stream": (None, [C.c_uint64, C.c_int64, P(C.c_uint64)]),
+        "ora_partition_predict": (C.c_int, [vp, P(A.ModelInfo), P(A.ForestArrays),
+                                            P(C.c_uint8), P(C.c_float), P(C.c_float)]),
+        "ora_binning_sample": (C.c_int64, [C.c_int64, C.c_uint64, P(C.c_uint32)]),
+        "ora_sample_rows": (C.c_int, [P(C.c_float), P(C.c_float), C.c_int64, P(A.GBDTConfigC),
+                                      P(A.ExtConfig), C.c_uint32, C.c_uint32, P(C.c_float),
+                                      P(C.c_float), P(C.c_uint32), P(C.c_int64)]),
+        "ora_sample_cols": (C.c_int, [C.c_uint32, P(A.GBDTConfigC), C.c_uint32, C.c_uint32,
+                                      P(C.c_uint32), P(C.c_int32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
 
 
 def _check(code):
