@@ -597,6 +597,10 @@ def main():
                     help="rows of the CPU baseline sample for c1/c3/c4")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the plain copy kernel 5x on this many bytes (HBM counter calibration)")
+    ap.add_argument("--graph-only", action="store_true",
+                    help="only the warmup and timed graph-schedule runs (no per-kernel event run, "
+                         "no batched or plain schedule): the run a kernel trace of the graph "
+                         "schedule is taken on")
     ap.add_argument("--hist-shape", default="",
                     help="fixed histogram launch shape FG,ROWS_MIN,BLOCKS_PER_CU instead of the "
                          "autotuner's choice (e.g. 32,2048,1)")
@@ -642,6 +646,13 @@ def main():
     tune = tuning(ctx)
     ms_tree = wall_ms / args.steps
     log(f"graph: {ms_tree:.3f} ms/tree ({args.steps} trees in {wall_ms:.1f} ms)")
+
+    if args.graph_only:
+        L.bst_model_destroy(h_model)
+        print(json.dumps({"metric": "train ms/tree (graph schedule only)", "value": round(ms_tree, 4),
+                          "unit": "ms/tree", "steps": args.steps, "warmup": args.warmup,
+                          "hist_tuning": tune}), flush=True)
+        return
 
     # Kernel statistics: the same schedule launched eagerly with HIP events
     # around each kernel on the training stream (a separate, untimed run).

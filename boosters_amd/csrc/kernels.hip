@@ -21,6 +21,7 @@
 
 #include <stdexcept>
 #include <type_traits>
+#include <utility>
 
 namespace bst {
 
@@ -595,44 +596,84 @@ void launch_init_rows_sums(hipStream_t s, uint32_t* idx, float2* ogh, const floa
 // Histograms
 // ===========================================================================
 // One block-iteration = one HistItem: a tile of a node's rows x one group of
-// FG features.  The tile is walked in chunks of hist_chunk<FG>() rows.  Staging:
-// one row per thread loads the row id and the row's (g, h), quantises the
-// pair to fixed point once (not once per feature lane) and stores the row's
-// byte offset and the pair in LDS, two chunks ahead of the adds (three
-// staging buffers).  Histogramming: lane (threadIdx % FG) owns one feature
-// and threadIdx / FG picks a run of RPT consecutive staged rows, so the FG
-// lanes of a row read FG consecutive bytes of that row.  The bin (and
-// hi-bit) loads of chunk c+1 are issued before chunk c is added, into a
-// second register set, so their latency overlaps the adds.  The add loop
-// is about five VALU instructions per row and lane: no bounds tests (rows
-// past the tile's end are staged as row 0 with (g, h) = 0, padding
-// features read zero bytes of the padded row) and no select for the NaN
-// bin (bin | hi << 8 is 256 for it).  The LDS sub-histogram is
-// laid out [bin][FG]: lanes of one row always hit FG distinct 8-byte words,
-// so ds_add_u64 is bank-conflict free whatever the bins are, and hot bins
-// never collide inside a row.  Global layout per slot: [group][bin][FG] of
-// (g, h) int64 pairs, which the flush writes with unit stride.
+// FG features.  Lane (threadIdx % FG) owns one feature and each wave holds
+// WAVE / FG row slots, so the FG lanes of a row read FG consecutive bytes of
+// that row.  The LDS sub-histogram is laid out [bin][FG]: lanes of one row
+// always hit FG distinct 8-byte words, so ds_add_u64 is bank-conflict free
+// whatever the bins are, and hot bins never collide inside a row.  Global
+// layout per slot: [group][bin][FG] of (g, h) int64 pairs, which the flush
+// writes with unit stride.
+//
+// The LDS pipe is the limit of this kernel: per step of a wave (one row per
+// slot, FG features each) it performs the two 64-bit atomics (g, h), and
+// nothing else goes through LDS.  Rows are staged in registers: the tile's
+// rows are dealt to the 16 waves in wave-chunks of WR = slots x RPT rows,
+// wave-chunk k going to wave k % 16, and lane j < RPT of each 16-lane DPP
+// row of slot s loads row (s, j) of its wave's chunk (row id, (g, h)),
+// quantises (g, h) to fixed point once and keeps the row's byte offset and
+// the pair in registers.  A DPP row broadcast (row_newbcast) then hands row
+// u's offset and pair to every lane of its slot: VALU moves instead of LDS
+// reads.  Waves run independently between the tile's zeroing and its flush
+// (two block barriers per tile, none per row).  Per wave-chunk a lane issues
+// the bin loads of all RPT rows of its slot at once and then adds them, so
+// RPT rows of gathers are in flight per lane and the other waves of the SIMD
+// cover their latency; the staging loads (row ids, (g, h)) of the next
+// wave-chunk are issued before and consumed after, so their latency is
+// covered by this wave-chunk's adds.
+//
+// The add is a few VALU instructions per row and lane: no bounds tests (rows
+// past the tile's end are staged as row 0 with (g, h) = 0, padding features
+// read zero bytes of the padded row) and no select for the NaN bin
+// (bin | hi << 8 is 256 for it; row 256 takes every add with nothing to
+// count and is never read).
 //
 // XCD-aware item order: work-groups are dispatched round-robin over the 8
 // XCDs (block b runs on XCD b % 8), each with its own L2.  Position p of the
 // grid-stride walk maps to tile (p / (8 NG)) * 8 + p % 8 and group
 // (p / 8) % NG, so the NG group blocks of one tile (which read disjoint
-// 32-byte pieces of the same 128-byte rows) run on one XCD and share its L2
-// instead of each fetching every row from HBM.  The grid is a multiple of 8.
+// pieces of the same rows) run on one XCD and share its L2 instead of each
+// fetching every row from HBM.  The grid is a multiple of 8.
 //
 // OFF64: row byte offsets need 64 bits (n * S >= 2^32); otherwise they are
 // 32-bit offsets from the uniform base, which keeps address registers small.
 constexpr int kHistThreads = 1024;
-constexpr int kHistRowBatch = 4;  // rows whose (g, h) are read from LDS together
-constexpr int kHistBufs = 3;      // staging buffers: chunks c (adds), c+1 (bin loads), c+2
-// Rows per staged chunk.  FG = 32 keeps a 128 KB sub-histogram: one block per
-// CU, 16 rows per thread per chunk.  FG = 16 halves it and the staging, so
-// that two blocks fit a CU's 160 KB of LDS (74 KB each) at 8 waves per SIMD,
-// with 4 rows per thread per chunk.
+constexpr int kHistWaves = kHistThreads / WAVE;
+#ifndef BST_HIST_RPT16
+#define BST_HIST_RPT16 8
+#endif
+// Rows per slot per wave-chunk (at most 16: one staging lane per row in each
+// 16-lane DPP row): the bin gathers a lane keeps in flight.
 template <int FG>
-constexpr int hist_chunk() { return FG == 16 ? 256 : 512; }
+constexpr int hist_rpt() { return FG == 16 ? BST_HIST_RPT16 : 8; }
 template <int FG>
 constexpr int hist_waves_per_eu() { return FG == 16 ? 8 : 4; }
+
+// Lane U's value of each 16-lane row, in every lane of that row.
+template <int U>
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + U, 0xF, 0xF, false);
+}
+template <int U>
+__device__ __forceinline__ long long row_bcast64(long long v) {
+  const uint32_t lo = row_bcast<U>((uint32_t)(unsigned long long)v);
+  const uint32_t hi = row_bcast<U>((uint32_t)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+template <typename F, int... U>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, U...>) {
+  (f(std::integral_constant<int, U>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// A staged row in its staging lane: byte offset (row id with OFF64) and the
+// quantised (g, h).
+struct HistStage {
+  uint32_t r;
+  long long g, h;
+};
 
 template <int FG, bool GATHER, bool HAS_HI, bool OFF64>
 __global__ __launch_bounds__(kHistThreads)
@@ -644,27 +685,30 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
     long long* __restrict__ node_sum, long long* __restrict__ partial,
     const uint32_t* __restrict__ dyn, const uint32_t* __restrict__ idx_alt,
     const float2* __restrict__ ogh_alt) {
-  // Bins 0..255, plus row 256 that takes every add with nothing to count
-  // (the NaN bin, rows past the tile end, padding features) so that the
-  // adds need no branch; row 256 is never read.
+  // Bins 0..255, plus row 256 that takes every add with nothing to count.
   // One array, g plane then h plane, so that an h add reaches its column
   // with the g address plus an immediate offset.
   __shared__ long long s_gh[2 * 257 * FG];
   long long* const s_g = s_gh;
   long long* const s_h = s_gh + 257 * FG;
-  constexpr int kHistChunk = hist_chunk<FG>();
-  __shared__ uint32_t s_row[kHistBufs][kHistChunk];
-  __shared__ longlong2 s_q[kHistBufs][kHistChunk];
-  __shared__ long long s_red[2][kHistThreads / WAVE];
-  constexpr int RPP = kHistThreads / FG;  // row runs per chunk
-  constexpr int RPT = kHistChunk / RPP;   // rows per thread per chunk
-  static_assert(RPT >= 1 && kHistChunk <= kHistThreads && RPT % kHistRowBatch == 0,
-                "chunk shape");
+  constexpr int RS = WAVE / FG;        // row slots per wave
+  constexpr int RPT = hist_rpt<FG>();  // rows per slot per wave-chunk
+  constexpr int WR = RS * RPT;         // rows per wave-chunk
+  static_assert(RPT <= 16 && RPT % 2 == 0, "wave-chunk shape");
+  __shared__ long long s_red[2][kHistWaves];
   const uint32_t n_items = dyn ? *dyn : n_items_static;
   const uint32_t n_tiles = n_items / (uint32_t)NG;
   const uint32_t n_pos = ((n_tiles + 7u) / 8u) * 8u * (uint32_t)NG;
-  const int lf = threadIdx.x % FG;
-  const int rs = threadIdx.x / FG;
+  const int wv = threadIdx.x / WAVE;
+  const int ln = threadIdx.x % WAVE;
+  const int lf = ln % FG;
+  const int rs = ln / FG;
+  // Staging lanes: lane j < RPT of each 16-lane DPP row stages row (rs, j);
+  // with FG = 32 both DPP rows of a slot stage the same rows (each needs its
+  // own copy for the broadcast) and only the first counts it in the totals.
+  const int sj = ln & 15;
+  const bool stager = sj < RPT;
+  const bool counter = stager && lf < 16;
   constexpr uint32_t kPlaneBytes = 257u * FG * 8u;  // g plane to h plane
   char* const lane_g = reinterpret_cast<char*>(s_g + lf);
   for (uint32_t p = blockIdx.x; p < n_pos; p += gridDim.x) {
@@ -685,27 +729,27 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
     // there, so they add into columns nobody reads.
     const int f = (int)it.group * FG + lf;
     const int hb = hi_off + (f >> 3), hs = f & 7;
-  // The NaN bit (bit hs of its byte) shifted to the bin-256 bit of the
-  // byte offset below: bit log2(256 * FG * 8).
-  constexpr int kHiBit = FG == 16 ? 15 : 16;
-  static_assert((1 << kHiBit) == 256 * FG * 8, "hi bit position");
-  const uint32_t hsh = (uint32_t)(kHiBit - hs);
+    // The NaN bit (bit hs of its byte) shifted to the bin-256 bit of the
+    // byte offset below: bit log2(256 * FG * 8).
+    constexpr int kHiBit = FG == 16 ? 15 : 16;
+    static_assert((1 << kHiBit) == 256 * FG * 8, "hi bit position");
+    const uint32_t hsh = (uint32_t)(kHiBit - hs);
     const double sg = ldexp(1.0, qexp[2 * tree]);
     const double sh = ldexp(1.0, qexp[2 * tree + 1]);
     const uint32_t* tidx = (alt ? idx_alt : idx) + (int64_t)tree * tree_stride;
     const float2* togh = (alt ? ogh_alt : ogh) + (int64_t)tree * tree_stride;
-    const uint32_t n_chunks = (it.count + kHistChunk - 1) / kHistChunk;
+    // This wave's wave-chunks: k = 0 .. nk - 1 are the tile's wave-chunks
+    // k * 16 + wv.
+    const uint32_t n_wc = (it.count + WR - 1) / WR;
+    const uint32_t nk = n_wc > (uint32_t)wv ? (n_wc - (uint32_t)wv + kHistWaves - 1) / kHistWaves : 0u;
     long long accg = 0, acch = 0;
-    // Staging registers: this thread's row of the next chunk.  A staged row
-    // is its byte offset in bins_rm (its id when offsets need 64 bits) and
-    // its quantised (g, h); rows past the tile's end stage row 0 with (0, 0),
-    // so the adds need no bounds checks (they add zeros).
+    // Raw staging registers of a staging lane: rows past the tile's end stage
+    // row 0 with (0, 0), so the adds need no bounds checks (they add zeros).
     uint32_t st_r = 0;
     float2 st_gh = make_float2(0.0f, 0.0f);
     bool st_ok = false;
-    const bool stager = threadIdx.x < kHistChunk;
-    auto stage_load = [&](uint32_t c) {
-      const uint32_t q = c * kHistChunk + threadIdx.x;
+    auto stage_load = [&](uint32_t k) {
+      const uint32_t q = (k * kHistWaves + (uint32_t)wv) * WR + (uint32_t)(rs * RPT + sj);
       st_ok = stager && q < it.count;
       if (st_ok) {
         const uint32_t pos = it.begin + q;
@@ -713,25 +757,23 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
         st_gh = togh[pos];
       }
     };
-    auto stage_store = [&](int buf) {
-      if (stager) {
-        const long long gq = st_ok ? __double2ll_rn((double)st_gh.x * sg) : 0ll;
-        const long long hq = st_ok ? __double2ll_rn((double)st_gh.y * sh) : 0ll;
-        s_row[buf][threadIdx.x] = st_ok ? (OFF64 ? st_r : st_r * (uint32_t)S) : 0u;
-        s_q[buf][threadIdx.x] = make_longlong2(gq, hq);
-        accg += gq;
-        acch += hq;
+    auto stage_quant = [&](HistStage& st) {
+      st.r = st_ok ? (OFF64 ? st_r : st_r * (uint32_t)S) : 0u;
+      st.g = st_ok ? __double2ll_rn((double)st_gh.x * sg) : 0ll;
+      st.h = st_ok ? __double2ll_rn((double)st_gh.y * sh) : 0ll;
+      if (counter) {
+        accg += st.g;
+        acch += st.h;
       }
     };
-    // Bin loads of chunk c from the staged row offsets.  The loaded bytes stay
-    // uint8_t until they are used: widened where they are loaded, the
-    // compiler would widen them in another block and wait for the loads
-    // there.
-    auto load_bins = [&](uint32_t c, uint8_t* bv, uint8_t* hv) {
-      const int buf = (int)(c % kHistBufs);
-#pragma unroll
-      for (int u = 0; u < RPT; u++) {
-        const uint32_t r = s_row[buf][rs * RPT + u];
+    // Bin loads of a wave-chunk: row u's offset broadcast from its staging
+    // lane.  The loaded bytes stay uint8_t until they are used: widened where
+    // they are loaded, the compiler would widen them in another block and
+    // wait for the loads there.
+    auto load_bins = [&](const HistStage& st, uint8_t* bv, uint8_t* hv) {
+      static_for<RPT>([&](auto U) {
+        constexpr int u = decltype(U)::value;
+        const uint32_t r = row_bcast<u>(st.r);
         if (OFF64) {
           const uint8_t* rp = bins_rm + (uint64_t)r * (uint64_t)S;
           bv[u] = rp[f];
@@ -740,88 +782,50 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
           bv[u] = bins_rm[r + (uint32_t)f];
           hv[u] = HAS_HI ? bins_rm[r + (uint32_t)hb] : 0u;
         }
-      }
+      });
     };
-    // Pipeline: chunk c+2 is staged (row ids, quantised (g, h)) and chunk
-    // c+1's bins are in flight while chunk c is added, so the latency of the
-    // gathered bin loads overlaps the LDS adds.  Three staging buffers.
-    if (n_chunks > 0) {
+    auto add_rows = [&](const HistStage& st, const uint8_t (&cb)[RPT], const uint8_t (&ch)[RPT]) {
+      static_for<RPT>([&](auto U) {
+        constexpr int u = decltype(U)::value;
+        const long long g = row_bcast64<u>(st.g);
+        const long long h = row_bcast64<u>(st.h);
+        // Byte offset of (bin, lf) in the g plane: bin * FG * 8 with
+        // bin = byte | NaN bit << 8, formed as two shifted adds.
+        uint32_t off = (uint32_t)cb[u] * (uint32_t)(FG * 8);
+        if (HAS_HI) off |= ((uint32_t)ch[u] << hsh) & (256u * FG * 8u);
+        char* const gp = lane_g + off;
+        atomicAdd((unsigned long long*)gp, (unsigned long long)g);
+        atomicAdd((unsigned long long*)(gp + kPlaneBytes), (unsigned long long)h);
+      });
+    };
+    __syncthreads();  // the LDS zeroing is done before any wave adds
+    HistStage sc{0u, 0ll, 0ll};
+    if (nk > 0) {
       stage_load(0);
-      stage_store(0);
+      stage_quant(sc);
     }
-    if (n_chunks > 1) {
-      stage_load(1);
-      stage_store(1);
-    }
-    __syncthreads();  // staging visible; also orders the LDS zeroing
-    // Bins of even chunks live in (av, ah), of odd chunks in (bv, bh): the
-    // loop below is unrolled by two so that no loaded register is copied
-    // (a copy would wait for the next chunk's loads).
-    uint8_t av[RPT], ah[RPT], bv[RPT], bh[RPT];
-#pragma unroll
-    for (int u = 0; u < RPT; u++) av[u] = ah[u] = bv[u] = bh[u] = 0;
-    if (n_chunks > 0) load_bins(0, av, ah);
-    // One chunk: issue the next chunk's bin loads (PF) into (nb, nh), add
-    // this one from (cb, ch).  The last chunk is peeled (PF false) so that
-    // inside the loop every path issues the same loads: the compiler then
-    // waits for exactly this chunk's bins, never for the next chunk's.
-    auto chunk = [&](uint32_t c, auto pf, const uint8_t (&cb)[RPT], const uint8_t (&ch)[RPT],
-                     uint8_t (&nb)[RPT], uint8_t (&nh)[RPT]) {
-      constexpr bool PF = decltype(pf)::value;
-      const int cur = (int)(c % kHistBufs);
-      if (c + 2 < n_chunks) stage_load(c + 2);
-      if constexpr (PF) load_bins(c + 1, nb, nh);
-      // Four rows at a time: their (g, h) pairs are read from LDS together,
-      // then added without branches.  The NaN bin of a 257-bin feature is
-      // stored as byte 0 with its hi bit set, so bin | hi << 8 sends it to
-      // row 256: it is not accumulated (it is the node total minus the 256
-      // real bins).
-#pragma unroll
-      for (int u0 = 0; u0 < RPT; u0 += kHistRowBatch) {
-        longlong2 q[kHistRowBatch];
-#pragma unroll
-        for (int k = 0; k < kHistRowBatch; k++) q[k] = s_q[cur][rs * RPT + u0 + k];
-#pragma unroll
-        for (int k = 0; k < kHistRowBatch; k++) {
-          const int u = u0 + k;
-          // Byte offset of (bin, lf) in the g plane: bin * FG * 8 with
-          // bin = byte | NaN bit << 8, formed as two shifted adds.
-          uint32_t off = (uint32_t)cb[u] * (uint32_t)(FG * 8);
-          if (HAS_HI) off |= ((uint32_t)ch[u] << hsh) & (256u * FG * 8u);
-          char* const gp = lane_g + off;
-          atomicAdd((unsigned long long*)gp, (unsigned long long)q[k].x);
-          atomicAdd((unsigned long long*)(gp + kPlaneBytes), (unsigned long long)q[k].y);
-        }
-      }
-      if (c + 2 < n_chunks) stage_store((int)((c + 2) % kHistBufs));
-      __syncthreads();  // chunk c+2 staged; chunk c's buffer may be rewritten
-    };
-    uint32_t c = 0;
-    for (; c + 2 < n_chunks; c += 2) {
-      chunk(c, std::true_type{}, av, ah, bv, bh);
-      chunk(c + 1, std::true_type{}, bv, bh, av, ah);
-    }
-    if (c + 1 < n_chunks) {
-      chunk(c, std::true_type{}, av, ah, bv, bh);
-      chunk(c + 1, std::false_type{}, bv, bh, av, ah);
-    } else if (c < n_chunks) {
-      chunk(c, std::false_type{}, av, ah, bv, bh);
+    for (uint32_t k = 0; k < nk; k++) {
+      uint8_t cb[RPT], ch[RPT];
+      load_bins(sc, cb, ch);
+      if (k + 1 < nk) stage_load(k + 1);
+      add_rows(sc, cb, ch);
+      if (k + 1 < nk) stage_quant(sc);
     }
     const bool do_sum = node_sum != nullptr && it.group == 0;
     if (do_sum) {
       accg = wave_sum_ll(accg);
       acch = wave_sum_ll(acch);
-      if ((threadIdx.x & (WAVE - 1)) == 0) {
-        s_red[0][threadIdx.x / WAVE] = accg;
-        s_red[1][threadIdx.x / WAVE] = acch;
+      if (ln == 0) {
+        s_red[0][wv] = accg;
+        s_red[1][wv] = acch;
       }
     }
-    __syncthreads();
+    __syncthreads();  // every wave's adds (and node-sum parts) are done
     if (do_sum && threadIdx.x == 0) {
       long long tg = 0, th = 0;
-      for (int wv = 0; wv < kHistThreads / WAVE; wv++) {
-        tg += s_red[0][wv];
-        th += s_red[1][wv];
+      for (int w = 0; w < kHistWaves; w++) {
+        tg += s_red[0][w];
+        th += s_red[1][w];
       }
       if (direct) {
         node_sum[2 * it.node] = tg;
