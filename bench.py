@@ -597,6 +597,9 @@ def main():
                     help="rows of the CPU baseline sample for c1/c3/c4")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the plain copy kernel 5x on this many bytes (HBM counter calibration)")
+    ap.add_argument("--predict-only", action="store_true",
+                    help="only the device-batch prediction of the 500-tree depth-8 model (the run "
+                         "a kernel trace of the traversal kernel is taken on)")
     ap.add_argument("--graph-only", action="store_true",
                     help="only the warmup and timed graph-schedule runs (no per-kernel event run, "
                          "no batched or plain schedule): the run a kernel trace of the graph "
@@ -626,6 +629,16 @@ def main():
         return
     if args.workload != "c2":
         run_workload(args, L, ctx)
+        return
+    if args.predict_only:
+        info, arrays = random_forest(500, 8, 100, seed=0xB002)
+        fa = A.forest_struct(arrays)
+        mh = C.c_void_p()
+        A.check(L.bst_model_import(ctx, C.byref(info), C.byref(fa), C.byref(mh)))
+        pr = predict_roofline(ctx, mh, 100, 1, 500, 8)
+        L.bst_model_destroy(mh)
+        print(json.dumps({"metric": "device-batch prediction rows/s (500 trees, depth 8, 100 features)",
+                          "value": round(pr["rows_per_s"]), "unit": "rows/s", "roofline": pr}), flush=True)
         return
     n, F, depth = args.rows, args.features, args.depth
     log(f"generating {n} x {F} binary-logistic data with 10% NaN")
