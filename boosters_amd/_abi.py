@@ -21,8 +21,11 @@ BST_E_FORMAT = -6
 BST_E_INVALID = -7
 
 OBJ_SQUARED, OBJ_LOGISTIC, OBJ_SOFTMAX, OBJ_LAMBDARANK = 0, 1, 2, 3
+OBJ_ABSOLUTE, OBJ_HINGE, OBJ_PINBALL, OBJ_PSEUDO_HUBER, OBJ_POISSON = 4, 5, 6, 7, 8
 GROW_DEPTHWISE, GROW_LEAFWISE = 0, 1
 METRIC_DEFAULT, METRIC_NONE, METRIC_RMSE, METRIC_LOGLOSS, METRIC_MLOGLOSS = -1, 0, 1, 2, 3
+(METRIC_MAE, METRIC_MAPE, METRIC_QUANTILE, METRIC_HUBER, METRIC_POISSON_DEVIANCE, METRIC_ACCURACY,
+ METRIC_MARGIN_ACCURACY, METRIC_MULTICLASS_ACCURACY) = 4, 5, 6, 7, 8, 9, 10, 11
 TRANSFORM_IDENTITY, TRANSFORM_SIGMOID, TRANSFORM_SOFTMAX = 0, 1, 2
 SCHED_PLAIN, SCHED_BATCHED, SCHED_GRAPH = 0, 1, 2
 
@@ -61,6 +64,11 @@ class GBDTConfigC(C.Structure):
         ("cache_size", C.c_uint64),
         ("seed", C.c_uint64),
         ("verbosity", C.c_int32),
+        ("huber_delta", C.c_float),
+        ("n_alphas", C.c_uint32),
+        ("alphas", C.POINTER(C.c_float)),
+        ("accuracy_threshold", C.c_float),
+        ("metric_huber_delta", C.c_double),
     ]
 
 
@@ -283,6 +291,11 @@ def _declare(lib):
             [vp, C.c_int32, C.c_int32, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int64,
              P(C.c_float), P(C.c_float)],
         ),
+        "bst_kernel_gradients_cfg": (
+            C.c_int,
+            [vp, P(GBDTConfigC), P(C.c_float), P(C.c_float), P(C.c_float), C.c_int64,
+             P(C.c_float), P(C.c_float)],
+        ),
         "bst_kernel_lambdarank": (
             C.c_int,
             [vp, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int64, P(C.c_int32), C.c_int64,
@@ -330,7 +343,8 @@ def exported_symbols():
         "bst_model_destroy", "bst_model_save_bstr", "bst_model_load_bstr",
         "bst_model_save_bstr_file", "bst_model_load_bstr_file", "bst_model_save_json",
         "bst_model_load_json", "bst_buffer_free", "bst_predict", "bst_predict_device", "bst_predict_device_noalloc", "bst_server_create",
-        "bst_server_submit", "bst_server_wait", "bst_server_destroy", "bst_kernel_gradients", "bst_kernel_lambdarank",
+        "bst_server_submit", "bst_server_wait", "bst_server_destroy", "bst_kernel_gradients", "bst_kernel_gradients_cfg",
+        "bst_kernel_lambdarank",
         "bst_kernel_histogram", "bst_dataset_hist_layout", "bst_kernel_partition",
         "bst_kernel_find_split",
     ]

@@ -19,6 +19,7 @@ SOURCES = [
     "kernels.hip",
     "grow_device.hip",
     "sample_device.hip",
+    "renew.hip",
     "binning.cpp",
     "trainer.cpp",
     "trainer_device.cpp",

@@ -181,6 +181,20 @@ int default_metric_of(int obj) {
                                  : (obj == BST_OBJ_SOFTMAX ? BST_METRIC_MLOGLOSS : BST_METRIC_RMSE);
 }
 
+// Objective::n_outputs, objectives/mod.rs:221-233.
+int n_outputs_of(const bst_gbdt_config& c) {
+  if (c.objective == BST_OBJ_SOFTMAX) return (int)c.n_classes;
+  if (c.objective == BST_OBJ_PINBALL) return (int)c.n_alphas;
+  return 1;
+}
+
+ObjParams obj_params_of(const bst_gbdt_config& c) {
+  ObjParams op;
+  op.huber_delta = c.huber_delta;
+  for (uint32_t i = 0; i < c.n_alphas && i < 16 && c.alphas; i++) op.alphas[i] = c.alphas[i];
+  return op;
+}
+
 int validate(const bst_gbdt_config* c) {
   if (!c) raise(BST_E_INVALID, "null config");
   // GBDTConfig::validate, model/gbdt/config.rs:252-310.
@@ -197,8 +211,25 @@ int validate(const bst_gbdt_config* c) {
   if (c->alpha < 0.0f) raise(BST_E_CONFIG, "alpha must be non-negative");
   if (c->min_child_weight < 0.0f) raise(BST_E_CONFIG, "min_child_weight must be non-negative");
   if (c->min_gain < 0.0f) raise(BST_E_CONFIG, "min_gain must be non-negative");
-  if (c->objective < 0 || c->objective > BST_OBJ_LAMBDARANK)
+  if (c->objective < 0 || c->objective > BST_OBJ_POISSON)
     raise(BST_E_CONFIG, "objective not supported by this engine build");
+  // Objective payloads (the bindings' Objective.huber / Objective.pinball checks).
+  if (c->objective == BST_OBJ_PSEUDO_HUBER && !(c->huber_delta > 0.0f))
+    raise(BST_E_CONFIG, "pseudo-Huber delta must be positive");
+  const bool quantile_metric = c->metric == BST_METRIC_QUANTILE ||
+                               (c->metric == BST_METRIC_DEFAULT && c->objective == BST_OBJ_PINBALL);
+  if (c->objective == BST_OBJ_PINBALL || quantile_metric) {
+    if (c->n_alphas < 1 || !c->alphas) raise(BST_E_CONFIG, "quantiles need at least one alpha");
+    for (uint32_t i = 0; i < c->n_alphas; i++)
+      if (!(c->alphas[i] > 0.0f && c->alphas[i] < 1.0f))
+        raise(BST_E_CONFIG, "each quantile alpha must be in (0, 1)");
+  }
+  if (c->objective == BST_OBJ_PINBALL && c->n_alphas > kPredMaxGroups)
+    raise(BST_E_CONFIG, "pinball supports at most 16 quantiles");
+  if (c->metric < BST_METRIC_DEFAULT || c->metric > BST_METRIC_MULTICLASS_ACCURACY)
+    raise(BST_E_CONFIG, "metric not supported by this engine build");
+  if (c->metric == BST_METRIC_HUBER && !(c->metric_huber_delta > 0.0))
+    raise(BST_E_CONFIG, "Huber metric delta must be positive");
   if (c->objective == BST_OBJ_SOFTMAX && (c->n_classes < 2 || c->n_classes > kPredMaxGroups))
     raise(BST_E_CONFIG, "softmax needs 2..16 classes");
   if (c->growth == BST_GROW_DEPTHWISE && c->max_depth > 16) raise(BST_E_CONFIG, "max_depth > 16");
@@ -363,6 +394,11 @@ void bst_gbdt_config_default(bst_gbdt_config* c) {
   c->cache_size = 8;
   c->seed = 42;
   c->verbosity = 0;
+  c->huber_delta = 1.0f;
+  c->n_alphas = 0;
+  c->alphas = nullptr;
+  c->accuracy_threshold = 0.5f;
+  c->metric_huber_delta = 1.0;
 }
 
 void bst_ext_config_default(bst_ext_config* e) {
@@ -624,7 +660,10 @@ static int train_impl(bst_ctx* ctx, bst_dataset* ds, const float* y_dev, int64_t
   opt.cfg = *cfg;
   bst_ext_config_default(&opt.ext);
   if (ext) opt.ext = *ext;
-  opt.K = cfg->objective == BST_OBJ_SOFTMAX ? (int)cfg->n_classes : 1;
+  opt.K = n_outputs_of(*cfg);
+  opt.obj = obj_params_of(*cfg);
+  if (cfg->alphas && cfg->n_alphas > 0) opt.alphas.assign(cfg->alphas, cfg->alphas + cfg->n_alphas);
+  opt.cfg.alphas = nullptr;  // borrowed for the call only: the trainer keeps opt.alphas
   need(opt.ext.schedule >= BST_SCHED_PLAIN && opt.ext.schedule <= BST_SCHED_GRAPH, BST_E_CONFIG,
        "unknown schedule");
   need(opt.ext.hist_fg == 0 || opt.ext.hist_fg == 16 || opt.ext.hist_fg == 32, BST_E_CONFIG,
@@ -989,30 +1028,53 @@ int bst_kernel_lambdarank(bst_ctx* ctx, const float* preds, const float* y, cons
   });
 }
 
+namespace {
+// One gradient launch over host buffers (the per-kernel parity entry points).
+int gradients_impl(bst_ctx* ctx, int32_t obj, int32_t K, const ObjParams& op, const float* preds,
+                   const float* y, const float* w, int64_t n, float* g, float* h) {
+  need(ctx && preds && y && g && h, BST_E_INVALID, "null argument");
+  need(n >= 0, BST_E_INVALID, "negative row count");
+  set_device(ctx);
+  hipStream_t s = ctx->stream;
+  DevBuf<float> dp((size_t)K * n), dy(n), dw, dg((size_t)K * n), dh((size_t)K * n);
+  DevBuf<uint32_t> am(2 * K);
+  h2d(s, dp.p, preds, (size_t)K * n);
+  h2d(s, dy.p, y, n);
+  if (w) {
+    dw.alloc(n);
+    h2d(s, dw.p, w, n);
+  }
+  BST_HIP(hipMemsetAsync(am.p, 0, am.bytes(), s));
+  launch_gradients(s, obj, op, dp.p, dy.p, dw.p, n, K, dg.p, dh.p, am.p);
+  d2h(s, g, dg.p, dg.n);
+  d2h(s, h, dh.p, dh.n);
+  BST_HIP(hipStreamSynchronize(s));
+  return BST_OK;
+}
+}  // namespace
+
 int bst_kernel_gradients(bst_ctx* ctx, int32_t obj, int32_t K, const float* preds,
                          const float* y, const float* w, int64_t n, float* g, float* h) {
   return guard([&] {
-    need(ctx && preds && y && g && h, BST_E_INVALID, "null argument");
-    need(obj >= 0 && obj <= BST_OBJ_SOFTMAX, BST_E_CONFIG, "objective");
+    need(obj >= 0 && obj <= BST_OBJ_POISSON && obj != BST_OBJ_LAMBDARANK, BST_E_CONFIG,
+         "objective");
+    need(obj != BST_OBJ_PINBALL, BST_E_CONFIG,
+         "pinball needs its alphas: use bst_kernel_gradients_cfg");
     need(K >= 1 && K <= 16 && (obj == BST_OBJ_SOFTMAX || K == 1), BST_E_CONFIG,
          "outputs must be 1, or 2..16 for softmax");
-    need(n >= 0, BST_E_INVALID, "negative row count");
-    set_device(ctx);
-    hipStream_t s = ctx->stream;
-    DevBuf<float> dp((size_t)K * n), dy(n), dw, dg((size_t)K * n), dh((size_t)K * n);
-    DevBuf<uint32_t> am(2 * K);
-    h2d(s, dp.p, preds, (size_t)K * n);
-    h2d(s, dy.p, y, n);
-    if (w) {
-      dw.alloc(n);
-      h2d(s, dw.p, w, n);
-    }
-    BST_HIP(hipMemsetAsync(am.p, 0, am.bytes(), s));
-    launch_gradients(s, obj, dp.p, dy.p, dw.p, n, K, dg.p, dh.p, am.p);
-    d2h(s, g, dg.p, dg.n);
-    d2h(s, h, dh.p, dh.n);
-    BST_HIP(hipStreamSynchronize(s));
-    return BST_OK;
+    return gradients_impl(ctx, obj, K, ObjParams(), preds, y, w, n, g, h);
+  });
+}
+
+int bst_kernel_gradients_cfg(bst_ctx* ctx, const bst_gbdt_config* cfg, const float* preds,
+                             const float* y, const float* w, int64_t n, float* g, float* h) {
+  return guard([&] {
+    need(cfg != nullptr, BST_E_INVALID, "null config");
+    validate(cfg);
+    need(cfg->objective != BST_OBJ_LAMBDARANK, BST_E_CONFIG,
+         "LambdaRank gradients: use bst_kernel_lambdarank");
+    const int K = n_outputs_of(*cfg);
+    return gradients_impl(ctx, cfg->objective, K, obj_params_of(*cfg), preds, y, w, n, g, h);
   });
 }
 

@@ -5,6 +5,8 @@
 #include <vector>
 #include <stdint.h>
 
+#include "../../include/boosters_amd.h"
+
 namespace bst {
 
 constexpr int kMaxBinsDev = 256;  // u8 bins per feature; bin 256 lives in the hi plane
@@ -128,8 +130,44 @@ void launch_value_to_bin(hipStream_t s, const float* X, int64_t n, int E, const 
                          int64_t hi_words);
 void launch_to_rowmajor(hipStream_t s, const DatasetDev& d);
 
-void launch_gradients(hipStream_t s, int objective, const float* preds, const float* y,
-                      const float* w, int64_t n, int K, float* g, float* h, uint32_t* absmax);
+// One leaf of a renewal pass: its rows are positions begin .. begin + count - 1
+// of the partition's row list; output picks the prediction row and alpha the
+// quantile.
+struct RenewLeaf {
+  uint32_t begin;
+  uint32_t count;
+  uint32_t output;
+  float alpha;
+};
+// Leaf renewal (quantile objectives): out[i] = the alpha-quantile of
+// y[row] - preds[output][row] over leaf i's rows (NaN for an empty leaf).
+size_t renew_temp_bytes(int64_t n_pos, int n_leaves);
+void renew_leaves(hipStream_t s, const RenewLeaf* leaves, int n_leaves, int64_t n_pos,
+                  const uint32_t* idx, const float* y, const float* preds, const float* w,
+                  int64_t n, unsigned long long* keys, unsigned long long* keys_sorted,
+                  uint32_t* seg_begin, uint32_t* seg_end, void* temp, size_t temp_bytes,
+                  float* out);
+
+// Metric payloads: accuracy threshold, Huber delta, one quantile alpha per
+// output.
+struct MetricParams {
+  float accuracy_threshold = 0.5f;
+  double huber_delta = 1.0;
+  float alphas[16] = {};
+  int n_alphas = 0;
+};
+void launch_metric(hipStream_t s, int metric, int transform, const MetricParams& mp,
+                   const float* preds, int K, const float* y, const float* w, int64_t n,
+                   double* partial, int n_blocks);
+
+// Objective payloads: pseudo-Huber delta, one pinball alpha per output.
+struct ObjParams {
+  float huber_delta = 1.0f;
+  float alphas[16] = {};
+};
+void launch_gradients(hipStream_t s, int objective, const ObjParams& op, const float* preds,
+                      const float* y, const float* w, int64_t n, int K, float* g, float* h,
+                      uint32_t* absmax);
 // LambdaRank gradients of one output over ragged query groups
 // (group_off[n_groups + 1] row offsets, each group <= 4096 rows); disc[r] =
 // 1 / log2(2 + r) for r below the largest group; absmax as launch_gradients.

@@ -215,9 +215,56 @@ __device__ __forceinline__ uint32_t abs_bits(float v) { return __float_as_uint(f
 // block and value, so the maxima cost no global contention.
 // KM: outputs the launch handles at most (1: one output and not softmax,
 // so the per-output maxima and their reduction hold one entry).
+// One output's (g, h) for the single-output objectives other than softmax:
+// squared regression.rs:20-42, logistic classification.rs:24-52, absolute
+// regression.rs:80-104, hinge classification.rs:87-116, pinball
+// regression.rs:129-156 (alpha of this output), pseudo-Huber
+// regression.rs:185-216, Poisson regression.rs:244-270.  f32 arithmetic in
+// the reference's order.
+__device__ __forceinline__ void objective_pair(int objective, const ObjParams& op, int k, float p0,
+                                               float yi, float wi, float& gv, float& hv) {
+  const float HESS_MIN = 1e-6f;
+  if (objective == BST_OBJ_LOGISTIC) {
+    const float p = 1.0f / (1.0f + exp32(-p0));
+    gv = wi * (p - yi);
+    hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
+  } else if (objective == BST_OBJ_ABSOLUTE) {
+    const float diff = p0 - yi;
+    gv = wi * (isnan(diff) ? diff : copysignf(1.0f, diff));  // f32::signum
+    hv = wi;
+  } else if (objective == BST_OBJ_HINGE) {
+    const float yy = yi > 0.5f ? 1.0f : -1.0f;
+    const float margin = yy * p0;
+    gv = margin < 1.0f ? -wi * yy : 0.0f;
+    hv = wi;
+  } else if (objective == BST_OBJ_PINBALL) {
+    const float alpha = op.alphas[k];
+    const float diff = p0 - yi;
+    gv = wi * (diff < 0.0f ? -alpha : 1.0f - alpha);
+    hv = wi;
+  } else if (objective == BST_OBJ_PSEUDO_HUBER) {
+    const float inv_delta_sq = 1.0f / (op.huber_delta * op.huber_delta);
+    const float residual = p0 - yi;
+    const float factor = 1.0f + residual * residual * inv_delta_sq;
+    const float sqrt_factor = sqrtf(factor);
+    gv = wi * residual / sqrt_factor;
+    hv = wi / (factor * sqrt_factor);
+  } else if (objective == BST_OBJ_POISSON) {
+    float pc = p0;  // f32::clamp keeps NaN
+    pc = pc < -30.0f ? -30.0f : pc;
+    pc = pc > 30.0f ? 30.0f : pc;
+    const float e = exp32(pc);
+    gv = wi * (e - yi);
+    hv = fmaxf(wi * e, HESS_MIN);
+  } else {  // squared
+    gv = wi * (p0 - yi);
+    hv = wi;
+  }
+}
+
 template <int KM>
 __global__ __launch_bounds__(kGradThreads) void k_gradients(
-    int objective, const float* __restrict__ preds, const float* __restrict__ y,
+    int objective, ObjParams op, const float* __restrict__ preds, const float* __restrict__ y,
     const float* __restrict__ w, int64_t n, int K, float* __restrict__ g, float* __restrict__ h,
     uint32_t* __restrict__ absmax) {
   const float HESS_MIN = 1e-6f;
@@ -247,14 +294,7 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
         const int64_t i = i_start + k * stride;
         if (i < n) {
           float gv, hv;
-          if (objective == 1) {  // logistic, classification.rs:24-52
-            const float p = 1.0f / (1.0f + exp32(-p0[k]));
-            gv = wv_[k] * (p - yv[k]);
-            hv = fmaxf(wv_[k] * p * (1.0f - p), HESS_MIN);
-          } else {  // squared, regression.rs:20-42
-            gv = wv_[k] * (p0[k] - yv[k]);
-            hv = wv_[k];
-          }
+          objective_pair(objective, op, 0, p0[k], yv[k], wv_[k], gv, hv);
           g[i] = gv;
           h[i] = hv;
           mx_g[0] = max(mx_g[0], abs_bits(gv));
@@ -295,14 +335,7 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
         if (k < K) {
           const float p0 = preds[(int64_t)k * n + i];
           float gv, hv;
-          if (objective == 1) {  // logistic, classification.rs:24-52
-            const float p = 1.0f / (1.0f + exp32(-p0));
-            gv = wi * (p - yi);
-            hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
-          } else {  // squared, regression.rs:20-42
-            gv = wi * (p0 - yi);
-            hv = wi;
-          }
+          objective_pair(objective, op, k, p0, yi, wi, gv, hv);
           g[(int64_t)k * n + i] = gv;
           h[(int64_t)k * n + i] = hv;
           mx_g[k] = max(mx_g[k], abs_bits(gv));
@@ -335,8 +368,9 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
   }
 }
 
-void launch_gradients(hipStream_t s, int objective, const float* preds, const float* y,
-                      const float* w, int64_t n, int K, float* g, float* h, uint32_t* absmax) {
+void launch_gradients(hipStream_t s, int objective, const ObjParams& op, const float* preds,
+                      const float* y, const float* w, int64_t n, int K, float* g, float* h,
+                      uint32_t* absmax) {
   // Callers validate 1 <= K <= kGradMaxK (softmax allows at most 16 classes).
   if (n <= 0 || K < 1 || K > kGradMaxK) return;
   // At most 256 blocks: each ends with one atomicMax per value on the same
@@ -345,10 +379,10 @@ void launch_gradients(hipStream_t s, int objective, const float* preds, const fl
   if (blocks > 256) blocks = 256;
   if (K == 1 && objective != 2)
     hipLaunchKernelGGL(k_gradients<1>, dim3((unsigned)blocks), dim3(kGradThreads), 0, s,
-                       objective, preds, y, w, n, K, g, h, absmax);
+                       objective, op, preds, y, w, n, K, g, h, absmax);
   else
     hipLaunchKernelGGL(k_gradients<kGradMaxK>, dim3((unsigned)blocks), dim3(kGradThreads), 0, s,
-                       objective, preds, y, w, n, K, g, h, absmax);
+                       objective, op, preds, y, w, n, K, g, h, absmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -2505,11 +2539,100 @@ __device__ __forceinline__ double logloss_term(double q, double l) {
   return hard ? lx : l * lx + (1.0 - l) * log(1.0 - q);
 }
 
-// Metric partial sums per block: (sum w*loss, sum w), f64.  RMSE gets one row
-// of partials per output (metrics/regression.rs:17-40); logloss and
-// multiclass logloss clamp the transformed probability to [1e-15, 1-1e-15]
-// (metrics/classification.rs:13-40, 332-360).
-__global__ __launch_bounds__(256) void k_metric(int metric, int transform,
+// Transformed prediction of output k for row i (OutputTransform,
+// transform.rs:23-115): sigmoid, max-subtracted softmax over the K outputs
+// (f32, summed in output order), or the raw score.
+__device__ __forceinline__ float metric_pred(int transform, const float* __restrict__ preds, int K,
+                                             int64_t n, int64_t i, int k) {
+  if (transform == BST_TRANSFORM_SIGMOID) return sigmoid32(preds[(int64_t)k * n + i]);
+  if (transform == BST_TRANSFORM_SOFTMAX) {
+    float mx = -INFINITY;
+    for (int c = 0; c < K; c++) mx = fmaxf(mx, preds[(int64_t)c * n + i]);
+    float ssum = 0.0f, ek = 0.0f;
+    for (int c = 0; c < K; c++) {
+      const float e = exp32(preds[(int64_t)c * n + i] - mx);
+      ssum += e;
+      if (c == k) ek = e;
+    }
+    return ssum > 0.0f ? ek / ssum : ek;
+  }
+  return preds[(int64_t)k * n + i];
+}
+
+// One row's (weighted loss, weight) contribution for metrics other than the
+// batched RMSE / logloss path (metrics/regression.rs:17-250,
+// classification.rs:13-360); k is the output for the per-output metrics.
+__device__ double metric_term(int metric, int transform, const MetricParams& mp,
+                              const float* __restrict__ preds, int K, int64_t n, int64_t i, int k,
+                              float yi) {
+  const double EPS = 1e-15;
+  if (metric == BST_METRIC_RMSE) {
+    const double d = (double)metric_pred(transform, preds, K, n, i, k) - (double)yi;
+    return d * d;
+  }
+  if (metric == BST_METRIC_LOGLOSS) {
+    const float p = metric_pred(transform, preds, K, n, i, 0);
+    const double q = fmin(fmax((double)p, EPS), 1.0 - EPS);
+    return -logloss_term(q, (double)yi);
+  }
+  if (metric == BST_METRIC_MLOGLOSS) {
+    const int lab = (int)rint((double)yi);
+    const float p = metric_pred(transform, preds, K, n, i, lab);
+    const double q = fmin(fmax((double)p, EPS), 1.0 - EPS);
+    return -log(q);
+  }
+  if (metric == BST_METRIC_MAE)
+    return fabs((double)metric_pred(transform, preds, K, n, i, k) - (double)yi);
+  if (metric == BST_METRIC_MAPE) {
+    const double pp = metric_pred(transform, preds, K, n, i, 0), l = yi;
+    return fabs(pp - l) / fmax(fabs(l), EPS);
+  }
+  if (metric == BST_METRIC_QUANTILE) {
+    const double a = (double)mp.alphas[k];
+    const double r = (double)yi - (double)metric_pred(transform, preds, K, n, i, k);
+    return r >= 0.0 ? a * r : (1.0 - a) * (-r);
+  }
+  if (metric == BST_METRIC_HUBER) {
+    const double r = fabs((double)metric_pred(transform, preds, K, n, i, 0) - (double)yi);
+    const double d = mp.huber_delta;
+    return r <= d ? 0.5 * r * r : d * (r - 0.5 * d);
+  }
+  if (metric == BST_METRIC_POISSON_DEVIANCE) {
+    const double E9 = 1e-9;
+    const double yy = yi, mu = fmax((double)metric_pred(transform, preds, K, n, i, 0), E9);
+    return yy > E9 ? 2.0 * (yy * log(yy / mu) - (yy - mu)) : 2.0 * mu;
+  }
+  if (metric == BST_METRIC_ACCURACY || metric == BST_METRIC_MARGIN_ACCURACY) {
+    const float t = metric == BST_METRIC_ACCURACY ? mp.accuracy_threshold : 0.0f;
+    const float pc = metric_pred(transform, preds, K, n, i, 0) >= t ? 1.0f : 0.0f;
+    return fabsf(pc - yi) < 0.5f ? 1.0 : 0.0;
+  }
+  // multiclass accuracy: the rounded score (one output) or the arg-max, the
+  // last of equal maxima (Iterator::max_by; NaN compares Less)
+  float pc;
+  if (K == 1) {
+    pc = roundf(metric_pred(transform, preds, K, n, i, 0));
+  } else {
+    int best = 0;
+    float vb = metric_pred(transform, preds, K, n, i, 0);
+    for (int c = 1; c < K; c++) {
+      const float vc = metric_pred(transform, preds, K, n, i, c);
+      if (isnan(vb) || isnan(vc) || !(vb > vc)) {
+        best = c;
+        vb = vc;
+      }
+    }
+    pc = (float)best;
+  }
+  return fabsf(pc - yi) < 0.5f ? 1.0 : 0.0;
+}
+
+// Metric partial sums per block: (sum w*loss, sum w), f64.  RMSE, MAE and
+// the quantile metric get one row of partials per output; the others read
+// output 0 (or every output, for the multiclass ones) and use one row.
+// Logloss and multiclass logloss clamp the transformed probability to
+// [1e-15, 1 - 1e-15] (metrics/classification.rs:13-40, 332-360).
+__global__ __launch_bounds__(256) void k_metric(int metric, int transform, MetricParams mp,
                                                 const float* __restrict__ preds, int K,
                                                 const float* __restrict__ y,
                                                 const float* __restrict__ w, int64_t n,
@@ -2520,7 +2643,9 @@ __global__ __launch_bounds__(256) void k_metric(int metric, int transform,
   double a = 0.0, b = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t i_start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (metric == 1 || metric == 2) {
+  const bool fast = (metric == BST_METRIC_RMSE && transform == BST_TRANSFORM_IDENTITY) ||
+                    (metric == BST_METRIC_LOGLOSS && transform != BST_TRANSFORM_SOFTMAX);
+  if (fast) {
     // kRowBatch rows per thread per pass, loads issued together.
     for (; i_start < n; i_start += kRowBatch * stride) {
       float pv[kRowBatch], yv[kRowBatch], wv_[kRowBatch];
@@ -2528,18 +2653,18 @@ __global__ __launch_bounds__(256) void k_metric(int metric, int transform,
       for (int q = 0; q < kRowBatch; q++) {
         const int64_t i = i_start + q * stride;
         const bool in = i < n;
-        pv[q] = in ? preds[(metric == 1 ? (int64_t)k * n : 0) + i] : 0.0f;
+        pv[q] = in ? preds[(metric == BST_METRIC_RMSE ? (int64_t)k * n : 0) + i] : 0.0f;
         yv[q] = in ? y[i] : 0.0f;
         wv_[q] = in ? (w ? w[i] : 1.0f) : 0.0f;
       }
 #pragma unroll
       for (int q = 0; q < kRowBatch; q++) {
         if (i_start + q * stride >= n) continue;
-        if (metric == 1) {
+        if (metric == BST_METRIC_RMSE) {
           const double d = (double)pv[q] - (double)yv[q];
           a += (double)wv_[q] * d * d;
         } else {
-          const float p = transform == 1 ? sigmoid32(pv[q]) : pv[q];
+          const float p = transform == BST_TRANSFORM_SIGMOID ? sigmoid32(pv[q]) : pv[q];
           const double qq = fmin(fmax((double)p, EPS), 1.0 - EPS);
           a += (double)wv_[q] * -logloss_term(qq, (double)yv[q]);
         }
@@ -2549,24 +2674,7 @@ __global__ __launch_bounds__(256) void k_metric(int metric, int transform,
   }
   for (int64_t i = i_start; i < n; i += stride) {
     const float wi = w ? w[i] : 1.0f;
-    if (metric == 1) {
-      const double d = (double)preds[(int64_t)k * n + i] - (double)y[i];
-      a += (double)wi * d * d;
-    } else if (metric == 2) {
-      const float p = transform == 1 ? sigmoid32(preds[i]) : preds[i];
-      const double q = fmin(fmax((double)p, EPS), 1.0 - EPS);
-      a += (double)wi * -logloss_term(q, (double)y[i]);
-    } else {
-      float mx = -INFINITY;
-      for (int c = 0; c < K; c++) mx = fmaxf(mx, preds[(int64_t)c * n + i]);
-      float ssum = 0.0f;
-      for (int c = 0; c < K; c++) ssum += exp32(preds[(int64_t)c * n + i] - mx);
-      const int lab = (int)rint((double)y[i]);
-      float p = exp32(preds[(int64_t)lab * n + i] - mx);
-      if (ssum > 0.0f) p /= ssum;
-      const double q = fmin(fmax((double)p, EPS), 1.0 - EPS);
-      a += (double)wi * (-log(q));
-    }
+    a += (double)wi * metric_term(metric, transform, mp, preds, K, n, i, k, y[i]);
     b += (double)wi;
   }
 #pragma unroll
@@ -2590,11 +2698,14 @@ __global__ __launch_bounds__(256) void k_metric(int metric, int transform,
   }
 }
 
-void launch_metric(hipStream_t s, int metric, int transform, const float* preds, int K,
-                   const float* y, const float* w, int64_t n, double* partial, int n_blocks) {
-  const int ky = metric == 1 ? K : 1;
-  hipLaunchKernelGGL(k_metric, dim3(n_blocks, ky), dim3(256), 0, s, metric, transform, preds, K,
-                     y, w, n, partial);
+void launch_metric(hipStream_t s, int metric, int transform, const MetricParams& mp,
+                   const float* preds, int K, const float* y, const float* w, int64_t n,
+                   double* partial, int n_blocks) {
+  const bool per_output = metric == BST_METRIC_RMSE || metric == BST_METRIC_MAE ||
+                          metric == BST_METRIC_QUANTILE;
+  const int ky = per_output ? K : 1;
+  hipLaunchKernelGGL(k_metric, dim3(n_blocks, ky), dim3(256), 0, s, metric, transform, mp, preds,
+                     K, y, w, n, partial);
 }
 
 }  // namespace bst

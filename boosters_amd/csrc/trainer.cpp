@@ -66,17 +66,46 @@ void upload(hipStream_t s, DevBuf<T>& d, const std::vector<T>& h) {
   h2d(s, d.p, h.data(), h.size());
 }
 
+// default_metric_for_objective, metrics/mod.rs:213-232.
 int default_metric(int obj) {
   if (obj == BST_OBJ_LAMBDARANK) return BST_METRIC_NONE;  // no ranking metric in the reference
+  if (obj == BST_OBJ_PINBALL) return BST_METRIC_QUANTILE;
+  if (obj == BST_OBJ_HINGE) return BST_METRIC_MARGIN_ACCURACY;
   return obj == BST_OBJ_LOGISTIC ? BST_METRIC_LOGLOSS
                                  : (obj == BST_OBJ_SOFTMAX ? BST_METRIC_MLOGLOSS : BST_METRIC_RMSE);
 }
 
+// Metric::higher_is_better, metrics/mod.rs:286-301.
+bool higher_is_better(int metric) {
+  return metric == BST_METRIC_ACCURACY || metric == BST_METRIC_MARGIN_ACCURACY ||
+         metric == BST_METRIC_MULTICLASS_ACCURACY;
+}
+
+// compute_weighted_quantile (objectives/regression.rs:307-333): pairs sorted
+// by value (a stable sort), total and running weights summed in f32 in
+// sorted order; the first value whose running weight reaches alpha * total.
+float weighted_quantile(const float* v, const float* w, int64_t n, float alpha) {
+  if (n == 0) return 0.0f;
+  std::vector<std::pair<float, float>> sorted((size_t)n);
+  for (int64_t i = 0; i < n; i++) sorted[(size_t)i] = {v[i], w ? w[i] : 1.0f};
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [](const std::pair<float, float>& a, const std::pair<float, float>& b) {
+                     return a.first < b.first;
+                   });
+  float total = 0.0f;
+  for (auto& p : sorted) total += p.second;
+  const float target = alpha * total;
+  float cum = 0.0f;
+  for (auto& p : sorted) {
+    cum += p.second;
+    if (cum >= target) return p.first;
+  }
+  return sorted.back().first;
+}
+
 }  // namespace
 
-// Device metric partial sums (metrics/{classification,regression}.rs), launched from here.
-void launch_metric(hipStream_t s, int metric, int transform, const float* preds, int K,
-                   const float* y, const float* w, int64_t n, double* partial, int n_blocks);
+
 
 struct Trainer::TreeState {
   int t = 0;
@@ -86,6 +115,7 @@ struct Trainer::TreeState {
   uint32_t n_part = 1;
   std::vector<uint32_t> free_slots;
   std::vector<float> leaf_lr;
+  std::vector<std::pair<uint32_t, uint32_t>> leaf_map;  // (tree node, partition id) per leaf
   std::vector<Cand> level, next;
   std::priority_queue<Cand, std::vector<Cand>, CandLess> heap;
   uint32_t n_leaves = 1;
@@ -157,7 +187,8 @@ void Trainer::launch_objective(const float* y, const float* w) {
     launch_lambdarank(s_, preds_.p, y, w, group_off_.p, (int64_t)opt_.group_off.size() - 1,
                       rank_disc_.p, g_.p, h_.p, absmax_.p);
   else
-    launch_gradients(s_, cfg.objective, preds_.p, y, w, n_, opt_.K, g_.p, h_.p, absmax_.p);
+    launch_gradients(s_, cfg.objective, opt_.obj, preds_.p, y, w, n_, opt_.K, g_.p, h_.p,
+                     absmax_.p);
 }
 
 uint32_t Trainer::alloc_slot(TreeState& ts) {
@@ -183,6 +214,7 @@ void Trainer::make_leaf(TreeState& ts, const Cand& c) {
   ts.tb.leaf[c.tree_node] = w;
   ts.tb.cover[c.tree_node] = (float)H;
   ts.leaf_lr[c.node] = w * lr_;
+  ts.leaf_map.push_back({c.tree_node, c.node});
   if (ts.pslot[c.node] != UINT32_MAX) {
     ts.free_slots.push_back(ts.pslot[c.node]);
     ts.pslot[c.node] = UINT32_MAX;
@@ -502,6 +534,50 @@ void Trainer::grow_round(std::vector<TreeState>& trees) {
       if (ts.tb.is_leaf[i]) ts.tb.leaf[i] = ts.tb.leaf[i] * lr_;
 }
 
+// Leaf renewal of one round's trees (trainer.rs:265-291): every leaf with
+// rows gets lr x the alpha-quantile of its rows' residuals, in the tree and
+// in the value the prediction update adds.
+void Trainer::renew_round(std::vector<TreeState>& trees, const float* y, const float* w) {
+  std::vector<RenewLeaf> leaves;
+  std::vector<std::pair<int, size_t>> where;  // (tree, leaf_map index) per entry
+  for (size_t ti = 0; ti < trees.size(); ti++) {
+    const TreeState& ts = trees[ti];
+    const float alpha =
+        opt_.cfg.objective == BST_OBJ_PINBALL ? opt_.alphas[(size_t)ts.output] : 0.5f;
+    for (size_t li = 0; li < ts.leaf_map.size(); li++) {
+      const uint32_t p = ts.leaf_map[li].second;
+      if (ts.pcount[p] == 0) continue;
+      leaves.push_back({(uint32_t)(ts.t * n_) + ts.pbegin[p], ts.pcount[p],
+                        (uint32_t)ts.output, alpha});
+      where.push_back({(int)ti, li});
+    }
+  }
+  if (leaves.empty()) return;
+  const int nl = (int)leaves.size();
+  const int64_t n_pos = (int64_t)T_ * n_;
+  upload(s_, d_renew_leaves_, leaves);
+  renew_keys_.ensure((size_t)n_pos);
+  renew_sorted_.ensure((size_t)n_pos);
+  renew_seg_.ensure((size_t)2 * nl);
+  renew_out_.ensure((size_t)nl);
+  const size_t tb = renew_temp_bytes(n_pos, nl);
+  renew_temp_.ensure(std::max<size_t>(tb, 1));
+  renew_leaves(s_, d_renew_leaves_.p, nl, n_pos, idx_.p, y, preds_.p, w, n_, renew_keys_.p,
+               renew_sorted_.p, renew_seg_.p, renew_seg_.p + nl, renew_temp_.p, tb,
+               renew_out_.p);
+  std::vector<float> vals((size_t)nl);
+  d2h(s_, vals.data(), renew_out_.p, (size_t)nl);
+  BST_HIP(hipStreamSynchronize(s_));
+  for (int i = 0; i < nl; i++) {
+    if (std::isnan(vals[(size_t)i])) continue;
+    TreeState& ts = trees[(size_t)where[(size_t)i].first];
+    const auto& lm = ts.leaf_map[where[(size_t)i].second];
+    const float sv = vals[(size_t)i] * lr_;
+    ts.tb.leaf[lm.first] = sv;
+    ts.leaf_lr[lm.second] = sv;
+  }
+}
+
 int Trainer::metric_kind() const {
   const auto& cfg = opt_.cfg;
   return cfg.metric == BST_METRIC_DEFAULT ? default_metric(cfg.objective) : cfg.metric;
@@ -510,18 +586,48 @@ int Trainer::metric_kind() const {
 void Trainer::metric_launch(const float* preds, const float* y, const float* w, int64_t n,
                             double* host_part) {
   const auto& cfg = opt_.cfg;
-  const int transform = cfg.objective == BST_OBJ_LOGISTIC ? BST_TRANSFORM_SIGMOID
-                        : cfg.objective == BST_OBJ_SOFTMAX ? BST_TRANSFORM_SOFTMAX
-                                                           : BST_TRANSFORM_IDENTITY;
+  // PredictionKind::Margin metrics read untransformed scores (eval.rs:150-184).
+  const int mk = metric_kind();
+  const int transform = mk == BST_METRIC_MARGIN_ACCURACY ? BST_TRANSFORM_IDENTITY
+                        : cfg.objective == BST_OBJ_LOGISTIC ? BST_TRANSFORM_SIGMOID
+                        : cfg.objective == BST_OBJ_SOFTMAX  ? BST_TRANSFORM_SOFTMAX
+                                                            : BST_TRANSFORM_IDENTITY;
   const int nb = kMetricBlocks;
   d_metric_.ensure((size_t)2 * nb * opt_.K);
-  launch_metric(s_, metric_kind(), transform, preds, opt_.K, y, w, n, d_metric_.p, nb);
+  MetricParams mp;
+  mp.accuracy_threshold = cfg.accuracy_threshold;
+  mp.huber_delta = cfg.metric_huber_delta;
+  for (size_t i = 0; i < opt_.alphas.size() && i < 16; i++) mp.alphas[i] = opt_.alphas[i];
+  mp.n_alphas = (int)std::min<size_t>(opt_.alphas.size(), 16);
+  launch_metric(s_, mk, transform, mp, preds, opt_.K, y, w, n, d_metric_.p, nb);
   d2h(s_, host_part, d_metric_.p, (size_t)2 * nb * opt_.K);
 }
 
 double Trainer::metric_finish(const double* part) const {
   const int nb = kMetricBlocks;
-  if (metric_kind() == BST_METRIC_RMSE) {
+  const int mk = metric_kind();
+  if (mk == BST_METRIC_MAE) {  // mean over outputs of each output's weighted mean
+    double acc = 0;
+    for (int k = 0; k < opt_.K; k++) {
+      double sa = 0, sw = 0;
+      for (int b = 0; b < nb; b++) {
+        sa += part[(size_t)(k * nb + b) * 2];
+        sw += part[(size_t)(k * nb + b) * 2 + 1];
+      }
+      acc += sw > 0.0 ? sa / sw : 0.0;
+    }
+    return acc / opt_.K;
+  }
+  if (mk == BST_METRIC_QUANTILE) {  // pooled over the outputs' sums
+    double sl = 0, sw = 0;
+    for (int k = 0; k < opt_.K; k++)
+      for (int b = 0; b < nb; b++) {
+        sl += part[(size_t)(k * nb + b) * 2];
+        sw += part[(size_t)(k * nb + b) * 2 + 1];
+      }
+    return sw > 0.0 ? sl / sw : 0.0;
+  }
+  if (mk == BST_METRIC_RMSE) {
     double acc = 0;
     for (int k = 0; k < opt_.K; k++) {
       double ss = 0, sw = 0;
@@ -538,6 +644,7 @@ double Trainer::metric_finish(const double* part) const {
     sl += part[(size_t)b * 2];
     sw += part[(size_t)b * 2 + 1];
   }
+  if (mk == BST_METRIC_MAPE) return sw > 0 ? (sl / sw) * 100.0 : 0.0;
   return sw > 0 ? sl / sw : 0.0;
 }
 
@@ -581,8 +688,23 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
                        .count());
     auto wt = [&](int64_t i) { return w ? wh[i] : 1.0f; };
     m->base_score.assign(K, 0.0f);
-    if (cfg.objective == BST_OBJ_LAMBDARANK) {
-      // Scores start at 0 (ranking has no prior).
+    if (cfg.objective == BST_OBJ_LAMBDARANK || cfg.objective == BST_OBJ_HINGE) {
+      // Scores start at 0 (ranking has no prior; hinge, classification.rs:118-123).
+    } else if (cfg.objective == BST_OBJ_ABSOLUTE || cfg.objective == BST_OBJ_PSEUDO_HUBER) {
+      // Weighted median, regression.rs:106-117,218-230.
+      m->base_score[0] = weighted_quantile(yh, wh, n_, 0.5f);
+    } else if (cfg.objective == BST_OBJ_PINBALL) {
+      // One weighted quantile per alpha, regression.rs:158-172.
+      for (int k = 0; k < K; k++) m->base_score[k] = weighted_quantile(yh, wh, n_, opt_.alphas[k]);
+    } else if (cfg.objective == BST_OBJ_POISSON) {
+      // log of the weighted mean target, regression.rs:272-296.
+      double sw = 0, swy = 0;
+      for (int64_t i = 0; i < n_; i++) {
+        sw += (double)wt(i);
+        swy += (double)wt(i) * (double)yh[i];
+      }
+      const double mean = sw > 0.0 ? std::max(swy / sw, 1e-7) : 1.0;
+      m->base_score[0] = (float)std::log(mean);
     } else if (cfg.objective == BST_OBJ_SQUARED) {
       double sw = 0, swy = 0;
       for (int64_t i = 0; i < n_; i++) {
@@ -764,6 +886,7 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
         ts.pcount.assign(MN_, 0);
         ts.pslot.assign(MN_, UINT32_MAX);
         ts.leaf_lr.assign(MN_, std::numeric_limits<float>::quiet_NaN());
+        ts.leaf_map.clear();
         ts.free_slots.clear();
         for (uint32_t s = SPT_; s-- > 0;) ts.free_slots.push_back(s);
         // Column sampling per tree (column.rs:266-283): the k columns with
@@ -796,6 +919,7 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
       }
       sp_.rightmost = sp_.rightmost;
       grow_round(trees);
+      if (needs_leaf_renew(cfg.objective)) renew_round(trees, y, w);
       // Prediction update and model append, trees in output order.
       h2d(s_, tree_output_.p, outs.data(), outs.size());
       for (auto& ts : trees) {
@@ -838,7 +962,8 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
       value = eval_metric(preds_.p, y, w, n_);
     metric_history.push_back(value);
     if (cfg.early_stopping_rounds > 0) {
-      const bool imp = !have_best || value < best_val;
+      const bool hib = higher_is_better(metric_kind());
+      const bool imp = !have_best || (hib ? value > best_val : value < best_val);
       if (imp) {
         best_val = value;
         have_best = true;

@@ -45,10 +45,17 @@ struct CandLess {  // BinaryHeap order: gain, then partition id
   }
 };
 
+// Objective::needs_leaf_renew, objectives/mod.rs:341-355.
+inline bool needs_leaf_renew(int objective) {
+  return objective == BST_OBJ_ABSOLUTE || objective == BST_OBJ_PINBALL;
+}
+
 struct TrainOptions {
   bst_gbdt_config cfg;
   bst_ext_config ext;
   int K = 1;  // outputs
+  ObjParams obj;              // objective payloads (pseudo-Huber delta, pinball alphas)
+  std::vector<float> alphas;  // the config's alphas, owned (pinball outputs, quantile metric)
   // LambdaRank: query group row offsets [n_groups + 1] and rank discounts
   // 1 / log2(2 + r) for r below the largest group.
   std::vector<int64_t> group_off;
@@ -76,6 +83,7 @@ class Trainer {
   void expand_batch(std::vector<TreeState>& trees, std::vector<std::pair<int, Cand>>& chosen);
   void root(std::vector<TreeState>& trees);
   void make_leaf(TreeState& ts, const Cand& c);
+  void renew_round(std::vector<TreeState>& trees, const float* y, const float* w);
   void finish_trees(std::vector<TreeState>& trees, bst_model* m);
   // Training metric: weights w (nullable); validation metric: unweighted.
   double eval_metric(const float* preds, const float* y, const float* w, int64_t n);
@@ -156,6 +164,13 @@ class Trainer {
   DevBuf<Segment> d_segs_;
   DevBuf<uint32_t> d_sampled_;
   DevBuf<double> d_metric_;
+  // Leaf renewal (quantile objectives): leaves, keys, sorted keys, segment
+  // offsets, sort scratch, renewed values.
+  DevBuf<RenewLeaf> d_renew_leaves_;
+  DevBuf<unsigned long long> renew_keys_, renew_sorted_;
+  DevBuf<uint32_t> renew_seg_;
+  DevBuf<uint8_t> renew_temp_;
+  DevBuf<float> renew_out_;
   std::vector<uint32_t> feats_h_;
 };
 

@@ -106,7 +106,10 @@ void Trainer::DevStateDelete::operator()(DevState* p) const { delete p; }
 bool Trainer::graph_capable() const {
   const auto& c = opt_.cfg;
   const bool depth_ok = c.growth != BST_GROW_DEPTHWISE || c.max_depth <= 12;
-  return sched_ == BST_SCHED_GRAPH && depth_ok;
+  // Leaf renewal (quantile objectives) reads each leaf's rows in the stable
+  // partition order of the host-driven schedule, which the graph's fused
+  // partition does not keep.
+  return sched_ == BST_SCHED_GRAPH && depth_ok && !needs_leaf_renew(c.objective);
 }
 
 bool Trainer::device_schedule_ok() const { return graph_capable() && T_ == opt_.K; }

@@ -13,7 +13,7 @@ import ctypes as C
 import os
 import threading
 from dataclasses import dataclass, field
-from typing import Optional, Sequence
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -39,16 +39,47 @@ class Schedule:
 
 @dataclass(frozen=True)
 class Objective:
+    """Objective enum of the bindings (_boosters_rs.pyi Objective) with its
+    payload: n_classes (softmax), delta (pseudo-Huber), alpha (pinball
+    quantiles, one output each)."""
+
     kind: int
     n_classes: int = 0
+    delta: float = 1.0
+    alpha: Tuple[float, ...] = ()
 
     @staticmethod
     def squared() -> "Objective":
         return Objective(A.OBJ_SQUARED)
 
     @staticmethod
+    def absolute() -> "Objective":
+        return Objective(A.OBJ_ABSOLUTE)
+
+    @staticmethod
+    def poisson() -> "Objective":
+        return Objective(A.OBJ_POISSON)
+
+    @staticmethod
     def logistic() -> "Objective":
         return Objective(A.OBJ_LOGISTIC)
+
+    @staticmethod
+    def hinge() -> "Objective":
+        return Objective(A.OBJ_HINGE)
+
+    @staticmethod
+    def huber(delta: float = 1.0) -> "Objective":
+        if not delta > 0.0:
+            raise ValueError("huber delta must be positive")
+        return Objective(A.OBJ_PSEUDO_HUBER, delta=float(delta))
+
+    @staticmethod
+    def pinball(alpha: Sequence[float]) -> "Objective":
+        a = tuple(float(v) for v in alpha)
+        if not a or not all(0.0 < v < 1.0 for v in a):
+            raise ValueError("pinball needs at least one alpha, each in (0, 1)")
+        return Objective(A.OBJ_PINBALL, alpha=a)
 
     @staticmethod
     def softmax(n_classes: int) -> "Objective":
@@ -56,14 +87,30 @@ class Objective:
             raise ValueError("softmax needs n_classes >= 2")
         return Objective(A.OBJ_SOFTMAX, int(n_classes))
 
+    @property
+    def n_outputs(self) -> int:
+        if self.kind == A.OBJ_SOFTMAX:
+            return self.n_classes
+        if self.kind == A.OBJ_PINBALL:
+            return len(self.alpha)
+        return 1
+
 
 Objective.Squared = Objective(A.OBJ_SQUARED)  # type: ignore[attr-defined]
 Objective.Logistic = Objective(A.OBJ_LOGISTIC)  # type: ignore[attr-defined]
+Objective.Absolute = Objective(A.OBJ_ABSOLUTE)  # type: ignore[attr-defined]
+Objective.Poisson = Objective(A.OBJ_POISSON)  # type: ignore[attr-defined]
+Objective.Hinge = Objective(A.OBJ_HINGE)  # type: ignore[attr-defined]
 
 
 @dataclass(frozen=True)
 class Metric:
+    """Metric enum of the bindings (_boosters_rs.pyi Metric) with its payload:
+    threshold (accuracy), alpha (quantile)."""
+
     kind: int
+    threshold: float = 0.5
+    alpha: Tuple[float, ...] = ()
 
     @staticmethod
     def none() -> "Metric":
@@ -74,12 +121,47 @@ class Metric:
         return Metric(A.METRIC_RMSE)
 
     @staticmethod
+    def mae() -> "Metric":
+        return Metric(A.METRIC_MAE)
+
+    @staticmethod
+    def mape() -> "Metric":
+        return Metric(A.METRIC_MAPE)
+
+    @staticmethod
     def logloss() -> "Metric":
         return Metric(A.METRIC_LOGLOSS)
 
     @staticmethod
+    def accuracy_at(threshold: float = 0.5) -> "Metric":
+        if not 0.0 < threshold <= 1.0:
+            raise ValueError("accuracy threshold must be in (0, 1]")
+        return Metric(A.METRIC_ACCURACY, threshold=float(threshold))
+
+    @staticmethod
+    def margin_accuracy() -> "Metric":
+        return Metric(A.METRIC_MARGIN_ACCURACY)
+
+    @staticmethod
     def multi_logloss() -> "Metric":
         return Metric(A.METRIC_MLOGLOSS)
+
+    multiclass_logloss = multi_logloss
+
+    @staticmethod
+    def multiclass_accuracy() -> "Metric":
+        return Metric(A.METRIC_MULTICLASS_ACCURACY)
+
+    @staticmethod
+    def quantile(alpha: Sequence[float]) -> "Metric":
+        a = tuple(float(v) for v in alpha)
+        if not a or not all(0.0 < v < 1.0 for v in a):
+            raise ValueError("quantile metric needs at least one alpha, each in (0, 1)")
+        return Metric(A.METRIC_QUANTILE, alpha=a)
+
+    @staticmethod
+    def poisson_deviance() -> "Metric":
+        return Metric(A.METRIC_POISSON_DEVIANCE)
 
 
 # ---------------------------------------------------------------------------
@@ -150,6 +232,17 @@ class GBDTConfig:
         c.cache_size = int(self.cache_size)
         c.seed = int(self.seed)
         c.verbosity = int(self.verbosity)
+        c.huber_delta = float(obj.delta)
+        c.accuracy_threshold = float(self.metric.threshold) if self.metric is not None else 0.5
+        c.metric_huber_delta = 1.0
+        alphas = obj.alpha or (self.metric.alpha if self.metric is not None else ())
+        if alphas:
+            arr = np.ascontiguousarray(alphas, np.float32)
+            c._alphas_keep = arr  # the struct borrows the buffer
+            c.alphas = A.ptr(arr)
+            c.n_alphas = len(arr)
+        else:
+            c.n_alphas = 0
         return c
 
     def ext_c(self, n_threads: int = 0) -> A.ExtConfig:

@@ -45,8 +45,7 @@
 //! `BinnedDataset::from_dataset`, bit-exact), so the `BinnedDataset` the CPU
 //! path builds beforehand is not needed on this path.
 //!
-//! Unsupported configurations (objectives other than squared / logistic /
-//! softmax, metrics other than RMSE / log-loss / multi-log-loss, linear
+//! Unsupported configurations (custom objectives and metrics, AUC, linear
 //! leaves) return `Error::Unsupported` rather than silently running on the
 //! CPU: the feature flag chose the engine.
 
@@ -174,22 +173,56 @@ fn categorical_mask(ds: &Dataset) -> Vec<u8> {
 }
 
 /// `GBDTConfig` -> `bst_gbdt_config`, field for field (`config.rs:111-226`).
-fn engine_config(cfg: &GBDTConfig) -> Result<sys::bst_gbdt_config, Error> {
+/// The quantile alphas are borrowed from `cfg` for the call (`alphas` must
+/// outlive the returned struct).
+fn engine_config<'a>(cfg: &'a GBDTConfig) -> Result<sys::bst_gbdt_config, Error> {
     let mut c = sys::bst_gbdt_config::engine_default();
     match &cfg.objective {
         Objective::SquaredLoss => c.objective = sys::BST_OBJ_SQUARED,
+        Objective::AbsoluteLoss => c.objective = sys::BST_OBJ_ABSOLUTE,
         Objective::LogisticLoss => c.objective = sys::BST_OBJ_LOGISTIC,
+        Objective::HingeLoss => c.objective = sys::BST_OBJ_HINGE,
         Objective::SoftmaxLoss { n_classes } => {
             c.objective = sys::BST_OBJ_SOFTMAX;
             c.n_classes = *n_classes as u32;
         }
+        Objective::PinballLoss { alphas } => {
+            c.objective = sys::BST_OBJ_PINBALL;
+            c.n_alphas = alphas.len() as u32;
+            c.alphas = alphas.as_ptr();
+        }
+        Objective::PseudoHuberLoss { delta } => {
+            c.objective = sys::BST_OBJ_PSEUDO_HUBER;
+            c.huber_delta = *delta;
+        }
+        Objective::PoissonLoss => c.objective = sys::BST_OBJ_POISSON,
         other => return Err(Error::Unsupported(format!("objective {other:?}"))),
     }
     c.metric = match &cfg.metric {
         Metric::None => sys::BST_METRIC_NONE,
         Metric::Rmse => sys::BST_METRIC_RMSE,
+        Metric::Mae => sys::BST_METRIC_MAE,
+        Metric::Mape => sys::BST_METRIC_MAPE,
         Metric::LogLoss => sys::BST_METRIC_LOGLOSS,
+        Metric::Accuracy { threshold } => {
+            c.accuracy_threshold = *threshold;
+            sys::BST_METRIC_ACCURACY
+        }
+        Metric::MarginAccuracy => sys::BST_METRIC_MARGIN_ACCURACY,
         Metric::MulticlassLogLoss => sys::BST_METRIC_MLOGLOSS,
+        Metric::MulticlassAccuracy => sys::BST_METRIC_MULTICLASS_ACCURACY,
+        Metric::Quantile { alphas } => {
+            if c.n_alphas == 0 {
+                c.n_alphas = alphas.len() as u32;
+                c.alphas = alphas.as_ptr();
+            }
+            sys::BST_METRIC_QUANTILE
+        }
+        Metric::Huber { delta } => {
+            c.metric_huber_delta = *delta;
+            sys::BST_METRIC_HUBER
+        }
+        Metric::PoissonDeviance => sys::BST_METRIC_POISSON_DEVIANCE,
         other => return Err(Error::Unsupported(format!("metric {other:?}"))),
     };
     if cfg.linear_leaves.is_some() {

@@ -39,6 +39,11 @@ pub const BST_OBJ_LOGISTIC: i32 = 1;
 pub const BST_OBJ_SOFTMAX: i32 = 2;
 /// Engine extension with no counterpart in the reference.
 pub const BST_OBJ_LAMBDARANK: i32 = 3;
+pub const BST_OBJ_ABSOLUTE: i32 = 4;
+pub const BST_OBJ_HINGE: i32 = 5;
+pub const BST_OBJ_PINBALL: i32 = 6;
+pub const BST_OBJ_PSEUDO_HUBER: i32 = 7;
+pub const BST_OBJ_POISSON: i32 = 8;
 pub const BST_GROW_DEPTHWISE: i32 = 0;
 pub const BST_GROW_LEAFWISE: i32 = 1;
 pub const BST_METRIC_DEFAULT: i32 = -1;
@@ -46,6 +51,14 @@ pub const BST_METRIC_NONE: i32 = 0;
 pub const BST_METRIC_RMSE: i32 = 1;
 pub const BST_METRIC_LOGLOSS: i32 = 2;
 pub const BST_METRIC_MLOGLOSS: i32 = 3;
+pub const BST_METRIC_MAE: i32 = 4;
+pub const BST_METRIC_MAPE: i32 = 5;
+pub const BST_METRIC_QUANTILE: i32 = 6;
+pub const BST_METRIC_HUBER: i32 = 7;
+pub const BST_METRIC_POISSON_DEVIANCE: i32 = 8;
+pub const BST_METRIC_ACCURACY: i32 = 9;
+pub const BST_METRIC_MARGIN_ACCURACY: i32 = 10;
+pub const BST_METRIC_MULTICLASS_ACCURACY: i32 = 11;
 pub const BST_TRANSFORM_IDENTITY: i32 = 0;
 pub const BST_TRANSFORM_SIGMOID: i32 = 1;
 pub const BST_TRANSFORM_SOFTMAX: i32 = 2;
@@ -66,9 +79,10 @@ pub struct bst_binning_config {
 }
 
 /// `GBDTConfig` (`model/gbdt/config.rs:111-226`), field for field; enums as
-/// ints, `early_stopping_rounds` 0 for `None`.
+/// ints, `early_stopping_rounds` 0 for `None`; the objective's and metric's
+/// enum payloads (Huber delta, quantile alphas, accuracy threshold) at the end.
 #[repr(C)]
-#[derive(Clone, Copy, Debug, Default)]
+#[derive(Clone, Copy, Debug)]
 pub struct bst_gbdt_config {
     pub objective: i32,
     pub n_classes: u32,
@@ -92,6 +106,45 @@ pub struct bst_gbdt_config {
     pub cache_size: u64,
     pub seed: u64,
     pub verbosity: i32,
+    pub huber_delta: f32,
+    pub n_alphas: u32,
+    pub alphas: *const f32,
+    pub accuracy_threshold: f32,
+    pub metric_huber_delta: f64,
+}
+
+impl Default for bst_gbdt_config {
+    fn default() -> Self {
+        bst_gbdt_config {
+            objective: 0,
+            n_classes: 0,
+            metric: 0,
+            n_trees: 0,
+            learning_rate: 0.0,
+            growth: 0,
+            max_depth: 0,
+            max_leaves: 0,
+            max_onehot_cats: 0,
+            lambda: 0.0,
+            alpha: 0.0,
+            min_child_weight: 0.0,
+            min_gain: 0.0,
+            min_samples_leaf: 0,
+            subsample: 0.0,
+            colsample_bytree: 0.0,
+            colsample_bylevel: 0.0,
+            binning: bst_binning_config::default(),
+            early_stopping_rounds: 0,
+            cache_size: 0,
+            seed: 0,
+            verbosity: 0,
+            huber_delta: 0.0,
+            n_alphas: 0,
+            alphas: std::ptr::null(),
+            accuracy_threshold: 0.0,
+            metric_huber_delta: 0.0,
+        }
+    }
 }
 
 /// Engine knobs with no counterpart in `GBDTConfig`.  `n_threads` keeps the
@@ -428,6 +481,16 @@ extern "C" {
         ctx: *mut bst_ctx,
         objective: i32,
         n_outputs: i32,
+        preds: *const f32,
+        targets: *const f32,
+        weights: *const f32,
+        n: i64,
+        grad: *mut f32,
+        hess: *mut f32,
+    ) -> c_int;
+    pub fn bst_kernel_gradients_cfg(
+        ctx: *mut bst_ctx,
+        cfg: *const bst_gbdt_config,
         preds: *const f32,
         targets: *const f32,
         weights: *const f32,

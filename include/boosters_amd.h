@@ -41,11 +41,18 @@ extern "C" {
 #define BST_E_INVALID (-7) /* null handle, bad argument                                  */
 
 /* ---- enums mirroring the reference ------------------------------------- */
-/* Objective enum, training/objectives/mod.rs:196-219 (in-scope variants). */
+/* Objective enum, training/objectives/mod.rs:196-219.  Payloads live in
+ * bst_gbdt_config: softmax n_classes, pinball alphas (one output per alpha),
+ * pseudo-Huber delta. */
 #define BST_OBJ_SQUARED 0
 #define BST_OBJ_LOGISTIC 1
 #define BST_OBJ_SOFTMAX 2
 #define BST_OBJ_LAMBDARANK 3 /* extension: no counterpart in the reference */
+#define BST_OBJ_ABSOLUTE 4   /* AbsoluteLoss: sign gradients, leaves renewed to medians */
+#define BST_OBJ_HINGE 5      /* HingeLoss: labels {0, 1} as {-1, +1} */
+#define BST_OBJ_PINBALL 6    /* PinballLoss { alphas }: leaves renewed to alpha-quantiles */
+#define BST_OBJ_PSEUDO_HUBER 7 /* PseudoHuberLoss { delta } */
+#define BST_OBJ_POISSON 8    /* PoissonLoss: log-space scores */
 
 /* GrowthStrategy, training/gbdt/expansion.rs:46-57 */
 #define BST_GROW_DEPTHWISE 0
@@ -58,6 +65,14 @@ extern "C" {
 #define BST_METRIC_RMSE 1
 #define BST_METRIC_LOGLOSS 2
 #define BST_METRIC_MLOGLOSS 3
+#define BST_METRIC_MAE 4                 /* regression.rs:55-87 */
+#define BST_METRIC_MAPE 5                /* regression.rs:89-124 */
+#define BST_METRIC_QUANTILE 6            /* regression.rs:126-174, alphas from the config */
+#define BST_METRIC_HUBER 7               /* regression.rs:216-250, metric_huber_delta */
+#define BST_METRIC_POISSON_DEVIANCE 8    /* regression.rs:176-214 */
+#define BST_METRIC_ACCURACY 9            /* classification.rs:46-83, accuracy_threshold */
+#define BST_METRIC_MARGIN_ACCURACY 10    /* classification.rs:85-122 */
+#define BST_METRIC_MULTICLASS_ACCURACY 11 /* classification.rs:124-192 */
 
 /* Output transform, model/transform.rs:23-31 */
 #define BST_TRANSFORM_IDENTITY 0
@@ -103,6 +118,12 @@ typedef struct bst_gbdt_config {
   uint64_t cache_size;        /* default 8 (accepted; device keeps every node histogram) */
   uint64_t seed;              /* default 42 */
   int32_t verbosity;          /* default 0  */
+  /* Objective and metric payloads (enum data of Objective / Metric). */
+  float huber_delta;          /* PseudoHuberLoss { delta }, default 1.0 */
+  uint32_t n_alphas;          /* PinballLoss { alphas } / Metric::Quantile: count */
+  const float* alphas;        /* [n_alphas], each in (0, 1); borrowed for the call */
+  float accuracy_threshold;   /* Metric::Accuracy { threshold }, default 0.5 */
+  double metric_huber_delta;  /* Metric::Huber { delta }, default 1.0 */
 } bst_gbdt_config;
 
 /* Extension knobs that have no counterpart in GBDTConfig. */
@@ -331,6 +352,13 @@ int bst_server_destroy(bst_server* srv);
 int bst_kernel_gradients(bst_ctx* ctx, int32_t objective, int32_t n_outputs, const float* preds,
                          const float* targets, const float* weights, int64_t n, float* grad,
                          float* hess);
+/* The same with the objective's payload from a config (pinball alphas, one
+ * output each; pseudo-Huber delta); outputs follow Objective::n_outputs.
+ * bst_kernel_gradients runs the payload-free objectives (pseudo-Huber with
+ * delta 1.0) and rejects pinball. */
+int bst_kernel_gradients_cfg(bst_ctx* ctx, const bst_gbdt_config* cfg, const float* preds,
+                             const float* targets, const float* weights, int64_t n, float* grad,
+                             float* hess);
 /* LambdaRank gradients over ragged query groups (group_sizes [n_groups],
  * each 1..4096 rows, summing to n; targets are integer relevance labels
  * 0..31).  No counterpart in the reference (its objectives stop at

@@ -223,9 +223,101 @@ Analysis analyze(const float* col, int64_t n, bool user_cat, const bst_binning_c
 // ---------------------------------------------------------------------------
 inline float wt(const float* w, int64_t i) { return w ? w[i] : 1.0f; }
 
+// The enum payloads of Objective / Metric that the arithmetic needs
+// (objectives/mod.rs:196-219; metrics/mod.rs:178-207).
+struct Payload {
+  float huber_delta = 1.0f;
+  std::vector<float> alphas;
+  float accuracy_threshold = 0.5f;
+  double metric_huber_delta = 1.0;
+};
+Payload payload_of(const bst_gbdt_config* c) {
+  Payload P;
+  if (!c) return P;
+  P.huber_delta = c->huber_delta;
+  if (c->alphas && c->n_alphas > 0) P.alphas.assign(c->alphas, c->alphas + c->n_alphas);
+  P.accuracy_threshold = c->accuracy_threshold;
+  P.metric_huber_delta = c->metric_huber_delta;
+  return P;
+}
+
+// Objective::n_outputs, objectives/mod.rs:221-233.
+int n_outputs_of(const bst_gbdt_config& c) {
+  if (c.objective == BST_OBJ_SOFTMAX) return (int)c.n_classes;
+  if (c.objective == BST_OBJ_PINBALL) return (int)c.n_alphas;
+  return 1;
+}
+
+// Rust f32::signum: +1 for +0.0 and positives, -1 for -0.0 and negatives,
+// NaN for NaN.
+inline float signum32(float v) { return std::isnan(v) ? v : std::copysign(1.0f, v); }
+
 void gradients(int obj, int K, const float* preds, const float* y, const float* w, int64_t n,
-               float* g, float* h) {
+               float* g, float* h, const Payload& P = Payload()) {
   const float HESS_MIN = 1e-6f;
+  if (obj == BST_OBJ_ABSOLUTE) {  // regression.rs:80-104
+    for (int k = 0; k < K; k++)
+      for (int64_t i = 0; i < n; i++) {
+        float wi = wt(w, i);
+        float diff = preds[k * n + i] - y[i];
+        g[k * n + i] = wi * signum32(diff);
+        h[k * n + i] = wi;
+      }
+    return;
+  }
+  if (obj == BST_OBJ_PINBALL) {  // regression.rs:129-156
+    for (int k = 0; k < K; k++) {
+      const float alpha = P.alphas[k];
+      for (int64_t i = 0; i < n; i++) {
+        float wi = wt(w, i);
+        float diff = preds[k * n + i] - y[i];
+        float gk = diff < 0.0f ? -alpha : 1.0f - alpha;
+        g[k * n + i] = wi * gk;
+        h[k * n + i] = wi;
+      }
+    }
+    return;
+  }
+  if (obj == BST_OBJ_PSEUDO_HUBER) {  // regression.rs:185-216
+    const float delta = P.huber_delta;
+    const float inv_delta_sq = 1.0f / (delta * delta);
+    for (int k = 0; k < K; k++)
+      for (int64_t i = 0; i < n; i++) {
+        float wi = wt(w, i);
+        float residual = preds[k * n + i] - y[i];
+        float r_sq = residual * residual;
+        float factor = 1.0f + r_sq * inv_delta_sq;
+        float sqrt_factor = std::sqrt(factor);
+        g[k * n + i] = wi * residual / sqrt_factor;
+        h[k * n + i] = wi / (factor * sqrt_factor);
+      }
+    return;
+  }
+  if (obj == BST_OBJ_POISSON) {  // regression.rs:244-270
+    const float MAX_EXP = 30.0f;
+    for (int k = 0; k < K; k++)
+      for (int64_t i = 0; i < n; i++) {
+        float wi = wt(w, i);
+        float pc = preds[k * n + i];  // f32::clamp keeps NaN
+        if (pc < -MAX_EXP) pc = -MAX_EXP;
+        if (pc > MAX_EXP) pc = MAX_EXP;
+        float e = exp32(pc);
+        g[k * n + i] = wi * (e - y[i]);
+        h[k * n + i] = fmaxf(wi * e, HESS_MIN);  // f32::max ignores NaN
+      }
+    return;
+  }
+  if (obj == BST_OBJ_HINGE) {  // classification.rs:87-116
+    for (int k = 0; k < K; k++)
+      for (int64_t i = 0; i < n; i++) {
+        float wi = wt(w, i);
+        float yy = y[i] > 0.5f ? 1.0f : -1.0f;
+        float margin = yy * preds[k * n + i];
+        g[k * n + i] = margin < 1.0f ? -wi * yy : 0.0f;
+        h[k * n + i] = wi;
+      }
+    return;
+  }
   if (obj == BST_OBJ_SQUARED) {  // regression.rs:20-42
     for (int k = 0; k < K; k++)
       for (int64_t i = 0; i < n; i++) {
@@ -329,8 +421,85 @@ void lambdarank_gradients(const float* preds, const float* y, const float* w, in
   }
 }
 
-std::vector<float> base_score(int obj, int K, const float* y, const float* w, int64_t n) {
+// compute_weighted_quantile, regression.rs:307-333: (value, weight) pairs
+// sorted by value (a stable sort), total weight summed in f32 in sorted
+// order, the first value whose running f32 weight reaches alpha * total.
+float weighted_quantile(const float* v, const float* w, int64_t n, float alpha) {
+  if (n == 0) return 0.0f;
+  std::vector<std::pair<float, float>> sorted((size_t)n);
+  for (int64_t i = 0; i < n; i++) sorted[i] = {v[i], wt(w, i)};
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [](const std::pair<float, float>& a, const std::pair<float, float>& b) {
+                     return a.first < b.first;
+                   });
+  float total = 0.0f;
+  for (auto& p : sorted) total += p.second;
+  const float target = alpha * total;
+  float cum = 0.0f;
+  for (auto& p : sorted) {
+    cum += p.second;
+    if (cum >= target) return p.first;
+  }
+  return sorted.back().first;
+}
+
+// unweighted_quantile_indexed, utils.rs:17-53: the k-th smallest value,
+// k = ceil(alpha * n as f32) - 1 clamped to [0, n - 1].
+template <typename V>
+float unweighted_quantile_indexed(const uint32_t* ind, uint32_t n, V value, float alpha) {
+  if (n == 0) return std::numeric_limits<float>::quiet_NaN();
+  if (n == 1) return value(ind[0]);
+  int64_t k = (int64_t)std::ceil(alpha * (float)n) - 1;
+  if (k < 0) k = 0;
+  if (k > (int64_t)n - 1) k = (int64_t)n - 1;
+  std::vector<float> vals(n);
+  for (uint32_t i = 0; i < n; i++) vals[i] = value(ind[i]);
+  std::nth_element(vals.begin(), vals.begin() + k, vals.end());
+  return vals[(size_t)k];
+}
+
+// weighted_quantile_indexed, utils.rs:81-126: indices sorted by value (stable,
+// ties keep the leaf's row order), total weight summed in f32 over the
+// indices in the leaf's order, running f32 weight in sorted order.
+template <typename V>
+float weighted_quantile_indexed(const uint32_t* ind, uint32_t n, V value, const float* w,
+                                float alpha) {
+  if (n == 0) return std::numeric_limits<float>::quiet_NaN();
+  if (n == 1) return value(ind[0]);
+  std::vector<uint32_t> sc(ind, ind + n);
+  std::stable_sort(sc.begin(), sc.end(), [&](uint32_t a, uint32_t b) { return value(a) < value(b); });
+  float total = 0.0f;
+  for (uint32_t i = 0; i < n; i++) total += w[ind[i]];
+  const float threshold = total * alpha;
+  float cum = 0.0f;
+  for (uint32_t i = 0; i < n; i++) {
+    cum += w[sc[i]];
+    if (cum >= threshold) return value(sc[i]);
+  }
+  return value(sc[n - 1]);
+}
+
+std::vector<float> base_score(int obj, int K, const float* y, const float* w, int64_t n,
+                              const Payload& P = Payload()) {
   if (obj == BST_OBJ_LAMBDARANK) return {0.0f};
+  if (obj == BST_OBJ_HINGE) return {0.0f};  // classification.rs:118-123
+  if (obj == BST_OBJ_ABSOLUTE || obj == BST_OBJ_PSEUDO_HUBER)  // regression.rs:106-117,218-230
+    return {weighted_quantile(y, w, n, 0.5f)};
+  if (obj == BST_OBJ_PINBALL) {  // regression.rs:158-172
+    std::vector<float> out;
+    for (int k = 0; k < K; k++) out.push_back(n == 0 ? 0.0f : weighted_quantile(y, w, n, P.alphas[k]));
+    return out;
+  }
+  if (obj == BST_OBJ_POISSON) {  // regression.rs:272-296
+    if (n == 0) return {0.0f};
+    double sw = 0, swy = 0;
+    for (int64_t i = 0; i < n; i++) {
+      sw += (double)wt(w, i);
+      swy += (double)wt(w, i) * (double)y[i];
+    }
+    double mean = sw > 0.0 ? std::max(swy / sw, 1e-7) : 1.0;
+    return {(float)std::log(mean)};
+  }
   if (obj == BST_OBJ_SQUARED) {  // regression.rs:44-68
     if (n == 0) return {0.0f};
     double sw = 0, swy = 0;
@@ -782,6 +951,7 @@ struct Grower {
   uint32_t n_part = 0;
   std::vector<std::vector<double>> hist;  // by partition node id
   std::vector<float> last_leaf;           // by partition node id (lr applied)
+  std::vector<std::pair<uint32_t, uint32_t>> leaf_map;  // (tree node, partition node) per leaf
   int64_t hist_rows = 0;
   std::vector<uint32_t> feats;
   uint32_t round = 0, output = 0;  // identify the tree for column sampling
@@ -827,6 +997,7 @@ struct Grower {
     n_part = 1;
     hist.assign(max_nodes + 2, {});
     last_leaf.assign(max_nodes + 2, std::numeric_limits<float>::quiet_NaN());
+    leaf_map.clear();
     sample_columns();
 
     OTree t;
@@ -842,6 +1013,7 @@ struct Grower {
       t.leaf[cd.tree_node] = w;
       t.cover[cd.tree_node] = (float)cd.h;
       last_leaf[cd.node] = w * c.learning_rate;
+      leaf_map.push_back({cd.tree_node, cd.node});
       hist[cd.node].clear();
     };
     auto expand = [&](const Cand& cd, std::vector<Cand>& out) {
@@ -1023,9 +1195,106 @@ void transform(int tr, float* out, int K, int64_t n) {
   }
 }
 
-double metric_value(int metric, const float* p, int K, const float* y, const float* w, int64_t n) {
+double metric_value(int metric, const float* p, int K, const float* y, const float* w, int64_t n,
+                    const Payload& P = Payload()) {
   if (n == 0) return 0.0;
   const double EPS = 1e-15;
+  if (metric == BST_METRIC_MAE) {  // regression.rs:55-87
+    double s = 0;
+    for (int k = 0; k < K; k++) {
+      double sa = 0, sw = 0;
+      for (int64_t i = 0; i < n; i++) {
+        double ae = std::fabs((double)p[k * n + i] - (double)y[i]);
+        sa += (double)wt(w, i) * ae;
+        sw += (double)wt(w, i);
+      }
+      s += sw > 0.0 ? sa / sw : 0.0;
+    }
+    return s / K;
+  }
+  if (metric == BST_METRIC_MAPE) {  // regression.rs:89-124
+    double sa = 0, sw = 0;
+    for (int64_t i = 0; i < n; i++) {
+      double pp = p[i], l = y[i];
+      double ape = std::fabs(pp - l) / std::max(std::fabs(l), EPS);
+      sa += (double)wt(w, i) * ape;
+      sw += (double)wt(w, i);
+    }
+    return sw > 0.0 ? (sa / sw) * 100.0 : 0.0;
+  }
+  if (metric == BST_METRIC_QUANTILE) {  // regression.rs:126-174
+    double tl = 0, tw = 0;
+    for (int q = 0; q < K && q < (int)P.alphas.size(); q++) {
+      double a = (double)P.alphas[q];
+      double l = 0, ww = 0;
+      for (int64_t i = 0; i < n; i++) {
+        double r = (double)y[i] - (double)p[q * n + i];
+        double wi = (double)wt(w, i);
+        double loss = r >= 0.0 ? a * r : (1.0 - a) * (-r);
+        l += wi * loss;
+        ww += wi;
+      }
+      tl += l;
+      tw += ww;
+    }
+    return tw > 0.0 ? tl / tw : 0.0;
+  }
+  if (metric == BST_METRIC_HUBER) {  // regression.rs:216-250
+    double sl = 0, sw = 0, d = P.metric_huber_delta;
+    for (int64_t i = 0; i < n; i++) {
+      double r = std::fabs((double)p[i] - (double)y[i]);
+      double wi = (double)wt(w, i);
+      double loss = r <= d ? 0.5 * r * r : d * (r - 0.5 * d);
+      sl += wi * loss;
+      sw += wi;
+    }
+    return sw > 0.0 ? sl / sw : 0.0;
+  }
+  if (metric == BST_METRIC_POISSON_DEVIANCE) {  // regression.rs:176-214
+    const double E9 = 1e-9;
+    double sd = 0, sw = 0;
+    for (int64_t i = 0; i < n; i++) {
+      double yy = y[i], mu = std::max((double)p[i], E9), wi = (double)wt(w, i);
+      double dev = yy > E9 ? 2.0 * (yy * std::log(yy / mu) - (yy - mu)) : 2.0 * mu;
+      sd += wi * dev;
+      sw += wi;
+    }
+    return sw > 0.0 ? sd / sw : 0.0;
+  }
+  if (metric == BST_METRIC_ACCURACY || metric == BST_METRIC_MARGIN_ACCURACY) {
+    // classification.rs:46-122 (margin accuracy: threshold 0 on raw scores)
+    const float t = metric == BST_METRIC_ACCURACY ? P.accuracy_threshold : 0.0f;
+    double sc = 0, sw = 0;
+    for (int64_t i = 0; i < n; i++) {
+      float pc = p[i] >= t ? 1.0f : 0.0f;
+      double correct = std::fabs(pc - y[i]) < 0.5f ? 1.0 : 0.0;
+      sc += (double)wt(w, i) * correct;
+      sw += (double)wt(w, i);
+    }
+    return sw > 0.0 ? sc / sw : 0.0;
+  }
+  if (metric == BST_METRIC_MULTICLASS_ACCURACY) {  // classification.rs:124-192
+    double sc = 0, sw = 0;
+    for (int64_t i = 0; i < n; i++) {
+      float pc;
+      if (K == 1) {
+        pc = std::round(p[i]);  // f32::round: half away from zero
+      } else {
+        // Iterator::max_by keeps the last of equal maxima; NaN compares Less.
+        int best = 0;
+        for (int k = 1; k < K; k++) {
+          float va = p[(int64_t)best * n + i], vb = p[(int64_t)k * n + i];
+          bool less = std::isnan(va) || std::isnan(vb) ? true : !(vb < va);
+          if (less) best = k;  // cmp(best, k) != Greater -> k becomes the max
+        }
+        pc = (float)best;
+      }
+      double correct = std::fabs(pc - y[i]) < 0.5f ? 1.0 : 0.0;
+      sc += (double)wt(w, i) * correct;
+      sw += (double)wt(w, i);
+    }
+    return sw > 0.0 ? sc / sw : 0.0;
+  }
   if (metric == BST_METRIC_RMSE) {  // regression.rs:17-40
     double s = 0;
     for (int k = 0; k < K; k++) {
@@ -1061,11 +1330,21 @@ double metric_value(int metric, const float* p, int K, const float* y, const flo
   return std::numeric_limits<double>::quiet_NaN();
 }
 
-int default_metric(int obj) {
+int default_metric(int obj) {  // default_metric_for_objective, metrics/mod.rs:213-232
   if (obj == BST_OBJ_LAMBDARANK) return BST_METRIC_NONE;  // no ranking metric in the reference
+  if (obj == BST_OBJ_PINBALL) return BST_METRIC_QUANTILE;
+  if (obj == BST_OBJ_HINGE) return BST_METRIC_MARGIN_ACCURACY;
   return obj == BST_OBJ_LOGISTIC ? BST_METRIC_LOGLOSS
                                  : (obj == BST_OBJ_SOFTMAX ? BST_METRIC_MLOGLOSS : BST_METRIC_RMSE);
 }
+
+// Metric::higher_is_better, metrics/mod.rs:286-301.
+bool higher_is_better(int metric) {
+  return metric == BST_METRIC_ACCURACY || metric == BST_METRIC_MARGIN_ACCURACY ||
+         metric == BST_METRIC_MULTICLASS_ACCURACY;
+}
+// PredictionKind::Margin metrics read untransformed scores, eval.rs:150-184.
+bool metric_on_margins(int metric) { return metric == BST_METRIC_MARGIN_ACCURACY; }
 
 int transform_of(int obj) {
   return obj == BST_OBJ_LOGISTIC ? BST_TRANSFORM_SIGMOID
@@ -1275,6 +1554,33 @@ int ora_base_score(int32_t obj, int32_t K, const float* y, const float* w, int64
   return BST_OK;
 }
 
+int ora_gradients_cfg(const bst_gbdt_config* cfg, const float* p, const float* y, const float* w,
+                      int64_t n, float* g, float* h) {
+  if (!cfg || !p || !y || !g || !h) return fail(BST_E_INVALID, "null");
+  if (cfg->objective == BST_OBJ_LAMBDARANK) return fail(BST_E_CONFIG, "use ora_lambdarank_gradients");
+  gradients(cfg->objective, n_outputs_of(*cfg), p, y, w, n, g, h, payload_of(cfg));
+  return BST_OK;
+}
+
+int ora_base_score_cfg(const bst_gbdt_config* cfg, const float* y, const float* w, int64_t n,
+                       float* out) {
+  if (!cfg || !out) return fail(BST_E_INVALID, "null");
+  auto b = base_score(cfg->objective, n_outputs_of(*cfg), y, w, n, payload_of(cfg));
+  for (size_t i = 0; i < b.size(); i++) out[i] = b[i];
+  return BST_OK;
+}
+
+float ora_weighted_quantile(const float* v, const float* w, int64_t n, float alpha) {
+  return weighted_quantile(v, w, n, alpha);
+}
+
+float ora_quantile_indexed(const uint32_t* ind, uint32_t n, const float* values, const float* w,
+                           float alpha) {
+  auto value = [&](uint32_t i) { return values[i]; };
+  return w ? weighted_quantile_indexed(ind, n, value, w, alpha)
+           : unweighted_quantile_indexed(ind, n, value, alpha);
+}
+
 int ora_histogram(const ora_binned* b, const float* g, const float* h, const uint32_t* rows,
                   int64_t nr, double* out) {
   if (!b || !g || !h || !out) return fail(BST_E_INVALID, "null");
@@ -1319,7 +1625,9 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
   if (extp) ext = *extp;
   int64_t n = b->n;
   if (ny < n) return fail(BST_E_TARGETS, "fewer targets than rows");
-  int K = c.objective == BST_OBJ_SOFTMAX ? (int)c.n_classes : 1;
+  const int K = n_outputs_of(c);
+  const Payload P = payload_of(&c);
+  if (K < 1) return fail(BST_E_CONFIG, "the objective has no outputs");
   if (c.objective == BST_OBJ_LAMBDARANK) {
     if (!ext.query_group_sizes || ext.n_query_groups < 1)
       return fail(BST_E_CONFIG, "LambdaRank needs query groups");
@@ -1336,7 +1644,7 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
   M->n_groups = K;
   M->n_features = b->F;
   M->transform = transform_of(c.objective);
-  M->base = base_score(c.objective, K, y, w, n);
+  M->base = base_score(c.objective, K, y, w, n, P);
   std::vector<float> preds((size_t)K * n), g((size_t)K * n), h((size_t)K * n);
   for (int k = 0; k < K; k++) std::fill(preds.begin() + (size_t)k * n, preds.begin() + (size_t)(k + 1) * n, M->base[k]);
   int metric = c.metric == BST_METRIC_DEFAULT ? default_metric(c.objective) : c.metric;
@@ -1350,7 +1658,10 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
   bool uniform = !goss && !(std::fabs(c.subsample - 1.0f) < 1e-6f);
   size_t warmup = goss ? goss_warmup(c.learning_rate) : 0;
   int es_patience = (int)c.early_stopping_rounds;
-  bool hib = false;
+  const bool hib = higher_is_better(metric);
+  // Leaf renewal (trainer.rs:265-291): quantile objectives replace each
+  // leaf's Newton value by the alpha-quantile of its rows' residuals.
+  const bool renew = c.objective == BST_OBJ_ABSOLUTE || c.objective == BST_OBJ_PINBALL;
   double best_val = 0;
   bool have_best = false;
   size_t best_round = 0, cur_round = 0, best_n_trees = 0;
@@ -1359,7 +1670,7 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
       lambdarank_gradients(preds.data(), y, w, n, ext.query_group_sizes, ext.n_query_groups,
                            g.data(), h.data());
     else
-      gradients(c.objective, K, preds.data(), y, w, n, g.data(), h.data());
+      gradients(c.objective, K, preds.data(), y, w, n, g.data(), h.data(), P);
     for (int k = 0; k < K; k++) {
       float* gk = g.data() + (size_t)k * n;
       float* hk = h.data() + (size_t)k * n;
@@ -1379,6 +1690,23 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
       M->trees.push_back(std::move(t));
       M->groups.push_back((uint32_t)k);
       OTree& tt = M->trees.back();
+      if (renew) {
+        const float alpha = c.objective == BST_OBJ_PINBALL ? P.alphas[k] : 0.5f;
+        const float* pk = preds.data() + (size_t)k * n;
+        auto residual = [&](uint32_t i) { return y[i] - pk[i]; };
+        for (auto& lm : gr.leaf_map) {
+          const uint32_t tn = lm.first, pn = lm.second;
+          const uint32_t cnt = gr.leaf_count[pn];
+          if (cnt == 0) continue;
+          const uint32_t* ind = gr.idx.data() + gr.leaf_begin[pn];
+          float v = w ? weighted_quantile_indexed(ind, cnt, residual, w, alpha)
+                      : unweighted_quantile_indexed(ind, cnt, residual, alpha);
+          if (std::isnan(v)) continue;
+          const float sv = v * c.learning_rate;
+          tt.leaf[tn] = sv;
+          gr.last_leaf[pn] = sv;
+        }
+      }
       // Prediction updates: partition path unless sampled (trainer.rs:331-339).
       if (!use_sample) {
         gr.update_preds(preds.data() + (size_t)k * n);
@@ -1445,14 +1773,15 @@ int ora_train(const ora_binned* b, const float* X, const float* y, int64_t ny, c
     // evaluate_round (eval.rs:218-260) on transformed predictions.
     double value = std::numeric_limits<double>::quiet_NaN();
     if (metric != BST_METRIC_NONE) {
+      const int tr = metric_on_margins(metric) ? BST_TRANSFORM_IDENTITY : M->transform;
       if (!vpreds.empty()) {
         std::vector<float> tp = vpreds;
-        transform(M->transform, tp.data(), K, nv);
-        value = metric_value(metric, tp.data(), K, vy, nullptr, nv);
+        transform(tr, tp.data(), K, nv);
+        value = metric_value(metric, tp.data(), K, vy, nullptr, nv, P);
       } else {
         std::vector<float> tp = preds;
-        transform(M->transform, tp.data(), K, n);
-        value = metric_value(metric, tp.data(), K, y, w, n);
+        transform(tr, tp.data(), K, n);
+        value = metric_value(metric, tp.data(), K, y, w, n, P);
       }
     }
     M->metric.push_back(value);
@@ -1677,5 +2006,12 @@ double ora_metric(int32_t metric, const float* p, int32_t K, const float* y, con
                   int64_t n) {
   return metric_value(metric, p, K, y, w, n);
 }
+
+double ora_metric_cfg(const bst_gbdt_config* cfg, int32_t metric, const float* p, int32_t K,
+                      const float* y, const float* w, int64_t n) {
+  return metric_value(metric, p, K, y, w, n, payload_of(cfg));
+}
+
+int ora_default_metric(int32_t objective) { return default_metric(objective); }
 
 }  // extern "C"

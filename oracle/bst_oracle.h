@@ -62,6 +62,18 @@ int ora_lambdarank_gradients(const float* preds, const float* targets, const flo
                              float* grad, float* hess);
 int ora_base_score(int32_t objective, int32_t n_outputs, const float* targets,
                    const float* weights, int64_t n, float* out);
+/* The same with the objective's payload from a config (pinball alphas,
+ * pseudo-Huber delta); n_outputs follows Objective::n_outputs. */
+int ora_gradients_cfg(const bst_gbdt_config* cfg, const float* preds, const float* targets,
+                      const float* weights, int64_t n, float* grad, float* hess);
+int ora_base_score_cfg(const bst_gbdt_config* cfg, const float* targets, const float* weights,
+                       int64_t n, float* out);
+/* compute_weighted_quantile (objectives/regression.rs:307-333, base scores)
+ * and the leaf-renewal quantiles over a leaf's rows (utils.rs:17-126;
+ * weights nullable = unweighted). */
+float ora_weighted_quantile(const float* values, const float* weights, int64_t n, float alpha);
+float ora_quantile_indexed(const uint32_t* indices, uint32_t n, const float* values,
+                           const float* weights, float alpha);
 
 /* Histogram of one node in partition order, (f64, f64) bins: ops.rs:79-283. */
 int ora_histogram(const ora_binned* b, const float* grad, const float* hess, const uint32_t* rows,
@@ -122,6 +134,12 @@ int ora_sample_cols(uint32_t n_feat, const bst_gbdt_config* cfg, uint32_t round,
 /* Metrics over transformed predictions [K][n]: metrics/{classification,regression}.rs. */
 double ora_metric(int32_t metric, const float* preds, int32_t n_outputs, const float* targets,
                   const float* weights, int64_t n);
+/* With the metric's payload (quantile alphas, accuracy threshold, Huber
+ * delta) from a config. */
+double ora_metric_cfg(const bst_gbdt_config* cfg, int32_t metric, const float* preds,
+                      int32_t n_outputs, const float* targets, const float* weights, int64_t n);
+/* default_metric_for_objective, metrics/mod.rs:213-232. */
+int ora_default_metric(int32_t objective);
 
 #ifdef __cplusplus
 }

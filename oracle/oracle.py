@@ -100,6 +100,17 @@ def _load(path):
                                       P(C.c_float), P(C.c_uint32), P(C.c_int64)]),
         "ora_sample_cols": (C.c_int, [C.c_uint32, P(A.GBDTConfigC), C.c_uint32, C.c_uint32,
                                       P(C.c_uint32), P(C.c_int32)]),
+        "ora_gradients_cfg": (C.c_int, [P(A.GBDTConfigC), P(C.c_float), P(C.c_float),
+                                        P(C.c_float), C.c_int64, P(C.c_float), P(C.c_float)]),
+        "ora_base_score_cfg": (C.c_int, [P(A.GBDTConfigC), P(C.c_float), P(C.c_float),
+                                         C.c_int64, P(C.c_float)]),
+        "ora_weighted_quantile": (C.c_float, [P(C.c_float), P(C.c_float), C.c_int64,
+                                              C.c_float]),
+        "ora_quantile_indexed": (C.c_float, [P(C.c_uint32), C.c_uint32, P(C.c_float),
+                                             P(C.c_float), C.c_float]),
+        "ora_metric_cfg": (C.c_double, [P(A.GBDTConfigC), C.c_int32, P(C.c_float), C.c_int32,
+                                        P(C.c_float), P(C.c_float), C.c_int64]),
+        "ora_default_metric": (C.c_int, [C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -214,6 +225,51 @@ def gradients(objective, preds, targets, weights=None):
                                A.ptr(np.ascontiguousarray(targets, np.float32)), A.ptr(w), n,
                                A.ptr(g), A.ptr(h)))
     return g, h
+
+
+def gradients_cfg(cfg, preds, targets, weights=None):
+    """Gradients with the objective's payload (pinball alphas, Huber delta)."""
+    preds = np.ascontiguousarray(preds, np.float32)
+    g = np.zeros_like(preds)
+    h = np.zeros_like(preds)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    _check(lib().ora_gradients_cfg(C.byref(cfg), A.ptr(preds),
+                                   A.ptr(np.ascontiguousarray(targets, np.float32)), A.ptr(w),
+                                   preds.shape[-1], A.ptr(g), A.ptr(h)))
+    return g, h
+
+
+def base_score_cfg(cfg, targets, n_outputs=1, weights=None):
+    out = np.zeros(n_outputs, np.float32)
+    t = np.ascontiguousarray(targets, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    _check(lib().ora_base_score_cfg(C.byref(cfg), A.ptr(t), A.ptr(w), len(t), A.ptr(out)))
+    return out
+
+
+def weighted_quantile(values, weights, alpha):
+    v = np.ascontiguousarray(values, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    return lib().ora_weighted_quantile(A.ptr(v), A.ptr(w), len(v), float(alpha))
+
+
+def quantile_indexed(indices, values, alpha, weights=None):
+    ind = np.ascontiguousarray(indices, np.uint32)
+    v = np.ascontiguousarray(values, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    return lib().ora_quantile_indexed(A.ptr(ind), len(ind), A.ptr(v), A.ptr(w), float(alpha))
+
+
+def metric_cfg(cfg, metric_id, preds, targets, weights=None):
+    preds = np.ascontiguousarray(preds, np.float32)
+    K, n = preds.shape
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    return lib().ora_metric_cfg(C.byref(cfg), metric_id, A.ptr(preds), K,
+                                A.ptr(np.ascontiguousarray(targets, np.float32)), A.ptr(w), n)
+
+
+def default_metric(objective):
+    return lib().ora_default_metric(objective)
 
 
 def lambdarank_gradients(preds, targets, group_sizes, weights=None):

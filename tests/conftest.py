@@ -54,6 +54,10 @@ def make_cfg(**kw):
     c.early_stopping_rounds = 0
     c.cache_size = 8
     c.seed = 42
+    c.huber_delta = 1.0
+    c.n_alphas = 0
+    c.accuracy_threshold = 0.5
+    c.metric_huber_delta = 1.0
     c.verbosity = 0
     for k, v in kw.items():
         setattr(c, "lambda_" if k == "lambda" else k, v)
