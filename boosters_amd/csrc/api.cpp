@@ -205,8 +205,6 @@ int validate(const bst_gbdt_config* c) {
     raise(BST_E_CONFIG, "colsample_bytree must be in (0, 1]");
   if (!(c->colsample_bylevel > 0.0f && c->colsample_bylevel <= 1.0f))
     raise(BST_E_CONFIG, "colsample_bylevel must be in (0, 1]");
-  if (!(std::fabs(c->colsample_bylevel - 1.0f) < 1e-6f))
-    raise(BST_E_CONFIG, "colsample_bylevel < 1 is not supported by this engine build");
   if (c->lambda < 0.0f) raise(BST_E_CONFIG, "lambda must be non-negative");
   if (c->alpha < 0.0f) raise(BST_E_CONFIG, "alpha must be non-negative");
   if (c->min_child_weight < 0.0f) raise(BST_E_CONFIG, "min_child_weight must be non-negative");

@@ -13,7 +13,7 @@
 
 namespace bst {
 
-constexpr uint32_t kStreamRows = 1, kStreamGoss = 2, kStreamCols = 3;
+constexpr uint32_t kStreamRows = 1, kStreamGoss = 2, kStreamCols = 3, kStreamLevel = 4;
 
 __host__ __device__ inline uint64_t splitmix64_mix(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
@@ -26,6 +26,12 @@ __host__ __device__ inline uint64_t sample_key_base(uint64_t seed, uint32_t stre
                                                     uint32_t round, uint32_t output) {
   return splitmix64_mix(seed ^ splitmix64_mix(((uint64_t)round << 32) |
                                               ((uint64_t)output << 8) | stream));
+}
+
+// Keys of a tree's level subset at `depth` (column sampling by level).
+__host__ __device__ inline uint64_t level_key_base(uint64_t seed, uint32_t round, uint32_t output,
+                                                   uint32_t depth) {
+  return sample_key_base(seed, kStreamLevel, round, output | (depth << 12));
 }
 
 __host__ __device__ inline uint32_t sample_key(uint64_t base, uint32_t i) {

@@ -131,7 +131,7 @@ Trainer::Trainer(bst_ctx* ctx, bst_dataset* ds, const TrainOptions& opt)
   const uint32_t max_leaves = c.growth == BST_GROW_DEPTHWISE ? (1u << c.max_depth) : c.max_leaves;
   MN_ = (uint32_t)std::max<uint64_t>(2ull * max_leaves - 1, 63) + 2;
   SPT_ = max_leaves + 2;
-  const bool col_sampling = !(std::fabs(c.colsample_bytree - 1.0f) < 1e-6f);
+  const bool col_sampling = !(std::fabs(c.colsample_bytree - 1.0f) < 1e-6f) || level_sampling();
   // Class trees of a round grow together, except on the host-driven path with
   // column sampling (its split launch takes one feature list).
   // Positions of the trees of a round are 32-bit (t * n + i): a round whose
@@ -442,6 +442,13 @@ void Trainer::expand_batch(std::vector<TreeState>& trees,
   launch_node_sums(s_, d_segs_.p, (int)segs.size(), ogh_.p, 0, qexp, node_sum_.p);
   launch_subtract(s_, d_u32b_.p, (int)quads_full.size() / 4, hist_.p, slot_elems_, node_sum_.p);
   launch_subtract(s_, d_u32c_.p, (int)quads_sums.size() / 4, hist_.p, 0, node_sum_.p);
+  if (!snodes.empty() && level_sampling()) {
+    // The children's level (sample_level(new_depth), grower.rs:443-444): one
+    // tree at a time on this path, and its expanded nodes share a depth.
+    uint32_t depth = 0;
+    for (auto& tc : chosen) depth = std::max(depth, tc.second.depth + 1);
+    set_level_features(depth);
+  }
   if (!snodes.empty()) {
     std::vector<uint32_t> both(snodes);
     both.insert(both.end(), sslots.begin(), sslots.end());
@@ -576,6 +583,32 @@ void Trainer::renew_round(std::vector<TreeState>& trees, const float* y, const f
     ts.tb.leaf[lm.first] = sv;
     ts.leaf_lr[lm.second] = sv;
   }
+}
+
+bool Trainer::level_sampling() const {
+  return !(std::fabs(opt_.cfg.colsample_bylevel - 1.0f) < 1e-6f);
+}
+
+// ColSampler::sample_level (column.rs:199-224): the level's features are the
+// ceil(|tree features| x colsample_bylevel) tree features with the smallest
+// sampling keys of (round, output, depth); without level sampling, the tree's.
+void Trainer::set_level_features(uint32_t depth) {
+  if (level_sampling()) {
+    const uint32_t kk = colsample_count((uint32_t)tree_feats_h_.size(), opt_.cfg.colsample_bylevel);
+    const uint64_t base = level_key_base(opt_.cfg.seed, cur_round_, (uint32_t)cur_output_, depth);
+    std::vector<uint64_t> kv(tree_feats_h_.size());
+    for (size_t j = 0; j < tree_feats_h_.size(); j++)
+      kv[j] = ((uint64_t)sample_key(base, tree_feats_h_[j]) << 32) | tree_feats_h_[j];
+    std::sort(kv.begin(), kv.end());
+    feats_h_.resize(kk);
+    for (uint32_t j = 0; j < kk; j++) feats_h_[j] = (uint32_t)(kv[j] & 0xFFFFFFFFu);
+    std::sort(feats_h_.begin(), feats_h_.end());
+  } else {
+    feats_h_ = tree_feats_h_;
+  }
+  h2d(s_, feats_.p, feats_h_.data(), feats_h_.size());
+  // The split kernel's tie rule depends on the number of candidate features.
+  sp_.rightmost = (opt_.ext.n_threads != 1 && feats_h_.size() >= 16) ? 1 : 0;
 }
 
 int Trainer::metric_kind() const {
@@ -753,7 +786,8 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   feats_h_.resize(E);
   for (uint32_t i = 0; i < E; i++) feats_h_[i] = i;
   h2d(s_, feats_.p, feats_h_.data(), E);
-  const bool col_sampling = !(std::fabs(cfg.colsample_bytree - 1.0f) < 1e-6f);
+  const bool bytree_sampling = !(std::fabs(cfg.colsample_bytree - 1.0f) < 1e-6f);
+  const bool col_sampling = bytree_sampling || level_sampling();
   const bool uniform = !(std::fabs(cfg.subsample - 1.0f) < 1e-6f);
 
 
@@ -889,14 +923,20 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
         ts.leaf_map.clear();
         ts.free_slots.clear();
         for (uint32_t s = SPT_; s-- > 0;) ts.free_slots.push_back(s);
-        // Column sampling per tree (column.rs:266-283): the k columns with
-        // the smallest sampling keys of (round, output).
+        // Column sampling per tree (column.rs:180-186): the k columns with
+        // the smallest sampling keys of (round, output); per level
+        // (column.rs:199-224) a subset of those for each depth.
         if (col_sampling) {
-          const uint32_t kk = colsample_count(E, cfg.colsample_bytree);
-          feats_h_ = smallest_keys(E, sample_key_base(cfg.seed, kStreamCols, round, (uint32_t)ts.output), kk);
-          h2d(s_, feats_.p, feats_h_.data(), feats_h_.size());
-          // The split kernel's tie rule depends on the number of candidate features.
-          sp_.rightmost = (opt_.ext.n_threads != 1 && feats_h_.size() >= 16) ? 1 : 0;
+          if (bytree_sampling) {
+            const uint32_t kk = colsample_count(E, cfg.colsample_bytree);
+            tree_feats_h_ = smallest_keys(E, sample_key_base(cfg.seed, kStreamCols, round, (uint32_t)ts.output), kk);
+          } else {
+            tree_feats_h_.resize(E);
+            for (uint32_t i = 0; i < E; i++) tree_feats_h_[i] = i;
+          }
+          cur_round_ = round;
+          cur_output_ = ts.output;
+          set_level_features(0);
         } else {
           sp_.rightmost = (opt_.ext.n_threads != 1 && E >= 16) ? 1 : 0;
         }

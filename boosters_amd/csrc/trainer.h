@@ -84,6 +84,13 @@ class Trainer {
   void root(std::vector<TreeState>& trees);
   void make_leaf(TreeState& ts, const Cand& c);
   void renew_round(std::vector<TreeState>& trees, const float* y, const float* w);
+  // Column sampling by level (host-driven schedules): the tree's features and
+  // the (round, output) whose level subsets set_level_features draws.
+  bool level_sampling() const;
+  void set_level_features(uint32_t depth);
+  std::vector<uint32_t> tree_feats_h_;
+  uint32_t cur_round_ = 0;
+  int cur_output_ = 0;
   void finish_trees(std::vector<TreeState>& trees, bst_model* m);
   // Training metric: weights w (nullable); validation metric: unweighted.
   double eval_metric(const float* preds, const float* y, const float* w, int64_t n);

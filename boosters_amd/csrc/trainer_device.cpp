@@ -109,7 +109,9 @@ bool Trainer::graph_capable() const {
   // Leaf renewal (quantile objectives) reads each leaf's rows in the stable
   // partition order of the host-driven schedule, which the graph's fused
   // partition does not keep.
-  return sched_ == BST_SCHED_GRAPH && depth_ok && !needs_leaf_renew(c.objective);
+  // Column sampling by level draws a feature subset per depth on the host.
+  const bool bylevel = !(std::fabs(c.colsample_bylevel - 1.0f) < 1e-6f);
+  return sched_ == BST_SCHED_GRAPH && depth_ok && !needs_leaf_renew(c.objective) && !bylevel;
 }
 
 bool Trainer::device_schedule_ok() const { return graph_capable() && T_ == opt_.K; }

@@ -847,7 +847,7 @@ struct CandLess {  // BinaryHeap max by (gain, partition node id)
 // with the smallest (key, index)", key = a 31-bit splitmix64 hash of (seed,
 // stream, round, output, candidate index): the same distributions (uniform
 // k-subsets) with draws that parallel hardware reproduces exactly.
-constexpr uint32_t kStreamRows = 1, kStreamGoss = 2, kStreamCols = 3;
+constexpr uint32_t kStreamRows = 1, kStreamGoss = 2, kStreamCols = 3, kStreamLevel = 4;
 
 inline uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
@@ -953,7 +953,7 @@ struct Grower {
   std::vector<float> last_leaf;           // by partition node id (lr applied)
   std::vector<std::pair<uint32_t, uint32_t>> leaf_map;  // (tree node, partition node) per leaf
   int64_t hist_rows = 0;
-  std::vector<uint32_t> feats;
+  std::vector<uint32_t> feats, tree_feats;
   uint32_t round = 0, output = 0;  // identify the tree for column sampling
 
   Grower(const ora_binned& b, const bst_gbdt_config& cfg, int nt)
@@ -1044,6 +1044,7 @@ struct Grower {
       for (size_t k = 0; k < hist[large].size(); k++) hist[large][k] -= hist[small][k];
       auto [sg, sh] = sums(small);
       double lg_ = cd.g - sg, lh_ = cd.h - sh;
+      set_level(cd.depth + 1);
       ora_split ss = split_of(small, sg, sh, scount);
       ora_split ls = split_of(large, lg_, lh_, gcount);
       Cand cl, cr;
@@ -1133,6 +1134,22 @@ struct Grower {
     float rate = c.colsample_bytree;
     if (!(std::fabs(rate - 1.0f) < 1e-6f))
       feats = smallest_keys(feats, key_base(c.seed, kStreamCols, round, output), colsample_k(F, rate));
+    tree_feats = feats;
+    set_level(0);
+  }
+
+  // ColSampler::sample_level (column.rs:199-224), called for the root (depth
+  // 0, grower.rs:279-280) and for each expansion's children
+  // (grower.rs:443-444): ceil(|tree features| x colsample_bylevel) of the
+  // tree's features, the smallest sampling keys of (round, output, depth).
+  void set_level(uint32_t depth) {
+    float rate = c.colsample_bylevel;
+    if (std::fabs(rate - 1.0f) < 1e-6f) {
+      feats = tree_feats;
+      return;
+    }
+    feats = smallest_keys(tree_feats, key_base(c.seed, kStreamLevel, round, output | (depth << 12)),
+                          colsample_k((uint32_t)tree_feats.size(), rate));
   }
 
   // update_predictions_from_last_tree, grower.rs:208-219.
@@ -1999,6 +2016,20 @@ int ora_sample_cols(uint32_t n_feat, const bst_gbdt_config* cfg, uint32_t round,
                           colsample_k(n_feat, cfg->colsample_bytree));
   std::copy(feats.begin(), feats.end(), out);
   *k = (int32_t)feats.size();
+  return BST_OK;
+}
+
+int ora_sample_level(const uint32_t* tree_feats, uint32_t n_tree_feats, const bst_gbdt_config* cfg,
+                     uint32_t round, uint32_t output, uint32_t depth, uint32_t* out, int32_t* k) {
+  if (!tree_feats || !cfg || !out || !k) return fail(BST_E_INVALID, "null");
+  std::vector<uint32_t> t(tree_feats, tree_feats + n_tree_feats), f;
+  if (std::fabs(cfg->colsample_bylevel - 1.0f) < 1e-6f)
+    f = t;
+  else
+    f = smallest_keys(t, key_base(cfg->seed, kStreamLevel, round, output | (depth << 12)),
+                      colsample_k(n_tree_feats, cfg->colsample_bylevel));
+  std::copy(f.begin(), f.end(), out);
+  *k = (int32_t)f.size();
   return BST_OK;
 }
 

@@ -131,6 +131,10 @@ int ora_sample_rows(const float* g, const float* h, int64_t n, const bst_gbdt_co
 int ora_sample_cols(uint32_t n_feat, const bst_gbdt_config* cfg, uint32_t round, uint32_t output,
                     uint32_t* out, int32_t* k);
 
+/* The level subset of a tree's features at `depth` (column.rs:199-224). */
+int ora_sample_level(const uint32_t* tree_feats, uint32_t n_tree_feats, const bst_gbdt_config* cfg,
+                     uint32_t round, uint32_t output, uint32_t depth, uint32_t* out, int32_t* k);
+
 /* Metrics over transformed predictions [K][n]: metrics/{classification,regression}.rs. */
 double ora_metric(int32_t metric, const float* preds, int32_t n_outputs, const float* targets,
                   const float* weights, int64_t n);

@@ -111,6 +111,8 @@ def _load(path):
         "ora_metric_cfg": (C.c_double, [P(A.GBDTConfigC), C.c_int32, P(C.c_float), C.c_int32,
                                         P(C.c_float), P(C.c_float), C.c_int64]),
         "ora_default_metric": (C.c_int, [C.c_int32]),
+        "ora_sample_level": (C.c_int, [P(C.c_uint32), C.c_uint32, P(A.GBDTConfigC), C.c_uint32,
+                                       C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -266,6 +268,15 @@ def metric_cfg(cfg, metric_id, preds, targets, weights=None):
     w = None if weights is None else np.ascontiguousarray(weights, np.float32)
     return lib().ora_metric_cfg(C.byref(cfg), metric_id, A.ptr(preds), K,
                                 A.ptr(np.ascontiguousarray(targets, np.float32)), A.ptr(w), n)
+
+
+def sample_level(tree_feats, cfg, round_, output, depth):
+    t = np.ascontiguousarray(tree_feats, np.uint32)
+    out = np.zeros(max(len(t), 1), np.uint32)
+    k = C.c_int32()
+    _check(lib().ora_sample_level(A.ptr(t), len(t), C.byref(cfg), round_, output, depth, A.ptr(out),
+                                  C.byref(k)))
+    return out[: k.value]
 
 
 def default_metric(objective):

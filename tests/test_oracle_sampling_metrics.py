@@ -297,3 +297,39 @@ def test_partition_routing_differs_from_threshold_at_rounded_up_bounds(oracle):
     assert np.all(part[50:] != trav[50:])          # b: right by bins, left by threshold
     assert np.all(part[50:] > part[:50])
     assert oracle.metric(A.METRIC_RMSE, part[None], y) == m.metric_history()[-1]
+
+
+# ---- column sampling by level (column.rs:199-224) -----------------------------
+
+@pytest.mark.parametrize("n_tree, rate", [(100, 0.5), (37, 0.3), (5, 0.01), (64, 0.999), (1, 0.5)])
+def test_level_sample_count_and_subset(oracle, n_tree, rate):
+    from tests.conftest import make_cfg
+    cfg = make_cfg(colsample_bylevel=rate, seed=7)
+    tree = np.sort(np.random.default_rng(n_tree).choice(500, n_tree, replace=False)).astype(np.uint32)
+    want_k = min(max(int(np.ceil(np.float32(n_tree) * np.float32(rate))), 1), n_tree)
+    seen = set()
+    for depth in range(6):
+        lv = oracle.sample_level(tree, cfg, 3, 1, depth)
+        assert len(lv) == want_k
+        assert np.all(np.diff(lv.astype(np.int64)) > 0)  # sorted, distinct
+        assert set(lv.tolist()) <= set(tree.tolist())
+        seen.add(tuple(lv.tolist()))
+    if 1 < want_k < n_tree:
+        assert len(seen) > 1  # a fresh subset per depth
+    # rate 1 keeps the tree's features
+    full = oracle.sample_level(tree, make_cfg(colsample_bylevel=1.0), 3, 1, 2)
+    np.testing.assert_array_equal(full, tree)
+
+
+def test_level_sample_is_uniform(oracle):
+    """Each tree feature is drawn with probability k / n over many depths."""
+    from tests.conftest import make_cfg
+    cfg = make_cfg(colsample_bylevel=0.25, seed=11)
+    tree = np.arange(40, dtype=np.uint32)
+    hits = np.zeros(40)
+    trials = 4000
+    for t in range(trials):
+        lv = oracle.sample_level(tree, cfg, t // 8, 0, t % 8)
+        hits[lv] += 1
+    p = hits / trials
+    assert np.all(np.abs(p - 0.25) < 0.035)
