@@ -691,6 +691,18 @@ def main():
     # Held-out validation loss of the timed model: n/4 new rows (another seed)
     # labelled by the training data's model.
     Xv, yv, _ = gen_binary_nan(n // 4, F, seed=1, w=w_true)
+
+    # The same training with that validation set (per-round validation
+    # prediction and metric inside the round's graph), from host targets.
+    A.check(L.bst_ctx_sync(ctx))
+    hv = C.c_void_p()
+    t0 = time.perf_counter()
+    A.check(L.bst_train(ctx, ds, A.ptr(y), n, None, A.ptr(Xv), A.ptr(yv), len(yv),
+                        C.byref(cfg_c2(args.steps, depth)), C.byref(ext(A.SCHED_GRAPH)), C.byref(hv)))
+    A.check(L.bst_ctx_sync(ctx))
+    valid_ms_tree = (time.perf_counter() - t0) * 1e3 / args.steps
+    L.bst_model_destroy(hv)
+    log(f"graph with a {len(yv)}-row validation set: {valid_ms_tree:.3f} ms/tree")
     pv = np.zeros(len(yv), np.float32)
     A.check(L.bst_predict(ctx, h_model, A.ptr(Xv), len(yv), F, 0, A.ptr(pv), None))
     p = np.clip(pv.astype(np.float64), 1e-15, 1 - 1e-15)
@@ -736,6 +748,8 @@ def main():
             "graph_speedup_vs_plain": None if plain_ms_tree is None else round(plain_ms_tree / ms_tree, 2),
             "binning_ms": round(bin_ms, 1),
             "val_logloss": round(val_logloss, 6),
+            "with_validation_ms_per_tree": round(valid_ms_tree, 4),
+            "validation_rows": len(yv),
             "kernel_ms_per_tree": {
                 "hist": round(hms / args.steps, 4),
                 "split": round(split[1] / args.steps, 4),

@@ -876,11 +876,20 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
   size_t best_round = 0, cur_round = 0, best_n_trees = 0;
   std::vector<uint32_t> sampled;
   // Device rounds are pipelined: round r+1 is queued before the host reads
-  // round r, so the host's tree assembly overlaps the device (without a
-  // validation set, whose updates the host issues per round).  The training
-  // metric then runs in the queue right after its round.
+  // round r, so the host's tree assembly overlaps the device.  The round's
+  // graph also adds each new tree to the validation predictions (a traversal
+  // of the validation rows on the device), and the round's metric (on the
+  // validation set when there is one, eval.rs:218-270) runs in the queue
+  // right after it: one host wait per round, for the oldest round in flight.
   // Profiled runs read each round's kernel events right after it, unpipelined.
-  const bool pipelined = on_device && !(vX && n_valid > 0) && !ctx_->profile;
+  valid_X_ = vX && vy && n_valid > 0 ? vX : nullptr;
+  valid_y_ = valid_X_ ? vy : nullptr;
+  valid_n_ = valid_X_ ? n_valid : 0;
+  // With early stopping, a round queued past the stop is discarded; its
+  // prediction updates would reach the kept training predictions, so a train
+  // that keeps them (and may stop early) runs unpipelined.
+  const bool pipelined =
+      on_device && !ctx_->profile && !(ctx_->keep_preds && cfg.early_stopping_rounds > 0);
   const bool queue_metric = pipelined && metric_kind() != BST_METRIC_NONE;
   if (pipelined && cfg.n_trees > 0) device_launch(y, w, queue_metric);
   for (uint32_t round = 0; round < cfg.n_trees; round++) {
@@ -896,11 +905,6 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
       }
       for (int t = 0; t < T_; t++) {
         bst_model one = single_tree(tbs[t], t);
-        if (vX && n_valid > 0) {
-          one.upload(s_);
-          valid_update(one);
-          BST_HIP(hipStreamSynchronize(s_));
-        }
         append_tree(m, one, t);
       }
     } else {
@@ -994,10 +998,10 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
     }
     // evaluate_round (eval.rs:218-260) and early stopping (callback.rs:78-118).
     double value;
-    if (vX && vy && n_valid > 0)
-      value = eval_metric(vpreds_.p, vy, nullptr, n_valid);
-    else if (pipelined)
+    if (pipelined)
       value = queued_value;
+    else if (vX && vy && n_valid > 0)
+      value = eval_metric(vpreds_.p, vy, nullptr, n_valid);
     else
       value = eval_metric(preds_.p, y, w, n_);
     metric_history.push_back(value);

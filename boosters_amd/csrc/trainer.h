@@ -89,6 +89,10 @@ class Trainer {
   bool level_sampling() const;
   void set_level_features(uint32_t depth);
   std::vector<uint32_t> tree_feats_h_;
+  // Validation set of the current train (device pointers; null without one).
+  const float* valid_X_ = nullptr;
+  const float* valid_y_ = nullptr;
+  int64_t valid_n_ = 0;
   uint32_t cur_round_ = 0;
   int cur_output_ = 0;
   void finish_trees(std::vector<TreeState>& trees, bst_model* m);

@@ -383,6 +383,13 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
     launch_leaf_update(s_, B.leaf_segs, g_lsegs, leaf_val_.p, idx_.p, 0, preds_.p,
                        tree_output_.p, n_, cnt + C_LSEGS, idx_tmp_.p);
   }
+  if (valid_n_ > 0) {
+    // Validation predictions += the round's trees (trainer.rs:341-352):
+    // the same traversal of raw features, over the validation rows.
+    launch_tree_update_raw(s_, (int)P.T, P.MNT, B.tr_split, B.tr_thr, B.tr_left, B.tr_right,
+                           B.tr_dleft, B.tr_isleaf, B.tr_stype, B.tr_cats, B.tr_leaf, valid_X_,
+                           valid_n_, tree_output_.p, vpreds_.p);
+  }
 }
 
 // Launch (or capture and launch) one round, wait, and return the trees.
@@ -418,7 +425,12 @@ void Trainer::device_launch(const float* y, const float* w, bool metric) {
     device_enqueue_round(y, w, sampled);
   }
   const int slot = (int)(S.launched & 1u);
-  if (metric) metric_launch(preds_.p, y, w, n_, S.h_metric[slot]);
+  if (metric) {
+    if (valid_n_ > 0)  // the validation metric is unweighted
+      metric_launch(vpreds_.p, valid_y_, nullptr, valid_n_, S.h_metric[slot]);
+    else
+      metric_launch(preds_.p, y, w, n_, S.h_metric[slot]);
+  }
   d2h(s_, S.h_out[slot], S.out.p, S.out_bytes);
   BST_HIP(hipEventRecord(S.done[slot], s_));
   S.launched++;
