@@ -314,6 +314,11 @@ def _declare(lib):
             [vp, vp, P(C.c_uint32), C.c_int64, C.c_int32, C.c_uint32, C.c_int32, P(C.c_uint32),
              C.c_int32, P(C.c_int64)],
         ),
+        "bst_kernel_partition_fused": (
+            C.c_int,
+            [vp, vp, P(C.c_uint32), C.c_int64, C.c_int32, C.c_uint32, C.c_int32, P(C.c_uint32),
+             C.c_int32, P(C.c_int64)],
+        ),
         "bst_kernel_find_split": (
             C.c_int,
             [vp, vp, P(C.c_double), C.c_double, C.c_double, C.c_uint32, P(GBDTConfigC), C.c_int32,
@@ -345,7 +350,7 @@ def exported_symbols():
         "bst_model_load_json", "bst_buffer_free", "bst_predict", "bst_predict_device", "bst_predict_device_noalloc", "bst_server_create",
         "bst_server_submit", "bst_server_wait", "bst_server_destroy", "bst_kernel_gradients", "bst_kernel_gradients_cfg",
         "bst_kernel_lambdarank",
-        "bst_kernel_histogram", "bst_dataset_hist_layout", "bst_kernel_partition",
+        "bst_kernel_histogram", "bst_dataset_hist_layout", "bst_kernel_partition", "bst_kernel_partition_fused",
         "bst_kernel_find_split",
     ]
 

@@ -1287,6 +1287,68 @@ int bst_kernel_partition(bst_ctx* ctx, const bst_dataset* ds, uint32_t* rows, in
   });
 }
 
+int bst_kernel_partition_fused(bst_ctx* ctx, const bst_dataset* ds, uint32_t* rows, int64_t n,
+                               int32_t e, uint32_t bin, int32_t default_left,
+                               const uint32_t* cats, int32_t n_words, int64_t* left_count) {
+  return guard([&] {
+    need(ctx && ds && rows, BST_E_INVALID, "null argument");
+    need(e >= 0 && e < ds->d.E, BST_E_INVALID, "bad feature");
+    need(n >= 0 && n < (int64_t)0x7FFFFFFF, BST_E_SHAPE, "row count");
+    for (int64_t i = 0; i < n; i++) need(rows[i] < ds->d.n, BST_E_SHAPE, "row id out of range");
+    set_device(ctx);
+    hipStream_t s = ctx->stream;
+    const int64_t m = std::max<int64_t>(n, 1);
+    DevBuf<uint32_t> idx0(m), idx1(m), cursor(2), dyn(1), tout(1);
+    DevBuf<float2> ogh0(m), ogh1(m);
+    DevBuf<DevSplit> split(1);
+    DevBuf<float> preds(1);
+    h2d(s, idx0.p, rows, n);
+    BST_HIP(hipMemsetAsync(ogh0.p, 0, ogh0.bytes(), s));
+    BST_HIP(hipMemsetAsync(cursor.p, 0, cursor.bytes(), s));
+    BST_HIP(hipMemsetAsync(tout.p, 0, tout.bytes(), s));
+    DevSplit sp;
+    std::memset(&sp, 0, sizeof(sp));
+    sp.gain = 1.0f;
+    sp.feature = (uint32_t)e;
+    sp.bin = bin;
+    sp.flags = (default_left ? 1u : 0u) | (cats ? 2u : 0u);
+    for (int w = 0; cats && w < n_words && w < kMaxCatWords; w++) sp.cats[w] = cats[w];
+    h2d(s, split.p, &sp, 1);
+    std::vector<PartTile> tiles;
+    for (int64_t b = 0; b < n || tiles.empty(); b += kFusedTileRows)
+      tiles.push_back({0u, (uint32_t)b,
+                       (uint32_t)std::max<int64_t>(0, std::min<int64_t>(kFusedTileRows, n - b)), 0u});
+    const FeatMeta& fm = ds->fmeta[(size_t)e];
+    PartItem it;
+    std::memset(&it, 0, sizeof(it));
+    it.node = 0;
+    it.begin = 0;
+    it.count = (uint32_t)n;
+    it.tile0 = 0;
+    it.n_tiles = (uint32_t)tiles.size();
+    it.right_node = 1;
+    it.feature = (uint32_t)e;
+    it.split_bin = bin;
+    it.split_flags = sp.flags;
+    it.miss_bin = fm.has_missing ? fm.default_bin : 0xFFFFFFFFu;
+    DevBuf<PartItem> ditem(1);
+    DevBuf<PartTile> dtiles(tiles.size());
+    const uint32_t nt = (uint32_t)tiles.size();
+    h2d(s, ditem.p, &it, 1);
+    h2d(s, dtiles.p, tiles.data(), tiles.size());
+    h2d(s, dyn.p, &nt, 1);
+    launch_partition_fused(s, ds->d, ditem.p, dtiles.p, (int)nt, split.p, idx0.p, ogh0.p, idx1.p,
+                           ogh1.p, cursor.p, dyn.p, tout.p, preds.p, 1, 1u << 30);
+    BST_HIP(hipGetLastError());
+    uint32_t cur[2] = {0, 0};
+    d2h(s, cur, cursor.p, 2);
+    d2h(s, rows, idx1.p, n);
+    BST_HIP(hipStreamSynchronize(s));
+    if (left_count) *left_count = cur[0];
+    return BST_OK;
+  });
+}
+
 // ---- serving (declared here, implemented in server.cpp) -------------------
 
 }  // extern "C"

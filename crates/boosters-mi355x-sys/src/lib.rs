@@ -538,6 +538,18 @@ extern "C" {
         n_cat_words: i32,
         left_count: *mut i64,
     ) -> c_int;
+    pub fn bst_kernel_partition_fused(
+        ctx: *mut bst_ctx,
+        ds: *const bst_dataset,
+        rows: *mut u32,
+        n_rows: i64,
+        eff_feature: i32,
+        split_bin: u32,
+        default_left: i32,
+        left_cats: *const u32,
+        n_cat_words: i32,
+        left_count: *mut i64,
+    ) -> c_int;
     pub fn bst_kernel_find_split(
         ctx: *mut bst_ctx,
         ds: *const bst_dataset,

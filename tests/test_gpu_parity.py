@@ -212,6 +212,51 @@ def test_partition_bit_exact(ctx, oracle):
         np.testing.assert_array_equal(r2, ref_rows)
 
 
+def test_fused_partition_row_sets(ctx, oracle):
+    """The graph schedule's one-pass partition (k_part_fused) sends exactly
+    the oracle's rows to each side (partition.rs:296-382): the left count is
+    equal, and each side holds the same rows, in tile order instead of row
+    order.  Multi-tile nodes (tiles of 4096 rows), numeric splits with NaN in
+    either direction, categorical splits."""
+    X, y, is_cat = multiclass_cat_data(20_000, 4, 2, 3, card=9, seed=17)
+    Xn, _ = binary_nan_data(20_000, 3, seed=18, nan_p=0.3)
+    X = np.concatenate([X, Xn])
+    is_cat = np.concatenate([is_cat, np.zeros(3, np.uint8)])
+    b = _binned(ctx, X, is_cat)
+    o = oracle.Binned(X, is_cat)
+    lay = b.layout()
+    rng = np.random.default_rng(19)
+    sizes = [1, 5, 4095, 4096, 4097, 12_288, 20_000]
+    for trial in range(28):
+        e = int(rng.integers(0, lay["n_eff"]))
+        n_rows = sizes[trial % len(sizes)] if trial < len(sizes) else int(rng.integers(1, 20_000))
+        rows = np.sort(rng.choice(20_000, n_rows, replace=False)).astype(np.uint32)
+        sp = A.SplitInfoC()
+        sp.feature = e
+        sp.gain = 1.0
+        sp.default_left = int(rng.integers(0, 2))
+        info = b.feature_info(int(lay["eff_to_orig"][e]))
+        sp.is_categorical = int(info["categorical"])
+        if sp.is_categorical:
+            cats = rng.integers(0, 2, 8 * 32).astype(bool)
+            for k in range(256):
+                if cats[k]:
+                    sp.left_cats[k >> 5] |= 1 << (k & 31)
+            cw = np.array(list(sp.left_cats), np.uint32)
+        else:
+            sp.bin = int(rng.integers(0, info["n_bins"]))
+            cw = None
+        ref_rows, ref_left = o.partition(rows, sp)
+        r2 = rows.copy()
+        lc = C.c_int64()
+        A.check(A.lib().bst_kernel_partition_fused(ctx.h, b.h, A.ptr(r2), len(r2), e, sp.bin,
+                                                   sp.default_left, A.ptr(cw),
+                                                   8 if cw is not None else 0, C.byref(lc)))
+        assert lc.value == ref_left, (trial, e, n_rows)
+        np.testing.assert_array_equal(np.sort(r2[:ref_left]), ref_rows[:ref_left])
+        np.testing.assert_array_equal(np.sort(r2[ref_left:]), ref_rows[ref_left:])
+
+
 def _forests_equal(gm, om_info, om_arrays, leaf_rtol=1e-6):
     ga = gm.export()
     assert gm.info.n_trees == om_info.n_trees
