@@ -458,10 +458,14 @@ WORKLOADS = {
 }
 
 
-def run_workload(args, L, ctx):
-    desc, gen, cfg_kw, ext_kw, default_steps = WORKLOADS[args.workload]
-    steps = args.steps if args.steps_given else default_steps
-    log(f"{args.workload}: generating {desc}")
+def run_workload(args, L, ctx, name=None, extra=False):
+    """Train one workload of WORKLOADS on the graph schedule and return its
+    bench line.  extra: a side measurement of the default run (its default
+    tree count, no CPU baseline)."""
+    name = name or args.workload
+    desc, gen, cfg_kw, ext_kw, default_steps = WORKLOADS[name]
+    steps = args.steps if (args.steps_given and not extra) else default_steps
+    log(f"{name}: generating {desc}")
     X, y, is_cat, sizes = gen()
     F, n = X.shape
     t0 = time.perf_counter()
@@ -498,6 +502,12 @@ def run_workload(args, L, ctx):
     n_trees = rounds * K
     ms_tree = wall_ms / n_trees
     log(f"graph: {ms_tree:.3f} ms/tree ({n_trees} trees in {wall_ms:.1f} ms)")
+    if args.graph_only and not extra:
+        L.bst_model_destroy(h_model)
+        L.bst_dataset_destroy(ds)
+        return {"metric": f"train ms/tree ({desc}, graph schedule only)",
+                "value": round(ms_tree, 4), "unit": "ms/tree", "steps": rounds,
+                "hist_tuning": tune}
     A.check(L.bst_ctx_set_profiling(ctx, 1))
     h_prof, _ = train_timed(ctx, ds, y_dev, n, cfg_of(rounds), ext_of())
     hist = kernel_stats(ctx, 0)
@@ -539,7 +549,7 @@ def run_workload(args, L, ctx):
             "avg_launch_us": round(hms / launches * 1e3, 2) if launches else None,
         },
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not extra:
         O, threads, cinfo = cpu_setup()
         O.set_threads(threads)
         m = min(n, args.cpu_rows)
@@ -569,7 +579,9 @@ def run_workload(args, L, ctx):
             **cinfo,
         }
     L.bst_model_destroy(h_model)
-    print(json.dumps(result), flush=True)
+    L.bst_dataset_destroy(ds)
+    y_dev.free()
+    return result
 
 
 def main():
@@ -600,6 +612,9 @@ def main():
     ap.add_argument("--predict-only", action="store_true",
                     help="only the device-batch prediction of the 500-tree depth-8 model (the run "
                          "a kernel trace of the traversal kernel is taken on)")
+    ap.add_argument("--no-leafwise", action="store_true",
+                    help="skip the leaf-wise workloads (softmax K=10 255 leaves, LambdaRank 127 "
+                         "leaves) measured beside the headline line")
     ap.add_argument("--graph-only", action="store_true",
                     help="only the warmup and timed graph-schedule runs (no per-kernel event run, "
                          "no batched or plain schedule): the run a kernel trace of the graph "
@@ -628,7 +643,7 @@ def main():
         print(json.dumps({"copy_bytes": nbytes, "launches": 5}), flush=True)
         return
     if args.workload != "c2":
-        run_workload(args, L, ctx)
+        print(json.dumps(run_workload(args, L, ctx)), flush=True)
         return
     if args.predict_only:
         info, arrays = random_forest(500, 8, 100, seed=0xB002)
@@ -793,6 +808,20 @@ def main():
             **cinfo,
         }
     L.bst_model_destroy(h_model)
+    if not args.no_leafwise:
+        # The leaf-wise workloads, measured the same way beside the headline
+        # line (their own data, graph schedule, per-kernel event run).
+        L.bst_dataset_destroy(ds)
+        ds = None
+        y_dev.free()
+        lw = {}
+        for w in ("c4", "c3"):
+            r = run_workload(args, L, ctx, name=w, extra=True)
+            lw[w] = {"workload": r["config"]["workload"], "ms_per_tree": r["value"],
+                     "trees": r["steps"] * r["config"]["trees_per_step"],
+                     "kernel_ms_per_tree": r["config"]["kernel_ms_per_tree"],
+                     "hist_tuning": r["config"]["hist_tuning"], "hist_frac": r["roofline"]["frac"]}
+        result["config"]["leafwise"] = lw
     print(json.dumps(result), flush=True)
 
 
