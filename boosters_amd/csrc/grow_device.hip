@@ -62,19 +62,6 @@ __device__ __forceinline__ uint32_t ord_gain(float v) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__device__ unsigned long long block_max_u64(unsigned long long v, unsigned long long* s_w) {
-  for (int m = 32; m >= 1; m >>= 1) {
-    const unsigned long long o = __shfl_xor(v, m, 64);
-    v = o > v ? o : v;
-  }
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
-  __syncthreads();
-  unsigned long long r = 0ull;
-  for (int i = 0; i < kCtlThreads / 64; i++) r = s_w[i] > r ? s_w[i] : r;
-  __syncthreads();
-  return r;
-}
-
 __device__ __forceinline__ bool expandable(const DevSplit& s, double min_gain) {
   return s.gain > -INFINITY && (double)s.gain > min_gain;
 }
@@ -453,12 +440,116 @@ __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int a
 // packed key holds the best gain and the winning feature under the tie
 // rule), then choose the nodes to expand this step, decide leaf / split,
 // allocate ids, write the tree nodes and the partition work list.
+// Leaf-wise step: every tree pops its open node of max (gain, partition id)
+// (BinaryHeap order of NodeCandidate) and, when that node splits, yields one
+// item.  Trees are independent, so wave t handles tree t and the T pops run
+// side by side; only the item and tile offsets are a prefix over the trees,
+// taken in tree order, so items, ids and tiles are those of a tree-by-tree
+// walk.  The popped node's state is loaded by the lane that found it, right
+// after the argmax, to keep the dependent global loads per tree few.
+__device__ void plan_leafwise(const CtlParams& P, const DeviceBufs& B, uint32_t* s_ex,
+                              uint32_t* s_nt, uint32_t* s_tfirst, uint32_t* s_tbegin,
+                              uint32_t* s_tcount, uint8_t* s_tsrc, uint32_t* s_items,
+                              uint32_t* s_tiles) {
+  const uint32_t lane = threadIdx.x & 63u, t = threadIdx.x >> 6;
+  const bool tree = t < P.T;
+  const uint64_t ob = (uint64_t)t * P.WO;
+  bool pop = false;
+  uint32_t p = 0, tn = 0, dep = 0, n_part = 0, n_tnodes = 0, count = 0, ntile = 0;
+  DevSplit sp;
+  if (tree) {
+    const TreeCtl c = B.ctl[t];
+    unsigned long long best = 0ull;
+    uint32_t bi = 0;
+    for (uint32_t i = lane; i < c.n_open; i += 64) {
+      const uint32_t q = B.open_node[ob + i];
+      const float g = B.node_split[t * P.MN + q].gain + 0.0f;  // -0 ties with +0
+      const unsigned long long key = ((unsigned long long)ord_gain(g) << 32) | q;
+      if (key > best) {
+        best = key;
+        bi = i;
+      }
+    }
+    unsigned long long wbest = best;
+    for (int m = 32; m >= 1; m >>= 1) {
+      const unsigned long long o = __shfl_xor(wbest, m, 64);
+      wbest = o > wbest ? o : wbest;
+    }
+    // Partition ids are unique in the open set, so one lane holds the max.
+    const unsigned long long hit = __ballot(c.n_open > 0 && best == wbest);
+    const int src = hit ? __ffsll((long long)hit) - 1 : 0;
+    bi = __shfl(bi, src, 64);
+    pop = c.n_leaves < P.max_leaves && c.n_open > 0;
+    if (pop && lane == 0) {
+      p = (uint32_t)(wbest & 0xFFFFFFFFull);
+      const uint32_t last = c.n_open - 1;
+      tn = B.open_tnode[ob + bi];
+      dep = B.open_depth[ob + bi];
+      const uint32_t ln = B.open_node[ob + last], lt = B.open_tnode[ob + last],
+                     ld = B.open_depth[ob + last];
+      sp = B.node_split[t * P.MN + p];
+      count = B.node_count[t * P.MN + p];
+      B.open_node[ob + bi] = ln;
+      B.open_tnode[ob + bi] = lt;
+      B.open_depth[ob + bi] = ld;
+      B.ctl[t].n_open = last;
+      B.ctl[t].n_leaves = c.n_leaves + 1;
+      n_part = c.n_part;
+      n_tnodes = c.n_tnodes;
+      const bool ex = expandable(sp, P.min_gain);
+      ntile = ex ? max(1u, (count + P.part_tile - 1) / P.part_tile) : 0u;
+      s_ex[t] = ex ? 1u : 0u;
+      s_nt[t] = ntile;
+    } else if (lane == 0) {
+      s_ex[t] = 0u;
+      s_nt[t] = 0u;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t e = 0, k = 0;
+    for (uint32_t u = 0; u < P.T; u++) {
+      const uint32_t x = s_ex[u], y = s_nt[u];
+      s_ex[u] = e;
+      s_nt[u] = k;
+      e += x;
+      k += y;
+    }
+    *s_items = e;
+    *s_tiles = k;
+  }
+  __syncthreads();
+  if (tree && pop && lane == 0) {
+    if (ntile == 0) {
+      make_leaf(P, B, t, p, tn);
+    } else {
+      const uint32_t gid = t * P.MN + p;
+      const int out = P.out0 + (int)t;
+      const long long sg = B.node_sum[2 * gid], sh = B.node_sum[2 * gid + 1];
+      const int qg = B.qexp[2 * out], qh = B.qexp[2 * out + 1];
+      const uint32_t begin = B.pbegin[gid], pb = B.pbuf[gid];
+      uint32_t orig, miss;
+      float thr;
+      split_feature_info(B, sp, &orig, &thr, &miss);
+      const uint32_t g = s_ex[t], tile0 = s_nt[t];
+      emit_expand(P, B, t, gid, tn, dep, sp, ldexp((double)sh, -qh), count, begin, pb, n_part,
+                  n_tnodes, 0u, g, tile0, ntile, orig, thr, miss, false, sg, sh, qg, qh);
+      s_tfirst[g] = tile0;
+      s_tbegin[g] = begin;
+      s_tcount[g] = count;
+      s_tsrc[g] = (uint8_t)(pb ? kTileAlt : 0u);
+      B.ctl[t].n_part = n_part + 1;
+      B.ctl[t].n_tnodes = n_tnodes + 2;
+    }
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBufs B,
                                                           int advance) {
   __shared__ uint32_t s_tmp[kCtlThreads / 64 + 1];
   __shared__ uint32_t s_items, s_tiles, s_nch;
-  __shared__ uint32_t s_part, s_tnodes, s_best;
-  __shared__ unsigned long long s_max64[kCtlThreads / 64];
+  __shared__ uint32_t s_part, s_tnodes;
   // Partition tiles of the first kPlanLdsItems items, written by the whole
   // block at the end: first tile, row range of each item.
   __shared__ uint32_t s_tfirst[kPlanLdsItems], s_tbegin[kPlanLdsItems], s_tcount[kPlanLdsItems];
@@ -466,6 +557,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
   __shared__ uint32_t s_tlds_end;
   __shared__ uint32_t s_any[kCtlThreads / 64];
   __shared__ uint32_t s_ptile;  // the step's partition tile
+  __shared__ uint32_t s_lw_ex[kCtlMaxTrees], s_lw_nt[kCtlMaxTrees];
   if (threadIdx.x == 0) {
     s_tlds_end = kNone;
     s_ptile = P.part_tile;
@@ -489,58 +581,26 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
       s_tiles = 0;
     }
     __syncthreads();
-    for (uint32_t t = 0; t < P.T; t++) {
+    if (P.growth == 1)
+      plan_leafwise(P, B, s_lw_ex, s_lw_nt, s_tfirst, s_tbegin, s_tcount, s_tsrc, &s_items,
+                    &s_tiles);
+    // Depth-wise: every tree's open level, tree by tree.
+    for (uint32_t t = 0; t < (P.growth == 1 ? 0u : P.T); t++) {
       const uint64_t ob = (uint64_t)t * P.WO, cb = (uint64_t)t * P.W;
-      if (P.growth == 1) {
-        // Block-parallel argmax of (gain, partition id) over the open set.
-        const uint32_t n_open = B.ctl[t].n_open;
-        // Partition ids are unique in the open set, so the key picks one node.
-        unsigned long long best = 0ull;
-        for (uint32_t i = threadIdx.x; i < n_open; i += blockDim.x) {
-          const uint32_t p = B.open_node[ob + i];
-          const float g = B.node_split[t * P.MN + p].gain + 0.0f;  // -0 ties with +0
-          const unsigned long long key = ((unsigned long long)ord_gain(g) << 32) | p;
-          best = key > best ? key : best;
-        }
-        best = block_max_u64(best, s_max64);
-        for (uint32_t i = threadIdx.x; i < n_open; i += blockDim.x)
-          if (B.open_node[ob + i] == (uint32_t)(best & 0xFFFFFFFFull)) s_best = i;
-        __syncthreads();
-      }
       if (threadIdx.x == 0) {
-        TreeCtl& c = B.ctl[t];
-        uint32_t nch = 0;
-        if (P.growth == 0) {
-          if (c.depth < P.max_depth) nch = c.n_open;
-        } else if (c.n_leaves < P.max_leaves && c.n_open > 0) {
-          // Max (gain, partition id): BinaryHeap order of NodeCandidate; the
-          // block found it (s_best) before this thread-0 section.
-          const uint32_t best = s_best;
-          B.ch_node[cb] = B.open_node[ob + best];
-          B.ch_tnode[cb] = B.open_tnode[ob + best];
-          B.ch_depth[cb] = B.open_depth[ob + best];
-          const uint32_t last = c.n_open - 1;
-          B.open_node[ob + best] = B.open_node[ob + last];
-          B.open_tnode[ob + best] = B.open_tnode[ob + last];
-          B.open_depth[ob + best] = B.open_depth[ob + last];
-          c.n_open = last;
-          c.n_leaves += 1;
-          nch = 1;
-        }
-        s_nch = nch;
+        const TreeCtl& c = B.ctl[t];
+        s_nch = c.depth < P.max_depth ? c.n_open : 0u;
         s_part = c.n_part;
         s_tnodes = c.n_tnodes;
       }
       __syncthreads();
       const uint32_t nch = s_nch;
-      if (P.growth == 0) {
-        for (uint32_t i = threadIdx.x; i < nch; i += blockDim.x) {
-          B.ch_node[cb + i] = B.open_node[ob + i];
-          B.ch_tnode[cb + i] = B.open_tnode[ob + i];
-          B.ch_depth[cb + i] = B.open_depth[ob + i];
-        }
-        __syncthreads();
+      for (uint32_t i = threadIdx.x; i < nch; i += blockDim.x) {
+        B.ch_node[cb + i] = B.open_node[ob + i];
+        B.ch_tnode[cb + i] = B.open_tnode[ob + i];
+        B.ch_depth[cb + i] = B.open_depth[ob + i];
       }
+      __syncthreads();
       uint32_t carry_e = 0, carry_t = 0;
       for (uint32_t base = 0; base < nch; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;

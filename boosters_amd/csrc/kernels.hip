@@ -389,21 +389,43 @@ void launch_gradients(hipStream_t s, int objective, const ObjParams& op, const f
 // LambdaRank over ragged query groups (no counterpart in the reference; the
 // published LambdaMART formulation with sigma = 1 and pair truncation):
 // one block per group.  Documents are ranked by score descending, ties by
-// index (bitonic sort of (score, index) keys in LDS); a pair of positions
-// (a, b) counts when the labels differ and min(a, b) < kRankTrunc;
+// index; a pair of positions (a, b) counts when the labels differ and
+// min(a, b) < kRankTrunc;
 //   rho = 1 / (1 + exp(s_hi - s_lo))                         (f32)
 //   dNDCG = (2^l_hi - 2^l_lo) |disc[a] - disc[b]| / maxDCG    (f64)
 // the higher-labelled document gets -rho dNDCG, the other +rho dNDCG, both
 // rho (1 - rho) dNDCG.  Each document sums its pairs in f64 in position
-// order (one thread per position), so the result does not depend on the
-// launch shape; disc[r] = 1 / log2(2 + r) is a host-computed table.
+// order, so the result does not depend on the launch shape; disc[r] =
+// 1 / log2(2 + r) is a host-computed table.
+//
+// Every counted pair has a top position (< T = min(kRankTrunc, m)), so the
+// pair terms of a group form a T x m matrix, walked in tiles of kRankChunk
+// positions: all threads evaluate a tile's terms (from the top position's
+// side) into LDS, then one thread per top position adds its row to its
+// running sums and one thread per lower position (>= T) adds its column —
+// the negated term, the same magnitude seen from the other side — and is
+// done.  Only the ordered f64 additions are serial; the exp, divisions and
+// table reads are spread over the block.  Keys (order-preserving score bits
+// above the index) are sorted by rank counting for m <= kRankCountSortMax,
+// by a bitonic network above.
 constexpr int kRankTrunc = 30;
 constexpr int kRankMaxGroup = 4096;
 constexpr int kRankThreads = 256;
+constexpr int kRankChunk = 32;
+constexpr int kRankStride = kRankChunk + 1;  // row stride of the term tile: rows on distinct banks
+constexpr int kRankCountSortMax = 2 * kRankThreads;
 
 __device__ __forceinline__ uint32_t ord_f32_asc(float v) {
   const uint32_t u = __float_as_uint(v);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+// Score and index held by a sort key ((~ord(score)) << 32 | index).
+__device__ __forceinline__ float rank_key_score(unsigned long long key) {
+  const uint32_t o = ~(uint32_t)(key >> 32);
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+__device__ __forceinline__ uint32_t rank_key_index(unsigned long long key) {
+  return (uint32_t)(key & 0xFFFFFFFFull);
 }
 
 __global__ __launch_bounds__(kRankThreads) void k_lambdarank(
@@ -411,90 +433,139 @@ __global__ __launch_bounds__(kRankThreads) void k_lambdarank(
     const int64_t* __restrict__ off, int64_t n_groups, const double* __restrict__ disc,
     float* __restrict__ g, float* __restrict__ h, uint32_t* __restrict__ absmax) {
   __shared__ unsigned long long s_key[kRankMaxGroup];
-  __shared__ float s_s[kRankMaxGroup];
   __shared__ uint8_t s_l[kRankMaxGroup];
+  __shared__ double s_tl[kRankTrunc * kRankStride];
+  __shared__ double s_th[kRankTrunc * kRankStride];
   __shared__ uint32_t s_cnt[32];
   __shared__ double s_inv;
   __shared__ uint32_t s_max[2][kRankThreads / WAVE];
+  const int tid = threadIdx.x;
   uint32_t mg = 0u, mh = 0u;
+  auto emit = [&](uint32_t i, double ga, double ha, int64_t b0) {
+    const double wi = w ? (double)w[b0 + i] : 1.0;
+    const float gv = (float)(ga * wi), hv = (float)(ha * wi);
+    g[b0 + i] = gv;
+    h[b0 + i] = hv;
+    mg = max(mg, abs_bits(gv));
+    mh = max(mh, abs_bits(hv));
+  };
   for (int64_t q = blockIdx.x; q < n_groups; q += gridDim.x) {
     const int64_t b0 = off[q];
     const int m = (int)(off[q + 1] - b0);
+    const int T = m < kRankTrunc ? m : kRankTrunc;
+    const bool count_sort = m <= kRankCountSortMax;
     int P = 1;
     while (P < m) P <<= 1;
-    if (threadIdx.x < 32) s_cnt[threadIdx.x] = 0u;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    if (tid < 32) s_cnt[tid] = 0u;
+    const int n_keys = count_sort ? m : P;
+    for (int i = tid; i < n_keys; i += kRankThreads) {
       unsigned long long key = ~0ull;
       if (i < m) {
         const float sv = preds[b0 + i] + 0.0f;  // -0 ranks with +0
         int l = (int)y[b0 + i];
         l = l < 0 ? 0 : (l > 31 ? 31 : l);
-        s_s[i] = sv;
         s_l[i] = (uint8_t)l;
         key = ((unsigned long long)(~ord_f32_asc(sv)) << 32) | (uint32_t)i;
       }
       s_key[i] = key;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < m; i += blockDim.x) atomicAdd(&s_cnt[s_l[i]], 1u);
-    for (int k = 2; k <= P; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < P; i += blockDim.x) {
-          const int ixj = i ^ j;
-          if (ixj > i) {
-            const unsigned long long a = s_key[i], b = s_key[ixj];
-            const bool up = (i & k) == 0;
-            if ((a > b) == up) {
-              s_key[i] = b;
-              s_key[ixj] = a;
+    for (int i = tid; i < m; i += kRankThreads) atomicAdd(&s_cnt[s_l[i]], 1u);
+    if (count_sort) {
+      // Keys are distinct (the index is part of the key): a key's position is
+      // the number of keys below it.
+      const unsigned long long k0 = tid < m ? s_key[tid] : ~0ull;
+      const unsigned long long k1 = tid + kRankThreads < m ? s_key[tid + kRankThreads] : ~0ull;
+      uint32_t r0 = 0u, r1 = 0u;
+      for (int j = 0; j < m; j++) {
+        const unsigned long long kj = s_key[j];
+        r0 += kj < k0 ? 1u : 0u;
+        r1 += kj < k1 ? 1u : 0u;
+      }
+      __syncthreads();
+      if (tid < m) s_key[r0] = k0;
+      if (tid + kRankThreads < m) s_key[r1] = k1;
+    } else {
+      for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int i = tid; i < P; i += kRankThreads) {
+            const int ixj = i ^ j;
+            if (ixj > i) {
+              const unsigned long long a = s_key[i], b = s_key[ixj];
+              const bool up = (i & k) == 0;
+              if ((a > b) == up) {
+                s_key[i] = b;
+                s_key[ixj] = a;
+              }
             }
           }
+          __syncthreads();
         }
-        __syncthreads();
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       // maxDCG@T: labels in descending order, ranks ascending.
       double md = 0.0;
       int r = 0;
-      const int lim = m < kRankTrunc ? m : kRankTrunc;
-      for (int l = 31; l >= 0 && r < lim; l--)
-        for (uint32_t c = 0; c < s_cnt[l] && r < lim; c++, r++)
-          md += (ldexp(1.0, l) - 1.0) * disc[r];
+      for (int l = 31; l >= 0 && r < T; l--)
+        for (uint32_t c = 0; c < s_cnt[l] && r < T; c++, r++) md += (ldexp(1.0, l) - 1.0) * disc[r];
       s_inv = md > 0.0 ? 1.0 / md : 0.0;
     }
     __syncthreads();
     const double inv = s_inv;
-    for (int a = threadIdx.x; a < m; a += blockDim.x) {
-      const uint32_t i = (uint32_t)(s_key[a] & 0xFFFFFFFFull);
-      double ga = 0.0, ha = 0.0;
-      if (inv > 0.0) {
-        const int li = s_l[i];
-        const float si = s_s[i];
-        const int bend = a < kRankTrunc ? m : (m < kRankTrunc ? m : kRankTrunc);
-        for (int b = 0; b < bend; b++) {
-          if (b == a) continue;
-          const uint32_t j = (uint32_t)(s_key[b] & 0xFFFFFFFFull);
-          const int lj = s_l[j];
-          if (li == lj) continue;
-          const bool ih = li > lj;
-          const float sj = s_s[j];
-          const float shi = ih ? si : sj, slo = ih ? sj : si;
-          const float rho = 1.0f / (1.0f + exp32(shi - slo));
-          const double gap = ldexp(1.0, ih ? li : lj) - ldexp(1.0, ih ? lj : li);
-          const double dn = gap * fabs(disc[a] - disc[b]) * inv;
-          const double lam = (double)rho * dn;
-          ga += ih ? -lam : lam;
-          ha += (double)rho * (1.0 - (double)rho) * dn;
+    if (inv > 0.0) {
+      double ga = 0.0, ha = 0.0;  // row sums of top position `tid` (tid < T)
+      for (int c0 = 0; c0 < m; c0 += kRankChunk) {
+        const int cn = m - c0 < kRankChunk ? m - c0 : kRankChunk;
+        for (int t = tid; t < T * kRankChunk; t += kRankThreads) {
+          const int a = t / kRankChunk, bb = t % kRankChunk, b = c0 + bb;
+          double lam = 0.0, hes = 0.0;
+          if (bb < cn && b != a) {
+            const unsigned long long ka = s_key[a], kb = s_key[b];
+            const int li = s_l[rank_key_index(ka)], lj = s_l[rank_key_index(kb)];
+            if (li != lj) {
+              const bool ih = li > lj;
+              const float si = rank_key_score(ka), sj = rank_key_score(kb);
+              const float shi = ih ? si : sj, slo = ih ? sj : si;
+              const float rho = 1.0f / (1.0f + exp32(shi - slo));
+              const double gap = ldexp(1.0, ih ? li : lj) - ldexp(1.0, ih ? lj : li);
+              const double dn = gap * fabs(disc[a] - disc[b]) * inv;
+              const double l = (double)rho * dn;
+              lam = ih ? -l : l;
+              hes = (double)rho * (1.0 - (double)rho) * dn;
+            }
+          }
+          s_tl[a * kRankStride + bb] = lam;
+          s_th[a * kRankStride + bb] = hes;
         }
+        __syncthreads();
+        if (tid < T) {
+          // Row of top position tid, in position order; the diagonal and
+          // equal-label terms are +0, which leaves a sum that is never -0
+          // unchanged.
+          const double* rl = s_tl + tid * kRankStride;
+          const double* rh = s_th + tid * kRankStride;
+          for (int bb = 0; bb < cn; bb++) {
+            ga += rl[bb];
+            ha += rh[bb];
+          }
+        } else if (tid >= WAVE && tid < WAVE + cn && c0 + (tid - WAVE) >= T) {
+          // Column of lower position b: its pairs are the top positions, in
+          // order; its term is the top position's, negated.
+          const int bb = tid - WAVE;
+          double gb = 0.0, hb = 0.0;
+          for (int a = 0; a < T; a++) {
+            gb += -s_tl[a * kRankStride + bb];
+            hb += s_th[a * kRankStride + bb];
+          }
+          emit(rank_key_index(s_key[c0 + bb]), gb, hb, b0);
+        }
+        __syncthreads();
       }
-      const double wi = w ? (double)w[b0 + i] : 1.0;
-      const float gv = (float)(ga * wi), hv = (float)(ha * wi);
-      g[b0 + i] = gv;
-      h[b0 + i] = hv;
-      mg = max(mg, abs_bits(gv));
-      mh = max(mh, abs_bits(hv));
+      if (tid < T) emit(rank_key_index(s_key[tid]), ga, ha, b0);
+    } else {
+      for (int a = tid; a < m; a += kRankThreads) emit(rank_key_index(s_key[a]), 0.0, 0.0, b0);
     }
     __syncthreads();  // LDS is reused by the next group
   }
@@ -503,15 +574,15 @@ __global__ __launch_bounds__(kRankThreads) void k_lambdarank(
     mg = max(mg, (uint32_t)__shfl_xor(mg, m2, WAVE));
     mh = max(mh, (uint32_t)__shfl_xor(mh, m2, WAVE));
   }
-  if ((threadIdx.x & (WAVE - 1)) == 0) {
-    s_max[0][threadIdx.x / WAVE] = mg;
-    s_max[1][threadIdx.x / WAVE] = mh;
+  if ((tid & (WAVE - 1)) == 0) {
+    s_max[0][tid / WAVE] = mg;
+    s_max[1][tid / WAVE] = mh;
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
+  if (tid < 2) {
     uint32_t v = 0u;
-    for (int i = 0; i < kRankThreads / WAVE; i++) v = max(v, s_max[threadIdx.x][i]);
-    atomicMax(&absmax[threadIdx.x], v);
+    for (int i = 0; i < kRankThreads / WAVE; i++) v = max(v, s_max[tid][i]);
+    atomicMax(&absmax[tid], v);
   }
 }
 
@@ -519,7 +590,7 @@ void launch_lambdarank(hipStream_t s, const float* preds, const float* y, const 
                        const int64_t* group_off, int64_t n_groups, const double* disc, float* g,
                        float* h, uint32_t* absmax) {
   if (n_groups <= 0) return;
-  const int64_t grid = n_groups < 2048 ? n_groups : 2048;
+  const int64_t grid = n_groups < 16384 ? n_groups : 16384;
   hipLaunchKernelGGL(k_lambdarank, dim3((unsigned)grid), dim3(kRankThreads), 0, s, preds, y, w,
                      group_off, n_groups, disc, g, h, absmax);
 }

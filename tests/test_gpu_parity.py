@@ -92,9 +92,14 @@ def test_gradients(ctx, oracle, obj, K):
 
 def test_lambdarank_gradients(ctx, oracle):
     rng = np.random.default_rng(21)
-    sizes = np.concatenate([[1, 2, 600, 31, 30, 29], rng.integers(1, 200, 80)]).astype(np.int32)
+    # 512 / 513: the two sort paths either side of their switch; 4096: the
+    # largest group; 7: a group whose labels are all equal (maxDCG = 0).
+    sizes = np.concatenate([[1, 2, 600, 31, 30, 29, 512, 513, 4096, 7],
+                            rng.integers(1, 200, 80)]).astype(np.int32)
     n = int(sizes.sum())
     y = rng.integers(0, 5, n).astype(np.float32)
+    z = int(sizes[:9].sum())
+    y[z:z + 7] = 2.0
     p = (rng.standard_normal(n) * 2).astype(np.float32)
     p[::7] = p[1::7][: len(p[::7])]  # ties inside groups
     p[5] = -0.0
