@@ -61,14 +61,16 @@ def cfg_c2(n_trees, depth):
     return c
 
 
-HIST_SHAPE = (0, 0, 0)  # (features per block, rows per block, blocks per CU); 0 = autotuned
+# (features per block, rows per block, blocks per CU, sub-histograms 1 LDS / 2 global);
+# 0 = autotuned
+HIST_SHAPE = (0, 0, 0, 0)
 
 
 def ext(schedule):
     e = A.ExtConfig()
     A.lib().bst_ext_config_default(C.byref(e))
     e.schedule = schedule
-    e.hist_fg, e.hist_rows_min, e.hist_blocks_per_cu = HIST_SHAPE
+    e.hist_fg, e.hist_rows_min, e.hist_blocks_per_cu, e.hist_priv = HIST_SHAPE
     return e
 
 
@@ -129,11 +131,12 @@ def hist_traffic_per_launch():
 
 def tuning(ctx):
     """Histogram launch shape of the last train (autotuned or fixed)."""
-    v = [C.c_int32() for _ in range(4)]
+    v = [C.c_int32() for _ in range(5)]
     ms = C.c_double()
     A.check(A.lib().bst_last_train_tuning(ctx, *[C.byref(x) for x in v], C.byref(ms)))
     return {"hist_fg": v[0].value, "hist_rows_min": v[1].value, "hist_blocks_per_cu": v[2].value,
-            "autotuned": bool(v[3].value)}
+            "hist_priv": "global" if v[3].value == 2 else "lds", "autotuned": bool(v[4].value),
+            "tune_ms": round(ms.value, 1)}
 
 
 def kernel_stats(ctx, kind):
@@ -620,12 +623,14 @@ def main():
                          "no batched or plain schedule): the run a kernel trace of the graph "
                          "schedule is taken on")
     ap.add_argument("--hist-shape", default="",
-                    help="fixed histogram launch shape FG,ROWS_MIN,BLOCKS_PER_CU instead of the "
-                         "autotuner's choice (e.g. 32,2048,1)")
+                    help="fixed histogram launch shape FG,ROWS_MIN,BLOCKS_PER_CU[,PRIV] instead of "
+                         "the autotuner's choice (e.g. 32,2048,1; PRIV 1 = LDS sub-histograms, "
+                         "2 = global memory, default 1)")
     args = ap.parse_args()
     if args.hist_shape:
         global HIST_SHAPE
-        HIST_SHAPE = tuple(int(v) for v in args.hist_shape.split(","))
+        v = [int(x) for x in args.hist_shape.split(",")]
+        HIST_SHAPE = tuple(v + [1] * (4 - len(v)))
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (row sharding "

@@ -84,6 +84,7 @@ class ExtConfig(C.Structure):
         ("hist_fg", C.c_int32),
         ("hist_rows_min", C.c_int32),
         ("hist_blocks_per_cu", C.c_int32),
+        ("hist_priv", C.c_int32),
     ]
 
 
@@ -256,7 +257,8 @@ def _declare(lib):
         "bst_ctx_set_keep_predictions": (C.c_int, [vp, C.c_int32]),
         "bst_last_train_predictions": (C.c_int, [vp, P(C.c_float), C.c_int64, P(C.c_int64)]),
         "bst_last_train_tuning": (
-            C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_double)]),
+            C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                      P(C.c_double)]),
         "bst_model_info_get": (C.c_int, [vp, P(ModelInfo)]),
         "bst_model_export": (C.c_int, [vp, P(ForestArrays)]),
         "bst_model_import": (C.c_int, [vp, P(ModelInfo), P(ForestArrays), P(vp)]),

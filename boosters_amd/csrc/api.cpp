@@ -673,6 +673,8 @@ static int train_impl(bst_ctx* ctx, bst_dataset* ds, const float* y_dev, int64_t
   }
   need(opt.ext.hist_blocks_per_cu >= 0 && opt.ext.hist_blocks_per_cu <= 8, BST_E_CONFIG,
        "hist_blocks_per_cu must be in 0..8");
+  need(opt.ext.hist_priv >= 0 && opt.ext.hist_priv <= 2, BST_E_CONFIG,
+       "hist_priv must be 0 (tuned), 1 (LDS) or 2 (global memory)");
   if (opt.ext.goss_top_rate > 0.0f) {  // RowSamplingParams::Goss
     need(opt.ext.goss_top_rate < 1.0f && opt.ext.goss_other_rate > 0.0f &&
              opt.ext.goss_other_rate <= 1.0f &&
@@ -798,13 +800,15 @@ int bst_last_train_stats(bst_ctx* ctx, double* ms_total, double* ms_per_tree, in
 }
 
 int bst_last_train_tuning(bst_ctx* ctx, int32_t* hist_fg, int32_t* hist_rows_min,
-                          int32_t* hist_blocks_per_cu, int32_t* tuned, double* tune_ms) {
+                          int32_t* hist_blocks_per_cu, int32_t* hist_priv, int32_t* tuned,
+                          double* tune_ms) {
   return guard([&] {
     need(ctx, BST_E_INVALID, "null context");
     const HistTuning& t = ctx->last_tuning;
     if (hist_fg) *hist_fg = t.fg;
     if (hist_rows_min) *hist_rows_min = (int32_t)t.tile_min;
     if (hist_blocks_per_cu) *hist_blocks_per_cu = (int32_t)t.blocks_per_cu;
+    if (hist_priv) *hist_priv = (int32_t)t.priv;
     if (tuned) *tuned = t.tuned;
     if (tune_ms) *tune_ms = t.tune_ms;
     return BST_OK;

@@ -1,14 +1,19 @@
 // autotune.cpp — measured choice of the histogram launch shape.
 //
 // No counterpart in the reference (its histograms are built on the CPU).
-// Three knobs decide how a step's histogram work is cut into blocks:
+// Four knobs decide how a step's histogram work is cut into blocks and where
+// each block keeps its sub-histogram:
 //   fg            features per block: 32 (128 KB LDS sub-histogram, one block
 //                 per CU, 4 waves per SIMD) or 16 (64 KB, two blocks per CU,
 //                 8 waves per SIMD, each row read by twice as many blocks);
 //   tile_min      fewest rows per block; a step's tile (hist_step_tile) is
 //                 the 512-row multiple that spreads its rows over at most
 //   blocks_per_cu x n_cu blocks (more blocks balance better, fewer make fewer
-//                 partial histograms for the finish pass to add).
+//                 partial histograms for the finish pass to add);
+//   priv          sub-histograms in LDS (zeroed, filled with LDS atomics and
+//                 flushed per item) or in global memory (the item's own
+//                 section, filled with device atomics at L2: no LDS zeroing
+//                 or flush, but an L2 atomic per (row, feature) and g/h).
 // Histogram sums are exact integers, so every shape gives the same
 // histograms: the choice changes time only, never the trees.
 //
@@ -142,9 +147,10 @@ struct Bench {
 
   // Time (ms) of one step's histogram + finish launches: the fastest of
   // `runs` runs.
-  float run(const Plan& pl, int fg, int runs) {
+  float run(const Plan& pl, int fg, int priv, int runs) {
     DatasetDev dd = d;
     dd.FG = fg;
+    dd.hist_priv = priv;
     dd.NG = (d.E + fg - 1) / fg;
     const int64_t slot_elems = (int64_t)dd.NG * 256 * fg * 2;
     h2d(ctx->stream, items.p, pl.items.data(), pl.items.size());
@@ -176,16 +182,18 @@ HistTuning choose_hist_tuning(bst_ctx* ctx, const DatasetDev& d, const bst_gbdt_
                               const bst_ext_config& ext, int T) {
   HistTuning t;
   const bool fix_fg = ext.hist_fg != 0, fix_tile = ext.hist_rows_min != 0,
-             fix_bpc = ext.hist_blocks_per_cu != 0;
+             fix_bpc = ext.hist_blocks_per_cu != 0, fix_priv = ext.hist_priv != 0;
   if (fix_fg) t.fg = ext.hist_fg;
   if (fix_tile) t.tile_min = (uint32_t)ext.hist_rows_min;
   if (fix_bpc) t.blocks_per_cu = (uint32_t)ext.hist_blocks_per_cu;
-  if (!ext.autotune || (fix_fg && fix_tile && fix_bpc) || d.n < 4096 || d.E <= 0) return t;
+  if (fix_priv) t.priv = ext.hist_priv;
+  if (!ext.autotune || (fix_fg && fix_tile && fix_bpc && fix_priv) || d.n < 4096 || d.E <= 0)
+    return t;
 
   const std::vector<int64_t> key = {d.n, d.E, d.S, d.hi_off >= 0, T, cfg.growth,
                                     cfg.growth == BST_GROW_DEPTHWISE ? cfg.max_depth : cfg.max_leaves,
                                     ext.hist_fg, ext.hist_rows_min, ext.hist_blocks_per_cu,
-                                    ctx->n_cu};
+                                    ext.hist_priv, ctx->n_cu};
   auto hit = ctx->tune_cache.find(key);
   if (hit != ctx->tune_cache.end()) {
     HistTuning c = hit->second;
@@ -198,6 +206,7 @@ HistTuning choose_hist_tuning(bst_ctx* ctx, const DatasetDev& d, const bst_gbdt_
       fix_tile ? std::vector<uint32_t>{t.tile_min} : std::vector<uint32_t>{512, 1024, 2048};
   std::vector<uint32_t> bpcs =
       fix_bpc ? std::vector<uint32_t>{t.blocks_per_cu} : std::vector<uint32_t>{1, 2, 4};
+  std::vector<int> privs = fix_priv ? std::vector<int>{t.priv} : std::vector<int>{1, 2};
   // Unsampled single-tree rounds read the root's rows in place (no gather).
   const bool sampled = !(std::fabs(cfg.subsample - 1.0f) < 1e-6f) || ext.goss_top_rate > 0.0f;
   const std::vector<Shape> shapes = tree_shapes(d.n, cfg, T, T > 1 || sampled);
@@ -206,14 +215,19 @@ HistTuning choose_hist_tuning(bst_ctx* ctx, const DatasetDev& d, const bst_gbdt_
   struct Cand {
     int fg;
     uint32_t tile_min, bpc;
+    int priv;
     std::vector<Plan> plans;
   };
   std::vector<Cand> cands;
   size_t max_items = 0, max_fin = 0, max_partial = 0;
+  for (int priv : privs)
   for (int fg : fgs)
     for (uint32_t tm : tiles)
       for (uint32_t bpc : bpcs) {
-        Cand c{fg, tm, bpc, {}};
+        // Global-memory sub-histograms are measured at the default spread
+        // (two blocks per CU): their cost is the atomics, not the block count.
+        if (priv == 2 && !fix_bpc && bpc != 2) continue;
+        Cand c{fg, tm, bpc, priv, {}};
         const uint32_t NG = (uint32_t)((d.E + fg - 1) / fg);
         const uint32_t target = (uint32_t)std::max(64, (int)bpc * ctx->n_cu);
         for (const Shape& sh : shapes) {
@@ -267,27 +281,73 @@ HistTuning choose_hist_tuning(bst_ctx* ctx, const DatasetDev& d, const bst_gbdt_
   // repeats spreads clock drift over every candidate alike.
   constexpr int kRounds = 3;
   std::vector<std::vector<float>> tmin(cands.size(), std::vector<float>(shapes.size(), 1e30f));
+  // LDS candidates (global ones too when the knob is fixed to global) first,
+  // over all rounds.
+  const bool lds_first = privs.size() > 1;
   for (int r = 0; r < kRounds; r++)
-    for (size_t ci = 0; ci < cands.size(); ci++)
+    for (size_t ci = 0; ci < cands.size(); ci++) {
+      if (lds_first && cands[ci].priv == 2) continue;
       for (size_t i = 0; i < shapes.size(); i++) {
-        const float ms = B.run(cands[ci].plans[i], cands[ci].fg, 1);
+        const float ms = B.run(cands[ci].plans[i], cands[ci].fg, cands[ci].priv, 1);
         if (r > 0) tmin[ci][i] = std::min(tmin[ci][i], ms);
       }
+    }
+  auto total_of = [&](size_t ci) {
+    double total = 0;
+    for (size_t i = 0; i < shapes.size(); i++) total += shapes[i].weight * tmin[ci][i];
+    return total;
+  };
+  if (lds_first) {
+    // Global-memory candidates after: the clocks and caches are warm, so one
+    // run per step, the cheapest steps first, and a candidate is dropped as
+    // soon as the steps measured so far already cost more than the best LDS
+    // candidate's whole tree (an L2 atomic per (row, feature) and g/h is
+    // tens of times the LDS cost on every shape measured, so this stops
+    // after a step or two instead of timing whole trees of it).
+    double best_lds = 1e300;
+    for (size_t ci = 0; ci < cands.size(); ci++)
+      if (cands[ci].priv != 2) best_lds = std::min(best_lds, total_of(ci));
+    std::vector<size_t> order(shapes.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+      return (double)shapes[a].nodes * shapes[a].rows * shapes[a].weight <
+             (double)shapes[b].nodes * shapes[b].rows * shapes[b].weight;
+    });
+    for (size_t ci = 0; ci < cands.size(); ci++) {
+      if (cands[ci].priv != 2) continue;
+      double part = 0;
+      for (size_t i : order) {
+        tmin[ci][i] = B.run(cands[ci].plans[i], cands[ci].fg, cands[ci].priv, 1);
+        part += shapes[i].weight * tmin[ci][i];
+        if (part > best_lds) break;  // the unmeasured steps stay at 1e30: never picked
+      }
+    }
+  }
   const bool trace = std::getenv("BST_TRACE_TUNE") != nullptr;
   double best = 1e300;
   HistTuning pick = t;
   for (size_t ci = 0; ci < cands.size(); ci++) {
     const Cand& c = cands[ci];
-    double total = 0;
-    for (size_t i = 0; i < shapes.size(); i++) total += shapes[i].weight * tmin[ci][i];
-    if (trace)
-      std::fprintf(stderr, "[tune] fg %d rows_min %u blocks/cu %u: %.3f ms\n", c.fg, c.tile_min,
-                   c.bpc, total);
+    const double total = total_of(ci);
+    if (trace) {
+      // A pruned global candidate shows the cost of the steps it was timed on.
+      double part = 0;
+      int timed = 0;
+      for (size_t i = 0; i < shapes.size(); i++)
+        if (tmin[ci][i] < 1e29f) {
+          part += shapes[i].weight * tmin[ci][i];
+          timed++;
+        }
+      std::fprintf(stderr, "[tune] %s fg %d rows_min %u blocks/cu %u: %.3f ms%s\n",
+                   c.priv == 2 ? "global" : "lds", c.fg, c.tile_min, c.bpc, part,
+                   timed < (int)shapes.size() ? " (pruned: steps timed so far)" : "");
+    }
     if (total < best) {
       best = total;
       pick.fg = c.fg;
       pick.tile_min = c.tile_min;
       pick.blocks_per_cu = c.bpc;
+      pick.priv = c.priv;
     }
   }
   BST_HIP(hipStreamSynchronize(ctx->stream));

@@ -116,6 +116,7 @@ struct DatasetDev {
   int32_t FG = 32;      // features per histogram group (lanes of one row)
   int32_t NG = 0;       // number of groups
   int32_t max_cat_bins = 0;  // most categories of a categorical feature (0: none)
+  int32_t hist_priv = 1;     // histogram sub-histograms: 1 in LDS, 2 in global memory
   int64_t hi_words = 0; // u32 words per feature in hi_cm
   uint8_t* bins_rm = nullptr;   // [n][S]
   uint8_t* bins_cm = nullptr;   // [E][n]

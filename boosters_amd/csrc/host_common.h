@@ -227,6 +227,7 @@ struct HistTuning {
   int fg = 32;                 // features per histogram block: 32 (one block per CU) or 16 (two)
   uint32_t tile_min = 1024;    // fewest rows per block in a step (power of two, 512..8192)
   uint32_t blocks_per_cu = 2;  // blocks a step's rows are spread over, per CU
+  int priv = 1;                // sub-histograms: 1 in LDS, 2 in global memory (device atomics)
   int tuned = 0;               // 1 = measured on this data shape
   double tune_ms = 0;          // time spent measuring (0 when fixed or cached)
 };

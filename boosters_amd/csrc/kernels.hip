@@ -965,6 +965,104 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
   }
 }
 
+// The same items built with global-memory privatisation instead of LDS: the
+// block owns its item's section in global memory — the node's slot for a
+// tile that covers the whole node, the item's partial otherwise — zeroes it
+// and adds each (row, feature) into it with two device atomics at L2; with
+// no partial buffer a tile adds into the shared, pre-zeroed slot.  No LDS
+// sub-histogram, so no zeroing and flushing of one per item and no LDS
+// limit on the block count, but every (row, feature) pays two L2 atomics
+// instead of two LDS atomics.  The sums are the same integers as k_hist's,
+// bin 256 (the NaN bin of a 257-bin feature) is dropped the same way, and
+// the node totals are formed the same way, so the outputs are identical; the
+// autotuner measures both and keeps the faster per data shape.
+template <int FG, bool GATHER, bool HAS_HI>
+__global__ __launch_bounds__(kHistThreads) void k_hist_gpriv(
+    const HistItem* __restrict__ items, uint32_t n_items_static, int NG,
+    const uint32_t* __restrict__ idx, const float2* __restrict__ ogh, int64_t tree_stride,
+    const uint8_t* __restrict__ bins_rm, int S, int hi_off, const int32_t* __restrict__ qexp,
+    long long* __restrict__ hist, int64_t slot_elems, long long* __restrict__ node_sum,
+    long long* __restrict__ partial, const uint32_t* __restrict__ dyn,
+    const uint32_t* __restrict__ idx_alt, const float2* __restrict__ ogh_alt) {
+  __shared__ long long s_red[2][kHistWaves];
+  constexpr int RS = WAVE / FG;  // rows per wave per pass, one lane per feature
+  const uint32_t n_items = dyn ? *dyn : n_items_static;
+  const uint32_t n_tiles = n_items / (uint32_t)NG;
+  const uint32_t n_pos = ((n_tiles + 7u) / 8u) * 8u * (uint32_t)NG;
+  const int wv = threadIdx.x / WAVE;
+  const int ln = threadIdx.x % WAVE;
+  const int lf = ln % FG;
+  const int rs = ln / FG;
+  for (uint32_t p = blockIdx.x; p < n_pos; p += gridDim.x) {
+    const uint32_t tile = (p / (8u * NG)) * 8u + (p & 7u);
+    const uint32_t grp = (p >> 3) % (uint32_t)NG;
+    if (tile >= n_tiles) continue;  // block-uniform
+    const uint32_t item_idx = tile * (uint32_t)NG + grp;
+    const HistItem it = items[item_idx];
+    const bool direct = (it.tree & 0x80000000u) != 0;
+    const bool alt = (it.tree & 0x40000000u) != 0;
+    const uint32_t tree = it.tree & 0x3FFFFFFFu;
+    long long* const slot_sec =
+        hist + (int64_t)it.slot * slot_elems + (int64_t)it.group * 256 * FG * 2;
+    const bool own = direct || partial != nullptr;
+    long long* const sec = direct || !partial ? slot_sec : partial + (int64_t)item_idx * 256 * FG * 2;
+    if (own)
+      for (int i = threadIdx.x; i < 256 * FG * 2; i += blockDim.x) sec[i] = 0;
+    __syncthreads();  // the section is zero before any wave adds into it
+    const int f = (int)it.group * FG + lf;
+    const int hb = hi_off + (f >> 3), hs = f & 7;
+    const double sg = ldexp(1.0, qexp[2 * tree]);
+    const double sh = ldexp(1.0, qexp[2 * tree + 1]);
+    const uint32_t* tidx = (alt ? idx_alt : idx) + (int64_t)tree * tree_stride;
+    const float2* togh = (alt ? ogh_alt : ogh) + (int64_t)tree * tree_stride;
+    long long accg = 0, acch = 0;
+    for (uint32_t q = (uint32_t)(wv * RS + rs); q < it.count; q += (uint32_t)(kHistWaves * RS)) {
+      const uint32_t pos = it.begin + q;
+      const uint32_t r = GATHER ? tidx[pos] : pos;
+      const float2 v = togh[pos];
+      const long long g = __double2ll_rn((double)v.x * sg);
+      const long long h = __double2ll_rn((double)v.y * sh);
+      if (lf == 0) {
+        accg += g;
+        acch += h;
+      }
+      const uint8_t* rp = bins_rm + (uint64_t)r * (uint64_t)S;
+      uint32_t b = rp[f];
+      if (HAS_HI) b |= ((uint32_t)(rp[hb] >> hs) & 1u) << 8;
+      if (b < 256u) {
+        long long* const e = sec + 2 * ((int64_t)b * FG + lf);
+        atomicAdd((unsigned long long*)e, (unsigned long long)g);
+        atomicAdd((unsigned long long*)(e + 1), (unsigned long long)h);
+      }
+    }
+    const bool do_sum = node_sum != nullptr && it.group == 0;
+    if (do_sum) {
+      accg = wave_sum_ll(accg);
+      acch = wave_sum_ll(acch);
+      if (ln == 0) {
+        s_red[0][wv] = accg;
+        s_red[1][wv] = acch;
+      }
+    }
+    __syncthreads();
+    if (do_sum && threadIdx.x == 0) {
+      long long tg = 0, th = 0;
+      for (int w = 0; w < kHistWaves; w++) {
+        tg += s_red[0][w];
+        th += s_red[1][w];
+      }
+      if (direct) {
+        node_sum[2 * it.node] = tg;
+        node_sum[2 * it.node + 1] = th;
+      } else {
+        atomicAdd((unsigned long long*)&node_sum[2 * it.node], (unsigned long long)tg);
+        atomicAdd((unsigned long long*)&node_sum[2 * it.node + 1], (unsigned long long)th);
+      }
+    }
+    __syncthreads();  // s_red is reused by the next item
+  }
+}
+
 // Finish the histograms of one step: for each built node (small child, or a
 // root) and group, the node histogram is the sum of its tiles' partials (or
 // was stored directly by a single tile), and the larger sibling, which holds
@@ -1077,6 +1175,19 @@ static void launch_hist_fg(hipStream_t s, const DatasetDev& d, const HistItem* i
   hipLaunchKernelGGL((k_hist<FG, G, H, O>), dim3((unsigned)grid), dim3(kHistThreads), 0, s, items, \
                      (uint32_t)n_items, d.NG, idx, ogh, tree_stride, d.bins_rm, d.S, d.hi_off,  \
                      d.E, qexp, hist, slot_elems, node_sum, partial, dyn, idx_alt, ogh_alt)
+  if (d.hist_priv == 2) {
+#define BST_GPRIV_LAUNCH(G, H)                                                                  \
+  hipLaunchKernelGGL((k_hist_gpriv<FG, G, H>), dim3((unsigned)grid), dim3(kHistThreads), 0, s,    \
+                     items, (uint32_t)n_items, d.NG, idx, ogh, tree_stride, d.bins_rm, d.S,      \
+                     d.hi_off, qexp, hist, slot_elems, node_sum, partial, dyn, idx_alt, ogh_alt)
+    if (gather) {
+      if (hi) BST_GPRIV_LAUNCH(true, true); else BST_GPRIV_LAUNCH(true, false);
+    } else {
+      if (hi) BST_GPRIV_LAUNCH(false, true); else BST_GPRIV_LAUNCH(false, false);
+    }
+#undef BST_GPRIV_LAUNCH
+    return;
+  }
   const bool off64 = (uint64_t)d.n * (uint64_t)d.S >= (1ull << 32);
   if (off64) {
     if (gather) {

@@ -142,6 +142,7 @@ Trainer::Trainer(bst_ctx* ctx, bst_dataset* ds, const TrainOptions& opt)
   ctx->last_tuning = tune_;
   dd_ = ds->d;
   dd_.FG = tune_.fg;
+  dd_.hist_priv = tune_.priv;
   dd_.NG = (dd_.E + dd_.FG - 1) / dd_.FG;
   slot_elems_ = (int64_t)dd_.NG * 256 * dd_.FG * 2;
   rightmost_ = opt_.ext.n_threads != 1 && ds->d.E >= 16;

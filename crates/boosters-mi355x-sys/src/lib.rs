@@ -164,6 +164,7 @@ pub struct bst_ext_config {
     pub hist_fg: i32,
     pub hist_rows_min: i32,
     pub hist_blocks_per_cu: i32,
+    pub hist_priv: i32,
 }
 
 impl Default for bst_ext_config {
@@ -179,6 +180,7 @@ impl Default for bst_ext_config {
             hist_fg: 0,
             hist_rows_min: 0,
             hist_blocks_per_cu: 0,
+            hist_priv: 0,
         };
         unsafe { bst_ext_config_default(&mut e) };
         e
@@ -361,6 +363,7 @@ extern "C" {
         hist_fg: *mut i32,
         hist_rows_min: *mut i32,
         hist_blocks_per_cu: *mut i32,
+        hist_priv: *mut i32,
         tuned: *mut i32,
         tune_ms: *mut f64,
     ) -> c_int;

@@ -144,6 +144,9 @@ typedef struct bst_ext_config {
                                  histogram block, a power of two 512..8192    */
   int32_t hist_blocks_per_cu; /* 0 = tuned / default 2; else 1..8 histogram
                                  blocks per CU that a step's rows spread over */
+  int32_t hist_priv;          /* 0 = tuned / default 1; 1 = per-block
+                                 sub-histograms in LDS, 2 = in global memory
+                                 (device atomics at L2)                       */
 } bst_ext_config;
 
 /* ---- forest exchange format -------------------------------------------- */
@@ -262,7 +265,8 @@ int bst_last_train_stats(bst_ctx* ctx, double* ms_total, double* ms_per_tree,
  * time the measurement took in that call (0 when cached).  No counterpart in
  * the reference. */
 int bst_last_train_tuning(bst_ctx* ctx, int32_t* hist_fg, int32_t* hist_rows_min,
-                          int32_t* hist_blocks_per_cu, int32_t* tuned, double* tune_ms);
+                          int32_t* hist_blocks_per_cu, int32_t* hist_priv, int32_t* tuned,
+                          double* tune_ms);
 /* Per-round metric values recorded by the last train ([n_rounds]). */
 int bst_last_train_metric(bst_ctx* ctx, double* out, int64_t cap, int64_t* n_rounds);
 /* Training predictions (raw margins, [n_outputs][n_rows]) after the last

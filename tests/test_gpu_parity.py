@@ -363,33 +363,39 @@ def test_graph_schedule_matches_oracle(ctx, oracle, case):
         np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
 
 
-HIST_SHAPES = [(16, 512, 4), (32, 2048, 1), (16, 8192, 1), (32, 512, 8)]
+# (features per block, fewest rows per block, blocks per CU, sub-histograms:
+# 1 in LDS, 2 in global memory)
+HIST_SHAPES = [(16, 512, 4, 1), (32, 2048, 1, 1), (16, 8192, 1, 1), (32, 512, 8, 1),
+               (16, 1024, 2, 2), (32, 512, 4, 2)]
 
 
 @pytest.mark.parametrize("case", ["logistic_nan_depth8", "softmax_leafwise_cat", "squared_leafwise"])
 def test_hist_launch_shapes_identical(ctx, oracle, case):
     """Every histogram launch shape the autotuner can pick (16- or 32-feature
-    groups, tile sizes, blocks per CU) grows the oracle's forest: the sums
-    are exact, so the shape changes time only."""
+    groups, tile sizes, blocks per CU, LDS or global-memory sub-histograms)
+    grows the oracle's forest: the sums are exact, so the shape changes time
+    only."""
     build, kw = GRAPH_CASES[case]
     X, y, is_cat = build()
     cfg = make_cfg(**kw)
     gm, om, b = _train_both(ctx, oracle, X, y, cfg, is_cat, schedule=2)
     ga = _forests_equal(gm, om.info, om.arrays)
     for sched in (2, 1):
-        for fg, rows_min, bpc in HIST_SHAPES:
+        for fg, rows_min, bpc, priv in HIST_SHAPES:
             ext = make_ext(sched)
             ext.autotune = 0
             ext.hist_fg, ext.hist_rows_min, ext.hist_blocks_per_cu = fg, rows_min, bpc
+            ext.hist_priv = priv
             h = C.c_void_p()
             A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0,
                                       C.byref(cfg), C.byref(ext), C.byref(h)))
             gb = GBDTModel(h, ctx).export()
             for k in ga:
-                np.testing.assert_array_equal(ga[k], gb[k], err_msg=f"{k} {sched} {fg} {rows_min} {bpc}")
-            got = [C.c_int32() for _ in range(4)]
+                np.testing.assert_array_equal(ga[k], gb[k],
+                                              err_msg=f"{k} {sched} {fg} {rows_min} {bpc} {priv}")
+            got = [C.c_int32() for _ in range(5)]
             A.check(A.lib().bst_last_train_tuning(ctx.h, *[C.byref(v) for v in got], None))
-            assert [v.value for v in got] == [fg, rows_min, bpc, 0]
+            assert [v.value for v in got] == [fg, rows_min, bpc, priv, 0]
 
 
 @pytest.mark.parametrize("E,nan", [(113, True), (40, False), (40, True), (7, True)])
@@ -449,20 +455,20 @@ def test_autotuner_choice_and_cache(ctx, oracle):
     cfg = make_cfg(objective=1, n_trees=3, max_depth=6)
     gm, om, b = _train_both(ctx, oracle, X, y, cfg, None, schedule=2)
     _forests_equal(gm, om.info, om.arrays)
-    fg, rows_min, bpc, tuned = (C.c_int32() for _ in range(4))
+    fg, rows_min, bpc, priv, tuned = (C.c_int32() for _ in range(5))
     ms = C.c_double()
     A.check(A.lib().bst_last_train_tuning(ctx.h, C.byref(fg), C.byref(rows_min), C.byref(bpc),
-                                          C.byref(tuned), C.byref(ms)))
+                                          C.byref(priv), C.byref(tuned), C.byref(ms)))
     assert tuned.value == 1 and fg.value in (16, 32) and rows_min.value in (512, 1024, 2048)
-    assert bpc.value in (1, 2, 4) and ms.value > 0
-    first = (fg.value, rows_min.value, bpc.value)
+    assert bpc.value in (1, 2, 4) and priv.value in (1, 2) and ms.value > 0
+    first = (fg.value, rows_min.value, bpc.value, priv.value)
     h = C.c_void_p()
     A.check(A.lib().bst_train(ctx.h, b.h, A.ptr(y), len(y), None, None, None, 0, C.byref(cfg),
                               C.byref(make_ext(2)), C.byref(h)))
     GBDTModel(h, ctx)
     A.check(A.lib().bst_last_train_tuning(ctx.h, C.byref(fg), C.byref(rows_min), C.byref(bpc),
-                                          C.byref(tuned), C.byref(ms)))
-    assert (fg.value, rows_min.value, bpc.value) == first and ms.value == 0.0
+                                          C.byref(priv), C.byref(tuned), C.byref(ms)))
+    assert (fg.value, rows_min.value, bpc.value, priv.value) == first and ms.value == 0.0
 
 
 @pytest.mark.parametrize("case", ["logistic", "softmax_cat"])
