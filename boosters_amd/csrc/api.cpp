@@ -79,10 +79,32 @@ bst_dataset* make_dataset(bst_ctx* ctx, const float* X, int64_t n, int32_t F,
   std::unique_ptr<bst_dataset> guard_ds(ds);
   ds->ctx = ctx;
   ds->mappers = build_mappers(X, n, F, is_cat, cfg);
+  // Effective columns (BinnedDataset::feature_views, dataset.rs:460-520):
+  // the bundles of sparse categorical features first, each one categorical
+  // column of total_bins codes whose code 0 (every feature at its default)
+  // is the missing bin; then every other non-trivial feature in feature order.
+  if (cfg.enable_bundling) ds->bundles = plan_bundles(X, n, F, ds->mappers, cfg);
+  std::vector<uint8_t> bundled((size_t)F, 0);
+  for (const Bundle& b : ds->bundles) {
+    if (b.total_bins > 256)
+      raise(BST_E_CONFIG, "a bundle of sparse categorical features needs " +
+                              std::to_string(b.total_bins) +
+                              " codes; at most 256 are supported (disable bundling)");
+    for (uint32_t f : b.features) bundled[f] = 1;
+    ds->eff_to_orig.push_back(b.features[0]);
+    FeatMeta fm{};
+    fm.n_bins = b.total_bins;
+    fm.default_bin = 0;
+    fm.is_cat = 1;
+    fm.has_missing = 1;
+    ds->fmeta.push_back(fm);
+    ds->d.max_cat_bins = std::max<int32_t>(ds->d.max_cat_bins, (int32_t)b.total_bins);
+  }
+  const int32_t n_bundles = (int32_t)ds->bundles.size();
   bool hi = false;
   for (int32_t f = 0; f < F; f++) {
     const Mapper& m = ds->mappers[f];
-    if (m.trivial) continue;
+    if (m.trivial || bundled[f]) continue;
     if (m.n_bins > 257 || (m.categorical && m.n_bins > 256))
       raise(BST_E_CONFIG, "feature " + std::to_string(f) + " has " + std::to_string(m.n_bins) +
                               " bins; at most 256 categories / 256 bins + NaN are supported");
@@ -127,6 +149,12 @@ bst_dataset* make_dataset(bst_ctx* ctx, const float* X, int64_t n, int32_t F,
     const Mapper& m = ds->mappers[ds->eff_to_orig[e]];
     BinMeta b{};
     b.orig = ds->eff_to_orig[e];
+    if (e < n_bundles) {
+      // Coded on the host below; the device pass writes zeros here.
+      b.is_cat = 1;
+      bm[e] = b;
+      continue;
+    }
     b.n_bins = m.n_bins;
     b.default_bin = m.default_bin;
     b.is_cat = m.categorical;
@@ -167,6 +195,13 @@ bst_dataset* make_dataset(bst_ctx* ctx, const float* X, int64_t n, int32_t F,
   launch_value_to_bin(s, ds->raw.p, n, E, d_bm.p, d_thr.p, d_cats.p, d.bins_cm, d.hi_cm,
                       d.hi_words);
   BST_HIP(hipGetLastError());
+  if (n_bundles > 0) {
+    std::vector<uint8_t> codes((size_t)n_bundles * n);
+    for (int32_t e = 0; e < n_bundles; e++)
+      encode_bundle(ds->bundles[e], X, n, ds->mappers, codes.data() + (size_t)e * n);
+    h2d(s, d.bins_cm, codes.data(), codes.size());  // columns 0 .. n_bundles - 1
+    BST_HIP(hipStreamSynchronize(s));               // `codes` is released on return
+  }
   launch_to_rowmajor(s, d);
   BST_HIP(hipGetLastError());
   BST_HIP(hipStreamSynchronize(s));
@@ -620,7 +655,22 @@ int bst_dataset_get_bins(const bst_dataset* ds, uint16_t* out) {
     }
     BST_HIP(hipStreamSynchronize(ds->ctx->stream));
     std::fill(out, out + (size_t)d.F * d.n, (uint16_t)0);
-    for (int32_t e = 0; e < d.E; e++) {
+    // A bundled feature's bin is decoded from its bundle's code: its own
+    // range of codes, else the bin of 0.0 (the value the code stands for).
+    for (size_t e = 0; e < ds->bundles.size(); e++) {
+      const Bundle& b = ds->bundles[e];
+      for (size_t j = 0; j < b.features.size(); j++) {
+        const Mapper& m = ds->mappers[b.features[j]];
+        const uint32_t lo_c = b.offsets[j], hi_c = b.offsets[j] + m.n_bins;
+        const uint16_t zero_bin = (uint16_t)cat_bin(m, 0.0f);
+        uint16_t* o = out + (size_t)b.features[j] * d.n;
+        for (int64_t r = 0; r < d.n; r++) {
+          const uint32_t c = lo[e * d.n + r];
+          o[r] = c >= lo_c && c < hi_c ? (uint16_t)(c - lo_c) : zero_bin;
+        }
+      }
+    }
+    for (int32_t e = (int32_t)ds->bundles.size(); e < d.E; e++) {
       uint16_t* o = out + (size_t)ds->eff_to_orig[e] * d.n;
       for (int64_t r = 0; r < d.n; r++) {
         uint16_t b = lo[(size_t)e * d.n + r];

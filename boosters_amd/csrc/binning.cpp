@@ -7,6 +7,8 @@
 //   sampling      sample_indices_for_binning (splitmix64, 200k rows), dataset.rs:598-629
 //   numeric edges bin_numeric, dataset.rs:683-750
 //   categories    create_bin_mappers categorical branch, dataset.rs:660-677
+//   bundling      Exclusive Feature Bundling of sparse categorical features,
+//                 bundling.rs:203-485, encoded by build_bundle, dataset.rs:898-952
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -23,6 +25,7 @@ namespace {
 struct Stats {
   bool trivial = false, numeric = true;
   int64_t n_nan = 0;
+  int64_t n_nonzero = 0;  // finite or infinite values != 0 (FeatureStats::non_zero_count)
 };
 
 Stats analyze(const float* col, int64_t n, bool user_cat, const bst_binning_config& cfg) {
@@ -30,7 +33,7 @@ Stats analyze(const float* col, int64_t n, bool user_cat, const bst_binning_conf
   std::vector<float> uniq;
   uniq.reserve(cap);
   bool exceeded = false, all_int = true;
-  int64_t valid = 0, nan = 0;
+  int64_t valid = 0, nan = 0, nonzero = 0;
   for (int64_t i = 0; i < n; i++) {
     const float v = col[i];
     if (std::isnan(v)) {
@@ -38,6 +41,7 @@ Stats analyze(const float* col, int64_t n, bool user_cat, const bst_binning_conf
       continue;
     }
     valid++;
+    if (v != 0.0f) nonzero++;
     if (all_int && (v - std::trunc(v)) != 0.0f) all_int = false;
     if (!exceeded) {
       bool seen = false;
@@ -60,6 +64,7 @@ Stats analyze(const float* col, int64_t n, bool user_cat, const bst_binning_conf
   const bool auto_cat = !user_cat && all_int && nu <= cfg.max_categorical_cardinality;
   s.numeric = user_cat ? false : (nu == 2 ? true : !auto_cat);
   s.n_nan = nan;
+  s.n_nonzero = nonzero;
   return s;
 }
 
@@ -175,9 +180,198 @@ std::vector<Mapper> build_mappers(const float* X, int64_t n, int32_t F, const ui
       m.n_bins = (uint32_t)m.cats.size();
     }
     m.trivial = st[f].trivial;
+    m.n_nonzero = st[f].n_nonzero;
     out[f] = std::move(m);
   });
   return out;
+}
+
+namespace {
+
+// The row sample of the conflict graph draws from rand_xoshiro 0.6's
+// Xoshiro256PlusPlus::seed_from_u64 (the state is the first four outputs of
+// SplitMix64 from the seed; xoshiro256++ of Blackman and Vigna) through rand
+// 0.8.5's UniformInt::sample_single_inclusive for 64-bit integers (widening
+// multiply, the high word accepted when the low word is within the
+// conservative zone (range << leading_zeros(range)) - 1).
+struct Xoshiro256pp {
+  uint64_t s[4];
+  explicit Xoshiro256pp(uint64_t seed) {
+    uint64_t x = seed;
+    for (uint64_t& w : s) {
+      x += 0x9E3779B97F4A7C15ULL;
+      uint64_t z = x;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+      w = z ^ (z >> 31);
+    }
+  }
+  static uint64_t rotl(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+  uint64_t next() {
+    const uint64_t r = rotl(s[0] + s[3], 23) + s[0];
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  // Uniform in [0, hi].
+  uint64_t below_incl(uint64_t hi) {
+    const uint64_t range = hi + 1;
+    if (range == 0) return next();
+    const uint64_t zone = (range << __builtin_clzll(range)) - 1;
+    for (;;) {
+      const unsigned __int128 m = (unsigned __int128)next() * range;
+      if ((uint64_t)m <= zone) return (uint64_t)(m >> 64);
+    }
+  }
+};
+
+// sample_rows, bundling.rs:203-251: the first and last tenth of the sample
+// are the first and last rows, the rest a reservoir sample of the middle.
+std::vector<uint32_t> bundle_rows(int64_t n, uint64_t max_samples, uint64_t seed) {
+  std::vector<uint32_t> out;
+  if ((uint64_t)n <= max_samples) {
+    out.resize((size_t)n);
+    for (int64_t i = 0; i < n; i++) out[(size_t)i] = (uint32_t)i;
+    return out;
+  }
+  const uint64_t n_first = max_samples / 10, n_last = max_samples / 10;
+  const uint64_t n_random = max_samples - n_first - n_last;
+  for (uint64_t i = 0; i < std::min<uint64_t>(n_first, (uint64_t)n); i++) out.push_back((uint32_t)i);
+  const uint64_t last_start = (uint64_t)n > n_last ? (uint64_t)n - n_last : 0;
+  for (uint64_t i = last_start; i < (uint64_t)n; i++) out.push_back((uint32_t)i);
+  if (last_start > n_first) {
+    const uint64_t len = last_start - n_first;
+    std::vector<uint32_t> sel;
+    for (uint64_t i = 0; i < std::min(n_random, len); i++) sel.push_back((uint32_t)(n_first + i));
+    Xoshiro256pp rng(seed);
+    for (uint64_t i = sel.size(); i < len; i++) {
+      const uint64_t j = rng.below_incl(i);
+      if (j < sel.size()) sel[j] = (uint32_t)(n_first + i);
+    }
+    out.insert(out.end(), sel.begin(), sel.end());
+  }
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
+
+}  // namespace
+
+// create_bundle_plan, bundling.rs:393-485, with BundlingConfig's defaults
+// (max conflict rate 1e-4, 256 features per bundle, 1000 sparse features,
+// 10000 sampled rows, seed 42) and min_sparsity = the binning config's
+// sparsity threshold (dataset.rs:229-236).
+std::vector<Bundle> plan_bundles(const float* X, int64_t n, int32_t F,
+                                 const std::vector<Mapper>& maps, const bst_binning_config& cfg) {
+  // Sparse categorical features: density (non-zero share, in f32) at most
+  // 1 - min_sparsity (feature_analysis.rs:250-261).
+  const float limit = 1.0f - cfg.sparsity_threshold;
+  std::vector<uint32_t> sparse;
+  std::vector<float> density;
+  for (int32_t f = 0; f < F; f++) {
+    const Mapper& m = maps[(size_t)f];
+    const float d = (float)m.n_nonzero / (float)n;
+    if (!m.trivial && m.categorical && d <= limit) {
+      sparse.push_back((uint32_t)f);
+      density.push_back(d);
+    }
+  }
+  const size_t S = sparse.size();
+  if (S < 2 || S > 1000) return {};
+  const std::vector<uint32_t> rows = bundle_rows(n, 10000, 42);
+  const size_t R = rows.size(), words = (R + 63) / 64;
+  // Non-zero, non-NaN rows of each feature among the sampled rows.
+  std::vector<uint64_t> bits(S * words, 0ull);
+  for (size_t i = 0; i < S; i++) {
+    const float* col = X + (int64_t)sparse[i] * n;
+    for (size_t k = 0; k < R; k++) {
+      const float v = col[rows[k]];
+      if (v != 0.0f && !std::isnan(v)) bits[i * words + k / 64] |= 1ull << (k % 64);
+    }
+  }
+  std::vector<uint64_t> conflict(S * S, 0);
+  size_t pairs = 0;
+  for (size_t i = 0; i < S; i++)
+    for (size_t j = i + 1; j < S; j++) {
+      uint64_t c = 0;
+      for (size_t w = 0; w < words; w++) c += __builtin_popcountll(bits[i * words + w] & bits[j * words + w]);
+      conflict[i * S + j] = conflict[j * S + i] = c;
+      pairs += c > 0;
+    }
+  const size_t total_pairs = S * (S - 1) / 2;
+  if ((float)pairs / (float)total_pairs > 0.5f) return {};
+  const uint64_t max_conflicts = (uint64_t)std::ceil(0.0001f * (float)R);
+  // Denser features first (a stable sort: ties keep feature order), each into
+  // the earliest bundle with the smallest conflict increase that stays within
+  // the budget, or a new bundle (assign_bundles, bundling.rs:277-339).
+  std::vector<size_t> order(S);
+  for (size_t i = 0; i < S; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return density[a] > density[b]; });
+  std::vector<std::vector<size_t>> members;
+  std::vector<uint64_t> used;
+  for (size_t p : order) {
+    size_t best = SIZE_MAX;
+    uint64_t best_inc = UINT64_MAX;
+    for (size_t b = 0; b < members.size(); b++) {
+      if (members[b].size() >= 256) continue;
+      uint64_t inc = 0;
+      for (size_t q : members[b]) inc += conflict[p * S + q];
+      if (used[b] + inc <= max_conflicts && inc < best_inc) {
+        best = b;
+        best_inc = inc;
+      }
+    }
+    if (best != SIZE_MAX) {
+      members[best].push_back(p);
+      used[best] += best_inc;
+    } else {
+      members.push_back({p});
+      used.push_back(0);
+    }
+  }
+  std::vector<Bundle> out;
+  for (const auto& mb : members) {
+    if (mb.size() < 2) continue;
+    Bundle b;
+    uint32_t off = 1;  // encoded bin 0: every feature of the bundle at its default
+    for (size_t p : mb) {
+      b.features.push_back(sparse[p]);
+      b.offsets.push_back(off);
+      off += maps[sparse[p]].n_bins;
+    }
+    b.total_bins = off;
+    out.push_back(std::move(b));
+  }
+  return out;
+}
+
+uint32_t cat_bin(const Mapper& m, float v) {
+  const int32_t c = to_i32_sat((double)v);
+  const auto it = std::lower_bound(m.cats.begin(), m.cats.end(), c);
+  return it != m.cats.end() && *it == c ? (uint32_t)(it - m.cats.begin()) : 0u;
+}
+
+// build_bundle, dataset.rs:898-952: a row takes the code offset + bin of the
+// first feature of the bundle (in bundle order) whose value is non-zero and
+// not NaN; 0 when there is none.
+void encode_bundle(const Bundle& b, const float* X, int64_t n, const std::vector<Mapper>& maps,
+                   uint8_t* out) {
+  std::fill(out, out + n, (uint8_t)0);
+  for (size_t j = 0; j < b.features.size(); j++) {
+    const Mapper& m = maps[b.features[j]];
+    const float* col = X + (int64_t)b.features[j] * n;
+    for (int64_t i = 0; i < n; i++) {
+      const float v = col[i];
+      if (out[i] == 0 && v != 0.0f && !std::isnan(v))
+        out[i] = (uint8_t)(b.offsets[j] + cat_bin(m, v));
+    }
+  }
 }
 
 // next_up_f32, training/gbdt/grower.rs:543-556.

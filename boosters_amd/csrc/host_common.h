@@ -209,7 +209,28 @@ struct Mapper {
   uint32_t n_bins = 0;
   int missing = 0;  // 0 None, 2 NaN
   uint32_t default_bin = 0;
+  int64_t n_nonzero = 0;  // values != 0 and not NaN (the density of sparsity detection)
 };
+
+// One Exclusive Feature Bundle: sparse categorical features sharing one
+// column whose code is offsets[j] + bin for a row where feature j (the first
+// in bundle order that is non-zero there) is set, 0 where none is.
+struct Bundle {
+  std::vector<uint32_t> features;  // original feature indices, bundle order
+  std::vector<uint32_t> offsets;   // code of bin 0 of each feature (the first is 1)
+  uint32_t total_bins = 0;         // 1 + the features' bins
+};
+
+// The bundles the reference builds for a [F][n] host matrix
+// (create_bundle_plan): empty when fewer than two sparse categorical
+// features, more than 1000, or too many conflicting pairs.
+std::vector<Bundle> plan_bundles(const float* X, int64_t n, int32_t F,
+                                 const std::vector<Mapper>& maps, const bst_binning_config& cfg);
+// A bundle's codes for every row (total_bins <= 256).
+void encode_bundle(const Bundle& b, const float* X, int64_t n, const std::vector<Mapper>& maps,
+                   uint8_t* out);
+// BinMapper::value_to_bin of a categorical feature for a non-NaN value.
+uint32_t cat_bin(const Mapper& m, float v);
 
 // Build bin mappers for every feature of a [F][n] host matrix
 // (BinnedDataset::from_dataset, data/binned/dataset.rs:207-309).
@@ -312,7 +333,9 @@ struct bst_ctx {
 struct bst_dataset {
   bst_ctx* ctx = nullptr;
   std::vector<bst::Mapper> mappers;      // per original feature
-  std::vector<uint32_t> eff_to_orig;     // effective column -> original feature
+  std::vector<uint32_t> eff_to_orig;     // effective column -> original feature (a
+                                         // bundle's: its first feature)
+  std::vector<bst::Bundle> bundles;      // effective columns 0 .. bundles.size() - 1
   std::vector<bst::FeatMeta> fmeta;      // per effective column (host copy)
   std::vector<float> thr_up;             // [E][256]: next_up((f32)bound[bin])
   bst::DatasetDev d;

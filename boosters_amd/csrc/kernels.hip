@@ -1612,6 +1612,21 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
       }
     }
   }
+  // A categorical column with a missing bin (an exclusive feature bundle:
+  // code 0, every feature of the bundle at its default) routes that bin by
+  // default_left, which a categorical split leaves false (find.rs:472-573),
+  // so its rows go right even when the category set counted them left.  The
+  // left sums kept for the children are those of the rows that do go left,
+  // as the reference takes the children's totals from their own rows
+  // (grower.rs:424-440); the gain keeps the set's sums, as the reference's.
+  if (fm.is_cat && fm.has_missing && best && out.flags == 2u) {
+    const uint32_t mb = fm.default_bin;
+    if (mb < 256u && ((out.cats[mb >> 5] >> (mb & 31)) & 1u)) {
+      const longlong2 v = load(mb);
+      out.lg -= v.x;
+      out.lh -= v.y;
+    }
+  }
   if (lane == 0) cand[(int64_t)node * E + f] = out;
   if (!best) return 0ull;
   const uint32_t tb = p.rightmost ? f : (0xFFFFFFFFu - f);

@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <queue>
 #include <string>
 #include <unordered_set>
@@ -175,8 +176,15 @@ struct ora_binned {
   std::vector<uint32_t> eff_to_orig;   // effective column -> original feature
   std::vector<uint32_t> eff_nbins, eff_offset;
   std::vector<uint8_t> eff_cat, eff_missing;
+  std::vector<uint32_t> eff_default;        // missing bin of each effective column
   std::vector<std::vector<uint16_t>> bins;  // per effective column [n]
   uint32_t total_bins = 0;
+  // Exclusive feature bundles: effective columns 0 .. bundles.size() - 1.
+  struct Bundle {
+    std::vector<uint32_t> features, offsets;  // original features, code of their bin 0
+    uint32_t total_bins = 0;
+  };
+  std::vector<Bundle> bundles;
 };
 
 namespace {
@@ -186,13 +194,14 @@ namespace {
 struct Analysis {
   bool trivial, numeric;
   int64_t n_nan;
+  int64_t n_nonzero;  // FeatureStats::non_zero_count (not NaN, != 0)
 };
 
 Analysis analyze(const float* col, int64_t n, bool user_cat, const bst_binning_config& cfg) {
   size_t max_unique = std::max<size_t>(cfg.max_categorical_cardinality, cfg.max_bins) + 10;
   std::vector<float> uniq;
   bool exceeded = false, all_int = true;
-  int64_t valid = 0, nan = 0;
+  int64_t valid = 0, nan = 0, nonzero = 0;
   for (int64_t i = 0; i < n; i++) {
     float v = col[i];
     if (std::isnan(v)) {
@@ -200,6 +209,7 @@ Analysis analyze(const float* col, int64_t n, bool user_cat, const bst_binning_c
       continue;
     }
     valid++;
+    if (v != 0.0f) nonzero++;
     if (all_int && (v - std::trunc(v)) != 0.0f) all_int = false;
     if (!exceeded && std::find(uniq.begin(), uniq.end(), v) == uniq.end()) {
       if (uniq.size() >= max_unique)
@@ -215,7 +225,161 @@ Analysis analyze(const float* col, int64_t n, bool user_cat, const bst_binning_c
   bool auto_cat = !user_cat && all_int && nu <= cfg.max_categorical_cardinality;
   a.numeric = user_cat ? false : (binary ? true : !auto_cat);
   a.n_nan = nan;
+  a.n_nonzero = nonzero;
   return a;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive Feature Bundling (data/binned/bundling.rs)
+// ---------------------------------------------------------------------------
+// The conflict-graph row sample draws from two crates the reference pins in
+// Cargo.lock and that are not in its tree: rand_xoshiro 0.6.0
+// (Xoshiro256PlusPlus::seed_from_u64 fills the 256-bit state with four
+// SplitMix64 outputs of the seed; next_u64 is Blackman and Vigna's
+// xoshiro256++) and rand 0.8.5 (gen_range(0..=i) on usize is
+// UniformInt::sample_single_inclusive: the high word of the 128-bit product of
+// a draw and the range, accepted when the low word is at most the
+// conservative zone (range << leading_zeros(range)) - 1).
+// (splitmix64_next above is the same SplitMix64 step.)
+struct Xoshiro256PlusPlus {
+  uint64_t s[4];
+  static Xoshiro256PlusPlus seed_from_u64(uint64_t seed) {
+    Xoshiro256PlusPlus r;
+    uint64_t sm = seed;
+    for (int i = 0; i < 4; i++) r.s[i] = splitmix64_next(sm);  // little-endian words of the seed
+    return r;
+  }
+  uint64_t next_u64() {
+    auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+    const uint64_t result = rotl(s[0] + s[3], 23) + s[0];
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return result;
+  }
+};
+
+template <class Draw>
+uint64_t sample_single_inclusive(uint64_t low, uint64_t high, Draw draw) {
+  const uint64_t range = high - low + 1;
+  if (range == 0) return draw();
+  const uint64_t zone = (range << __builtin_clzll(range)) - 1;
+  for (;;) {
+    const unsigned __int128 m = (unsigned __int128)draw() * range;
+    if ((uint64_t)m <= zone) return low + (uint64_t)(m >> 64);
+  }
+}
+
+// sample_rows, bundling.rs:203-251.
+std::vector<size_t> efb_sample_rows(size_t n_rows, size_t max_samples, uint64_t seed) {
+  std::vector<size_t> result;
+  if (n_rows <= max_samples) {
+    for (size_t i = 0; i < n_rows; i++) result.push_back(i);
+    return result;
+  }
+  const size_t n_first = max_samples / 10, n_last = max_samples / 10;
+  const size_t n_random = max_samples - n_first - n_last;
+  for (size_t i = 0; i < std::min(n_first, n_rows); i++) result.push_back(i);
+  const size_t last_start = n_rows > n_last ? n_rows - n_last : 0;
+  for (size_t i = last_start; i < n_rows; i++) result.push_back(i);
+  const size_t middle_start = n_first, middle_end = last_start;
+  if (middle_end > middle_start) {
+    Xoshiro256PlusPlus rng = Xoshiro256PlusPlus::seed_from_u64(seed);
+    std::vector<size_t> middle;
+    for (size_t i = middle_start; i < middle_end; i++) middle.push_back(i);
+    std::vector<size_t> selected(middle.begin(),
+                                 middle.begin() + (ptrdiff_t)std::min(n_random, middle.size()));
+    for (size_t i = selected.size(); i < middle.size(); i++) {
+      const size_t j = (size_t)sample_single_inclusive(0, i, [&] { return rng.next_u64(); });
+      if (j < selected.size()) selected[j] = middle[i];
+    }
+    result.insert(result.end(), selected.begin(), selected.end());
+  }
+  std::sort(result.begin(), result.end());
+  result.erase(std::unique(result.begin(), result.end()), result.end());
+  return result;
+}
+
+// create_bundle_plan, bundling.rs:393-485 (ConflictGraph::build :91-165,
+// assign_bundles :277-339), BundlingConfig defaults (:55-65) with
+// min_sparsity = the binning config's sparsity threshold (dataset.rs:229-236).
+// Returns the bundles of two or more features, in creation order, each in
+// insertion order (original feature indices).
+std::vector<std::vector<uint32_t>> create_bundle_plan(const std::vector<Analysis>& an,
+                                                      const std::vector<Mapper>& maps,
+                                                      const float* X, int64_t n,
+                                                      const bst_binning_config& cfg) {
+  const float max_conflict_rate = 0.0001f;
+  const size_t max_bundle_size = 256, max_sparse_features = 1000, max_sample_rows = 10000;
+  auto density = [&](size_t f) { return (float)an[f].n_nonzero / (float)n; };
+  const float sparse_threshold = 1.0f - cfg.sparsity_threshold;
+  std::vector<size_t> sparse;  // original indices of sparse categorical features
+  for (size_t f = 0; f < an.size(); f++)
+    if (!maps[f].trivial && !an[f].numeric && density(f) <= sparse_threshold) sparse.push_back(f);
+  if (sparse.size() < 2 || sparse.size() > max_sparse_features) return {};
+  const std::vector<size_t> rows = efb_sample_rows((size_t)n, max_sample_rows, 42);
+  // Bitset per feature over the sampled rows, conflicts = |intersection|.
+  const size_t m = sparse.size();
+  std::vector<std::vector<bool>> bits(m, std::vector<bool>(rows.size(), false));
+  for (size_t i = 0; i < m; i++)
+    for (size_t r = 0; r < rows.size(); r++) {
+      const float v = X[(int64_t)sparse[i] * n + (int64_t)rows[r]];
+      bits[i][r] = v != 0.0f && !std::isnan(v);
+    }
+  std::map<std::pair<size_t, size_t>, size_t> conflicts;
+  for (size_t i = 0; i < m; i++)
+    for (size_t j = i + 1; j < m; j++) {
+      size_t c = 0;
+      for (size_t r = 0; r < rows.size(); r++) c += bits[i][r] && bits[j][r];
+      if (c > 0) conflicts[{i, j}] = c;
+    }
+  auto get_conflict = [&](size_t i, size_t j) {
+    auto it = conflicts.find(i < j ? std::make_pair(i, j) : std::make_pair(j, i));
+    return it == conflicts.end() ? (size_t)0 : it->second;
+  };
+  const size_t total_pairs = m * (m - 1) / 2;
+  if ((float)conflicts.size() / (float)total_pairs > 0.5f) return {};
+  const size_t max_conflicts = (size_t)std::ceil(max_conflict_rate * (float)rows.size());
+  // Denser first; sort_by is stable.
+  std::vector<size_t> sorted(m);
+  for (size_t i = 0; i < m; i++) sorted[i] = i;
+  std::stable_sort(sorted.begin(), sorted.end(), [&](size_t a, size_t b) {
+    return density(sparse[b]) < density(sparse[a]);
+  });
+  std::vector<std::vector<size_t>> bundles;  // sparse positions
+  std::vector<size_t> bundle_conflicts;
+  for (size_t p : sorted) {
+    size_t best_bundle = SIZE_MAX, best_conflict_increase = SIZE_MAX;
+    for (size_t b = 0; b < bundles.size(); b++) {
+      if (bundles[b].size() >= max_bundle_size) continue;
+      size_t conflict_sum = 0;
+      for (size_t q : bundles[b]) conflict_sum += get_conflict(p, q);
+      if (bundle_conflicts[b] + conflict_sum <= max_conflicts &&
+          conflict_sum < best_conflict_increase) {
+        best_bundle = b;
+        best_conflict_increase = conflict_sum;
+      }
+    }
+    if (best_bundle != SIZE_MAX) {
+      bundles[best_bundle].push_back(p);
+      bundle_conflicts[best_bundle] += best_conflict_increase;
+    } else {
+      bundles.push_back({p});
+      bundle_conflicts.push_back(0);
+    }
+  }
+  std::vector<std::vector<uint32_t>> out;
+  for (auto& b : bundles)
+    if (b.size() >= 2) {
+      std::vector<uint32_t> fs;
+      for (size_t p : b) fs.push_back((uint32_t)sparse[p]);
+      out.push_back(fs);
+    }
+  return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -745,9 +909,9 @@ void build_hist(const ora_binned& B, const float* g, const float* h, const uint3
 // Goes-left predicate, partition.rs:321-337.
 inline bool goes_left(const ora_binned& B, const ora_split& s, uint32_t bin) {
   uint32_t f = s.feature;
-  const Mapper& m = B.mappers[B.eff_to_orig[f]];
-  bool has_missing = m.missing != 0;
-  if (has_missing && bin == m.default_bin) return s.default_left != 0;
+  // Bundles: missing bin 0 (partition.rs:311-318); other columns: the mapper's.
+  bool has_missing = B.eff_missing[f] != 0;
+  if (has_missing && bin == B.eff_default[f]) return s.default_left != 0;
   if (s.is_categorical) return bin < 256 && ((s.left_cats[bin >> 5] >> (bin & 31)) & 1u);
   return bin <= s.bin;
 }
@@ -1455,21 +1619,59 @@ int ora_bin(const float* X, int64_t n, int32_t F, const uint8_t* is_cat,
     m.trivial = an[f].trivial;
     B->mappers[f] = m;
   }
+  // Optional EFB (dataset.rs:229-236); the effective columns are the bundles
+  // first, then the other non-trivial features (feature_views, dataset.rs:460-520).
+  std::vector<uint8_t> bundled((size_t)F, 0);
   uint32_t off = 0;
+  if (cfg.enable_bundling) {
+    for (const auto& fs : create_bundle_plan(an, B->mappers, X, n, cfg)) {
+      // build_bundle, dataset.rs:898-952: codes offset + bin, offsets from 1.
+      ora_binned::Bundle bd;
+      bd.features = fs;
+      uint32_t o = 1;
+      for (uint32_t f : fs) {
+        bd.offsets.push_back(o);
+        o += B->mappers[f].n_bins;
+        bundled[f] = 1;
+      }
+      bd.total_bins = o;
+      B->bundles.push_back(bd);
+      B->eff_to_orig.push_back(fs[0]);  // decode_split_to_original: the bundle's first feature
+      B->eff_nbins.push_back(o);
+      B->eff_offset.push_back(off);
+      B->eff_cat.push_back(1);      // bundles are categorical columns ...
+      B->eff_missing.push_back(1);  // ... whose code 0 (all defaults) is the missing bin
+      B->eff_default.push_back(0);
+      off += o;
+    }
+  }
+  const size_t n_bundles = B->bundles.size();
   for (int32_t f = 0; f < F; f++) {
-    if (B->mappers[f].trivial) continue;
+    if (B->mappers[f].trivial || bundled[f]) continue;
     B->eff_to_orig.push_back((uint32_t)f);
     B->eff_nbins.push_back(B->mappers[f].n_bins);
     B->eff_offset.push_back(off);
     B->eff_cat.push_back(B->mappers[f].categorical);
     B->eff_missing.push_back(B->mappers[f].missing != 0);
+    B->eff_default.push_back(B->mappers[f].default_bin);
     off += B->mappers[f].n_bins;
   }
   B->total_bins = off;
   size_t E = B->eff_to_orig.size();
   B->bins.resize(E);
+  for (size_t e = 0; e < n_bundles; e++) {
+    const auto& bd = B->bundles[e];
+    auto& v = B->bins[e];
+    v.assign((size_t)n, 0);
+    for (size_t k = 0; k < bd.features.size(); k++) {
+      const float* col = X + (int64_t)bd.features[k] * n;
+      for (int64_t i = 0; i < n; i++)
+        if (v[i] == 0 && col[i] != 0.0f && !std::isnan(col[i]))
+          v[i] = (uint16_t)(bd.offsets[k] + value_to_bin(B->mappers[bd.features[k]], (double)col[i]));
+    }
+  }
 #pragma omp parallel for schedule(dynamic, 1)
-  for (size_t e = 0; e < E; e++) {
+  for (size_t e = n_bundles; e < E; e++) {
     const Mapper& m = B->mappers[B->eff_to_orig[e]];
     const float* col = X + (int64_t)B->eff_to_orig[e] * n;
     auto& v = B->bins[e];
@@ -1518,7 +1720,22 @@ int ora_bin_bounds(const ora_binned* b, int32_t f, double* out, int64_t cap) {
 int ora_get_bins(const ora_binned* b, uint16_t* out) {
   if (!b || !out) return fail(BST_E_INVALID, "null");
   std::fill(out, out + (size_t)b->F * b->n, 0);
-  for (size_t e = 0; e < b->eff_to_orig.size(); e++)
+  // A bundled feature: its own codes decoded (BundleStorage::decode), any
+  // other code stands for the value 0.0.
+  for (size_t e = 0; e < b->bundles.size(); e++) {
+    const auto& bd = b->bundles[e];
+    for (size_t k = 0; k < bd.features.size(); k++) {
+      const Mapper& m = b->mappers[bd.features[k]];
+      const uint32_t lo = bd.offsets[k], hi = lo + m.n_bins;
+      const uint16_t zero_bin = (uint16_t)value_to_bin(m, 0.0);
+      uint16_t* o = out + (size_t)bd.features[k] * b->n;
+      for (int64_t r = 0; r < b->n; r++) {
+        const uint32_t c = b->bins[e][r];
+        o[r] = c >= lo && c < hi ? (uint16_t)(c - lo) : zero_bin;
+      }
+    }
+  }
+  for (size_t e = b->bundles.size(); e < b->eff_to_orig.size(); e++)
     std::copy(b->bins[e].begin(), b->bins[e].end(), out + (size_t)b->eff_to_orig[e] * b->n);
   return BST_OK;
 }
@@ -1948,9 +2165,9 @@ int ora_partition_predict(const ora_binned* b, const bst_model_info* info,
         while (!f->is_leaf[base + node]) {
           const int64_t k = base + node;
           const uint32_t of = f->split_index[k];
-          const Mapper& m = b->mappers[of];
-          const uint32_t bin = b->bins[orig_to_eff[of]][r];
-          const bool missing = m.missing != 0 && bin == m.default_bin;
+          const int32_t e = orig_to_eff[of];
+          const uint32_t bin = b->bins[e][r];
+          const bool missing = b->eff_missing[e] != 0 && bin == b->eff_default[e];
           bool go_left;
           if (!f->split_type[k]) {
             go_left = missing ? f->default_left[k] != 0 : bin <= (uint32_t)split_bin[k];
@@ -1974,6 +2191,50 @@ int ora_partition_predict(const ora_binned* b, const bst_model_info* info,
 
 void ora_splitmix64_stream(uint64_t state, int64_t count, uint64_t* out) {
   for (int64_t i = 0; i < count; i++) out[i] = splitmix64_next(state);
+}
+
+void ora_xoshiro256pp(const uint64_t* state, uint64_t seed, int64_t count, uint64_t* out) {
+  Xoshiro256PlusPlus r = Xoshiro256PlusPlus::seed_from_u64(seed);
+  if (state) std::copy(state, state + 4, r.s);
+  for (int64_t i = 0; i < count; i++) out[i] = r.next_u64();
+}
+
+int ora_sample_inclusive(const uint64_t* draws, int64_t n_draws, uint64_t low, uint64_t high,
+                         uint64_t* value, int64_t* used) {
+  if (!draws || !value || low > high) return fail(BST_E_INVALID, "bad argument");
+  int64_t k = 0;
+  bool ran_out = false;
+  const uint64_t v = sample_single_inclusive(low, high, [&] {
+    if (k >= n_draws) {
+      ran_out = true;
+      return (uint64_t)0;  // the low word of 0 * range is 0: accepted
+    }
+    return draws[k++];
+  });
+  if (ran_out) return fail(BST_E_INVALID, "every draw was rejected");
+  *value = v;
+  if (used) *used = k;
+  return BST_OK;
+}
+
+int64_t ora_efb_sample_rows(int64_t n, int64_t max_samples, uint64_t seed, int64_t* out,
+                            int64_t cap) {
+  const std::vector<size_t> r = efb_sample_rows((size_t)n, (size_t)max_samples, seed);
+  for (size_t i = 0; i < r.size() && (int64_t)i < cap; i++) out[i] = (int64_t)r[i];
+  return (int64_t)r.size();
+}
+
+int32_t ora_bundles(const ora_binned* b, int32_t* sizes, uint32_t* features, int64_t cap) {
+  if (!b) return fail(BST_E_INVALID, "null");
+  int64_t k = 0;
+  for (size_t i = 0; i < b->bundles.size(); i++) {
+    if (sizes) sizes[i] = (int32_t)b->bundles[i].features.size();
+    for (uint32_t f : b->bundles[i].features) {
+      if (features && k < cap) features[k] = f;
+      k++;
+    }
+  }
+  return (int32_t)b->bundles.size();
 }
 
 int64_t ora_binning_sample(int64_t n, uint64_t cnt, uint32_t* out) {

@@ -120,6 +120,17 @@ int ora_partition_predict(const ora_binned* b, const bst_model_info* info,
  * (the mixer of dataset.rs:598-629's edge sample); the edge sample itself. */
 void ora_splitmix64_stream(uint64_t state, int64_t count, uint64_t* out);
 int64_t ora_binning_sample(int64_t n, uint64_t cnt, uint32_t* out);
+/* The EFB row sample's generator (bundling.rs:203-251): xoshiro256++ outputs
+ * from an explicit state [4] (nullable: seed_from_u64(seed) instead), rand's
+ * inclusive range sampling over scripted draws (value, draws consumed), the
+ * row sample, and the bundles a binned dataset made (sizes [n_bundles],
+ * features concatenated, original indices). */
+void ora_xoshiro256pp(const uint64_t* state, uint64_t seed, int64_t count, uint64_t* out);
+int ora_sample_inclusive(const uint64_t* draws, int64_t n_draws, uint64_t low, uint64_t high,
+                         uint64_t* value, int64_t* used);
+int64_t ora_efb_sample_rows(int64_t n, int64_t max_samples, uint64_t seed, int64_t* out,
+                            int64_t cap);
+int32_t ora_bundles(const ora_binned* b, int32_t* sizes, uint32_t* features, int64_t cap);
 /* The row sample of (round, output) as ora_train draws it: uniform
  * (row.rs:170-208) or GOSS (row.rs:211-270, after the warm-up of :109-112).
  * g_out/h_out: the edited pairs; rows_out [n] the kept rows in order;

@@ -95,6 +95,12 @@ def _load(path):
         "ora_partition_predict": (C.c_int, [vp, P(A.ModelInfo), P(A.ForestArrays),
                                             P(C.c_uint8), P(C.c_float), P(C.c_float)]),
         "ora_binning_sample": (C.c_int64, [C.c_int64, C.c_uint64, P(C.c_uint32)]),
+        "ora_xoshiro256pp": (None, [P(C.c_uint64), C.c_uint64, C.c_int64, P(C.c_uint64)]),
+        "ora_sample_inclusive": (C.c_int, [P(C.c_uint64), C.c_int64, C.c_uint64, C.c_uint64,
+                                           P(C.c_uint64), P(C.c_int64)]),
+        "ora_efb_sample_rows": (C.c_int64, [C.c_int64, C.c_int64, C.c_uint64, P(C.c_int64),
+                                            C.c_int64]),
+        "ora_bundles": (C.c_int32, [vp, P(C.c_int32), P(C.c_uint32), C.c_int64]),
         "ora_sample_rows": (C.c_int, [P(C.c_float), P(C.c_float), C.c_int64, P(A.GBDTConfigC),
                                       P(A.ExtConfig), C.c_uint32, C.c_uint32, P(C.c_float),
                                       P(C.c_float), P(C.c_uint32), P(C.c_int64)]),
@@ -150,6 +156,18 @@ class Binned:
         if getattr(self, "h", None):
             lib().ora_binned_free(self.h)
             self.h = None
+
+    def bundles(self):
+        """The exclusive feature bundles (original feature indices, bundle order)."""
+        n = lib().ora_bundles(self.h, None, None, 0)
+        sizes = np.zeros(max(n, 1), np.int32)
+        feats = np.zeros(self.n_feat, np.uint32)
+        lib().ora_bundles(self.h, A.ptr(sizes), A.ptr(feats), len(feats))
+        out, k = [], 0
+        for s in sizes[:n]:
+            out.append(feats[k:k + s].tolist())
+            k += s
+        return out
 
     def feature_info(self, f):
         nb, mt, db, cat, tr = C.c_uint32(), C.c_int32(), C.c_uint32(), C.c_int32(), C.c_int32()
@@ -371,6 +389,31 @@ def splitmix64_stream(state, count):
     out = np.zeros(count, np.uint64)
     lib().ora_splitmix64_stream(C.c_uint64(state), count, out.ctypes.data_as(C.POINTER(C.c_uint64)))
     return out
+
+
+def xoshiro256pp(count, seed=0, state=None):
+    """xoshiro256++ outputs from `state` (4 words) or from seed_from_u64(seed)."""
+    out = np.zeros(count, np.uint64)
+    st = None if state is None else np.ascontiguousarray(state, np.uint64)
+    lib().ora_xoshiro256pp(None if st is None else st.ctypes.data_as(C.POINTER(C.c_uint64)),
+                           C.c_uint64(seed), count, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
+
+
+def sample_inclusive(draws, low, high):
+    """rand's gen_range(low..=high) over the scripted u64 draws: (value, draws used)."""
+    d = np.ascontiguousarray(draws, np.uint64)
+    v, used = C.c_uint64(), C.c_int64()
+    _check(lib().ora_sample_inclusive(d.ctypes.data_as(C.POINTER(C.c_uint64)), len(d),
+                                      C.c_uint64(low), C.c_uint64(high), C.byref(v), C.byref(used)))
+    return v.value, used.value
+
+
+def efb_sample_rows(n, max_samples, seed=42):
+    out = np.zeros(max(min(int(n), int(max_samples)), 1), np.int64)
+    k = lib().ora_efb_sample_rows(int(n), int(max_samples), C.c_uint64(seed),
+                                  out.ctypes.data_as(C.POINTER(C.c_int64)), len(out))
+    return out[:k].copy()
 
 
 def binning_sample(n, cnt):

@@ -137,3 +137,32 @@ def ranking_data(n, F, seed=0, zero_p=0.3, group_lo=5, group_hi=500):
     q = np.quantile(s, [0.2, 0.4, 0.6, 0.8])
     y = np.searchsorted(q, s, side="right").astype(np.float32)
     return X, y, np.array(sizes, np.int32)
+
+
+def sparse_cat_data(n, n_dense, n_sparse, card=6, active=0.05, conflicts=0, seed=0):
+    """Dense N(0,1) features followed by sparse categorical ones: each row is
+    set (a category 1..card-1; 0 is the default) in at most one sparse
+    feature, each feature on about `active` of the rows, plus `conflicts`
+    rows set in the first two sparse features at once.  y = x0 + a per
+    (feature, category) effect + N(0, 0.1^2).  Returns X, y, is_cat."""
+    rng = np.random.default_rng(seed)
+    Xd = rng.standard_normal((n_dense, n), dtype=np.float32)
+    Xs = np.zeros((n_sparse, n), np.float32)
+    owner = rng.integers(0, n_sparse, n)
+    on = rng.random(n) < active * n_sparse
+    cats = rng.integers(1, card, n).astype(np.float32)
+    Xs[owner[on], np.nonzero(on)[0]] = cats[on]
+    if conflicts and n_sparse >= 2:
+        rows = rng.choice(np.nonzero(~on)[0], conflicts, replace=False)  # rows set nowhere else
+        Xs[0, rows] = rng.integers(1, card, conflicts)
+        Xs[1, rows] = rng.integers(1, card, conflicts)
+    eff = rng.normal(0, 1.5, (n_sparse, card))
+    eff[:, 0] = 0.0
+    y = (Xd[0] if n_dense else np.zeros(n, np.float32)).astype(np.float64)
+    for j in range(n_sparse):
+        y += eff[j, Xs[j].astype(np.int64)]
+    y = (y + rng.normal(0, 0.1, n)).astype(np.float32)
+    X = np.concatenate([Xd, Xs], axis=0)
+    is_cat = np.zeros(n_dense + n_sparse, np.uint8)
+    is_cat[n_dense:] = 1
+    return X, y, is_cat
