@@ -263,8 +263,8 @@ int validate(const bst_gbdt_config* c) {
     raise(BST_E_CONFIG, "metric not supported by this engine build");
   if (c->metric == BST_METRIC_HUBER && !(c->metric_huber_delta > 0.0))
     raise(BST_E_CONFIG, "Huber metric delta must be positive");
-  if (c->objective == BST_OBJ_SOFTMAX && (c->n_classes < 2 || c->n_classes > kPredMaxGroups))
-    raise(BST_E_CONFIG, "softmax needs 2..16 classes");
+  if (c->objective == BST_OBJ_SOFTMAX && (c->n_classes < 2 || c->n_classes > kMaxOutputs))
+    raise(BST_E_CONFIG, "softmax needs 2..1024 classes");
   if (c->growth == BST_GROW_DEPTHWISE && c->max_depth > 16) raise(BST_E_CONFIG, "max_depth > 16");
   if (c->growth == BST_GROW_LEAFWISE && (c->max_leaves < 1 || c->max_leaves > 65536))
     raise(BST_E_CONFIG, "max_leaves must be in [1, 65536]");
@@ -910,7 +910,7 @@ int bst_model_import(bst_ctx* ctx, const bst_model_info* info, const bst_forest_
                      bst_model** out) {
   return guard([&] {
     need(info && in && out, BST_E_INVALID, "null argument");
-    need(info->n_trees >= 0 && info->n_groups >= 1 && info->n_groups <= kPredMaxGroups &&
+    need(info->n_trees >= 0 && info->n_groups >= 1 && info->n_groups <= kMaxOutputs &&
              info->total_nodes >= 0 && info->total_cat_words >= 0 && info->n_features >= 0,
          BST_E_FORMAT, "bad model shape");
     std::unique_ptr<bst_model> m(new bst_model());
@@ -1112,8 +1112,8 @@ int bst_kernel_gradients(bst_ctx* ctx, int32_t obj, int32_t K, const float* pred
          "objective");
     need(obj != BST_OBJ_PINBALL, BST_E_CONFIG,
          "pinball needs its alphas: use bst_kernel_gradients_cfg");
-    need(K >= 1 && K <= 16 && (obj == BST_OBJ_SOFTMAX || K == 1), BST_E_CONFIG,
-         "outputs must be 1, or 2..16 for softmax");
+    need(K >= 1 && K <= kMaxOutputs && (obj == BST_OBJ_SOFTMAX || K == 1), BST_E_CONFIG,
+         "outputs must be 1, or 2..1024 for softmax");
     return gradients_impl(ctx, obj, K, ObjParams(), preds, y, w, n, g, h);
   });
 }

@@ -365,7 +365,11 @@ struct PredModel {
   int has_cat;     // some compact node is a categorical split
   int64_t n_trees;
   const float* base;
-  int K;
+  int K;   // outputs of the model
+  // The outputs one launch accumulates, g0 .. g0 + Kw - 1 (Kw <= kPredMaxK =
+  // 16 registers per row): a model of more outputs is predicted in windows,
+  // each launch adding only its window's trees.
+  int g0, Kw;
 };
 // ---- serving: one persistent traversal launch --------------------------------
 // The serving kernel keeps one block per CU resident and takes 256-row tiles
@@ -436,6 +440,7 @@ void launch_predict_accum(hipStream_t s, const PredModel& m, const float* X, int
 bool pred_two_phase(const PredModel& m, int64_t n, int n_cu, size_t lds_limit,
                     int64_t* scratch_floats);
 void launch_copy16(hipStream_t s, void* dst, const void* src, int64_t bytes);
-constexpr int kPredMaxGroups = 16;
+constexpr int kPredMaxGroups = 16;   // outputs one prediction launch (and the serving loop) holds
+constexpr int kMaxOutputs = 1024;    // outputs of a model (softmax classes)
 
 }  // namespace bst

@@ -394,6 +394,8 @@ struct bst_model {
     v.n_trees = info.n_trees;
     v.base = d_base.p;
     v.K = (int)info.n_groups;
+    v.g0 = 0;
+    v.Kw = v.K < bst::kPredMaxGroups ? v.K : bst::kPredMaxGroups;
     return v;
   }
 };

@@ -368,15 +368,56 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
   }
 }
 
+// Softmax with more classes than k_gradients keeps in registers
+// (classification.rs:141-187): per row the same f32 operations in class order
+// (the max, the sum of exps, each class's probability), each class's |g| and
+// |h| maxima in LDS, then one atomicMax per value and block.
+__global__ __launch_bounds__(kGradThreads) void k_softmax_wide(
+    const float* __restrict__ preds, const float* __restrict__ y, const float* __restrict__ w,
+    int64_t n, int K, float* __restrict__ g, float* __restrict__ h, uint32_t* __restrict__ absmax) {
+  extern __shared__ uint32_t s_wmax[];  // [2 K]
+  const float HESS_MIN = 1e-6f;
+  for (int i = threadIdx.x; i < 2 * K; i += blockDim.x) s_wmax[i] = 0u;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float wi = w ? w[i] : 1.0f;
+    const int label = (int)y[i];
+    float mx = -INFINITY;
+    for (int k = 0; k < K; k++) mx = fmaxf(mx, preds[(int64_t)k * n + i]);
+    float s = 0.0f;
+    for (int k = 0; k < K; k++) s += exp32(preds[(int64_t)k * n + i] - mx);
+    for (int k = 0; k < K; k++) {
+      const float p = exp32(preds[(int64_t)k * n + i] - mx) / s;
+      const float ind = k == label ? 1.0f : 0.0f;
+      const float gv = wi * (p - ind);
+      const float hv = fmaxf(wi * p * (1.0f - p), HESS_MIN);
+      g[(int64_t)k * n + i] = gv;
+      h[(int64_t)k * n + i] = hv;
+      atomicMax(&s_wmax[2 * k], abs_bits(gv));
+      atomicMax(&s_wmax[2 * k + 1], abs_bits(hv));
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * K; i += blockDim.x)
+    if (s_wmax[i]) atomicMax(&absmax[i], s_wmax[i]);
+}
+
 void launch_gradients(hipStream_t s, int objective, const ObjParams& op, const float* preds,
                       const float* y, const float* w, int64_t n, int K, float* g, float* h,
                       uint32_t* absmax) {
-  // Callers validate 1 <= K <= kGradMaxK (softmax allows at most 16 classes).
-  if (n <= 0 || K < 1 || K > kGradMaxK) return;
+  // Callers validate 1 <= K <= kMaxOutputs, and K <= kGradMaxK unless softmax.
+  if (n <= 0 || K < 1) return;
   // At most 256 blocks: each ends with one atomicMax per value on the same
   // words, and same-address atomics serialise in L2.
   int64_t blocks = (n + kGradThreads - 1) / kGradThreads;
   if (blocks > 256) blocks = 256;
+  if (K > kGradMaxK) {
+    if (objective != 2) return;
+    hipLaunchKernelGGL(k_softmax_wide, dim3((unsigned)blocks), dim3(kGradThreads),
+                       (size_t)2 * K * sizeof(uint32_t), s, preds, y, w, n, K, g, h, absmax);
+    return;
+  }
   if (K == 1 && objective != 2)
     hipLaunchKernelGGL(k_gradients<1>, dim3((unsigned)blocks), dim3(kGradThreads), 0, s,
                        objective, op, preds, y, w, n, K, g, h, absmax);
@@ -597,15 +638,15 @@ void launch_lambdarank(hipStream_t s, const float* preds, const float* y, const 
 
 // Scale exponent per (tree, g/h): largest e with n * max|x| * 2^e <= 2^62.
 __global__ void k_qscale(uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
-  const int i = threadIdx.x;
-  if (i >= 2 * K) return;
-  qexp[i] = qscale_exponent(__uint_as_float(absmax[i]), n_rows);
-  absmax[i] = 0u;  // ready for the next round's gradients
+  for (int i = threadIdx.x; i < 2 * K; i += blockDim.x) {
+    qexp[i] = qscale_exponent(__uint_as_float(absmax[i]), n_rows);
+    absmax[i] = 0u;  // ready for the next round's gradients
+  }
 }
 
 void launch_qscale(hipStream_t s, uint32_t* absmax, int K, int64_t n_rows, int32_t* qexp) {
-  hipLaunchKernelGGL(k_qscale, dim3(1), dim3(64 * ((2 * K + 63) / 64)), 0, s, absmax, K, n_rows,
-                     qexp);
+  const int threads = std::min(1024, 64 * ((2 * K + 63) / 64));
+  hipLaunchKernelGGL(k_qscale, dim3(1), dim3(threads), 0, s, absmax, K, n_rows, qexp);
 }
 
 // idx = row ids (sampled list or 0..m-1); ogh = gradient pairs in that order.
@@ -2266,7 +2307,8 @@ __device__ __forceinline__ void pred_block(
   const uint32_t* __restrict__ cat_size = m.cat_size;
   const uint32_t* __restrict__ cat_words = m.cat_words;
   const PredChunk* __restrict__ chunks = m.chunks;
-  const int K = m.K;
+  const int K = m.Kw;         // outputs accumulated here: m.g0 .. m.g0 + K - 1
+  const bool one = m.K == 1;  // the model has one output (every tree adds to it)
   // Rows are read from X only when the model tests features (a forest of
   // leaves may come with no feature rows at all).
   const bool xok = in && m.n_features > 0;
@@ -2323,7 +2365,8 @@ __device__ __forceinline__ void pred_block(
   if (MODE != 2) {
 #pragma unroll
     for (int k = 0; k < kPredMaxK; k++)
-      acc[k] = (k < K && in) ? (MODE == 1 ? out[(int64_t)k * out_stride + row] : m.base[k]) : 0.0f;
+      acc[k] = (k < K && in) ? (MODE == 1 ? out[(int64_t)k * out_stride + row] : m.base[m.g0 + k])
+                             : 0.0f;
   }
   if (c0 < c1) {
     const PredChunk ch = chunks[c0];
@@ -2421,10 +2464,12 @@ __device__ __forceinline__ void pred_block(
           if (in && leaf_idx) leaf_idx[(int64_t)t * leaf_stride + row] = leafid[j];
           if (MODE == 2) {
             if (in) leaf_out[(int64_t)t * n + row] = leafv[j];
-          } else if (K == 1) {
+          } else if (one) {
             acc[0] += leafv[j];
           } else {
-            const uint32_t g = tree_group[t];
+            // Trees of other windows match no accumulator (the difference
+            // wraps for groups below g0).
+            const uint32_t g = tree_group[t] - (uint32_t)m.g0;
 #pragma unroll
             for (int k = 0; k < kPredMaxK; k++)
               if ((uint32_t)k == g) acc[k] += leafv[j];
@@ -2563,22 +2608,46 @@ __global__ __launch_bounds__(256) void k_predict_sum(PredModel m, const float* _
   if (row >= n) return;
   float acc[kPredMaxK];
 #pragma unroll
-  for (int k = 0; k < kPredMaxK; k++) acc[k] = k < m.K ? m.base[k] : 0.0f;
+  for (int k = 0; k < kPredMaxK; k++) acc[k] = k < m.Kw ? m.base[m.g0 + k] : 0.0f;
   if (m.K == 1) {
     for (int64_t t = 0; t < m.n_trees; t++) acc[0] += leaf_in[t * n + row];
   } else {
     for (int64_t t = 0; t < m.n_trees; t++) {
-      const uint32_t g = m.tree_group[t];
+      const uint32_t g = m.tree_group[t] - (uint32_t)m.g0;
       const float v = leaf_in[t * n + row];
 #pragma unroll
       for (int k = 0; k < kPredMaxK; k++)
         if ((uint32_t)k == g) acc[k] += v;
     }
   }
-  pred_transform(acc, m.K, transform);
+  pred_transform(acc, m.Kw, transform);
 #pragma unroll
   for (int k = 0; k < kPredMaxK; k++)
-    if (k < m.K) out[(int64_t)k * out_stride + row] = acc[k];
+    if (k < m.Kw) out[(int64_t)k * out_stride + row] = acc[k];
+}
+
+// Output transform of rows whose outputs were accumulated in windows
+// (transform.rs:83-115): the same per-row f32 operations as pred_transform,
+// over any number of outputs.
+__global__ __launch_bounds__(256) void k_transform_rows(float* __restrict__ out, int64_t n,
+                                                        int64_t stride, int K, int transform) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  float* o = out + row;
+  if (transform == 1) {
+    for (int k = 0; k < K; k++) o[k * stride] = sigmoid32(o[k * stride]);
+  } else if (transform == 2) {
+    float mx = -INFINITY;
+    for (int k = 0; k < K; k++) mx = fmaxf(mx, o[k * stride]);
+    float ssum = 0.0f;
+    for (int k = 0; k < K; k++) {
+      const float e = exp32(o[k * stride] - mx);
+      o[k * stride] = e;
+      ssum += e;
+    }
+    if (ssum > 0.0f)
+      for (int k = 0; k < K; k++) o[k * stride] /= ssum;
+  }
 }
 
 size_t pred_lds_bytes(int F, size_t lds_limit, bool* stage_rows, int* rows_per_block) {
@@ -2652,28 +2721,64 @@ bool pred_two_phase(const PredModel& m, int64_t n, int n_cu, size_t lds_limit,
   return two;
 }
 
+// Every tree's leaf value per row into leaf_scratch [n_trees][n] (the first
+// phase of a two-phase launch: (row tile, chunk range) blocks).
+static void launch_leaf_values(hipStream_t s, const PredModel& m, const float* X, int64_t n,
+                               int64_t x_stride, uint32_t* leaf_idx, int64_t leaf_stride,
+                               float* leaf_scratch, size_t lds_limit, int n_cu) {
+  int R = kPredMaxRows;
+  (void)pred_lds_bytes(m.n_features, lds_limit, nullptr, &R);
+  const int64_t tiles = (n + R - 1) / R;
+  const int want = (int)std::min<int64_t>(m.n_chunks,
+                                          (pred_target_blocks(n_cu) + tiles - 1) / tiles);
+  const int per = (m.n_chunks + want - 1) / want;
+  const unsigned groups = (unsigned)((m.n_chunks + per - 1) / per);
+  launch_pred_lds<2>(s, m, groups, X, n, x_stride, 0, nullptr, 0, leaf_idx, leaf_stride,
+                     leaf_scratch, per, lds_limit);
+}
+
 void launch_predict_rows(hipStream_t s, const PredModel& m, const float* X, int64_t n,
                          int64_t x_stride, int F, int transform, float* out, int64_t out_stride,
                          uint32_t* leaf_idx, int64_t leaf_stride, float* leaf_scratch,
                          size_t lds_limit, int n_cu) {
   (void)F;  // rows stage the model's features (F >= m.n_features is checked by the caller)
   if (n <= 0) return;
-  if (leaf_scratch && m.n_chunks > 1) {
-    int R = kPredMaxRows;
-    (void)pred_lds_bytes(m.n_features, lds_limit, nullptr, &R);
-    const int64_t tiles = (n + R - 1) / R;
-    const int want = (int)std::min<int64_t>(m.n_chunks,
-                                            (pred_target_blocks(n_cu) + tiles - 1) / tiles);
-    const int per = (m.n_chunks + want - 1) / want;
-    const unsigned groups = (unsigned)((m.n_chunks + per - 1) / per);
-    launch_pred_lds<2>(s, m, groups, X, n, x_stride, transform, nullptr, 0, leaf_idx, leaf_stride,
-                       leaf_scratch, per, lds_limit);
-    hipLaunchKernelGGL(k_predict_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m,
-                       leaf_scratch, n, transform, out, out_stride);
+  const bool two = leaf_scratch && m.n_chunks > 1;
+  const unsigned sum_grid = (unsigned)((n + 255) / 256);
+  if (m.K <= kPredMaxK) {
+    if (two) {
+      launch_leaf_values(s, m, X, n, x_stride, leaf_idx, leaf_stride, leaf_scratch, lds_limit,
+                         n_cu);
+      hipLaunchKernelGGL(k_predict_sum, dim3(sum_grid), dim3(256), 0, s, m, leaf_scratch, n,
+                         transform, out, out_stride);
+    } else {
+      launch_pred_lds<0>(s, m, 1, X, n, x_stride, transform, out, out_stride, leaf_idx,
+                         leaf_stride, nullptr, m.n_chunks, lds_limit);
+    }
     return;
   }
-  launch_pred_lds<0>(s, m, 1, X, n, x_stride, transform, out, out_stride, leaf_idx, leaf_stride,
-                     nullptr, m.n_chunks, lds_limit);
+  // More outputs than one launch holds: windows of kPredMaxK outputs, each
+  // adding only its own trees (in forest order), raw; then the transform
+  // over all outputs of each row.  A two-phase launch takes the leaf values
+  // once and sums them per window.
+  if (two)
+    launch_leaf_values(s, m, X, n, x_stride, leaf_idx, leaf_stride, leaf_scratch, lds_limit, n_cu);
+  for (int g0 = 0; g0 < m.K; g0 += kPredMaxK) {
+    PredModel w = m;
+    w.g0 = g0;
+    w.Kw = std::min(kPredMaxK, m.K - g0);
+    float* wout = out + (int64_t)g0 * out_stride;
+    if (two)
+      hipLaunchKernelGGL(k_predict_sum, dim3(sum_grid), dim3(256), 0, s, w, leaf_scratch, n, 0,
+                         wout, out_stride);
+    else
+      launch_pred_lds<0>(s, w, 1, X, n, x_stride, 0, wout, out_stride,
+                         g0 == 0 ? leaf_idx : nullptr, leaf_stride, nullptr, m.n_chunks,
+                         lds_limit);
+  }
+  if (transform)
+    hipLaunchKernelGGL(k_transform_rows, dim3(sum_grid), dim3(256), 0, s, out, n, out_stride, m.K,
+                       transform);
 }
 
 void launch_predict(hipStream_t s, const PredModel& m, const float* X, int64_t n,
@@ -2705,8 +2810,13 @@ void launch_predict_accum(hipStream_t s, const PredModel& m, const float* X, int
                           size_t lds_limit) {
   (void)F;
   if (n <= 0) return;
-  launch_pred_lds<1>(s, m, 1, X, n, x_stride, 0, out, out_stride, nullptr, 0, nullptr,
-                     m.n_chunks, lds_limit);
+  for (int g0 = 0; g0 < m.K; g0 += kPredMaxK) {  // one window unless K > kPredMaxK
+    PredModel w = m;
+    w.g0 = g0;
+    w.Kw = std::min(kPredMaxK, m.K - g0);
+    launch_pred_lds<1>(s, w, 1, X, n, x_stride, 0, out + (int64_t)g0 * out_stride, out_stride,
+                       nullptr, 0, nullptr, m.n_chunks, lds_limit);
+  }
 }
 
 // Plain 16-byte streaming copy, the calibration point for HBM byte counters.

@@ -9,12 +9,17 @@ namespace bst {
 // The class trees of a softmax round grow together (class batching) only
 // while every row position t * n + i of the T trees fits a 32-bit index (the
 // partition's row positions, histogram items and leaf segments are u32).
-// Returns how many trees of a round grow together: K when batching is wanted
-// and the positions fit, else 1 (one class tree at a time, positions < n).
+// At most kBatchTrees trees grow together (the device controllers' limit; a
+// round of more class trees grows them in batches of that many).
+// Returns how many trees of a round grow together: min(K, kBatchTrees) when
+// batching is wanted and the positions fit, else 1 (one class tree at a
+// time, positions < n).
+constexpr int kBatchTrees = 16;
 inline int class_batch_trees(int K, int64_t n_rows, bool want_batching) {
   if (!want_batching || K <= 1) return 1;
-  if ((uint64_t)K * (uint64_t)n_rows >= (1ull << 32)) return 1;
-  return K;
+  const int T = K < kBatchTrees ? K : kBatchTrees;
+  if ((uint64_t)T * (uint64_t)n_rows >= (1ull << 32)) return 1;
+  return T;
 }
 
 // Columns a tree keeps under column sampling, ColSampler::sample_tree

@@ -1173,8 +1173,8 @@ void append_tree(bst_model& m, const TreeSchema& t, const std::vector<double>& l
 
 std::unique_ptr<bst_model> model_from_schema(ModelSchema& ms) {
   // ForestSchema -> Forest (convert.rs:432-470).
-  if (ms.n_groups < 1 || ms.n_groups > (uint64_t)kPredMaxGroups)
-    bad("n_groups must be in 1.." + std::to_string(kPredMaxGroups));
+  if (ms.n_groups < 1 || ms.n_groups > (uint64_t)kMaxOutputs)
+    bad("n_groups must be in 1.." + std::to_string(kMaxOutputs));
   const uint32_t K = (uint32_t)ms.n_groups;
   if (ms.base_score.size() != K) bad("Forest schema base_score must have length n_groups");
   if (ms.has_tree_groups && ms.tree_groups.size() != ms.trees.size())
@@ -1373,7 +1373,7 @@ namespace bst {
 void check_forest(const bst_model& m) {
   const bst_model_info& I = m.info;
   const int64_t T = I.n_trees, N = I.total_nodes, W = I.total_cat_words, K = I.n_groups;
-  if (K < 1 || K > kPredMaxGroups) bad("n_groups must be in 1.." + std::to_string(kPredMaxGroups));
+  if (K < 1 || K > kMaxOutputs) bad("n_groups must be in 1.." + std::to_string(kMaxOutputs));
   if (T < 0 || N < 0 || W < 0 || I.n_features < 0) bad("negative model sizes");
   if ((int64_t)m.base_score.size() != K) bad("base_score must have n_groups entries");
   if ((int64_t)m.tree_offset.size() != T + 1 || (int64_t)m.tree_group.size() != T)

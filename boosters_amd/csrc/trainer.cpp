@@ -913,10 +913,11 @@ bst_model* Trainer::train(const float* y, const float* w, const float* vX, const
     launch_objective(y, w);
     launch_qscale(s_, absmax_.p, K, n_, qexp_.p);
     for (int k0 = 0; k0 < K; k0 += T_) {
-      std::vector<TreeState> trees(T_);
+      const int nb = std::min(T_, K - k0);  // the last batch of a round may be short
+      std::vector<TreeState> trees(nb);
       BST_HIP(hipMemsetAsync(node_sum_.p, 0, node_sum_.bytes(), s_));
-      std::vector<uint32_t> outs(T_);
-      for (int t = 0; t < T_; t++) {
+      std::vector<uint32_t> outs(nb);
+      for (int t = 0; t < nb; t++) {
         TreeState& ts = trees[t];
         ts.t = t;
         ts.output = k0 + t;

@@ -20,6 +20,9 @@ int main() {
     {3, 2147483647, 1},         // 3 * (2^31 - 1) >= 2^32
     {10, 1000, 0},              // batching not wanted
     {1, 3000000000LL, 1},       // one output never batches
+    {40, 1000000, 1},           // more classes than a batch: batches of 16
+    {17, 268435455, 1},         // batches of 16 that fit
+    {40, 268435456, 1},         // 16 * 2^28 = 2^32: one tree at a time
   };
   for (auto& c : cases)
     std::printf("%d\n", bst::class_batch_trees((int)c[0], c[1], c[2] != 0));
@@ -35,4 +38,4 @@ def test_class_batching_keeps_positions_32_bit(tmp_path):
     subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "boosters_amd", "csrc"),
                     str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    assert [int(v) for v in out] == [10, 16, 1, 1, 2, 1, 1, 1]
+    assert [int(v) for v in out] == [10, 16, 1, 1, 2, 1, 1, 1, 16, 16, 1]

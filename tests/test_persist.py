@@ -557,7 +557,7 @@ def _with(fn):
     lambda s: s["forest"].update(tree_groups=[1]),
     lambda s: s["forest"].update(tree_groups=[0, 0]),
     lambda s: s["forest"].update(n_groups=0, base_score=[]),
-    lambda s: s["forest"].update(n_groups=17, base_score=[0.0] * 17),
+    lambda s: s["forest"].update(n_groups=1025, base_score=[0.0] * 1025),
     lambda s: s.update(output_transform="tanh"),
     lambda s: s.pop("meta"),
     lambda s: s["meta"].update(feature_types=["numeric", "ordinal"]),
