@@ -319,7 +319,7 @@ def _declare(lib):
         "bst_kernel_partition_fused": (
             C.c_int,
             [vp, vp, P(C.c_uint32), C.c_int64, C.c_int32, C.c_uint32, C.c_int32, P(C.c_uint32),
-             C.c_int32, P(C.c_int64)],
+             C.c_int32, C.c_int32, P(C.c_int64)],
         ),
         "bst_kernel_find_split": (
             C.c_int,

@@ -1343,7 +1343,8 @@ int bst_kernel_partition(bst_ctx* ctx, const bst_dataset* ds, uint32_t* rows, in
 
 int bst_kernel_partition_fused(bst_ctx* ctx, const bst_dataset* ds, uint32_t* rows, int64_t n,
                                int32_t e, uint32_t bin, int32_t default_left,
-                               const uint32_t* cats, int32_t n_words, int64_t* left_count) {
+                               const uint32_t* cats, int32_t n_words, int32_t mode,
+                               int64_t* left_count) {
   return guard([&] {
     need(ctx && ds && rows, BST_E_INVALID, "null argument");
     need(e >= 0 && e < ds->d.E, BST_E_INVALID, "bad feature");
@@ -1352,14 +1353,21 @@ int bst_kernel_partition_fused(bst_ctx* ctx, const bst_dataset* ds, uint32_t* ro
     set_device(ctx);
     hipStream_t s = ctx->stream;
     const int64_t m = std::max<int64_t>(n, 1);
-    DevBuf<uint32_t> idx0(m), idx1(m), cursor(2), dyn(1), tout(1);
+    DevBuf<uint32_t> idx0(m), idx1(m), cursor(2), dyn(1), tout(1), ticket(2);
     DevBuf<float2> ogh0(m), ogh1(m);
     DevBuf<DevSplit> split(1);
     DevBuf<float> preds(1);
-    h2d(s, idx0.p, rows, n);
+    // The node's rows as the partition stores a right child (last position
+    // first) with BST_PART_REVERSED: the same rows in the same order.
+    const bool input_reversed = (mode & BST_PART_REVERSED) != 0;
+    std::vector<uint32_t> src(rows, rows + n);
+    if (input_reversed) std::reverse(src.begin(), src.end());
+    BST_HIP(hipMemsetAsync(idx0.p, 0, idx0.bytes(), s));
+    h2d(s, idx0.p, src.data(), n);
     BST_HIP(hipMemsetAsync(ogh0.p, 0, ogh0.bytes(), s));
     BST_HIP(hipMemsetAsync(cursor.p, 0, cursor.bytes(), s));
     BST_HIP(hipMemsetAsync(tout.p, 0, tout.bytes(), s));
+    BST_HIP(hipMemsetAsync(ticket.p, 0, ticket.bytes(), s));
     DevSplit sp;
     std::memset(&sp, 0, sizeof(sp));
     sp.gain = 1.0f;
@@ -1368,10 +1376,17 @@ int bst_kernel_partition_fused(bst_ctx* ctx, const bst_dataset* ds, uint32_t* ro
     sp.flags = (default_left ? 1u : 0u) | (cats ? 2u : 0u);
     for (int w = 0; cats && w < n_words && w < kMaxCatWords; w++) sp.cats[w] = cats[w];
     h2d(s, split.p, &sp, 1);
+    const uint32_t src_bits = input_reversed ? kTileRev : 0u;
     std::vector<PartTile> tiles;
-    for (int64_t b = 0; b < n || tiles.empty(); b += kFusedTileRows)
-      tiles.push_back({0u, (uint32_t)b,
-                       (uint32_t)std::max<int64_t>(0, std::min<int64_t>(kFusedTileRows, n - b)), 0u});
+    for (int64_t b = 0; b < n || tiles.empty(); b += kFusedTileRows) {
+      const uint32_t b0 = (uint32_t)b;
+      const uint32_t begin = input_reversed ? (uint32_t)(n - 1 - b) : b0;
+      tiles.push_back({0u, n > 0 ? begin : 0u,
+                       (uint32_t)std::max<int64_t>(0, std::min<int64_t>(kFusedTileRows, n - b)),
+                       src_bits});
+    }
+    DevBuf<unsigned long long> tstat(tiles.size());
+    BST_HIP(hipMemsetAsync(tstat.p, 0, tstat.bytes(), s));
     const FeatMeta& fm = ds->fmeta[(size_t)e];
     PartItem it;
     std::memset(&it, 0, sizeof(it));
@@ -1391,13 +1406,21 @@ int bst_kernel_partition_fused(bst_ctx* ctx, const bst_dataset* ds, uint32_t* ro
     h2d(s, ditem.p, &it, 1);
     h2d(s, dtiles.p, tiles.data(), tiles.size());
     h2d(s, dyn.p, &nt, 1);
-    launch_partition_fused(s, ds->d, ditem.p, dtiles.p, (int)nt, split.p, idx0.p, ogh0.p, idx1.p,
-                           ogh1.p, cursor.p, dyn.p, tout.p, preds.p, 1, 1u << 30);
+    // BST_PART_TICKETS: half as many blocks as tiles (rounded up).
+    const int grid = (mode & BST_PART_TICKETS) ? (int)((nt + 1) / 2) : (int)nt;
+    launch_partition_fused(s, ds->d, ditem.p, dtiles.p, grid, split.p, idx0.p, ogh0.p, idx1.p,
+                           ogh1.p, cursor.p, tstat.p, ticket.p, ticket.p + 1, dyn.p, tout.p,
+                           preds.p, 1, 1u << 30);
     BST_HIP(hipGetLastError());
-    uint32_t cur[2] = {0, 0};
+    uint32_t cur[2] = {0, 0}, tk[2] = {0, 0};
     d2h(s, cur, cursor.p, 2);
+    d2h(s, tk, ticket.p, 2);
     d2h(s, rows, idx1.p, n);
     BST_HIP(hipStreamSynchronize(s));
+    if (tk[1]) raise(BST_E_DEVICE, "a partition tile's look-back did not finish within its wait bound");
+    // The right child's rows are stored from the end of the range down:
+    // hand them back in row order.
+    std::reverse(rows + cur[0], rows + n);
     if (left_count) *left_count = cur[0];
     return BST_OK;
   });

@@ -80,14 +80,16 @@ struct PartItem {
 };
 constexpr uint32_t kItemLeaves = 4u;
 // PartTile::src bits: 1 = rows in the alternate buffer, 2 = the item applies
-// leaf values (kItemLeaves), so the tile needs no (g, h).
-constexpr uint32_t kTileAlt = 1u, kTileLeaves = 2u;
+// leaf values (kItemLeaves), so the tile needs no (g, h), 4 = the node's rows
+// are stored from its last position down (a right child of the one-pass
+// partition), so the tile's rows are begin, begin - 1, ...
+constexpr uint32_t kTileAlt = 1u, kTileLeaves = 2u, kTileRev = 4u;
 
 struct PartTile {
   uint32_t item;   // index into PartItem list
-  uint32_t begin;  // absolute first position
+  uint32_t begin;  // absolute position of the tile's first row (in row order)
   uint32_t count;
-  uint32_t src;    // device planner: kTileAlt / kTileLeaves bits; host: 0
+  uint32_t src;    // device planner: kTileAlt / kTileLeaves / kTileRev bits; host: 0
 };
 
 // Per-node segment for sums / leaf updates.
@@ -298,26 +300,29 @@ void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items,
                       uint32_t* item_left, uint32_t* node_count, long long* node_sum,
                       const uint32_t* dyn_items = nullptr, const uint32_t* dyn_tiles = nullptr);
 
-// Device-driven partition, one pass: each tile reserves its left rows from
-// the front and its right rows from the back of the node's range in the other
-// buffer of the pair (cursor[2 item] / cursor[2 item + 1], zeroed by the
-// caller; the final left cursor is the left count).  Row order inside a node
-// is not kept (no result depends on it: all sums are exact integers; the
-// children's totals come from the split's left sums).  Counts read from
-// dyn_tiles; tiles[k].src names the source buffer; grid_tiles must not
-// exceed the tile buffer's capacity (each block reads its first tile ahead
-// of the count).  Items flagged kItemLeaves instead add leaf_l / leaf_r to
+// Device-driven partition, one pass, stable: each node's left rows fill its
+// range in the other buffer of the pair from the front in row order, its
+// right rows fill the range from the back in row order (so the right child
+// is read from its last position down: kTileRev).  A node's tiles are
+// consecutive in the tile list and are claimed in order through *ticket
+// (zeroed by the caller); tstat[k] (zeroed by the caller) carries tile k's
+// left count and then its inclusive prefix among its node's tiles, and the
+// last tile of item i writes the node's left count to cursor[2 i].  A
+// look-back that does not finish sets *err.  Counts read from dyn_tiles;
+// tiles[k].src names the source buffer and direction.  Items flagged
+// kItemLeaves instead add leaf_l / leaf_r to
 // preds[tree_output[node / max_nodes] * n_rows + row] for each of their rows
 // (rows stay in place, cursors untouched).
 constexpr uint32_t kFusedTileRows = 4096;  // rows per tile of launch_partition_fused (at most)
-// Tile of the steps with many nodes: their cursor atomics spread over many
-// words, and shorter tiles shorten each block's chain of dependent loads.
+// Tile of the steps with many nodes: shorter tiles shorten each block's
+// chain of dependent loads, and many nodes keep the look-back chains short.
 constexpr uint32_t kFusedTileRowsSmall = 2048;
 constexpr uint32_t kFusedSmallTileNodes = 32;  // nodes in a step from which it is used
 void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
                             const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
                             uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,
-                            uint32_t* cursor, const uint32_t* dyn_tiles,
+                            uint32_t* cursor, unsigned long long* tstat, uint32_t* ticket,
+                            uint32_t* err, const uint32_t* dyn_tiles,
                             const uint32_t* tree_output, float* preds, int64_t n_rows,
                             uint32_t max_nodes);
 

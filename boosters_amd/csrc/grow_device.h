@@ -60,8 +60,11 @@ struct TreeCtl {
 // Count slots in DeviceBufs::counts.
 // C_HROWS / C_HNODES accumulate over a whole round (rows and nodes
 // histogrammed), for the algorithmic-byte accounting.
+// C_TICKET is the one-pass partition's tile ticket (reset by each plan);
+// C_ERR is set when a partition look-back did not finish (reset per round).
 enum : int {
-  C_ITEMS = 0, C_TILES, C_HIST, C_FIN, C_SPLIT, C_LSEGS, C_HROWS, C_HNODES, C_NCOUNTS
+  C_ITEMS = 0, C_TILES, C_HIST, C_FIN, C_SPLIT, C_LSEGS, C_HROWS, C_HNODES, C_TICKET, C_ERR,
+  C_NCOUNTS
 };
 
 // Per-round outputs, laid out back to back in one device buffer so that one
@@ -83,7 +86,8 @@ struct DeviceBufs {
   uint32_t *ch_node, *ch_tnode, *ch_depth;        // [T][W]
   uint32_t* pbegin;                               // [T*MN] global positions
   uint32_t* pslot;                                // [T*MN] tree-local slot, ~0 = none
-  uint8_t* pbuf;                                  // [T*MN] 1: rows in the alternate buffer
+  uint8_t* pbuf;                                  // [T*MN] bit 0: rows in the alternate buffer,
+                                                  // bit 1: rows stored from the last position down
   uint32_t* free_slot;                            // [T][SPT] stack
   float* leaf_val;                                // [T*MN]
   // Tree arrays [T][MNT] (cats: [T][MNT][8]).
@@ -93,6 +97,7 @@ struct DeviceBufs {
   // Step work lists.
   PartItem* items;
   PartTile* tiles;
+  unsigned long long* tstat;  // [tiles] look-back status of the step's partition tiles
   uint32_t* pend;  // [items][5]: tree, left tree node, right tree node, rank, child depth
   HistItem* hitems;
   uint32_t* fin;   // [n][4]: small slot, large slot or ~0, first item, tiles

@@ -62,6 +62,19 @@ __device__ __forceinline__ uint32_t ord_gain(float v) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// PartTile::src of a node's tiles from its pbuf bits (bit 0 alternate
+// buffer, bit 1 rows stored from the last position down).
+__device__ __forceinline__ uint32_t tile_src(uint32_t pbuf, bool leaves) {
+  return ((pbuf & 1u) ? kTileAlt : 0u) | ((pbuf & 2u) ? kTileRev : 0u) |
+         (leaves ? kTileLeaves : 0u);
+}
+
+// Position of the node's row b0 (in row order) for a tile of `src`.
+__device__ __forceinline__ uint32_t tile_begin(uint32_t begin, uint32_t count, uint32_t b0,
+                                               uint32_t src) {
+  return (src & kTileRev) ? begin + count - 1u - b0 : begin + b0;
+}
+
 __device__ __forceinline__ bool expandable(const DevSplit& s, double min_gain) {
   return s.gain > -INFINITY && (double)s.gain > min_gain;
 }
@@ -417,7 +430,7 @@ __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int a
     s_tfirst[e] = toff;  // e < kCtlThreads <= kPlanLdsItems
     s_tbegin[e] = begin;
     s_tcount[e] = count;
-    s_tsrc[e] = (uint8_t)((pbuf ? kTileAlt : 0u) | (lv ? kTileLeaves : 0u));
+    s_tsrc[e] = (uint8_t)tile_src(pbuf, lv);
   }
   if (i == 0) {
     // The level is consumed (advance_level + the step's updates); n_free is
@@ -537,7 +550,7 @@ __device__ void plan_leafwise(const CtlParams& P, const DeviceBufs& B, uint32_t*
       s_tfirst[g] = tile0;
       s_tbegin[g] = begin;
       s_tcount[g] = count;
-      s_tsrc[g] = (uint8_t)(pb ? kTileAlt : 0u);
+      s_tsrc[g] = (uint8_t)tile_src(pb, false);
       B.ctl[t].n_part = n_part + 1;
       B.ctl[t].n_tnodes = n_tnodes + 2;
     }
@@ -630,7 +643,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
           const uint32_t g = s_items + e;
           const uint32_t begin = B.pbegin[gid], pb = B.pbuf[gid];
           const bool lv = P.leaf_apply && P.growth == 0 && dep + 1u >= P.max_depth;
-          const uint32_t src = (pb ? kTileAlt : 0u) | (lv ? kTileLeaves : 0u);
+          const uint32_t src = tile_src(pb, lv);
           uint32_t orig, miss;
           float thr;
           split_feature_info(B, sp, &orig, &thr, &miss);
@@ -648,10 +661,11 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
               const uint32_t b0 = q * P.part_tile;
               PartTile pt;
               pt.item = g;
-              pt.begin = begin + b0;
+              pt.begin = tile_begin(begin, count, b0, src);
               pt.count = count > b0 ? min(P.part_tile, count - b0) : 0u;
               pt.src = src;
               B.tiles[s_tiles + toff + q] = pt;
+              B.tstat[s_tiles + toff + q] = 0ull;
             }
           }
         }
@@ -687,14 +701,16 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_plan(CtlParams P, DeviceBuf
     const uint32_t b0 = (k - s_tfirst[lo]) * ptile, count = s_tcount[lo];
     PartTile pt;
     pt.item = lo;
-    pt.begin = s_tbegin[lo] + b0;
-    pt.count = count > b0 ? min(ptile, count - b0) : 0u;
     pt.src = s_tsrc[lo];
+    pt.begin = tile_begin(s_tbegin[lo], count, b0, pt.src);
+    pt.count = count > b0 ? min(ptile, count - b0) : 0u;
     B.tiles[k] = pt;
+    B.tstat[k] = 0ull;
   }
   if (threadIdx.x == 0) {
     B.counts[C_ITEMS] = s_items;
     B.counts[C_TILES] = s_tiles;
+    B.counts[C_TICKET] = 0u;
   }
 }
 
@@ -822,9 +838,11 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_children(CtlParams P, Devic
       B.node_sum[2 * parent + 1] = c.lh;
       B.node_sum[2 * right] = c.pg - c.lg;
       B.node_sum[2 * right + 1] = c.ph - c.lh;
-      const uint8_t cbuf = (uint8_t)((it.pad & 1u) ^ 1u);  // the partition wrote the other buffer
+      // The partition wrote the other buffer, the right child's rows from the
+      // end of the parent's range down.
+      const uint8_t cbuf = (uint8_t)((it.pad & 1u) ^ 1u);
       B.pbuf[parent] = cbuf;
-      B.pbuf[right] = cbuf;
+      B.pbuf[right] = (uint8_t)(cbuf | 2u);
       const uint32_t pb = c.pb;
       B.pbegin[right] = pb + L;
       const bool left_small = L <= R;
@@ -1004,7 +1022,7 @@ __global__ __launch_bounds__(kCtlThreads) void k_ctl_finalize(CtlParams P, Devic
       const uint32_t gid = t * P.MN + (in ? p : 0u);
       const float lv = B.leaf_val[gid];
       const uint32_t cnt = B.node_count[gid], b = B.pbegin[gid];
-      const uint32_t alt = B.pbuf[gid] ? 0x80000000u : 0u;
+      const uint32_t alt = (B.pbuf[gid] & 1u) ? 0x80000000u : 0u;
       const uint32_t ns = in && !isnan(lv) ? (cnt + P.seg_tile - 1) / P.seg_tile : 0u;
       uint32_t tot;
       const uint32_t s0 = block_excl_scan(ns, s_tmp, &tot) + carry_s;

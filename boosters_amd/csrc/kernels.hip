@@ -1987,17 +1987,48 @@ __device__ __forceinline__ long long block_sum_ll(long long v, long long* s_red)
   return t;
 }
 
-// One-pass partition of a tile of a node's rows (device-driven schedule).
-// Rows are read strided (row j * kFusedThreads + thread), so every load is
-// coalesced and each thread has kFusedRows independent row loads in flight.
-// Left rows keep their tile order in the front region, right rows in the
-// back region; the tile's place in each region comes from one atomic per
-// region, so tiles of a node may land in any order.  The children's totals
-// are not summed here: the split recorded the exact left sums.
+// One-pass stable partition of a tile of a node's rows (device-driven
+// schedule; partition.rs:73-145 keeps both sides in row order).
+//
+// Every tile a block waits on is one of its node's earlier tiles and has a
+// lower index.  When the grid covers every tile, block b takes tile b: work
+// groups start in index order (each XCD walks its share of the grid in
+// order), so the lowest unfinished tile always belongs to a running block.
+// Otherwise tiles are claimed in order through a ticket counter, so every
+// tile waited on was claimed by a running block.  Each tile counts its left
+// rows (ballots + a block scan over (row pass, wave)), publishes that count
+// at once, and then finds its exclusive prefix among the tiles of its node
+// by a decoupled look-back: one wave reads up to 64 preceding tiles' status
+// words (four windows of 64 per round trip), adds their counts back to the
+// nearest tile that has published its
+// inclusive prefix, and waits only for tiles that have not yet published a
+// count (which they do before any waiting of their own).  With the prefix,
+// every row's destination follows from its rank: left rows fill the node's
+// range from its start in row order; right rows fill it from its end
+// backwards, so the right child holds its rows in row order read from its
+// last position down (kTileRev marks such a node as a source: its tiles
+// start at their highest position and walk down).  The last tile of a node
+// records the node's left count.  Rows are read strided (row j *
+// kFusedThreads + thread), so every load is coalesced and each thread has
+// kFusedRows independent row loads in flight.  The children's totals are not
+// summed here: the split recorded the exact left sums.
+//
+// The look-back wait is bounded: past kLookbackSpins rounds without
+// progress the tile sets *err and goes on (the trainer then fails the
+// round), so no launch can hang.
 constexpr int kFusedThreads = 1024;
 constexpr int kFusedRows = kFusedTileRows / kFusedThreads;  // rows per thread
 constexpr int kFusedWaves = kFusedThreads / WAVE;
 static_assert(kFusedRows * kFusedWaves <= WAVE, "one prefix value per lane");
+constexpr unsigned long long kStatAgg = 1ull << 62;     // tile count published
+constexpr unsigned long long kStatPrefix = 2ull << 62;  // inclusive prefix published
+constexpr uint32_t kLookbackSpins = 1u << 22;
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += (uint32_t)__shfl_xor((int)v, m, WAVE);
+  return v;
+}
 
 __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
     const PartItem* __restrict__ items, const PartTile* __restrict__ tiles,
@@ -2005,20 +2036,37 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
     const uint8_t* __restrict__ bins_cm, const uint32_t* __restrict__ hi_cm, int64_t n,
     int64_t hi_words, uint32_t* __restrict__ idx0, float2* __restrict__ ogh0,
     uint32_t* __restrict__ idx1, float2* __restrict__ ogh1, uint32_t* __restrict__ cursor,
-    const uint32_t* __restrict__ dyn_tiles, const uint32_t* __restrict__ tree_output,
-    float* __restrict__ preds, int64_t n_rows, uint32_t max_nodes) {
+    unsigned long long* __restrict__ tstat, uint32_t* __restrict__ ticket,
+    uint32_t* __restrict__ err, const uint32_t* __restrict__ dyn_tiles,
+    const uint32_t* __restrict__ tree_output, float* __restrict__ preds, int64_t n_rows,
+    uint32_t max_nodes) {
   __shared__ uint32_t s_l[kFusedRows * kFusedWaves];  // lefts per (row pass, wave)
   __shared__ uint32_t s_v[kFusedRows * kFusedWaves];  // rows per (row pass, wave)
   __shared__ uint32_t s_base[2];
+  __shared__ uint32_t s_wi;
   const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  // The first tile is read with the count (the grid does not exceed the tile
-  // buffer), and the rows with the item: the tile names the source buffer.
+  // The block's own tile is read with the count (the grid never exceeds the
+  // tile buffer's capacity).
   const PartTile first = tiles[blockIdx.x];
   const uint32_t n_work = *dyn_tiles;
-  for (uint32_t wi = blockIdx.x; wi < n_work; wi += gridDim.x) {
-    const PartTile t = wi == blockIdx.x ? first : tiles[wi];
+  if (blockIdx.x >= n_work) return;  // no tile, no ticket
+  const bool direct = n_work <= gridDim.x;
+  for (uint32_t round = 0;; round++) {
+    uint32_t wi;
+    if (direct) {
+      if (round > 0) break;
+      wi = blockIdx.x;
+    } else {
+      __syncthreads();  // the previous tile's shared values have been read
+      if (threadIdx.x == 0) s_wi = atomicAdd(ticket, 1u);
+      __syncthreads();
+      wi = s_wi;
+      if (wi >= n_work) break;  // block-uniform
+    }
+    const PartTile t = direct ? first : tiles[wi];
     const bool src1 = (t.src & kTileAlt) != 0;
+    const bool rev = (t.src & kTileRev) != 0;  // source rows run down from t.begin
     const bool leaves = (t.src & kTileLeaves) != 0;  // block-uniform
     const uint32_t* sidx = src1 ? idx1 : idx0;
     const float2* sogh = src1 ? ogh1 : ogh0;
@@ -2029,9 +2077,12 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
 #pragma unroll
     for (int j = 0; j < kFusedRows; j++) {
       const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
-      const uint32_t pos = p < t.count ? t.begin + p : 0u;  // position 0 always exists
-      rid[j] = sidx[pos];
-      gh[j] = leaves ? make_float2(0.0f, 0.0f) : sogh[pos];
+      // Positions past the tile's rows are not read: they stand for row 0
+      // (which exists), so the bin loads below stay in bounds.
+      const bool in = p < t.count;
+      const uint32_t pos = rev ? t.begin - p : t.begin + p;
+      rid[j] = in ? sidx[pos] : 0u;
+      gh[j] = in && !leaves ? sogh[pos] : make_float2(0.0f, 0.0f);
     }
     const PartItem it = items[t.item];
     // The split decision comes with the item (goes_left on it): only a
@@ -2049,7 +2100,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
     if (leaves) {
       // The children are leaves: add their values to the rows' predictions
       // (the rows of one tree are distinct, so the updates are independent;
-      // the same float add as the leaf-range update).
+      // the same float add as the leaf-range update).  No row moves.
       float* pr = preds + (int64_t)tree_output[it.node / max_nodes] * n_rows;
       float x[kFusedRows];
       bool l[kFusedRows];
@@ -2067,7 +2118,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
         const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
         if (p < t.count) pr[rid[j]] = x[j] + (l[j] ? it.leaf_l : it.leaf_r);
       }
-      continue;  // block-uniform; no shared state used
+      continue;  // block-uniform
     }
     unsigned long long lbal[kFusedRows], vbal[kFusedRows];
 #pragma unroll
@@ -2087,7 +2138,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
       }
     }
     __syncthreads();
-    // Exclusive prefix over (row pass, wave) in position order; one wave.
+    // Exclusive prefix over (row pass, wave) in position order, then the
+    // tile's place among its node's tiles; one wave.
     if (wv == 0) {
       constexpr int NS = kFusedRows * kFusedWaves;  // 64 = one value per lane
       uint32_t lv = lane < NS ? s_l[lane] : 0u, vv = lane < NS ? s_v[lane] : 0u;
@@ -2100,26 +2152,75 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
           vi += b;
         }
       }
-      const uint32_t TL = __shfl(li, 63, 64), TV = __shfl(vi, 63, 64);
+      const uint32_t TL = __shfl(li, 63, 64);
       if (lane < NS) {
         s_l[lane] = li - lv;
         s_v[lane] = vi - vv;
       }
+      const uint32_t q = wi - it.tile0;  // the tile's index within its node
+      if (lane == 0)
+        __hip_atomic_store(&tstat[wi], (q == 0 ? kStatPrefix : kStatAgg) | TL, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t excl = 0;
+      if (q > 0) {
+        constexpr int kWin = 4;  // windows of 64 tiles read per round trip
+        int64_t j = (int64_t)wi - 1;
+        const int64_t lo = (int64_t)it.tile0;
+        uint32_t spins = 0;
+        bool done = false;
+        while (!done) {
+          unsigned long long v[kWin];
+#pragma unroll
+          for (int w = 0; w < kWin; w++) {
+            const int64_t jj = j - w * WAVE - lane;
+            v[w] = jj >= lo ? __hip_atomic_load(&tstat[jj], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : 0ull;
+          }
+          bool retry = false;
+#pragma unroll
+          for (int w = 0; w < kWin; w++) {
+            if (done || retry) continue;  // wave-uniform
+            const bool valid = j - w * WAVE - lane >= lo;
+            const uint32_t fl = (uint32_t)(v[w] >> 62);
+            // Nearest preceding tile with its prefix (the node's first tile
+            // always publishes one), and whether every tile up to it has a
+            // count.
+            const unsigned long long pm = __ballot(valid && fl == 2u);
+            const int pl = pm ? __ffsll((long long)pm) - 1 : WAVE;
+            const unsigned long long zm = __ballot(valid && fl == 0u && lane <= pl);
+            if (zm != 0ull) {
+              retry = true;  // read again from this window
+              j -= w * WAVE;
+            } else {
+              excl += wave_sum_u32(valid && lane <= pl ? (uint32_t)v[w] : 0u);
+              done = pl < WAVE;  // else: 64 counts, no prefix: the next window
+            }
+          }
+          if (done) break;
+          if (!retry) {
+            j -= kWin * WAVE;
+          } else if (++spins > kLookbackSpins) {
+            if (lane == 0) atomicOr(err, 1u);
+            break;
+          }
+        }
+      }
       if (lane == 0) {
-        // One 64-bit add reserves both sides (left count in the low word,
-        // right count in the high word; neither can carry past a node's
-        // count), so the tiles of a node serialise on half as many atomics.
-        // Right rows fill the node's range from its end backwards.
-        const uint32_t TR = TV - TL;
-        const unsigned long long old = atomicAdd(
-            reinterpret_cast<unsigned long long*>(&cursor[2 * t.item]),
-            (unsigned long long)TL | ((unsigned long long)TR << 32));
-        s_base[0] = (uint32_t)old;
-        s_base[1] = it.count - (uint32_t)(old >> 32) - TR;
+        const uint32_t incl = excl + TL;
+        if (q > 0)
+          __hip_atomic_store(&tstat[wi], kStatPrefix | incl, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        if (q + 1 == it.n_tiles) cursor[2 * t.item] = incl;  // the node's left count
+        // Rows of the node before this tile (in row order), left ones among them.
+        const uint32_t before = rev ? it.begin + it.count - 1u - t.begin : t.begin - it.begin;
+        s_base[0] = excl;
+        s_base[1] = before - excl;
       }
     }
     __syncthreads();
-    const uint32_t lbase = s_base[0], rbase = s_base[1];
+    const uint32_t lbase = it.begin + s_base[0];
+    const uint32_t rtop = it.begin + it.count - 1u - s_base[1];  // next right row goes here, down
 #pragma unroll
     for (int j = 0; j < kFusedRows; j++) {
       const uint32_t p = (uint32_t)j * kFusedThreads + threadIdx.x;
@@ -2127,25 +2228,26 @@ __global__ __launch_bounds__(kFusedThreads) void k_part_fused(
         const bool l = (lbal[j] >> lane) & 1ull;
         const uint32_t lrank = s_l[j * kFusedWaves + wv] + (uint32_t)__popcll(lbal[j] & lt_mask);
         const uint32_t vrank = s_v[j * kFusedWaves + wv] + (uint32_t)__popcll(vbal[j] & lt_mask);
-        const uint32_t dst = it.begin + (l ? lbase + lrank : rbase + (vrank - lrank));
+        const uint32_t dst = l ? lbase + lrank : rtop - (vrank - lrank);
         didx[dst] = rid[j];
         dogh[dst] = gh[j];
       }
     }
-    __syncthreads();  // s_l, s_v, s_base are rewritten by the next tile
   }
 }
 
 void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
                             const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
                             uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,
-                            uint32_t* cursor, const uint32_t* dyn_tiles,
+                            uint32_t* cursor, unsigned long long* tstat, uint32_t* ticket,
+                            uint32_t* err, const uint32_t* dyn_tiles,
                             const uint32_t* tree_output, float* preds, int64_t n_rows,
                             uint32_t max_nodes) {
   if (grid_tiles <= 0) return;
   hipLaunchKernelGGL(k_part_fused, dim3(grid_tiles), dim3(kFusedThreads), 0, s, items, tiles,
                      node_split, d.meta, d.bins_cm, d.hi_cm, d.n, d.hi_words, idx0, ogh0, idx1,
-                     ogh1, cursor, dyn_tiles, tree_output, preds, n_rows, max_nodes);
+                     ogh1, cursor, tstat, ticket, err, dyn_tiles, tree_output, preds, n_rows,
+                     max_nodes);
 }
 
 void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items, int n_items,

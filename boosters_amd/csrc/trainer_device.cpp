@@ -46,6 +46,7 @@ struct Trainer::DevState {
   DevBuf<char> out;  // TreeOut arrays, back to back
   DevBuf<PartItem> items;
   DevBuf<PartTile> tiles;
+  DevBuf<unsigned long long> tstat;  // look-back status per partition tile
   DevBuf<uint32_t> pend;
   DevBuf<HistItem> hitems;
   DevBuf<uint32_t> fin, split_nodes, split_slots, split_minus, rows, eff_to_orig;
@@ -228,6 +229,7 @@ void Trainer::device_setup() {
   }
   s.items.alloc(s.cap_items);
   s.tiles.alloc(s.cap_tiles);
+  s.tstat.alloc(s.cap_tiles);
   s.pend.alloc((size_t)5 * s.cap_items);
   s.hitems.alloc(s.cap_hist);
   s.fin.alloc((size_t)4 * s.cap_fin);
@@ -277,6 +279,7 @@ void Trainer::device_setup() {
   B.tr_stype = s.O.tr_stype;
   B.items = s.items.p;
   B.tiles = s.tiles.p;
+  B.tstat = s.tstat.p;
   B.pend = s.pend.p;
   B.hitems = s.hitems.p;
   B.fin = s.fin.p;
@@ -361,8 +364,8 @@ void Trainer::device_enqueue_round(const float* y, const float* w, bool sampled)
     launch_ctl_plan(s_, P, B, /*advance=*/step > 0);
     int pi = ctx_->prof_begin(2, 0.0);
     launch_partition_fused(s_, d, B.items, B.tiles, g_tiles, node_split_.p, idx_.p, ogh_.p,
-                           idx_tmp_.p, ogh_tmp_.p, B.cursor, cnt + C_TILES, tree_output_.p,
-                           preds_.p, n_, P.MN);
+                           idx_tmp_.p, ogh_tmp_.p, B.cursor, B.tstat, cnt + C_TICKET,
+                           cnt + C_ERR, cnt + C_TILES, tree_output_.p, preds_.p, n_, P.MN);
     ctx_->prof_end(pi);
     // The last step's children are never split (depth limit, or the leaf
     // budget is spent), so their histograms are not built; when its
@@ -444,6 +447,8 @@ void Trainer::device_collect(std::vector<TreeBuild>& out, double* metric_value) 
   BST_HIP(hipGetLastError());
   S.collected++;
   const TreeOut& H = S.H[slot];
+  if (H.counts[C_ERR])
+    raise(BST_E_DEVICE, "a partition tile's look-back did not finish within its wait bound");
   if (metric_value) *metric_value = metric_finish(S.h_metric[slot]);
   if (ctx_->profile) {
     // Algorithmic bytes of the round's histogram launches, spread evenly.

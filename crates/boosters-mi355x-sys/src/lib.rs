@@ -65,6 +65,8 @@ pub const BST_TRANSFORM_SOFTMAX: i32 = 2;
 pub const BST_SCHED_PLAIN: i32 = 0;
 pub const BST_SCHED_BATCHED: i32 = 1;
 pub const BST_SCHED_GRAPH: i32 = 2;
+pub const BST_PART_REVERSED: i32 = 1;
+pub const BST_PART_TICKETS: i32 = 2;
 
 // ---- configuration --------------------------------------------------------
 /// `BinningConfig` (`data/binned/feature_analysis.rs:29-63`), field for field.
@@ -551,6 +553,7 @@ extern "C" {
         default_left: i32,
         left_cats: *const u32,
         n_cat_words: i32,
+        mode: i32,
         left_count: *mut i64,
     ) -> c_int;
     pub fn bst_kernel_find_split(

@@ -386,15 +386,21 @@ int bst_kernel_partition(bst_ctx* ctx, const bst_dataset* ds, uint32_t* rows, in
                          int32_t eff_feature, uint32_t split_bin, int32_t default_left,
                          const uint32_t* left_cats, int32_t n_cat_words, int64_t* left_count);
 
-/* The graph schedule's one-pass partition of one node (no counterpart in
- * the reference): the same split rule and the same left / right row sets as
- * bst_kernel_partition, left rows first, but each side's rows are grouped
- * by 4096-row tiles in the order the tiles finish, so within a side the
- * order is not the input order. */
+/* The graph schedule's one-pass partition of one node (a test hook of the
+ * stable partition of partition.rs:73-145; no counterpart in the reference's
+ * API): the same split rule and the same result as bst_kernel_partition
+ * (left rows first, both sides in input order), computed by the training
+ * kernel in 4096-row tiles.  mode bits: BST_PART_REVERSED stores the node's
+ * rows the way the kernel stores a right child (last position first);
+ * BST_PART_TICKETS launches fewer blocks than tiles, so that tiles are
+ * claimed through the ticket counter as on steps with more tiles than the
+ * grid.  Neither may change the result. */
+#define BST_PART_REVERSED 1
+#define BST_PART_TICKETS 2
 int bst_kernel_partition_fused(bst_ctx* ctx, const bst_dataset* ds, uint32_t* rows,
                                int64_t n_rows, int32_t eff_feature, uint32_t split_bin,
                                int32_t default_left, const uint32_t* left_cats,
-                               int32_t n_cat_words, int64_t* left_count);
+                               int32_t n_cat_words, int32_t mode, int64_t* left_count);
 
 /* SplitInfo (training/gbdt/split/types.rs:34-55): feature is an effective
  * column; gain = -inf marks "no valid split". */

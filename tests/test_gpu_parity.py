@@ -217,12 +217,14 @@ def test_partition_bit_exact(ctx, oracle):
         np.testing.assert_array_equal(r2, ref_rows)
 
 
-def test_fused_partition_row_sets(ctx, oracle):
-    """The graph schedule's one-pass partition (k_part_fused) sends exactly
-    the oracle's rows to each side (partition.rs:296-382): the left count is
-    equal, and each side holds the same rows, in tile order instead of row
-    order.  Multi-tile nodes (tiles of 4096 rows), numeric splits with NaN in
-    either direction, categorical splits."""
+def test_fused_partition_stable(ctx, oracle):
+    """The graph schedule's one-pass partition (k_part_fused) is the oracle's
+    stable partition (partition.rs:73-145, 296-382): equal left count, and
+    both sides in row order, with the node's rows given in order or stored
+    the way the kernel stores a right child (last position first), and with
+    tiles taken by block index or claimed through the ticket counter.  Nodes
+    of zero to five 4096-row tiles (the tiles' look-back chain), numeric
+    splits with NaN in either direction, categorical splits."""
     X, y, is_cat = multiclass_cat_data(20_000, 4, 2, 3, card=9, seed=17)
     Xn, _ = binary_nan_data(20_000, 3, seed=18, nan_p=0.3)
     X = np.concatenate([X, Xn])
@@ -231,11 +233,13 @@ def test_fused_partition_row_sets(ctx, oracle):
     o = oracle.Binned(X, is_cat)
     lay = b.layout()
     rng = np.random.default_rng(19)
-    sizes = [1, 5, 4095, 4096, 4097, 12_288, 20_000]
-    for trial in range(28):
+    sizes = [0, 1, 5, 4095, 4096, 4097, 12_288, 20_000]
+    for trial in range(32):
         e = int(rng.integers(0, lay["n_eff"]))
-        n_rows = sizes[trial % len(sizes)] if trial < len(sizes) else int(rng.integers(1, 20_000))
-        rows = np.sort(rng.choice(20_000, n_rows, replace=False)).astype(np.uint32)
+        n_rows = sizes[trial % len(sizes)] if trial < 2 * len(sizes) else int(rng.integers(1, 20_000))
+        rows = rng.choice(20_000, n_rows, replace=False).astype(np.uint32)
+        if trial % 3 == 0:
+            rows = np.sort(rows)
         sp = A.SplitInfoC()
         sp.feature = e
         sp.gain = 1.0
@@ -252,14 +256,15 @@ def test_fused_partition_row_sets(ctx, oracle):
             sp.bin = int(rng.integers(0, info["n_bins"]))
             cw = None
         ref_rows, ref_left = o.partition(rows, sp)
-        r2 = rows.copy()
-        lc = C.c_int64()
-        A.check(A.lib().bst_kernel_partition_fused(ctx.h, b.h, A.ptr(r2), len(r2), e, sp.bin,
-                                                   sp.default_left, A.ptr(cw),
-                                                   8 if cw is not None else 0, C.byref(lc)))
-        assert lc.value == ref_left, (trial, e, n_rows)
-        np.testing.assert_array_equal(np.sort(r2[:ref_left]), ref_rows[:ref_left])
-        np.testing.assert_array_equal(np.sort(r2[ref_left:]), ref_rows[ref_left:])
+        for mode in (0, 1, 2, 3):  # bit 0 reversed storage, bit 1 ticket claims
+            r2 = rows.copy()
+            lc = C.c_int64()
+            A.check(A.lib().bst_kernel_partition_fused(ctx.h, b.h, A.ptr(r2), len(r2), e, sp.bin,
+                                                       sp.default_left, A.ptr(cw),
+                                                       8 if cw is not None else 0, mode,
+                                                       C.byref(lc)))
+            assert lc.value == ref_left, (trial, e, n_rows, mode)
+            np.testing.assert_array_equal(r2, ref_rows, err_msg=f"trial {trial} mode {mode}")
 
 
 def _forests_equal(gm, om_info, om_arrays, leaf_rtol=1e-6):
