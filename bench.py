@@ -602,7 +602,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--qps", type=float, default=1000.0)
-    ap.add_argument("--qps-sweep", default="500,1000,2000,4000,8000",
+    ap.add_argument("--qps-sweep", default="500,1000,2000,2500,3000,3500,4000,6000,8000",
                     help="offered request rates of the serving capacity sweep ('' = none)")
     ap.add_argument("--slo-ms", type=float, default=50.0,
                     help="p99 latency bound at which serving capacity is compared")

@@ -1662,7 +1662,13 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
   // (grower.rs:424-440); the gain keeps the set's sums, as the reference's.
   if (fm.is_cat && fm.has_missing && best && out.flags == 2u) {
     const uint32_t mb = fm.default_bin;
-    if (mb < 256u && ((out.cats[mb >> 5] >> (mb & 31)) & 1u)) {
+    // The set's word holding bin mb, selected word by word so that `out`
+    // stays in registers (an index known only at run time would put it in
+    // scratch memory).
+    uint32_t mword = 0u;
+#pragma unroll
+    for (int w = 0; w < kMaxCatWords; w++) mword = (uint32_t)w == (mb >> 5) ? out.cats[w] : mword;
+    if (mb < 256u && ((mword >> (mb & 31)) & 1u)) {
       const longlong2 v = load(mb);
       out.lg -= v.x;
       out.lh -= v.y;
