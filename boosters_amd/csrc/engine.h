@@ -314,10 +314,9 @@ void launch_partition(hipStream_t s, const DatasetDev& d, const PartItem* items,
 // preds[tree_output[node / max_nodes] * n_rows + row] for each of their rows
 // (rows stay in place, cursors untouched).
 constexpr uint32_t kFusedTileRows = 4096;  // rows per tile of launch_partition_fused (at most)
-// Tile of the steps with many nodes: shorter tiles shorten each block's
-// chain of dependent loads, and many nodes keep the look-back chains short.
+// Tile of the depth-wise steps whose partition applies leaf values (random
+// prediction updates: more, shorter tiles keep more of them in flight).
 constexpr uint32_t kFusedTileRowsSmall = 2048;
-constexpr uint32_t kFusedSmallTileNodes = 32;  // nodes in a step from which it is used
 void launch_partition_fused(hipStream_t s, const DatasetDev& d, const PartItem* items,
                             const PartTile* tiles, int grid_tiles, const DevSplit* node_split,
                             uint32_t* idx0, float2* ogh0, uint32_t* idx1, float2* ogh1,

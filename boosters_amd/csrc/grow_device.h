@@ -41,7 +41,7 @@ struct CtlParams {
   uint32_t max_depth, max_leaves;
   uint32_t NG, E, FG;
   uint32_t hist_tile, part_tile, seg_tile;  // hist_tile: the largest histogram tile
-  uint32_t part_tile_small;                  // partition tile of steps with many nodes
+  uint32_t part_tile_small;                  // partition tile of leaf-applying steps
   uint32_t hist_tile_min, hist_target;       // smallest tile; items wanted per step
   uint32_t rightmost;   // split tie rule across features (rayon: rightmost max)
   int64_t stride;       // positions per tree slot in idx/ogh

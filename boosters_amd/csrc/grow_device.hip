@@ -416,7 +416,12 @@ __device__ bool plan_level_paired(const CtlParams& P, const DeviceBufs& B, int a
   const uint32_t depth = c0.depth + (advance ? 1u : 0u);
   const bool act = in && depth < P.max_depth;
   const bool ex = act && expandable(sp, P.min_gain);
-  const uint32_t ptile = nn >= kFusedSmallTileNodes ? P.part_tile_small : P.part_tile;
+  // A step whose partition applies the children's leaf values reads and
+  // writes every row's prediction at random: shorter tiles put more blocks
+  // on those accesses.  Steps that move rows take full tiles (fewer tiles,
+  // fewer look-back rounds).
+  const bool lv_step = P.leaf_apply && depth + 1u >= P.max_depth;
+  const uint32_t ptile = lv_step ? P.part_tile_small : P.part_tile;
   const uint32_t ntile = ex ? max(1u, (count + ptile - 1) / ptile) : 0u;
   uint32_t tot_e, tot_t;
   const uint32_t e = block_excl_scan(ex ? 1u : 0u, s_tmp, &tot_e);
