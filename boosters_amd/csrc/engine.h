@@ -384,25 +384,37 @@ struct PredModel {
 // reads and writes directly; the descriptors and counters below are the
 // whole protocol.
 constexpr int kServeRing = 16;  // batches in flight at most
+// A batch is a run of work units (claimed in order).  A batch of `split` = 0
+// has one unit per tile of rows, each traversing the whole forest.  A batch
+// too small to occupy the device has split > 0: tiles x split leaf units
+// (tile u / split, the u % split-th of split ranges of the forest's chunks),
+// each writing its trees' leaf values to the slot's leaf buffer, then one
+// sum unit per tile, which waits until every leaf unit of the batch is done
+// and adds the leaves of each row in forest order.
 struct ServeDesc {
-  // The batch's tiles, one 64-bit word so that a reader never sees half of a
-  // rewrite: first tile number (tiles are numbered globally) << 16 | count.
+  // The batch's units, one 64-bit word so that a reader never sees half of a
+  // rewrite: first unit number (units are numbered globally) << 16 | count.
   uint64_t range;
   uint64_t in_off, out_off;  // float offsets of its rows and outputs in the arenas
-  uint64_t done_target;  // ServeDevState::tiles_done[slot] once its last tile is done
+  uint64_t done_target;  // ServeDevState::tiles_done[slot] once its last unit is done
+  uint64_t leaf_target;  // split > 0: tiles_done[slot] once every leaf unit is done
+  uint64_t leaf_off;     // split > 0: float offset of the slot's leaf values [trees][rows]
   uint32_t rows, seq;
+  uint32_t split, tiles;
 };
-constexpr uint32_t kServeMaxTiles = 65535;  // tiles of one batch (16 bits of range)
+constexpr uint32_t kServeMaxTiles = 65535;  // units of one batch (16 bits of range)
+constexpr uint32_t kServeMaxSplit = 8;      // leaf units per tile at most (each reads its tile's rows)
 struct ServeHostState {  // pinned coherent host memory
-  uint64_t published;    // tiles published so far (host; a release store)
+  uint64_t published;    // units published so far (host; a release store)
   uint32_t stop;         // host: exit now
-  uint32_t error;        // kernel: a claimed tile matched no descriptor (never expected)
+  uint32_t error;        // kernel: a claimed unit matched no descriptor, or a sum unit's
+                         // wait for its batch's leaf units ran out (never expected)
   ServeDesc desc[kServeRing];
   uint32_t done_seq[kServeRing];  // kernel: seq of the slot's finished batch (a release store)
 };
 struct ServeDevState {   // device memory, zeroed before the first launch
-  uint64_t next_tile;    // the next tile to claim
-  uint64_t tiles_done[kServeRing];
+  uint64_t next_tile;    // the next unit to claim
+  uint64_t tiles_done[kServeRing];  // units done per slot, over all its batches
 };
 // Tile rows of the serving kernel for a model testing F features (0 when the
 // rows cannot be staged in LDS).
@@ -411,8 +423,8 @@ int serve_tile_rows(int F, size_t lds_limit);
 // pinned state; the kernel exits when h->stop is set or after idle_ticks of
 // the device's constant-rate clock with nothing to claim.
 void launch_serve(hipStream_t s, const PredModel& m, ServeHostState* h, ServeDevState* d,
-                  const float* in_base, float* out_base, uint64_t idle_ticks, size_t lds_limit,
-                  int n_cu);
+                  const float* in_base, float* out_base, float* leaf_base, uint64_t idle_ticks,
+                  size_t lds_limit, int n_cu);
 
 // Rows per block and LDS bytes of a predict launch for a model testing F
 // features: rows are staged feature-major when F x rows x 4 B fits beside the

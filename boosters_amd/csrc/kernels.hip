@@ -2611,20 +2611,50 @@ __global__ __launch_bounds__(kPredMaxRows) void k_predict_lds(
                                  transform, out, out_stride, leaf_idx, leaf_stride, leaf_out);
 }
 
+// Second phase of the two-phase launch: per row, base + every tree's leaf in
+// forest order (the one-phase kernel's exact f32 sum), then the transform.
+// leaf_in is [n_trees][n]; also the serving kernel's sum units.
+__device__ __forceinline__ void pred_sum_row(const PredModel& m, const float* __restrict__ leaf_in,
+                                             int64_t n, int64_t row, int transform,
+                                             float* __restrict__ out, int64_t out_stride) {
+  float acc[kPredMaxK];
+#pragma unroll
+  for (int k = 0; k < kPredMaxK; k++) acc[k] = k < m.Kw ? m.base[m.g0 + k] : 0.0f;
+  if (m.K == 1) {
+    for (int64_t t = 0; t < m.n_trees; t++) acc[0] += leaf_in[t * n + row];
+  } else {
+    for (int64_t t = 0; t < m.n_trees; t++) {
+      const uint32_t g = m.tree_group[t] - (uint32_t)m.g0;
+      const float v = leaf_in[t * n + row];
+#pragma unroll
+      for (int k = 0; k < kPredMaxK; k++)
+        if ((uint32_t)k == g) acc[k] += v;
+    }
+  }
+  pred_transform(acc, m.Kw, transform);
+#pragma unroll
+  for (int k = 0; k < kPredMaxK; k++)
+    if (k < m.Kw) out[(int64_t)k * out_stride + row] = acc[k];
+}
+
 // ---- serving kernel --------------------------------------------------------
-// Claims tiles in order (a compare-and-swap on next_tile, only below the
-// published count, so a block never holds a tile that is not yet described),
-// finds the tile's batch among the ring's descriptors, traverses its rows
-// with the prediction body (raw margins), makes the outputs visible to the
-// host, and counts the tile done; the block that finishes a batch's last
-// tile publishes the batch's seq.  Every wave leaves the loop on the stop
-// flag, on idle_ticks of the constant-rate clock without work, or after a
-// bounded number of polls.
+// Claims work units in order (a compare-and-swap on next_tile, only below the
+// published count, so a block never holds a unit that is not yet described),
+// finds the unit's batch among the ring's descriptors and runs it with the
+// prediction body: a whole tile (raw margins), a leaf unit (a range of the
+// forest's chunks, leaf values to the slot's leaf buffer) or a sum unit (the
+// tile's rows summed over every tree in forest order, after every leaf unit
+// of the batch is done: those were claimed before it, by running blocks that
+// wait for nothing).  It makes the outputs visible to the host and counts the
+// unit done; the block that finishes a batch's last unit publishes the
+// batch's seq.  Every wave leaves the loop on the stop flag, on idle_ticks of
+// the constant-rate clock without work, or after a bounded number of polls.
 template <bool CAT>
 __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostState* h,
                                                         ServeDevState* d,
                                                         const float* __restrict__ in_base,
                                                         float* __restrict__ out_base,
+                                                        float* __restrict__ leaf_base,
                                                         uint64_t idle_ticks) {
   __shared__ uint64_t s_tile;
   __shared__ uint32_t s_slot;
@@ -2682,10 +2712,50 @@ __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostSt
     const uint64_t target = __hip_atomic_load(&dp->done_target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const uint32_t rows = __hip_atomic_load(&dp->rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const uint32_t seq = __hip_atomic_load(&dp->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    const int64_t row0 = (int64_t)(g - tile_begin) * blockDim.x;
-    pred_block<true, 0, CAT>(m, in_base + in_off, rows, rows, row0, 0, m.n_chunks, 0,
-                             out_base + out_off, rows, nullptr, 0, nullptr);
-    __threadfence_system();  // this thread's outputs reach the host before the count
+    const uint32_t split = __hip_atomic_load(&dp->split, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint64_t u = g - tile_begin;  // the unit within its batch
+    if (split == 0u) {
+      const int64_t row0 = (int64_t)u * blockDim.x;
+      pred_block<true, 0, CAT>(m, in_base + in_off, rows, rows, row0, 0, m.n_chunks, 0,
+                               out_base + out_off, rows, nullptr, 0, nullptr);
+      __threadfence_system();  // this thread's outputs reach the host before the count
+    } else {
+      const uint32_t tiles = __hip_atomic_load(&dp->tiles, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      const uint64_t leaf_off =
+          __hip_atomic_load(&dp->leaf_off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      float* leaf = leaf_base + leaf_off;
+      if (u < (uint64_t)tiles * split) {
+        // Leaf unit: tile u / split, chunk range u % split of split.
+        // The host picks split so that every range holds chunks.
+        const int per = (m.n_chunks + (int)split - 1) / (int)split;
+        const int c0 = (int)(u % split) * per, c1 = min(c0 + per, m.n_chunks);
+        if (c0 < c1)  // block-uniform
+          pred_block<true, 2, CAT>(m, in_base + in_off, rows, rows,
+                                   (int64_t)(u / split) * blockDim.x, c0, c1, 0, nullptr, 0,
+                                   nullptr, 0, leaf);
+        __threadfence();  // leaf values visible on the device before the count
+      } else {
+        // Sum unit of tile u - tiles * split.
+        const uint64_t leaf_target =
+            __hip_atomic_load(&dp->leaf_target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (threadIdx.x == 0) {
+          uint32_t poll = 0;
+          while (__hip_atomic_load(&d->tiles_done[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+                 leaf_target) {
+            if (++poll > kMaxPolls) {
+              __hip_atomic_store(&h->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+          }
+        }
+        __syncthreads();
+        __threadfence();  // every leaf unit's values are visible to this block
+        const int64_t row = (int64_t)(u - (uint64_t)tiles * split) * blockDim.x + threadIdx.x;
+        if (row < (int64_t)rows) pred_sum_row(m, leaf, rows, row, 0, out_base + out_off, rows);
+        __threadfence_system();  // this thread's outputs reach the host before the count
+      }
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
       const uint64_t old = __hip_atomic_fetch_add(&d->tiles_done[slot], 1ull, __ATOMIC_ACQ_REL,
@@ -2707,31 +2777,13 @@ int serve_tile_rows(int F, size_t lds_limit) {
   return stage ? R : 0;
 }
 
-// Second phase of the two-phase launch: per row, base + every tree's leaf in
-// forest order (the one-phase kernel's exact f32 sum), then the transform.
+// Second phase of the two-phase launch (pred_sum_row per row).
 __global__ __launch_bounds__(256) void k_predict_sum(PredModel m, const float* __restrict__ leaf_in,
                                                      int64_t n, int transform,
                                                      float* __restrict__ out, int64_t out_stride) {
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
-  float acc[kPredMaxK];
-#pragma unroll
-  for (int k = 0; k < kPredMaxK; k++) acc[k] = k < m.Kw ? m.base[m.g0 + k] : 0.0f;
-  if (m.K == 1) {
-    for (int64_t t = 0; t < m.n_trees; t++) acc[0] += leaf_in[t * n + row];
-  } else {
-    for (int64_t t = 0; t < m.n_trees; t++) {
-      const uint32_t g = m.tree_group[t] - (uint32_t)m.g0;
-      const float v = leaf_in[t * n + row];
-#pragma unroll
-      for (int k = 0; k < kPredMaxK; k++)
-        if ((uint32_t)k == g) acc[k] += v;
-    }
-  }
-  pred_transform(acc, m.Kw, transform);
-#pragma unroll
-  for (int k = 0; k < kPredMaxK; k++)
-    if (k < m.Kw) out[(int64_t)k * out_stride + row] = acc[k];
+  pred_sum_row(m, leaf_in, n, row, transform, out, out_stride);
 }
 
 // Output transform of rows whose outputs were accumulated in windows
@@ -2897,19 +2949,19 @@ void launch_predict(hipStream_t s, const PredModel& m, const float* X, int64_t n
 }
 
 void launch_serve(hipStream_t s, const PredModel& m, ServeHostState* h, ServeDevState* d,
-                  const float* in_base, float* out_base, uint64_t idle_ticks, size_t lds_limit,
-                  int n_cu) {
+                  const float* in_base, float* out_base, float* leaf_base, uint64_t idle_ticks,
+                  size_t lds_limit, int n_cu) {
   bool stage = false;
   int R = kPredMaxRows;
   const size_t lds = pred_lds_bytes(m.n_features, lds_limit - kServeLdsReserve, &stage, &R);
   if (m.has_cat) {
     pred_allow_lds(k_serve<true>, lds);
     hipLaunchKernelGGL(k_serve<true>, dim3((unsigned)std::max(n_cu, 1)), dim3(R), lds, s, m, h, d,
-                       in_base, out_base, idle_ticks);
+                       in_base, out_base, leaf_base, idle_ticks);
   } else {
     pred_allow_lds(k_serve<false>, lds);
     hipLaunchKernelGGL(k_serve<false>, dim3((unsigned)std::max(n_cu, 1)), dim3(R), lds, s, m, h, d,
-                       in_base, out_base, idle_ticks);
+                       in_base, out_base, leaf_base, idle_ticks);
   }
 }
 

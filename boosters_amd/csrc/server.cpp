@@ -7,17 +7,22 @@
 // queued requests into batches, one per free slot of a ring of kServeRing
 // slots, packs each batch's rows side by side into the slot's pinned input
 // arena ([F][rows]), and publishes it to the serving kernel (k_serve): one
-// block per CU stays resident and takes the published batches' 256-row tiles
+// block per CU stays resident and takes the published batches' work units
 // in order, so batches that arrive while others are on the GPU are traversed
-// alongside them.  The kernel reads the rows and writes the raw margins
-// directly in pinned coherent host memory; the block that finishes a batch's
-// last tile publishes the batch's sequence number, and the worker copies the
-// outputs to the requests and frees the slot.
+// alongside them.  A unit is a 256-row tile traversed through the whole
+// forest; a batch too small to occupy the device is instead cut into leaf
+// units (its tiles times ranges of the forest, leaf values to the slot's
+// leaf buffer in HBM) and one sum unit per tile, which adds each row's
+// leaves in forest order once the leaf units are done, so that a small
+// request is spread over many CUs.  The kernel reads the rows and writes the
+// raw margins directly in pinned coherent host memory; the block that
+// finishes a batch's last unit publishes the batch's sequence number, and
+// the worker copies the outputs to the requests and frees the slot.
 //
-// Admission: the slot arenas are sized at creation from the caller's budget,
-// capped by the device's free HBM (hipMemGetInfo), and allocated then; a
-// request is admitted when a slot is free and its rows fit the slot beside
-// the requests already packed there.  The kernel leaves after 50 ms without
+// Admission: the slot arenas and leaf buffers are sized at creation from the
+// caller's budget, capped by the device's free HBM (hipMemGetInfo), and
+// allocated then; a request is admitted when a slot is free and its rows fit
+// the slot beside the requests already packed there.  The kernel leaves after 50 ms without
 // work (or on destroy) and the worker relaunches it when a batch is published
 // and no launch is live.  Every row is traversed by the prediction kernel's
 // own tile code with the same per-row summation order, so a result is
@@ -98,6 +103,12 @@ struct bst_server {
   int32_t F = 0, K = 1;
   int tile_rows = 256;
   int64_t slot_rows = 0;   // rows one batch holds
+  // Batches of at most split_rows rows whose tiles leave most CUs idle are
+  // cut into leaf and sum units; leaf holds [n_trees][split_rows] per slot.
+  int64_t split_rows = 0;
+  int n_chunks = 0, n_cu = 1;
+  int64_t n_trees = 0;
+  DevBuf<float> leaf;
   std::vector<Slot> slots;
   // Pinned coherent host memory: state, input and output arenas, and their
   // device addresses.
@@ -136,7 +147,7 @@ void ensure_kernel(bst_server* srv) {
   }
   const bst_model* m = srv->model;
   launch_serve(srv->stream, m->view(), srv->hs_dev, srv->dstate.p, srv->in_dev, srv->out_dev,
-               srv->idle_ticks, srv->ctx->lds_per_block, srv->ctx->n_cu);
+               srv->leaf.p, srv->idle_ticks, srv->ctx->lds_per_block, srv->ctx->n_cu);
   BST_HIP(hipGetLastError());
   BST_HIP(hipEventRecord(srv->kernel_done, srv->stream));
   srv->launched = true;
@@ -161,16 +172,33 @@ void publish(bst_server* srv, int si) {
     }
   });
   const uint32_t n_tiles = (uint32_t)((rows + srv->tile_rows - 1) / srv->tile_rows);
-  srv->cum_done[si] += n_tiles;
-  s.seq = srv->next_seq++;
+  // Leaf units per tile: enough to spread the batch over the CUs, when its
+  // tiles alone would leave more than half of them idle.
+  uint32_t split = 0;
+  if (rows <= srv->split_rows && srv->n_chunks > 1 && 2 * (int64_t)n_tiles <= srv->n_cu) {
+    split = (uint32_t)std::min<int64_t>(
+        {(int64_t)kServeMaxSplit, (int64_t)srv->n_chunks,
+         std::max<int64_t>(2, srv->n_cu / (int64_t)n_tiles)});
+    // Ranges of `per` chunks: only as many as cover the forest, so that no
+    // range starts past its last chunk.
+    const uint32_t per = ((uint32_t)srv->n_chunks + split - 1) / split;
+    split = ((uint32_t)srv->n_chunks + per - 1) / per;
+  }
+  const uint32_t n_units = split ? n_tiles * (split + 1) : n_tiles;
   ServeDesc& d = srv->hs->desc[si];
+  d.leaf_target = srv->cum_done[si] + (uint64_t)n_tiles * split;
+  srv->cum_done[si] += n_units;
+  s.seq = srv->next_seq++;
   d.in_off = (uint64_t)si * srv->slot_rows * F;
   d.out_off = (uint64_t)si * srv->slot_rows * srv->K;
+  d.leaf_off = (uint64_t)si * (uint64_t)srv->n_trees * (uint64_t)srv->split_rows;
   d.done_target = srv->cum_done[si];
   d.rows = (uint32_t)rows;
   d.seq = s.seq;
-  __atomic_store_n(&d.range, (srv->next_tile << 16) | n_tiles, __ATOMIC_RELAXED);
-  srv->next_tile += n_tiles;
+  d.split = split;
+  d.tiles = n_tiles;
+  __atomic_store_n(&d.range, (srv->next_tile << 16) | n_units, __ATOMIC_RELAXED);
+  srv->next_tile += n_units;
   // The rows and the descriptor are written before the tiles are published.
   __atomic_store_n(&srv->hs->published, srv->next_tile, __ATOMIC_RELEASE);
 }
@@ -227,7 +255,9 @@ void worker_loop(bst_server* srv) {
           finished.push_back(si);
       }
       if (__atomic_load_n(&srv->hs->error, __ATOMIC_ACQUIRE))
-        raise(BST_E_DEVICE, "serving kernel found a tile without a batch");
+        raise(BST_E_DEVICE,
+              "serving kernel found a unit without a batch, or a batch's leaf units did not "
+              "finish within the wait bound");
       for (int si : finished) unpack(srv, si);
       bool any_busy = false;
       {
@@ -359,9 +389,24 @@ int bst_server_create(bst_ctx* ctx, const bst_model* model, int64_t budget, int3
     srv->budget = std::min<int64_t>(budget, (int64_t)(free_b / 10 * 9));
     const int n_slots = std::max(1, std::min<int>(max_inflight, kServeRing));
     const int64_t row_bytes = (int64_t)(std::max(F, 1) + K) * 4;
-    srv->slot_rows = std::min<int64_t>(kMaxSlotRows, srv->budget / (n_slots * row_bytes));
+    const bst::PredModel pm = m->view();
+    srv->n_chunks = pm.n_chunks;
+    srv->n_trees = pm.n_trees;
+    srv->n_cu = std::max(1, ctx->n_cu);
+    // Leaf buffers (HBM) for batches of up to half the CUs' worth of tiles,
+    // within a quarter of the budget; none if that holds less than a tile.
+    const int64_t leaf_row_bytes = (int64_t)n_slots * std::max<int64_t>(srv->n_trees, 1) * 4;
+    srv->split_rows = std::min<int64_t>((int64_t)(srv->n_cu / 2) * R, srv->budget / 4 / leaf_row_bytes);
+    srv->split_rows = srv->split_rows / R * R;
+    if (srv->split_rows < R || srv->n_chunks < 2) srv->split_rows = 0;
+    const int64_t leaf_bytes = srv->split_rows * leaf_row_bytes;
+    srv->slot_rows =
+        std::min<int64_t>(kMaxSlotRows, (srv->budget - leaf_bytes) / (n_slots * row_bytes));
     srv->slot_rows = std::min<int64_t>(srv->slot_rows / R * R, (int64_t)kServeMaxTiles * R);
     if (srv->slot_rows < R) raise(BST_E_OOM, "serving budget holds less than one tile per slot");
+    srv->split_rows = std::min(srv->split_rows, srv->slot_rows);
+    if (srv->split_rows > 0)
+      srv->leaf.alloc((size_t)n_slots * (size_t)srv->n_trees * (size_t)srv->split_rows);
     srv->slots.resize(n_slots);
     srv->hs = pinned<ServeHostState>(sizeof(ServeHostState), &srv->hs_dev);
     std::memset(srv->hs, 0, sizeof(ServeHostState));

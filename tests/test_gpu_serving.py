@@ -3,10 +3,11 @@ against bst_predict_device on the same rows.
 
 Every request's output must equal the device prediction of its rows bit for
 bit (the server runs the prediction kernel's own tile code): requests of one
-row to many tiles, more requests than slots (slots reused), submissions from
-several threads at once, a pause longer than the kernel's idle time (the
-kernel leaves and is launched again), softmax outputs, and requests still
-queued when the server is destroyed.
+row to many tiles, batches small enough to be cut into leaf and sum units
+and batches traversed tile by tile, more requests than slots (slots reused),
+submissions from several threads at once, a pause longer than the kernel's
+idle time (the kernel leaves and is launched again), softmax outputs, and
+requests still queued when the server is destroyed.
 """
 import threading
 import time
@@ -84,11 +85,13 @@ def test_server_concurrent_submitters(ctx):
 def test_server_budget_and_shape_errors(ctx):
     info, arr = _random_forest(20, 4, 8, seed=31)
     gm = GBDTModel.from_arrays(info, arr, ctx)
-    # 1 MiB over 2 slots of 9 floats per row: 14,563 rows per slot, 14,336
-    # after rounding to 256-row tiles; a larger request is refused.
+    # The 20-tree forest fits one LDS chunk, so batches are never cut into
+    # leaf units and no leaf buffer is taken from the budget: 1 MiB over 2
+    # slots of 9 floats per row, 14,563 rows per slot, 14,336 after rounding
+    # to 256-row tiles; a larger request is refused.
     srv = Server(gm, hbm_budget_bytes=1 << 20, max_inflight=2)
     with pytest.raises(A.BoostersError) as e:
-        srv.submit(np.zeros((8, 20000), F32))
+        srv.submit(np.zeros((8, 14337), F32))
     assert e.value.code == A.BST_E_OOM
     with pytest.raises(A.BoostersError) as e:
         srv.submit(np.zeros((7, 10), F32))
