@@ -93,6 +93,9 @@ int pack_threads(size_t bytes) {
 
 constexpr int64_t kMaxSlotRows = 1 << 20;
 constexpr double kIdleMs = 50.0;
+// Batches of at least this many input bytes are packed on a thread of their
+// own, so that smaller batches admitted meanwhile are not held behind the copy.
+constexpr size_t kAsyncPackBytes = (size_t)4 << 20;
 
 }  // namespace
 
@@ -134,6 +137,10 @@ struct bst_server {
   bool stop = false;
   std::thread worker;
   std::string worker_error;
+  // Background packing of large batches: the slot's thread, and its state
+  // (0 idle, 1 packing, 2 packed, 3 failed).
+  std::thread packer[kServeRing];
+  std::atomic<int> pack_state[kServeRing] = {};
 };
 
 namespace {
@@ -153,9 +160,9 @@ void ensure_kernel(bst_server* srv) {
   srv->launched = true;
 }
 
-// Pack a slot's requests into its input arena and publish the batch.
-void publish(bst_server* srv, int si) {
-  Slot& s = srv->slots[si];
+// Pack a slot's requests side by side into its input arena ([F][rows]).
+void pack(bst_server* srv, int si) {
+  const Slot& s = srv->slots[si];
   const int32_t F = srv->F;
   const int64_t rows = s.rows;
   float* base = srv->in_arena + (size_t)si * srv->slot_rows * F;
@@ -171,6 +178,13 @@ void publish(bst_server* srv, int si) {
       std::memcpy(base + (size_t)f * rows + offs[i], r->X + (size_t)f * r->n, (size_t)r->n * 4);
     }
   });
+}
+
+// Describe a packed slot's batch and publish its units.
+void publish(bst_server* srv, int si) {
+  Slot& s = srv->slots[si];
+  const int32_t F = srv->F;
+  const int64_t rows = s.rows;
   const uint32_t n_tiles = (uint32_t)((rows + srv->tile_rows - 1) / srv->tile_rows);
   // Leaf units per tile: enough to spread the batch over the CUs, when its
   // tiles alone would leave more than half of them idle.
@@ -248,10 +262,12 @@ void worker_loop(bst_server* srv) {
     for (;;) {
       finished.clear();
       admitted.clear();
-      // Batches whose last tile is done (their seq published by the kernel).
+      // Batches whose last unit is done (their seq published by the kernel);
+      // a slot still being packed holds its previous batch's seq.
       for (int si = 0; si < (int)srv->slots.size(); si++) {
         const Slot& s = srv->slots[si];
-        if (s.busy && __atomic_load_n(&srv->hs->done_seq[si], __ATOMIC_ACQUIRE) == s.seq)
+        if (s.busy && srv->pack_state[si].load(std::memory_order_acquire) == 0 &&
+            __atomic_load_n(&srv->hs->done_seq[si], __ATOMIC_ACQUIRE) == s.seq)
           finished.push_back(si);
       }
       if (__atomic_load_n(&srv->hs->error, __ATOMIC_ACQUIRE))
@@ -300,9 +316,39 @@ void worker_loop(bst_server* srv) {
                                 [&] { return srv->stop || !srv->queue.empty(); });
       }
       if (!finished.empty()) srv->cv_done.notify_all();
-      // Packing and publishing outside the lock: only this thread touches an
-      // admitted slot, and its requests' buffers belong to it until done.
-      for (int si : admitted) publish(srv, si);
+      // Packing and publishing outside the lock: only this thread (and the
+      // slot's packer) touches an admitted slot, and its requests' buffers
+      // belong to it until done.  Large batches are packed in the background
+      // and published once packed; the others at once.
+      for (int si : admitted) {
+        const Slot& s = srv->slots[si];
+        if ((size_t)s.rows * srv->F * 4 < kAsyncPackBytes) {
+          pack(srv, si);
+          publish(srv, si);
+        } else {
+          srv->pack_state[si].store(1, std::memory_order_relaxed);
+          srv->packer[si] = std::thread([srv, si] {
+            int st = 2;
+            try {
+              pack(srv, si);
+            } catch (...) {
+              st = 3;
+            }
+            srv->pack_state[si].store(st, std::memory_order_release);
+          });
+        }
+      }
+      // Packed in the background: publish (which gives the slot its new
+      // seq), then mark the slot idle for the finished check.
+      for (int si = 0; si < (int)srv->slots.size(); si++) {
+        const int st = srv->pack_state[si].load(std::memory_order_acquire);
+        if (st < 2) continue;
+        srv->packer[si].join();
+        if (st == 3) raise(BST_E_OOM, "serving worker: packing a batch failed");
+        publish(srv, si);
+        srv->pack_state[si].store(0, std::memory_order_release);
+        admitted.push_back(si);  // the loop does not idle after a publish
+      }
       if (any_busy || !admitted.empty()) ensure_kernel(srv);
       if (finished.empty() && admitted.empty() && any_busy)
         std::this_thread::sleep_for(std::chrono::microseconds(10));
@@ -316,6 +362,8 @@ void worker_loop(bst_server* srv) {
   } catch (...) {
     fail_pending(srv, BST_E_DEVICE, "serving worker: unknown failure");
   }
+  for (auto& th : srv->packer)
+    if (th.joinable()) th.join();
   stop_kernel(srv);
 }
 
