@@ -415,13 +415,15 @@ struct ServeHostState {  // pinned coherent host memory
 struct ServeDevState {   // device memory, zeroed before the first launch
   uint64_t next_tile;    // the next unit to claim
   uint64_t tiles_done[kServeRing];  // units done per slot, over all its batches
+  uint64_t last_claim;   // constant-rate clock of the latest claim by any block
 };
 // Tile rows of the serving kernel for a model testing F features (0 when the
 // rows cannot be staged in LDS).
 int serve_tile_rows(int F, size_t lds_limit);
 // One persistent launch of n_cu blocks; `h` is the device address of the
-// pinned state; the kernel exits when h->stop is set or after idle_ticks of
-// the device's constant-rate clock with nothing to claim.
+// pinned state; the kernel exits when h->stop is set or once no block has
+// claimed a unit for idle_ticks of the device's constant-rate clock (the
+// blocks leave together, so a launch never lingers with a few of them).
 void launch_serve(hipStream_t s, const PredModel& m, ServeHostState* h, ServeDevState* d,
                   const float* in_base, float* out_base, float* leaf_base, uint64_t idle_ticks,
                   size_t lds_limit, int n_cu);

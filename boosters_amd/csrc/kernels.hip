@@ -2647,8 +2647,9 @@ __device__ __forceinline__ void pred_sum_row(const PredModel& m, const float* __
 // of the batch is done: those were claimed before it, by running blocks that
 // wait for nothing).  It makes the outputs visible to the host and counts the
 // unit done; the block that finishes a batch's last unit publishes the
-// batch's seq.  Every wave leaves the loop on the stop flag, on idle_ticks of
-// the constant-rate clock without work, or after a bounded number of polls.
+// batch's seq.  Every wave leaves the loop on the stop flag, once no block
+// has claimed a unit for idle_ticks of the constant-rate clock, or after a
+// bounded number of polls.
 template <bool CAT>
 __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostState* h,
                                                         ServeDevState* d,
@@ -2664,6 +2665,8 @@ __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostSt
     if (threadIdx.x == 0) {
       uint64_t got = ~0ull;
       const uint64_t t0 = wall_clock64();
+      // The poll bound counts polls since any block last claimed a unit.
+      uint64_t seen_last = 0;
       for (uint32_t poll = 0; poll < kMaxPolls; poll++) {
         const uint64_t pub = __hip_atomic_load(&h->published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
         uint64_t cur = __hip_atomic_load(&d->next_tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2671,12 +2674,23 @@ __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostSt
           if (__hip_atomic_compare_exchange_strong(&d->next_tile, &cur, cur + 1, __ATOMIC_RELAXED,
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             got = cur;
+            __hip_atomic_store(&d->last_claim, wall_clock64(), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
           continue;
         }
         if (__hip_atomic_load(&h->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
-        if (wall_clock64() - t0 > idle_ticks) break;
+        // Idle: leave only when no block has claimed a unit for idle_ticks
+        // (a block that finds nothing while others work stays resident).
+        const uint64_t now = wall_clock64();
+        const uint64_t last =
+            __hip_atomic_load(&d->last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (now - (last > t0 ? last : t0) > idle_ticks) break;
+        if (last != seen_last) {
+          seen_last = last;
+          poll = 0;
+        }
         __builtin_amdgcn_s_sleep(127);  // ~8k cycles between polls of host memory
       }
       uint32_t slot = kServeRing;
