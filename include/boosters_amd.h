@@ -340,7 +340,15 @@ int bst_predict_device_noalloc(bst_ctx* ctx, const bst_model* model, const float
                                int64_t n_rows, int32_t n_feat, int32_t raw, float* out_dev,
                                uint32_t* leaf_idx_dev);
 
-/* ---- serving loop (extension: continuous batching) --------------------- */
+/* ---- serving loop (extension: continuous batching) ---------------------
+ * One persistent traversal launch serves every request.  Requests are
+ * coalesced into up to max_inflight (at most 16) batches held in pinned host
+ * arenas; a batch too small to occupy the device is spread over it through a
+ * per-slot leaf-value buffer in HBM.  The arenas and leaf buffers are sized
+ * from hbm_budget_bytes (capped at 90% of the device's free HBM) at creation;
+ * nothing is allocated afterwards, and a request larger than a batch is
+ * refused (BST_E_OOM).  Outputs equal bst_predict's raw margins bit for bit,
+ * whether a request ran alone or batched with others. */
 int bst_server_create(bst_ctx* ctx, const bst_model* model, int64_t hbm_budget_bytes,
                       int32_t max_inflight, bst_server** out);
 /* rows: host [n_feat][n_rows]; out: host [n_groups][n_rows] raw margins. */
