@@ -391,9 +391,14 @@ constexpr int kServeRing = 16;  // batches in flight at most
 // each writing its trees' leaf values to the slot's leaf buffer, then one
 // sum unit per tile, which waits until every leaf unit of the batch is done
 // and adds the leaves of each row in forest order.
+// Units are claimed in two lanes, each in publication order: lane 0 holds
+// the batches cut into leaf and sum units (small ones), lane 1 the others,
+// and a block looks at lane 0 first, so a small request is not queued behind
+// a large batch's tiles.
 struct ServeDesc {
   // The batch's units, one 64-bit word so that a reader never sees half of a
-  // rewrite: first unit number (units are numbered globally) << 16 | count.
+  // rewrite: lane << 63 | first unit number (units are numbered per lane)
+  // << 16 | count.
   uint64_t range;
   uint64_t in_off, out_off;  // float offsets of its rows and outputs in the arenas
   uint64_t done_target;  // ServeDevState::tiles_done[slot] once its last unit is done
@@ -403,9 +408,10 @@ struct ServeDesc {
   uint32_t split, tiles;
 };
 constexpr uint32_t kServeMaxTiles = 65535;  // units of one batch (16 bits of range)
+constexpr uint64_t kServeUnitMask = (1ull << 47) - 1;  // first-unit field of range
 constexpr uint32_t kServeMaxSplit = 8;      // leaf units per tile at most (each reads its tile's rows)
 struct ServeHostState {  // pinned coherent host memory
-  uint64_t published;    // units published so far (host; a release store)
+  uint64_t published[2];  // units published so far per lane (host; release stores)
   uint32_t stop;         // host: exit now
   uint32_t error;        // kernel: a claimed unit matched no descriptor, or a sum unit's
                          // wait for its batch's leaf units ran out (never expected)
@@ -413,7 +419,8 @@ struct ServeHostState {  // pinned coherent host memory
   uint32_t done_seq[kServeRing];  // kernel: seq of the slot's finished batch (a release store)
 };
 struct ServeDevState {   // device memory, zeroed before the first launch
-  uint64_t next_tile;    // the next unit to claim
+  uint64_t next_unit[2];  // the next unit to claim per lane
+  uint64_t seen_pub[2];   // published[] as last read from host memory by any block
   uint64_t tiles_done[kServeRing];  // units done per slot, over all its batches
   uint64_t last_claim;   // constant-rate clock of the latest claim by any block
 };

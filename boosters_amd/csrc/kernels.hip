@@ -2638,8 +2638,9 @@ __device__ __forceinline__ void pred_sum_row(const PredModel& m, const float* __
 }
 
 // ---- serving kernel --------------------------------------------------------
-// Claims work units in order (a compare-and-swap on next_tile, only below the
-// published count, so a block never holds a unit that is not yet described),
+// Claims work units in order, lane 0 (small batches) before lane 1 (a
+// compare-and-swap on the lane's next unit, only below its published count,
+// so a block never holds a unit that is not yet described),
 // finds the unit's batch among the ring's descriptors and runs it with the
 // prediction body: a whole tile (raw margins), a leaf unit (a range of the
 // forest's chunks, leaf values to the slot's leaf buffer) or a sum unit (the
@@ -2667,19 +2668,40 @@ __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostSt
       const uint64_t t0 = wall_clock64();
       // The poll bound counts polls since any block last claimed a unit.
       uint64_t seen_last = 0;
+      uint32_t glane = 0;
       for (uint32_t poll = 0; poll < kMaxPolls; poll++) {
-        const uint64_t pub = __hip_atomic_load(&h->published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-        uint64_t cur = __hip_atomic_load(&d->next_tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cur < pub) {
-          if (__hip_atomic_compare_exchange_strong(&d->next_tile, &cur, cur + 1, __ATOMIC_RELAXED,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            got = cur;
-            __hip_atomic_store(&d->last_claim, wall_clock64(), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            break;
+        bool lost = false;  // a compare-and-swap lost a race: look again at once
+        // A block reads the host's published counts on one poll in 8 and
+        // folds them into a device copy that the other polls read: far fewer
+        // reads across the host link while many blocks wait for work.
+        const bool host_poll = ((poll + blockIdx.x) & 7u) == 0u;
+        for (uint32_t ln = 0; ln < 2u && got == ~0ull && !lost; ln++) {
+          uint64_t pub;
+          if (host_poll) {
+            pub = __hip_atomic_load(&h->published[ln], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_fetch_max(&d->seen_pub[ln], pub, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          } else {
+            pub = __hip_atomic_load(&d->seen_pub[ln], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
           }
-          continue;
+          uint64_t cur =
+              __hip_atomic_load(&d->next_unit[ln], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (cur < pub) {
+            if (__hip_atomic_compare_exchange_strong(&d->next_unit[ln], &cur, cur + 1,
+                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+              got = cur;
+              glane = ln;
+            } else {
+              lost = true;
+            }
+          }
         }
+        if (got != ~0ull) {
+          __hip_atomic_store(&d->last_claim, wall_clock64(), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        if (lost) continue;
         if (__hip_atomic_load(&h->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
         // Idle: leave only when no block has claimed a unit for idle_ticks
         // (a block that finds nothing while others work stays resident).
@@ -2698,8 +2720,8 @@ __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostSt
         for (uint32_t k = 0; k < (uint32_t)kServeRing; k++) {
           const uint32_t i = (cursor + k) % kServeRing;
           const uint64_t rg = __hip_atomic_load(&h->desc[i].range, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-          const uint64_t b = rg >> 16, nt = rg & 0xFFFFu;
-          if (got >= b && got < b + nt) {
+          const uint64_t b = (rg >> 16) & kServeUnitMask, nt = rg & 0xFFFFu;
+          if ((uint32_t)(rg >> 63) == glane && got >= b && got < b + nt) {
             slot = i;
             break;
           }
@@ -2720,7 +2742,8 @@ __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostSt
     if (g == ~0ull) return;
     const ServeDesc* dp = &h->desc[slot];
     const uint64_t tile_begin =
-        __hip_atomic_load(&dp->range, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >> 16;
+        (__hip_atomic_load(&dp->range, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >> 16) &
+        kServeUnitMask;
     const uint64_t in_off = __hip_atomic_load(&dp->in_off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const uint64_t out_off = __hip_atomic_load(&dp->out_off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const uint64_t target = __hip_atomic_load(&dp->done_target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);

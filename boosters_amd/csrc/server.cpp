@@ -126,7 +126,7 @@ struct bst_server {
   hipEvent_t kernel_done = nullptr;
   bool launched = false;
   uint64_t idle_ticks = 0;
-  uint64_t next_tile = 0;                // tiles published so far
+  uint64_t next_unit[2] = {0, 0};        // units published so far per lane
   uint64_t cum_done[kServeRing] = {};    // tiles_done[slot] once its batches finish
   uint32_t next_seq = 1;
   std::mutex mu;
@@ -211,10 +211,13 @@ void publish(bst_server* srv, int si) {
   d.seq = s.seq;
   d.split = split;
   d.tiles = n_tiles;
-  __atomic_store_n(&d.range, (srv->next_tile << 16) | n_units, __ATOMIC_RELAXED);
-  srv->next_tile += n_units;
-  // The rows and the descriptor are written before the tiles are published.
-  __atomic_store_n(&srv->hs->published, srv->next_tile, __ATOMIC_RELEASE);
+  // Batches cut into units go to lane 0, which blocks look at first.
+  const uint64_t lane = split ? 0u : 1u;
+  __atomic_store_n(&d.range, (lane << 63) | (srv->next_unit[lane] << 16) | n_units,
+                   __ATOMIC_RELAXED);
+  srv->next_unit[lane] += n_units;
+  // The rows and the descriptor are written before the units are published.
+  __atomic_store_n(&srv->hs->published[lane], srv->next_unit[lane], __ATOMIC_RELEASE);
 }
 
 // Copy a finished slot's outputs to its requests (outside the lock).
