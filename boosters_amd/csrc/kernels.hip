@@ -2669,12 +2669,13 @@ __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostSt
       // The poll bound counts polls since any block last claimed a unit.
       uint64_t seen_last = 0;
       uint32_t glane = 0;
+      uint32_t hp = blockIdx.x;  // counts every poll (the bound's count restarts)
       for (uint32_t poll = 0; poll < kMaxPolls; poll++) {
         bool lost = false;  // a compare-and-swap lost a race: look again at once
         // A block reads the host's published counts on one poll in 8 and
         // folds them into a device copy that the other polls read: far fewer
         // reads across the host link while many blocks wait for work.
-        const bool host_poll = ((poll + blockIdx.x) & 7u) == 0u;
+        const bool host_poll = (hp++ & 7u) == 0u;
         for (uint32_t ln = 0; ln < 2u && got == ~0ull && !lost; ln++) {
           uint64_t pub;
           if (host_poll) {
@@ -2702,7 +2703,9 @@ __global__ __launch_bounds__(kPredMaxRows) void k_serve(PredModel m, ServeHostSt
           break;
         }
         if (lost) continue;
-        if (__hip_atomic_load(&h->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
+        // The stop flag is read with the published counts (host-link reads).
+        if (host_poll && __hip_atomic_load(&h->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+          break;
         // Idle: leave only when no block has claimed a unit for idle_ticks
         // (a block that finds nothing while others work stays resident).
         const uint64_t now = wall_clock64();
