@@ -262,13 +262,20 @@ __device__ __forceinline__ void objective_pair(int objective, const ObjParams& o
   }
 }
 
+// Threads per block of k_gradients<KM>: the many-output instance keeps KM
+// running maxima and the class loop's values per row, which fit in registers
+// only at 256 threads per block (1024 would spill them to scratch).
 template <int KM>
-__global__ __launch_bounds__(kGradThreads) void k_gradients(
+constexpr int grad_threads() { return KM == 1 ? kGradThreads : 256; }
+
+template <int KM>
+__global__ __launch_bounds__(grad_threads<KM>()) void k_gradients(
     int objective, ObjParams op, const float* __restrict__ preds, const float* __restrict__ y,
     const float* __restrict__ w, int64_t n, int K, float* __restrict__ g, float* __restrict__ h,
     uint32_t* __restrict__ absmax) {
   const float HESS_MIN = 1e-6f;
-  __shared__ uint32_t s_max[2 * KM][kGradThreads / WAVE];
+  constexpr int NT = grad_threads<KM>();
+  __shared__ uint32_t s_max[2 * KM][NT / WAVE];
   uint32_t mx_g[KM], mx_h[KM];
 #pragma unroll
   for (int k = 0; k < KM; k++) {
@@ -363,7 +370,7 @@ __global__ __launch_bounds__(kGradThreads) void k_gradients(
   __syncthreads();
   if ((int)threadIdx.x < 2 * K) {
     uint32_t v = 0u;
-    for (int i = 0; i < kGradThreads / WAVE; i++) v = max(v, s_max[threadIdx.x][i]);
+    for (int i = 0; i < NT / WAVE; i++) v = max(v, s_max[threadIdx.x][i]);
     atomicMax(&absmax[threadIdx.x], v);
   }
 }
@@ -418,12 +425,15 @@ void launch_gradients(hipStream_t s, int objective, const ObjParams& op, const f
                        (size_t)2 * K * sizeof(uint32_t), s, preds, y, w, n, K, g, h, absmax);
     return;
   }
-  if (K == 1 && objective != 2)
+  if (K == 1 && objective != 2) {
     hipLaunchKernelGGL(k_gradients<1>, dim3((unsigned)blocks), dim3(kGradThreads), 0, s,
                        objective, op, preds, y, w, n, K, g, h, absmax);
-  else
-    hipLaunchKernelGGL(k_gradients<kGradMaxK>, dim3((unsigned)blocks), dim3(kGradThreads), 0, s,
-                       objective, op, preds, y, w, n, K, g, h, absmax);
+  } else {
+    constexpr int nt = grad_threads<kGradMaxK>();
+    const int64_t b = std::min<int64_t>((n + nt - 1) / nt, 1024);
+    hipLaunchKernelGGL(k_gradients<kGradMaxK>, dim3((unsigned)b), dim3(nt), 0, s, objective, op,
+                       preds, y, w, n, K, g, h, absmax);
+  }
 }
 
 // ---------------------------------------------------------------------------
