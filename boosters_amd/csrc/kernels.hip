@@ -880,10 +880,12 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
       s_g[i] = 0;
       s_h[i] = 0;
     }
-    // Padding lanes (f >= E) read their own bytes: the row stride covers
-    // every group's FG bin bytes and FG / 8 NaN-bit bytes, which are zero
-    // there, so they add into columns nobody reads.
+    // Padding lanes (f >= E) load their own bytes (the row stride covers
+    // every group's FG bin bytes and FG / 8 NaN-bit bytes) but add nothing:
+    // their columns are never read, and the LDS atomics they skip (12 of 16
+    // lanes in the last group of 100 features) are the kernel's bottleneck.
     const int f = (int)it.group * FG + lf;
+    const bool live = f < E;
     const int hb = hi_off + (f >> 3), hs = f & 7;
     // The NaN bit (bit hs of its byte) shifted to the bin-256 bit of the
     // byte offset below: bit log2(256 * FG * 8).
@@ -950,8 +952,10 @@ __attribute__((amdgpu_waves_per_eu(hist_waves_per_eu<FG>()))) void k_hist(
         uint32_t off = (uint32_t)cb[u] * (uint32_t)(FG * 8);
         if (HAS_HI) off |= ((uint32_t)ch[u] << hsh) & (256u * FG * 8u);
         char* const gp = lane_g + off;
-        atomicAdd((unsigned long long*)gp, (unsigned long long)g);
-        atomicAdd((unsigned long long*)(gp + kPlaneBytes), (unsigned long long)h);
+        if (live) {
+          atomicAdd((unsigned long long*)gp, (unsigned long long)g);
+          atomicAdd((unsigned long long*)(gp + kPlaneBytes), (unsigned long long)h);
+        }
       });
     };
     __syncthreads();  // the LDS zeroing is done before any wave adds
@@ -1690,6 +1694,16 @@ __device__ unsigned long long split_feature(BinLoad load, const FeatMeta& fm, ui
   return ((unsigned long long)ord_f32(out.gain) << 32) | tb;
 }
 
+// Staged bin b of the block's feature w: features 257 entries apart and the
+// low two bits of b rotated by bits 3-4 and 5-6 of b, so that the staging
+// stores (8 consecutive lanes: 8 features of one bin) and the scans' loads
+// (lane i: bins 4i .. 4i + 3 of one feature) both spread each 8 or 16
+// consecutive lanes over distinct 16-byte bank groups (unpadded, they took 8
+// and 4 passes).
+__device__ __forceinline__ int split_lds_index(int w, int b) {
+  return w * 257 + (b ^ (((b >> 3) + (b >> 5)) & 3));
+}
+
 // One block = one node x kSplitWaves consecutive candidate features.  The
 // block first stages the features' 256 bins into LDS ([feature][bin]): with
 // the [group][bin][FG] histogram layout the features' pairs of one bin are
@@ -1713,7 +1727,7 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
   constexpr int kScratch = SORTED ? 256 : 1;
   __shared__ double s_ratio[kSplitWaves][kScratch];
   __shared__ uint16_t s_order[kSplitWaves][kScratch];
-  __shared__ longlong2 s_bins[kSplitWaves][256];
+  __shared__ longlong2 s_bins[kSplitWaves * 257];
   __shared__ unsigned long long s_key[kSplitWaves];
   constexpr int kThreads = kSplitWaves * 64;
   const int wv = threadIdx.x / WAVE;
@@ -1762,12 +1776,12 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
             v.x -= m.x;
             v.y -= m.y;
             if (wback) *(longlong2*)(base + e) = v;
-            s_bins[w][b] = v;
+            s_bins[split_lds_index(w, b)] = v;
           }
         } else {
           for (int b = threadIdx.x / kSplitWaves; b < 256; b += kThreads / kSplitWaves) {
             const int64_t e = ((int64_t)(grp * 256 + b) * FG + lf) * 2;
-            s_bins[w][b] = *(const longlong2*)(base + e);
+            s_bins[split_lds_index(w, b)] = *(const longlong2*)(base + e);
           }
         }
       }
@@ -1777,7 +1791,7 @@ __global__ __launch_bounds__(kSplitWaves * 64) void k_split(
     if (fok) {
       const double ig = ldexp(1.0, -qg), ih = ldexp(1.0, -qh);
       const GainCtx ctx((double)G * ig, (double)H * ih, p);
-      auto load = [&](uint32_t b) { return s_bins[wv][b]; };
+      auto load = [&](uint32_t b) { return s_bins[split_lds_index(wv, (int)b)]; };
       key = split_feature<SORTED>(load, fm, node, f, G, H, cnt, ig, ih, p, ctx, s_ratio[wv],
                                   s_order[wv], E, cand);
     }
