@@ -298,10 +298,10 @@ void bst_model::upload(hipStream_t s) {
   h2d(s, d_cat_words.p, cat_words.data(), cat_words.size());
   h2d(s, d_base.p, base_score.data(), base_score.size());
   // Predictor layout: trees of at most kPredChunkNodes nodes testing features
-  // below 2^16 get a compact copy, renumbered breadth first so that a node's
-  // children are adjacent (left, left + 1), and are grouped into LDS chunks of
-  // consecutive trees; any other tree is a chunk of its own, traversed in the
-  // 16-byte layout above.
+  // below 2^16 - 1 get a compact copy (engine.h), renumbered breadth first so
+  // that a node's children are adjacent (left, left + 1), and are grouped
+  // into LDS chunks of consecutive trees; any other tree is a chunk of its
+  // own, traversed in the 16-byte layout above.
   const size_t T = tree_offset.empty() ? 0 : tree_offset.size() - 1;
   need(T == 0 || tree_offset.back() < (int64_t)UINT32_MAX, BST_E_SHAPE,
        "forest has more than 2^32 nodes");
@@ -311,56 +311,71 @@ void bst_model::upload(hipStream_t s) {
     return u;
   };
   std::vector<uint2> cn;
-  std::vector<uint32_t> ctoff(std::max<size_t>(T, 1), 0), cref;
+  std::vector<uint32_t> ctoff(std::max<size_t>(T, 1), 0), cdep(std::max<size_t>(T, 1), 0), cref;
+  std::vector<uint32_t> corig;  // per compact node
   std::vector<PredChunk> ch;
   std::vector<uint32_t> new_id, order;
   auto compactable = [&](size_t t) {
     const int64_t b = tree_offset[t], e = tree_offset[t + 1];
     if (e - b + 1 > kPredChunkNodes) return false;  // with the chunk's sentinel leaf
     for (int64_t k = b; k < e; k++)
-      if (!is_leaf[k] && split_index[k] >= 65536u) return false;
+      if (!is_leaf[k] && split_index[k] >= 65535u) return false;
     return true;
   };
-  auto append_compact = [&](size_t t) {
+  // Tree t appended to the chunk that starts at compact node `begin`.
+  std::vector<uint32_t> depth;
+  auto append_compact = [&](size_t t, uint32_t begin) {
     const int64_t b = tree_offset[t];
     const uint32_t nn = (uint32_t)(tree_offset[t + 1] - b);
     new_id.assign(nn, UINT32_MAX);
+    depth.assign(nn, 0u);
     order.clear();
     new_id[0] = 0;
     order.push_back(0);
-    uint32_t next = 1;
+    uint32_t next = 1, levels = 0;
     for (size_t i = 0; i < order.size(); i++) {
       const uint32_t o = order[i];
+      levels = std::max(levels, depth[o]);
       if (is_leaf[b + o]) continue;
       const uint32_t l = left[b + o], r = right[b + o];
       new_id[l] = next;
       new_id[r] = next + 1;
+      depth[l] = depth[r] = depth[o] + 1u;
       next += 2;
       order.push_back(l);
       order.push_back(r);
     }
     const size_t c0 = cn.size();
+    const uint32_t rel = (uint32_t)c0 - begin;  // the tree's root index in the chunk
     ctoff[t] = (uint32_t)c0;
+    cdep[t] = levels;
     cn.resize(c0 + next, make_uint2(0u, 0u));
+    corig.resize(c0 + next, 0u);
     for (uint32_t o : order) {
       const int64_t k = b + o;
       uint2 w;
       if (is_leaf[k]) {
-        w.x = o | kCLeaf | (new_id[o] << kCShift);
-        w.y = fbits(leaf_value[k]);
+        const float v = leaf_value[k];
+        w.x = (rel + new_id[o]) << kCShift;
+        w.y = ~(std::isnan(v) ? 0x7FC00000u : fbits(v));
       } else {
-        w.x = split_index[k] | (default_left[k] ? kCDefLeft : 0u) |
-              (new_id[left[k]] << kCShift);
+        const uint32_t dl = default_left[k] ? 1u : 0u;
+        w.x = (split_index[k] + 1u) | (dl ? kCDefLeft : 0u) |
+              ((rel + new_id[left[k]]) << kCShift);
         if (split_type[k]) {
           w.x |= kCCat;
-          w.y = (uint32_t)(cref.size() / 2);
+          w.y = (uint32_t)(cref.size() / 2) + 1u;
           cref.push_back(cat_start[k]);
           cref.push_back(cat_size[k]);
         } else {
-          w.y = fbits(split_threshold[k]);
+          // A NaN threshold sends every value right (x < NaN is false) and
+          // NaN by default_left: key 0.
+          const float thr = split_threshold[k];
+          w.y = (std::isnan(thr) ? 0u : pred_order_key(fbits(thr))) + dl;
         }
       }
       cn[c0 + new_id[o]] = w;
+      corig[c0 + new_id[o]] = o;
     }
   };
   for (size_t t = 0; t < T;) {
@@ -369,29 +384,44 @@ void bst_model::upload(hipStream_t s) {
       t++;
       continue;
     }
-    if (cn.size() & 1) cn.push_back(make_uint2(0u, 0u));  // chunks start 16-byte aligned
+    if (cn.size() & 1) {  // chunks start 16-byte aligned
+      cn.push_back(make_uint2(0u, 0u));
+      corig.push_back(0u);
+    }
     const uint32_t begin = (uint32_t)cn.size();
-    size_t e = t;
-    // The chunk's trees and its sentinel leaf fit the buffer.
+    // The trees that fit the buffer with the chunk's sentinel leaf; a few
+    // more than a multiple of kPredILP are cut back to it.
+    size_t e = t, fill = 0;
     while (e < T && compactable(e) &&
-           cn.size() - begin + (size_t)(tree_offset[e + 1] - tree_offset[e]) + 1 <=
-               (size_t)kPredChunkNodes)
-      append_compact(e++);
-    cn.push_back(make_uint2(kCLeaf, 0u));  // sentinel leaf
+           fill + (size_t)(tree_offset[e + 1] - tree_offset[e]) + 1 <= (size_t)kPredChunkNodes) {
+      fill += (size_t)(tree_offset[e + 1] - tree_offset[e]);
+      e++;
+    }
+    const size_t nt = e - t, idle = (kPredILP - nt % kPredILP) % kPredILP;
+    if (nt > (size_t)kPredILP && idle * 10 > nt) e = t + nt / kPredILP * kPredILP;
+    for (size_t u = t; u < e; u++) append_compact(u, begin);
+    // Sentinel leaf (value 0) at its own index.
+    cn.push_back(make_uint2(((uint32_t)cn.size() - begin) << kCShift, 0xFFFFFFFFu));
+    corig.push_back(0u);
     ch.push_back({(uint32_t)t, (uint32_t)e, begin, (uint32_t)(cn.size() - begin)});
     t = e;
   }
   if (cn.size() & 1) cn.push_back(make_uint2(0u, 0u));
   if (cn.empty()) cn.assign(2, make_uint2(0u, 0u));
+  corig.resize(cn.size(), 0u);
   pred_has_cat = cref.empty() ? 0 : 1;
   if (cref.empty()) cref.assign(2, 0u);
   n_chunks = (int)ch.size();
   d_cnodes.alloc(cn.size());
   d_ctree_off.alloc(ctoff.size());
+  d_cdepth.alloc(cdep.size());
+  d_corig.alloc(corig.size());
   d_cat_ref.alloc(cref.size());
   d_chunks.alloc(std::max<size_t>(ch.size(), 1));
   h2d(s, d_cnodes.p, cn.data(), cn.size());
   h2d(s, d_ctree_off.p, ctoff.data(), ctoff.size());
+  h2d(s, d_cdepth.p, cdep.data(), cdep.size());
+  h2d(s, d_corig.p, corig.data(), corig.size());
   h2d(s, d_cat_ref.p, cref.data(), cref.size());
   h2d(s, d_chunks.p, ch.data(), ch.size());
   BST_HIP(hipStreamSynchronize(s));

@@ -340,16 +340,38 @@ void launch_leaf_update(hipStream_t s, const Segment* segs, int n_segs, const fl
 // that is traversed from global memory in the 16-byte layout (n_nodes = 0).
 constexpr int kPredChunkNodes = 3584;  // 28 KiB of 8-byte nodes per LDS buffer
 constexpr int kPredMaxRows = 256;      // rows per block at most, one per lane
-// Compact node, x: split: feature (bits 0..15) | default_left (17) |
-// categorical (18) | the left child's id in the compact tree (bits 19..31;
-// the right child is the next id); leaf: its original tree-local id (bits
-// 0..15, leaf indices report original ids) | leaf (16) | its own compact id
-// (bits 19..31), so that a traversal step on a leaf stays on it.  y:
-// threshold bits, leaf value bits, or for a categorical split its index in
-// PredModel::cat_ref.  Every LDS chunk ends with a sentinel leaf (x = leaf
-// flag only) that chains past the chunk's last tree sit on.
-constexpr uint32_t kCLeaf = 1u << 16, kCDefLeft = 1u << 17, kCCat = 1u << 18;
+// Trees a lane traverses together (independent chains).  3584 = 7 x 512, so
+// a chunk holds 7 complete depth-8 trees (511 nodes), 14 of depth 7, ...;
+// a chunk that fits a few more than a multiple of 7 takes only that
+// multiple (a group of fewer trees leaves chains idle).
+constexpr int kPredILP = 7;
+// A block stages the rows' feature values as order keys (pred_order_key):
+// staged row 0 is all zeros and feature f is staged row f + 1.
+// Compact node, x: split: staged row of its feature (f + 1, bits 0..15) |
+// default_left (17) | categorical (18) | the left child's index in the LDS
+// chunk (bits 19..31; the right child is the next index); leaf: staged row 0
+// (bits 0..15 zero) | its own index in the chunk (bits 19..31).  y: numeric
+// split: the threshold's order key + default_left; categorical split: its
+// index in PredModel::cat_ref + 1; leaf: the bitwise NOT of its value's bits
+// (a NaN value canonical, 0x7FC00000).  A traversal step goes to the child
+// (x >> 19) + [key(x) + default_left >= y] (unsigned): NaN (key 0xFFFFFFFF)
+// wraps to 0 with default_left and goes left, else right, and a leaf (key 0,
+// y never 0) stays where it is.  Every LDS chunk ends with a sentinel leaf
+// that chains past the chunk's last tree sit on.
+constexpr uint32_t kCDefLeft = 1u << 17, kCCat = 1u << 18;
 constexpr int kCShift = 19;
+// Order key of an f32 for the compact traversal: unsigned order of keys is
+// the order of the values (-0 and +0 equal), NaN is 0xFFFFFFFF, and every
+// other key is at most 0xFF800000 (+inf).
+__host__ __device__ inline uint32_t pred_order_key(uint32_t u) {
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0xFFFFFFFFu;  // NaN
+  if (u == 0x80000000u) u = 0u;                                // -0 -> +0
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+// The value of an order key (-0 comes back as +0, NaN as 0x7FFFFFFF).
+__host__ __device__ inline uint32_t pred_key_bits(uint32_t k) {
+  return (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+}
 struct PredChunk {
   uint32_t first_tree, end_tree;  // trees [first_tree, end_tree)
   uint32_t node_begin;            // LDS chunk: first compact node (even); else unused
@@ -360,6 +382,8 @@ struct PredModel {
   const int64_t* tree_off;     // [n_trees + 1] into nodes
   const uint2* cnodes;         // compact nodes of the LDS chunks' trees
   const uint32_t* ctree_off;   // [n_trees] first compact node of each compact tree
+  const uint32_t* cdepth;      // [n_trees] levels of each compact tree (its deepest leaf)
+  const uint32_t* corig;       // [compact nodes] the node's original tree-local id
   const uint32_t* cat_ref;     // [2 x categorical compact nodes]: cat_start, cat_size
   const uint32_t* tree_group;
   const uint32_t *cat_start, *cat_size, *cat_words;

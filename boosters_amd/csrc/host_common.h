@@ -368,9 +368,10 @@ struct bst_model {
   bst::DevBuf<uint32_t> d_tree_group, d_cat_start, d_cat_size, d_cat_words;
   bst::DevBuf<float> d_base;
   // Predictor layout (engine.h): compact nodes of the LDS chunks' trees, their
-  // offsets, the categorical references and the chunk table.
+  // offsets, levels and original ids, the categorical references and the
+  // chunk table.
   bst::DevBuf<uint2> d_cnodes;
-  bst::DevBuf<uint32_t> d_ctree_off, d_cat_ref;
+  bst::DevBuf<uint32_t> d_ctree_off, d_cdepth, d_corig, d_cat_ref;
   bst::DevBuf<bst::PredChunk> d_chunks;
   int n_chunks = 0;
   int pred_has_cat = 0;
@@ -386,6 +387,8 @@ struct bst_model {
     v.cat_words = d_cat_words.p;
     v.cnodes = d_cnodes.p;
     v.ctree_off = d_ctree_off.p;
+    v.cdepth = d_cdepth.p;
+    v.corig = d_corig.p;
     v.cat_ref = d_cat_ref.p;
     v.chunks = d_chunks.p;
     v.n_chunks = n_chunks;

@@ -2359,7 +2359,6 @@ __device__ __forceinline__ float sigmoid32(float x) {  // transform.rs:106-115
 }
 
 constexpr int kPredMaxK = 16;
-constexpr int kPredILP = 8;  // trees a lane traverses together (independent chains)
 constexpr int kPredChunk16 = kPredChunkNodes / 2;          // uint4 per node buffer
 constexpr int kPredPer = (kPredChunk16 + kPredMaxRows - 1) / kPredMaxRows;  // staged uint4 per lane
 
@@ -2400,13 +2399,15 @@ __device__ __forceinline__ void pred_transform(float* acc, int K, int transform)
 
 // Traversal with the forest staged in LDS, one lane per row.
 //
-// A block owns blockDim.x rows (staged feature-major in LDS when STAGE_X, so
-// a node test reads s_x[f][lane]: consecutive lanes, conflict free) and walks
-// the chunks [c0, c1) of the forest in order.  An LDS chunk's compact nodes
-// are copied into one of two LDS buffers: the next chunk's global loads are
-// issued before the current chunk is traversed and written to the other
-// buffer after it, one barrier per chunk.  A lane follows kPredILP trees of
-// a chunk at once (independent chains of dependent LDS loads in flight) and
+// A block owns blockDim.x rows (staged feature-major in LDS when STAGE_X, as
+// order keys behind a row of zeros, so a node test reads s_x[f + 1][lane]:
+// consecutive lanes, conflict free) and walks the chunks [c0, c1) of the
+// forest in order.  An LDS chunk's compact nodes are copied into one of two
+// LDS buffers: the next chunk's global loads are issued before the current
+// chunk is traversed and written to the other buffer after it, one barrier
+// per chunk.  A lane follows kPredILP trees of a chunk at once (independent
+// chains of dependent LDS loads in flight) for as many steps as the deepest
+// of them has levels, each step one compare and one add (engine.h), then
 // adds their leaves to its accumulators in forest order, so every row's f32
 // sum is the reference's (predictor.rs:227-229,280-301).  Trees kept in the
 // 16-byte layout are traversed from global memory the same way.
@@ -2414,7 +2415,7 @@ __device__ __forceinline__ void pred_transform(float* acc, int K, int transform)
 // MODE 0: out = transform(base + sum); MODE 1: out += sum (training updates);
 // MODE 2: each tree's leaf value to leaf_out[t][row] (two-phase launch; the
 // block's chunk range is blockIdx.y's).
-extern __shared__ uint4 s_pred_mem[];  // [2][kPredChunk16] node buffers, then [F][rows] floats
+extern __shared__ uint4 s_pred_mem[];  // [2][kPredChunk16] node buffers, then [F + 1][rows] keys
 
 // The rows [row0, row0 + blockDim.x) of a block through the chunks [c0, c1):
 // the body of k_predict_lds and of the serving kernel's tiles.
@@ -2425,7 +2426,7 @@ __device__ __forceinline__ void pred_block(
     uint32_t* __restrict__ leaf_idx, int64_t leaf_stride, float* __restrict__ leaf_out) {
   uint4* s_mem = s_pred_mem;
   const int R = blockDim.x;
-  float* s_x = reinterpret_cast<float*>(s_mem + 2 * kPredChunk16);
+  uint32_t* s_x = reinterpret_cast<uint32_t*>(s_mem + 2 * kPredChunk16);
   const int lane = threadIdx.x;
   const int64_t row = row0 + lane;
   const bool in = row < n;
@@ -2433,6 +2434,7 @@ __device__ __forceinline__ void pred_block(
   const uint4* __restrict__ gc16 = reinterpret_cast<const uint4*>(m.cnodes);
   const int64_t* __restrict__ tree_off = m.tree_off;
   const uint32_t* __restrict__ ctree_off = m.ctree_off;
+  const uint32_t* __restrict__ cdepth = m.cdepth;
   const uint32_t* __restrict__ cat_ref = m.cat_ref;
   const uint32_t* __restrict__ tree_group = m.tree_group;
   const uint32_t* __restrict__ cat_start = m.cat_start;
@@ -2445,18 +2447,23 @@ __device__ __forceinline__ void pred_block(
   // leaves may come with no feature rows at all).
   const bool xok = in && m.n_features > 0;
   if (STAGE_X) {
+    // Staged row 0: zeros (what leaves read); row f + 1: feature f's keys.
     // Eight feature rows' loads in flight at a time.
     const int nf = m.n_features;
     const int64_t r = in ? row : 0;
+    s_x[lane] = 0u;
+    uint32_t* sx1 = s_x + R;
     int f = 0;
     for (; f + 8 <= nf; f += 8) {
       float v[8];
 #pragma unroll
       for (int q = 0; q < 8; q++) v[q] = X[(int64_t)(f + q) * x_stride + r];
 #pragma unroll
-      for (int q = 0; q < 8; q++) s_x[(f + q) * R + lane] = in ? v[q] : 0.0f;
+      for (int q = 0; q < 8; q++)
+        sx1[(f + q) * R + lane] = in ? pred_order_key(__float_as_uint(v[q])) : 0u;
     }
-    for (; f < nf; f++) s_x[f * R + lane] = in ? X[(int64_t)f * x_stride + r] : 0.0f;
+    for (; f < nf; f++)
+      sx1[f * R + lane] = in ? pred_order_key(__float_as_uint(X[(int64_t)f * x_stride + r])) : 0u;
   }
   // The next LDS chunk's nodes, in flight during the current chunk.  Loads
   // past the chunk's end read its first pair instead (always in range);
@@ -2514,53 +2521,52 @@ __device__ __forceinline__ void pred_block(
     const uint2* sn = reinterpret_cast<const uint2*>(s_mem + ((c - c0) & 1) * kPredChunk16);
     for (uint32_t t0 = ch.first_tree; t0 < ch.end_tree; t0 += kPredILP) {
       // Chain j follows tree t0 + j from its root; a chain past the chunk's
-      // last tree starts (and stays) on a leaf.
-      uint32_t node[kPredILP], base[kPredILP], leafid[kPredILP];
+      // last tree starts (and stays) on the sentinel leaf.
+      uint32_t leafid[kPredILP];
       float leafv[kPredILP];
       if (ch.n_nodes) {
-        // A chain past the chunk's last tree sits on the sentinel leaf.
+        uint32_t pos[kPredILP];  // node index in the chunk
+        uint32_t steps = 0;      // levels of the group's deepest tree (uniform)
 #pragma unroll
         for (int j = 0; j < kPredILP; j++) {
           const uint32_t t = t0 + j;
-          base[j] = t < ch.end_tree ? ctree_off[t] - ch.node_begin : ch.n_nodes - 1;
-          node[j] = 0;
+          const bool ok = t < ch.end_tree;
+          pos[j] = ok ? ctree_off[t] - ch.node_begin : ch.n_nodes - 1;
+          steps = max(steps, ok ? cdepth[t] : 0u);
         }
-        // Branch-free steps: every chain reads its node and its feature value
-        // (a leaf reads feature 0) and moves to left + (go right); a leaf
-        // moves to itself.  The loop ends when every chain is on a leaf,
-        // whose word is then in v.
-        uint2 v[kPredILP];
-        uint32_t any = 1;
-        while (any) {
-          any = 0;
+        // Every chain takes the same number of steps; a chain on a leaf
+        // reads staged row 0 (key 0) and stays (engine.h).
+        for (uint32_t d = 0; d < steps; d++) {
+          uint2 v[kPredILP];
 #pragma unroll
-          for (int j = 0; j < kPredILP; j++) v[j] = sn[base[j] + node[j]];
-          float xv[kPredILP];
+          for (int j = 0; j < kPredILP; j++) v[j] = sn[pos[j]];
+          uint32_t kx[kPredILP];
 #pragma unroll
           for (int j = 0; j < kPredILP; j++) {
-            const uint32_t leaf = (v[j].x >> 16) & 1u;
-            const uint32_t f = (v[j].x & 0xFFFFu) & (leaf - 1u);
-            xv[j] = STAGE_X ? s_x[f * R + lane] : (xok ? X[(int64_t)f * x_stride + row] : 0.0f);
+            const uint32_t f = v[j].x & 0xFFFFu;  // staged row
+            if (STAGE_X)
+              kx[j] = s_x[f * R + lane];
+            else
+              kx[j] = f != 0u && xok
+                          ? pred_order_key(__float_as_uint(X[(int64_t)(f - 1u) * x_stride + row]))
+                          : 0u;
           }
 #pragma unroll
           for (int j = 0; j < kPredILP; j++) {
             const uint32_t w = v[j].x;
-            const uint32_t leaf = (w >> 16) & 1u;
-            const float x = xv[j];
-            // NaN compares false: NaN goes left only with default_left.
-            uint32_t left = (x < __uint_as_float(v[j].y) ? 1u : 0u) |
-                            (isnan(x) ? ((w >> 17) & 1u) : 0u) | leaf;
-            if (CAT && (w & kCCat) && !isnan(x))
-              left = pred_cat_right(x, cat_ref[2 * v[j].y], cat_ref[2 * v[j].y + 1], cat_words)
-                         ? 0u : 1u;
-            node[j] = (w >> kCShift) + (left ^ 1u);
-            any |= leaf ^ 1u;
+            uint32_t r = kx[j] + ((w >> 17) & 1u) >= v[j].y ? 1u : 0u;
+            if (CAT && (w & kCCat) && kx[j] != 0xFFFFFFFFu) {
+              const uint32_t c = v[j].y - 1u;
+              r = pred_cat_right(__uint_as_float(pred_key_bits(kx[j])), cat_ref[2 * c],
+                                 cat_ref[2 * c + 1], cat_words) ? 1u : 0u;
+            }
+            pos[j] = (w >> kCShift) + r;
           }
         }
 #pragma unroll
         for (int j = 0; j < kPredILP; j++) {
-          leafv[j] = __uint_as_float(v[j].y);
-          leafid[j] = v[j].x & 0xFFFFu;
+          leafv[j] = __uint_as_float(~sn[pos[j]].y);
+          leafid[j] = pos[j];  // the original id is looked up below when wanted
         }
       } else {
         // One tree in the 16-byte layout, from global memory.
@@ -2569,7 +2575,8 @@ __device__ __forceinline__ void pred_block(
         uint4 v = gn16[b];
         while (!(v.x & kNodeLeaf)) {
           const uint32_t f = v.x & 0xFFFFFFu;
-          const float x = STAGE_X ? s_x[f * R + lane] : (xok ? X[(int64_t)f * x_stride + row] : 0.0f);
+          const float x = STAGE_X ? __uint_as_float(pred_key_bits(s_x[(f + 1u) * R + lane]))
+                                  : (xok ? X[(int64_t)f * x_stride + row] : 0.0f);
           bool left;
           if (isnan(x))
             left = (v.x & kNodeDefLeft) != 0;
@@ -2593,7 +2600,10 @@ __device__ __forceinline__ void pred_block(
       for (int j = 0; j < kPredILP; j++) {
         const uint32_t t = t0 + j;
         if (t < t_end) {
-          if (in && leaf_idx) leaf_idx[(int64_t)t * leaf_stride + row] = leafid[j];
+          if (in && leaf_idx)
+            leaf_idx[(int64_t)t * leaf_stride + row] =
+                ch.n_nodes ? m.corig[ch.node_begin + min(leafid[j], ch.n_nodes - 1u)]
+                           : leafid[j];
           if (MODE == 2) {
             if (in) leaf_out[(int64_t)t * n + row] = leafv[j];
           } else if (one) {
@@ -2876,9 +2886,9 @@ __global__ __launch_bounds__(256) void k_transform_rows(float* __restrict__ out,
 
 size_t pred_lds_bytes(int F, size_t lds_limit, bool* stage_rows, int* rows_per_block) {
   const size_t nodes = (size_t)2 * kPredChunk16 * sizeof(uint4);
-  F = std::max(F, 1);  // a leaf's feature field (0) always reads a staged row
   for (int r = kPredMaxRows; r >= 64; r >>= 1) {
-    const size_t rows = (size_t)F * r * sizeof(float);
+    // The features' keys behind the row of zeros that leaves read.
+    const size_t rows = (size_t)(std::max(F, 0) + 1) * r * sizeof(uint32_t);
     if (nodes + rows <= lds_limit) {
       if (stage_rows) *stage_rows = true;
       if (rows_per_block) *rows_per_block = r;
