@@ -285,18 +285,25 @@ void bst_model::upload(hipStream_t s) {
   }
   d_nodes.alloc(std::max<size_t>(nn, 1));
   d_tree_off.alloc(tree_offset.size());
-  d_tree_group.alloc(std::max<size_t>(tree_group.size(), 1));
+  // Padded by 8 entries: a prediction block loads 8 trees' groups at once.
+  std::vector<uint32_t> group_pad(tree_group);
+  group_pad.resize(tree_group.size() + 8, 0u);
+  d_tree_group.alloc(group_pad.size());
   d_cat_start.alloc(std::max<size_t>(nn, 1));
   d_cat_size.alloc(std::max<size_t>(nn, 1));
   d_cat_words.alloc(std::max<size_t>(cat_words.size(), 1));
-  d_base.alloc(std::max<size_t>(base_score.size(), 1));
+  // Base scores padded by kPredMaxGroups zeros: a prediction block loads a
+  // whole window of them unconditionally.
+  std::vector<float> base_pad(base_score);
+  base_pad.resize(base_score.size() + kPredMaxGroups, 0.0f);
+  d_base.alloc(base_pad.size());
   h2d(s, d_nodes.p, nodes.data(), nn);
   h2d(s, d_tree_off.p, tree_offset.data(), tree_offset.size());
-  h2d(s, d_tree_group.p, tree_group.data(), tree_group.size());
+  h2d(s, d_tree_group.p, group_pad.data(), group_pad.size());
   h2d(s, d_cat_start.p, cat_start.data(), nn);
   h2d(s, d_cat_size.p, cat_size.data(), nn);
   h2d(s, d_cat_words.p, cat_words.data(), cat_words.size());
-  h2d(s, d_base.p, base_score.data(), base_score.size());
+  h2d(s, d_base.p, base_pad.data(), base_pad.size());
   // Predictor layout: trees of at most kPredChunkNodes nodes testing features
   // below 2^16 - 1 get a compact copy (engine.h), renumbered breadth first so
   // that a node's children are adjacent (left, left + 1), and are grouped
@@ -390,7 +397,8 @@ void bst_model::upload(hipStream_t s) {
     }
     const uint32_t begin = (uint32_t)cn.size();
     // The trees that fit the buffer with the chunk's sentinel leaf; a few
-    // more than a multiple of kPredILP are cut back to it.
+    // more than a multiple of kPredILP are cut back to it when later trees
+    // follow (they fill the next chunk's groups instead).
     size_t e = t, fill = 0;
     while (e < T && compactable(e) &&
            fill + (size_t)(tree_offset[e + 1] - tree_offset[e]) + 1 <= (size_t)kPredChunkNodes) {
@@ -398,7 +406,7 @@ void bst_model::upload(hipStream_t s) {
       e++;
     }
     const size_t nt = e - t, idle = (kPredILP - nt % kPredILP) % kPredILP;
-    if (nt > (size_t)kPredILP && idle * 10 > nt) e = t + nt / kPredILP * kPredILP;
+    if (e < T && nt > (size_t)kPredILP && idle * 10 > nt) e = t + nt / kPredILP * kPredILP;
     for (size_t u = t; u < e; u++) append_compact(u, begin);
     // Sentinel leaf (value 0) at its own index.
     cn.push_back(make_uint2(((uint32_t)cn.size() - begin) << kCShift, 0xFFFFFFFFu));
@@ -413,6 +421,10 @@ void bst_model::upload(hipStream_t s) {
   if (cref.empty()) cref.assign(2, 0u);
   n_chunks = (int)ch.size();
   d_cnodes.alloc(cn.size());
+  // Tree offsets and levels padded by 8 entries: a block loads a group's
+  // (at most 8) in one unconditional load.
+  ctoff.resize(ctoff.size() + 8, 0u);
+  cdep.resize(cdep.size() + 8, 0u);
   d_ctree_off.alloc(ctoff.size());
   d_cdepth.alloc(cdep.size());
   d_corig.alloc(corig.size());

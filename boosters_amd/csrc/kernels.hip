@@ -2359,6 +2359,9 @@ __device__ __forceinline__ float sigmoid32(float x) {  // transform.rs:106-115
 }
 
 constexpr int kPredMaxK = 16;
+struct PredGroup8 {  // eight consecutive per-tree words, read in one load
+  uint32_t v[8];
+};
 constexpr int kPredChunk16 = kPredChunkNodes / 2;          // uint4 per node buffer
 constexpr int kPredPer = (kPredChunk16 + kPredMaxRows - 1) / kPredMaxRows;  // staged uint4 per lane
 
@@ -2446,25 +2449,6 @@ __device__ __forceinline__ void pred_block(
   // Rows are read from X only when the model tests features (a forest of
   // leaves may come with no feature rows at all).
   const bool xok = in && m.n_features > 0;
-  if (STAGE_X) {
-    // Staged row 0: zeros (what leaves read); row f + 1: feature f's keys.
-    // Eight feature rows' loads in flight at a time.
-    const int nf = m.n_features;
-    const int64_t r = in ? row : 0;
-    s_x[lane] = 0u;
-    uint32_t* sx1 = s_x + R;
-    int f = 0;
-    for (; f + 8 <= nf; f += 8) {
-      float v[8];
-#pragma unroll
-      for (int q = 0; q < 8; q++) v[q] = X[(int64_t)(f + q) * x_stride + r];
-#pragma unroll
-      for (int q = 0; q < 8; q++)
-        sx1[(f + q) * R + lane] = in ? pred_order_key(__float_as_uint(v[q])) : 0u;
-    }
-    for (; f < nf; f++)
-      sx1[f * R + lane] = in ? pred_order_key(__float_as_uint(X[(int64_t)f * x_stride + r])) : 0u;
-  }
   // The next LDS chunk's nodes, in flight during the current chunk.  Loads
   // past the chunk's end read its first pair instead (always in range);
   // only the chunk's own pairs are stored.
@@ -2504,17 +2488,52 @@ __device__ __forceinline__ void pred_block(
   if (MODE != 2) {
 #pragma unroll
     for (int k = 0; k < kPredMaxK; k++)
-      acc[k] = (k < K && in) ? (MODE == 1 ? out[(int64_t)k * out_stride + row] : m.base[m.g0 + k])
-                             : 0.0f;
+      acc[k] = (k < K && in) ? (MODE == 1 ? out[(int64_t)k * out_stride + row] : 0.0f) : 0.0f;
+    if (MODE == 0) {
+      // The base scores are padded past the last output, so the window's
+      // loads need no condition (and issue together).
+      float b[kPredMaxK];
+#pragma unroll
+      for (int k = 0; k < kPredMaxK; k++) b[k] = m.base[m.g0 + k];
+#pragma unroll
+      for (int k = 0; k < kPredMaxK; k++) acc[k] = (k < K && in) ? b[k] : 0.0f;
+    }
   }
-  if (c0 < c1) {
-    const PredChunk ch = chunks[c0];
-    BST_PRED_LOAD(ch);
-    BST_PRED_STORE(ch, s_mem);
+  // The first chunk's loads are issued before the rows are staged.
+  if (c0 < c1) BST_PRED_LOAD(chunks[c0]);
+  if (STAGE_X) {
+    // Staged row 0: zeros (what leaves read); row f + 1: feature f's keys.
+    // 32 feature rows' loads in flight at a time (with the first chunk's).
+    const int nf = m.n_features;
+    const int64_t r = in ? row : 0;
+    s_x[lane] = 0u;
+    uint32_t* sx1 = s_x + R;
+    int f = 0;
+    for (; f + 32 <= nf; f += 32) {
+      float v[32];
+#pragma unroll
+      for (int q = 0; q < 32; q++) v[q] = X[(int64_t)(f + q) * x_stride + r];
+#pragma unroll
+      for (int q = 0; q < 32; q++)
+        sx1[(f + q) * R + lane] = in ? pred_order_key(__float_as_uint(v[q])) : 0u;
+    }
+    for (; f + 8 <= nf; f += 8) {
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = X[(int64_t)(f + q) * x_stride + r];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        sx1[(f + q) * R + lane] = in ? pred_order_key(__float_as_uint(v[q])) : 0u;
+    }
+    for (; f < nf; f++)
+      sx1[f * R + lane] = in ? pred_order_key(__float_as_uint(X[(int64_t)f * x_stride + r])) : 0u;
   }
+  if (c0 < c1) BST_PRED_STORE(chunks[c0], s_mem);
   __syncthreads();
+  // The current chunk's descriptor is carried over from the previous
+  // iteration's next one: one descriptor load per chunk.
+  PredChunk ch = chunks[c0 < c1 ? c0 : 0];
   for (int c = c0; c < c1; c++) {
-    const PredChunk ch = chunks[c];
     const bool next = c + 1 < c1;
     const PredChunk chn = chunks[next ? c + 1 : c];
     if (next) BST_PRED_LOAD(chn);
@@ -2527,12 +2546,16 @@ __device__ __forceinline__ void pred_block(
       if (ch.n_nodes) {
         uint32_t pos[kPredILP];  // node index in the chunk
         uint32_t steps = 0;      // levels of the group's deepest tree (uniform)
+        // The group's roots and levels as two 8-entry loads (the arrays are
+        // padded by 8 entries past the last tree).
+        static_assert(kPredILP <= 8, "one 8-entry load per array");
+        const PredGroup8 roots = *reinterpret_cast<const PredGroup8*>(ctree_off + t0);
+        const PredGroup8 levels = *reinterpret_cast<const PredGroup8*>(cdepth + t0);
 #pragma unroll
         for (int j = 0; j < kPredILP; j++) {
-          const uint32_t t = t0 + j;
-          const bool ok = t < ch.end_tree;
-          pos[j] = ok ? ctree_off[t] - ch.node_begin : ch.n_nodes - 1;
-          steps = max(steps, ok ? cdepth[t] : 0u);
+          const bool ok = t0 + j < ch.end_tree;
+          pos[j] = ok ? roots.v[j] - ch.node_begin : ch.n_nodes - 1;
+          steps = max(steps, ok ? levels.v[j] : 0u);
         }
         // Every chain takes the same number of steps; a chain on a leaf
         // reads staged row 0 (key 0) and stays (engine.h).
@@ -2596,6 +2619,9 @@ __device__ __forceinline__ void pred_block(
         leafid[0] = nd;
       }
       const uint32_t t_end = ch.n_nodes ? ch.end_tree : t0 + 1;
+      // The group's output groups in one load (the array is padded by 8).
+      PredGroup8 tg{};
+      if (MODE != 2 && !one) tg = *reinterpret_cast<const PredGroup8*>(tree_group + t0);
 #pragma unroll
       for (int j = 0; j < kPredILP; j++) {
         const uint32_t t = t0 + j;
@@ -2611,7 +2637,7 @@ __device__ __forceinline__ void pred_block(
           } else {
             // Trees of other windows match no accumulator (the difference
             // wraps for groups below g0).
-            const uint32_t g = tree_group[t] - (uint32_t)m.g0;
+            const uint32_t g = tg.v[j] - (uint32_t)m.g0;
 #pragma unroll
             for (int k = 0; k < kPredMaxK; k++)
               if ((uint32_t)k == g) acc[k] += leafv[j];
@@ -2622,6 +2648,7 @@ __device__ __forceinline__ void pred_block(
     }
     if (next) BST_PRED_STORE(chn, s_mem + ((c + 1 - c0) & 1) * kPredChunk16);
     __syncthreads();
+    ch = chn;
   }
 #undef BST_PRED_L1
 #undef BST_PRED_S1

@@ -594,6 +594,75 @@ def test_predict_and_leaf_indices(ctx, oracle):
         np.testing.assert_allclose(out, ref, rtol=1e-6, atol=1e-7)
 
 
+def _mixed_forest(seed):
+    """Complete trees of depths 3, 5 and 8, two of depth 13 (16383 nodes: too
+    large for an LDS chunk, traversed in the 16-byte layout from global
+    memory) and caterpillar trees (one leaf per level, depth 20), so that one
+    group of chains holds trees of different depths and leaves at every
+    depth."""
+    rng = np.random.default_rng(seed)
+    F = 12
+    trees = []  # (is_leaf, left, right) per tree, original ids
+    for kind in [3, 13, 5, "cat", 8, 13] + [8] * 16 + ["cat", 5] * 5:
+        if kind == "cat":
+            D = 20
+            nn = 2 * D + 1
+            ids = np.arange(nn)
+            leaf = (ids % 2 == 1) | (ids == 2 * D)
+            left = np.where(leaf, 0, ids + 1)
+            right = np.where(leaf, 0, ids + 2)
+        else:
+            nn = (1 << (kind + 1)) - 1
+            ids = np.arange(nn)
+            leaf = ids >= (1 << kind) - 1
+            left = np.where(leaf, 0, 2 * ids + 1)
+            right = np.where(leaf, 0, 2 * ids + 2)
+        trees.append((leaf, left, right))
+    info = A.ModelInfo()
+    info.n_trees, info.n_groups, info.n_features = len(trees), 1, F
+    info.total_nodes = sum(len(t[0]) for t in trees)
+    info.total_cat_words = 0
+    info.output_transform = A.TRANSFORM_IDENTITY
+    info.objective = A.OBJ_SQUARED
+    a = A.alloc_forest(info)
+    off = 0
+    for i, (leaf, left, right) in enumerate(trees):
+        nn = len(leaf)
+        s = slice(off, off + nn)
+        a["is_leaf"][s] = leaf
+        a["split_index"][s] = np.where(leaf, 0, rng.integers(0, F, nn))
+        # A third of the thresholds exactly 0 (x = -0.0 and +0.0 both go right).
+        thr = np.where(rng.random(nn) < 0.3, 0.0, rng.uniform(-2, 2, nn))
+        a["split_threshold"][s] = np.where(leaf, 0, thr).astype(np.float32)
+        a["default_left"][s] = rng.integers(0, 2, nn)
+        a["left"][s] = left
+        a["right"][s] = right
+        a["leaf_value"][s] = np.where(leaf, rng.uniform(-1, 1, nn), 0).astype(np.float32)
+        a["tree_offset"][i] = off
+        off += nn
+    a["tree_offset"][len(trees)] = off
+    a["tree_group"][:] = 0
+    a["base_score"][:] = 0.25
+    return info, a
+
+
+def test_predict_mixed_tree_layouts(ctx, oracle):
+    """LDS-chunk trees and global-memory trees in one forest, trees of
+    different depths in one group of chains: outputs and leaf indices equal
+    the oracle's, NaN values included."""
+    info, arr = _mixed_forest(7)
+    gm = GBDTModel.from_arrays(info, arr, ctx)
+    rng = np.random.default_rng(8)
+    X = rng.uniform(-2.5, 2.5, (info.n_features, 3000)).astype(np.float32)
+    X[rng.random(X.shape) < 0.1] = np.nan
+    X[:, :40] = 0.0
+    X[:, 40:80] = -0.0
+    out, li = gm.predict_feature_major(X, raw=True, leaf_index=True)
+    ref, rli = oracle.predict(info, arr, X, raw=True, leaf_idx=True)
+    np.testing.assert_array_equal(li, rli)
+    np.testing.assert_allclose(out, ref, rtol=1e-6, atol=1e-7)
+
+
 def test_bstr_round_trip_predictions(ctx, oracle):
     """A trained forest written as .bstr (MessagePack, zstd) and .bstr.json and
     read back (persist/mod.rs:88-143) has the same arrays and predicts the
