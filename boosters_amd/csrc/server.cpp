@@ -93,6 +93,8 @@ int pack_threads(size_t bytes) {
 
 constexpr int64_t kMaxSlotRows = 1 << 20;
 constexpr double kIdleMs = 50.0;
+// Host-link bytes that the leaf units of one batch may read in all.
+constexpr int64_t kSplitHostBytes = (int64_t)2 << 20;
 // Batches of at least this many input bytes are packed on a thread of their
 // own, so that smaller batches admitted meanwhile are not held behind the copy.
 constexpr size_t kAsyncPackBytes = (size_t)4 << 20;
@@ -187,11 +189,17 @@ void publish(bst_server* srv, int si) {
   const int64_t rows = s.rows;
   const uint32_t n_tiles = (uint32_t)((rows + srv->tile_rows - 1) / srv->tile_rows);
   // Leaf units per tile: enough to spread the batch over the CUs, when its
-  // tiles alone would leave more than half of them idle.
+  // tiles alone would leave more than half of them idle.  Each leaf unit
+  // reads its tile's rows across the host link, so the units of a batch
+  // read at most kSplitHostBytes in all (a batch of more rows takes fewer
+  // units per tile, or none).
   uint32_t split = 0;
-  if (rows <= srv->split_rows && srv->n_chunks > 1 && 2 * (int64_t)n_tiles <= srv->n_cu) {
+  const int64_t in_bytes = rows * F * 4;
+  const int64_t by_bytes = kSplitHostBytes / std::max<int64_t>(in_bytes, 1);
+  if (rows <= srv->split_rows && srv->n_chunks > 1 && 2 * (int64_t)n_tiles <= srv->n_cu &&
+      by_bytes >= 2) {
     split = (uint32_t)std::min<int64_t>(
-        {(int64_t)kServeMaxSplit, (int64_t)srv->n_chunks,
+        {(int64_t)kServeMaxSplit, (int64_t)srv->n_chunks, by_bytes,
          std::max<int64_t>(2, srv->n_cu / (int64_t)n_tiles)});
     // Ranges of `per` chunks: only as many as cover the forest, so that no
     // range starts past its last chunk.
